@@ -1,0 +1,182 @@
+// Internal header of the gfx950 runtime behind include/biocfd_hip.h.
+//
+// Device layout: every field stays in the reference's Fortran layout
+// (column-major, 1-based, one ghost layer) so host<->device transfers are a
+// single copy and the i index is the contiguous (coalesced) one.  Cell tags
+// live on the device as uint8 (the reference keeps int64); they are widened
+// only when downloaded.  Index lists are int32 triplets, built by ordered
+// (stable) scans so their order is the reference's serial i-fastest order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "biocfd_hip.h"
+
+namespace bcfd {
+
+constexpr int kMaxBlocks = 64;
+constexpr int kMaxInterfaces = 64;
+
+void set_error(const char *fmt, ...);
+
+#define BCFD_HIP(call)                                                                   \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) {                                                              \
+      ::bcfd::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+      return -1;                                                                         \
+    }                                                                                    \
+  } while (0)
+
+#define BCFD_CHECK_LAUNCH()                                                              \
+  do {                                                                                   \
+    hipError_t e_ = hipGetLastError();                                                   \
+    if (e_ != hipSuccess) {                                                              \
+      ::bcfd::set_error("%s:%d launch: %s", __FILE__, __LINE__, hipGetErrorString(e_));  \
+      return -1;                                                                         \
+    }                                                                                    \
+  } while (0)
+
+// Per-block scalars a kernel needs, passed by value.
+struct Dims {
+  int nx, ny, nz;     // interior cells
+  int NX, NY, NZ;     // nx+2 ...
+  int TX, TY;         // nx+3, ny+3 (tall tag arrays)
+};
+
+__host__ __device__ inline int64_t ix3(const Dims &d, int i, int j, int k) {
+  return (int64_t)(i - 1) + (int64_t)d.NX * ((int64_t)(j - 1) + (int64_t)d.NY * (int64_t)(k - 1));
+}
+__host__ __device__ inline int64_t ixt(const Dims &d, int i, int j, int k) {
+  return (int64_t)(i - 1) + (int64_t)d.TX * ((int64_t)(j - 1) + (int64_t)d.TY * (int64_t)(k - 1));
+}
+
+// SOR control state in device memory (one per block).
+struct SorState {
+  long long n_iter;              // nIterPcor (accumulates within one poissonSolver call)
+  double derr4;                  // last update norm (11111 at call start)
+  double derr2;                  // block%derr2
+  unsigned long long max_bits;   // running max of |pc-pco| (bit pattern of a double >= 0)
+  int done;
+  int pad;
+  double derr_std_st;            // steady-state norm
+  unsigned long long std_bits;
+};
+
+struct IbData {
+  int64_t n_nodes = 0, n_elems = 0;
+  int search[6] = {0, 0, 0, 0, 0, 0};
+  int *el[3] = {nullptr, nullptr, nullptr};   // 1-based node numbers
+  int *surf = nullptr;                        // per element
+  double *node[3] = {nullptr, nullptr, nullptr};  // xnode1..znode1
+  double *elem[BE_NELEM] = {};                // xcent..element_length
+  std::vector<int> h_surf;
+  // per IB cell (capacity grows)
+  int64_t cap_ib = 0;
+  int64_t n_mapped = 0;                       // entries of ib_ijk currently in ib_map
+  int *ib_ijk = nullptr;                      // 3*cap: i list, j list, k list
+  int *nel[BN_NNEL] = {};
+  double *normdis[BN_NNEL] = {};
+  int *ib_map = nullptr;                      // n2: IB ordinal (0-based) or -1
+  // TS cells
+  int64_t cap_ts = 0;
+  int *ts_ijk = nullptr;
+  int *index_ts = nullptr;                    // 1-based IB ordinal
+  double *ghost[BGH_N] = {};
+  // solid list
+  int64_t cap_solid = 0;
+  int *solid_ijk = nullptr;
+  // forcing scratch (per task)
+  int64_t cap_task = 0;
+  double *task_out = nullptr;
+  int *task_done = nullptr;
+  int64_t cap_done = 0;
+  bool present = false;
+};
+
+struct Block {
+  bool live = false;
+  Dims d{};
+  int64_t n2 = 0, n3 = 0;
+  double omega = 1.0, dx = 0, dy = 0, dz = 0;
+  int64_t cintp = 3;
+  std::vector<double> hg[BG_NGRID];   // host copies of the grid arrays
+  double *g[BG_NGRID] = {};           // device grid arrays
+  double *f[BF_NFIELD] = {};          // device fields (pco is not stored: it equals pc)
+  uint8_t *tag[BT_NTAG] = {};
+  double *A[3] = {};                  // Acx, Acy, Acz (n,3)
+  std::vector<double> hA[3];
+  double *ca[BTB_N] = {}, *ck[BTB_N] = {};
+  std::vector<double> hca[BTB_N], hck[BTB_N];
+  SorState *sor = nullptr;            // device
+  int64_t counts[BC_N] = {0, 0, 0, 0, 0, 0};
+  IbData ib;
+  // scan scratch
+  int64_t *scan_tmp = nullptr;
+  int64_t scan_cap = 0;
+  int *list_tmp = nullptr;
+  int64_t list_cap = 0;
+};
+
+struct Interface {
+  bool live = false;
+  int a = 0, b = 0;                    // 1-based block numbers
+  int64_t a_msh = 1, b_msh = 1;
+  double ext[6] = {0, 0, 0, 0, 0, 0};
+  // det tables [comp p,u,v,w][axis x,y,z]: host (3,cap) and device copies
+  std::vector<int64_t> hdet[4][3];
+  int64_t counter[4][3] = {};
+  int *det[4][3] = {};                 // device int32 (3, counter)
+};
+
+struct Runtime {
+  bool init = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Block blk[kMaxBlocks + 1];
+  Interface itf[kMaxInterfaces + 1];
+  int n_itf = 0;
+  double *h_pinned = nullptr;          // small pinned readback buffer
+  int64_t *h_pinned_i = nullptr;
+};
+
+Runtime &rt();
+Block *get_block(int id);
+int ensure_init();
+
+template <typename T>
+int dev_alloc(T **p, int64_t n) {
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (n <= 0) n = 1;
+  hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%lld bytes): %s", (long long)(sizeof(T) * n), hipGetErrorString(e));
+    *p = nullptr;
+    return -1;
+  }
+  return 0;
+}
+
+template <typename T>
+int grow(T **p, int64_t *cap, int64_t need) {
+  if (need <= *cap && *p) return 0;
+  int64_t c = need < 64 ? 64 : need + need / 4;
+  if (dev_alloc(p, c)) return -1;
+  *cap = c;
+  return 0;
+}
+
+inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+// internal entry points shared between translation units
+int poisson_block_setup(Block &b);
+int ib_list_capacity(Block &b, int64_t n_ib);
+int build_cell_lists(Block &b, bool ib_lists);
+int build_ts(Block &b);
+
+}  // namespace bcfd

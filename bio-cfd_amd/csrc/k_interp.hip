@@ -1,0 +1,283 @@
+// Coarse/fine block coupling on gfx950: trilinear fine->coarse restriction
+// (reference src/CoarseInterp.f90) and coarse->fine ghost-face prolongation
+// (src/FineInterp_bound.f90), driven by the per-axis interface tables built
+// on the host (src/InterfaceDetail.f90:168-241).  One thread per table entry
+// (restriction) or per fine target cell (prolongation).
+#include "biocfd_rt.hpp"
+
+namespace bcfd {
+
+// set_interface_detail: (coarse index, fine start, fine end) per coarse cell
+// overlapped along one axis; first and last entries are the ghost entries
+static void interface_axis(const std::vector<double> &cv, double lo, double hi, int64_t factor,
+                           int64_t starter, std::vector<int64_t> &det, int64_t *counter) {
+  const int64_t size = (int64_t)cv.size();
+  int64_t first = 0, last = 0;
+  for (int64_t i = 1; i <= size; ++i)
+    if (cv[i - 1] > lo) { first = i; break; }
+  for (int64_t i = first; i <= size; ++i)
+    if (cv[i - 1] > hi) { last = i - 1; break; }
+  det.clear();
+  auto push = [&](int64_t a, int64_t b, int64_t c) { det.push_back(a); det.push_back(b); det.push_back(c); };
+  push(first - 1, starter - 1, starter - 1);
+  int64_t lo_f = starter, hi_f = starter + factor - 1;
+  for (int64_t c = first;; ++c) {
+    push(c, lo_f, hi_f);
+    lo_f = hi_f + 1;
+    hi_f = hi_f + factor;
+    if (c + 1 > last) break;
+  }
+  push(last + 1, lo_f, lo_f);
+  *counter = (int64_t)det.size() / 3;
+}
+
+struct ITab {
+  const int *det[3];
+  int cnt[3];
+};
+
+__device__ inline int DET(const ITab &t, int ax, int row, int n) { return t.det[ax][(row - 1) + 3 * (n - 1)]; }
+
+__device__ inline double bl(double x, double y, double x1, double x2, double y1, double y2, double f1, double f2,
+                            double f3, double f4) {
+  const double numer = f1 * (x2 - x) * (y2 - y) + f3 * (x2 - x) * (y - y1) + f2 * (x - x1) * (y2 - y) +
+                       f4 * (x - x1) * (y - y1);
+  return numer / ((x2 - x1) * (y2 - y1));
+}
+
+struct GridSet {
+  const double *x, *y, *z;
+};
+
+// fine -> coarse (CoarseInterp.f90:447-502 trilinear)
+__global__ void k_restrict(Dims cd, Dims fd, ITab t, int comp, int cintp, GridSet cg, GridSet fg,
+                           const double *__restrict__ fvar, double *__restrict__ cvar, double *__restrict__ cvar2) {
+  const int ni = t.cnt[0] - 2 * cintp + 1, nj = t.cnt[1] - 2 * cintp + 1, nk = t.cnt[2] - 2 * cintp + 1;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)ni * nj * nk) return;
+  const int i = cintp + (int)(q % ni), j = cintp + (int)((q / ni) % nj), k = cintp + (int)(q / ((int64_t)ni * nj));
+  const int tx = DET(t, 0, 1, i), ty = DET(t, 1, 1, j), tz = DET(t, 2, 1, k);
+  int lx = (DET(t, 0, 2, i) + DET(t, 0, 3, i) - 1) / 2;
+  int ly = (DET(t, 1, 2, j) + DET(t, 1, 3, j) - 1) / 2;
+  int lz = (DET(t, 2, 2, k) + DET(t, 2, 3, k) - 1) / 2;
+  int ox = 0, oy = 0, oz = 0;
+  if (comp == 1) { lx = DET(t, 0, 3, i); ox = 1; }
+  if (comp == 2) { ly = DET(t, 1, 3, j); oy = 1; }
+  if (comp == 3) { lz = DET(t, 2, 3, k); oz = 1; }
+  int ii = lx, jj = ly, kk = lz, vi = lx, vj = ly, vk = lz;
+  if (comp == 1) { ii = lx - 1; vi = lx - 2; }
+  else if (comp == 2) { jj = ly - 1; vj = ly - 2; }
+  else if (comp == 3) { kk = lz - 1; vk = lz - 2; }
+  if (vi < 1 || vj < 1 || vk < 1 || vi + 1 > fd.NX || vj + 1 > fd.NY || vk + 1 > fd.NZ || ii < 1 || jj < 1 ||
+      kk < 1 || tx - ox < 1 || ty - oy < 1 || tz - oz < 1 || tx > cd.NX + 1 || ty > cd.NY + 1 || tz > cd.NZ + 1)
+    return;  // interface outside the block: nothing to restrict here
+  const double x = cg.x[tx - 1], y = cg.y[ty - 1], z = cg.z[tz - 1];
+  const double X0 = fg.x[ii - 1], X1 = fg.x[ii], Y0 = fg.y[jj - 1], Y1 = fg.y[jj];
+  const double z1 = bl(x, y, X0, X1, Y0, Y1, fvar[ix3(fd, vi, vj, vk)], fvar[ix3(fd, vi + 1, vj, vk)],
+                       fvar[ix3(fd, vi, vj + 1, vk)], fvar[ix3(fd, vi + 1, vj + 1, vk)]);
+  const double z2 = bl(x, y, X0, X1, Y0, Y1, fvar[ix3(fd, vi, vj, vk + 1)], fvar[ix3(fd, vi + 1, vj, vk + 1)],
+                       fvar[ix3(fd, vi, vj + 1, vk + 1)], fvar[ix3(fd, vi + 1, vj + 1, vk + 1)]);
+  const double Z0 = fg.z[kk - 1], Z1 = fg.z[kk];
+  const double out = z1 + (z - Z0) * (z2 - z1) / (Z1 - Z0);
+  const int64_t dst = ix3(cd, tx - ox, ty - oy, tz - oz);
+  cvar[dst] = out;
+  if (cvar2) cvar2[dst] = out;
+}
+
+// coarse -> fine on the six faces (FineInterp_bound.f90:554-603 trilinear
+// over a two-coarse-cell span).  Entries: face-major enumeration of the
+// (i,j,k) table entries with one index pinned to the first/last entry.
+__global__ void k_prolong(Dims cd, Dims fd, ITab t, int comp, int axis, GridSet cg, GridSet fg,
+                          const double *__restrict__ cvar, double *__restrict__ fvar, double *__restrict__ fvar2,
+                          int factor) {
+  // entries of this axis pass: the pinned axis takes {1, cnt}
+  int n[3] = {t.cnt[0], t.cnt[1], t.cnt[2]};
+  n[axis] = 2;
+  const int64_t nent = (int64_t)n[0] * n[1] * n[2];
+  const int per = factor * factor * factor;  // fine cells per entry (upper bound)
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nent * per) return;
+  const int64_t e = q / per;
+  const int sub = (int)(q % per);
+  int idx[3] = {(int)(e % n[0]), (int)((e / n[0]) % n[1]), (int)(e / ((int64_t)n[0] * n[1]))};
+  for (int a = 0; a < 3; ++a) idx[a] += 1;
+  if (idx[axis] == 2) idx[axis] = t.cnt[axis];
+  const int i = idx[0], j = idx[1], k = idx[2];
+  const int x1 = DET(t, 0, 2, i), x2 = DET(t, 0, 3, i);
+  const int y1 = DET(t, 1, 2, j), y2 = DET(t, 1, 3, j);
+  const int z1 = DET(t, 2, 2, k), z2 = DET(t, 2, 3, k);
+  const int sx = sub % factor, sy = (sub / factor) % factor, sz = sub / (factor * factor);
+  const int tx = x1 + sx, ty = y1 + sy, tz = z1 + sz;
+  if (tx > x2 || ty > y2 || tz > z2) return;
+  const int lx = DET(t, 0, 1, i), ly = DET(t, 1, 1, j), lz = DET(t, 2, 1, k);
+  int vi = lx, vj = ly, vk = lz;
+  if (comp == 1) vi = lx - 1;
+  else if (comp == 2) vj = ly - 1;
+  else if (comp == 3) vk = lz - 1;
+  const int ox_ = (comp == 1), oy_ = (comp == 2), oz_ = (comp == 3);
+  if (vi - 1 < 1 || vj - 1 < 1 || vk - 1 < 1 || vi + 1 > cd.NX || vj + 1 > cd.NY || vk + 1 > cd.NZ || lx < 2 ||
+      ly < 2 || lz < 2 || tx - ox_ < 1 || ty - oy_ < 1 || tz - oz_ < 1 || tx - ox_ > fd.NX || ty - oy_ > fd.NY ||
+      tz - oz_ > fd.NZ)
+    return;  // the fine block touches the coarse block's edge: no coarse stencil
+  const double x = fg.x[tx - 1], y = fg.y[ty - 1], z = fg.z[tz - 1];
+  const double X0 = cg.x[lx - 2], X1 = cg.x[lx], Y0 = cg.y[ly - 2], Y1 = cg.y[ly];
+  const double za = bl(x, y, X0, X1, Y0, Y1, cvar[ix3(cd, vi - 1, vj - 1, vk - 1)], cvar[ix3(cd, vi + 1, vj - 1, vk - 1)],
+                       cvar[ix3(cd, vi - 1, vj + 1, vk - 1)], cvar[ix3(cd, vi + 1, vj + 1, vk - 1)]);
+  const double zb = bl(x, y, X0, X1, Y0, Y1, cvar[ix3(cd, vi - 1, vj - 1, vk + 1)], cvar[ix3(cd, vi + 1, vj - 1, vk + 1)],
+                       cvar[ix3(cd, vi - 1, vj + 1, vk + 1)], cvar[ix3(cd, vi + 1, vj + 1, vk + 1)]);
+  const double Z0 = cg.z[lz - 2], Z1 = cg.z[lz];
+  const double out = za + (z - Z0) * (zb - za) / (Z1 - Z0);
+  const int ox = (comp == 1), oy = (comp == 2), oz = (comp == 3);
+  const int64_t dst = ix3(fd, tx - ox, ty - oy, tz - oz);
+  fvar[dst] = out;
+  if (fvar2) fvar2[dst] = out;
+}
+
+static GridSet grids_of(const Block &b, int comp) {
+  switch (comp) {
+    case 0: return {b.g[BG_XP], b.g[BG_YP], b.g[BG_ZP]};
+    case 1: return {b.g[BG_XU], b.g[BG_YU], b.g[BG_ZU]};
+    case 2: return {b.g[BG_XV], b.g[BG_YV], b.g[BG_ZV]};
+    default: return {b.g[BG_XW], b.g[BG_YW], b.g[BG_ZW]};
+  }
+}
+
+static ITab tab_of(const Interface &it, int comp) {
+  ITab t;
+  for (int a = 0; a < 3; ++a) { t.det[a] = it.det[comp][a]; t.cnt[a] = (int)it.counter[comp][a]; }
+  return t;
+}
+
+// which: 0 = p,u,v,w ; 1 = ut,vt,wt ; 2 = pc
+int interface_coarse_update(int which) {
+  hipStream_t s = rt().stream;
+  for (int q = 1; q <= kMaxInterfaces; ++q) {
+    Interface &it = rt().itf[q];
+    if (!it.live) continue;
+    Block &a = rt().blk[it.a], &b = rt().blk[it.b];
+    const int cintp = (int)b.cintp;
+    auto run = [&](int comp, const double *fv, double *cv, double *cv2) {
+      ITab t = tab_of(it, comp);
+      const int64_t n = (int64_t)(t.cnt[0] - 2 * cintp + 1) * (t.cnt[1] - 2 * cintp + 1) * (t.cnt[2] - 2 * cintp + 1);
+      if (n <= 0) return;
+      hipLaunchKernelGGL(k_restrict, dim3(div_up(n, 256)), dim3(256), 0, s, a.d, b.d, t, comp, cintp,
+                         grids_of(a, comp), grids_of(b, comp), fv, cv, cv2);
+    };
+    if (which == 0) {
+      run(0, b.f[BF_P], a.f[BF_P], nullptr);
+      run(1, b.f[BF_U], a.f[BF_U], nullptr);
+      run(2, b.f[BF_V], a.f[BF_V], nullptr);
+      run(3, b.f[BF_W], a.f[BF_W], nullptr);
+    } else if (which == 1) {
+      run(1, b.f[BF_UT], a.f[BF_UT], nullptr);
+      run(2, b.f[BF_VT], a.f[BF_VT], nullptr);
+      run(3, b.f[BF_WT], a.f[BF_WT], nullptr);
+    } else {
+      run(0, b.f[BF_PC], a.f[BF_PC], nullptr);
+    }
+  }
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+int interface_fine_update(int which) {
+  hipStream_t s = rt().stream;
+  for (int q = 1; q <= kMaxInterfaces; ++q) {
+    Interface &it = rt().itf[q];
+    if (!it.live) continue;
+    Block &a = rt().blk[it.a], &b = rt().blk[it.b];
+    const int factor = (int)(it.b_msh / it.a_msh);
+    auto run = [&](int comp, const double *cv, double *fv) {
+      ITab t = tab_of(it, comp);
+      for (int axis = 0; axis < 3; ++axis) {
+        int n[3] = {t.cnt[0], t.cnt[1], t.cnt[2]};
+        n[axis] = 2;
+        const int64_t total = (int64_t)n[0] * n[1] * n[2] * factor * factor * factor;
+        hipLaunchKernelGGL(k_prolong, dim3(div_up(total, 256)), dim3(256), 0, s, a.d, b.d, t, comp, axis,
+                           grids_of(a, comp), grids_of(b, comp), cv, fv, (double *)nullptr, factor);
+      }
+    };
+    if (which == 0) {
+      run(0, a.f[BF_P], b.f[BF_P]);
+      run(1, a.f[BF_U], b.f[BF_U]);
+      run(2, a.f[BF_V], b.f[BF_V]);
+      run(3, a.f[BF_W], b.f[BF_W]);
+    } else if (which == 1) {
+      run(1, a.f[BF_UT], b.f[BF_UT]);
+      run(2, a.f[BF_VT], b.f[BF_VT]);
+      run(3, a.f[BF_WT], b.f[BF_WT]);
+    } else {
+      run(0, a.f[BF_PC], b.f[BF_PC]);
+    }
+  }
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace bcfd
+
+using namespace bcfd;
+
+extern "C" {
+
+int biocfd_interface_create(int itf, int a_blk, int b_blk, int64_t a_msh, int64_t b_msh, const double ext[6]) {
+  if (ensure_init()) return -1;
+  if (itf < 1 || itf > kMaxInterfaces) { set_error("interface id %d out of range", itf); return -1; }
+  Block *a = get_block(a_blk);
+  Block *b = get_block(b_blk);
+  if (!a || !b) return -1;
+  if (a_msh <= 0 || b_msh % a_msh != 0) { set_error("interface: refinement ratio must be an integer"); return -1; }
+  Interface &it = rt().itf[itf];
+  for (auto &row : it.det)
+    for (auto &p : row)
+      if (p) { (void)hipFree(p); p = nullptr; }
+  it = Interface();
+  it.a = a_blk; it.b = b_blk; it.a_msh = a_msh; it.b_msh = b_msh;
+  for (int q = 0; q < 6; ++q) it.ext[q] = ext[q];
+  const int64_t factor = b_msh / a_msh;
+  const int gx[4] = {BG_XP, BG_XU, BG_XV, BG_XW}, gy[4] = {BG_YP, BG_YU, BG_YV, BG_YW},
+            gz[4] = {BG_ZP, BG_ZU, BG_ZV, BG_ZW};
+  for (int c = 0; c < 4; ++c) {
+    const int grid[3] = {gx[c], gy[c], gz[c]};
+    for (int ax = 0; ax < 3; ++ax) {
+      if (a->hg[grid[ax]].empty()) { set_error("interface: coarse grids not uploaded"); return -1; }
+      const int64_t starter = (c == ax + 1) ? 3 : 2;  // normal velocity component
+      interface_axis(a->hg[grid[ax]], ext[2 * ax], ext[2 * ax + 1], factor, starter, it.hdet[c][ax],
+                     &it.counter[c][ax]);
+      std::vector<int> t32(it.hdet[c][ax].begin(), it.hdet[c][ax].end());
+      if (dev_alloc(&it.det[c][ax], (int64_t)t32.size())) return -1;
+      BCFD_HIP(hipMemcpy(it.det[c][ax], t32.data(), sizeof(int) * t32.size(), hipMemcpyHostToDevice));
+    }
+  }
+  it.live = true;
+  return 0;
+}
+
+int biocfd_interface_table(int itf, int comp, int axis, int64_t *host, int64_t *counter, int64_t cap) {
+  if (ensure_init()) return -1;
+  if (itf < 1 || itf > kMaxInterfaces || !rt().itf[itf].live || comp < 0 || comp > 3 || axis < 0 || axis > 2) {
+    set_error("interface_table: bad request");
+    return -1;
+  }
+  const Interface &it = rt().itf[itf];
+  *counter = it.counter[comp][axis];
+  const std::vector<int64_t> &v = it.hdet[comp][axis];
+  if ((int64_t)v.size() > 3 * cap) { set_error("interface_table: capacity too small"); return -1; }
+  for (size_t q = 0; q < v.size(); ++q) host[q] = v[q];
+  return 0;
+}
+
+int biocfd_coarse_update(int which) {
+  if (ensure_init()) return -1;
+  if (which < 0 || which > 2) { set_error("coarse_update: bad selector %d", which); return -1; }
+  return interface_coarse_update(which);
+}
+
+int biocfd_fine_update_bd(int which) {
+  if (ensure_init()) return -1;
+  if (which < 0 || which > 2) { set_error("fine_update_bd: bad selector %d", which); return -1; }
+  return interface_fine_update(which);
+}
+
+}  // extern "C"

@@ -1,0 +1,438 @@
+// Explicit momentum step on gfx950 (reference nsMomentum2order,
+// src/NavierStokes.f90:252-912): ut = u + dt*(-convection + laplacian/re
+// - grad p) per fluid cell, third-order non-uniform upwind convection away
+// from the body and first-order donor-cell convection next to it.  FP64, same
+// operand order as the reference.
+//
+// Thin-surface (TS) cells: for fluid cells face-adjacent to a TS cell the
+// reference temporarily substitutes ghost values into neighbouring u,v,w,p
+// entries, evaluates the stencil and then writes the "t" ghost values back
+// (NavierStokes.f90:310-490, 860-907), all in place and in fluid-list order.
+// That order is reproduced exactly: cells whose 5x5x5 neighbourhood contains
+// a position that any TS-adjacent cell writes are processed afterwards, one
+// by one in list order, by a single workgroup; every other fluid cell reads
+// only untouched values and runs in the dense parallel kernel.
+#include "biocfd_rt.hpp"
+
+namespace bcfd {
+
+struct MomArgs {
+  Dims d;
+  const uint8_t *cell, *cell2;
+  const double *dx, *dy, *dz;
+  const double *ca[BTB_N], *ck[BTB_N];
+  double dt, re;
+};
+
+struct Fields {
+  const double *u, *v, *w, *p;
+};
+
+__device__ inline bool adj_ok(const Dims &d, const uint8_t *t, bool tall, int i, int j, int k, int s) {
+  auto at = [&](int a, int b, int c) { return tall ? t[ixt(d, a, b, c)] : t[ix3(d, a, b, c)]; };
+  if (at(i - s, j, k) == 2) return false;
+  if (at(i + s, j, k) == 2) return false;
+  if (at(i, j - s, k) == 2) return false;
+  if (at(i, j + s, k) == 2) return false;
+  if (at(i, j, k - s) == 2) return false;
+  if (at(i, j, k + s) == 2) return false;
+  return true;
+}
+
+// vel*(ca1 f+2 + ca2 f+1 + ca3 f0 + ca4 f-1 + ca5 f-2)/(ca6 hA)
+//   + |vel|*(ck1 f+2 + ... + ck5 f-2)/(2 ck6 hB)       (NavierStokes.f90:619-647)
+__device__ inline double upwind3(double vel, const double *ca, const double *ck, int col, double f2,
+                                 double f1, double f0, double fm1, double fm2, double hA, double hB) {
+  const double *a = ca + 6 * (col - 1);
+  const double *c = ck + 6 * (col - 1);
+  return vel * (a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2) / (a[5] * hA) +
+         fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2) / (2.0 * c[5] * hB);
+}
+
+__device__ void momentum_cell(const MomArgs &a, const Fields &F, int i, int j, int k, double &out_u,
+                              double &out_v, double &out_w) {
+  const Dims &d = a.d;
+  const int64_t c = ix3(d, i, j, k);
+  const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
+  const double *u = F.u, *v = F.v, *w = F.w, *p = F.p;
+  const double alpha = 1.0;
+#define U(di, dj, dk) u[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
+#define V(di, dj, dk) v[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
+#define W(di, dj, dk) w[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
+#define P(di, dj, dk) p[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
+  const double dxr = a.dx[i], dx = a.dx[i - 1], dxl = a.dx[i - 2];
+  const double dye = a.dy[j], dy = a.dy[j - 1], dyw = a.dy[j - 2];
+  const double dzt = a.dz[k], dz = a.dz[k - 1], dzb = a.dz[k - 2];
+
+  const double u1a = U(-1, 0, 0) + U(0, 0, 0), u22 = U(-1, 0, 0) - U(0, 0, 0);
+  const double u3 = U(0, 0, 0) + U(1, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
+  const double u5 = U(0, -1, 0) + U(0, 0, 0), u6 = U(0, -1, 0) - U(0, 0, 0);
+  const double u7 = U(0, 0, 0) + U(0, 1, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
+  const double u9 = U(0, 0, -1) + U(0, 0, 0), u10 = U(0, 0, -1) - U(0, 0, 0);
+  const double u11 = U(0, 0, 0) + U(0, 0, 1), u12 = U(0, 0, 0) - U(0, 0, 1);
+  const double u13 = U(-1, 0, 0) + U(-1, 1, 0), u14 = u7;
+  const double u15 = U(-1, 0, 0) + U(-1, 0, 1), u16 = u11;
+
+  const double v1a = V(0, -1, 0) + V(1, -1, 0), v22 = V(0, 0, 0) + V(1, 0, 0);
+  const double v3 = V(-1, 0, 0) + V(0, 0, 0), v4 = V(-1, 0, 0) - V(0, 0, 0);
+  const double v5 = V(0, 0, 0) + V(1, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
+  const double v7 = V(0, -1, 0) + V(0, 0, 0), v8 = V(0, -1, 0) - V(0, 0, 0);
+  const double v9 = V(0, 0, 0) + V(0, 1, 0), v10 = V(0, 0, 0) - V(0, 1, 0);
+  const double v11 = V(0, 0, -1) + V(0, 0, 0), v12 = V(0, 0, -1) - V(0, 0, 0);
+  const double v13 = V(0, 0, 0) + V(0, 0, 1), v14 = V(0, 0, 0) - V(0, 0, 1);
+  const double v15 = V(0, -1, 0) + V(0, -1, 1), v16 = v13;
+
+  const double w1a = W(0, 0, -1) + W(1, 0, -1), w22 = W(0, 0, 0) + W(1, 0, 0);
+  const double w3 = W(0, 0, -1) + W(0, 1, -1), w4 = W(0, 0, 0) + W(0, 1, 0);
+  const double w5 = W(-1, 0, 0) + W(0, 0, 0), w6 = W(-1, 0, 0) - W(0, 0, 0);
+  const double w7 = W(0, 0, 0) + W(1, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
+  const double w9 = W(0, -1, 0) + W(0, 0, 0), w10 = W(0, -1, 0) - W(0, 0, 0);
+  const double w11 = W(0, 0, 0) + W(0, 1, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
+  const double w13 = W(0, 0, -1) + W(0, 0, 0), w14 = W(0, 0, -1) - W(0, 0, 0);
+  const double w15 = W(0, 0, 0) + W(0, 0, 1), w16 = W(0, 0, 0) - W(0, 0, 1);
+
+  const double dpdx = (P(0, 0, 0) - P(1, 0, 0)) / (0.5 * (dxr + dx));
+  const double dpdy = (P(0, 0, 0) - P(0, 1, 0)) / (0.5 * (dye + dy));
+  const double dpdz = (P(0, 0, 0) - P(0, 0, 1)) / (0.5 * (dzt + dz));
+  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw;
+  const double dz2zt = dz + dzt, dz2zb = dz + dzb;
+  const int nx = d.nx, ny = d.ny, nz = d.nz;
+  // the near-body test is shared by the three components; only the index
+  // window differs (NavierStokes.f90:602, 690, 782)
+  // (evaluated only inside the windows, as the reference's nested ifs do)
+  const bool near_ok = i > 2 && j > 2 && k > 2 && adj_ok(d, a.cell, false, i, j, k, 1) &&
+                       adj_ok(d, a.cell2, true, i, j, k, 2);
+
+  // ---- u ----
+  double duudx, dvudy, dwudz;
+  {
+    const double vu_e = V(1, 0, 0) + (dxr / (dx2xr)) * (V(0, 0, 0) - V(1, 0, 0));
+    const double vu_w = V(1, -1, 0) + (dxr / (dx2xr)) * (V(0, -1, 0) - V(1, -1, 0));
+    const double v_in = 0.5 * (vu_e + vu_w);
+    const double wu_n = W(1, 0, 0) + (dxr / (dx2xr)) * (W(0, 0, 0) - W(1, 0, 0));
+    const double wu_s = W(1, 0, -1) + (dxr / (dx2xr)) * (W(0, 0, -1) - W(1, 0, -1));
+    const double w_in = 0.5 * (wu_n + wu_s);
+    if (i > 2 && i < nx && j > 2 && j < ny + 1 && k > 2 && k < nz + 1 && near_ok) {
+      const double ddy = 0.5 * (a.dy[j - 1] + a.dy[j - 2]), ddye = 0.5 * (a.dy[j - 1] + a.dy[j]);
+      const double ddz = 0.5 * (a.dz[k - 1] + a.dz[k - 2]), ddzr = 0.5 * (a.dz[k - 1] + a.dz[k]);
+      duudx = upwind3(U(0, 0, 0), a.ca[BTB_UU], a.ck[BTB_UU], i, U(2, 0, 0), U(1, 0, 0), U(0, 0, 0),
+                      U(-1, 0, 0), U(-2, 0, 0), a.dx[i], a.dx[i - 1]);
+      dvudy = upwind3(v_in, a.ca[BTB_VU], a.ck[BTB_VU], j, U(0, 2, 0), U(0, 1, 0), U(0, 0, 0),
+                      U(0, -1, 0), U(0, -2, 0), ddye, ddy);
+      dwudz = upwind3(w_in, a.ca[BTB_WU], a.ck[BTB_WU], k, U(0, 0, 2), U(0, 0, 1), U(0, 0, 0),
+                      U(0, 0, -1), U(0, 0, -2), ddzr, ddz);
+    } else {
+      duudx = -0.25 * (u1a * u1a + alpha * fabs(u1a) * u22 - u3 * u3 - alpha * fabs(u3) * u4) / a.dx[i - 1];
+      dvudy = -0.25 * (v1a * u5 + alpha * fabs(v1a) * u6 - v22 * u7 - alpha * fabs(v22) * u8) / a.dy[j - 1];
+      dwudz = -0.25 * (w1a * u9 + alpha * fabs(w1a) * u10 - w22 * u11 - alpha * fabs(w22) * u12) / a.dz[k - 1];
+    }
+    const double d2x = (2.0 / dx2xr) * ((-u4 / dxr) + (u22 / dx));
+    const double d2y = (2.0 / (0.5 * (dy2ye + dy2yw))) * ((-u8 / (0.5 * dy2ye)) + (u6 / (0.5 * dy2yw)));
+    const double d2z = (2.0 / (0.5 * (dz2zt + dz2zb))) * ((-u12 / (0.5 * dz2zt)) + (u10 / (0.5 * dz2zb)));
+    const double visc = (d2x + d2y + d2z) / a.re;
+    const double res = (-duudx - dvudy - dwudz + visc);
+    out_u = U(0, 0, 0) + a.dt * (res + dpdx);
+  }
+  // ---- v ----
+  {
+    const double uv_e = U(0, 1, 0) + (dye / (dy2ye)) * (U(0, 0, 0) - U(0, 1, 0));
+    const double uv_w = U(-1, 1, 0) + (dye / (dy2ye)) * (U(-1, 0, 0) - U(-1, 1, 0));
+    const double u_in = 0.5 * (uv_e + uv_w);
+    const double wv_n = W(0, 1, 0) + (dye / (dy2ye)) * (W(0, 0, 0) - W(0, 1, 0));
+    const double wv_s = W(0, 1, -1) + (dye / (dy2ye)) * (W(0, 0, -1) - W(0, 1, -1));
+    const double w_in = 0.5 * (wv_n + wv_s);
+    double duvdx, dvvdy, dwvdz;
+    if (i > 2 && i < nx + 1 && j > 2 && j < ny && k > 2 && k < nz + 1 && near_ok) {
+      const double ddx = 0.5 * (a.dx[i - 1] + a.dx[i - 2]), ddxr = 0.5 * (a.dx[i - 1] + a.dx[i]);
+      const double ddz = 0.5 * (a.dz[k - 1] + a.dz[k - 2]), ddzr = 0.5 * (a.dz[k - 1] + a.dz[k]);
+      duvdx = upwind3(u_in, a.ca[BTB_UW], a.ck[BTB_UW], i, V(2, 0, 0), V(1, 0, 0), V(0, 0, 0),
+                      V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx);
+      dvvdy = upwind3(V(0, 0, 0), a.ca[BTB_VV], a.ck[BTB_VV], j, V(0, 2, 0), V(0, 1, 0), V(0, 0, 0),
+                      V(0, -1, 0), V(0, -2, 0), a.dy[j], a.dy[j - 1]);
+      dwvdz = upwind3(w_in, a.ca[BTB_WU], a.ck[BTB_WU], k, V(0, 0, 2), V(0, 0, 1), V(0, 0, 0),
+                      V(0, 0, -1), V(0, 0, -2), ddzr, ddz);
+    } else {
+      duvdx = -0.25 * (u13 * v3 + alpha * fabs(u13) * v4 - u14 * v5 - alpha * fabs(u14) * v6) / a.dx[i - 1];
+      dvvdy = -0.25 * (v7 * v7 + alpha * fabs(v7) * v8 - v9 * v9 - alpha * fabs(v9) * v10) / a.dy[j - 1];
+      dwvdz = -0.25 * (w3 * v11 + alpha * fabs(w3) * v12 - w4 * v13 - alpha * fabs(w4) * v14) / a.dz[k - 1];
+    }
+    const double d2x = (2.0 / (0.5 * (dx2xr + dx2xl))) * ((-v6 / (0.5 * dx2xr)) + (v4 / (0.5 * dx2xl)));
+    const double d2y = (2.0 / dy2ye) * ((-v10 / dye) + (v8 / dy));
+    const double d2z = (2.0 / (0.5 * (dz2zt + dz2zb))) * ((-v14 / (0.5 * dz2zt)) + (v12 / (0.5 * dz2zb)));
+    const double visc = (d2x + d2y + d2z) / a.re;
+    const double res = (-duvdx - dvvdy - dwvdz + visc);
+    out_v = V(0, 0, 0) + a.dt * (res + dpdy);
+  }
+  // ---- w ----
+  {
+    const double uw_e = U(0, 0, 1) + (dzt / (dz2zt)) * (U(0, 0, 0) - U(0, 0, 1));
+    const double uw_w = U(-1, 0, 1) + (dzt / (dz2zt)) * (U(-1, 0, 0) - U(-1, 0, 1));
+    const double u_in = 0.5 * (uw_e + uw_w);
+    const double vw_n = V(0, 0, 1) + (dzt / (dz2zt)) * (V(0, 0, 0) - V(0, 0, 1));
+    const double vw_s = V(0, -1, 1) + (dzt / (dz2zt)) * (V(0, -1, 0) - V(0, -1, 1));
+    const double v_in = 0.5 * (vw_n + vw_s);
+    double duwdx, dvwdy, dwwdz;
+    if (i > 2 && i < nx + 1 && j > 2 && j < ny + 1 && k > 2 && k < nz && near_ok) {
+      const double ddx = 0.5 * (a.dx[i - 1] + a.dx[i - 2]), ddxr = 0.5 * (a.dx[i - 1] + a.dx[i]);
+      const double ddy = 0.5 * (a.dy[j - 1] + a.dy[j - 2]), ddye = 0.5 * (a.dy[j - 1] + a.dy[j]);
+      duwdx = upwind3(u_in, a.ca[BTB_UW], a.ck[BTB_UW], i, W(2, 0, 0), W(1, 0, 0), W(0, 0, 0),
+                      W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx);
+      dvwdy = upwind3(v_in, a.ca[BTB_VW], a.ck[BTB_VW], j, W(0, 2, 0), W(0, 1, 0), W(0, 0, 0),
+                      W(0, -1, 0), W(0, -2, 0), ddye, ddy);
+      dwwdz = upwind3(W(0, 0, 0), a.ca[BTB_WW], a.ck[BTB_WW], k, W(0, 0, 2), W(0, 0, 1), W(0, 0, 0),
+                      W(0, 0, -1), W(0, 0, -2), a.dz[k], a.dz[k - 1]);
+    } else {
+      duwdx = -0.5 * (u15 * w5 + alpha * fabs(u15) * w6 - u16 * w7 - alpha * fabs(u16) * w8) / a.dx[i - 1];
+      dvwdy = -0.5 * (v15 * w9 + alpha * fabs(v15) * w10 - v16 * w11 - alpha * fabs(v16) * w12) / a.dy[j - 1];
+      dwwdz = -0.5 * (w13 * w13 + alpha * fabs(w13) * w14 - w15 * w15 - alpha * fabs(w15) * w16) / a.dz[k - 1];
+    }
+    const double d2x = (2.0 / (0.5 * (dx2xr + dx2xl))) * ((-w8 / (0.5 * dx2xr)) + (w6 / (0.5 * dx2xl)));
+    const double d2y = (2.0 / (0.5 * (dy2ye + dy2yw))) * ((-w12 / (0.5 * dy2ye)) + (w10 / (0.5 * dy2yw)));
+    const double d2z = (2.0 / dz2zt) * ((-w16 / dzt) + (w14 / dz));
+    const double visc = (d2x + d2y + d2z) / a.re;
+    const double res = (-duwdx - dvwdy - dwwdz + visc);
+    out_w = W(0, 0, 0) + a.dt * (res + dpdz);
+  }
+#undef U
+#undef V
+#undef W
+#undef P
+}
+
+// Dense momentum kernel over fluid cells not deferred to the ordered pass.
+__global__ void __launch_bounds__(256) k_momentum(MomArgs a, Fields F, double *__restrict__ ut,
+                                                  double *__restrict__ vt, double *__restrict__ wt,
+                                                  const uint8_t *__restrict__ deferred) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > a.d.nx + 1 || j > a.d.ny + 1) return;
+  const int64_t c = ix3(a.d, i, j, k);
+  if (a.cell[c] != 0) return;
+  if (deferred && deferred[c]) return;
+  double ou, ov, ow;
+  momentum_cell(a, F, i, j, k, ou, ov, ow);
+  ut[c] = ou;
+  vt[c] = ov;
+  wt[c] = ow;
+}
+
+// ---------------------------------------------------------------------------
+// TS handling
+// ---------------------------------------------------------------------------
+struct TsArgs {
+  const int *ts_ijk;          // 3*cap (i list, j list, k list)
+  int64_t cap_ts;
+  const int *index_ts;        // 1-based IB ordinal
+  const int *nel[BN_NNEL];
+  const double *elem[BE_NELEM];
+  const double *ghost[BGH_N];
+  const double *x1, *y1, *z1, *xp, *yp, *zp;
+  int n_ts;
+};
+
+// positions the ordered pass may write, per direction (dir 0..5 = +x,-x,+y,-y,+z,-z):
+// {field (0 u,1 v,2 w,3 p), di, dj, dk, dot-test (BN_*), ghost, restore ghost}
+struct Target { int8_t f, di, dj, dk, test, g, gt; };
+__constant__ Target kTargets[6][6] = {
+    {{0, 1, 0, 0, BN_U2, BGH_U2, BGH_U2T}, {1, 1, 0, 0, BN_V2, BGH_V2, BGH_V2T}, {1, 1, -1, 0, BN_V1, BGH_V1, BGH_V1T},
+     {2, 1, 0, 0, BN_W2, BGH_W2, BGH_W2T}, {2, 1, 0, -1, BN_W1, BGH_W1, BGH_W1T}, {3, 1, 0, 0, BN_P, BGH_P, BGH_PT}},
+    {{0, -2, 0, 0, BN_U1, BGH_U1, BGH_U1T}, {1, -1, 0, 0, BN_V2, BGH_V2, BGH_V2T}, {1, -1, -1, 0, BN_V1, BGH_V1, BGH_V1T},
+     {2, -1, 0, 0, BN_W2, BGH_W2, BGH_W2T}, {2, -1, 0, -1, BN_W1, BGH_W1, BGH_W1T}, {3, -1, 0, 0, BN_P, BGH_P, BGH_PT}},
+    {{0, 0, 1, 0, BN_U2, BGH_U2, BGH_U2T}, {0, -1, 1, 0, BN_U1, BGH_U1, BGH_U1T}, {1, 0, 1, 0, BN_V2, BGH_V2, BGH_V2T},
+     {2, 0, 1, 0, BN_W2, BGH_W2, BGH_W2T}, {2, 0, 1, -1, BN_W1, BGH_W1, BGH_W1T}, {3, 0, 1, 0, BN_P, BGH_P, BGH_PT}},
+    {{0, 0, -1, 0, BN_U2, BGH_U2, BGH_U2T}, {0, -1, -1, 0, BN_U1, BGH_U1, BGH_U1T}, {1, 0, -2, 0, BN_V1, BGH_V1, BGH_V1T},
+     {2, 0, -1, 0, BN_W2, BGH_W2, BGH_W2T}, {2, 0, -1, -1, BN_W1, BGH_W1, BGH_W1T}, {3, 0, -1, 0, BN_P, BGH_P, BGH_PT}},
+    {{0, 0, 0, 1, BN_U2, BGH_U2, BGH_U2T}, {0, -1, 0, 1, BN_U1, BGH_U1, BGH_U1T}, {1, 0, 0, 1, BN_V2, BGH_V2, BGH_V2T},
+     {1, 0, -1, 1, BN_V1, BGH_V1, BGH_V1T}, {2, 0, 0, 1, BN_W2, BGH_W2, BGH_W2T}, {3, 0, 0, 1, BN_P, BGH_P, BGH_PT}},
+    {{0, 0, 0, -1, BN_U2, BGH_U2, BGH_U2T}, {0, -1, 0, -1, BN_U1, BGH_U1, BGH_U1T}, {1, 0, 0, -1, BN_V2, BGH_V2, BGH_V2T},
+     {1, 0, -1, -1, BN_V1, BGH_V1, BGH_V1T}, {2, 0, 0, -2, BN_W1, BGH_W1, BGH_W1T}, {3, 0, 0, -1, BN_P, BGH_P, BGH_PT}},
+};
+__constant__ int8_t kDir[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+
+__device__ inline bool ts_adjacent(const Dims &d, const uint8_t *cell2, int i, int j, int k, unsigned *mask) {
+  unsigned m = 0;
+  for (int q = 0; q < 6; ++q)
+    if (cell2[ixt(d, i + kDir[q][0], j + kDir[q][1], k + kDir[q][2])] == 2) m |= 1u << q;
+  *mask = m;
+  return m != 0;
+}
+
+// mark every position a TS-adjacent fluid cell may overwrite
+__global__ void k_ts_touch(Dims d, const uint8_t *__restrict__ cell, const uint8_t *__restrict__ cell2,
+                           uint8_t *__restrict__ touched) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > d.nx + 1 || j > d.ny + 1) return;
+  if (cell[ix3(d, i, j, k)] != 0) return;
+  unsigned m;
+  if (!ts_adjacent(d, cell2, i, j, k, &m)) return;
+  for (int q = 0; q < 6; ++q) {
+    if (!(m & (1u << q))) continue;
+    for (int t = 0; t < 6; ++t) {
+      const Target T = kTargets[q][t];
+      touched[ix3(d, i + T.di, j + T.dj, k + T.dk)] = 1;
+    }
+  }
+}
+
+// defer a fluid cell if it is TS-adjacent or its 5x5x5 neighbourhood holds a
+// touched position
+__global__ void k_ts_defer(Dims d, const uint8_t *__restrict__ cell, const uint8_t *__restrict__ cell2,
+                           const uint8_t *__restrict__ touched, uint8_t *__restrict__ deferred) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  if (i > d.nx + 1 || j > d.ny + 1) return;
+  const int64_t c = ix3(d, i, j, k);
+  uint8_t r = 0;
+  if (cell[c] == 0) {
+    unsigned m;
+    if (ts_adjacent(d, cell2, i, j, k, &m)) r = 1;
+    for (int dk = -2; dk <= 2 && !r; ++dk)
+      for (int dj = -2; dj <= 2 && !r; ++dj)
+        for (int di = -2; di <= 2 && !r; ++di) {
+          const int a = i + di, b = j + dj, e = k + dk;
+          if (a < 1 || b < 1 || e < 1 || a > d.NX || b > d.NY || e > d.NZ) continue;
+          if (touched[ix3(d, a, b, e)]) r = 1;
+        }
+  }
+  deferred[c] = r;
+}
+
+// Ordered pass over the deferred cells (list in fluid order), one workgroup.
+__global__ void __launch_bounds__(256) k_ts_ordered(MomArgs a, double *u, double *v, double *w, double *p,
+                                                    double *ut, double *vt, double *wt, const int *list,
+                                                    int n_list, TsArgs ts) {
+  __shared__ int s_best[36];
+  __shared__ double s_val[36];
+  const Dims &d = a.d;
+  double *fld[4] = {u, v, w, p};
+  for (int q = 0; q < n_list; ++q) {
+    const int lin = list[q];
+    const int i = 2 + lin % d.nx;
+    const int j = 2 + (lin / d.nx) % d.ny;
+    const int k = 2 + lin / (d.nx * d.ny);
+    unsigned m = 0;
+    const bool adj = ts_adjacent(d, a.cell2, i, j, k, &m);
+    if (adj) {
+      if (threadIdx.x < 36) s_best[threadIdx.x] = 0;
+      __syncthreads();
+      const double xs[7] = {ts.x1[i], ts.x1[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1]};
+      const double ys[7] = {ts.yp[j - 1], ts.yp[j - 1], ts.y1[j], ts.y1[j - 1], ts.yp[j - 1], ts.yp[j - 1], ts.yp[j - 1]};
+      const double zs[7] = {ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.z1[k], ts.z1[k - 1], ts.zp[k - 1]};
+      // the seven tests in the order u2,u1,v2,v1,w2,w1,p mapped to BN_* slots
+      const int test_nel[7] = {BN_U2, BN_U1, BN_V2, BN_V1, BN_W2, BN_W1, BN_P};
+      for (int nn = 1 + threadIdx.x; nn <= ts.n_ts; nn += blockDim.x) {
+        const int bt = ts.index_ts[nn - 1];
+        bool neg[BN_NNEL];
+        for (int t = 0; t < 7; ++t) {
+          const int e = ts.nel[test_nel[t]][bt - 1];
+          const double dn = (xs[t] - ts.elem[BE_XCENT][e - 1]) * ts.elem[BE_COSA][e - 1] +
+                            (ys[t] - ts.elem[BE_YCENT][e - 1]) * ts.elem[BE_COSB][e - 1] +
+                            (zs[t] - ts.elem[BE_ZCENT][e - 1]) * ts.elem[BE_COSG][e - 1];
+          neg[test_nel[t]] = dn < 0;
+        }
+        for (int dq = 0; dq < 6; ++dq) {
+          if (!(m & (1u << dq))) continue;
+          for (int t = 0; t < 6; ++t)
+            if (neg[kTargets[dq][t].test]) atomicMax(&s_best[dq * 6 + t], nn);
+        }
+      }
+      __syncthreads();
+      if (threadIdx.x < 36) {
+        const int dq = threadIdx.x / 6, t = threadIdx.x % 6;
+        const int nn = s_best[threadIdx.x];
+        if ((m & (1u << dq)) && nn > 0) {
+          const Target T = kTargets[dq][t];
+          fld[T.f][ix3(d, i + T.di, j + T.dj, k + T.dk)] = ts.ghost[T.g][nn - 1];
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      Fields F{u, v, w, p};
+      double ou, ov, ow;
+      momentum_cell(a, F, i, j, k, ou, ov, ow);
+      const int64_t c = ix3(d, i, j, k);
+      ut[c] = ou;
+      vt[c] = ov;
+      wt[c] = ow;
+      if (adj) {
+        for (int dq = 0; dq < 6; ++dq) {
+          if (!(m & (1u << dq))) continue;
+          for (int t = 0; t < 6; ++t) {
+            const Target T = kTargets[dq][t];
+            fld[T.f][ix3(d, i + T.di, j + T.dj, k + T.dk)] = ts.ghost[T.gt][ts.n_ts - 1];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+int compact_mask_lin(Block &b, const uint8_t *mask, int **out, int64_t *cap, int64_t *count);
+
+}  // namespace bcfd
+
+using namespace bcfd;
+
+extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
+  Block *b = get_block(id);
+  if (!b) return -1;
+  for (int t = 0; t < BTB_N; ++t)
+    if (!b->ca[t]) { set_error("non_uni_coeff has not been called for block %d", id); return -1; }
+  if (!b->g[BG_DX]) { set_error("grids not uploaded"); return -1; }
+  MomArgs a;
+  a.d = b->d;
+  a.cell = b->tag[BT_CELL];
+  a.cell2 = b->tag[BT_CELL2];
+  a.dx = b->g[BG_DX]; a.dy = b->g[BG_DY]; a.dz = b->g[BG_DZ];
+  for (int t = 0; t < BTB_N; ++t) { a.ca[t] = b->ca[t]; a.ck[t] = b->ck[t]; }
+  a.dt = dt;
+  a.re = re;
+  Fields F{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
+  hipStream_t s = rt().stream;
+  dim3 blk(64, 4, 1);
+  dim3 grd(div_up(b->d.nx, 64), div_up(b->d.ny, 4), (unsigned)b->d.nz);
+  const int n_ts = (int)b->counts[BC_TS];
+  if (n_ts == 0) {
+    hipLaunchKernelGGL(k_momentum, grd, blk, 0, s, a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT],
+                       (const uint8_t *)nullptr);
+    BCFD_CHECK_LAUNCH();
+    return 0;
+  }
+  // TS path: touched positions -> deferred mask -> dense pass -> ordered pass
+  uint8_t *touched = nullptr, *deferred = nullptr;
+  if (dev_alloc(&touched, b->n2) || dev_alloc(&deferred, b->n2)) return -1;
+  BCFD_HIP(hipMemsetAsync(touched, 0, b->n2, s));
+  BCFD_HIP(hipMemsetAsync(deferred, 0, b->n2, s));
+  hipLaunchKernelGGL(k_ts_touch, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched);
+  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched, deferred);
+  hipLaunchKernelGGL(k_momentum, grd, blk, 0, s, a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT],
+                     (const uint8_t *)deferred);
+  BCFD_CHECK_LAUNCH();
+  int *list = nullptr;
+  int64_t cap = 0, cnt = 0;
+  if (compact_mask_lin(*b, deferred, &list, &cap, &cnt)) return -1;
+  TsArgs ts{};
+  IbData &ib = b->ib;
+  ts.ts_ijk = ib.ts_ijk; ts.cap_ts = ib.cap_ts; ts.index_ts = ib.index_ts;
+  for (int q = 0; q < BN_NNEL; ++q) ts.nel[q] = ib.nel[q];
+  for (int q = 0; q < BE_NELEM; ++q) ts.elem[q] = ib.elem[q];
+  for (int q = 0; q < BGH_N; ++q) ts.ghost[q] = ib.ghost[q];
+  ts.x1 = b->g[BG_X1]; ts.y1 = b->g[BG_Y1]; ts.z1 = b->g[BG_Z1];
+  ts.xp = b->g[BG_XP]; ts.yp = b->g[BG_YP]; ts.zp = b->g[BG_ZP];
+  ts.n_ts = n_ts;
+  if (cnt > 0)
+    hipLaunchKernelGGL(k_ts_ordered, dim3(1), dim3(256), 0, s, a, b->f[BF_U], b->f[BF_V], b->f[BF_W],
+                       b->f[BF_P], b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, ts);
+  BCFD_CHECK_LAUNCH();
+  BCFD_HIP(hipStreamSynchronize(s));
+  (void)hipFree(list);
+  (void)hipFree(touched);
+  (void)hipFree(deferred);
+  return 0;
+}

@@ -1,0 +1,180 @@
+/*
+ * biocfd_hip.h -- C ABI of the MI355X (gfx950) implementation of the Bio-CFD
+ * per-time-step hot path.
+ *
+ * Every entry point returns 0 on success and a negative code on failure; the
+ * message of the last failure is available from biocfd_last_error().  Blocks
+ * are identified by the reference's 1-based block number; arrays crossing the
+ * boundary use the reference's host layout (Fortran column-major, 1-based
+ * with one ghost layer: field(i,j,k), i = 1..nx+2, at
+ * f[(i-1) + (nx+2)*((j-1) + (ny+2)*(k-1))]; index lists (count,3) int64
+ * column-major; tag arrays int64).  Device copies stay resident in HBM between
+ * calls; uploads/downloads are explicit.
+ *
+ * Each compute entry point names the reference procedure it replaces
+ * (reference file:line).
+ */
+#ifndef BIOCFD_HIP_H
+#define BIOCFD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- identifiers ------------------------------------------------------- */
+enum biocfd_field { BF_U = 0, BF_V, BF_W, BF_UT, BF_VT, BF_WT, BF_P, BF_B, BF_PC, BF_PCO, BF_NFIELD };
+enum biocfd_tag { BT_CELL = 0, BT_CELL2, BT_NODEIDTAG, BT_CELL_N, BT_CELL_PR, BT_NTAG };
+/* grid arrays in Block_t order (Blocks.f90:26-27) */
+enum biocfd_grid { BG_X1 = 0, BG_Y1, BG_Z1, BG_DX, BG_DY, BG_DZ, BG_XU, BG_YU, BG_ZU, BG_XV, BG_YV,
+                   BG_ZV, BG_XW, BG_YW, BG_ZW, BG_XP, BG_YP, BG_ZP, BG_NGRID };
+enum biocfd_list { BL_FLUID = 0, BL_RED, BL_BLACK, BL_SOLID, BL_INTERCEPTED, BL_TS, BL_NLIST };
+/* per-IB-cell arrays: nearest element indices (Blocks.f90:49) */
+enum biocfd_nel { BN_P = 0, BN_U1, BN_U2, BN_V1, BN_V2, BN_W1, BN_W2, BN_NNEL };
+/* per-element arrays (Blocks.f90:50-52) */
+enum biocfd_elem { BE_XCENT = 0, BE_YCENT, BE_ZCENT, BE_COSA, BE_COSB, BE_COSG, BE_ELEN, BE_NELEM };
+/* TS ghost arrays (Blocks.f90:55-58) */
+enum biocfd_ghost { BGH_P = 0, BGH_PT, BGH_U1, BGH_U1T, BGH_U2, BGH_U2T, BGH_V1, BGH_V1T, BGH_V2,
+                    BGH_V2T, BGH_W1, BGH_W1T, BGH_W2, BGH_W2T, BGH_N };
+/* counts (Blocks.f90:42-43) */
+enum biocfd_count { BC_FLUID = 0, BC_SOLID, BC_IB, BC_RED, BC_BLACK, BC_TS, BC_N };
+/* body kinematic scalars read by the forcing routines (Blocks.f90:14-15, 72-73) */
+enum biocfd_kin { BK_XDOT = 0, BK_YDOT, BK_YDDOT, BK_THETADOT1, BK_THETADDOT1, BK_THETADOT2,
+                  BK_THETADDOT2, BK_ALPHADOT, BK_PIVX, BK_PIVY, BK_PIVZ, BK_N };
+/* 3rd-order upwind tables, Blocks.f90:29-31 order */
+enum biocfd_table { BTB_UU = 0, BTB_VV, BTB_WW, BTB_UV, BTB_UW, BTB_VU, BTB_VW, BTB_WU, BTB_WV, BTB_N };
+
+/* ---- runtime ----------------------------------------------------------- */
+const char *biocfd_last_error(void);
+/* Select the HIP device and create the runtime's stream (no reference
+ * counterpart: replaces set_gpu, gpu_helpers.F90:17-40, on one GPU). */
+int biocfd_init(int device);
+int biocfd_finalize(void);
+int biocfd_synchronize(void);
+
+/* ---- block data (Block_t, Blocks.f90:13-81; AllocateArrays.f90:15-57) ---- */
+int biocfd_block_create(int blk, int64_t nx, int64_t ny, int64_t nz, double omega, int64_t cintp,
+                        double dx, double dy, double dz);
+int biocfd_block_destroy(int blk);
+int biocfd_grid_upload(int blk, int which, const double *host, int64_t n);
+int biocfd_field_upload(int blk, int field, const double *host);
+int biocfd_field_download(int blk, int field, double *host);
+int biocfd_tag_upload(int blk, int tag, const int64_t *host);
+int biocfd_tag_download(int blk, int tag, int64_t *host);
+int biocfd_counts(int blk, int64_t out[BC_N]);
+/* materialise an index list in the reference's (count,3) int64 layout and
+ * serial i-fastest order (cellCount_solid, Search.F90:653-741) */
+int biocfd_list_download(int blk, int which, int64_t *host, int64_t count);
+int biocfd_nel_download(int blk, int which, int64_t *host, int64_t count);
+int biocfd_normdis_download(int blk, int which, double *host, int64_t count);
+int biocfd_ghost_download(int blk, int which, double *host, int64_t count);
+int biocfd_elem_download(int blk, int which, double *host, int64_t count);
+int biocfd_index_ts_download(int blk, int64_t *host, int64_t count);
+int biocfd_coef_download(int blk, int axis, double *host, int64_t n);       /* Acx/Acy/Acz (n,3) */
+int biocfd_table_download(int blk, int which, int is_ck, double *host, int64_t ncols);
+/* SOR state of the last poissonSolver call: nIterPcor, derr2, derrStdSt */
+int biocfd_sor_state(int blk, int64_t *n_iter, double *derr2, double *derr_std_st);
+
+/* ---- immersed body (readSurfaceMeshGmsh target arrays, ReadInput.f90:375-420) ---- */
+int biocfd_body_create(int blk, int64_t n_nodes, int64_t n_elems, const int64_t *el_p1,
+                       const int64_t *el_p2, const int64_t *el_p3, const int64_t *surf_id,
+                       const int64_t search_box[6]);
+
+/* ---- set-up numerics (host-side, results kept on the device) ---- */
+/* coefficientMatrix, CoefficientMatrix.f90:10-71 */
+int biocfd_coefficient_matrix(int blk, int is_coarse);
+/* non_uni_coeff, NavierStokes.f90:13-250 */
+int biocfd_non_uni_coeff(int blk);
+
+/* ---- momentum and boundary conditions ---- */
+/* nsMomentum2order, NavierStokes.f90:252-912 */
+int biocfd_ns_momentum(int blk, double dt, double re);
+/* velocityBC, BoundaryConditions.f90:11-74 */
+int biocfd_velocity_bc(int blk, double dt, double uc);
+/* solidCellBC, BoundaryConditions.f90:76-92 */
+int biocfd_solid_cell_bc(int blk);
+/* solidCellBC_move, BoundaryConditions.f90:94-115 */
+int biocfd_solid_cell_bc_move(int blk, int64_t move_check);
+/* updateVelocity_newv, PcorVcor.F90:379-397 */
+int biocfd_update_velocity_newv(int blk, int64_t move_check);
+
+/* ---- pressure correction ---- */
+/* poissonSolver, PcorVcor.F90:22-220, over blocks first..last (1-based)
+ * with the interfaces registered by biocfd_interface_create */
+int biocfd_poisson_solver(int first, int last, int64_t pc_ita_max, double epsi, double dt,
+                          double uc);
+/* the pieces of poissonSolver, for unit tests: computeDiv :222-243,
+ * REDBLACKSOR_linear :288-377, correctPressure+correctVelocity :245-286 */
+int biocfd_compute_div(int blk);
+int biocfd_redblack_sor(int blk, int64_t pc_ita_max, double dt, double epsi);
+int biocfd_correct(int blk, double dt);
+
+/* ---- immersed-boundary geometry, search and tagging (Search.F90) ---- */
+/* computeSurfaceNorm :210-258 (nodes are the moved positions xnode1..) */
+int biocfd_compute_surface_norm(int blk, const double *xnode1, const double *ynode1,
+                                const double *znode1, int64_t inor);
+/* tagging_th_core :260-386 */
+int biocfd_tagging(int blk);
+/* selectiveRetagging_th :520-651 */
+int biocfd_selective_retagging(int blk, int64_t blk_mv_tag);
+/* cellCount_solid :653-741 (counts + device lists) */
+int biocfd_cell_count_solid(int blk);
+/* cellCount_solid_coarse :1463-1532 */
+int biocfd_cell_count_solid_coarse(int blk);
+/* fine_block_cell :863-914, for the coarse block of every registered interface */
+int biocfd_fine_block_cell(int coarse_blk);
+/* computeNormDistance :743-861 */
+int biocfd_compute_norm_distance(int blk);
+/* findTScells :434-518 */
+int biocfd_find_ts_cells(int blk);
+
+/* ---- sharp-interface reconstruction (Forcing.f90) ---- */
+int biocfd_velocity_forcing1(int blk, const double kin[BK_N]);      /* :92-360 */
+int biocfd_pressure_forcing1(int blk, const double kin[BK_N]);      /* :14-90 */
+int biocfd_velocity_forcing_field(int blk, const double kin[BK_N]); /* :813-1075 */
+int biocfd_pressure_forcing_field(int blk, const double kin[BK_N]); /* :737-811 */
+int biocfd_velocity_forcing_ghost(int blk, const double kin[BK_N]); /* :450-734 */
+int biocfd_pressure_forcing_ghost(int blk, const double kin[BK_N]); /* :362-448 */
+/* stress_calculation, StressCalc.f90:14-208: out = {viscous_drag,
+ * pressure_drag, viscous_lift, pressure_lift} coefficients */
+int biocfd_stress_calculation(int blk, double mu_f, double rho_f, const double kin[BK_N],
+                              double out[4]);
+
+/* ---- coarse/fine interfaces ---- */
+/* register interface `itf` (1-based) between coarse block a and fine block b
+ * and build its index tables (interfaceDetail / set_interface_detail,
+ * InterfaceDetail.f90:10-241) */
+int biocfd_interface_create(int itf, int a_blk, int b_blk, int64_t a_msh, int64_t b_msh,
+                            const double extent[6]);
+/* which: 0 = p,u,v,w ; 1 = ut,vt,wt ; 2 = pc (+pco) */
+/* coarseUpdate / coarseUpdate_newv / coarseUpdate_pc, CoarseInterp.f90:16-437 */
+int biocfd_coarse_update(int which);
+/* fineUpdate_bd_mv (which 0) / fineUpdate_newv_bd (1) / fineUpdate_pc_bd (2),
+ * FineInterp_bound.f90:17-544 */
+int biocfd_fine_update_bd(int which);
+int biocfd_interface_table(int itf, int comp, int axis, int64_t *host, int64_t *counter, int64_t cap);
+
+/* ---- body kinematics (host arithmetic on host arrays, Search.F90:21-208) ---- */
+/* state: {xshift,yshift,zshift,a0,yamp,bfreq, xpth1,xpth2,ypth2,xchg,ychg,yt,ydot,yddot,xt,xdot,
+ *         thetaDot,thetaDDot,thetaDot1,thetaDDot1,thetaDot2,thetaDDot2,alphaDot,
+ *         piv_x,piv_y,piv_z,xmove,ymove,zmove,inity_cent,nxty_cent,initx_cent,nxtx_cent,
+ *         initz_cent,nxtz_cent} -- 35 doubles, updated in place */
+enum { BKS_N = 35 };
+/* shiftSurfaceNodesInitial :49-124 */
+int biocfd_shift_surface_nodes_initial(int64_t n_nodes, const double *xnode, const double *ynode,
+                                       const double *znode, const int64_t *node_id, double aoa,
+                                       double piv_pt, int64_t ita, double dt, double dxmin,
+                                       double totime, double *state, double *xnode1,
+                                       double *ynode1, double *znode1);
+/* computeSurfaceVariables :126-208 (without its per-step log file) */
+int biocfd_compute_surface_variables(int64_t n_nodes, const double *xnode, const double *ynode,
+                                     const double *znode, const int64_t *node_id,
+                                     double phase_angle, double piv_pt, int64_t ita, double dt,
+                                     double dxmin, double freq, double totime, double *state,
+                                     double *xnode1, double *ynode1, double *znode1);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

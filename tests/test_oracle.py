@@ -1,0 +1,169 @@
+"""CPU checks of the oracle (the C restatement under oracle/) against what
+pins it: the reference's own CPU build (golden fixtures made by
+tests/golden/make_golden.py), the reference's unit-test vectors
+(test/test_interpolation.f90:26-63, test/test_search.f90:67-83) and closed
+forms / invariants."""
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from biocfd_amd import cases as CS
+from biocfd_amd.dumps import read_dump
+from parity import ORC_RUN, ROOT, compare
+
+import sys
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+import make_golden  # noqa: E402
+
+LIBORC = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+def orc():
+    lib = C.CDLL(LIBORC)
+    lib.orc_linear_interpolation.restype = C.c_double
+    lib.orc_linear_interpolation.argtypes = [C.c_double] * 5
+    lib.orc_bilinear_interpolation.restype = C.c_double
+    lib.orc_bilinear_interpolation.argtypes = [C.c_double] * 6 + [C.POINTER(C.c_double)]
+    lib.orc_noise_u.restype = C.c_double
+    lib.orc_noise_u.argtypes = [C.c_int64] * 3
+    return lib
+
+
+def run_case(case, steps):
+    wd = tempfile.mkdtemp(prefix="orc_t_")
+    case.dump_steps = tuple(steps)
+    p = os.path.join(wd, "case.bin")
+    CS.write_case(case, p)
+    subprocess.run([ORC_RUN, p, wd], check=True, capture_output=True)
+    return wd
+
+
+@pytest.mark.parametrize("name", ["sphere", "channel", "nested", "wing"])
+def test_oracle_matches_reference_fixtures(name):
+    """Bit-identical to the reference CPU build: every dumped array equal."""
+    case = make_golden.golden_cases()[name]
+    fx = np.load(make_golden.fixture_path(name), allow_pickle=False)
+    wd = run_case(case, make_golden.STEPS)
+    bad = []
+    for s in make_golden.STEPS:
+        for g in range(len(case.blocks)):
+            got = read_dump(os.path.join(wd, f"dump_s{s}_b{g + 1}.bin"))
+            pre = f"s{s}_b{g + 1}_"
+            ref = {k[len(pre):]: fx[k] for k in fx.files if k.startswith(pre)}
+            bad += compare(ref, got, label=f"{name} {pre}", field_rtol=0.0)
+            missing = [k for k in ref if k not in got]
+            bad += [f"{name} {pre}{k}: not produced" for k in missing]
+    assert not bad, "\n".join(bad[:30])
+
+
+def test_fixtures_exercise_every_path():
+    """The fixture set covers thin-surface cells, a moving body, two blocks."""
+    w = np.load(make_golden.fixture_path("wing"))
+    assert int(w["s2_b1_TSCellCount"]) > 0
+    n = np.load(make_golden.fixture_path("nested"))
+    assert "s2_b2_u" in n.files and float(n["s2_b2_ydot"]) != 0.0
+    s = np.load(make_golden.fixture_path("sphere"))
+    assert int(s["s2_b1_ibCellCount"]) > 0 and int(s["s2_b1_nIterPcor"]) == 13
+
+
+def test_reference_interpolation_vectors():
+    """test/test_interpolation.f90:26-63 known answers."""
+    lib = orc()
+    assert lib.orc_linear_interpolation(1.5, 2.0, 1.0, 2.0, 1.0) == 1.5
+    assert lib.orc_linear_interpolation(3.0, 2.0, 1.0, 2.0, 1.0) == 3.0
+    assert lib.orc_linear_interpolation(50.0, 100.0, 1.0, 1.0, 1.0) == 1.0
+    assert lib.orc_linear_interpolation(0.5, 1.0, 0.0, 0.0, 10.0) == 5.0
+    f = (C.c_double * 4)(1.0, 1.0, 1.0, 1.0)
+    assert lib.orc_bilinear_interpolation(0.5, 0.5, 0.0, 1.0, 0.0, 1.0, f) == 1.0
+    f = (C.c_double * 4)(1.0, 2.0, 3.0, 4.0)
+    assert lib.orc_bilinear_interpolation(2.0, 0.5, 0.0, 1.0, 0.0, 1.0, f) == 4.0
+
+
+def test_all_fluid_block_counts():
+    """test/test_search.f90:67-83: an all-fluid 5^3 block has 125 fluid cells
+    and red + black = fluid (via the channel case of that size)."""
+    case = CS.config_channel(n=(5, 5, 5), nsteps=0)
+    wd = run_case(case, (0,))
+    d = read_dump(os.path.join(wd, "dump_s0_b1.bin"))
+    assert int(d["fluidCellCount"]) == 125
+    assert int(d["redCellCount"]) + int(d["blackCellCount"]) == 125
+    # red = (i+j+k) odd with 1-based indices including the ghost layer
+    red = d["redCellIndexPtr"]
+    assert np.all(red.sum(axis=1) % 2 == 1)
+
+
+def test_lists_are_i_fastest_and_partition_the_interior():
+    wd = run_case(make_golden.golden_cases()["sphere"], (0,))
+    d = read_dump(os.path.join(wd, "dump_s0_b1.bin"))
+    nx, ny, nz = (s - 2 for s in d["cell"].shape)
+    for k in ("fluidIndexPtr", "solidIndexPtr", "interceptedIndexPtr", "redCellIndexPtr", "blackCellIndexPtr"):
+        ijk = d[k]
+        lin = (ijk[:, 0] - 2) + nx * ((ijk[:, 1] - 2) + ny * (ijk[:, 2] - 2))
+        assert np.all(np.diff(lin) > 0), k
+    total = int(d["fluidCellCount"]) + int(d["solidCellCount"]) + int(d["ibCellCount"])
+    assert total == nx * ny * nz
+    cell = d["cell"][1:-1, 1:-1, 1:-1]
+    assert int(np.sum(cell == 0)) == int(d["fluidCellCount"])
+    assert int(np.sum(cell == 2)) == int(d["ibCellCount"])
+
+
+def test_coefficient_matrix_uniform_closed_form():
+    """Uniform spacing h: rows (1,-2,1)/h^2, coarse folds at the ends
+    (CoefficientMatrix.f90:49-69): Dirichlet in x, Neumann in y and z."""
+    case = CS.config_channel(n=(8, 8, 8), nsteps=0)
+    case.blocks[0].hi = (1.0, 1.0, 1.0)
+    wd = run_case(case, (0,))
+    d = read_dump(os.path.join(wd, "dump_s0_b1.bin"))
+    h2 = (1.0 / 8) ** 2
+    ax, ay = d["Acx"], d["Acy"]
+    np.testing.assert_allclose(ax[3], [1 / h2, -2 / h2, 1 / h2], rtol=1e-12)
+    np.testing.assert_allclose(ax[0], [0.0, -3 / h2, 1 / h2], rtol=1e-12)
+    np.testing.assert_allclose(ax[-1], [1 / h2, -3 / h2, 0.0], rtol=1e-12)
+    np.testing.assert_allclose(ay[0], [0.0, -1 / h2, 1 / h2], rtol=1e-12)
+    np.testing.assert_allclose(ay[-1], [1 / h2, -1 / h2, 0.0], rtol=1e-12)
+
+
+def test_divergence_free_after_projection():
+    """After the SOR has converged, the corrected velocity is discretely
+    divergence free on a body-free block (PcorVcor.F90 projection) away from
+    the two boundary layers that velocityBC rewrites after the correction."""
+    case = CS.config_channel(n=(10, 8, 8), nsteps=1, pc_ita_max=4000, epsi=1e-13)
+    wd = run_case(case, (1,))
+    d = read_dump(os.path.join(wd, "dump_s1_b1.bin"))
+    u, v, w = d["u"], d["v"], d["w"]
+    g = case.blocks[0].grids()
+    dx, dy, dz = g[0].delta, g[1].delta, g[2].delta
+    div = ((u[1:-1, 1:-1, 1:-1] - u[:-2, 1:-1, 1:-1]) / dx[1:-1, None, None]
+           + (v[1:-1, 1:-1, 1:-1] - v[1:-1, :-2, 1:-1]) / dy[None, 1:-1, None]
+           + (w[1:-1, 1:-1, 1:-1] - w[1:-1, 1:-1, :-2]) / dz[None, None, 1:-1])
+    b0 = np.max(np.abs(d["b"]))
+    assert b0 > 1e-6
+    inner = div[2:-2, 2:-2, 2:-2]
+    assert np.max(np.abs(inner)) < 1e-10 * b0
+    assert np.max(np.abs(div)) > 1e-4 * b0  # the boundary layers are not (planted-check)
+
+
+def test_noise_generator_matches_python():
+    lib = orc()
+    lin = np.array([0, 1, 17, 123456], dtype=np.int64)
+    py = CS.noise_uniform(20261203, 2, lin)
+    c = [lib.orc_noise_u(20261203, 2, int(x)) for x in lin]
+    assert np.array_equal(py, np.array(c))
+
+
+def test_nearest_element_tie_break_lowest_index():
+    """Brute-force argmin over a duplicated mesh: duplicated elements tie
+    exactly, the strict '<' keeps the first (Search.F90:311-318)."""
+    b = CS.sphere_body(64, 0.4)
+    elems = np.concatenate([b.elems, b.elems])
+    body = CS.Body(b.nodes, b.node_id, elems, np.concatenate([b.surf_id, b.surf_id]))
+    case = CS.config_sphere_64(n=12, markers=64, nsteps=0)
+    case.blocks[0].body = CS.BodyPlacement(body, (0.5, 0.5, 0.5))
+    wd = run_case(case, (0,))
+    d = read_dump(os.path.join(wd, "dump_s0_b1.bin"))
+    for k in ("nelp", "nelu1", "nelu2", "nelv1", "nelv2", "nelw1", "nelw2"):
+        assert np.all(d[k] <= 64), k
