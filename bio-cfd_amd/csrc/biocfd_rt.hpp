@@ -121,6 +121,11 @@ struct Block {
   int64_t scan_cap = 0;
   int *list_tmp = nullptr;
   int64_t list_cap = 0;
+  // colour-split pc and b used inside the SOR iterations
+  double *pcs[2] = {nullptr, nullptr}, *bs[2] = {nullptr, nullptr};
+  int64_t split_n = 0;
+  double *partials = nullptr;         // per-workgroup maxima of the reductions
+  int64_t cap_partials = 0;
 };
 
 struct Interface {

@@ -29,11 +29,6 @@ __device__ inline double wave_max(double v) {
   return v;
 }
 
-// bit pattern of a non-negative double orders like its value
-__device__ inline void atomic_max_nonneg(unsigned long long *dst, double v) {
-  atomicMax(dst, (unsigned long long)__double_as_longlong(v));
-}
-
 __global__ void k_zero_pcor(double *__restrict__ b, double *__restrict__ pc, int64_t n) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) { b[q] = 0.0; pc[q] = 0.0; }
@@ -63,52 +58,104 @@ __global__ void k_div(Dims d, const uint8_t *__restrict__ cell, const double *__
          (wt[c] - wt[c - (int64_t)d.NX * d.NY]) / dzv[k - 1];
 }
 
-// One colour of REDBLACKSOR_linear (PcorVcor.F90:313-330 / 332-349).
-// colour 1 = red ((i+j+k) odd), 0 = black.  Each thread owns one cell of the
-// colour in row (j,k).
-__global__ void k_sor_colour(Dims d, int colour, const uint8_t *__restrict__ cell,
-                             const double *__restrict__ b, double *__restrict__ pc,
-                             const double *__restrict__ Ax, const double *__restrict__ Ay,
-                             const double *__restrict__ Az, double inv_dt_unused, double dt,
-                             double omega, SorState *st) {
-  (void)inv_dt_unused;
-  if (st->done) return;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
-  const int i0 = 2 + ((colour - (2 + j + k)) & 1);
-  const int i = i0 + 2 * (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const bool check = ((st->n_iter + 1) % 5) == 0;
-  double delta = 0.0;
-  if (j <= d.ny + 1 && i <= d.nx + 1) {
-    const int64_t c = ix3(d, i, j, k);
-    if (cell[c] == 0) {
-      const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
-      const int nx = d.nx, ny = d.ny, nz = d.nz;
-      const double old = pc[c];
-      const double gs = (b[c] / dt - Ax[(i - 2)] * pc[c - 1] - Ax[(i - 2) + 2 * nx] * pc[c + 1] -
-                         Ay[(j - 2)] * pc[c - sy] - Ay[(j - 2) + 2 * ny] * pc[c + sy] -
-                         Az[(k - 2)] * pc[c - sz] - Az[(k - 2) + 2 * nz] * pc[c + sz]) /
-                        (Ax[(i - 2) + nx] + Ay[(j - 2) + ny] + Az[(k - 2) + nz]);
-      const double nw = (1.0 - omega) * old + omega * gs;
-      pc[c] = nw;
-      delta = fabs(nw - old);
-    }
-  }
-  if (check) {
-    delta = wave_max(delta);
-    if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0 && delta > 0.0)
-      atomic_max_nonneg(&st->max_bits, delta);
+// Colour-split storage of pc and b for the SOR iterations: colour 1 (red,
+// (i+j+k) odd) and colour 0 (black) each hold their cells of every row in
+// a contiguous half-row, h = (i-1)>>1.  In row (j,k) the cells of colour c
+// sit at i = 2h + 2 - ((c - j - k) & 1).  A colour sweep then reads its own
+// half-array, its b half-array and the other colour's half-array once, fully
+// coalesced.
+__device__ inline int64_t sidx(int NXh, int NY, int h, int j, int k) {
+  return (int64_t)h + (int64_t)NXh * ((int64_t)(j - 1) + (int64_t)NY * (int64_t)(k - 1));
+}
+
+__global__ void k_split(Dims d, int NXh, const double *__restrict__ full, double *__restrict__ red,
+                        double *__restrict__ black) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.NX || j > d.NY) return;
+  const double v = full[ix3(d, i, j, k)];
+  double *dst = ((i + j + k) & 1) ? red : black;
+  dst[sidx(NXh, d.NY, (i - 1) >> 1, j, k)] = v;
+}
+
+__global__ void k_merge(Dims d, int NXh, double *__restrict__ full, const double *__restrict__ red,
+                        const double *__restrict__ black) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.NX || j > d.NY) return;
+  const double *src = ((i + j + k) & 1) ? red : black;
+  full[ix3(d, i, j, k)] = src[sidx(NXh, d.NY, (i - 1) >> 1, j, k)];
+}
+
+// workgroup max of a non-negative value -> partials[workgroup]
+__device__ inline void wg_max_store(double v, double *partials) {
+  __shared__ double s_wave[16];
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nw = (blockDim.x * blockDim.y + 63) / 64;
+  v = wave_max(v);
+  if ((t & 63) == 0) s_wave[t >> 6] = v;
+  __syncthreads();
+  if (t == 0) {
+    double m = 0.0;
+    for (int q = 0; q < nw; ++q) m = m > s_wave[q] ? m : s_wave[q];
+    partials[blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] = m;
   }
 }
 
-// end-of-iteration bookkeeping of REDBLACKSOR_linear (PcorVcor.F90:309, 351-375)
-__global__ void k_sor_iter_end(SorState *st, long long pc_ita_max, double epsi) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || st->done) return;
+// One colour of REDBLACKSOR_linear (PcorVcor.F90:313-330 red / 332-349 black)
+// on the split layout; `other` holds the neighbours' colour.
+__global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, const uint8_t *__restrict__ cell,
+                                                   const double *__restrict__ bown, double *__restrict__ own,
+                                                   const double *__restrict__ other, const double *__restrict__ Ax,
+                                                   const double *__restrict__ Ay, const double *__restrict__ Az,
+                                                   double dt, double omega, const SorState *__restrict__ st,
+                                                   double *__restrict__ partials) {
+  if (st->done) return;
+  const bool check = ((st->n_iter + 1) % 5) == 0;
+  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 2 + blockIdx.z;
+  const int i = 2 * h + 2 - ((colour - j - k) & 1);
+  double delta = 0.0;
+  if (j <= d.ny + 1 && i >= 2 && i <= d.nx + 1 && cell[ix3(d, i, j, k)] == 0) {
+    const int nx = d.nx, ny = d.ny, nz = d.nz, NY = d.NY;
+    const int64_t c = sidx(NXh, NY, h, j, k);
+    const int64_t sy = NXh, sz = (int64_t)NXh * NY;
+    const int64_t cw = sidx(NXh, NY, (i - 2) >> 1, j, k), ce = sidx(NXh, NY, i >> 1, j, k);
+    const double old = own[c];
+    const double gs = (bown[c] / dt - Ax[(i - 2)] * other[cw] - Ax[(i - 2) + 2 * nx] * other[ce] -
+                       Ay[(j - 2)] * other[c - sy] - Ay[(j - 2) + 2 * ny] * other[c + sy] -
+                       Az[(k - 2)] * other[c - sz] - Az[(k - 2) + 2 * nz] * other[c + sz]) /
+                      (Ax[(i - 2) + nx] + Ay[(j - 2) + ny] + Az[(k - 2) + nz]);
+    const double nw = (1.0 - omega) * old + omega * gs;
+    own[c] = nw;
+    delta = fabs(nw - old);
+  }
+  if (check) wg_max_store(delta, partials);
+}
+
+// end-of-iteration bookkeeping of REDBLACKSOR_linear (PcorVcor.F90:309, 351-375);
+// on check iterations reduces the sweeps' workgroup maxima (one workgroup)
+__global__ void __launch_bounds__(1024) k_sor_iter_end(SorState *st, long long pc_ita_max, double epsi,
+                                                       const double *partials, int64_t n_partials) {
+  if (st->done) return;
   const long long it = st->n_iter + 1;
+  __shared__ double s_m[16];
+  if (it % 5 == 0) {
+    double m = 0.0;
+    for (int64_t q = threadIdx.x; q < n_partials; q += blockDim.x) m = m > partials[q] ? m : partials[q];
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   st->n_iter = it;
   if (it % 5 == 0) {
-    st->derr4 = __longlong_as_double((long long)st->max_bits);
-    st->max_bits = 0ull;
+    double m = 0.0;
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
+    st->derr4 = m;
   }
   st->derr2 = st->derr4;
   if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
@@ -138,7 +185,7 @@ __global__ void k_correct(Dims d, const uint8_t *__restrict__ cell, const double
 __global__ void k_norm_copy(Dims d, const uint8_t *__restrict__ cell, double *__restrict__ u,
                             double *__restrict__ v, double *__restrict__ w,
                             const double *__restrict__ ut, const double *__restrict__ vt,
-                            const double *__restrict__ wt, double dt, SorState *st) {
+                            const double *__restrict__ wt, double dt, double *__restrict__ partials) {
   const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k = 1 + blockIdx.z;
@@ -155,15 +202,21 @@ __global__ void k_norm_copy(Dims d, const uint8_t *__restrict__ cell, double *__
     }
     u[c] = a; v[c] = b; w[c] = e;
   }
-  m = wave_max(m);
-  if ((threadIdx.x + threadIdx.y * blockDim.x) % 64 == 0 && m > 0.0) atomic_max_nonneg(&st->std_bits, m);
+  wg_max_store(m, partials);
 }
 
-__global__ void k_norm_begin(SorState *st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st->std_bits = 0ull;
-}
-__global__ void k_norm_end(SorState *st) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) st->derr_std_st = __longlong_as_double((long long)st->std_bits);
+__global__ void __launch_bounds__(1024) k_norm_end(SorState *st, const double *partials, int64_t n) {
+  __shared__ double s_m[16];
+  double m = 0.0;
+  for (int64_t q = threadIdx.x; q < n; q += blockDim.x) m = m > partials[q] ? m : partials[q];
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) r = r > s_m[q] ? r : s_m[q];
+    st->derr_std_st = r;
+  }
 }
 
 // velocityBC (BoundaryConditions.f90:11-74): three face passes in order
@@ -259,28 +312,64 @@ int compute_div(Block &b) {
 
 // REDBLACKSOR_linear: iterations are enqueued in chunks; the device decides
 // when to stop (done flag), the host only polls between chunks.
+// optional HIP-event timing of the SOR sweeps (events on the runtime stream)
+struct SorTiming {
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+  int64_t iters = 0;
+  int64_t cells = 0;  // fluid cells swept per iteration, summed over calls
+};
+static SorTiming g_tm;
+
+static int ensure_split(Block &b) {
+  const int NXh = (b.d.NX + 1) / 2;
+  const int64_t n = (int64_t)NXh * b.d.NY * b.d.NZ;
+  if (b.split_n != n) {
+    for (int c = 0; c < 2; ++c)
+      if (dev_alloc(&b.pcs[c], n) || dev_alloc(&b.bs[c], n)) return -1;
+    b.split_n = n;
+  }
+  // one partial per workgroup of each colour sweep (the norm copy needs fewer)
+  const int64_t nwg = (int64_t)div_up(NXh, 64) * div_up(b.d.ny, 4) * b.d.nz;
+  if (grow(&b.partials, &b.cap_partials, 2 * nwg + (int64_t)div_up(b.d.NX, 64) * div_up(b.d.NY, 4) * b.d.NZ))
+    return -1;
+  return 0;
+}
+
 int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   hipStream_t s = rt().stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ensure_split(b)) return -1;
+  const int NXh = (b.d.NX + 1) / 2;
+  dim3 fb(64, 4, 1), fg(div_up(b.d.NX, 64), div_up(b.d.NY, 4), (unsigned)b.d.NZ);
+  hipLaunchKernelGGL(k_split, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
+  hipLaunchKernelGGL(k_split, fg, fb, 0, s, b.d, NXh, b.f[BF_B], b.bs[1], b.bs[0]);
   hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
   BCFD_CHECK_LAUNCH();
   dim3 blk(64, 4, 1);
-  const int half = (b.d.nx + 1) / 2 + 1;
-  dim3 grd(div_up(half, blk.x), div_up(b.d.ny, blk.y), (unsigned)b.d.nz);
+  dim3 grd(div_up(NXh, blk.x), div_up(b.d.ny, blk.y), (unsigned)b.d.nz);
+  const int64_t nwg = (int64_t)grd.x * grd.y * grd.z;
   SorState *h = (SorState *)rt().h_pinned;
   BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
   BCFD_HIP(hipStreamSynchronize(s));
   long long start = h->n_iter;
   long long remaining = pc_ita_max + 1 - start;
+  if (g_tm.on) {
+    BCFD_HIP(hipEventCreate(&e0));
+    BCFD_HIP(hipEventCreate(&e1));
+    BCFD_HIP(hipEventRecord(e0, s));
+  }
   if (remaining < 1) remaining = 1;
   const long long chunk = 50;
   while (remaining > 0) {
     long long n = remaining < chunk ? remaining : chunk;
     for (long long q = 0; q < n; ++q) {
-      hipLaunchKernelGGL(k_sor_colour, grd, blk, 0, s, b.d, 1, b.tag[BT_CELL], b.f[BF_B], b.f[BF_PC],
-                         b.A[0], b.A[1], b.A[2], 0.0, dt, b.omega, b.sor);
-      hipLaunchKernelGGL(k_sor_colour, grd, blk, 0, s, b.d, 0, b.tag[BT_CELL], b.f[BF_B], b.f[BF_PC],
-                         b.A[0], b.A[1], b.A[2], 0.0, dt, b.omega, b.sor);
-      hipLaunchKernelGGL(k_sor_iter_end, dim3(1), dim3(64), 0, s, b.sor, (long long)pc_ita_max, epsi);
+      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 1, b.tag[BT_CELL], b.bs[1], b.pcs[1], b.pcs[0],
+                         b.A[0], b.A[1], b.A[2], dt, b.omega, b.sor, b.partials);
+      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 0, b.tag[BT_CELL], b.bs[0], b.pcs[0], b.pcs[1],
+                         b.A[0], b.A[1], b.A[2], dt, b.omega, b.sor, b.partials + nwg);
+      hipLaunchKernelGGL(k_sor_iter_end, dim3(1), dim3(1024), 0, s, b.sor, (long long)pc_ita_max, epsi,
+                         b.partials, 2 * nwg);
     }
     BCFD_CHECK_LAUNCH();
     remaining -= n;
@@ -289,6 +378,16 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
     BCFD_HIP(hipStreamSynchronize(s));
     if (h->done) break;
   }
+  if (g_tm.on) {
+    BCFD_HIP(hipEventRecord(e1, s));
+    BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+    BCFD_HIP(hipStreamSynchronize(s));
+    g_tm.ev.emplace_back(e0, e1);
+    g_tm.iters += h->n_iter - start;
+    g_tm.cells += (h->n_iter - start) * (int64_t)b.d.nx * b.d.ny * b.d.nz;
+  }
+  hipLaunchKernelGGL(k_merge, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
+  BCFD_CHECK_LAUNCH();
   return 0;
 }
 
@@ -318,10 +417,10 @@ int norm_copy(Block &b, double dt) {
   hipStream_t s = rt().stream;
   dim3 blk(64, 4, 1);
   dim3 grd(div_up(b.d.NX, blk.x), div_up(b.d.NY, blk.y), (unsigned)b.d.NZ);
-  hipLaunchKernelGGL(k_norm_begin, dim3(1), dim3(1), 0, s, b.sor);
+  if (ensure_split(b)) return -1;
   hipLaunchKernelGGL(k_norm_copy, grd, blk, 0, s, b.d, b.tag[BT_CELL], b.f[BF_U], b.f[BF_V],
-                     b.f[BF_W], b.f[BF_UT], b.f[BF_VT], b.f[BF_WT], dt, b.sor);
-  hipLaunchKernelGGL(k_norm_end, dim3(1), dim3(1), 0, s, b.sor);
+                     b.f[BF_W], b.f[BF_UT], b.f[BF_VT], b.f[BF_WT], dt, b.partials);
+  hipLaunchKernelGGL(k_norm_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, (int64_t)grd.x * grd.y * grd.z);
   BCFD_CHECK_LAUNCH();
   return 0;
 }
@@ -369,6 +468,29 @@ int biocfd_update_velocity_newv(int id, int64_t move_check) {
   hipLaunchKernelGGL(k_copy3, dim3(div_up(b->n2, 256)), dim3(256), 0, rt().stream, b->f[BF_UT],
                      b->f[BF_VT], b->f[BF_WT], b->f[BF_U], b->f[BF_V], b->f[BF_W], b->n2);
   BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+int biocfd_timing_enable(int on) {
+  if (ensure_init()) return -1;
+  for (auto &pr : g_tm.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
+  g_tm = SorTiming();
+  g_tm.on = on != 0;
+  return 0;
+}
+
+int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_iters) {
+  if (ensure_init()) return -1;
+  BCFD_HIP(hipStreamSynchronize(rt().stream));
+  double ms = 0.0;
+  for (auto &pr : g_tm.ev) {
+    float t = 0.f;
+    BCFD_HIP(hipEventElapsedTime(&t, pr.first, pr.second));
+    ms += t;
+  }
+  *sor_ms = ms;
+  *sor_iters = g_tm.iters;
+  *sor_cell_iters = g_tm.cells;
   return 0;
 }
 

@@ -706,7 +706,12 @@ int biocfd_cell_count_solid(int id) {
 int biocfd_cell_count_solid_coarse(int id) {
   Block *b = get_block(id);
   if (!b) return -1;
-  return build_cell_lists(*b, false);
+  // only the fluid/red/black counts change (Search.F90:1468-1506)
+  const int64_t solid = b->counts[BC_SOLID], ib = b->counts[BC_IB];
+  if (build_cell_lists(*b, false)) return -1;
+  b->counts[BC_SOLID] = solid;
+  b->counts[BC_IB] = ib;
+  return 0;
 }
 
 int biocfd_fine_block_cell(int coarse) {

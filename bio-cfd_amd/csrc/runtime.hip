@@ -201,6 +201,11 @@ static void free_block(Block &b) {
   if (b.sor) (void)hipFree(b.sor);
   if (b.scan_tmp) (void)hipFree(b.scan_tmp);
   if (b.list_tmp) (void)hipFree(b.list_tmp);
+  for (int c = 0; c < 2; ++c) {
+    if (b.pcs[c]) (void)hipFree(b.pcs[c]);
+    if (b.bs[c]) (void)hipFree(b.bs[c]);
+  }
+  if (b.partials) (void)hipFree(b.partials);
   IbData &ib = b.ib;
   for (auto &p : ib.el) if (p) (void)hipFree(p);
   if (ib.surf) (void)hipFree(ib.surf);

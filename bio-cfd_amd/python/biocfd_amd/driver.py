@@ -144,6 +144,8 @@ class Simulation:
             w[fl] = w[fl] + c.noise_amp * (2.0 * CS.noise_uniform(seed, 3, lin) - 1.0)
         for name, a in (("u", u), ("ut", u), ("v", v), ("vt", v), ("w", w), ("wt", w), ("p", p)):
             self.upload(st.g, name, a)
+        zero_pr = np.zeros(tuple(s + 1 for s in shp), dtype=np.int64, order="F")
+        H.call("biocfd_tag_upload", st.g, H.BT["cell_pr"], H.ptr(zero_pr))
 
     # ---------------------------------------------------------------- time step
     def step(self):

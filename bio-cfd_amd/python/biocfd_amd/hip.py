@@ -41,6 +41,8 @@ _D = C.c_double
 SIGNATURES = {
     "biocfd_init": [_I],
     "biocfd_finalize": [],
+    "biocfd_timing_enable": [_I],
+    "biocfd_timing_read": [_P, _P, _P],
     "biocfd_synchronize": [],
     "biocfd_block_create": [_I, _L, _L, _L, _D, _L, _D, _D, _D],
     "biocfd_block_destroy": [_I],

@@ -53,6 +53,13 @@ int biocfd_init(int device);
 int biocfd_finalize(void);
 int biocfd_synchronize(void);
 
+/* Time the red-black SOR sweeps with HIP events on the runtime's stream (no
+ * reference counterpart: measurement hook).  read: total GPU milliseconds of
+ * the SOR iteration loops since enable, iterations executed, and interior
+ * cells x iterations swept. */
+int biocfd_timing_enable(int on);
+int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_iters);
+
 /* ---- block data (Block_t, Blocks.f90:13-81; AllocateArrays.f90:15-57) ---- */
 int biocfd_block_create(int blk, int64_t nx, int64_t ny, int64_t nz, double omega, int64_t cintp,
                         double dx, double dy, double dz);
