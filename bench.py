@@ -53,6 +53,9 @@ def gpu_run(args):
 
     case = make_case(args)
     cells = case.cells()
+    if args.copy_probe_mb > 0:
+        H.call("biocfd_init", 0)
+        H.call("biocfd_copy_probe", int(args.copy_probe_mb) << 20, 4)
     sim = Simulation(case)
     for _ in range(args.warmup):
         sim.step()
@@ -139,6 +142,8 @@ def main():
     ap.add_argument("--epsi", type=float, default=1e-8)
     ap.add_argument("--precision", choices=["fp64"], default="fp64")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (byte-counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-timeout", type=float, default=600.0)
     args = ap.parse_args()

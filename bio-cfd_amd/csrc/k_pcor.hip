@@ -104,34 +104,63 @@ __device__ inline void wg_max_store(double v, double *partials) {
   }
 }
 
+// b/dt of the fluid cells in the split layout; non-fluid cells carry a NaN
+// with a private payload so the sweeps need no tag array (b/dt is the same
+// division the reference performs inside every sweep, done once)
+constexpr unsigned long long kNotFluid = 0x7ff4dead0000beefull;
+
+__global__ void k_split_bdt(Dims d, int NXh, const double *__restrict__ b, const uint8_t *__restrict__ cell,
+                            double dt, double *__restrict__ red, double *__restrict__ black) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.NX || j > d.NY) return;
+  const int64_t c = ix3(d, i, j, k);
+  const bool interior = i >= 2 && i <= d.nx + 1 && j >= 2 && j <= d.ny + 1 && k >= 2 && k <= d.nz + 1;
+  const double v = (interior && cell[c] == 0) ? b[c] / dt : __longlong_as_double((long long)kNotFluid);
+  double *dst = ((i + j + k) & 1) ? red : black;
+  dst[sidx(NXh, d.NY, (i - 1) >> 1, j, k)] = v;
+}
+
+constexpr int kSorKz = 4;  // planes per thread
+
 // One colour of REDBLACKSOR_linear (PcorVcor.F90:313-330 red / 332-349 black)
-// on the split layout; `other` holds the neighbours' colour.
-__global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, const uint8_t *__restrict__ cell,
-                                                   const double *__restrict__ bown, double *__restrict__ own,
-                                                   const double *__restrict__ other, const double *__restrict__ Ax,
-                                                   const double *__restrict__ Ay, const double *__restrict__ Az,
-                                                   double dt, double omega, const SorState *__restrict__ st,
-                                                   double *__restrict__ partials) {
+// on the split layout; `other` holds the neighbours' colour.  Thread t of a
+// row owns the t-th cell of the colour (i = i0 + 2t) on kSorKz planes.
+__global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, const double *__restrict__ bdt,
+                                                   double *__restrict__ own, const double *__restrict__ other,
+                                                   const double *__restrict__ Ax, const double *__restrict__ Ay,
+                                                   const double *__restrict__ Az, double omega,
+                                                   const SorState *__restrict__ st, double *__restrict__ partials) {
   if (st->done) return;
   const bool check = ((st->n_iter + 1) % 5) == 0;
-  const int h = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
-  const int i = 2 * h + 2 - ((colour - j - k) & 1);
+  const int nx = d.nx, ny = d.ny, nz = d.nz, NY = d.NY;
+  const int64_t sy = NXh, sz = (int64_t)NXh * NY;
   double delta = 0.0;
-  if (j <= d.ny + 1 && i >= 2 && i <= d.nx + 1 && cell[ix3(d, i, j, k)] == 0) {
-    const int nx = d.nx, ny = d.ny, nz = d.nz, NY = d.NY;
-    const int64_t c = sidx(NXh, NY, h, j, k);
-    const int64_t sy = NXh, sz = (int64_t)NXh * NY;
-    const int64_t cw = sidx(NXh, NY, (i - 2) >> 1, j, k), ce = sidx(NXh, NY, i >> 1, j, k);
-    const double old = own[c];
-    const double gs = (bown[c] / dt - Ax[(i - 2)] * other[cw] - Ax[(i - 2) + 2 * nx] * other[ce] -
-                       Ay[(j - 2)] * other[c - sy] - Ay[(j - 2) + 2 * ny] * other[c + sy] -
-                       Az[(k - 2)] * other[c - sz] - Az[(k - 2) + 2 * nz] * other[c + sz]) /
-                      (Ax[(i - 2) + nx] + Ay[(j - 2) + ny] + Az[(k - 2) + nz]);
-    const double nw = (1.0 - omega) * old + omega * gs;
-    own[c] = nw;
-    delta = fabs(nw - old);
+  if (j <= ny + 1) {
+    const double axy = Ay[(j - 2) + ny];
+    const double ays = Ay[(j - 2)], ayn = Ay[(j - 2) + 2 * ny];
+    for (int kk = 0; kk < kSorKz; ++kk) {
+      const int k = 2 + blockIdx.z * kSorKz + kk;
+      if (k > nz + 1) break;
+      const int i = 2 + ((colour - j - k) & 1) + 2 * t;
+      if (i > nx + 1) continue;
+      const int h = (i - 1) >> 1;
+      const int64_t c = sidx(NXh, NY, h, j, k);
+      const double rhs = bdt[c];
+      if (__double_as_longlong(rhs) == (long long)kNotFluid) continue;
+      const int64_t cw = sidx(NXh, NY, (i - 2) >> 1, j, k), ce = sidx(NXh, NY, i >> 1, j, k);
+      const double old = own[c];
+      const double gs = (rhs - Ax[(i - 2)] * other[cw] - Ax[(i - 2) + 2 * nx] * other[ce] - ays * other[c - sy] -
+                         ayn * other[c + sy] - Az[(k - 2)] * other[c - sz] - Az[(k - 2) + 2 * nz] * other[c + sz]) /
+                        (Ax[(i - 2) + nx] + axy + Az[(k - 2) + nz]);
+      const double nw = (1.0 - omega) * old + omega * gs;
+      own[c] = nw;
+      const double dd = fabs(nw - old);
+      delta = delta > dd ? delta : dd;
+    }
   }
   if (check) wg_max_store(delta, partials);
 }
@@ -343,11 +372,11 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   const int NXh = (b.d.NX + 1) / 2;
   dim3 fb(64, 4, 1), fg(div_up(b.d.NX, 64), div_up(b.d.NY, 4), (unsigned)b.d.NZ);
   hipLaunchKernelGGL(k_split, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
-  hipLaunchKernelGGL(k_split, fg, fb, 0, s, b.d, NXh, b.f[BF_B], b.bs[1], b.bs[0]);
+  hipLaunchKernelGGL(k_split_bdt, fg, fb, 0, s, b.d, NXh, b.f[BF_B], b.tag[BT_CELL], dt, b.bs[1], b.bs[0]);
   hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
   BCFD_CHECK_LAUNCH();
   dim3 blk(64, 4, 1);
-  dim3 grd(div_up(NXh, blk.x), div_up(b.d.ny, blk.y), (unsigned)b.d.nz);
+  dim3 grd(div_up((b.d.nx + 1) / 2, blk.x), div_up(b.d.ny, blk.y), div_up(b.d.nz, kSorKz));
   const int64_t nwg = (int64_t)grd.x * grd.y * grd.z;
   SorState *h = (SorState *)rt().h_pinned;
   BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
@@ -364,10 +393,10 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   while (remaining > 0) {
     long long n = remaining < chunk ? remaining : chunk;
     for (long long q = 0; q < n; ++q) {
-      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 1, b.tag[BT_CELL], b.bs[1], b.pcs[1], b.pcs[0],
-                         b.A[0], b.A[1], b.A[2], dt, b.omega, b.sor, b.partials);
-      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 0, b.tag[BT_CELL], b.bs[0], b.pcs[0], b.pcs[1],
-                         b.A[0], b.A[1], b.A[2], dt, b.omega, b.sor, b.partials + nwg);
+      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 1, b.bs[1], b.pcs[1], b.pcs[0], b.A[0], b.A[1],
+                         b.A[2], b.omega, b.sor, b.partials);
+      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 0, b.bs[0], b.pcs[0], b.pcs[1], b.A[0], b.A[1],
+                         b.A[2], b.omega, b.sor, b.partials + nwg);
       hipLaunchKernelGGL(k_sor_iter_end, dim3(1), dim3(1024), 0, s, b.sor, (long long)pc_ita_max, epsi,
                          b.partials, 2 * nwg);
     }

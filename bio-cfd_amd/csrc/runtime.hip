@@ -182,6 +182,21 @@ int biocfd_init(int device) {
   return 0;
 }
 
+int biocfd_copy_probe(int64_t bytes, int reps) {
+  if (ensure_init()) return -1;
+  if (bytes <= 0 || reps <= 0) { set_error("copy_probe: bad size"); return -1; }
+  void *a = nullptr, *b = nullptr;
+  BCFD_HIP(hipMalloc(&a, (size_t)bytes));
+  BCFD_HIP(hipMalloc(&b, (size_t)bytes));
+  BCFD_HIP(hipMemsetAsync(a, 0, (size_t)bytes, g_rt.stream));
+  for (int r = 0; r < reps; ++r)
+    BCFD_HIP(hipMemcpyAsync(b, a, (size_t)bytes, hipMemcpyDeviceToDevice, g_rt.stream));
+  BCFD_HIP(hipStreamSynchronize(g_rt.stream));
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return 0;
+}
+
 int biocfd_synchronize(void) {
   if (ensure_init()) return -1;
   BCFD_HIP(hipStreamSynchronize(g_rt.stream));

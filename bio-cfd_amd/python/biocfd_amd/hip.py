@@ -43,6 +43,7 @@ SIGNATURES = {
     "biocfd_finalize": [],
     "biocfd_timing_enable": [_I],
     "biocfd_timing_read": [_P, _P, _P],
+    "biocfd_copy_probe": [_L, _I],
     "biocfd_synchronize": [],
     "biocfd_block_create": [_I, _L, _L, _L, _D, _L, _D, _D, _D],
     "biocfd_block_destroy": [_I],

@@ -59,6 +59,9 @@ int biocfd_synchronize(void);
  * cells x iterations swept. */
 int biocfd_timing_enable(int on);
 int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_iters);
+/* Device-to-device copy of `bytes` bytes, `reps` times (no reference
+ * counterpart: calibrates HBM byte counters against a copy of known size). */
+int biocfd_copy_probe(int64_t bytes, int reps);
 
 /* ---- block data (Block_t, Blocks.f90:13-81; AllocateArrays.f90:15-57) ---- */
 int biocfd_block_create(int blk, int64_t nx, int64_t ny, int64_t nz, double omega, int64_t cintp,
