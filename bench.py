@@ -57,6 +57,19 @@ def gpu_run(args):
         H.call("biocfd_init", 0)
         H.call("biocfd_copy_probe", int(args.copy_probe_mb) << 20, 4)
     sim = Simulation(case)
+    tuned = None
+    if args.sor_variant == "auto":
+        from biocfd_amd import tune
+
+        v, planes, table = tune.autotune(block=1, dt=case.dt)
+        tuned = {"variant": v, "planes": planes, "candidates": table}
+        print(json.dumps({"sor_autotune": table}), file=sys.stderr)
+    else:
+        H.call("biocfd_sor_config", int(args.sor_variant), args.sor_planes)
+        v, planes = int(args.sor_variant), args.sor_planes
+    from biocfd_amd.tune import variants
+
+    shape = [t for t in variants() if t[0] == v][0]
     for _ in range(args.warmup):
         sim.step()
     H.call("biocfd_synchronize")
@@ -91,6 +104,8 @@ def gpu_run(args):
         "sor_us_per_iteration": 1e3 * float(sor_ms[0]) / max(1, int(sor_it[0])),
         "sor_achieved_gbs": achieved,
         "sor_bytes_per_iteration": SOR_BYTES_PER_CELL_ITER * cells,
+        "sor_kernel": {"variant": v, "tile": [shape[1], shape[2]], "threads": shape[3], "planes": planes,
+                       "autotuned": tuned is not None},
     }
 
 
@@ -141,6 +156,9 @@ def main():
     ap.add_argument("--pc-ita-max", type=int, default=500)
     ap.add_argument("--epsi", type=float, default=1e-8)
     ap.add_argument("--precision", choices=["fp64"], default="fp64")
+    ap.add_argument("--sor-variant", default="auto",
+                    help="SOR kernel variant (0 = colour-split sweeps, >=1 fused tiles) or 'auto'")
+    ap.add_argument("--sor-planes", type=int, default=32, help="planes per workgroup of a fused variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (byte-counter calibration)")
@@ -168,7 +186,7 @@ def main():
         "data": "synthetic (seeded 1e-3 noise on u=1 inflow; Fibonacci-hull sphere)",
         "config": {"workload": f"sphere{args.n}: single block {args.n}^3 on [0,8]x[0,4]x[0,4], sphere d=1, "
                                f"{args.markers} markers, Re=300, dt=5e-4, omega=1.8",
-                   "sor_iterations_per_step": g["sor_iters_per_step"]},
+                   "sor_iterations_per_step": g["sor_iters_per_step"], "sor_kernel": g["sor_kernel"]},
         "roofline": {
             "bound": "hbm",
             "achieved": round(g["sor_achieved_gbs"], 1),
@@ -176,7 +194,10 @@ def main():
             "unit": "GB/s",
             "frac": round(g["sor_achieved_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": "red-black SOR iteration (2 colour sweeps), 24 B/cell/iteration algorithmic",
+            "kernel": ("red-black SOR iteration (fused red+black launch + norm reduction launch)"
+                       if g["sor_kernel"]["variant"] > 0 else
+                       "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
+                      + ", 24 B/cell/iteration algorithmic",
             "us_per_iteration": round(g["sor_us_per_iteration"], 2),
         },
         "cpu_baseline": None,

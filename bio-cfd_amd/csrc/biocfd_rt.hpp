@@ -66,6 +66,7 @@ struct SorState {
   int pad;
   double derr_std_st;            // steady-state norm
   unsigned long long std_bits;
+  long long call_iters;          // iterations executed by the current SOR call
 };
 
 struct IbData {
@@ -124,6 +125,8 @@ struct Block {
   // colour-split pc and b used inside the SOR iterations
   double *pcs[2] = {nullptr, nullptr}, *bs[2] = {nullptr, nullptr};
   int64_t split_n = 0;
+  double *pc2 = nullptr;              // second pc array of the fused SOR iterations
+  int64_t pc2_n = 0;
   double *partials = nullptr;         // per-workgroup maxima of the reductions
   int64_t cap_partials = 0;
 };

@@ -15,6 +15,7 @@
 // colour by colour is the same computation; |pc - pco| of an iteration is
 // the per-cell change, reduced with one atomic per wavefront.
 #include "biocfd_rt.hpp"
+#include "sor_fused.hpp"
 
 namespace bcfd {
 
@@ -40,6 +41,7 @@ __global__ void k_sor_reset_call(SorState *st, int reset_iter) {
     st->derr4 = 11111.0;
     st->max_bits = 0ull;
     st->done = 0;
+    st->call_iters = 0;
   }
 }
 
@@ -64,9 +66,6 @@ __global__ void k_div(Dims d, const uint8_t *__restrict__ cell, const double *__
 // sit at i = 2h + 2 - ((c - j - k) & 1).  A colour sweep then reads its own
 // half-array, its b half-array and the other colour's half-array once, fully
 // coalesced.
-__device__ inline int64_t sidx(int NXh, int NY, int h, int j, int k) {
-  return (int64_t)h + (int64_t)NXh * ((int64_t)(j - 1) + (int64_t)NY * (int64_t)(k - 1));
-}
 
 __global__ void k_split(Dims d, int NXh, const double *__restrict__ full, double *__restrict__ red,
                         double *__restrict__ black) {
@@ -107,7 +106,6 @@ __device__ inline void wg_max_store(double v, double *partials) {
 // b/dt of the fluid cells in the split layout; non-fluid cells carry a NaN
 // with a private payload so the sweeps need no tag array (b/dt is the same
 // division the reference performs inside every sweep, done once)
-constexpr unsigned long long kNotFluid = 0x7ff4dead0000beefull;
 
 __global__ void k_split_bdt(Dims d, int NXh, const double *__restrict__ b, const uint8_t *__restrict__ cell,
                             double dt, double *__restrict__ red, double *__restrict__ black) {
@@ -165,6 +163,67 @@ __global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, 
   if (check) wg_max_store(delta, partials);
 }
 
+// One red-black iteration per launch on a 1-D grid of gx*gy*gz tiles (the
+// tile schedule is described with SorTile).  Non-fluid cells of the tile are
+// copied through, so src and dst agree on every cell the iteration leaves.
+template <int TI, int TJ, int NT>
+__global__ void __launch_bounds__(NT) k_sor_fused(FusedArgs a, int gx, int gy, const SorState *__restrict__ st,
+                                                  double *__restrict__ partials) {
+  using T = SorTile<TI, TJ, NT>;
+  if (st->done) return;
+  const bool check = ((st->n_iter + 1) % 5) == 0;
+  __shared__ double lds[T::LDS_DOUBLES];
+  const int t = T::xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
+  const int iL = TI * tx, jL = TJ * ty, k0 = 2 + a.KS * tz;
+  const int k1 = min(k0 + a.KS - 1, a.d.nz + 1);
+  const int tid = threadIdx.x;
+  double pcn[T::NL], rn[T::NR], rc[T::NR], bn[T::NB], bc[T::NB];
+  T::load_coef(a, lds, tid, iL, jL);
+  Coef3 czm = T::cz_of(a, k0 - 2), cz0 = T::cz_of(a, k0 - 1), czp;
+  T::load_plane(a, tid, k0 - 2, iL, jL, pcn);
+  T::store_plane(lds, tid, k0 - 2, pcn);
+  T::load_plane(a, tid, k0 - 1, iL, jL, pcn);
+  T::store_plane(lds, tid, k0 - 1, pcn);
+  T::load_plane(a, tid, k0, iL, jL, pcn);
+  T::load_bdt_red(a, tid, k0 - 1, iL, jL, rn);
+#pragma unroll
+  for (int q = 0; q < T::NB; ++q) bn[q] = 0.0;
+  double delta = 0.0;
+  for (int s = k0 - 1; s <= k1 + 1; ++s) {
+    __syncthreads();
+    T::store_plane(lds, tid, s + 1, pcn);
+#pragma unroll
+    for (int q = 0; q < T::NR; ++q) rc[q] = rn[q];
+#pragma unroll
+    for (int q = 0; q < T::NB; ++q) bc[q] = bn[q];
+    if (s + 2 <= k1 + 2) T::load_plane(a, tid, s + 2, iL, jL, pcn);
+    if (s + 1 <= k1 + 1) T::load_bdt_red(a, tid, s + 1, iL, jL, rn);
+    if (s >= k0 && s <= k1) T::load_bdt_black(a, tid, s, iL, jL, bn);
+    czp = T::cz_of(a, s + 1);
+    __syncthreads();
+    const double dr = T::red_phase(a, lds, tid, s, cz0, rc);
+    delta = delta > dr ? delta : dr;
+    __syncthreads();
+    if (s - 1 >= k0) {
+      const double db = T::black_phase(a, lds, tid, s - 1, iL, jL, czm, bc);
+      delta = delta > db ? delta : db;
+    }
+    czm = cz0;
+    cz0 = czp;
+  }
+  if (check) wg_max_store(delta, partials);
+}
+
+// the iterations of one SOR call alternate pc -> pc2 -> pc ...; after an odd
+// count the result is in pc2
+__global__ void k_sor_finish(const SorState *__restrict__ st, double *__restrict__ pc,
+                             const double *__restrict__ pc2, int64_t n) {
+  if ((st->call_iters & 1) == 0) return;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    pc[q] = pc2[q];
+}
+
 // end-of-iteration bookkeeping of REDBLACKSOR_linear (PcorVcor.F90:309, 351-375);
 // on check iterations reduces the sweeps' workgroup maxima (one workgroup)
 __global__ void __launch_bounds__(1024) k_sor_iter_end(SorState *st, long long pc_ita_max, double epsi,
@@ -181,6 +240,7 @@ __global__ void __launch_bounds__(1024) k_sor_iter_end(SorState *st, long long p
   }
   if (threadIdx.x != 0) return;
   st->n_iter = it;
+  st->call_iters += 1;
   if (it % 5 == 0) {
     double m = 0.0;
     for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
@@ -365,19 +425,136 @@ static int ensure_split(Block &b) {
   return 0;
 }
 
-int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
+// SOR kernel variants: 0 = one launch per colour on the colour-split arrays,
+// 1.. = one fused launch per iteration with the tile shapes below
+struct FusedVariant {
+  int ti, tj, nt;
+  void (*launch)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, const SorState *, double *);
+};
+
+template <int TI, int TJ, int NT>
+static void launch_fused(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, const SorState *st,
+                         double *partials) {
+  hipLaunchKernelGGL((k_sor_fused<TI, TJ, NT>), g, blk, 0, s, a, gx, gy, st, partials);
+}
+
+static const FusedVariant kFused[] = {
+    {0, 0, 0, nullptr},
+    {64, 16, 256, launch_fused<64, 16, 256>},
+    {64, 8, 256, launch_fused<64, 8, 256>},
+    {128, 8, 256, launch_fused<128, 8, 256>},
+    {128, 16, 512, launch_fused<128, 16, 512>},
+    {32, 16, 256, launch_fused<32, 16, 256>},
+    {64, 32, 512, launch_fused<64, 32, 512>},
+    {128, 4, 256, launch_fused<128, 4, 256>},
+};
+constexpr int kNumSorVariants = (int)(sizeof(kFused) / sizeof(kFused[0]));
+static int g_sor_variant = 1;
+static int g_sor_planes = 32;
+
+static int enqueue_split_iterations(Block &b, SorState *st, long long n, int64_t pc_ita_max, double epsi) {
   hipStream_t s = rt().stream;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (ensure_split(b)) return -1;
   const int NXh = (b.d.NX + 1) / 2;
-  dim3 fb(64, 4, 1), fg(div_up(b.d.NX, 64), div_up(b.d.NY, 4), (unsigned)b.d.NZ);
-  hipLaunchKernelGGL(k_split, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
-  hipLaunchKernelGGL(k_split_bdt, fg, fb, 0, s, b.d, NXh, b.f[BF_B], b.tag[BT_CELL], dt, b.bs[1], b.bs[0]);
-  hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
-  BCFD_CHECK_LAUNCH();
   dim3 blk(64, 4, 1);
   dim3 grd(div_up((b.d.nx + 1) / 2, blk.x), div_up(b.d.ny, blk.y), div_up(b.d.nz, kSorKz));
   const int64_t nwg = (int64_t)grd.x * grd.y * grd.z;
+  for (long long q = 0; q < n; ++q) {
+    hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 1, b.bs[1], b.pcs[1], b.pcs[0], b.A[0], b.A[1],
+                       b.A[2], b.omega, st, b.partials);
+    hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 0, b.bs[0], b.pcs[0], b.pcs[1], b.A[0], b.A[1],
+                       b.A[2], b.omega, st, b.partials + nwg);
+    hipLaunchKernelGGL(k_sor_iter_end, dim3(1), dim3(1024), 0, s, st, (long long)pc_ita_max, epsi, b.partials,
+                       2 * nwg);
+  }
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+static void fused_grid(const Block &b, const FusedVariant &v, int planes, int &gx, int &gy, int &gz) {
+  gx = (int)div_up(b.d.nx, v.ti);
+  gy = (int)div_up(b.d.ny, v.tj);
+  gz = (int)div_up(b.d.nz, planes);
+}
+
+// launches `first` .. first+n-1 of this call (even: pc -> pc2, odd: pc2 -> pc)
+static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int planes, long long first, long long n,
+                                    int64_t pc_ita_max, double epsi) {
+  hipStream_t s = rt().stream;
+  const FusedVariant &v = kFused[variant];
+  int gx, gy, gz;
+  fused_grid(b, v, planes, gx, gy, gz);
+  const int64_t nwg = (int64_t)gx * gy * gz;
+  FusedArgs a;
+  a.d = b.d;
+  a.NXh = (b.d.NX + 1) / 2;
+  a.KS = planes;
+  a.bred = b.bs[1];
+  a.bblack = b.bs[0];
+  a.Ax = b.A[0];
+  a.Ay = b.A[1];
+  a.Az = b.A[2];
+  a.omega = b.omega;
+  for (long long q = first; q < first + n; ++q) {
+    a.src = (q & 1) ? b.pc2 : b.f[BF_PC];
+    a.dst = (q & 1) ? b.f[BF_PC] : b.pc2;
+    v.launch(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy, st, b.partials);
+    hipLaunchKernelGGL(k_sor_iter_end, dim3(1), dim3(1024), 0, s, st, (long long)pc_ita_max, epsi, b.partials,
+                       nwg);
+  }
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+// buffers and the per-call set-up of the SOR iterations: b/dt with the
+// not-fluid marker, and pc copied to its partner array (fused) or split
+static int sor_prepare(Block &b, int variant, int planes, double dt) {
+  hipStream_t s = rt().stream;
+  if (ensure_split(b)) return -1;
+  const int NXh = (b.d.NX + 1) / 2;
+  dim3 fb(64, 4, 1), fg(div_up(b.d.NX, 64), div_up(b.d.NY, 4), (unsigned)b.d.NZ);
+  if (variant > 0) {
+    if (b.pc2_n != b.n2) {
+      if (dev_alloc(&b.pc2, b.n2)) return -1;
+      b.pc2_n = b.n2;
+    }
+    int gx, gy, gz;
+    fused_grid(b, kFused[variant], planes, gx, gy, gz);
+    if (grow(&b.partials, &b.cap_partials, (int64_t)gx * gy * gz)) return -1;
+    BCFD_HIP(hipMemcpyAsync(b.pc2, b.f[BF_PC], sizeof(double) * (size_t)b.n2, hipMemcpyDeviceToDevice, s));
+  } else {
+    hipLaunchKernelGGL(k_split, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
+  }
+  hipLaunchKernelGGL(k_split_bdt, fg, fb, 0, s, b.d, NXh, b.f[BF_B], b.tag[BT_CELL], dt, b.bs[1], b.bs[0]);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+// the iterate back into pc after the call's iterations
+static int sor_finish(Block &b, const SorState *st, int variant) {
+  hipStream_t s = rt().stream;
+  const int NXh = (b.d.NX + 1) / 2;
+  dim3 fb(64, 4, 1), fg(div_up(b.d.NX, 64), div_up(b.d.NY, 4), (unsigned)b.d.NZ);
+  if (variant > 0)
+    hipLaunchKernelGGL(k_sor_finish, dim3(1024), dim3(256), 0, s, st, b.f[BF_PC], b.pc2, b.n2);
+  else
+    hipLaunchKernelGGL(k_merge, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+static int enqueue_iterations(Block &b, SorState *st, int variant, int planes, long long first, long long n,
+                              int64_t pc_ita_max, double epsi) {
+  return variant > 0 ? enqueue_fused_iterations(b, st, variant, planes, first, n, pc_ita_max, epsi)
+                     : enqueue_split_iterations(b, st, n, pc_ita_max, epsi);
+}
+
+int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
+  hipStream_t s = rt().stream;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const int variant = g_sor_variant, planes = g_sor_planes;
+  if (sor_prepare(b, variant, planes, dt)) return -1;
+  hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
+  BCFD_CHECK_LAUNCH();
   SorState *h = (SorState *)rt().h_pinned;
   BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
   BCFD_HIP(hipStreamSynchronize(s));
@@ -390,17 +567,11 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   }
   if (remaining < 1) remaining = 1;
   const long long chunk = 50;
+  long long launched = 0;
   while (remaining > 0) {
     long long n = remaining < chunk ? remaining : chunk;
-    for (long long q = 0; q < n; ++q) {
-      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 1, b.bs[1], b.pcs[1], b.pcs[0], b.A[0], b.A[1],
-                         b.A[2], b.omega, b.sor, b.partials);
-      hipLaunchKernelGGL(k_sor_split, grd, blk, 0, s, b.d, NXh, 0, b.bs[0], b.pcs[0], b.pcs[1], b.A[0], b.A[1],
-                         b.A[2], b.omega, b.sor, b.partials + nwg);
-      hipLaunchKernelGGL(k_sor_iter_end, dim3(1), dim3(1024), 0, s, b.sor, (long long)pc_ita_max, epsi,
-                         b.partials, 2 * nwg);
-    }
-    BCFD_CHECK_LAUNCH();
+    if (enqueue_iterations(b, b.sor, variant, planes, launched, n, pc_ita_max, epsi)) return -1;
+    launched += n;
     remaining -= n;
     if (remaining <= 0) break;
     BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
@@ -415,9 +586,41 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
     g_tm.iters += h->n_iter - start;
     g_tm.cells += (h->n_iter - start) * (int64_t)b.d.nx * b.d.ny * b.d.nz;
   }
-  hipLaunchKernelGGL(k_merge, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
-  BCFD_CHECK_LAUNCH();
-  return 0;
+  return sor_finish(b, b.sor, variant);
+}
+
+// `iters` SOR iterations of the given variant on the block's current pc and
+// b, timed with HIP events; pc and the block's SOR state are left unchanged
+static SorState *g_probe_state = nullptr;
+
+int sor_time(Block &b, int variant, int planes, int iters, double dt, double *ms) {
+  hipStream_t s = rt().stream;
+  if (!g_probe_state) {
+    if (dev_alloc(&g_probe_state, 1)) return -1;
+  }
+  double *backup = nullptr;
+  if (dev_alloc(&backup, b.n2)) return -1;
+  BCFD_HIP(hipMemcpyAsync(backup, b.f[BF_PC], sizeof(double) * (size_t)b.n2, hipMemcpyDeviceToDevice, s));
+  int rc = sor_prepare(b, variant, planes, dt);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc) {
+    hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, g_probe_state, 1);
+    rc = hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+         hipEventRecord(e0, s) != hipSuccess;
+  }
+  // pcItaMax out of reach and a negative tolerance: exactly `iters` iterations
+  if (!rc) rc = enqueue_iterations(b, g_probe_state, variant, planes, 0, iters, (int64_t)1 << 62, -1.0);
+  float t = 0.f;
+  if (!rc) rc = hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&t, e0, e1) != hipSuccess;
+  if (rc) set_error("sor_time: variant %d planes %d failed", variant, planes);
+  (void)hipMemcpyAsync(b.f[BF_PC], backup, sizeof(double) * (size_t)b.n2, hipMemcpyDeviceToDevice, s);
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(backup);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  *ms = t;
+  return rc ? -1 : 0;
 }
 
 int correct(Block &b, double dt) {
@@ -521,6 +724,42 @@ int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_ite
   *sor_iters = g_tm.iters;
   *sor_cell_iters = g_tm.cells;
   return 0;
+}
+
+int biocfd_sor_config(int variant, int planes_per_workgroup) {
+  if (variant < 0 || variant >= kNumSorVariants) {
+    set_error("sor_config: variant %d outside 0..%d", variant, kNumSorVariants - 1);
+    return -1;
+  }
+  if (planes_per_workgroup < 1 || planes_per_workgroup > 4096) {
+    set_error("sor_config: planes per workgroup %d outside 1..4096", planes_per_workgroup);
+    return -1;
+  }
+  g_sor_variant = variant;
+  g_sor_planes = planes_per_workgroup;
+  return 0;
+}
+
+int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads) {
+  if (variant < 0 || variant >= kNumSorVariants) {
+    set_error("sor_variant_shape: variant %d outside 0..%d", variant, kNumSorVariants - 1);
+    return -1;
+  }
+  *tile_i = kFused[variant].ti;
+  *tile_j = kFused[variant].tj;
+  *threads = kFused[variant].nt;
+  return 0;
+}
+
+int biocfd_sor_time(int id, int variant, int planes_per_workgroup, int iters, double dt, double *ms) {
+  Block *b = get_block(id);
+  if (!b || check_ready(*b)) return -1;
+  if (variant < 0 || variant >= kNumSorVariants || planes_per_workgroup < 1 || planes_per_workgroup > 4096 ||
+      iters < 1) {
+    set_error("sor_time: bad variant %d / planes %d / iterations %d", variant, planes_per_workgroup, iters);
+    return -1;
+  }
+  return sor_time(*b, variant, planes_per_workgroup, iters, dt, ms);
 }
 
 int biocfd_compute_div(int id) {

@@ -220,6 +220,7 @@ static void free_block(Block &b) {
     if (b.pcs[c]) (void)hipFree(b.pcs[c]);
     if (b.bs[c]) (void)hipFree(b.bs[c]);
   }
+  if (b.pc2) (void)hipFree(b.pc2);
   if (b.partials) (void)hipFree(b.partials);
   IbData &ib = b.ib;
   for (auto &p : ib.el) if (p) (void)hipFree(p);

@@ -119,6 +119,18 @@ int biocfd_poisson_solver(int first, int last, int64_t pc_ita_max, double epsi, 
 int biocfd_compute_div(int blk);
 int biocfd_redblack_sor(int blk, int64_t pc_ita_max, double dt, double epsi);
 int biocfd_correct(int blk, double dt);
+/* Kernel variant of the SOR iterations (no reference counterpart; the
+ * reference has one loop nest).  variant 0: one launch per colour on
+ * colour-split arrays; variant >= 1: one fused red+black launch per iteration
+ * over tiles of tile_i x tile_j cells and `planes_per_workgroup` z-planes.
+ * Every variant gives the same bits; the choice only changes speed. */
+int biocfd_sor_config(int variant, int planes_per_workgroup);
+/* tile shape of a variant (0,0,0 for variant 0); fails past the last one */
+int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads);
+/* Autotuning probe (no reference counterpart): time `iters` SOR iterations
+ * of a variant on block `blk`'s current pc and b (milliseconds, HIP events);
+ * pc and the block's SOR state are restored afterwards. */
+int biocfd_sor_time(int blk, int variant, int planes_per_workgroup, int iters, double dt, double *ms);
 
 /* ---- immersed-boundary geometry, search and tagging (Search.F90) ---- */
 /* computeSurfaceNorm :210-258 (nodes are the moved positions xnode1..) */

@@ -1,0 +1,220 @@
+// Host run of the fused SOR tile schedule of bio-cfd_amd/csrc/sor_fused.hpp.
+//
+// The kernel's phases (plane store + prefetch, red update, black update) are
+// executed thread by thread, workgroup by workgroup, with the same per-thread
+// bodies the gfx950 kernel runs and the same barrier placement, on grids whose
+// sides are not multiples of the tiles.  Built with AddressSanitizer on the
+// host, so any index outside the arrays or the LDS image aborts.  The result
+// of every iteration must equal, bit for bit, the textbook in-place red-black
+// sweep (all red cells, then all black cells) on the same arrays.
+//
+//   sor_tile_host            exit 0 when every case matches
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "sor_fused.hpp"
+
+using namespace bcfd;
+
+struct Problem {
+  Dims d;
+  int NXh;
+  std::vector<double> pc, bred, bblack, Ax, Ay, Az;
+  std::vector<unsigned char> fluid;
+  double omega;
+};
+
+static unsigned long long g_rng = 88172645463325252ull;
+static double urand() {
+  g_rng ^= g_rng << 13;
+  g_rng ^= g_rng >> 7;
+  g_rng ^= g_rng << 17;
+  return (double)(g_rng >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static Problem make_problem(int nx, int ny, int nz, double solid_frac) {
+  Problem p;
+  p.d = Dims{nx, ny, nz, nx + 2, ny + 2, nz + 2, nx + 3, ny + 3};
+  p.NXh = (p.d.NX + 1) / 2;
+  const int64_t n = (int64_t)p.d.NX * p.d.NY * p.d.NZ;
+  const int64_t nh = (int64_t)p.NXh * p.d.NY * p.d.NZ;
+  p.pc.resize(n);
+  p.fluid.assign(n, 0);
+  p.bred.assign(nh, not_fluid_marker());
+  p.bblack.assign(nh, not_fluid_marker());
+  for (auto &v : p.pc) v = urand() - 0.5;
+  auto axis = [](std::vector<double> &A, int m) {
+    A.resize(3 * (size_t)m);
+    for (int q = 0; q < m; ++q) {
+      const double h = 0.8 + 0.4 * urand();
+      A[q] = 1.0 / (h * h);
+      A[q + 2 * m] = 1.0 / (h * h) * (0.9 + 0.2 * urand());
+      A[q + m] = -(A[q] + A[q + 2 * m]);
+    }
+  };
+  axis(p.Ax, nx);
+  axis(p.Ay, ny);
+  axis(p.Az, nz);
+  for (int k = 2; k <= nz + 1; ++k)
+    for (int j = 2; j <= ny + 1; ++j)
+      for (int i = 2; i <= nx + 1; ++i) {
+        if (urand() < solid_frac) continue;
+        p.fluid[ix3(p.d, i, j, k)] = 1;
+        const double v = (urand() - 0.5) * 4.0;
+        auto &dst = ((i + j + k) & 1) ? p.bred : p.bblack;
+        dst[sidx(p.NXh, p.d.NY, (i - 1) >> 1, j, k)] = v;
+      }
+  p.omega = 1.7;
+  return p;
+}
+
+static FusedArgs args_of(const Problem &p, int KS) {
+  FusedArgs a;
+  a.d = p.d;
+  a.NXh = p.NXh;
+  a.KS = KS;
+  a.src = nullptr;
+  a.dst = nullptr;
+  a.bred = p.bred.data();
+  a.bblack = p.bblack.data();
+  a.Ax = p.Ax.data();
+  a.Ay = p.Ay.data();
+  a.Az = p.Az.data();
+  a.omega = p.omega;
+  return a;
+}
+
+// textbook in-place red-black iteration; returns max |change|
+static double inplace_iteration(const Problem &p, std::vector<double> &pc) {
+  const FusedArgs a = args_of(p, 1);
+  const Dims &d = p.d;
+  const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
+  double delta = 0.0;
+  for (int colour = 1; colour >= 0; --colour)
+    for (int k = 2; k <= d.nz + 1; ++k)
+      for (int j = 2; j <= d.ny + 1; ++j)
+        for (int i = 2; i <= d.nx + 1; ++i) {
+          if (((i + j + k) & 1) != colour) continue;
+          const auto &bt = colour ? p.bred : p.bblack;
+          const double rhs = bt[sidx(p.NXh, d.NY, (i - 1) >> 1, j, k)];
+          if (sor_skip(rhs)) continue;
+          const int64_t c = ix3(d, i, j, k);
+          const double old = pc[c];
+          const double nw = sor_cell(a.omega, coef_at(a.Ax, d.nx, i), coef_at(a.Ay, d.ny, j), coef_at(a.Az, d.nz, k),
+                                     rhs, old, pc[c - 1], pc[c + 1], pc[c - sy], pc[c + sy], pc[c - sz], pc[c + sz]);
+          pc[c] = nw;
+          const double dd = std::fabs(nw - old);
+          delta = delta > dd ? delta : dd;
+        }
+  return delta;
+}
+
+// one fused iteration src -> dst, emulating every workgroup and thread
+template <int TI, int TJ, int NT>
+static double fused_iteration(const Problem &p, int KS, const std::vector<double> &src, std::vector<double> &dst) {
+  using T = SorTile<TI, TJ, NT>;
+  FusedArgs a = args_of(p, KS);
+  a.src = src.data();
+  a.dst = dst.data();
+  const int gx = (p.d.nx + TI - 1) / TI, gy = (p.d.ny + TJ - 1) / TJ, gz = (p.d.nz + KS - 1) / KS;
+  const int n = gx * gy * gz;
+  std::vector<double> lds(T::LDS_DOUBLES);
+  std::vector<double> pcn((size_t)NT * T::NL), rn((size_t)NT * T::NR), rc((size_t)NT * T::NR),
+      bn((size_t)NT * T::NB), bc((size_t)NT * T::NB);
+  std::vector<int> seen(n, 0);
+  double delta = 0.0;
+  for (int L = 0; L < n; ++L) {
+    const int t = T::xcd_tile(L, n);
+    if (t < 0 || t >= n || seen[t]++) {
+      std::printf("xcd_tile is not a bijection at %d of %d\n", L, n);
+      std::exit(2);
+    }
+    const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
+    const int iL = TI * tx, jL = TJ * ty, k0 = 2 + KS * tz;
+    const int k1 = std::min(k0 + KS - 1, p.d.nz + 1);
+    for (auto &v : lds) v = std::nan("");
+    for (int tid = 0; tid < NT; ++tid) T::load_coef(a, lds.data(), tid, iL, jL);
+    Coef3 czm = T::cz_of(a, k0 - 2), cz0 = T::cz_of(a, k0 - 1), czp;
+    for (int tid = 0; tid < NT; ++tid) {
+      double *r = &pcn[(size_t)tid * T::NL];
+      T::load_plane(a, tid, k0 - 2, iL, jL, r);
+      T::store_plane(lds.data(), tid, k0 - 2, r);
+      T::load_plane(a, tid, k0 - 1, iL, jL, r);
+      T::store_plane(lds.data(), tid, k0 - 1, r);
+      T::load_plane(a, tid, k0, iL, jL, r);
+      T::load_bdt_red(a, tid, k0 - 1, iL, jL, &rn[(size_t)tid * T::NR]);
+      for (int q = 0; q < T::NB; ++q) bn[(size_t)tid * T::NB + q] = 0.0;
+    }
+    for (int s = k0 - 1; s <= k1 + 1; ++s) {
+      for (int tid = 0; tid < NT; ++tid) {  // barrier before
+        double *r = &pcn[(size_t)tid * T::NL];
+        T::store_plane(lds.data(), tid, s + 1, r);
+        for (int q = 0; q < T::NR; ++q) rc[(size_t)tid * T::NR + q] = rn[(size_t)tid * T::NR + q];
+        for (int q = 0; q < T::NB; ++q) bc[(size_t)tid * T::NB + q] = bn[(size_t)tid * T::NB + q];
+        if (s + 2 <= k1 + 2) T::load_plane(a, tid, s + 2, iL, jL, r);
+        if (s + 1 <= k1 + 1) T::load_bdt_red(a, tid, s + 1, iL, jL, &rn[(size_t)tid * T::NR]);
+        if (s >= k0 && s <= k1) T::load_bdt_black(a, tid, s, iL, jL, &bn[(size_t)tid * T::NB]);
+      }
+      czp = T::cz_of(a, s + 1);
+      for (int tid = 0; tid < NT; ++tid) {  // barrier before
+        const double dr = T::red_phase(a, lds.data(), tid, s, cz0, &rc[(size_t)tid * T::NR]);
+        delta = delta > dr ? delta : dr;
+      }
+      if (s - 1 >= k0)
+        for (int tid = 0; tid < NT; ++tid) {  // barrier before
+          const double db = T::black_phase(a, lds.data(), tid, s - 1, iL, jL, czm, &bc[(size_t)tid * T::NB]);
+          delta = delta > db ? delta : db;
+        }
+      czm = cz0;
+      cz0 = czp;
+    }
+  }
+  return delta;
+}
+
+template <int TI, int TJ, int NT>
+static int run_case(int nx, int ny, int nz, int KS, int iters) {
+  Problem p = make_problem(nx, ny, nz, 0.1);
+  std::vector<double> ref = p.pc, A = p.pc, B = p.pc;
+  for (int it = 0; it < iters; ++it) {
+    const double dref = inplace_iteration(p, ref);
+    const double dfus = (it & 1) ? fused_iteration<TI, TJ, NT>(p, KS, B, A) : fused_iteration<TI, TJ, NT>(p, KS, A, B);
+    const std::vector<double> &cur = (it & 1) ? A : B;
+    if (std::memcmp(cur.data(), ref.data(), sizeof(double) * ref.size()) != 0 || dref != dfus) {
+      int64_t bad = -1;
+      for (size_t q = 0; q < ref.size(); ++q)
+        if (std::memcmp(&cur[q], &ref[q], sizeof(double))) { bad = (int64_t)q; break; }
+      std::printf("MISMATCH tile %dx%d/%d grid %dx%dx%d KS=%d iteration %d: first cell %lld, delta %.17g vs %.17g\n",
+                  TI, TJ, NT, nx, ny, nz, KS, it + 1, (long long)bad, dref, dfus);
+      return 1;
+    }
+  }
+  std::printf("ok tile %dx%d/%d grid %dx%dx%d KS=%d iterations=%d\n", TI, TJ, NT, nx, ny, nz, KS, iters);
+  return 0;
+}
+
+template <int TI, int TJ, int NT>
+static int run_shape() {
+  int bad = 0;
+  bad += run_case<TI, TJ, NT>(37, 21, 13, 4, 3);
+  bad += run_case<TI, TJ, NT>(TI, TJ, 9, 32, 2);
+  bad += run_case<TI, TJ, NT>(TI + 6, 2 * TJ + 3, 11, 3, 4);
+  bad += run_case<TI, TJ, NT>(5, 3, 7, 1, 3);
+  return bad;
+}
+
+int main() {
+  int bad = 0;
+  bad += run_shape<64, 16, 256>();
+  bad += run_shape<64, 8, 256>();
+  bad += run_shape<128, 8, 256>();
+  bad += run_shape<128, 16, 512>();
+  bad += run_shape<32, 16, 256>();
+  bad += run_shape<64, 32, 512>();
+  bad += run_shape<128, 4, 256>();
+  std::printf(bad ? "FAILED %d\n" : "all cases bit-identical\n", bad);
+  return bad ? 1 : 0;
+}
