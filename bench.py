@@ -104,8 +104,8 @@ def gpu_run(args):
         "sor_us_per_iteration": 1e3 * float(sor_ms[0]) / max(1, int(sor_it[0])),
         "sor_achieved_gbs": achieved,
         "sor_bytes_per_iteration": SOR_BYTES_PER_CELL_ITER * cells,
-        "sor_kernel": {"variant": v, "tile": [shape[1], shape[2]], "threads": shape[3], "planes": planes,
-                       "autotuned": tuned is not None},
+        "sor_kernel": {"variant": v, "tile": [shape[1], shape[2]], "threads": shape[3], "depth": shape[4],
+                       "planes": planes, "autotuned": tuned is not None},
     }
 
 
@@ -194,7 +194,7 @@ def main():
             "unit": "GB/s",
             "frac": round(g["sor_achieved_gbs"] / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": ("red-black SOR iteration (fused red+black launch + norm reduction launch)"
+            "kernel": ("red-black SOR iteration (one fused red+black launch, norm reduced in-kernel)"
                        if g["sor_kernel"]["variant"] > 0 else
                        "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
                       + ", 24 B/cell/iteration algorithmic",

@@ -163,56 +163,74 @@ __global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, 
   if (check) wg_max_store(delta, partials);
 }
 
-// One red-black iteration per launch on a 1-D grid of gx*gy*gz tiles (the
-// tile schedule is described with SorTile).  Non-fluid cells of the tile are
+// One red-black iteration per launch on a 1-D grid of gx*gy*gz tiles; the
+// tile schedule is described with SorTile.  Non-fluid cells of the tile are
 // copied through, so src and dst agree on every cell the iteration leaves.
-template <int TI, int TJ, int NT>
-__global__ void __launch_bounds__(NT) k_sor_fused(FusedArgs a, int gx, int gy, const SorState *__restrict__ st,
-                                                  double *__restrict__ partials) {
-  using T = SorTile<TI, TJ, NT>;
+// On check iterations (every fifth, numbered by the host) each workgroup
+// posts its largest update to partials.
+template <int TI, int TJ, int NT, int D>
+__global__ void __launch_bounds__(NT) k_sor_fused(FusedArgs a, int gx, int gy, int check,
+                                                  const SorState *__restrict__ st, double *__restrict__ partials) {
+  using T = SorTile<TI, TJ, NT, D>;
   if (st->done) return;
-  const bool check = ((st->n_iter + 1) % 5) == 0;
   __shared__ double lds[T::LDS_DOUBLES];
-  const int t = T::xcd_tile(blockIdx.x, gridDim.x);
-  const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
-  const int iL = TI * tx, jL = TJ * ty, k0 = 2 + a.KS * tz;
-  const int k1 = min(k0 + a.KS - 1, a.d.nz + 1);
   const int tid = threadIdx.x;
-  double pcn[T::NL], rn[T::NR], rc[T::NR], bn[T::NB], bc[T::NB];
-  T::load_coef(a, lds, tid, iL, jL);
-  Coef3 czm = T::cz_of(a, k0 - 2), cz0 = T::cz_of(a, k0 - 1), czp;
-  T::load_plane(a, tid, k0 - 2, iL, jL, pcn);
-  T::store_plane(lds, tid, k0 - 2, pcn);
-  T::load_plane(a, tid, k0 - 1, iL, jL, pcn);
-  T::store_plane(lds, tid, k0 - 1, pcn);
-  T::load_plane(a, tid, k0, iL, jL, pcn);
-  T::load_bdt_red(a, tid, k0 - 1, iL, jL, rn);
-#pragma unroll
-  for (int q = 0; q < T::NB; ++q) bn[q] = 0.0;
+  const typename T::Tile tile = T::tile_of(a, T::xcd_tile(blockIdx.x, gridDim.x), gx, gy);
+  typename T::Thread th;
+  T::init_thread(a, tile, tid, th);
+  T::load_coef(a, tile, lds, tid);
+  T::prologue(a, tile, lds, th);
   double delta = 0.0;
-  for (int s = k0 - 1; s <= k1 + 1; ++s) {
+  for (int s = tile.k0 - 1; s <= tile.k1 + 1; s += D) {
     __syncthreads();
-    T::store_plane(lds, tid, s + 1, pcn);
-#pragma unroll
-    for (int q = 0; q < T::NR; ++q) rc[q] = rn[q];
-#pragma unroll
-    for (int q = 0; q < T::NB; ++q) bc[q] = bn[q];
-    if (s + 2 <= k1 + 2) T::load_plane(a, tid, s + 2, iL, jL, pcn);
-    if (s + 1 <= k1 + 1) T::load_bdt_red(a, tid, s + 1, iL, jL, rn);
-    if (s >= k0 && s <= k1) T::load_bdt_black(a, tid, s, iL, jL, bn);
-    czp = T::cz_of(a, s + 1);
+    T::phase_a(a, tile, lds, th, s, th.pcn[0]);
     __syncthreads();
-    const double dr = T::red_phase(a, lds, tid, s, cz0, rc);
+    const double dr = T::phase_b(a, lds, th, s);
     delta = delta > dr ? delta : dr;
     __syncthreads();
-    if (s - 1 >= k0) {
-      const double db = T::black_phase(a, lds, tid, s - 1, iL, jL, czm, bc);
-      delta = delta > db ? delta : db;
+    const double db = T::phase_c(a, tile, lds, th, s);
+    delta = delta > db ? delta : db;
+    if (D > 1 && s + 1 <= tile.k1 + 1) {
+      __syncthreads();
+      T::phase_a(a, tile, lds, th, s + 1, th.pcn[D - 1]);
+      __syncthreads();
+      const double dr2 = T::phase_b(a, lds, th, s + 1);
+      delta = delta > dr2 ? delta : dr2;
+      __syncthreads();
+      const double db2 = T::phase_c(a, tile, lds, th, s + 1);
+      delta = delta > db2 ? delta : db2;
     }
-    czm = cz0;
-    cz0 = czp;
   }
   if (check) wg_max_store(delta, partials);
+}
+
+// Bookkeeping of fused iteration `it` (PcorVcor.F90:309, 351-375), launched
+// after check iterations and after the last iteration the call may run: the
+// norm only changes on check iterations and the pcItaMax exit only fires on
+// the last one, so the state after any other iteration needs no update until
+// the next of these.
+__global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long it, long long start, int check,
+                                                        long long pc_ita_max, double epsi, const double *partials,
+                                                        int64_t n_partials) {
+  if (st->done) return;
+  __shared__ double s_m[16];
+  if (check) {
+    double m = 0.0;
+    for (int64_t q = threadIdx.x; q < n_partials; q += blockDim.x) m = m > partials[q] ? m : partials[q];
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (check) {
+    double m = 0.0;
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
+    st->derr4 = m;
+  }
+  st->n_iter = it;
+  st->call_iters = it - start;
+  st->derr2 = st->derr4;
+  if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
 }
 
 // the iterations of one SOR call alternate pc -> pc2 -> pc ...; after an odd
@@ -425,31 +443,34 @@ static int ensure_split(Block &b) {
   return 0;
 }
 
-// SOR kernel variants: 0 = one launch per colour on the colour-split arrays,
-// 1.. = one fused launch per iteration with the tile shapes below
+// SOR kernel variants: 0 = one launch per colour on the colour-split arrays
+// (plus one for the norm), 1.. = one fused launch per iteration with the tile
+// shapes and prefetch depths below
 struct FusedVariant {
-  int ti, tj, nt;
-  void (*launch)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, const SorState *, double *);
+  int ti, tj, nt, depth;
+  void (*launch)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, int, const SorState *, double *);
 };
 
-template <int TI, int TJ, int NT>
-static void launch_fused(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, const SorState *st,
-                         double *partials) {
-  hipLaunchKernelGGL((k_sor_fused<TI, TJ, NT>), g, blk, 0, s, a, gx, gy, st, partials);
+template <int TI, int TJ, int NT, int D>
+static void launch_fused(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, int check,
+                         const SorState *st, double *partials) {
+  hipLaunchKernelGGL((k_sor_fused<TI, TJ, NT, D>), g, blk, 0, s, a, gx, gy, check, st, partials);
 }
 
 static const FusedVariant kFused[] = {
-    {0, 0, 0, nullptr},
-    {64, 16, 256, launch_fused<64, 16, 256>},
-    {64, 8, 256, launch_fused<64, 8, 256>},
-    {128, 8, 256, launch_fused<128, 8, 256>},
-    {128, 16, 512, launch_fused<128, 16, 512>},
-    {32, 16, 256, launch_fused<32, 16, 256>},
-    {64, 32, 512, launch_fused<64, 32, 512>},
-    {128, 4, 256, launch_fused<128, 4, 256>},
+    {0, 0, 0, 0, nullptr},
+    {64, 16, 256, 1, launch_fused<64, 16, 256, 1>},
+    {64, 8, 256, 1, launch_fused<64, 8, 256, 1>},
+    {64, 8, 256, 2, launch_fused<64, 8, 256, 2>},
+    {128, 16, 512, 1, launch_fused<128, 16, 512, 1>},
+    {128, 16, 512, 2, launch_fused<128, 16, 512, 2>},
+    {64, 32, 512, 1, launch_fused<64, 32, 512, 1>},
+    {64, 32, 512, 2, launch_fused<64, 32, 512, 2>},
+    {32, 16, 256, 2, launch_fused<32, 16, 256, 2>},
+    {128, 8, 256, 2, launch_fused<128, 8, 256, 2>},
 };
 constexpr int kNumSorVariants = (int)(sizeof(kFused) / sizeof(kFused[0]));
-static int g_sor_variant = 1;
+static int g_sor_variant = 6;
 static int g_sor_planes = 32;
 
 static int enqueue_split_iterations(Block &b, SorState *st, long long n, int64_t pc_ita_max, double epsi) {
@@ -476,9 +497,11 @@ static void fused_grid(const Block &b, const FusedVariant &v, int planes, int &g
   gz = (int)div_up(b.d.nz, planes);
 }
 
-// launches `first` .. first+n-1 of this call (even: pc -> pc2, odd: pc2 -> pc)
-static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int planes, long long first, long long n,
-                                    int64_t pc_ita_max, double epsi) {
+// launches `first` .. first+n-1 of this call (even: pc -> pc2, odd: pc2 -> pc);
+// launch q runs iteration start + q + 1, launch total-1 is the last the call may run
+static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int planes, long long start,
+                                    long long total, long long first, long long n, int64_t pc_ita_max,
+                                    double epsi) {
   hipStream_t s = rt().stream;
   const FusedVariant &v = kFused[variant];
   int gx, gy, gz;
@@ -497,9 +520,12 @@ static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int pla
   for (long long q = first; q < first + n; ++q) {
     a.src = (q & 1) ? b.pc2 : b.f[BF_PC];
     a.dst = (q & 1) ? b.f[BF_PC] : b.pc2;
-    v.launch(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy, st, b.partials);
-    hipLaunchKernelGGL(k_sor_iter_end, dim3(1), dim3(1024), 0, s, st, (long long)pc_ita_max, epsi, b.partials,
-                       nwg);
+    const long long it = start + q + 1;
+    const int check = it % 5 == 0;
+    v.launch(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy, check, st, b.partials);
+    if (check || q == total - 1)
+      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, it, start, check, (long long)pc_ita_max,
+                         epsi, b.partials, nwg);
   }
   BCFD_CHECK_LAUNCH();
   return 0;
@@ -542,9 +568,9 @@ static int sor_finish(Block &b, const SorState *st, int variant) {
   return 0;
 }
 
-static int enqueue_iterations(Block &b, SorState *st, int variant, int planes, long long first, long long n,
-                              int64_t pc_ita_max, double epsi) {
-  return variant > 0 ? enqueue_fused_iterations(b, st, variant, planes, first, n, pc_ita_max, epsi)
+static int enqueue_iterations(Block &b, SorState *st, int variant, int planes, long long start, long long total,
+                              long long first, long long n, int64_t pc_ita_max, double epsi) {
+  return variant > 0 ? enqueue_fused_iterations(b, st, variant, planes, start, total, first, n, pc_ita_max, epsi)
                      : enqueue_split_iterations(b, st, n, pc_ita_max, epsi);
 }
 
@@ -566,11 +592,12 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
     BCFD_HIP(hipEventRecord(e0, s));
   }
   if (remaining < 1) remaining = 1;
+  const long long total = remaining;
   const long long chunk = 50;
   long long launched = 0;
   while (remaining > 0) {
     long long n = remaining < chunk ? remaining : chunk;
-    if (enqueue_iterations(b, b.sor, variant, planes, launched, n, pc_ita_max, epsi)) return -1;
+    if (enqueue_iterations(b, b.sor, variant, planes, start, total, launched, n, pc_ita_max, epsi)) return -1;
     launched += n;
     remaining -= n;
     if (remaining <= 0) break;
@@ -609,7 +636,7 @@ int sor_time(Block &b, int variant, int planes, int iters, double dt, double *ms
          hipEventRecord(e0, s) != hipSuccess;
   }
   // pcItaMax out of reach and a negative tolerance: exactly `iters` iterations
-  if (!rc) rc = enqueue_iterations(b, g_probe_state, variant, planes, 0, iters, (int64_t)1 << 62, -1.0);
+  if (!rc) rc = enqueue_iterations(b, g_probe_state, variant, planes, 0, iters, 0, iters, (int64_t)1 << 62, -1.0);
   float t = 0.f;
   if (!rc) rc = hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&t, e0, e1) != hipSuccess;
@@ -740,7 +767,7 @@ int biocfd_sor_config(int variant, int planes_per_workgroup) {
   return 0;
 }
 
-int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads) {
+int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth) {
   if (variant < 0 || variant >= kNumSorVariants) {
     set_error("sor_variant_shape: variant %d outside 0..%d", variant, kNumSorVariants - 1);
     return -1;
@@ -748,6 +775,7 @@ int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads
   *tile_i = kFused[variant].ti;
   *tile_j = kFused[variant].tj;
   *threads = kFused[variant].nt;
+  *depth = kFused[variant].depth;
   return 0;
 }
 

@@ -3,13 +3,14 @@
 // interior cells over KS planes and marches it in k with the pc planes
 // staged in LDS.
 //
-// Step s of the march (planes in a ring of four LDS slots):
-//   store plane s+1 (prefetched into registers during step s-1),
-//   issue the loads of plane s+2, of the b/dt values and of the Acz row that
-//   step s+1 needs (Acx and Acy of the tile sit in LDS for the whole march),
-//   red update of plane s on the tile plus a one-cell halo (reads old black),
-//   black update of plane s-1 on the tile (reads the new red of planes
-//   s-2..s) and write plane s-1 of the tile to the destination array.
+// Step s of the march (planes in a ring of four LDS slots), phases separated
+// by workgroup barriers:
+//   A: store plane s+1 (loaded into registers D steps earlier), issue the
+//      loads of plane s+1+D, of the b/dt values step s+1 needs and of the
+//      Acz row of plane s+1 (Acx, Acy of the tile sit in LDS for the march);
+//   B: red update of plane s on the tile plus a one-cell halo (old black);
+//   C: black update of plane s-1 on the tile (new red of planes s-2..s) and
+//      write-back of plane s-1 of the tile, both colours, to the destination.
 // The halo red values are recomputed by every tile that needs them from the
 // same old black values, so they equal the owner's.  Source and destination
 // arrays alternate between iterations (the old black values of a tile's halo
@@ -20,9 +21,13 @@
 // origin minus two.  TI and TJ are even, so the colour of slot cell (x, y) in
 // plane k is (x + y + k) & 1 (1 = red, as (i + j + k) odd).
 //
-// The functions below are the per-thread bodies of the kernel's phases; a
-// host build of this header runs the same bodies thread by thread to check
-// the schedule (bounds and bit-identity with the in-place sweep).
+// Index work per step is 32-bit: every thread keeps the slot positions of its
+// cells (computed once) and offsets within one plane; the plane's base
+// pointer is uniform.
+//
+// The phase functions are the per-thread bodies of the kernel; a host build
+// of this header runs the same bodies thread by thread, phase by phase, to
+// check the schedule (bounds and bit-identity with the in-place sweep).
 #pragma once
 
 #include "biocfd_rt.hpp"
@@ -74,9 +79,11 @@ __host__ __device__ inline Coef3 coef_at(const double *A, int n, int q) {
   return Coef3{A[q - 2], A[(q - 2) + n], A[(q - 2) + 2 * n]};
 }
 
-template <int TI, int TJ, int NT>
+template <int TI, int TJ, int NT, int D>
 struct SorTile {
   static_assert(TI % 2 == 0 && TJ % 2 == 0, "tile sides must be even");
+  static_assert(TI <= 250 && TJ <= 250, "slot positions are packed in 8 bits");
+  static_assert(D == 1 || D == 2, "prefetch depth is 1 or 2 planes");
   static constexpr int PW = TI + 4;                 // slot row width in cells
   static constexpr int SH = TJ + 4;                 // slot rows
   static constexpr int SW = PW / 2;                 // half-row per colour
@@ -92,20 +99,93 @@ struct SorTile {
   static constexpr int CXW = TI + 2, CYW = TJ + 2;  // staged coefficients per axis column
   static constexpr int LDS_DOUBLES = SLOTS + 3 * CXW + 3 * CYW;
 
+  // where a workgroup's tile sits
+  struct Tile {
+    int iL, jL, k0, k1;
+  };
+
+  // per-thread registers of the march
+  struct Thread {
+    double pcn[D][NL];        // planes in flight
+    double rn[NR], rc[NR];    // b/dt of the red cells: next step, this step
+    double bn[NB], bc[NB];    // b/dt of the black cells: next step, this step
+    int ld_off[NL];           // in-plane offset of the loaded slot cell, -1 outside the array
+    int ld_lds[NL];           // 2 * (y * SW + (x >> 1)) + ((x + y) & 1), -1 past the slot
+    int red_yr[NR];           // y << 8 | r: row y, r-th red cell of the row; -1 past the end
+    int pair_yh[NB];          // y << 8 | h of a red/black pair; -1 past the end
+    Coef3 czm, cz0, czp;      // Acz of planes s-1, s, s+1
+  };
+
   __host__ __device__ static double *slot(double *lds, int k, int colour) {
     return lds + ((k & 3) * 2 + colour) * (SH * SW);
   }
 
+  // tile coordinates of linear workgroup L of n: consecutive tiles go to one
+  // XCD (workgroups are dispatched to the 8 XCDs round-robin), so the halo
+  // rows two neighbouring tiles share are fetched into the same L2
+  __host__ __device__ static int xcd_tile(int L, int n) {
+    const int per = n / 8, rem = n % 8, xcd = L % 8, q = L / 8;
+    return xcd < rem ? xcd * (per + 1) + q : rem * (per + 1) + (xcd - rem) * per + q;
+  }
+
+  __host__ __device__ static Tile tile_of(const FusedArgs &a, int t, int gx, int gy) {
+    const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
+    Tile T;
+    T.iL = TI * tx;
+    T.jL = TJ * ty;
+    T.k0 = 2 + a.KS * tz;
+    T.k1 = T.k0 + a.KS - 1 < a.d.nz + 1 ? T.k0 + a.KS - 1 : a.d.nz + 1;
+    return T;
+  }
+
+  __host__ __device__ static Coef3 cz_of(const FusedArgs &a, int k) {
+    if (k < 2 || k > a.d.nz + 1) return Coef3{0.0, 0.0, 0.0};
+    return coef_at(a.Az, a.d.nz, k);
+  }
+
+  __host__ __device__ static void init_thread(const FusedArgs &a, const Tile &T, int tid, Thread &th) {
+    for (int q = 0; q < NL; ++q) {
+      const int e = tid + q * NT;
+      th.ld_off[q] = -1;
+      th.ld_lds[q] = -1;
+      if (e < PLANE) {
+        const int y = e / PW, x = e - y * PW;
+        const int i = T.iL + x, j = T.jL + y;
+        th.ld_lds[q] = 2 * (y * SW + (x >> 1)) + ((x + y) & 1);
+        if (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY) th.ld_off[q] = (i - 1) + a.d.NX * (j - 1);
+      }
+    }
+    for (int q = 0; q < NR; ++q) {
+      const int e = tid + q * NT;
+      th.red_yr[q] = -1;
+      if (e < NRC) {
+        const int y = 1 + e / RW;
+        th.red_yr[q] = (y << 8) | (e - (y - 1) * RW);
+      }
+    }
+    for (int q = 0; q < NB; ++q) {
+      const int e = tid + q * NT;
+      th.pair_yh[q] = -1;
+      if (e < NBC) {
+        const int y = 2 + e / HB;
+        th.pair_yh[q] = (y << 8) | (1 + (e - (y - 2) * HB));
+      }
+    }
+    th.czm = cz_of(a, T.k0 - 2);
+    th.cz0 = cz_of(a, T.k0 - 1);
+    th.czp = th.cz0;
+  }
+
   // Acx over slot columns x = 1..TI+2 and Acy over slot rows y = 1..TJ+2
   // (zero outside the interior, where no cell is updated)
-  __host__ __device__ static void load_coef(const FusedArgs &a, double *lds, int tid, int iL, int jL) {
+  __host__ __device__ static void load_coef(const FusedArgs &a, const Tile &T, double *lds, int tid) {
     double *cx = lds + SLOTS, *cy = cx + 3 * CXW;
     for (int e = tid; e < 3 * CXW + 3 * CYW; e += NT) {
       if (e < 3 * CXW) {
-        const int m = e / CXW, i = iL + 1 + (e - m * CXW);
+        const int m = e / CXW, i = T.iL + 1 + (e - m * CXW);
         cx[e] = (i >= 2 && i <= a.d.nx + 1) ? a.Ax[(i - 2) + m * a.d.nx] : 0.0;
       } else {
-        const int f = e - 3 * CXW, m = f / CYW, j = jL + 1 + (f - m * CYW);
+        const int f = e - 3 * CXW, m = f / CYW, j = T.jL + 1 + (f - m * CYW);
         cy[f] = (j >= 2 && j <= a.d.ny + 1) ? a.Ay[(j - 2) + m * a.d.ny] : 0.0;
       }
     }
@@ -118,98 +198,83 @@ struct SorTile {
     const double *cy = lds + SLOTS + 3 * CXW;
     return Coef3{cy[y - 1], cy[CYW + y - 1], cy[2 * CYW + y - 1]};
   }
-  // Acz of plane k (zero outside the interior)
-  __host__ __device__ static Coef3 cz_of(const FusedArgs &a, int k) {
-    if (k < 2 || k > a.d.nz + 1) return Coef3{0.0, 0.0, 0.0};
-    return coef_at(a.Az, a.d.nz, k);
-  }
 
   // plane p of src over the slot footprint (zero outside the array)
-  __host__ __device__ static void load_plane(const FusedArgs &a, int tid, int p, int iL, int jL, double *r) {
+  __host__ __device__ static void load_plane(const FusedArgs &a, const Thread &th, int p, double *r) {
     const bool pin = p >= 1 && p <= a.d.NZ;
+    const double *P = a.src + (int64_t)(pin ? p - 1 : 0) * ((int64_t)a.d.NX * a.d.NY);
+    for (int q = 0; q < NL; ++q) r[q] = (pin && th.ld_off[q] >= 0) ? P[th.ld_off[q]] : 0.0;
+  }
+
+  __host__ __device__ static void store_plane(double *lds, const Thread &th, int p, const double *r) {
+    double *S = slot(lds, p, 0);
     for (int q = 0; q < NL; ++q) {
-      const int e = tid + q * NT;
-      double v = 0.0;
-      if (pin && e < PLANE) {
-        const int y = e / PW, x = e - y * PW;
-        const int i = iL + x, j = jL + y;
-        if (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY) v = a.src[ix3(a.d, i, j, p)];
-      }
-      r[q] = v;
+      const int l = th.ld_lds[q];
+      if (l >= 0) S[((l & 1) ^ (p & 1)) * (SH * SW) + (l >> 1)] = r[q];
     }
   }
 
-  __host__ __device__ static void store_plane(double *lds, int tid, int p, const double *r) {
-    for (int q = 0; q < NL; ++q) {
-      const int e = tid + q * NT;
-      if (e < PLANE) {
-        const int y = e / PW, x = e - y * PW;
-        slot(lds, p, (x + y + p) & 1)[y * SW + (x >> 1)] = r[q];
-      }
-    }
-  }
-
-  // red cell e of plane s in the tile + halo: slot row y, half-column h, x = 2h + off
-  __host__ __device__ static void red_cell(int e, int s, int &y, int &h, int &x) {
-    y = 1 + e / RW;
-    const int r = e - (y - 1) * RW;
+  // red cell of packed position yr in plane s: slot row y, half-column h, x = 2h + off
+  __host__ __device__ static void red_cell(int yr, int s, int &y, int &h, int &x) {
+    y = yr >> 8;
     const int off = (1 - y - s) & 1;
-    h = 1 - off + r;
+    h = 1 - off + (yr & 255);
     x = 2 * h + off;
   }
 
-  __host__ __device__ static void load_bdt_red(const FusedArgs &a, int tid, int s, int iL, int jL, double *r) {
+  __host__ __device__ static void load_bdt_red(const FusedArgs &a, const Tile &T, const Thread &th, int s,
+                                               double *r) {
+    const bool kin = s >= 2 && s <= a.d.nz + 1;
+    const double *P = a.bred + (int64_t)(kin ? s - 1 : 0) * ((int64_t)a.NXh * a.d.NY);
     for (int q = 0; q < NR; ++q) {
-      const int e = tid + q * NT;
       double v = not_fluid_marker();
-      if (e < NRC) {
+      if (kin && th.red_yr[q] >= 0) {
         int y, h, x;
-        red_cell(e, s, y, h, x);
-        const int i = iL + x, j = jL + y;
-        if (i >= 2 && i <= a.d.nx + 1 && j >= 2 && j <= a.d.ny + 1 && s >= 2 && s <= a.d.nz + 1)
-          v = a.bred[sidx(a.NXh, a.d.NY, (i - 1) >> 1, j, s)];
+        red_cell(th.red_yr[q], s, y, h, x);
+        const int i = T.iL + x, j = T.jL + y;
+        if (i >= 2 && i <= a.d.nx + 1 && j >= 2 && j <= a.d.ny + 1) v = P[((i - 1) >> 1) + a.NXh * (j - 1)];
       }
       r[q] = v;
     }
   }
 
-  // pair e of plane k in the tile: slot row y, half-column h; the black cell
-  // is at x = 2h + off, the red one at x = 2h + 1 - off
-  __host__ __device__ static void pair_cell(int e, int k, int &y, int &h, int &off) {
-    y = 2 + e / HB;
-    h = 1 + (e - (y - 2) * HB);
+  // pair of packed position yh in plane k: the black cell is at x = 2h + off,
+  // the red one at x = 2h + 1 - off
+  __host__ __device__ static void pair_cell(int yh, int k, int &y, int &h, int &off) {
+    y = yh >> 8;
+    h = yh & 255;
     off = (0 - y - k) & 1;
   }
 
-  __host__ __device__ static void load_bdt_black(const FusedArgs &a, int tid, int k, int iL, int jL, double *r) {
+  __host__ __device__ static void load_bdt_black(const FusedArgs &a, const Tile &T, const Thread &th, int k,
+                                                 double *r) {
+    const double *P = a.bblack + (int64_t)(k - 1) * ((int64_t)a.NXh * a.d.NY);
     for (int q = 0; q < NB; ++q) {
-      const int e = tid + q * NT;
       double v = not_fluid_marker();
-      if (e < NBC) {
+      if (th.pair_yh[q] >= 0) {
         int y, h, off;
-        pair_cell(e, k, y, h, off);
-        const int i = iL + 2 * h + off, j = jL + y;
-        if (i <= a.d.nx + 1 && j <= a.d.ny + 1) v = a.bblack[sidx(a.NXh, a.d.NY, (i - 1) >> 1, j, k)];
+        pair_cell(th.pair_yh[q], k, y, h, off);
+        const int i = T.iL + 2 * h + off, j = T.jL + y;
+        if (i <= a.d.nx + 1 && j <= a.d.ny + 1) v = P[((i - 1) >> 1) + a.NXh * (j - 1)];
       }
       r[q] = v;
     }
   }
 
   // red update of plane s (tile + halo); returns the largest |change|
-  __host__ __device__ static double red_phase(const FusedArgs &a, double *lds, int tid, int s, const Coef3 &cz,
-                                              const double *rb) {
+  __host__ __device__ static double red_phase(const FusedArgs &a, double *lds, const Thread &th, int s) {
     double *R = slot(lds, s, 1);
     const double *B = slot(lds, s, 0), *Bm = slot(lds, s - 1, 0), *Bp = slot(lds, s + 1, 0);
     double delta = 0.0;
     for (int q = 0; q < NR; ++q) {
-      const int e = tid + q * NT;
-      if (e >= NRC || sor_skip(rb[q])) continue;
+      if (th.red_yr[q] < 0 || sor_skip(th.rc[q])) continue;
       int y, h, x;
-      red_cell(e, s, y, h, x);
+      red_cell(th.red_yr[q], s, y, h, x);
       const int c = y * SW + h;
       const double old = R[c];
-      const double nw = sor_cell(a.omega, cx_at(lds, x), cy_at(lds, y), cz, rb[q], old, B[y * SW + ((x - 1) >> 1)],
-                                 B[y * SW + ((x + 1) >> 1)], B[c - SW], B[c + SW], Bm[c], Bp[c]);
+      const double nw = sor_cell(a.omega, cx_at(lds, x), cy_at(lds, y), th.cz0, th.rc[q], old,
+                                 B[y * SW + ((x - 1) >> 1)], B[y * SW + ((x + 1) >> 1)], B[c - SW], B[c + SW], Bm[c],
+                                 Bp[c]);
       R[c] = nw;
       const double dd = fabs(nw - old);
       delta = delta > dd ? delta : dd;
@@ -218,42 +283,70 @@ struct SorTile {
   }
 
   // black update of plane k (tile) and write-back of the tile's plane k
-  __host__ __device__ static double black_phase(const FusedArgs &a, double *lds, int tid, int k, int iL, int jL,
-                                                const Coef3 &cz, const double *bb) {
+  __host__ __device__ static double black_phase(const FusedArgs &a, const Tile &T, double *lds, const Thread &th,
+                                                int k) {
     const double *R = slot(lds, k, 1), *Rm = slot(lds, k - 1, 1), *Rp = slot(lds, k + 1, 1);
     const double *B = slot(lds, k, 0);
+    double *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
     double delta = 0.0;
     for (int q = 0; q < NB; ++q) {
-      const int e = tid + q * NT;
-      if (e >= NBC) continue;
+      if (th.pair_yh[q] < 0) continue;
       int y, h, off;
-      pair_cell(e, k, y, h, off);
-      const int j = jL + y;
+      pair_cell(th.pair_yh[q], k, y, h, off);
+      const int j = T.jL + y;
       if (j > a.d.ny + 1) continue;
-      const int x = 2 * h + off, i = iL + x, ir = iL + 2 * h + 1 - off;
+      const int x = 2 * h + off, i = T.iL + x, ir = T.iL + 2 * h + 1 - off;
       const int c = y * SW + h;
+      const int row = a.d.NX * (j - 1) - 1;
       if (i <= a.d.nx + 1) {
         const double old = B[c];
         double nw = old;
-        if (!sor_skip(bb[q])) {
-          nw = sor_cell(a.omega, cx_at(lds, x), cy_at(lds, y), cz, bb[q], old, R[y * SW + ((x - 1) >> 1)],
+        if (!sor_skip(th.bc[q])) {
+          nw = sor_cell(a.omega, cx_at(lds, x), cy_at(lds, y), th.czm, th.bc[q], old, R[y * SW + ((x - 1) >> 1)],
                         R[y * SW + ((x + 1) >> 1)], R[c - SW], R[c + SW], Rm[c], Rp[c]);
           const double dd = fabs(nw - old);
           delta = delta > dd ? delta : dd;
         }
-        a.dst[ix3(a.d, i, j, k)] = nw;
+        P[row + i] = nw;
       }
-      if (ir <= a.d.nx + 1) a.dst[ix3(a.d, ir, j, k)] = R[c];
+      if (ir <= a.d.nx + 1) P[row + ir] = R[c];
     }
     return delta;
   }
 
-  // tile coordinates of linear workgroup L of n: consecutive tiles go to one
-  // XCD (workgroups are dispatched to the 8 XCDs round-robin), so the halo
-  // rows two neighbouring tiles share are fetched into the same L2
-  __host__ __device__ static int xcd_tile(int L, int n) {
-    const int per = n / 8, rem = n % 8, xcd = L % 8, q = L / 8;
-    return xcd < rem ? xcd * (per + 1) + q : rem * (per + 1) + (xcd - rem) * per + q;
+  // ---- the march: prologue, then per step s = k0-1 .. k1+1 the phases A | B | C ----
+  __host__ __device__ static void prologue(const FusedArgs &a, const Tile &T, double *lds, Thread &th) {
+    double *r = th.pcn[0];
+    load_plane(a, th, T.k0 - 2, r);
+    store_plane(lds, th, T.k0 - 2, r);
+    load_plane(a, th, T.k0 - 1, r);
+    store_plane(lds, th, T.k0 - 1, r);
+    for (int d = 0; d < D; ++d) load_plane(a, th, T.k0 + d, th.pcn[d]);
+    load_bdt_red(a, T, th, T.k0 - 1, th.rn);
+    for (int q = 0; q < NB; ++q) th.bn[q] = 0.0;
+  }
+
+  // buf holds plane s+1 on entry and plane s+1+D on exit
+  __host__ __device__ static void phase_a(const FusedArgs &a, const Tile &T, double *lds, Thread &th, int s,
+                                          double *buf) {
+    store_plane(lds, th, s + 1, buf);
+    for (int q = 0; q < NR; ++q) th.rc[q] = th.rn[q];
+    for (int q = 0; q < NB; ++q) th.bc[q] = th.bn[q];
+    if (s + 1 + D <= T.k1 + 2) load_plane(a, th, s + 1 + D, buf);
+    if (s + 1 <= T.k1 + 1) load_bdt_red(a, T, th, s + 1, th.rn);
+    if (s >= T.k0 && s <= T.k1) load_bdt_black(a, T, th, s, th.bn);
+    th.czp = cz_of(a, s + 1);
+  }
+
+  __host__ __device__ static double phase_b(const FusedArgs &a, double *lds, const Thread &th, int s) {
+    return red_phase(a, lds, th, s);
+  }
+
+  __host__ __device__ static double phase_c(const FusedArgs &a, const Tile &T, double *lds, Thread &th, int s) {
+    const double d = (s - 1 >= T.k0) ? black_phase(a, T, lds, th, s - 1) : 0.0;
+    th.czm = th.cz0;
+    th.cz0 = th.czp;
+    return d;
   }
 };
 

@@ -75,7 +75,7 @@ SIGNATURES = {
     "biocfd_redblack_sor": [_I, _L, _D, _D],
     "biocfd_correct": [_I, _D],
     "biocfd_sor_config": [_I, _I],
-    "biocfd_sor_variant_shape": [_I, _P, _P, _P],
+    "biocfd_sor_variant_shape": [_I, _P, _P, _P, _P],
     "biocfd_sor_time": [_I, _I, _I, _I, _D, _P],
     "biocfd_compute_surface_norm": [_I, _P, _P, _P, _L],
     "biocfd_tagging": [_I],

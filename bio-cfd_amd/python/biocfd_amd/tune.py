@@ -18,12 +18,12 @@ DEFAULT_PLANES = (8, 16, 32, 64)
 
 
 def variants():
-    """[(variant, tile_i, tile_j, threads)] of the compiled SOR kernels."""
+    """[(variant, tile_i, tile_j, threads, prefetch_depth)] of the compiled SOR kernels."""
     out = []
     v = 0
-    a, b, t = C.c_int(), C.c_int(), C.c_int()
-    while H.lib().biocfd_sor_variant_shape(v, C.byref(a), C.byref(b), C.byref(t)) == 0:
-        out.append((v, a.value, b.value, t.value))
+    a, b, t, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    while H.lib().biocfd_sor_variant_shape(v, C.byref(a), C.byref(b), C.byref(t), C.byref(d)) == 0:
+        out.append((v, a.value, b.value, t.value, d.value))
         v += 1
     return out
 
@@ -43,10 +43,10 @@ def autotune(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PL
     (best_variant, best_planes, table) with table rows
     {"variant", "tile", "threads", "planes", "us_per_iteration"}."""
     table = []
-    for v, ti, tj, nt in variants():
+    for v, ti, tj, nt, depth in variants():
         for p in (planes if v > 0 else (32,)):
             ms = time_variant(block, v, p, iters, dt)
-            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "planes": p,
+            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "planes": p,
                           "us_per_iteration": round(1e3 * ms, 2)})
     best = min(table, key=lambda r: r["us_per_iteration"])
     if install:

@@ -125,8 +125,9 @@ int biocfd_correct(int blk, double dt);
  * over tiles of tile_i x tile_j cells and `planes_per_workgroup` z-planes.
  * Every variant gives the same bits; the choice only changes speed. */
 int biocfd_sor_config(int variant, int planes_per_workgroup);
-/* tile shape of a variant (0,0,0 for variant 0); fails past the last one */
-int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads);
+/* tile shape, threads per workgroup and z-plane prefetch depth of a variant
+ * (all 0 for variant 0); fails past the last variant */
+int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth);
 /* Autotuning probe (no reference counterpart): time `iters` SOR iterations
  * of a variant on block `blk`'s current pc and b (milliseconds, HIP events);
  * pc and the block's SOR state are restored afterwards. */

@@ -2,23 +2,20 @@
 each compiled shape, several planes-per-workgroup counts) gives the oracle's
 time step: the SOR iteration count exactly and the fields within 1e-12
 relative (bit-identical in practice)."""
-import ctypes as C
-
 import pytest
 
 from parity import compare, run_oracle, small_cases
 
 pytestmark = pytest.mark.gpu
 
+# the library's default SOR variant, restored after each test
+DEFAULT_VARIANT = 6
+
 
 def n_variants():
-    from biocfd_amd import hip as H
+    from biocfd_amd import tune
 
-    n = 0
-    a, b, t = C.c_int(), C.c_int(), C.c_int()
-    while H.lib().biocfd_sor_variant_shape(n, C.byref(a), C.byref(b), C.byref(t)) == 0:
-        n += 1
-    return n
+    return len(tune.variants())
 
 
 def cases():
@@ -53,7 +50,7 @@ def test_sor_variants_parity(name):
                 for g, (r, o) in enumerate(zip(ref, got)):
                     bad += compare(r, o, label=f"{name} variant {v} planes {planes} block {g + 1}: ")
     finally:
-        H.call("biocfd_sor_config", 1, 32)
+        H.call("biocfd_sor_config", DEFAULT_VARIANT, 32)
     assert not bad, "\n".join(bad[:40])
 
 
@@ -82,4 +79,4 @@ def test_sor_time_leaves_state_unchanged():
         assert not bad, "\n".join(bad[:40])
     finally:
         sim.close()
-        H.call("biocfd_sor_config", 1, 32)
+        H.call("biocfd_sor_config", DEFAULT_VARIANT, 32)

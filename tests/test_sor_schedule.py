@@ -28,3 +28,14 @@ def test_fused_sor_schedule_matches_inplace_sweep(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, (r.stdout[-3000:], r.stderr[-3000:])
     assert "all cases bit-identical" in r.stdout
+
+
+def test_every_compiled_shape_is_emulated():
+    """The host emulation runs every (tile, threads, depth) the library instantiates."""
+    import re
+
+    src = open(os.path.join(ROOT, "bio-cfd_amd", "csrc", "k_pcor.hip")).read()
+    emu = open(os.path.join(ROOT, "tests", "sor_tile_host.cpp")).read()
+    lib = set(re.findall(r"launch_fused<(\d+), (\d+), (\d+), (\d+)>\}", src))
+    run = set(re.findall(r"run_shape<(\d+), (\d+), (\d+), (\d+)>\(\)", emu))
+    assert lib and lib <= run, sorted(lib - run)
