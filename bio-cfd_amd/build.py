@@ -1,6 +1,9 @@
 """Build the gfx950 runtime library libbiocfd_hip.so in-tree with hipcc.
 
     python bio-cfd_amd/build.py            # compile (always) into bio-cfd_amd/lib/
+
+Also builds the Fortran bindings (fortran/biocfd_hip.f90) and the Fortran host
+driver (fortran/biocfd_run.f90 -> lib/biocfd_run) with amdflang.
 """
 from __future__ import annotations
 
@@ -44,6 +47,43 @@ def _compile(src: str) -> str:
     return obj
 
 
+FORTRAN_DIR = os.path.join(HERE, "fortran")
+FORTRAN_SOURCES = ["biocfd_hip.f90", "biocfd_run.f90"]
+FORTRAN_EXE = os.path.join(LIB_DIR, "biocfd_run")
+
+
+def flang() -> str:
+    for cand in (os.environ.get("FC_AMD"), "/opt/rocm/bin/amdflang", "/opt/rocm/llvm/bin/flang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return "amdflang"
+
+
+def build_fortran(verbose: bool = True) -> str:
+    """The Fortran bindings module and the Fortran host driver, linked
+    against the library (run path $ORIGIN)."""
+    mod_dir = os.path.join(OBJ_DIR, "fortran")
+    os.makedirs(mod_dir, exist_ok=True)
+    objs = []
+    for src in FORTRAN_SOURCES:
+        obj = os.path.join(mod_dir, os.path.splitext(src)[0] + ".o")
+        cmd = [flang(), "-O2", "-c", os.path.join(FORTRAN_DIR, src), "-o", obj, "-module-dir", mod_dir,
+               "-I", mod_dir]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed for {src}:\n{r.stdout}\n{r.stderr}")
+        objs.append(obj)
+    tmp = FORTRAN_EXE + ".tmp"
+    cmd = [flang(), *objs, "-o", tmp, "-L", LIB_DIR, "-lbiocfd_hip", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, FORTRAN_EXE)
+    if verbose:
+        print(f"built {FORTRAN_EXE}")
+    return FORTRAN_EXE
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ_DIR, exist_ok=True)
     srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
@@ -57,6 +97,7 @@ def build(verbose: bool = True) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_fortran(verbose)
     return LIB
 
 

@@ -1,0 +1,574 @@
+! Fortran host driver of the gfx950 hot path: reads a case file, runs the
+! set-up and the time steps through the C ABI (module biocfd_hip) in the call
+! order of the reference driver (app/main.F90:43-265), and writes record
+! dumps of the block state at the requested steps.
+!
+!   biocfd_run CASE_FILE OUT_DIR [--time]
+!
+! Case file: the stream written by biocfd_amd.cases.write_case (magic
+! 20261203, version 1).  Dumps: OUT_DIR/dump_s<step>_b<block>.bin, records of
+! (name(24), kind, ndim, shape, data) as read by biocfd_amd.dumps.read_dump.
+! --time: no dumps; writes OUT_DIR/timing.txt ("steps N seconds S") for the
+! time-step loop, set-up excluded.
+!
+! The single-block-with-body configuration runs the IB routines on block 1
+! together with the domain boundary conditions; a coarse block 1 without a
+! body gets fine_block_cell + cellCount_solid_coarse (main.F90:93-94).
+module biocfd_host
+  use, intrinsic :: iso_fortran_env, only: int64, real64
+  use, intrinsic :: iso_c_binding, only: c_int
+  use biocfd_hip
+  implicit none
+
+  integer(int64), parameter :: p_hash = 2147483647_int64
+
+  ! kinematic state slots of biocfd_shift_surface_nodes_initial (0-based)
+  integer, parameter :: k_ydot = 12, k_yddot = 13, k_xdot = 15, k_thetadot1 = 18, k_thetaddot1 = 19, &
+                        k_thetadot2 = 20, k_thetaddot2 = 21, k_alphadot = 22, k_pivx = 23, k_pivy = 24, &
+                        k_pivz = 25, k_xpth2 = 7, k_ymove = 27
+
+  type :: host_block
+    integer(int64) :: nx = 0, ny = 0, nz = 0, cintp = 3, has_ib = 0, moving = 0
+    real(real64) :: dx = 0, dy = 0, dz = 0, omega = 1
+    real(real64), allocatable :: x1(:), y1(:), z1(:), deltax(:), deltay(:), deltaz(:)
+    real(real64), allocatable :: xu(:), yu(:), zu(:), xv(:), yv(:), zv(:), xw(:), yw(:), zw(:)
+    real(real64), allocatable :: xp(:), yp(:), zp(:)
+    integer(int64) :: nn = 0, ne = 0, box(6) = 0
+    real(real64) :: centre(3) = 0, a0 = 0, yamp = 0, bfreq = 0
+    real(real64), allocatable :: xnode(:), ynode(:), znode(:), xnode1(:), ynode1(:), znode1(:)
+    integer(int64), allocatable :: node_id(:), el1(:), el2(:), el3(:), surf(:)
+    real(real64) :: kin(0:BKS_N - 1) = 0
+  end type
+
+  type :: host_interface
+    integer(int64) :: a = 0, b = 0, a_msh = 1, b_msh = 1
+    real(real64) :: ext(6) = 0
+  end type
+
+  integer(int64) :: nblocks, nintf, ib_block1, pc_ita_max, nsteps, inor, seed
+  real(real64) :: dt, re, uc, epsi, noise_amp, aoa, phase_deg, piv_pt, dxmin, freq, mu_f, rho_f
+  type(host_block), allocatable, target :: blk(:)
+  type(host_interface), allocatable :: itf(:)
+  integer(int64), allocatable :: dump_steps(:)
+  real(real64) :: totime, phase
+  integer(int64) :: ita
+
+contains
+
+  subroutine read_case(path)
+    character(len=*), intent(in) :: path
+    integer :: u
+    integer(int64) :: magic, version, g, hdr(9)
+    open(newunit=u, file=path, access='stream', form='unformatted', status='old')
+    read(u) hdr
+    magic = hdr(1)
+    version = hdr(2)
+    if (magic /= 20261203_int64 .or. version /= 1_int64) then
+      write(*, '(a)') 'biocfd_run: not a case file: '//trim(path)
+      error stop 3
+    end if
+    nblocks = hdr(3)
+    nintf = hdr(4)
+    ib_block1 = hdr(5)
+    pc_ita_max = hdr(6)
+    nsteps = hdr(7)
+    inor = hdr(8)
+    seed = hdr(9)
+    read(u) dt, re, uc, epsi, noise_amp, aoa, phase_deg, piv_pt, dxmin, freq, mu_f, rho_f
+    allocate(blk(nblocks), itf(nintf))
+    do g = 1, nblocks
+      call read_block(u, blk(g))
+    end do
+    do g = 1, nintf
+      read(u) itf(g)%a, itf(g)%b, itf(g)%a_msh, itf(g)%b_msh
+      read(u) itf(g)%ext
+    end do
+    read(u) g
+    allocate(dump_steps(g))
+    if (g > 0) read(u) dump_steps
+    close(u)
+  end subroutine
+
+  subroutine read_block(u, b)
+    integer, intent(in) :: u
+    type(host_block), intent(inout) :: b
+    integer(int64) :: nx, ny, nz
+    read(u) b%nx, b%ny, b%nz, b%cintp, b%has_ib, b%moving
+    read(u) b%dx, b%dy, b%dz, b%omega
+    nx = b%nx
+    ny = b%ny
+    nz = b%nz
+    allocate(b%x1(nx + 3), b%y1(ny + 3), b%z1(nz + 3), b%deltax(nx + 2), b%deltay(ny + 2), b%deltaz(nz + 2))
+    allocate(b%xu(nx + 3), b%yu(ny + 2), b%zu(nz + 2), b%xv(nx + 2), b%yv(ny + 3), b%zv(nz + 2))
+    allocate(b%xw(nx + 2), b%yw(ny + 2), b%zw(nz + 3), b%xp(nx + 2), b%yp(ny + 2), b%zp(nz + 2))
+    read(u) b%x1, b%y1, b%z1
+    read(u) b%deltax, b%deltay, b%deltaz
+    read(u) b%xu, b%yu, b%zu
+    read(u) b%xv, b%yv, b%zv
+    read(u) b%xw, b%yw, b%zw
+    read(u) b%xp, b%yp, b%zp
+    if (b%has_ib == 1) then
+      read(u) b%nn, b%ne
+      read(u) b%box
+      read(u) b%centre, b%a0, b%yamp, b%bfreq
+      allocate(b%xnode(b%nn), b%ynode(b%nn), b%znode(b%nn), b%node_id(b%nn))
+      allocate(b%xnode1(b%nn), b%ynode1(b%nn), b%znode1(b%nn))
+      allocate(b%el1(b%ne), b%el2(b%ne), b%el3(b%ne), b%surf(b%ne))
+      read(u) b%xnode, b%ynode, b%znode
+      read(u) b%node_id
+      read(u) b%el1, b%el2, b%el3, b%surf
+      b%xnode1 = 0
+      b%ynode1 = 0
+      b%znode1 = 0
+    end if
+  end subroutine
+
+  ! the forcing routines' kinematic scalars (enum biocfd_kin order)
+  function kin_vec(b) result(k)
+    type(host_block), intent(in) :: b
+    real(real64) :: k(0:BK_N - 1)
+    k = [b%kin(k_xdot), b%kin(k_ydot), b%kin(k_yddot), b%kin(k_thetadot1), b%kin(k_thetaddot1), &
+         b%kin(k_thetadot2), b%kin(k_thetaddot2), b%kin(k_alphadot), b%kin(k_pivx), b%kin(k_pivy), &
+         b%kin(k_pivz)]
+  end function
+
+  ! uniform number in [0,1] of the seeded integer hash (the oracle's noise)
+  pure function noise_uniform(s, field, lin) result(r)
+    integer(int64), intent(in) :: s, field, lin
+    real(real64) :: r
+    integer(int64) :: x
+    integer :: q
+    x = mod(s + 2654435_int64 * field + lin, p_hash)
+    do q = 1, 3
+      x = mod(48271_int64 * x + 11_int64, p_hash)
+    end do
+    r = real(x, real64) / 2147483647.0_real64
+  end function
+
+  ! initialConditions (InitialConditions.f90:10-36) plus the seeded noise on
+  ! interior fluid cells, uploaded
+  subroutine initial_fields(g)
+    integer, intent(in) :: g
+    type(host_block), pointer :: b
+    real(real64), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), p(:, :, :)
+    integer(int64), allocatable :: cell(:, :, :), zero_pr(:, :, :)
+    integer(int64) :: i, j, k, nxg, nyg, lin, s
+    b => blk(g)
+    nxg = b%nx + 2
+    nyg = b%ny + 2
+    allocate(u(nxg, nyg, b%nz + 2), v(nxg, nyg, b%nz + 2), w(nxg, nyg, b%nz + 2), p(nxg, nyg, b%nz + 2))
+    allocate(cell(nxg, nyg, b%nz + 2), zero_pr(nxg + 1, nyg + 1, b%nz + 3))
+    call biocfd_check(biocfd_tag_download(g, BT_CELL, cell), 'tag_download')
+    u = uc
+    v = 0
+    w = 0
+    p = 0
+    if (noise_amp /= 0) then
+      s = seed + 1000_int64 * g
+      do k = 2, b%nz + 1
+        do j = 2, b%ny + 1
+          do i = 2, b%nx + 1
+            if (cell(i, j, k) /= 0) cycle
+            lin = (i - 1) + nxg * ((j - 1) + nyg * (k - 1))
+            u(i, j, k) = u(i, j, k) + noise_amp * (2.0_real64 * noise_uniform(s, 1_int64, lin) - 1.0_real64)
+            v(i, j, k) = v(i, j, k) + noise_amp * (2.0_real64 * noise_uniform(s, 2_int64, lin) - 1.0_real64)
+            w(i, j, k) = w(i, j, k) + noise_amp * (2.0_real64 * noise_uniform(s, 3_int64, lin) - 1.0_real64)
+          end do
+        end do
+      end do
+    end if
+    call biocfd_check(biocfd_field_upload(g, BF_U, u), 'field_upload u')
+    call biocfd_check(biocfd_field_upload(g, BF_UT, u), 'field_upload ut')
+    call biocfd_check(biocfd_field_upload(g, BF_V, v), 'field_upload v')
+    call biocfd_check(biocfd_field_upload(g, BF_VT, v), 'field_upload vt')
+    call biocfd_check(biocfd_field_upload(g, BF_W, w), 'field_upload w')
+    call biocfd_check(biocfd_field_upload(g, BF_WT, w), 'field_upload wt')
+    call biocfd_check(biocfd_field_upload(g, BF_P, p), 'field_upload p')
+    zero_pr = 0
+    call biocfd_check(biocfd_tag_upload(g, BT_CELL_PR, zero_pr), 'tag_upload cell_pr')
+  end subroutine
+
+  ! set-up, main.F90:43-127
+  subroutine setup()
+    integer :: g
+    integer(int64) :: q
+    type(host_block), pointer :: b
+    call biocfd_check(biocfd_init(0_c_int), 'init')
+    do g = 1, int(nblocks)
+      b => blk(g)
+      call biocfd_check(biocfd_block_create(g, b%nx, b%ny, b%nz, b%omega, b%cintp, b%dx, b%dy, b%dz), &
+                        'block_create')
+      call upload_grid(g, BG_X1, b%x1)
+      call upload_grid(g, BG_Y1, b%y1)
+      call upload_grid(g, BG_Z1, b%z1)
+      call upload_grid(g, BG_DX, b%deltax)
+      call upload_grid(g, BG_DY, b%deltay)
+      call upload_grid(g, BG_DZ, b%deltaz)
+      call upload_grid(g, BG_XU, b%xu)
+      call upload_grid(g, BG_YU, b%yu)
+      call upload_grid(g, BG_ZU, b%zu)
+      call upload_grid(g, BG_XV, b%xv)
+      call upload_grid(g, BG_YV, b%yv)
+      call upload_grid(g, BG_ZV, b%zv)
+      call upload_grid(g, BG_XW, b%xw)
+      call upload_grid(g, BG_YW, b%yw)
+      call upload_grid(g, BG_ZW, b%zw)
+      call upload_grid(g, BG_XP, b%xp)
+      call upload_grid(g, BG_YP, b%yp)
+      call upload_grid(g, BG_ZP, b%zp)
+    end do
+    do q = 1, nintf
+      call biocfd_check(biocfd_interface_create(int(q, c_int), int(itf(q)%a, c_int), int(itf(q)%b, c_int), &
+                                                itf(q)%a_msh, itf(q)%b_msh, itf(q)%ext), 'interface_create')
+    end do
+    do g = 1, int(nblocks)
+      b => blk(g)
+      if (b%has_ib /= 1) cycle
+      call biocfd_check(biocfd_body_create(g, b%nn, b%ne, b%el1, b%el2, b%el3, b%surf, b%box), 'body_create')
+      b%kin = 0
+      b%kin(0:5) = [b%centre(1), b%centre(2), b%centre(3), b%a0, b%yamp, b%bfreq]
+      call biocfd_check(biocfd_shift_surface_nodes_initial(b%nn, b%xnode, b%ynode, b%znode, b%node_id, aoa, &
+                        piv_pt, 0_int64, dt, dxmin, 0.0_real64, b%kin, b%xnode1, b%ynode1, b%znode1), &
+                        'shift_surface_nodes_initial')
+      call biocfd_check(biocfd_compute_surface_norm(g, b%xnode1, b%ynode1, b%znode1, inor), 'compute_surface_norm')
+    end do
+    do g = 1, int(nblocks)
+      if (blk(g)%has_ib /= 1) cycle
+      call biocfd_check(biocfd_tagging(g), 'tagging')
+      call biocfd_check(biocfd_cell_count_solid(g), 'cell_count_solid')
+    end do
+    if (blk(1)%has_ib /= 1) then
+      call biocfd_check(biocfd_fine_block_cell(1_c_int), 'fine_block_cell')
+      call biocfd_check(biocfd_cell_count_solid_coarse(1_c_int), 'cell_count_solid_coarse')
+    end if
+    do g = 1, int(nblocks)
+      call initial_fields(g)
+    end do
+    do g = 1, int(nblocks)
+      if (blk(g)%has_ib == 1) then
+        call biocfd_check(biocfd_compute_norm_distance(g), 'compute_norm_distance')
+        call biocfd_check(biocfd_find_ts_cells(g), 'find_ts_cells')
+      end if
+      call biocfd_check(biocfd_coefficient_matrix(g, merge(1_c_int, 0_c_int, g == 1)), 'coefficient_matrix')
+    end do
+    do g = 1, int(nblocks)
+      call biocfd_check(biocfd_non_uni_coeff(g), 'non_uni_coeff')
+    end do
+    phase = phase_deg * acos(-1.0_real64) / 180.0_real64
+    totime = dt
+    ita = 0
+  end subroutine
+
+  subroutine upload_grid(g, which, a)
+    integer, intent(in) :: g
+    integer(c_int), intent(in) :: which
+    real(real64), intent(in) :: a(:)
+    call biocfd_check(biocfd_grid_upload(g, which, a, int(size(a), int64)), 'grid_upload')
+  end subroutine
+
+  ! one time step, main.F90:133-263
+  subroutine time_step()
+    integer :: g
+    type(host_block), pointer :: b
+    real(real64) :: kv(0:BK_N - 1)
+    ita = ita + 1
+    totime = totime + dt
+    do g = 1, int(nblocks)
+      b => blk(g)
+      call biocfd_check(biocfd_ns_momentum(g, dt, re), 'ns_momentum')
+      if (g == 1) call biocfd_check(biocfd_velocity_bc(g, dt, uc), 'velocity_bc')
+      if (b%has_ib == 1) then
+        call biocfd_check(biocfd_solid_cell_bc(g), 'solid_cell_bc')
+        kv = kin_vec(b)
+        call biocfd_check(biocfd_velocity_forcing1(g, kv), 'velocity_forcing1')
+      end if
+      if (g == 1) call biocfd_check(biocfd_velocity_bc(g, dt, uc), 'velocity_bc')
+    end do
+    call biocfd_check(biocfd_poisson_solver(1_c_int, int(nblocks, c_int), pc_ita_max, epsi, dt, uc), &
+                      'poisson_solver')
+    do g = 1, int(nblocks)
+      b => blk(g)
+      if (b%has_ib /= 1) cycle
+      kv = kin_vec(b)
+      call biocfd_check(biocfd_pressure_forcing1(g, kv), 'pressure_forcing1')
+    end do
+    do g = 1, int(nblocks)
+      b => blk(g)
+      if (b%has_ib /= 1) cycle
+      call biocfd_check(biocfd_compute_surface_variables(b%nn, b%xnode, b%ynode, b%znode, b%node_id, phase, &
+                        piv_pt, ita, dt, dxmin, freq, totime, b%kin, b%xnode1, b%ynode1, b%znode1), &
+                        'compute_surface_variables')
+    end do
+    do g = 1, int(nblocks)
+      b => blk(g)
+      if (b%has_ib /= 1) cycle
+      call biocfd_check(biocfd_compute_surface_norm(g, b%xnode1, b%ynode1, b%znode1, inor), 'compute_surface_norm')
+    end do
+    do g = 1, int(nblocks)
+      b => blk(g)
+      if (b%has_ib /= 1) cycle
+      kv = kin_vec(b)
+      call biocfd_check(biocfd_selective_retagging(g, 0_int64), 'selective_retagging')
+      call biocfd_check(biocfd_cell_count_solid(g), 'cell_count_solid')
+      call biocfd_check(biocfd_solid_cell_bc_move(g, 0_int64), 'solid_cell_bc_move')
+      call biocfd_check(biocfd_update_velocity_newv(g, 0_int64), 'update_velocity_newv')
+      call biocfd_check(biocfd_compute_norm_distance(g), 'compute_norm_distance')
+      call biocfd_check(biocfd_find_ts_cells(g), 'find_ts_cells')
+      call biocfd_check(biocfd_velocity_forcing_field(g, kv), 'velocity_forcing_field')
+      call biocfd_check(biocfd_pressure_forcing_field(g, kv), 'pressure_forcing_field')
+      call biocfd_check(biocfd_velocity_forcing_ghost(g, kv), 'velocity_forcing_ghost')
+      call biocfd_check(biocfd_pressure_forcing_ghost(g, kv), 'pressure_forcing_ghost')
+    end do
+  end subroutine
+
+  ! ---------------------------------------------------------------- dumps
+  subroutine dump_all(out_dir, step)
+    character(len=*), intent(in) :: out_dir
+    integer(int64), intent(in) :: step
+    integer :: g, u, q
+    character(len=600) :: fname
+    type(host_block), pointer :: b
+    integer(int64) :: cnt(0:BC_N - 1), it(1), nxg, nyg, nzg
+    real(real64) :: e2(1), es(1), res(4), kv(0:BK_N - 1)
+    real(real64), allocatable :: f3(:, :, :), r1(:), a2(:, :)
+    integer(int64), allocatable :: t3(:, :, :), l2(:, :), i1(:)
+    character(len=24), parameter :: fields(9) = [character(len=24) :: 'u', 'v', 'w', 'p', 'ut', 'vt', 'wt', &
+                                                 'pc', 'b']
+    integer(c_int), parameter :: field_ids(9) = [BF_U, BF_V, BF_W, BF_P, BF_UT, BF_VT, BF_WT, BF_PC, BF_B]
+    character(len=24), parameter :: tags(4) = [character(len=24) :: 'cell', 'cell2', 'nodeIdTag', 'cell_n']
+    integer(c_int), parameter :: tag_ids(4) = [BT_CELL, BT_CELL2, BT_NODEIDTAG, BT_CELL_N]
+    character(len=24), parameter :: nels(7) = [character(len=24) :: 'nelp', 'nelu1', 'nelu2', 'nelv1', &
+                                               'nelv2', 'nelw1', 'nelw2']
+    character(len=24), parameter :: nds(7) = [character(len=24) :: 'pNormDis', 'u1NormDis', 'u2NormDis', &
+                                              'v1NormDis', 'v2NormDis', 'w1NormDis', 'w2NormDis']
+    character(len=24), parameter :: els(7) = [character(len=24) :: 'xcent', 'ycent', 'zcent', 'cosAlpha', &
+                                              'cosBeta', 'cosGamma', 'element_length']
+    character(len=24), parameter :: ghs(14) = [character(len=24) :: 'p_ghost', 'pt_ghost', 'u1_ghost', &
+                                               'u1t_ghost', 'u2_ghost', 'u2t_ghost', 'v1_ghost', 'v1t_ghost', &
+                                               'v2_ghost', 'v2t_ghost', 'w1_ghost', 'w1t_ghost', 'w2_ghost', &
+                                               'w2t_ghost']
+    do g = 1, int(nblocks)
+      b => blk(g)
+      nxg = b%nx + 2
+      nyg = b%ny + 2
+      nzg = b%nz + 2
+      write(fname, '(a,a,i0,a,i0,a)') trim(out_dir), '/dump_s', step, '_b', g, '.bin'
+      open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+      allocate(f3(nxg, nyg, nzg))
+      do q = 1, 9
+        call biocfd_check(biocfd_field_download(g, field_ids(q), f3), 'field_download')
+        call rec_r3(u, fields(q), f3)
+      end do
+      deallocate(f3)
+      do q = 1, 4
+        if (q == 1 .or. q == 4) then
+          allocate(t3(nxg, nyg, nzg))
+        else
+          allocate(t3(nxg + 1, nyg + 1, nzg + 1))
+        end if
+        call biocfd_check(biocfd_tag_download(g, tag_ids(q), t3), 'tag_download')
+        call rec_i3(u, tags(q), t3)
+        deallocate(t3)
+      end do
+      call biocfd_check(biocfd_sor_state(g, it, e2, es), 'sor_state')
+      call rec_i0(u, 'nIterPcor', it(1))
+      call rec_r0(u, 'derr2', e2(1))
+      call rec_r0(u, 'derrStdSt', es(1))
+      call biocfd_check(biocfd_counts(g, cnt), 'counts')
+      call rec_i0(u, 'fluidCellCount', cnt(BC_FLUID))
+      call rec_i0(u, 'solidCellCount', cnt(BC_SOLID))
+      call rec_i0(u, 'ibCellCount', cnt(BC_IB))
+      call rec_i0(u, 'redCellCount', cnt(BC_RED))
+      call rec_i0(u, 'blackCellCount', cnt(BC_BLACK))
+      call rec_i0(u, 'TSCellCount', cnt(BC_TS))
+      call rec_list(u, g, 'fluidIndexPtr', BL_FLUID, cnt(BC_FLUID))
+      call rec_list(u, g, 'redCellIndexPtr', BL_RED, cnt(BC_RED))
+      call rec_list(u, g, 'blackCellIndexPtr', BL_BLACK, cnt(BC_BLACK))
+      allocate(a2(b%nx, 3))
+      call biocfd_check(biocfd_coef_download(g, 0_c_int, a2, b%nx), 'coef_download')
+      call rec_r2(u, 'Acx', a2)
+      deallocate(a2)
+      allocate(a2(b%ny, 3))
+      call biocfd_check(biocfd_coef_download(g, 1_c_int, a2, b%ny), 'coef_download')
+      call rec_r2(u, 'Acy', a2)
+      deallocate(a2)
+      allocate(a2(b%nz, 3))
+      call biocfd_check(biocfd_coef_download(g, 2_c_int, a2, b%nz), 'coef_download')
+      call rec_r2(u, 'Acz', a2)
+      deallocate(a2)
+      if (b%has_ib == 1) then
+        call rec_list(u, g, 'interceptedIndexPtr', BL_INTERCEPTED, cnt(BC_IB))
+        call rec_list(u, g, 'solidIndexPtr', BL_SOLID, cnt(BC_SOLID))
+        call rec_list(u, g, 'TSIndexPtr', BL_TS, cnt(BC_TS))
+        allocate(i1(cnt(BC_TS)))
+        call biocfd_check(biocfd_index_ts_download(g, i1, cnt(BC_TS)), 'index_ts_download')
+        call rec_i1(u, 'index_ts', i1)
+        deallocate(i1)
+        allocate(i1(cnt(BC_IB)), r1(cnt(BC_IB)))
+        do q = 1, 7
+          call biocfd_check(biocfd_nel_download(g, int(q - 1, c_int), i1, cnt(BC_IB)), 'nel_download')
+          call rec_i1(u, nels(q), i1)
+          call biocfd_check(biocfd_normdis_download(g, int(q - 1, c_int), r1, cnt(BC_IB)), 'normdis_download')
+          call rec_r1(u, nds(q), r1)
+        end do
+        deallocate(i1, r1)
+        allocate(r1(b%ne))
+        do q = 1, 7
+          call biocfd_check(biocfd_elem_download(g, int(q - 1, c_int), r1, b%ne), 'elem_download')
+          call rec_r1(u, els(q), r1)
+        end do
+        deallocate(r1)
+        allocate(r1(cnt(BC_TS)))
+        do q = 1, 14
+          call biocfd_check(biocfd_ghost_download(g, int(q - 1, c_int), r1, cnt(BC_TS)), 'ghost_download')
+          call rec_r1(u, ghs(q), r1)
+        end do
+        deallocate(r1)
+        call rec_r1(u, 'xnode1', b%xnode1)
+        call rec_r1(u, 'ynode1', b%ynode1)
+        call rec_r1(u, 'znode1', b%znode1)
+        call rec_r0(u, 'xdot', b%kin(k_xdot))
+        call rec_r0(u, 'ydot', b%kin(k_ydot))
+        call rec_r0(u, 'yddot', b%kin(k_yddot))
+        call rec_r0(u, 'thetaDot1', b%kin(k_thetadot1))
+        call rec_r0(u, 'thetaDDot1', b%kin(k_thetaddot1))
+        call rec_r0(u, 'piv_x', b%kin(k_pivx))
+        call rec_r0(u, 'piv_y', b%kin(k_pivy))
+        call rec_r0(u, 'piv_z', b%kin(k_pivz))
+        call rec_r0(u, 'xpth2', b%kin(k_xpth2))
+        call rec_r0(u, 'ymove', b%kin(k_ymove))
+        kv = kin_vec(b)
+        call biocfd_check(biocfd_stress_calculation(g, mu_f, rho_f, kv, res), 'stress_calculation')
+        call rec_r0(u, 'viscous_drag', res(1))
+        call rec_r0(u, 'pressure_drag', res(2))
+        call rec_r0(u, 'viscous_lift', res(3))
+        call rec_r0(u, 'pressure_lift', res(4))
+      end if
+      close(u)
+    end do
+  end subroutine
+
+  subroutine rec_list(u, g, name, which, n)
+    integer, intent(in) :: u, g
+    character(len=*), intent(in) :: name
+    integer(c_int), intent(in) :: which
+    integer(int64), intent(in) :: n
+    integer(int64), allocatable :: l2(:, :)
+    allocate(l2(n, 3))
+    call biocfd_check(biocfd_list_download(g, which, l2, n), 'list_download')
+    call rec_hdr(u, name, 2_int64, 2_int64)
+    write(u) n, 3_int64
+    write(u) l2
+  end subroutine
+
+  subroutine rec_hdr(u, name, kind, nd)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    integer(int64), intent(in) :: kind, nd
+    character(len=24) :: nm
+    nm = name
+    write(u) nm, kind, nd
+  end subroutine
+
+  subroutine rec_r3(u, name, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    real(real64), intent(in) :: a(:, :, :)
+    call rec_hdr(u, name, 1_int64, 3_int64)
+    write(u) int(size(a, 1), int64), int(size(a, 2), int64), int(size(a, 3), int64)
+    write(u) a
+  end subroutine
+
+  subroutine rec_i3(u, name, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    integer(int64), intent(in) :: a(:, :, :)
+    call rec_hdr(u, name, 2_int64, 3_int64)
+    write(u) int(size(a, 1), int64), int(size(a, 2), int64), int(size(a, 3), int64)
+    write(u) a
+  end subroutine
+
+  subroutine rec_r2(u, name, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    real(real64), intent(in) :: a(:, :)
+    call rec_hdr(u, name, 1_int64, 2_int64)
+    write(u) int(size(a, 1), int64), int(size(a, 2), int64)
+    write(u) a
+  end subroutine
+
+  subroutine rec_r1(u, name, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    real(real64), intent(in) :: a(:)
+    call rec_hdr(u, name, 1_int64, 1_int64)
+    write(u) int(size(a, 1), int64)
+    write(u) a
+  end subroutine
+
+  subroutine rec_i1(u, name, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    integer(int64), intent(in) :: a(:)
+    call rec_hdr(u, name, 2_int64, 1_int64)
+    write(u) int(size(a, 1), int64)
+    write(u) a
+  end subroutine
+
+  subroutine rec_r0(u, name, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    real(real64), intent(in) :: a
+    call rec_hdr(u, name, 1_int64, 0_int64)
+    write(u) a
+  end subroutine
+
+  subroutine rec_i0(u, name, a)
+    integer, intent(in) :: u
+    character(len=*), intent(in) :: name
+    integer(int64), intent(in) :: a
+    call rec_hdr(u, name, 2_int64, 0_int64)
+    write(u) a
+  end subroutine
+
+end module biocfd_host
+
+program biocfd_run
+  use, intrinsic :: iso_fortran_env, only: int64, real64
+  use biocfd_hip
+  use biocfd_host
+  implicit none
+  character(len=512) :: case_path, out_dir, arg
+  logical :: do_time
+  integer(int64) :: s, t0, t1, rate
+  integer :: u
+
+  if (command_argument_count() < 2) then
+    write(*, '(a)') 'usage: biocfd_run CASE_FILE OUT_DIR [--time]'
+    error stop 2
+  end if
+  call get_command_argument(1, case_path)
+  call get_command_argument(2, out_dir)
+  do_time = .false.
+  if (command_argument_count() >= 3) then
+    call get_command_argument(3, arg)
+    do_time = trim(arg) == '--time'
+  end if
+  call read_case(trim(case_path))
+  call setup()
+  if (.not. do_time .and. any(dump_steps == 0)) call dump_all(trim(out_dir), 0_int64)
+  call biocfd_check(biocfd_synchronize(), 'synchronize')
+  call system_clock(t0, rate)
+  do s = 1, nsteps
+    call time_step()
+    if (.not. do_time .and. any(dump_steps == s)) call dump_all(trim(out_dir), s)
+  end do
+  call biocfd_check(biocfd_synchronize(), 'synchronize')
+  call system_clock(t1)
+  if (do_time) then
+    open(newunit=u, file=trim(out_dir)//'/timing.txt', status='replace')
+    write(u, '(a,i0,a,es23.15)') 'steps ', nsteps, ' seconds ', real(t1 - t0, real64) / real(rate, real64)
+    close(u)
+  end if
+  call biocfd_check(biocfd_finalize(), 'finalize')
+end program biocfd_run
