@@ -1,4 +1,5 @@
-"""Throughput of the Bio-CFD time step on one MI355X.
+"""Throughput of the Bio-CFD time step on one MI355X, FP64 method and mixed
+precision side by side.
 
 Workload: single block 256x256x256 on [0,8]x[0,4]x[0,4] with a rigid sphere
 of diameter 1 at (2,2,2) carried by 8192 surface triangles, Re = 300,
@@ -7,12 +8,16 @@ dt = 0.0005, inflow u = 1 plus seeded 1e-3 noise, SOR omega = 1.8.  One
 pressure-correction projection, IB retagging / list rebuild / forcing), with
 all fields resident in HBM.
 
-    python bench.py [--steps K] [--warmup W] [--precision fp64|mixed]
+    python bench.py [--steps K] [--warmup W] [--precision both|fp64|mixed]
 
-Prints one JSON line.  `value` is Mcell-steps/s of the whole job (one GPU).
-`roofline` describes the red-black SOR sweep (the dominant kernel) measured
-with HIP events on the runtime's stream; `cpu_baseline` times the reference's
-own CPU build (oracle/_ref/ref_harness) on one step of the same workload.
+Prints one JSON line.  `value` is the FP64 method's Mcell-steps/s (one GPU);
+`mixed` holds the mixed-precision run of the same steps on the same inputs
+(FP32 SOR smoothing, 4 sweeps per FP64 defect correction, the same sweep
+budget), its speed-up and its per-field error against the FP64 run.
+`roofline` describes the FP64 red-black SOR iteration (the dominant kernel)
+measured with HIP events on the runtime's stream; `cpu_baseline` times the
+reference's own CPU build (oracle/_ref/ref_harness) on one step of the same
+workload.
 """
 from __future__ import annotations
 
@@ -33,6 +38,23 @@ HBM_PEAK_GBS = 8000.0
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 ORC_RUN = os.path.join(ROOT, "oracle", "orc_run")
 
+# algorithmic HBM bytes per interior cell of one red-black SOR iteration (FP64):
+# read pc once, read b/dt once, write pc once
+SOR_BYTES_PER_CELL_ITER = 24.0
+# HBM bytes per fused FP64 SOR launch measured with rocprofv3 (FETCH_SIZE x 2.00
+# + WRITE_SIZE x 1.00, calibrated on a 512 MiB copy), keyed by (n, tile_i,
+# tile_j, threads, depth, planes); quoted only for the configuration measured
+SOR_TRAFFIC_MEASURED: dict = {}
+
+
+def mixed_bytes_per_cell(sweeps_per_correction: int) -> float:
+    """Algorithmic bytes per cell of one correction cycle of the mixed solve:
+    k FP32 sweeps (12 B each, the first reads no iterate: 8 B) and one pass
+    that updates pc and takes the residual (pc 8, e 4, b/dt 8 read; pc 8,
+    FP32 r 4 written)."""
+    k = sweeps_per_correction
+    return (12 * k - 4) + 32
+
 
 def make_case(args):
     from biocfd_amd import cases as CS
@@ -42,34 +64,31 @@ def make_case(args):
                                 pc_ita_max=args.pc_ita_max, epsi=args.epsi)
 
 
-# algorithmic HBM bytes per interior cell of one red-black SOR iteration (FP64):
-# read pc once, read b once, write pc once
-SOR_BYTES_PER_CELL_ITER = 24.0
-
-
-def gpu_run(args):
+def run_path(args, precision: str, keep_fields: bool):
     from biocfd_amd import hip as H
+    from biocfd_amd import tune
     from biocfd_amd.driver import Simulation
 
     case = make_case(args)
     cells = case.cells()
-    if args.copy_probe_mb > 0:
+    if args.copy_probe_mb > 0 and precision == "fp64":
         H.call("biocfd_init", 0)
         H.call("biocfd_copy_probe", int(args.copy_probe_mb) << 20, 4)
-    sim = Simulation(case)
-    tuned = None
-    if args.sor_variant == "auto":
-        from biocfd_amd import tune
-
-        v, planes, table = tune.autotune(block=1, dt=case.dt)
-        tuned = {"variant": v, "planes": planes, "candidates": table}
-        print(json.dumps({"sor_autotune": table}), file=sys.stderr)
+    sim = Simulation(case, precision=precision)
+    if precision == "fp64":
+        if args.sor_variant == "auto":
+            v, planes, table = tune.autotune(block=1, dt=case.dt)
+            print(json.dumps({"sor_autotune": table}), file=sys.stderr)
+        else:
+            v, planes = int(args.sor_variant), args.sor_planes
+            H.call("biocfd_sor_config", v, planes)
+        shape = [t for t in tune.variants() if t[0] == v][0]
     else:
-        H.call("biocfd_sor_config", int(args.sor_variant), args.sor_planes)
-        v, planes = int(args.sor_variant), args.sor_planes
-    from biocfd_amd.tune import variants
-
-    shape = [t for t in variants() if t[0] == v][0]
+        v, planes, table = tune.autotune_mixed(block=1, dt=case.dt, sweeps=args.sweeps, residual_tol=args.epsi)
+        print(json.dumps({"mixed_autotune": table}), file=sys.stderr)
+        shape = [t for t in tune.mixed_variants() if t[0] == v][0]
+    kernel = {"variant": v, "tile": [shape[1], shape[2]], "threads": shape[3], "depth": shape[4],
+              "min_wg_per_cu": shape[5], "planes": planes, "autotuned": precision == "mixed" or args.sor_variant == "auto"}
     for _ in range(args.warmup):
         sim.step()
     H.call("biocfd_synchronize")
@@ -88,25 +107,40 @@ def gpu_run(args):
     sor_ci = np.zeros(1, dtype=np.int64)
     H.call("biocfd_timing_read", H.ptr(sor_ms), H.ptr(sor_it), H.ptr(sor_ci))
     H.call("biocfd_timing_enable", 0)
-    sim.close()
-    sec = t1 - t0
-    sweep_bytes = SOR_BYTES_PER_CELL_ITER * float(sor_ci[0])
-    sor_s = float(sor_ms[0]) * 1e-3
-    achieved = sweep_bytes / sor_s / 1e9 if sor_s > 0 else 0.0
-    return {
+    out = {
         "cells": cells,
-        "seconds": sec,
-        "ms_per_step": 1e3 * sec / args.steps,
-        "mcell_steps_per_s": cells * args.steps / sec / 1e6,
+        "seconds": t1 - t0,
+        "ms_per_step": 1e3 * (t1 - t0) / args.steps,
+        "mcell_steps_per_s": cells * args.steps / (t1 - t0) / 1e6,
         "sor_iters_per_step": float(np.mean(iters)) if iters else 0.0,
         "sor_gpu_ms": float(sor_ms[0]),
         "sor_iterations": int(sor_it[0]),
-        "sor_us_per_iteration": 1e3 * float(sor_ms[0]) / max(1, int(sor_it[0])),
-        "sor_achieved_gbs": achieved,
-        "sor_bytes_per_iteration": SOR_BYTES_PER_CELL_ITER * cells,
-        "sor_kernel": {"variant": v, "tile": [shape[1], shape[2]], "threads": shape[3], "depth": shape[4],
-                       "planes": planes, "autotuned": tuned is not None},
+        "sor_cell_iterations": int(sor_ci[0]),
+        "kernel": kernel,
+        "residual_linf": sim.residual_norm(1),
     }
+    if keep_fields:
+        out["fields"] = {name: sim.field(1, name) for name in ("u", "v", "w", "p")}
+        res = np.zeros(4)
+        kv = sim.blocks[0].kin_vec()
+        H.call("biocfd_stress_calculation", 1, case.mu_f, case.rho_f, H.ptr(kv), H.ptr(res))
+        out["forces"] = res.copy()
+    sim.close()
+    return out
+
+
+def field_errors(ref, got):
+    """max and L2 relative differences per field, and of the four body-force
+    coefficients (viscous/pressure drag and lift)."""
+    err = {}
+    for k, a in ref["fields"].items():
+        b = got["fields"][k]
+        d = b - a
+        err[k] = {"max_rel": float(np.max(np.abs(d)) / max(np.max(np.abs(a)), 1e-300)),
+                  "l2_rel": float(np.linalg.norm(d) / max(np.linalg.norm(a), 1e-300))}
+    fa, fb = ref["forces"], got["forces"]
+    err["body_force"] = {"max_rel": float(np.max(np.abs(fb - fa)) / max(np.max(np.abs(fa)), 1e-300))}
+    return err
 
 
 def cpu_baseline(args):
@@ -154,10 +188,12 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--markers", type=int, default=8192)
     ap.add_argument("--pc-ita-max", type=int, default=500)
-    ap.add_argument("--epsi", type=float, default=1e-8)
-    ap.add_argument("--precision", choices=["fp64"], default="fp64")
+    ap.add_argument("--epsi", type=float, default=1e-8,
+                    help="SOR tolerance: update norm (FP64 method) / residual max-norm (mixed)")
+    ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--sweeps", type=int, default=4, help="FP32 sweeps per defect correction (mixed)")
     ap.add_argument("--sor-variant", default="auto",
-                    help="SOR kernel variant (0 = colour-split sweeps, >=1 fused tiles) or 'auto'")
+                    help="FP64 SOR kernel variant (0 = colour-split sweeps, >=1 fused tiles) or 'auto'")
     ap.add_argument("--sor-planes", type=int, default=32, help="planes per workgroup of a fused variant")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -170,38 +206,70 @@ def main():
               "path is a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
 
-    g = gpu_run(args)
+    both = args.precision == "both"
+    g = run_path(args, "fp64", keep_fields=both) if args.precision in ("both", "fp64") else None
+    m = run_path(args, "mixed", keep_fields=both) if args.precision in ("both", "mixed") else None
+    main_run = g if g is not None else m
     out = {
-        "metric": "Mcell-steps/s at 256^3 (FP64 method, reference SOR rule epsi=1e-8, pcItaMax=500)",
-        "value": round(g["mcell_steps_per_s"], 3),
+        "metric": "Mcell-steps/s at 256^3 FP64 vs mixed-precision at 1e-8 residual; HBM %peak",
+        "value": round(main_run["mcell_steps_per_s"], 3),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(g["ms_per_step"], 3),
+        "ms_per_step": round(main_run["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if g is not None else "f32+f64",
         "data": "synthetic (seeded 1e-3 noise on u=1 inflow; Fibonacci-hull sphere)",
         "config": {"workload": f"sphere{args.n}: single block {args.n}^3 on [0,8]x[0,4]x[0,4], sphere d=1, "
-                               f"{args.markers} markers, Re=300, dt=5e-4, omega=1.8",
-                   "sor_iterations_per_step": g["sor_iters_per_step"], "sor_kernel": g["sor_kernel"]},
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(g["sor_achieved_gbs"], 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(g["sor_achieved_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": ("red-black SOR iteration (one fused red+black launch, norm reduced in-kernel)"
-                       if g["sor_kernel"]["variant"] > 0 else
-                       "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
-                      + ", 24 B/cell/iteration algorithmic",
-            "us_per_iteration": round(g["sor_us_per_iteration"], 2),
-        },
+                               f"{args.markers} markers, Re=300, dt=5e-4, omega=1.8, pcItaMax={args.pc_ita_max}, "
+                               f"epsi={args.epsi}",
+                   "value_is": "FP64 method" if g is not None else "mixed precision",
+                   "sor_iterations_per_step": main_run["sor_iters_per_step"],
+                   "sor_kernel": main_run["kernel"]},
+        "roofline": None,
         "cpu_baseline": None,
     }
+    if g is not None:
+        achieved = SOR_BYTES_PER_CELL_ITER * g["sor_cell_iterations"] / (g["sor_gpu_ms"] * 1e-3) / 1e9
+        kk = g["kernel"]
+        traffic = SOR_TRAFFIC_MEASURED.get((args.n, kk["tile"][0], kk["tile"][1], kk["threads"], kk["depth"],
+                                            kk["planes"]))
+        out["roofline"] = {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "kernel": ("red-black SOR iteration (k_sor_fused<double>: one fused red+black launch per iteration)"
+                       if kk["variant"] > 0 else
+                       "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
+                      + ", 24 B/cell/iteration algorithmic",
+            "us_per_iteration": round(1e3 * g["sor_gpu_ms"] / max(1, g["sor_iterations"]), 2),
+        }
+        out["fp64_residual_linf"] = g["residual_linf"]
+    if m is not None:
+        kb = mixed_bytes_per_cell(args.sweeps)
+        solver_s = m["sor_gpu_ms"] * 1e-3
+        mixed = {
+            "value": round(m["mcell_steps_per_s"], 3),
+            "ms_per_step": round(m["ms_per_step"], 3),
+            "sweeps_per_step": m["sor_iters_per_step"],
+            "sweeps_per_correction": args.sweeps,
+            "residual_linf": m["residual_linf"],
+            "solver_us_per_sweep": round(1e3 * m["sor_gpu_ms"] / max(1, m["sor_iterations"]), 2),
+            "solver_achieved_gbs": round(kb / args.sweeps * m["sor_cell_iterations"] / solver_s / 1e9, 1)
+            if solver_s > 0 else None,
+            "solver_bytes_per_cell_sweep": kb / args.sweeps,
+            "sor_kernel": m["kernel"],
+        }
+        if g is not None:
+            mixed["speedup_vs_fp64"] = round(m["mcell_steps_per_s"] / g["mcell_steps_per_s"], 3)
+            mixed["rel_err_vs_fp64"] = field_errors(g, m)
+        out["mixed"] = mixed
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args)

@@ -127,6 +127,10 @@ struct Block {
   int64_t split_n = 0;
   double *pc2 = nullptr;              // second pc array of the fused SOR iterations
   int64_t pc2_n = 0;
+  // mixed-precision solve: FP32 correction pair (full layout) and FP32
+  // residual halves (colour-split)
+  float *e32[2] = {nullptr, nullptr}, *r32[2] = {nullptr, nullptr};
+  int64_t mx_n = 0;
   double *partials = nullptr;         // per-workgroup maxima of the reductions
   int64_t cap_partials = 0;
 };
@@ -186,5 +190,7 @@ int poisson_block_setup(Block &b);
 int ib_list_capacity(Block &b, int64_t n_ib);
 int build_cell_lists(Block &b, bool ib_lists);
 int build_ts(Block &b);
+int precision_mode();
+int mixed_sor(Block &b, int64_t pc_ita_max, double dt);
 
 }  // namespace bcfd

@@ -1,2 +1,383 @@
-// mixed precision path (later)
+// Mixed-precision pressure solve: the FP32-smoothing variant of
+// REDBLACKSOR_linear (PcorVcor.F90:288-377) as FP64 defect correction.
+//
+//   r  = b/dt - A pc          FP64 arithmetic, stored as FP32 (colour-split),
+//                             with its FP64 max-norm
+//   e  = S^k(0; r)            k FP32 red-black SOR sweeps from e = 0 (the
+//                             fused tile kernel of sor_fused.hpp, Real = float)
+//   pc = pc + e               FP64
+//
+// until ||r||inf < tol or the sweep budget of the call (pcItaMax + 1 -
+// nIterPcor, the FP64 method's iteration budget) is spent.  k sweeps of a
+// stationary iteration from a zero guess on the residual equation are k
+// steps of the same iteration on pc, so the iterates follow the FP64 SOR's up
+// to the FP32 rounding of each correction; the residual is always recomputed
+// from the FP64 pc, so those roundings do not accumulate.
+//
+// Traffic per cycle of k sweeps at n cells: k sweeps 4+4 read (the first
+// reads no iterate), 4 written; update + residual (one pass) pc 8, e 4, b/dt 8
+// read, pc 8, r 4 written: (32 + 12k - 4) n bytes.  The FP32 iterate pair and residual
+// (3 x 4 B per cell, 201 MB at 256^3) fit in MI355X's 256 MB Infinity Cache.
 #include "biocfd_rt.hpp"
+#include "reduce.hpp"
+#include "sor_kernel.hpp"
+
+namespace bcfd {
+
+constexpr int kResPlanes = 16;  // z-planes a residual thread marches
+
+// With kUpdate: pc' = pc + e (every interior cell; e is zero off the fluid)
+// is written to pc_out and the residual is taken of pc'; without, of pc.
+// r = b/dt - A pc' at interior fluid cells in FP64, stored as FP32 into the
+// colour-split arrays (marker at the other interior cells), with the
+// per-workgroup max |r|.  A thread owns one (i, j) column and marches
+// kResPlanes planes keeping the k-1, k, k+1 values in registers; its i and j
+// neighbours are its neighbours' own loads, served by L1/L2.
+template <bool kUpdate>
+__global__ void __launch_bounds__(256) k_mx_resupd(Dims d, int NXh, const double *__restrict__ pc,
+                                                   const float *__restrict__ e, double *__restrict__ pc_out,
+                                                   const double *__restrict__ bred, const double *__restrict__ bblack,
+                                                   const double *__restrict__ Ax, const double *__restrict__ Ay,
+                                                   const double *__restrict__ Az, float *__restrict__ rred,
+                                                   float *__restrict__ rblack, const SorState *__restrict__ st,
+                                                   double *__restrict__ partials) {
+  if (st->done) return;
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k0 = 2 + blockIdx.z * kResPlanes;
+  const int k1 = k0 + kResPlanes - 1 < d.nz + 1 ? k0 + kResPlanes - 1 : d.nz + 1;
+  double m = 0.0;
+  if (i <= d.nx + 1 && j <= d.ny + 1) {
+    const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
+    auto val = [&](int64_t c) -> double { return kUpdate ? pc[c] + (double)e[c] : pc[c]; };
+    const Coef3 ax = coef_at(Ax, d.nx, i), ay = coef_at(Ay, d.ny, j);
+    int64_t c = ix3(d, i, j, k0);
+    double vm = val(c - sz), v0 = val(c);
+    for (int k = k0; k <= k1; ++k, c += sz) {
+      const double vp = val(c + sz);
+      const bool red = ((i + j + k) & 1) != 0;
+      const int64_t h = sidx(NXh, d.NY, (i - 1) >> 1, j, k);
+      const double rhs = red ? bred[h] : bblack[h];
+      if (kUpdate) pc_out[c] = v0;
+      float rv = not_fluid_marker<float>();
+      if (!sor_skip(rhs)) {
+        const Coef3 az = coef_at(Az, d.nz, k);
+        const double r = (rhs - ax.m * val(c - 1) - ax.p * val(c + 1) - ay.m * val(c - sy) - ay.p * val(c + sy) -
+                          az.m * vm - az.p * vp) -
+                         (ax.c + ay.c + az.c) * v0;
+        rv = (float)r;
+        const double ar = fabs(r);
+        m = m > ar ? m : ar;
+      }
+      (red ? rred : rblack)[h] = rv;
+      vm = v0;
+      v0 = vp;
+    }
+  }
+  wg_max_store(m, partials);
+}
+
+// the FP32 marker in every entry of a colour-split residual array (the
+// entries of non-interior cells are never written afterwards)
+__global__ void k_mx_fill_marker(float *__restrict__ r, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) r[q] = not_fluid_marker<float>();
+}
+
+// after an odd number of updates the iterate is in pc2
+__global__ void k_mx_finish(const SorState *__restrict__ st, double *__restrict__ pc, const double *__restrict__ pc2,
+                            int64_t n) {
+  if ((st->call_iters & 1) == 0) return;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+    pc[q] = pc2[q];
+}
+
+// call set-up of the mixed solve (reset_iter: also nIterPcor and derr2)
+__global__ void k_mx_reset(SorState *st, int reset_iter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (reset_iter) { st->n_iter = 0; st->derr2 = 0.0; st->derr_std_st = 0.0; }
+    st->derr4 = 11111.0;
+    st->done = 0;
+    st->call_iters = 0;
+  }
+}
+
+// end of a cycle: ||r||inf, nIterPcor = start + sweeps, updates so far, stop test
+__global__ void __launch_bounds__(1024) k_mx_end(SorState *st, const double *partials, int64_t n, double tol,
+                                                 long long start, long long sweeps, long long budget,
+                                                 long long updates) {
+  if (st->done) return;
+  __shared__ double s_r;
+  block_max_partials(partials, n, &s_r);
+  if (threadIdx.x != 0) return;
+  st->derr4 = s_r;
+  st->derr2 = s_r;
+  st->n_iter = start + sweeps;
+  st->call_iters = updates;
+  if (s_r < tol || sweeps >= budget) st->done = 1;
+}
+
+// ---------------------------------------------------------------- host
+
+struct Sweep32Variant {
+  int ti, tj, nt, depth, minb;
+  void (*launch)(dim3, dim3, hipStream_t, const FusedArgsT<float> &, int, int, const SorState *, double *);
+};
+
+template <int TI, int TJ, int NT, int D, int MINB>
+static void launch_sweep32(dim3 g, dim3 blk, hipStream_t s, const FusedArgsT<float> &a, int gx, int gy,
+                           const SorState *st, double *partials) {
+  hipLaunchKernelGGL((k_sor_fused<float, TI, TJ, NT, D, MINB>), g, blk, 0, s, a, gx, gy, 0, st, partials);
+}
+
+// {tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers must allow}
+static const Sweep32Variant kSweep32[] = {
+    {64, 16, 256, 1, 1, launch_sweep32<64, 16, 256, 1, 1>},
+    {64, 32, 512, 1, 1, launch_sweep32<64, 32, 512, 1, 1>},
+    {64, 32, 512, 2, 1, launch_sweep32<64, 32, 512, 2, 1>},
+    {128, 16, 512, 1, 1, launch_sweep32<128, 16, 512, 1, 1>},
+    {128, 16, 512, 2, 1, launch_sweep32<128, 16, 512, 2, 1>},
+    {128, 32, 512, 1, 1, launch_sweep32<128, 32, 512, 1, 1>},
+    {64, 32, 512, 1, 3, launch_sweep32<64, 32, 512, 1, 3>},
+    {64, 32, 512, 2, 2, launch_sweep32<64, 32, 512, 2, 2>},
+    {64, 16, 256, 1, 6, launch_sweep32<64, 16, 256, 1, 6>},
+    {64, 16, 256, 2, 4, launch_sweep32<64, 16, 256, 2, 4>},
+    {128, 16, 512, 1, 2, launch_sweep32<128, 16, 512, 1, 2>},
+};
+constexpr int kNumSweep32 = (int)(sizeof(kSweep32) / sizeof(kSweep32[0]));
+
+struct MixedConfig {
+  int precision = 0;        // 0: FP64 method, 1: mixed
+  int variant = 1;
+  int planes = 32;
+  int sweeps = 4;           // FP32 sweeps per correction
+  double tol = 1e-8;        // max-norm of the FP64 residual
+};
+static MixedConfig g_mx;
+static SorState *g_mx_probe = nullptr;
+
+int precision_mode() { return g_mx.precision; }
+
+int64_t residual_workgroups(const Block &b);
+int ensure_split(Block &b);
+
+static int ensure_mixed(Block &b) {
+  const int NXh = (b.d.NX + 1) / 2;
+  const int64_t nh = (int64_t)NXh * b.d.NY * b.d.NZ;
+  if (b.mx_n != b.n2) {
+    for (int q = 0; q < 2; ++q) {
+      if (dev_alloc(&b.e32[q], b.n2) || dev_alloc(&b.r32[q], nh)) return -1;
+      // ghost cells of the corrections and residuals are never written: set once
+      BCFD_HIP(hipMemsetAsync(b.e32[q], 0, sizeof(float) * (size_t)b.n2, rt().stream));
+      hipLaunchKernelGGL(k_mx_fill_marker, dim3(div_up(nh, 256)), dim3(256), 0, rt().stream, b.r32[q], nh);
+      BCFD_CHECK_LAUNCH();
+    }
+    b.mx_n = b.n2;
+  }
+  if (b.pc2_n != b.n2) {
+    if (dev_alloc(&b.pc2, b.n2)) return -1;
+    b.pc2_n = b.n2;
+  }
+  if (ensure_split(b)) return -1;
+  if (grow(&b.partials, &b.cap_partials, residual_workgroups(b))) return -1;
+  return 0;
+}
+
+int64_t residual_workgroups(const Block &b) {
+  return (int64_t)div_up(b.d.nx, 64) * div_up(b.d.ny, 4) * div_up(b.d.nz, kResPlanes);
+}
+
+// b/dt of the fluid cells into the colour-split FP64 arrays (marker elsewhere)
+int split_bdt(Block &b, double dt);
+
+// residual of pc (src, with e added when e != nullptr, written to dst)
+static int64_t residual_launch(Block &b, const double *src, const float *e, double *dst, const SorState *st) {
+  const int NXh = (b.d.NX + 1) / 2;
+  dim3 blk(64, 4, 1), grd(div_up(b.d.nx, 64), div_up(b.d.ny, 4), div_up(b.d.nz, kResPlanes));
+  if (e)
+    hipLaunchKernelGGL(k_mx_resupd<true>, grd, blk, 0, rt().stream, b.d, NXh, src, e, dst, b.bs[1], b.bs[0], b.A[0],
+                       b.A[1], b.A[2], b.r32[1], b.r32[0], st, b.partials);
+  else
+    hipLaunchKernelGGL(k_mx_resupd<false>, grd, blk, 0, rt().stream, b.d, NXh, src, e, dst, b.bs[1], b.bs[0],
+                       b.A[0], b.A[1], b.A[2], b.r32[1], b.r32[0], st, b.partials);
+  return (int64_t)grd.x * grd.y * grd.z;
+}
+
+// k sweeps from a zero correction; returns the array holding the result
+static float *sweeps_launch(Block &b, int variant, int planes, int k, const SorState *st) {
+  const Sweep32Variant &v = kSweep32[variant];
+  const int gx = (int)div_up(b.d.nx, v.ti), gy = (int)div_up(b.d.ny, v.tj), gz = (int)div_up(b.d.nz, planes);
+  FusedArgsT<float> a;
+  a.d = b.d;
+  a.NXh = (b.d.NX + 1) / 2;
+  a.KS = planes;
+  a.bred = b.r32[1];
+  a.bblack = b.r32[0];
+  a.Ax = b.A[0];
+  a.Ay = b.A[1];
+  a.Az = b.A[2];
+  a.omega = b.omega;
+  float *out = b.e32[0];
+  for (int q = 0; q < k; ++q) {
+    a.zero_src = q == 0;
+    a.src = (q & 1) ? b.e32[0] : b.e32[1];
+    a.dst = (q & 1) ? b.e32[1] : b.e32[0];
+    out = a.dst;
+    v.launch(dim3((unsigned)(gx * gy * gz)), dim3(v.nt), rt().stream, a, gx, gy, st, b.partials);
+  }
+  return out;
+}
+
+int sor_timing_on();
+int sor_timing_begin(hipEvent_t *e0);
+int sor_timing_end(hipEvent_t e0, const Block &b, long long start);
+
+int mixed_sor(Block &b, int64_t pc_ita_max, double dt) {
+  hipStream_t s = rt().stream;
+  if (ensure_mixed(b) || split_bdt(b, dt)) return -1;
+  BCFD_HIP(hipMemcpyAsync(b.pc2, b.f[BF_PC], sizeof(double) * (size_t)b.n2, hipMemcpyDeviceToDevice, s));
+  hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(1), 0, s, b.sor, 0);
+  BCFD_CHECK_LAUNCH();
+  SorState *h = (SorState *)rt().h_pinned;
+  BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  const long long start = h->n_iter;
+  long long budget = pc_ita_max + 1 - start;
+  if (budget < 1) budget = 1;
+  hipEvent_t e0 = nullptr;
+  if (sor_timing_begin(&e0)) return -1;
+  const int k = g_mx.sweeps;
+  const long long cycles = (budget + k - 1) / k;
+  int64_t nwg = residual_launch(b, b.f[BF_PC], nullptr, nullptr, b.sor);
+  hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, nwg, g_mx.tol, start, 0ll, budget, 0ll);
+  BCFD_CHECK_LAUNCH();
+  const long long chunk = (48 + k - 1) / k;  // host polls about every 48 sweeps
+  for (long long c = 0; c < cycles; ++c) {
+    // the last correction takes only the sweeps left in the budget
+    const int kc = (int)((budget - c * k) < k ? (budget - c * k) : k);
+    const float *e = sweeps_launch(b, g_mx.variant, g_mx.planes, kc, b.sor);
+    // update c reads the iterate from pc (c even) or pc2 and writes the other
+    const double *src = (c & 1) ? b.pc2 : b.f[BF_PC];
+    double *dst = (c & 1) ? b.f[BF_PC] : b.pc2;
+    nwg = residual_launch(b, src, e, dst, b.sor);
+    const long long done_sweeps = c * k + kc;
+    hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, nwg, g_mx.tol, start, done_sweeps,
+                       budget, c + 1);
+    BCFD_CHECK_LAUNCH();
+    if ((c + 1) % chunk == 0 && c + 1 < cycles) {
+      BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+      BCFD_HIP(hipStreamSynchronize(s));
+      if (h->done) break;
+    }
+  }
+  if (sor_timing_end(e0, b, start)) return -1;
+  hipLaunchKernelGGL(k_mx_finish, dim3(1024), dim3(256), 0, s, b.sor, b.f[BF_PC], b.pc2, b.n2);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+// `iters` FP32 sweeps of a variant on the residual of the current pc, timed
+int mixed_time(Block &b, int variant, int planes, int iters, double dt, double *ms) {
+  hipStream_t s = rt().stream;
+  if (ensure_mixed(b) || split_bdt(b, dt)) return -1;
+  if (!g_mx_probe && dev_alloc(&g_mx_probe, 1)) return -1;
+  hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(1), 0, s, g_mx_probe, 1);
+  residual_launch(b, b.f[BF_PC], nullptr, nullptr, g_mx_probe);
+  hipEvent_t e0, e1;
+  BCFD_HIP(hipEventCreate(&e0));
+  BCFD_HIP(hipEventCreate(&e1));
+  BCFD_HIP(hipEventRecord(e0, s));
+  sweeps_launch(b, variant, planes, iters, g_mx_probe);
+  BCFD_HIP(hipEventRecord(e1, s));
+  BCFD_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  BCFD_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  BCFD_CHECK_LAUNCH();
+  *ms = t;
+  return 0;
+}
+
+// max-norm of b/dt - A pc over the fluid cells (diagnostic, FP64)
+int residual_norm(Block &b, double dt, double *out) {
+  hipStream_t s = rt().stream;
+  if (ensure_mixed(b) || split_bdt(b, dt)) return -1;
+  if (!g_mx_probe && dev_alloc(&g_mx_probe, 1)) return -1;
+  hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(1), 0, s, g_mx_probe, 1);
+  const int64_t nwg = residual_launch(b, b.f[BF_PC], nullptr, nullptr, g_mx_probe);
+  hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, g_mx_probe, b.partials, nwg, -1.0, 0ll, 0ll, 1ll, 0ll);
+  BCFD_CHECK_LAUNCH();
+  SorState *h = (SorState *)rt().h_pinned;
+  BCFD_HIP(hipMemcpyAsync(h, g_mx_probe, sizeof(SorState), hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  *out = h->derr4;
+  return 0;
+}
+
+}  // namespace bcfd
+
+using namespace bcfd;
+
+extern "C" {
+
+int biocfd_set_precision(int mode) {
+  if (mode != 0 && mode != 1) {
+    set_error("set_precision: mode %d is neither 0 (FP64) nor 1 (mixed)", mode);
+    return -1;
+  }
+  g_mx.precision = mode;
+  return 0;
+}
+
+int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_correction, double residual_tol) {
+  if (variant < 0 || variant >= kNumSweep32) {
+    set_error("mixed_config: variant %d outside 0..%d", variant, kNumSweep32 - 1);
+    return -1;
+  }
+  if (planes_per_workgroup < 1 || planes_per_workgroup > 4096 || sweeps_per_correction < 1 ||
+      sweeps_per_correction > 64) {
+    set_error("mixed_config: planes %d / sweeps %d out of range", planes_per_workgroup, sweeps_per_correction);
+    return -1;
+  }
+  g_mx.variant = variant;
+  g_mx.planes = planes_per_workgroup;
+  g_mx.sweeps = sweeps_per_correction;
+  g_mx.tol = residual_tol;
+  return 0;
+}
+
+int biocfd_mixed_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth,
+                               int *min_wg_per_cu) {
+  if (variant < 0 || variant >= kNumSweep32) {
+    set_error("mixed_variant_shape: variant %d outside 0..%d", variant, kNumSweep32 - 1);
+    return -1;
+  }
+  *tile_i = kSweep32[variant].ti;
+  *tile_j = kSweep32[variant].tj;
+  *threads = kSweep32[variant].nt;
+  *depth = kSweep32[variant].depth;
+  *min_wg_per_cu = kSweep32[variant].minb;
+  return 0;
+}
+
+int biocfd_mixed_time(int id, int variant, int planes_per_workgroup, int iters, double dt, double *ms) {
+  Block *b = get_block(id);
+  if (!b) return -1;
+  if (!b->A[0]) { set_error("coefficientMatrix has not been called"); return -1; }
+  if (variant < 0 || variant >= kNumSweep32 || planes_per_workgroup < 1 || planes_per_workgroup > 4096 ||
+      iters < 1) {
+    set_error("mixed_time: bad variant %d / planes %d / iterations %d", variant, planes_per_workgroup, iters);
+    return -1;
+  }
+  return mixed_time(*b, variant, planes_per_workgroup, iters, dt, ms);
+}
+
+int biocfd_residual_norm(int id, double dt, double *out) {
+  Block *b = get_block(id);
+  if (!b) return -1;
+  if (!b->A[0]) { set_error("coefficientMatrix has not been called"); return -1; }
+  return residual_norm(*b, dt, out);
+}
+
+}  // extern "C"

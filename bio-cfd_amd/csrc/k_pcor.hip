@@ -15,20 +15,14 @@
 // colour by colour is the same computation; |pc - pco| of an iteration is
 // the per-cell change, reduced with one atomic per wavefront.
 #include "biocfd_rt.hpp"
-#include "sor_fused.hpp"
+#include "reduce.hpp"
+#include "sor_kernel.hpp"
 
 namespace bcfd {
 
 int interface_coarse_update(int which);
 int interface_fine_update(int which);
 
-__device__ inline double wave_max(double v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    double w = __shfl_xor(v, o, 64);
-    v = v > w ? v : w;
-  }
-  return v;
-}
 
 __global__ void k_zero_pcor(double *__restrict__ b, double *__restrict__ pc, int64_t n) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -88,20 +82,6 @@ __global__ void k_merge(Dims d, int NXh, double *__restrict__ full, const double
   full[ix3(d, i, j, k)] = src[sidx(NXh, d.NY, (i - 1) >> 1, j, k)];
 }
 
-// workgroup max of a non-negative value -> partials[workgroup]
-__device__ inline void wg_max_store(double v, double *partials) {
-  __shared__ double s_wave[16];
-  const int t = threadIdx.x + threadIdx.y * blockDim.x;
-  const int nw = (blockDim.x * blockDim.y + 63) / 64;
-  v = wave_max(v);
-  if ((t & 63) == 0) s_wave[t >> 6] = v;
-  __syncthreads();
-  if (t == 0) {
-    double m = 0.0;
-    for (int q = 0; q < nw; ++q) m = m > s_wave[q] ? m : s_wave[q];
-    partials[blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z)] = m;
-  }
-}
 
 // b/dt of the fluid cells in the split layout; non-fluid cells carry a NaN
 // with a private payload so the sweeps need no tag array (b/dt is the same
@@ -158,47 +138,6 @@ __global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, 
       own[c] = nw;
       const double dd = fabs(nw - old);
       delta = delta > dd ? delta : dd;
-    }
-  }
-  if (check) wg_max_store(delta, partials);
-}
-
-// One red-black iteration per launch on a 1-D grid of gx*gy*gz tiles; the
-// tile schedule is described with SorTile.  Non-fluid cells of the tile are
-// copied through, so src and dst agree on every cell the iteration leaves.
-// On check iterations (every fifth, numbered by the host) each workgroup
-// posts its largest update to partials.
-template <int TI, int TJ, int NT, int D>
-__global__ void __launch_bounds__(NT) k_sor_fused(FusedArgs a, int gx, int gy, int check,
-                                                  const SorState *__restrict__ st, double *__restrict__ partials) {
-  using T = SorTile<TI, TJ, NT, D>;
-  if (st->done) return;
-  __shared__ double lds[T::LDS_DOUBLES];
-  const int tid = threadIdx.x;
-  const typename T::Tile tile = T::tile_of(a, T::xcd_tile(blockIdx.x, gridDim.x), gx, gy);
-  typename T::Thread th;
-  T::init_thread(a, tile, tid, th);
-  T::load_coef(a, tile, lds, tid);
-  T::prologue(a, tile, lds, th);
-  double delta = 0.0;
-  for (int s = tile.k0 - 1; s <= tile.k1 + 1; s += D) {
-    __syncthreads();
-    T::phase_a(a, tile, lds, th, s, th.pcn[0]);
-    __syncthreads();
-    const double dr = T::phase_b(a, lds, th, s);
-    delta = delta > dr ? delta : dr;
-    __syncthreads();
-    const double db = T::phase_c(a, tile, lds, th, s);
-    delta = delta > db ? delta : db;
-    if (D > 1 && s + 1 <= tile.k1 + 1) {
-      __syncthreads();
-      T::phase_a(a, tile, lds, th, s + 1, th.pcn[D - 1]);
-      __syncthreads();
-      const double dr2 = T::phase_b(a, lds, th, s + 1);
-      delta = delta > dr2 ? delta : dr2;
-      __syncthreads();
-      const double db2 = T::phase_c(a, tile, lds, th, s + 1);
-      delta = delta > db2 ? delta : db2;
     }
   }
   if (check) wg_max_store(delta, partials);
@@ -428,7 +367,7 @@ struct SorTiming {
 };
 static SorTiming g_tm;
 
-static int ensure_split(Block &b) {
+int ensure_split(Block &b) {
   const int NXh = (b.d.NX + 1) / 2;
   const int64_t n = (int64_t)NXh * b.d.NY * b.d.NZ;
   if (b.split_n != n) {
@@ -447,30 +386,37 @@ static int ensure_split(Block &b) {
 // (plus one for the norm), 1.. = one fused launch per iteration with the tile
 // shapes and prefetch depths below
 struct FusedVariant {
-  int ti, tj, nt, depth;
+  int ti, tj, nt, depth, minb;
   void (*launch)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, int, const SorState *, double *);
 };
 
-template <int TI, int TJ, int NT, int D>
+template <int TI, int TJ, int NT, int D, int MINB>
 static void launch_fused(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, int check,
                          const SorState *st, double *partials) {
-  hipLaunchKernelGGL((k_sor_fused<TI, TJ, NT, D>), g, blk, 0, s, a, gx, gy, check, st, partials);
+  hipLaunchKernelGGL((k_sor_fused<double, TI, TJ, NT, D, MINB>), g, blk, 0, s, a, gx, gy, check, st, partials);
 }
 
+// {tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers must allow}
 static const FusedVariant kFused[] = {
-    {0, 0, 0, 0, nullptr},
-    {64, 16, 256, 1, launch_fused<64, 16, 256, 1>},
-    {64, 8, 256, 1, launch_fused<64, 8, 256, 1>},
-    {64, 8, 256, 2, launch_fused<64, 8, 256, 2>},
-    {128, 16, 512, 1, launch_fused<128, 16, 512, 1>},
-    {128, 16, 512, 2, launch_fused<128, 16, 512, 2>},
-    {64, 32, 512, 1, launch_fused<64, 32, 512, 1>},
-    {64, 32, 512, 2, launch_fused<64, 32, 512, 2>},
-    {32, 16, 256, 2, launch_fused<32, 16, 256, 2>},
-    {128, 8, 256, 2, launch_fused<128, 8, 256, 2>},
+    {0, 0, 0, 0, 0, nullptr},
+    {64, 16, 256, 1, 1, launch_fused<64, 16, 256, 1, 1>},
+    {64, 8, 256, 1, 1, launch_fused<64, 8, 256, 1, 1>},
+    {64, 8, 256, 2, 1, launch_fused<64, 8, 256, 2, 1>},
+    {128, 16, 512, 1, 1, launch_fused<128, 16, 512, 1, 1>},
+    {128, 16, 512, 2, 1, launch_fused<128, 16, 512, 2, 1>},
+    {64, 32, 512, 1, 1, launch_fused<64, 32, 512, 1, 1>},
+    {64, 32, 512, 2, 1, launch_fused<64, 32, 512, 2, 1>},
+    {32, 16, 256, 2, 1, launch_fused<32, 16, 256, 2, 1>},
+    {128, 8, 256, 2, 1, launch_fused<128, 8, 256, 2, 1>},
+    {64, 16, 256, 1, 3, launch_fused<64, 16, 256, 1, 3>},
+    {64, 16, 256, 2, 3, launch_fused<64, 16, 256, 2, 3>},
+    {64, 8, 256, 1, 4, launch_fused<64, 8, 256, 1, 4>},
+    {64, 8, 256, 2, 4, launch_fused<64, 8, 256, 2, 4>},
+    {32, 16, 256, 1, 4, launch_fused<32, 16, 256, 1, 4>},
+    {128, 8, 256, 1, 2, launch_fused<128, 8, 256, 1, 2>},
 };
 constexpr int kNumSorVariants = (int)(sizeof(kFused) / sizeof(kFused[0]));
-static int g_sor_variant = 6;
+static int g_sor_variant = 7;
 static int g_sor_planes = 32;
 
 static int enqueue_split_iterations(Block &b, SorState *st, long long n, int64_t pc_ita_max, double epsi) {
@@ -511,6 +457,7 @@ static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int pla
   a.d = b.d;
   a.NXh = (b.d.NX + 1) / 2;
   a.KS = planes;
+  a.zero_src = 0;
   a.bred = b.bs[1];
   a.bblack = b.bs[0];
   a.Ax = b.A[0];
@@ -527,6 +474,16 @@ static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int pla
       hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, it, start, check, (long long)pc_ita_max,
                          epsi, b.partials, nwg);
   }
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+// b/dt of the fluid cells into the colour-split FP64 arrays (marker elsewhere)
+int split_bdt(Block &b, double dt) {
+  if (ensure_split(b)) return -1;
+  const int NXh = (b.d.NX + 1) / 2;
+  dim3 fb(64, 4, 1), fg(div_up(b.d.NX, 64), div_up(b.d.NY, 4), (unsigned)b.d.NZ);
+  hipLaunchKernelGGL(k_split_bdt, fg, fb, 0, rt().stream, b.d, NXh, b.f[BF_B], b.tag[BT_CELL], dt, b.bs[1], b.bs[0]);
   BCFD_CHECK_LAUNCH();
   return 0;
 }
@@ -574,9 +531,37 @@ static int enqueue_iterations(Block &b, SorState *st, int variant, int planes, l
                      : enqueue_split_iterations(b, st, n, pc_ita_max, epsi);
 }
 
-int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
+int sor_timing_on() { return g_tm.on ? 1 : 0; }
+
+// HIP-event timing of one SOR call (when enabled): begin records e0
+int sor_timing_begin(hipEvent_t *e0) {
+  *e0 = nullptr;
+  if (!g_tm.on) return 0;
+  BCFD_HIP(hipEventCreate(e0));
+  BCFD_HIP(hipEventRecord(*e0, rt().stream));
+  return 0;
+}
+
+// end records e1 and the iterations the call ran (nIterPcor - start)
+int sor_timing_end(hipEvent_t e0, const Block &b, long long start) {
+  if (!g_tm.on) return 0;
   hipStream_t s = rt().stream;
-  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEvent_t e1 = nullptr;
+  BCFD_HIP(hipEventCreate(&e1));
+  BCFD_HIP(hipEventRecord(e1, s));
+  SorState *h = (SorState *)rt().h_pinned;
+  BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  g_tm.ev.emplace_back(e0, e1);
+  g_tm.iters += h->n_iter - start;
+  g_tm.cells += (h->n_iter - start) * (int64_t)b.d.nx * b.d.ny * b.d.nz;
+  return 0;
+}
+
+int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
+  if (precision_mode() == 1) return mixed_sor(b, pc_ita_max, dt);
+  hipStream_t s = rt().stream;
+  hipEvent_t e0 = nullptr;
   const int variant = g_sor_variant, planes = g_sor_planes;
   if (sor_prepare(b, variant, planes, dt)) return -1;
   hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
@@ -586,11 +571,7 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   BCFD_HIP(hipStreamSynchronize(s));
   long long start = h->n_iter;
   long long remaining = pc_ita_max + 1 - start;
-  if (g_tm.on) {
-    BCFD_HIP(hipEventCreate(&e0));
-    BCFD_HIP(hipEventCreate(&e1));
-    BCFD_HIP(hipEventRecord(e0, s));
-  }
+  if (sor_timing_begin(&e0)) return -1;
   if (remaining < 1) remaining = 1;
   const long long total = remaining;
   const long long chunk = 50;
@@ -605,14 +586,7 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
     BCFD_HIP(hipStreamSynchronize(s));
     if (h->done) break;
   }
-  if (g_tm.on) {
-    BCFD_HIP(hipEventRecord(e1, s));
-    BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
-    BCFD_HIP(hipStreamSynchronize(s));
-    g_tm.ev.emplace_back(e0, e1);
-    g_tm.iters += h->n_iter - start;
-    g_tm.cells += (h->n_iter - start) * (int64_t)b.d.nx * b.d.ny * b.d.nz;
-  }
+  if (sor_timing_end(e0, b, start)) return -1;
   return sor_finish(b, b.sor, variant);
 }
 
@@ -767,7 +741,7 @@ int biocfd_sor_config(int variant, int planes_per_workgroup) {
   return 0;
 }
 
-int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth) {
+int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth, int *min_wg_per_cu) {
   if (variant < 0 || variant >= kNumSorVariants) {
     set_error("sor_variant_shape: variant %d outside 0..%d", variant, kNumSorVariants - 1);
     return -1;
@@ -776,6 +750,7 @@ int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads
   *tile_j = kFused[variant].tj;
   *threads = kFused[variant].nt;
   *depth = kFused[variant].depth;
+  *min_wg_per_cu = kFused[variant].minb;
   return 0;
 }
 

@@ -221,6 +221,10 @@ static void free_block(Block &b) {
     if (b.bs[c]) (void)hipFree(b.bs[c]);
   }
   if (b.pc2) (void)hipFree(b.pc2);
+  for (int q = 0; q < 2; ++q) {
+    if (b.e32[q]) (void)hipFree(b.e32[q]);
+    if (b.r32[q]) (void)hipFree(b.r32[q]);
+  }
   if (b.partials) (void)hipFree(b.partials);
   IbData &ib = b.ib;
   for (auto &p : ib.el) if (p) (void)hipFree(p);

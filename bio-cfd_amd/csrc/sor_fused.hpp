@@ -28,6 +28,9 @@
 // The phase functions are the per-thread bodies of the kernel; a host build
 // of this header runs the same bodies thread by thread, phase by phase, to
 // check the schedule (bounds and bit-identity with the in-place sweep).
+//
+// Real = double is the reference's SOR; Real = float the smoothing sweeps of
+// the mixed-precision solve (same schedule, FP32 iterate and rhs).
 #pragma once
 
 #include "biocfd_rt.hpp"
@@ -39,50 +42,72 @@ __host__ __device__ inline int64_t sidx(int NXh, int NY, int h, int j, int k) {
   return (int64_t)h + (int64_t)NXh * ((int64_t)(j - 1) + (int64_t)NY * (int64_t)(k - 1));
 }
 
-// b/dt of a cell that the SOR does not update (not interior, or not fluid):
-// a NaN with a private payload, tested bitwise
+// right-hand side of a cell that the SOR does not update (not interior, or
+// not fluid): a NaN with a private payload, tested bitwise
 constexpr unsigned long long kNotFluid = 0x7ff4dead0000beefull;
+constexpr unsigned kNotFluid32 = 0x7fa0beefu;
 __host__ __device__ inline bool sor_skip(double v) {
   return __builtin_bit_cast(unsigned long long, v) == kNotFluid;
 }
-__host__ __device__ inline double not_fluid_marker() { return __builtin_bit_cast(double, kNotFluid); }
+__host__ __device__ inline bool sor_skip(float v) { return __builtin_bit_cast(unsigned, v) == kNotFluid32; }
+template <typename Real>
+__host__ __device__ inline Real not_fluid_marker();
+template <>
+__host__ __device__ inline double not_fluid_marker<double>() {
+  return __builtin_bit_cast(double, kNotFluid);
+}
+template <>
+__host__ __device__ inline float not_fluid_marker<float>() {
+  return __builtin_bit_cast(float, kNotFluid32);
+}
+__host__ __device__ inline double not_fluid_marker() { return not_fluid_marker<double>(); }
 
-struct FusedArgs {
+// Real = double: the reference's SOR on pc with rhs b/dt (bit-identical);
+// Real = float: the smoothing sweeps of the mixed-precision solve, on the
+// correction e with the FP64 residual rounded to FP32 as rhs.
+template <typename Real>
+struct FusedArgsT {
   Dims d;
-  int NXh;                 // (NX + 1) / 2, row length of the colour-split b/dt arrays
+  int NXh;                 // (NX + 1) / 2, row length of the colour-split rhs arrays
   int KS;                  // planes per workgroup
-  const double *src;       // pc at the start of the iteration (full layout)
-  double *dst;             // pc after the iteration (full layout)
-  const double *bred;      // b/dt of red cells, colour-split (marker where skipped)
-  const double *bblack;    // b/dt of black cells
+  int zero_src;            // src is all zero (first sweep from a zero guess): not read
+  const Real *src;         // iterate at the start of the iteration (full layout)
+  Real *dst;               // iterate after the iteration (full layout)
+  const Real *bred;        // rhs of red cells, colour-split (marker where skipped)
+  const Real *bblack;      // rhs of black cells
   const double *Ax, *Ay, *Az;
   double omega;
 };
+using FusedArgs = FusedArgsT<double>;
 
 // stencil coefficients of one axis at one index: west/south/bottom, centre,
 // east/north/top (columns 1..3 of Acx/Acy/Acz, CoefficientMatrix.f90)
-struct Coef3 {
-  double m, c, p;
+template <typename Real>
+struct Coef3T {
+  Real m, c, p;
 };
+using Coef3 = Coef3T<double>;
 
 // the SOR update of one cell: operand order of PcorVcor.F90:318-327
-__host__ __device__ inline double sor_cell(double omega, const Coef3 &ax, const Coef3 &ay, const Coef3 &az,
-                                           double rhs, double old, double pw, double pe, double ps, double pn,
-                                           double pb, double pt) {
-  const double gs = (rhs - ax.m * pw - ax.p * pe - ay.m * ps - ay.p * pn - az.m * pb - az.p * pt) /
-                    (ax.c + ay.c + az.c);
-  return (1.0 - omega) * old + omega * gs;
+template <typename Real>
+__host__ __device__ inline Real sor_cell(Real omega, const Coef3T<Real> &ax, const Coef3T<Real> &ay,
+                                         const Coef3T<Real> &az, Real rhs, Real old, Real pw, Real pe, Real ps,
+                                         Real pn, Real pb, Real pt) {
+  const Real gs = (rhs - ax.m * pw - ax.p * pe - ay.m * ps - ay.p * pn - az.m * pb - az.p * pt) /
+                  (ax.c + ay.c + az.c);
+  return (Real(1) - omega) * old + omega * gs;
 }
 
 // coefficients of axis index q (2-based interior index, n interior cells)
-__host__ __device__ inline Coef3 coef_at(const double *A, int n, int q) {
-  return Coef3{A[q - 2], A[(q - 2) + n], A[(q - 2) + 2 * n]};
+template <typename Real = double>
+__host__ __device__ inline Coef3T<Real> coef_at(const double *A, int n, int q) {
+  return Coef3T<Real>{Real(A[q - 2]), Real(A[(q - 2) + n]), Real(A[(q - 2) + 2 * n])};
 }
 
-template <int TI, int TJ, int NT, int D>
+template <typename Real, int TI, int TJ, int NT, int D>
 struct SorTile {
   static_assert(TI % 2 == 0 && TJ % 2 == 0, "tile sides must be even");
-  static_assert(TI <= 250 && TJ <= 250, "slot positions are packed in 8 bits");
+  static_assert(TI <= 500 && TJ <= 250, "slot positions are packed in 8 bits");
   static_assert(D == 1 || D == 2, "prefetch depth is 1 or 2 planes");
   static constexpr int PW = TI + 4;                 // slot row width in cells
   static constexpr int SH = TJ + 4;                 // slot rows
@@ -104,19 +129,25 @@ struct SorTile {
     int iL, jL, k0, k1;
   };
 
+  using Args = FusedArgsT<Real>;
+  using C3 = Coef3T<Real>;
+  // the update norm of the reference's convergence test; the FP32 sweeps
+  // of the mixed solve are judged by the FP64 residual instead
+  static constexpr bool kNorm = sizeof(Real) == sizeof(double);
+
   // per-thread registers of the march
   struct Thread {
-    double pcn[D][NL];        // planes in flight
-    double rn[NR], rc[NR];    // b/dt of the red cells: next step, this step
-    double bn[NB], bc[NB];    // b/dt of the black cells: next step, this step
+    Real pcn[D][NL];          // planes in flight
+    Real rn[NR], rc[NR];      // rhs of the red cells: next step, this step
+    Real bn[NB], bc[NB];      // rhs of the black cells: next step, this step
     int ld_off[NL];           // in-plane offset of the loaded slot cell, -1 outside the array
     int ld_lds[NL];           // 2 * (y * SW + (x >> 1)) + ((x + y) & 1), -1 past the slot
     int red_yr[NR];           // y << 8 | r: row y, r-th red cell of the row; -1 past the end
     int pair_yh[NB];          // y << 8 | h of a red/black pair; -1 past the end
-    Coef3 czm, cz0, czp;      // Acz of planes s-1, s, s+1
+    C3 czm, cz0, czp;         // Acz of planes s-1, s, s+1
   };
 
-  __host__ __device__ static double *slot(double *lds, int k, int colour) {
+  __host__ __device__ static Real *slot(Real *lds, int k, int colour) {
     return lds + ((k & 3) * 2 + colour) * (SH * SW);
   }
 
@@ -128,7 +159,7 @@ struct SorTile {
     return xcd < rem ? xcd * (per + 1) + q : rem * (per + 1) + (xcd - rem) * per + q;
   }
 
-  __host__ __device__ static Tile tile_of(const FusedArgs &a, int t, int gx, int gy) {
+  __host__ __device__ static Tile tile_of(const Args &a, int t, int gx, int gy) {
     const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
     Tile T;
     T.iL = TI * tx;
@@ -138,12 +169,12 @@ struct SorTile {
     return T;
   }
 
-  __host__ __device__ static Coef3 cz_of(const FusedArgs &a, int k) {
-    if (k < 2 || k > a.d.nz + 1) return Coef3{0.0, 0.0, 0.0};
-    return coef_at(a.Az, a.d.nz, k);
+  __host__ __device__ static C3 cz_of(const Args &a, int k) {
+    if (k < 2 || k > a.d.nz + 1) return C3{Real(0), Real(0), Real(0)};
+    return coef_at<Real>(a.Az, a.d.nz, k);
   }
 
-  __host__ __device__ static void init_thread(const FusedArgs &a, const Tile &T, int tid, Thread &th) {
+  __host__ __device__ static void init_thread(const Args &a, const Tile &T, int tid, Thread &th) {
     for (int q = 0; q < NL; ++q) {
       const int e = tid + q * NT;
       th.ld_off[q] = -1;
@@ -178,36 +209,36 @@ struct SorTile {
 
   // Acx over slot columns x = 1..TI+2 and Acy over slot rows y = 1..TJ+2
   // (zero outside the interior, where no cell is updated)
-  __host__ __device__ static void load_coef(const FusedArgs &a, const Tile &T, double *lds, int tid) {
-    double *cx = lds + SLOTS, *cy = cx + 3 * CXW;
+  __host__ __device__ static void load_coef(const Args &a, const Tile &T, Real *lds, int tid) {
+    Real *cx = lds + SLOTS, *cy = cx + 3 * CXW;
     for (int e = tid; e < 3 * CXW + 3 * CYW; e += NT) {
       if (e < 3 * CXW) {
         const int m = e / CXW, i = T.iL + 1 + (e - m * CXW);
-        cx[e] = (i >= 2 && i <= a.d.nx + 1) ? a.Ax[(i - 2) + m * a.d.nx] : 0.0;
+        cx[e] = (i >= 2 && i <= a.d.nx + 1) ? Real(a.Ax[(i - 2) + m * a.d.nx]) : Real(0);
       } else {
         const int f = e - 3 * CXW, m = f / CYW, j = T.jL + 1 + (f - m * CYW);
-        cy[f] = (j >= 2 && j <= a.d.ny + 1) ? a.Ay[(j - 2) + m * a.d.ny] : 0.0;
+        cy[f] = (j >= 2 && j <= a.d.ny + 1) ? Real(a.Ay[(j - 2) + m * a.d.ny]) : Real(0);
       }
     }
   }
-  __host__ __device__ static Coef3 cx_at(const double *lds, int x) {
-    const double *cx = lds + SLOTS;
-    return Coef3{cx[x - 1], cx[CXW + x - 1], cx[2 * CXW + x - 1]};
+  __host__ __device__ static C3 cx_at(const Real *lds, int x) {
+    const Real *cx = lds + SLOTS;
+    return C3{cx[x - 1], cx[CXW + x - 1], cx[2 * CXW + x - 1]};
   }
-  __host__ __device__ static Coef3 cy_at(const double *lds, int y) {
-    const double *cy = lds + SLOTS + 3 * CXW;
-    return Coef3{cy[y - 1], cy[CYW + y - 1], cy[2 * CYW + y - 1]};
+  __host__ __device__ static C3 cy_at(const Real *lds, int y) {
+    const Real *cy = lds + SLOTS + 3 * CXW;
+    return C3{cy[y - 1], cy[CYW + y - 1], cy[2 * CYW + y - 1]};
   }
 
   // plane p of src over the slot footprint (zero outside the array)
-  __host__ __device__ static void load_plane(const FusedArgs &a, const Thread &th, int p, double *r) {
-    const bool pin = p >= 1 && p <= a.d.NZ;
-    const double *P = a.src + (int64_t)(pin ? p - 1 : 0) * ((int64_t)a.d.NX * a.d.NY);
-    for (int q = 0; q < NL; ++q) r[q] = (pin && th.ld_off[q] >= 0) ? P[th.ld_off[q]] : 0.0;
+  __host__ __device__ static void load_plane(const Args &a, const Thread &th, int p, Real *r) {
+    const bool pin = p >= 1 && p <= a.d.NZ && !a.zero_src;
+    const Real *P = a.src + (int64_t)(pin ? p - 1 : 0) * ((int64_t)a.d.NX * a.d.NY);
+    for (int q = 0; q < NL; ++q) r[q] = (pin && th.ld_off[q] >= 0) ? P[th.ld_off[q]] : Real(0);
   }
 
-  __host__ __device__ static void store_plane(double *lds, const Thread &th, int p, const double *r) {
-    double *S = slot(lds, p, 0);
+  __host__ __device__ static void store_plane(Real *lds, const Thread &th, int p, const Real *r) {
+    Real *S = slot(lds, p, 0);
     for (int q = 0; q < NL; ++q) {
       const int l = th.ld_lds[q];
       if (l >= 0) S[((l & 1) ^ (p & 1)) * (SH * SW) + (l >> 1)] = r[q];
@@ -222,12 +253,11 @@ struct SorTile {
     x = 2 * h + off;
   }
 
-  __host__ __device__ static void load_bdt_red(const FusedArgs &a, const Tile &T, const Thread &th, int s,
-                                               double *r) {
+  __host__ __device__ static void load_bdt_red(const Args &a, const Tile &T, const Thread &th, int s, Real *r) {
     const bool kin = s >= 2 && s <= a.d.nz + 1;
-    const double *P = a.bred + (int64_t)(kin ? s - 1 : 0) * ((int64_t)a.NXh * a.d.NY);
+    const Real *P = a.bred + (int64_t)(kin ? s - 1 : 0) * ((int64_t)a.NXh * a.d.NY);
     for (int q = 0; q < NR; ++q) {
-      double v = not_fluid_marker();
+      Real v = not_fluid_marker<Real>();
       if (kin && th.red_yr[q] >= 0) {
         int y, h, x;
         red_cell(th.red_yr[q], s, y, h, x);
@@ -246,11 +276,10 @@ struct SorTile {
     off = (0 - y - k) & 1;
   }
 
-  __host__ __device__ static void load_bdt_black(const FusedArgs &a, const Tile &T, const Thread &th, int k,
-                                                 double *r) {
-    const double *P = a.bblack + (int64_t)(k - 1) * ((int64_t)a.NXh * a.d.NY);
+  __host__ __device__ static void load_bdt_black(const Args &a, const Tile &T, const Thread &th, int k, Real *r) {
+    const Real *P = a.bblack + (int64_t)(k - 1) * ((int64_t)a.NXh * a.d.NY);
     for (int q = 0; q < NB; ++q) {
-      double v = not_fluid_marker();
+      Real v = not_fluid_marker<Real>();
       if (th.pair_yh[q] >= 0) {
         int y, h, off;
         pair_cell(th.pair_yh[q], k, y, h, off);
@@ -262,32 +291,33 @@ struct SorTile {
   }
 
   // red update of plane s (tile + halo); returns the largest |change|
-  __host__ __device__ static double red_phase(const FusedArgs &a, double *lds, const Thread &th, int s) {
-    double *R = slot(lds, s, 1);
-    const double *B = slot(lds, s, 0), *Bm = slot(lds, s - 1, 0), *Bp = slot(lds, s + 1, 0);
+  __host__ __device__ static double red_phase(const Args &a, Real *lds, const Thread &th, int s) {
+    Real *R = slot(lds, s, 1);
+    const Real *B = slot(lds, s, 0), *Bm = slot(lds, s - 1, 0), *Bp = slot(lds, s + 1, 0);
     double delta = 0.0;
     for (int q = 0; q < NR; ++q) {
       if (th.red_yr[q] < 0 || sor_skip(th.rc[q])) continue;
       int y, h, x;
       red_cell(th.red_yr[q], s, y, h, x);
       const int c = y * SW + h;
-      const double old = R[c];
-      const double nw = sor_cell(a.omega, cx_at(lds, x), cy_at(lds, y), th.cz0, th.rc[q], old,
-                                 B[y * SW + ((x - 1) >> 1)], B[y * SW + ((x + 1) >> 1)], B[c - SW], B[c + SW], Bm[c],
-                                 Bp[c]);
+      const Real old = R[c];
+      const Real nw = sor_cell<Real>(Real(a.omega), cx_at(lds, x), cy_at(lds, y), th.cz0, th.rc[q], old,
+                                     B[y * SW + ((x - 1) >> 1)], B[y * SW + ((x + 1) >> 1)], B[c - SW], B[c + SW],
+                                     Bm[c], Bp[c]);
       R[c] = nw;
-      const double dd = fabs(nw - old);
-      delta = delta > dd ? delta : dd;
+      if (kNorm) {
+        const double dd = fabs(double(nw - old));
+        delta = delta > dd ? delta : dd;
+      }
     }
     return delta;
   }
 
   // black update of plane k (tile) and write-back of the tile's plane k
-  __host__ __device__ static double black_phase(const FusedArgs &a, const Tile &T, double *lds, const Thread &th,
-                                                int k) {
-    const double *R = slot(lds, k, 1), *Rm = slot(lds, k - 1, 1), *Rp = slot(lds, k + 1, 1);
-    const double *B = slot(lds, k, 0);
-    double *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
+  __host__ __device__ static double black_phase(const Args &a, const Tile &T, Real *lds, const Thread &th, int k) {
+    const Real *R = slot(lds, k, 1), *Rm = slot(lds, k - 1, 1), *Rp = slot(lds, k + 1, 1);
+    const Real *B = slot(lds, k, 0);
+    Real *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
     double delta = 0.0;
     for (int q = 0; q < NB; ++q) {
       if (th.pair_yh[q] < 0) continue;
@@ -299,13 +329,16 @@ struct SorTile {
       const int c = y * SW + h;
       const int row = a.d.NX * (j - 1) - 1;
       if (i <= a.d.nx + 1) {
-        const double old = B[c];
-        double nw = old;
+        const Real old = B[c];
+        Real nw = old;
         if (!sor_skip(th.bc[q])) {
-          nw = sor_cell(a.omega, cx_at(lds, x), cy_at(lds, y), th.czm, th.bc[q], old, R[y * SW + ((x - 1) >> 1)],
-                        R[y * SW + ((x + 1) >> 1)], R[c - SW], R[c + SW], Rm[c], Rp[c]);
-          const double dd = fabs(nw - old);
-          delta = delta > dd ? delta : dd;
+          nw = sor_cell<Real>(Real(a.omega), cx_at(lds, x), cy_at(lds, y), th.czm, th.bc[q], old,
+                              R[y * SW + ((x - 1) >> 1)], R[y * SW + ((x + 1) >> 1)], R[c - SW], R[c + SW], Rm[c],
+                              Rp[c]);
+          if (kNorm) {
+            const double dd = fabs(double(nw - old));
+            delta = delta > dd ? delta : dd;
+          }
         }
         P[row + i] = nw;
       }
@@ -315,20 +348,19 @@ struct SorTile {
   }
 
   // ---- the march: prologue, then per step s = k0-1 .. k1+1 the phases A | B | C ----
-  __host__ __device__ static void prologue(const FusedArgs &a, const Tile &T, double *lds, Thread &th) {
-    double *r = th.pcn[0];
+  __host__ __device__ static void prologue(const Args &a, const Tile &T, Real *lds, Thread &th) {
+    Real *r = th.pcn[0];
     load_plane(a, th, T.k0 - 2, r);
     store_plane(lds, th, T.k0 - 2, r);
     load_plane(a, th, T.k0 - 1, r);
     store_plane(lds, th, T.k0 - 1, r);
     for (int d = 0; d < D; ++d) load_plane(a, th, T.k0 + d, th.pcn[d]);
     load_bdt_red(a, T, th, T.k0 - 1, th.rn);
-    for (int q = 0; q < NB; ++q) th.bn[q] = 0.0;
+    for (int q = 0; q < NB; ++q) th.bn[q] = Real(0);
   }
 
   // buf holds plane s+1 on entry and plane s+1+D on exit
-  __host__ __device__ static void phase_a(const FusedArgs &a, const Tile &T, double *lds, Thread &th, int s,
-                                          double *buf) {
+  __host__ __device__ static void phase_a(const Args &a, const Tile &T, Real *lds, Thread &th, int s, Real *buf) {
     store_plane(lds, th, s + 1, buf);
     for (int q = 0; q < NR; ++q) th.rc[q] = th.rn[q];
     for (int q = 0; q < NB; ++q) th.bc[q] = th.bn[q];
@@ -338,11 +370,11 @@ struct SorTile {
     th.czp = cz_of(a, s + 1);
   }
 
-  __host__ __device__ static double phase_b(const FusedArgs &a, double *lds, const Thread &th, int s) {
+  __host__ __device__ static double phase_b(const Args &a, Real *lds, const Thread &th, int s) {
     return red_phase(a, lds, th, s);
   }
 
-  __host__ __device__ static double phase_c(const FusedArgs &a, const Tile &T, double *lds, Thread &th, int s) {
+  __host__ __device__ static double phase_c(const Args &a, const Tile &T, Real *lds, Thread &th, int s) {
     const double d = (s - 1 >= T.k0) ? black_phase(a, T, lds, th, s - 1) : 0.0;
     th.czm = th.cz0;
     th.cz0 = th.czp;

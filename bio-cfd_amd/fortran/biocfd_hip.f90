@@ -315,14 +315,15 @@ module biocfd_hip
       integer(c_int), value :: planes_per_workgroup
     end function
 
-    integer(c_int) function biocfd_sor_variant_shape(variant, tile_i, tile_j, threads, &
-        depth) bind(C, name="biocfd_sor_variant_shape")
+    integer(c_int) function biocfd_sor_variant_shape(variant, tile_i, tile_j, threads, depth, &
+        min_wg_per_cu) bind(C, name="biocfd_sor_variant_shape")
       import :: c_int, c_int64_t, c_double, c_ptr
       integer(c_int), value :: variant
       integer(c_int) :: tile_i(*)
       integer(c_int) :: tile_j(*)
       integer(c_int) :: threads(*)
       integer(c_int) :: depth(*)
+      integer(c_int) :: min_wg_per_cu(*)
     end function
 
     integer(c_int) function biocfd_sor_time(blk, variant, planes_per_workgroup, iters, dt, &
@@ -334,6 +335,49 @@ module biocfd_hip
       integer(c_int), value :: iters
       real(c_double), value :: dt
       real(c_double) :: ms(*)
+    end function
+
+    integer(c_int) function biocfd_set_precision(mode) bind(C, name="biocfd_set_precision")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      integer(c_int), value :: mode
+    end function
+
+    integer(c_int) function biocfd_mixed_config(variant, planes_per_workgroup, &
+        sweeps_per_correction, residual_tol) bind(C, name="biocfd_mixed_config")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      integer(c_int), value :: variant
+      integer(c_int), value :: planes_per_workgroup
+      integer(c_int), value :: sweeps_per_correction
+      real(c_double), value :: residual_tol
+    end function
+
+    integer(c_int) function biocfd_mixed_variant_shape(variant, tile_i, tile_j, threads, depth, &
+        min_wg_per_cu) bind(C, name="biocfd_mixed_variant_shape")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      integer(c_int), value :: variant
+      integer(c_int) :: tile_i(*)
+      integer(c_int) :: tile_j(*)
+      integer(c_int) :: threads(*)
+      integer(c_int) :: depth(*)
+      integer(c_int) :: min_wg_per_cu(*)
+    end function
+
+    integer(c_int) function biocfd_mixed_time(blk, variant, planes_per_workgroup, iters, dt, &
+        ms) bind(C, name="biocfd_mixed_time")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      integer(c_int), value :: blk
+      integer(c_int), value :: variant
+      integer(c_int), value :: planes_per_workgroup
+      integer(c_int), value :: iters
+      real(c_double), value :: dt
+      real(c_double) :: ms(*)
+    end function
+
+    integer(c_int) function biocfd_residual_norm(blk, dt, out) bind(C, name="biocfd_residual_norm")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      integer(c_int), value :: blk
+      real(c_double), value :: dt
+      real(c_double) :: out(*)
     end function
 
     integer(c_int) function biocfd_compute_surface_norm(blk, xnode1, ynode1, znode1, inor) bind(C, &

@@ -54,9 +54,15 @@ def grid_arrays(st: BlockState) -> List[np.ndarray]:
 class Simulation:
     """One case on the GPU; the device holds all state between calls."""
 
-    def __init__(self, case: CS.Case, device: int = 0):
+    def __init__(self, case: CS.Case, device: int = 0, precision: str = "fp64"):
+        """precision: "fp64" (the reference's method) or "mixed" (FP32 SOR
+        smoothing inside FP64 defect correction, include/biocfd_hip.h)."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         self.case = case
+        self.precision = precision
         H.call("biocfd_init", device)
+        H.call("biocfd_set_precision", 0 if precision == "fp64" else 1)
         self.blocks = [BlockState(g + 1, spec) for g, spec in enumerate(case.blocks)]
         self.totime = 0.0
         self.ita = 0
@@ -298,5 +304,12 @@ class Simulation:
         H.call("biocfd_list_download", g, H.BL[name], H.ptr(a), n)
         return a
 
+    def residual_norm(self, g: int = 1) -> float:
+        """max |b/dt - A pc| over the fluid cells of block g (FP64)."""
+        out = np.zeros(1)
+        H.call("biocfd_residual_norm", g, self.case.dt, H.ptr(out))
+        return float(out[0])
+
     def close(self):
+        H.call("biocfd_set_precision", 0)
         H.call("biocfd_finalize")

@@ -17,15 +17,19 @@ from . import hip as H
 DEFAULT_PLANES = (8, 16, 32, 64)
 
 
-def variants():
-    """[(variant, tile_i, tile_j, threads, prefetch_depth)] of the compiled SOR kernels."""
+def _shapes(fn):
     out = []
     v = 0
-    a, b, t, d = C.c_int(), C.c_int(), C.c_int(), C.c_int()
-    while H.lib().biocfd_sor_variant_shape(v, C.byref(a), C.byref(b), C.byref(t), C.byref(d)) == 0:
-        out.append((v, a.value, b.value, t.value, d.value))
+    a, b, t, d, m = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    while fn(v, C.byref(a), C.byref(b), C.byref(t), C.byref(d), C.byref(m)) == 0:
+        out.append((v, a.value, b.value, t.value, d.value, m.value))
         v += 1
     return out
+
+
+def variants():
+    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu)] of the FP64 SOR kernels."""
+    return _shapes(H.lib().biocfd_sor_variant_shape)
 
 
 def time_variant(block: int, variant: int, planes: int, iters: int, dt: float, repeats: int = 2) -> float:
@@ -43,12 +47,37 @@ def autotune(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PL
     (best_variant, best_planes, table) with table rows
     {"variant", "tile", "threads", "planes", "us_per_iteration"}."""
     table = []
-    for v, ti, tj, nt, depth in variants():
+    for v, ti, tj, nt, depth, minb in variants():
         for p in (planes if v > 0 else (32,)):
             ms = time_variant(block, v, p, iters, dt)
-            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "planes": p,
-                          "us_per_iteration": round(1e3 * ms, 2)})
+            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "min_wg_per_cu": minb,
+                          "planes": p, "us_per_iteration": round(1e3 * ms, 2)})
     best = min(table, key=lambda r: r["us_per_iteration"])
     if install:
         H.call("biocfd_sor_config", best["variant"], best["planes"])
+    return best["variant"], best["planes"], table
+
+
+def mixed_variants():
+    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu)] of the FP32 smoothing kernels."""
+    return _shapes(H.lib().biocfd_mixed_variant_shape)
+
+
+def autotune_mixed(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PLANES, sweeps: int = 4,
+                   residual_tol: float = 1e-8, install: bool = True):
+    """Time every FP32 smoothing candidate on `block` (the residual of its
+    current pc as rhs; pc is not changed); return (variant, planes, table)."""
+    table = []
+    ms = np.zeros(1)
+    for v, ti, tj, nt, depth, minb in mixed_variants():
+        for p in planes:
+            best = float("inf")
+            for _ in range(2):
+                H.call("biocfd_mixed_time", block, v, p, iters, dt, H.ptr(ms))
+                best = min(best, float(ms[0]) / iters)
+            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "min_wg_per_cu": minb,
+                          "planes": p, "us_per_sweep": round(1e3 * best, 2)})
+    best = min(table, key=lambda r: r["us_per_sweep"])
+    if install:
+        H.call("biocfd_mixed_config", best["variant"], best["planes"], sweeps, residual_tol)
     return best["variant"], best["planes"], table

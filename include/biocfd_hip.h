@@ -125,13 +125,32 @@ int biocfd_correct(int blk, double dt);
  * over tiles of tile_i x tile_j cells and `planes_per_workgroup` z-planes.
  * Every variant gives the same bits; the choice only changes speed. */
 int biocfd_sor_config(int variant, int planes_per_workgroup);
-/* tile shape, threads per workgroup and z-plane prefetch depth of a variant
- * (all 0 for variant 0); fails past the last variant */
-int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth);
+/* tile shape, threads per workgroup, z-plane prefetch depth and the
+ * workgroups per CU its register budget is compiled for (all 0 for
+ * variant 0); fails past the last variant */
+int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth,
+                             int *min_wg_per_cu);
 /* Autotuning probe (no reference counterpart): time `iters` SOR iterations
  * of a variant on block `blk`'s current pc and b (milliseconds, HIP events);
  * pc and the block's SOR state are restored afterwards. */
 int biocfd_sor_time(int blk, int variant, int planes_per_workgroup, int iters, double dt, double *ms);
+
+/* ---- mixed precision (no reference counterpart: the reference solves in
+ * FP64 only).  mode 1 replaces every REDBLACKSOR_linear call of
+ * poissonSolver by FP64 defect correction with FP32 red-black SOR sweeps:
+ * r = b/dt - A pc (FP64), `sweeps_per_correction` FP32 sweeps on A e = r from
+ * e = 0, pc += e (FP64), until max|r| < residual_tol or the call's sweep
+ * budget (pcItaMax + 1 - nIterPcor) is spent; nIterPcor counts sweeps and
+ * derr2 holds max|r|.  mode 0 (default) is the FP64 method. */
+int biocfd_set_precision(int mode);
+int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_correction,
+                        double residual_tol);
+int biocfd_mixed_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth,
+                               int *min_wg_per_cu);
+/* time `iters` FP32 sweeps of a variant on the residual of block `blk`'s pc */
+int biocfd_mixed_time(int blk, int variant, int planes_per_workgroup, int iters, double dt, double *ms);
+/* max over fluid cells of |b/dt - A pc| for block `blk` (FP64 diagnostic) */
+int biocfd_residual_norm(int blk, double dt, double *out);
 
 /* ---- immersed-boundary geometry, search and tagging (Search.F90) ---- */
 /* computeSurfaceNorm :210-258 (nodes are the moved positions xnode1..) */

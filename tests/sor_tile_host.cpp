@@ -13,16 +13,19 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "sor_fused.hpp"
 
 using namespace bcfd;
 
+template <typename Real>
 struct Problem {
   Dims d;
   int NXh;
-  std::vector<double> pc, bred, bblack, Ax, Ay, Az;
+  std::vector<Real> pc, bred, bblack;
+  std::vector<double> Ax, Ay, Az;
   std::vector<unsigned char> fluid;
   double omega;
 };
@@ -35,17 +38,18 @@ static double urand() {
   return (double)(g_rng >> 11) * (1.0 / 9007199254740992.0);
 }
 
-static Problem make_problem(int nx, int ny, int nz, double solid_frac) {
-  Problem p;
+template <typename Real>
+static Problem<Real> make_problem(int nx, int ny, int nz, double solid_frac) {
+  Problem<Real> p;
   p.d = Dims{nx, ny, nz, nx + 2, ny + 2, nz + 2, nx + 3, ny + 3};
   p.NXh = (p.d.NX + 1) / 2;
   const int64_t n = (int64_t)p.d.NX * p.d.NY * p.d.NZ;
   const int64_t nh = (int64_t)p.NXh * p.d.NY * p.d.NZ;
   p.pc.resize(n);
   p.fluid.assign(n, 0);
-  p.bred.assign(nh, not_fluid_marker());
-  p.bblack.assign(nh, not_fluid_marker());
-  for (auto &v : p.pc) v = urand() - 0.5;
+  p.bred.assign(nh, not_fluid_marker<Real>());
+  p.bblack.assign(nh, not_fluid_marker<Real>());
+  for (auto &v : p.pc) v = Real(urand() - 0.5);
   auto axis = [](std::vector<double> &A, int m) {
     A.resize(3 * (size_t)m);
     for (int q = 0; q < m; ++q) {
@@ -63,7 +67,7 @@ static Problem make_problem(int nx, int ny, int nz, double solid_frac) {
       for (int i = 2; i <= nx + 1; ++i) {
         if (urand() < solid_frac) continue;
         p.fluid[ix3(p.d, i, j, k)] = 1;
-        const double v = (urand() - 0.5) * 4.0;
+        const Real v = Real((urand() - 0.5) * 4.0);
         auto &dst = ((i + j + k) & 1) ? p.bred : p.bblack;
         dst[sidx(p.NXh, p.d.NY, (i - 1) >> 1, j, k)] = v;
       }
@@ -71,11 +75,13 @@ static Problem make_problem(int nx, int ny, int nz, double solid_frac) {
   return p;
 }
 
-static FusedArgs args_of(const Problem &p, int KS) {
-  FusedArgs a;
+template <typename Real>
+static FusedArgsT<Real> args_of(const Problem<Real> &p, int KS) {
+  FusedArgsT<Real> a;
   a.d = p.d;
   a.NXh = p.NXh;
   a.KS = KS;
+  a.zero_src = 0;
   a.src = nullptr;
   a.dst = nullptr;
   a.bred = p.bred.data();
@@ -88,8 +94,9 @@ static FusedArgs args_of(const Problem &p, int KS) {
 }
 
 // textbook in-place red-black iteration; returns max |change|
-static double inplace_iteration(const Problem &p, std::vector<double> &pc) {
-  const FusedArgs a = args_of(p, 1);
+template <typename Real>
+static double inplace_iteration(const Problem<Real> &p, std::vector<Real> &pc) {
+  const FusedArgsT<Real> a = args_of(p, 1);
   const Dims &d = p.d;
   const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
   double delta = 0.0;
@@ -99,14 +106,15 @@ static double inplace_iteration(const Problem &p, std::vector<double> &pc) {
         for (int i = 2; i <= d.nx + 1; ++i) {
           if (((i + j + k) & 1) != colour) continue;
           const auto &bt = colour ? p.bred : p.bblack;
-          const double rhs = bt[sidx(p.NXh, d.NY, (i - 1) >> 1, j, k)];
+          const Real rhs = bt[sidx(p.NXh, d.NY, (i - 1) >> 1, j, k)];
           if (sor_skip(rhs)) continue;
           const int64_t c = ix3(d, i, j, k);
-          const double old = pc[c];
-          const double nw = sor_cell(a.omega, coef_at(a.Ax, d.nx, i), coef_at(a.Ay, d.ny, j), coef_at(a.Az, d.nz, k),
-                                     rhs, old, pc[c - 1], pc[c + 1], pc[c - sy], pc[c + sy], pc[c - sz], pc[c + sz]);
+          const Real old = pc[c];
+          const Real nw = sor_cell<Real>(Real(a.omega), coef_at<Real>(a.Ax, d.nx, i), coef_at<Real>(a.Ay, d.ny, j),
+                                         coef_at<Real>(a.Az, d.nz, k), rhs, old, pc[c - 1], pc[c + 1], pc[c - sy],
+                                         pc[c + sy], pc[c - sz], pc[c + sz]);
           pc[c] = nw;
-          const double dd = std::fabs(nw - old);
+          const double dd = std::fabs(double(nw - old));
           delta = delta > dd ? delta : dd;
         }
   return delta;
@@ -114,15 +122,17 @@ static double inplace_iteration(const Problem &p, std::vector<double> &pc) {
 
 // one fused iteration src -> dst, emulating every workgroup and thread: the
 // kernel's step loop with each barrier-separated phase run for all threads
-template <int TI, int TJ, int NT, int D>
-static double fused_iteration(const Problem &p, int KS, const std::vector<double> &src, std::vector<double> &dst) {
-  using T = SorTile<TI, TJ, NT, D>;
-  FusedArgs a = args_of(p, KS);
+template <typename Real, int TI, int TJ, int NT, int D>
+static double fused_iteration(const Problem<Real> &p, int KS, const std::vector<Real> &src, std::vector<Real> &dst,
+                              bool zero_src) {
+  using T = SorTile<Real, TI, TJ, NT, D>;
+  FusedArgsT<Real> a = args_of(p, KS);
   a.src = src.data();
   a.dst = dst.data();
+  a.zero_src = zero_src ? 1 : 0;
   const int gx = (p.d.nx + TI - 1) / TI, gy = (p.d.ny + TJ - 1) / TJ, gz = (p.d.nz + KS - 1) / KS;
   const int n = gx * gy * gz;
-  std::vector<double> lds(T::LDS_DOUBLES);
+  std::vector<Real> lds(T::LDS_DOUBLES);
   std::vector<typename T::Thread> th(NT);
   std::vector<int> seen(n, 0);
   double delta = 0.0;
@@ -144,7 +154,7 @@ static double fused_iteration(const Problem &p, int KS, const std::vector<double
       std::exit(2);
     }
     const typename T::Tile tile = T::tile_of(a, t, gx, gy);
-    for (auto &v : lds) v = std::nan("");
+    for (auto &v : lds) v = Real(std::nan(""));
     for (int tid = 0; tid < NT; ++tid) {
       T::init_thread(a, tile, tid, th[tid]);
       T::load_coef(a, tile, lds.data(), tid);
@@ -158,53 +168,78 @@ static double fused_iteration(const Problem &p, int KS, const std::vector<double
   return delta;
 }
 
-template <int TI, int TJ, int NT, int D>
-static int run_case(int nx, int ny, int nz, int KS, int iters) {
-  Problem p = make_problem(nx, ny, nz, 0.1);
-  std::vector<double> ref = p.pc, A = p.pc, B = p.pc;
+// `iters` fused iterations against the in-place sweep; with zero_first the
+// first one starts from a zero iterate without reading it (the first
+// smoothing sweep of a mixed-precision correction)
+template <typename Real, int TI, int TJ, int NT, int D>
+static int run_case(int nx, int ny, int nz, int KS, int iters, bool zero_first) {
+  Problem<Real> p = make_problem<Real>(nx, ny, nz, 0.1);
+  std::vector<Real> ref = p.pc, A = p.pc, B = p.pc;
+  if (zero_first) {
+    std::fill(ref.begin(), ref.end(), Real(0));
+    std::fill(B.begin(), B.end(), Real(0));  // ghosts of the destination hold zeros
+    // A keeps garbage: the first iteration must not read it
+  }
   for (int it = 0; it < iters; ++it) {
+    const bool z = zero_first && it == 0;
     const double dref = inplace_iteration(p, ref);
-    const double dfus = (it & 1) ? fused_iteration<TI, TJ, NT, D>(p, KS, B, A)
-                                : fused_iteration<TI, TJ, NT, D>(p, KS, A, B);
-    const std::vector<double> &cur = (it & 1) ? A : B;
-    if (std::memcmp(cur.data(), ref.data(), sizeof(double) * ref.size()) != 0 || dref != dfus) {
+    const double dfus = (it & 1) ? fused_iteration<Real, TI, TJ, NT, D>(p, KS, B, A, z)
+                                 : fused_iteration<Real, TI, TJ, NT, D>(p, KS, A, B, z);
+    const std::vector<Real> &cur = (it & 1) ? A : B;
+    const bool norm_ok = sizeof(Real) == sizeof(double) ? dref == dfus : dfus == 0.0;
+    if (std::memcmp(cur.data(), ref.data(), sizeof(Real) * ref.size()) != 0 || !norm_ok) {
       int64_t bad = -1;
       for (size_t q = 0; q < ref.size(); ++q)
-        if (std::memcmp(&cur[q], &ref[q], sizeof(double))) { bad = (int64_t)q; break; }
-      std::printf("MISMATCH tile %dx%d/%d depth %d grid %dx%dx%d KS=%d iteration %d: first cell %lld, "
+        if (std::memcmp(&cur[q], &ref[q], sizeof(Real))) { bad = (int64_t)q; break; }
+      std::printf("MISMATCH %s tile %dx%d/%d depth %d grid %dx%dx%d KS=%d iteration %d: first cell %lld, "
                   "delta %.17g vs %.17g\n",
-                  TI, TJ, NT, D, nx, ny, nz, KS, it + 1, (long long)bad, dref, dfus);
+                  sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, nx, ny, nz, KS, it + 1, (long long)bad, dref,
+                  dfus);
       return 1;
     }
+    if (z) std::fill(A.begin(), A.end(), Real(0));  // later sweeps read A: zero ghosts as the runtime keeps them
   }
-  std::printf("ok tile %dx%d/%d depth %d grid %dx%dx%d KS=%d iterations=%d\n", TI, TJ, NT, D, nx, ny, nz, KS,
-              iters);
+  std::printf("ok %s tile %dx%d/%d depth %d grid %dx%dx%d KS=%d iterations=%d%s\n",
+              sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, nx, ny, nz, KS, iters,
+              zero_first ? " (zero first)" : "");
   return 0;
 }
 
-template <int TI, int TJ, int NT, int D>
+template <typename Real, int TI, int TJ, int NT, int D>
 static int run_shape() {
   int bad = 0;
-  bad += run_case<TI, TJ, NT, D>(37, 21, 13, 4, 3);
-  bad += run_case<TI, TJ, NT, D>(TI, TJ, 9, 32, 2);
-  bad += run_case<TI, TJ, NT, D>(TI + 6, 2 * TJ + 3, 11, 3, 4);
-  bad += run_case<TI, TJ, NT, D>(5, 3, 7, 1, 3);
-  bad += run_case<TI, TJ, NT, D>(19, 9, 8, 2, 3);
+  bad += run_case<Real, TI, TJ, NT, D>(37, 21, 13, 4, 3, false);
+  bad += run_case<Real, TI, TJ, NT, D>(TI, TJ, 9, 32, 2, false);
+  bad += run_case<Real, TI, TJ, NT, D>(TI + 6, 2 * TJ + 3, 11, 3, 4, false);
+  bad += run_case<Real, TI, TJ, NT, D>(5, 3, 7, 1, 3, false);
+  bad += run_case<Real, TI, TJ, NT, D>(19, 9, 8, 2, 3, false);
+  bad += run_case<Real, TI, TJ, NT, D>(TI + 6, 2 * TJ + 3, 11, 3, 4, true);
   return bad;
 }
 
 int main() {
   int bad = 0;
-  // the shapes the library compiles
-  bad += run_shape<64, 16, 256, 1>();
-  bad += run_shape<64, 8, 256, 1>();
-  bad += run_shape<64, 8, 256, 2>();
-  bad += run_shape<128, 16, 512, 1>();
-  bad += run_shape<128, 16, 512, 2>();
-  bad += run_shape<64, 32, 512, 1>();
-  bad += run_shape<64, 32, 512, 2>();
-  bad += run_shape<32, 16, 256, 2>();
-  bad += run_shape<128, 8, 256, 2>();
+  // the FP64 shapes the library compiles (k_pcor.hip)
+  bad += run_shape<double, 64, 16, 256, 1>();
+  bad += run_shape<double, 64, 8, 256, 1>();
+  bad += run_shape<double, 64, 8, 256, 2>();
+  bad += run_shape<double, 128, 16, 512, 1>();
+  bad += run_shape<double, 128, 16, 512, 2>();
+  bad += run_shape<double, 64, 32, 512, 1>();
+  bad += run_shape<double, 64, 32, 512, 2>();
+  bad += run_shape<double, 32, 16, 256, 2>();
+  bad += run_shape<double, 128, 8, 256, 2>();
+  bad += run_shape<double, 64, 16, 256, 2>();
+  bad += run_shape<double, 32, 16, 256, 1>();
+  bad += run_shape<double, 128, 8, 256, 1>();
+  // the FP32 smoothing shapes (k_mixed.hip)
+  bad += run_shape<float, 64, 16, 256, 1>();
+  bad += run_shape<float, 64, 32, 512, 1>();
+  bad += run_shape<float, 64, 32, 512, 2>();
+  bad += run_shape<float, 128, 16, 512, 1>();
+  bad += run_shape<float, 128, 16, 512, 2>();
+  bad += run_shape<float, 128, 32, 512, 1>();
+  bad += run_shape<float, 64, 16, 256, 2>();
   std::printf(bad ? "FAILED %d\n" : "all cases bit-identical\n", bad);
   return bad ? 1 : 0;
 }

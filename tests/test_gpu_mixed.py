@@ -1,0 +1,88 @@
+"""Mixed precision (FP32 red-black SOR smoothing inside FP64 defect
+correction) against the FP64 oracle on the parity cases.
+
+The pressure solve keeps the FP64 method's sweep budget (the mixed solve is
+given no residual tolerance here, so both run pcItaMax + 1 sweeps), so the
+FP32 path follows the FP64 iterates up to the FP32 rounding of each
+correction.  Integer data (tags, index lists, nearest elements, counts,
+nIterPcor) must match exactly; fields within the per-field relative
+tolerances below (max |diff| / max |oracle|)."""
+import numpy as np
+import pytest
+
+from parity import run_oracle, small_cases
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 3
+# max |mixed - fp64| / max |fp64| allowed per field
+TOL = {"u": 1e-7, "v": 1e-7, "w": 1e-7, "ut": 1e-7, "vt": 1e-7, "wt": 1e-7, "p": 1e-7, "pc": 1e-6, "b": 1e-6}
+TOL_OTHER = 1e-6          # every other FP64 array (ghosts, normal distances, forces ...)
+SKIP = ("derr2", "pco")   # derr2 is the residual norm in the mixed solve, pco is not kept
+
+
+def rel_err(ref, got):
+    ref = np.asarray(ref, dtype=np.float64)
+    got = np.asarray(got, dtype=np.float64)
+    scale = max(float(np.max(np.abs(ref))), 1e-300) if ref.size else 1.0
+    return float(np.max(np.abs(got - ref))) / scale if ref.size else 0.0
+
+
+@pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing"])
+def test_mixed_step_against_fp64_oracle(name):
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = small_cases()[name]
+    ref = run_oracle(case, (STEPS,))[STEPS]
+    sim = Simulation(case, precision="mixed")
+    try:
+        H.call("biocfd_mixed_config", 1, 32, 4, 0.0)
+        for _ in range(STEPS):
+            sim.step()
+        got = sim.dump()
+    finally:
+        sim.close()
+        H.call("biocfd_mixed_config", 1, 32, 4, 1e-8)
+    bad, report = [], []
+    for g, (r, o) in enumerate(zip(ref, got)):
+        for k, x in r.items():
+            if k in SKIP or k not in o or k.startswith(("ca_", "ck_")):
+                continue
+            x = np.asarray(x)
+            y = np.asarray(o[k])
+            if x.shape != y.shape:
+                bad.append(f"block {g + 1} {k}: shape {x.shape} vs {y.shape}")
+            elif x.dtype.kind in "iu":
+                if not np.array_equal(x, y):
+                    bad.append(f"block {g + 1} {k}: {np.count_nonzero(x != y)} integer mismatches")
+            else:
+                e = rel_err(x, y)
+                report.append(f"{k}={e:.2e}")
+                if not e <= TOL.get(k, TOL_OTHER):
+                    bad.append(f"block {g + 1} {k}: relative error {e:.3e} > {TOL.get(k, TOL_OTHER):.0e}")
+    print(f"{name}: " + " ".join(report))
+    assert not bad, "\n".join(bad[:40])
+
+
+def test_mixed_stops_at_residual_tolerance():
+    """With a residual tolerance the correction loop stops early and the
+    final FP64 residual is below it."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = CS.config_sphere_64(n=24, markers=256, nsteps=2, pc_ita_max=400, epsi=0.0)
+    sim = Simulation(case, precision="mixed")
+    try:
+        H.call("biocfd_mixed_config", 1, 32, 4, 1e-6)
+        sim.step()
+        d = sim.dump()[0]
+        r = sim.residual_norm(1)
+    finally:
+        sim.close()
+        H.call("biocfd_mixed_config", 1, 32, 4, 1e-8)
+    assert int(d["nIterPcor"]) < 401
+    assert int(d["nIterPcor"]) % 4 == 0
+    assert r < 1e-6
+    assert abs(float(d["derr2"]) - r) <= 1e-12 * max(1.0, r)

@@ -31,11 +31,14 @@ def test_fused_sor_schedule_matches_inplace_sweep(tmp_path):
 
 
 def test_every_compiled_shape_is_emulated():
-    """The host emulation runs every (tile, threads, depth) the library instantiates."""
+    """The host emulation runs every (type, tile, threads, depth) the library instantiates."""
     import re
 
-    src = open(os.path.join(ROOT, "bio-cfd_amd", "csrc", "k_pcor.hip")).read()
+    csrc = os.path.join(ROOT, "bio-cfd_amd", "csrc")
     emu = open(os.path.join(ROOT, "tests", "sor_tile_host.cpp")).read()
-    lib = set(re.findall(r"launch_fused<(\d+), (\d+), (\d+), (\d+)>\}", src))
-    run = set(re.findall(r"run_shape<(\d+), (\d+), (\d+), (\d+)>\(\)", emu))
-    assert lib and lib <= run, sorted(lib - run)
+    run = set(re.findall(r"run_shape<(\w+), (\d+), (\d+), (\d+), (\d+)>\(\)", emu))
+    f64 = open(os.path.join(csrc, "k_pcor.hip")).read()
+    f32 = open(os.path.join(csrc, "k_mixed.hip")).read()
+    lib = {("double",) + t for t in re.findall(r"launch_fused<(\d+), (\d+), (\d+), (\d+), \d+>\}", f64)}
+    lib |= {("float",) + t for t in re.findall(r"launch_sweep32<(\d+), (\d+), (\d+), (\d+), \d+>\}", f32)}
+    assert len(lib) > 9 and lib <= run, sorted(lib - run)
