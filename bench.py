@@ -44,7 +44,12 @@ SOR_BYTES_PER_CELL_ITER = 24.0
 # HBM bytes per fused FP64 SOR launch measured with rocprofv3 (FETCH_SIZE x 2.00
 # + WRITE_SIZE x 1.00, calibrated on a 512 MiB copy), keyed by (n, tile_i,
 # tile_j, threads, depth, planes); quoted only for the configuration measured
-SOR_TRAFFIC_MEASURED: dict = {}
+SOR_TRAFFIC_MEASURED = {
+    # (n, tile_i, tile_j, threads, depth, iterations per pass, planes): bytes per launch of the
+    # kernel with that many iterations per pass (profiles/sor_fp64_sphere256_hbm_traffic.md)
+    (256, 64, 32, 512, 1, 2, 32): 5.7318e+08,
+    (256, 64, 32, 512, 1, 1, 32): 4.2695e+08,
+}
 
 
 def mixed_bytes_per_cell(sweeps_per_correction: int) -> float:
@@ -88,7 +93,7 @@ def run_path(args, precision: str, keep_fields: bool):
         print(json.dumps({"mixed_autotune": table}), file=sys.stderr)
         shape = [t for t in tune.mixed_variants() if t[0] == v][0]
     kernel = {"variant": v, "tile": [shape[1], shape[2]], "threads": shape[3], "depth": shape[4],
-              "min_wg_per_cu": shape[5], "planes": planes, "autotuned": precision == "mixed" or args.sor_variant == "auto"}
+              "min_wg_per_cu": shape[5], "iters_per_pass": shape[6], "planes": planes, "autotuned": precision == "mixed" or args.sor_variant == "auto"}
     for _ in range(args.warmup):
         sim.step()
     H.call("biocfd_synchronize")
@@ -236,7 +241,7 @@ def main():
         achieved = SOR_BYTES_PER_CELL_ITER * g["sor_cell_iterations"] / (g["sor_gpu_ms"] * 1e-3) / 1e9
         kk = g["kernel"]
         traffic = SOR_TRAFFIC_MEASURED.get((args.n, kk["tile"][0], kk["tile"][1], kk["threads"], kk["depth"],
-                                            kk["planes"]))
+                                            kk["iters_per_pass"], kk["planes"]))
         out["roofline"] = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -244,10 +249,11 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": ("red-black SOR iteration (k_sor_fused<double>: one fused red+black launch per iteration)"
+            "kernel": (f"red-black SOR (k_sor_fused<double>: {kk['iters_per_pass']} red+black iteration(s) per "
+                       f"launch, tile {kk['tile'][0]}x{kk['tile'][1]})"
                        if kk["variant"] > 0 else
                        "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
-                      + ", 24 B/cell/iteration algorithmic",
+                      + ", 24 B/cell/iteration algorithmic; traffic = HBM bytes per launch",
             "us_per_iteration": round(1e3 * g["sor_gpu_ms"] / max(1, g["sor_iterations"]), 2),
         }
         out["fp64_residual_linf"] = g["residual_linf"]

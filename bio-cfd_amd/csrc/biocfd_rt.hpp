@@ -63,7 +63,7 @@ struct SorState {
   double derr2;                  // block%derr2
   unsigned long long max_bits;   // running max of |pc-pco| (bit pattern of a double >= 0)
   int done;
-  int pad;
+  int passes;                    // fused launches of the current SOR call
   double derr_std_st;            // steady-state norm
   unsigned long long std_bits;
   long long call_iters;          // iterations executed by the current SOR call

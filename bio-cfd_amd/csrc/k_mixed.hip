@@ -119,31 +119,41 @@ __global__ void __launch_bounds__(1024) k_mx_end(SorState *st, const double *par
 
 // ---------------------------------------------------------------- host
 
+using Sweep32Launch = void (*)(dim3, dim3, hipStream_t, const FusedArgsT<float> &, int, int, const SorState *,
+                               double *);
 struct Sweep32Variant {
-  int ti, tj, nt, depth, minb;
-  void (*launch)(dim3, dim3, hipStream_t, const FusedArgsT<float> &, int, int, const SorState *, double *);
+  int ti, tj, nt, depth, minb, nit;
+  Sweep32Launch launch, launch1;
 };
 
-template <int TI, int TJ, int NT, int D, int MINB>
+template <int TI, int TJ, int NT, int D, int MINB, int NIT>
 static void launch_sweep32(dim3 g, dim3 blk, hipStream_t s, const FusedArgsT<float> &a, int gx, int gy,
                            const SorState *st, double *partials) {
-  hipLaunchKernelGGL((k_sor_fused<float, TI, TJ, NT, D, MINB>), g, blk, 0, s, a, gx, gy, 0, st, partials);
+  hipLaunchKernelGGL((k_sor_fused<float, TI, TJ, NT, D, NIT, MINB>), g, blk, 0, s, a, gx, gy, 0, st, partials);
 }
 
-// {tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers must allow}
+// SV(tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers
+// must allow, sweeps per pass)
+#define SV(TI, TJ, NT, D, MINB, NIT) \
+  { TI, TJ, NT, D, MINB, NIT, launch_sweep32<TI, TJ, NT, D, MINB, NIT>, launch_sweep32<TI, TJ, NT, D, MINB, 1> }
 static const Sweep32Variant kSweep32[] = {
-    {64, 16, 256, 1, 1, launch_sweep32<64, 16, 256, 1, 1>},
-    {64, 32, 512, 1, 1, launch_sweep32<64, 32, 512, 1, 1>},
-    {64, 32, 512, 2, 1, launch_sweep32<64, 32, 512, 2, 1>},
-    {128, 16, 512, 1, 1, launch_sweep32<128, 16, 512, 1, 1>},
-    {128, 16, 512, 2, 1, launch_sweep32<128, 16, 512, 2, 1>},
-    {128, 32, 512, 1, 1, launch_sweep32<128, 32, 512, 1, 1>},
-    {64, 32, 512, 1, 3, launch_sweep32<64, 32, 512, 1, 3>},
-    {64, 32, 512, 2, 2, launch_sweep32<64, 32, 512, 2, 2>},
-    {64, 16, 256, 1, 6, launch_sweep32<64, 16, 256, 1, 6>},
-    {64, 16, 256, 2, 4, launch_sweep32<64, 16, 256, 2, 4>},
-    {128, 16, 512, 1, 2, launch_sweep32<128, 16, 512, 1, 2>},
+    SV(64, 16, 256, 1, 1, 1),
+    SV(64, 32, 512, 1, 1, 1),
+    SV(64, 32, 512, 2, 1, 1),
+    SV(128, 16, 512, 1, 1, 1),
+    SV(128, 16, 512, 2, 1, 1),
+    SV(64, 32, 512, 1, 3, 1),
+    SV(64, 16, 256, 1, 1, 2),
+    SV(64, 32, 512, 1, 1, 2),
+    SV(128, 16, 512, 1, 1, 2),
+    SV(128, 32, 512, 1, 1, 2),
+    SV(64, 16, 512, 1, 1, 2),
+    SV(64, 32, 1024, 1, 1, 1),
+    SV(64, 32, 1024, 1, 1, 2),
+    SV(128, 32, 1024, 1, 1, 1),
+    SV(128, 16, 1024, 1, 1, 1),
 };
+#undef SV
 constexpr int kNumSweep32 = (int)(sizeof(kSweep32) / sizeof(kSweep32[0]));
 
 struct MixedConfig {
@@ -203,7 +213,8 @@ static int64_t residual_launch(Block &b, const double *src, const float *e, doub
   return (int64_t)grd.x * grd.y * grd.z;
 }
 
-// k sweeps from a zero correction; returns the array holding the result
+// k sweeps from a zero correction, in passes of up to `nit` sweeps; returns
+// the array holding the result
 static float *sweeps_launch(Block &b, int variant, int planes, int k, const SorState *st) {
   const Sweep32Variant &v = kSweep32[variant];
   const int gx = (int)div_up(b.d.nx, v.ti), gy = (int)div_up(b.d.ny, v.tj), gz = (int)div_up(b.d.nz, planes);
@@ -218,12 +229,16 @@ static float *sweeps_launch(Block &b, int variant, int planes, int k, const SorS
   a.Az = b.A[2];
   a.omega = b.omega;
   float *out = b.e32[0];
-  for (int q = 0; q < k; ++q) {
+  int q = 0;
+  for (int done = 0; done < k; ++q) {
+    const int L = k - done < v.nit ? k - done : v.nit;
     a.zero_src = q == 0;
     a.src = (q & 1) ? b.e32[0] : b.e32[1];
     a.dst = (q & 1) ? b.e32[1] : b.e32[0];
     out = a.dst;
-    v.launch(dim3((unsigned)(gx * gy * gz)), dim3(v.nt), rt().stream, a, gx, gy, st, b.partials);
+    (L == v.nit ? v.launch : v.launch1)(dim3((unsigned)(gx * gy * gz)), dim3(v.nt), rt().stream, a, gx, gy, st,
+                                         b.partials);
+    done += L;
   }
   return out;
 }
@@ -347,17 +362,18 @@ int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_co
   return 0;
 }
 
-int biocfd_mixed_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth,
-                               int *min_wg_per_cu) {
+int biocfd_mixed_variant_info(int variant, int info[6]) {
   if (variant < 0 || variant >= kNumSweep32) {
-    set_error("mixed_variant_shape: variant %d outside 0..%d", variant, kNumSweep32 - 1);
+    set_error("mixed_variant_info: variant %d outside 0..%d", variant, kNumSweep32 - 1);
     return -1;
   }
-  *tile_i = kSweep32[variant].ti;
-  *tile_j = kSweep32[variant].tj;
-  *threads = kSweep32[variant].nt;
-  *depth = kSweep32[variant].depth;
-  *min_wg_per_cu = kSweep32[variant].minb;
+  const Sweep32Variant &v = kSweep32[variant];
+  info[0] = v.ti;
+  info[1] = v.tj;
+  info[2] = v.nt;
+  info[3] = v.depth;
+  info[4] = v.minb;
+  info[5] = v.nit;
   return 0;
 }
 

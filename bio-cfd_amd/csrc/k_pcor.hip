@@ -36,6 +36,7 @@ __global__ void k_sor_reset_call(SorState *st, int reset_iter) {
     st->max_bits = 0ull;
     st->done = 0;
     st->call_iters = 0;
+    st->passes = 0;
   }
 }
 
@@ -148,9 +149,9 @@ __global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, 
 // norm only changes on check iterations and the pcItaMax exit only fires on
 // the last one, so the state after any other iteration needs no update until
 // the next of these.
-__global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long it, long long start, int check,
-                                                        long long pc_ita_max, double epsi, const double *partials,
-                                                        int64_t n_partials) {
+__global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long it, long long start,
+                                                        long long passes, int check, long long pc_ita_max,
+                                                        double epsi, const double *partials, int64_t n_partials) {
   if (st->done) return;
   __shared__ double s_m[16];
   if (check) {
@@ -168,15 +169,16 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
   }
   st->n_iter = it;
   st->call_iters = it - start;
+  st->passes = (int)passes;
   st->derr2 = st->derr4;
   if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
 }
 
-// the iterations of one SOR call alternate pc -> pc2 -> pc ...; after an odd
+// the passes of one SOR call alternate pc -> pc2 -> pc ...; after an odd
 // count the result is in pc2
 __global__ void k_sor_finish(const SorState *__restrict__ st, double *__restrict__ pc,
                              const double *__restrict__ pc2, int64_t n) {
-  if ((st->call_iters & 1) == 0) return;
+  if ((st->passes & 1) == 0) return;
   for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
     pc[q] = pc2[q];
 }
@@ -383,38 +385,50 @@ int ensure_split(Block &b) {
 }
 
 // SOR kernel variants: 0 = one launch per colour on the colour-split arrays
-// (plus one for the norm), 1.. = one fused launch per iteration with the tile
-// shapes and prefetch depths below
+// (plus one for the norm), 1.. = fused launches of `nit` iterations with the
+// tile shapes and prefetch depths below (a pass shortened by a check
+// iteration or by the budget runs the same tile with one iteration)
+using FusedLaunch = void (*)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, int, const SorState *, double *);
 struct FusedVariant {
-  int ti, tj, nt, depth, minb;
-  void (*launch)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, int, const SorState *, double *);
+  int ti, tj, nt, depth, minb, nit;
+  FusedLaunch launch, launch1;
 };
 
-template <int TI, int TJ, int NT, int D, int MINB>
+template <int TI, int TJ, int NT, int D, int MINB, int NIT>
 static void launch_fused(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, int check,
                          const SorState *st, double *partials) {
-  hipLaunchKernelGGL((k_sor_fused<double, TI, TJ, NT, D, MINB>), g, blk, 0, s, a, gx, gy, check, st, partials);
+  hipLaunchKernelGGL((k_sor_fused<double, TI, TJ, NT, D, NIT, MINB>), g, blk, 0, s, a, gx, gy, check, st,
+                     partials);
 }
 
-// {tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers must allow}
+// FV(tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers
+// must allow, iterations per pass)
+#define FV(TI, TJ, NT, D, MINB, NIT) \
+  { TI, TJ, NT, D, MINB, NIT, launch_fused<TI, TJ, NT, D, MINB, NIT>, launch_fused<TI, TJ, NT, D, MINB, 1> }
 static const FusedVariant kFused[] = {
-    {0, 0, 0, 0, 0, nullptr},
-    {64, 16, 256, 1, 1, launch_fused<64, 16, 256, 1, 1>},
-    {64, 8, 256, 1, 1, launch_fused<64, 8, 256, 1, 1>},
-    {64, 8, 256, 2, 1, launch_fused<64, 8, 256, 2, 1>},
-    {128, 16, 512, 1, 1, launch_fused<128, 16, 512, 1, 1>},
-    {128, 16, 512, 2, 1, launch_fused<128, 16, 512, 2, 1>},
-    {64, 32, 512, 1, 1, launch_fused<64, 32, 512, 1, 1>},
-    {64, 32, 512, 2, 1, launch_fused<64, 32, 512, 2, 1>},
-    {32, 16, 256, 2, 1, launch_fused<32, 16, 256, 2, 1>},
-    {128, 8, 256, 2, 1, launch_fused<128, 8, 256, 2, 1>},
-    {64, 16, 256, 1, 3, launch_fused<64, 16, 256, 1, 3>},
-    {64, 16, 256, 2, 3, launch_fused<64, 16, 256, 2, 3>},
-    {64, 8, 256, 1, 4, launch_fused<64, 8, 256, 1, 4>},
-    {64, 8, 256, 2, 4, launch_fused<64, 8, 256, 2, 4>},
-    {32, 16, 256, 1, 4, launch_fused<32, 16, 256, 1, 4>},
-    {128, 8, 256, 1, 2, launch_fused<128, 8, 256, 1, 2>},
+    {0, 0, 0, 0, 0, 1, nullptr, nullptr},
+    FV(64, 16, 256, 1, 1, 1),
+    FV(64, 8, 256, 1, 1, 1),
+    FV(64, 8, 256, 2, 1, 1),
+    FV(128, 16, 512, 1, 1, 1),
+    FV(128, 16, 512, 2, 1, 1),
+    FV(64, 32, 512, 1, 1, 1),
+    FV(64, 32, 512, 2, 1, 1),
+    FV(128, 8, 256, 2, 1, 1),
+    FV(64, 8, 256, 1, 4, 1),
+    FV(64, 16, 256, 1, 1, 2),
+    FV(64, 16, 512, 1, 1, 2),
+    FV(64, 32, 512, 1, 1, 2),
+    FV(64, 8, 256, 1, 1, 2),
+    FV(32, 16, 256, 1, 1, 2),
+    FV(64, 8, 256, 1, 2, 2),
+    FV(64, 32, 1024, 1, 1, 1),
+    FV(64, 32, 1024, 1, 1, 2),
+    FV(128, 16, 1024, 1, 1, 1),
+    FV(64, 16, 1024, 1, 1, 1),
+    FV(128, 32, 1024, 1, 1, 1),
 };
+#undef FV
 constexpr int kNumSorVariants = (int)(sizeof(kFused) / sizeof(kFused[0]));
 static int g_sor_variant = 7;
 static int g_sor_planes = 32;
@@ -443,11 +457,15 @@ static void fused_grid(const Block &b, const FusedVariant &v, int planes, int &g
   gz = (int)div_up(b.d.nz, planes);
 }
 
-// launches `first` .. first+n-1 of this call (even: pc -> pc2, odd: pc2 -> pc);
-// launch q runs iteration start + q + 1, launch total-1 is the last the call may run
-static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int planes, long long start,
-                                    long long total, long long first, long long n, int64_t pc_ita_max,
-                                    double epsi) {
+// Passes of a call: pass q reads pc (q even) or pc2 and writes the other.
+// A pass runs up to `nit` iterations but always ends at a check iteration
+// (every fifth, where the reference tests the update norm) and at the end of
+// the call's budget of `total` iterations, so the stopping test sees exactly
+// the reference's iterates.  Enqueues passes until `chunk` more iterations
+// are queued or the budget is spent; `done_iters` and `q` carry over.
+static int enqueue_fused_passes(Block &b, SorState *st, int variant, int planes, long long start, long long total,
+                                long long chunk, long long &done_iters, long long &q, int64_t pc_ita_max,
+                                double epsi) {
   hipStream_t s = rt().stream;
   const FusedVariant &v = kFused[variant];
   int gx, gy, gz;
@@ -464,15 +482,23 @@ static int enqueue_fused_iterations(Block &b, SorState *st, int variant, int pla
   a.Ay = b.A[1];
   a.Az = b.A[2];
   a.omega = b.omega;
-  for (long long q = first; q < first + n; ++q) {
+  const long long stop = done_iters + chunk < total ? done_iters + chunk : total;
+  while (done_iters < stop) {
+    const long long it = start + done_iters + 1;
+    const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
+    long long L = v.nit;
+    if (L > total - done_iters) L = total - done_iters;
+    if (L > to_check) L = to_check;
+    const long long last_it = it + L - 1;
+    const int check = last_it % 5 == 0;
     a.src = (q & 1) ? b.pc2 : b.f[BF_PC];
     a.dst = (q & 1) ? b.f[BF_PC] : b.pc2;
-    const long long it = start + q + 1;
-    const int check = it % 5 == 0;
-    v.launch(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy, check, st, b.partials);
-    if (check || q == total - 1)
-      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, it, start, check, (long long)pc_ita_max,
-                         epsi, b.partials, nwg);
+    (L == v.nit ? v.launch : v.launch1)(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy, check, st, b.partials);
+    done_iters += L;
+    ++q;
+    if (check || done_iters == total)
+      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, check,
+                         (long long)pc_ita_max, epsi, b.partials, nwg);
   }
   BCFD_CHECK_LAUNCH();
   return 0;
@@ -525,10 +551,15 @@ static int sor_finish(Block &b, const SorState *st, int variant) {
   return 0;
 }
 
+// queue about `chunk` more iterations of the call (see enqueue_fused_passes)
 static int enqueue_iterations(Block &b, SorState *st, int variant, int planes, long long start, long long total,
-                              long long first, long long n, int64_t pc_ita_max, double epsi) {
-  return variant > 0 ? enqueue_fused_iterations(b, st, variant, planes, start, total, first, n, pc_ita_max, epsi)
-                     : enqueue_split_iterations(b, st, n, pc_ita_max, epsi);
+                              long long chunk, long long &done_iters, long long &q, int64_t pc_ita_max,
+                              double epsi) {
+  if (variant > 0)
+    return enqueue_fused_passes(b, st, variant, planes, start, total, chunk, done_iters, q, pc_ita_max, epsi);
+  const long long n = chunk < total - done_iters ? chunk : total - done_iters;
+  done_iters += n;
+  return enqueue_split_iterations(b, st, n, pc_ita_max, epsi);
 }
 
 int sor_timing_on() { return g_tm.on ? 1 : 0; }
@@ -575,13 +606,11 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   if (remaining < 1) remaining = 1;
   const long long total = remaining;
   const long long chunk = 50;
-  long long launched = 0;
-  while (remaining > 0) {
-    long long n = remaining < chunk ? remaining : chunk;
-    if (enqueue_iterations(b, b.sor, variant, planes, start, total, launched, n, pc_ita_max, epsi)) return -1;
-    launched += n;
-    remaining -= n;
-    if (remaining <= 0) break;
+  long long done_iters = 0, q = 0;
+  while (done_iters < total) {
+    if (enqueue_iterations(b, b.sor, variant, planes, start, total, chunk, done_iters, q, pc_ita_max, epsi))
+      return -1;
+    if (done_iters >= total) break;
     BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
     BCFD_HIP(hipStreamSynchronize(s));
     if (h->done) break;
@@ -610,7 +639,10 @@ int sor_time(Block &b, int variant, int planes, int iters, double dt, double *ms
          hipEventRecord(e0, s) != hipSuccess;
   }
   // pcItaMax out of reach and a negative tolerance: exactly `iters` iterations
-  if (!rc) rc = enqueue_iterations(b, g_probe_state, variant, planes, 0, iters, 0, iters, (int64_t)1 << 62, -1.0);
+  long long done_iters = 0, q = 0;
+  if (!rc)
+    rc = enqueue_iterations(b, g_probe_state, variant, planes, 0, iters, iters, done_iters, q, (int64_t)1 << 62,
+                            -1.0);
   float t = 0.f;
   if (!rc) rc = hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&t, e0, e1) != hipSuccess;
@@ -741,16 +773,18 @@ int biocfd_sor_config(int variant, int planes_per_workgroup) {
   return 0;
 }
 
-int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth, int *min_wg_per_cu) {
+int biocfd_sor_variant_info(int variant, int info[6]) {
   if (variant < 0 || variant >= kNumSorVariants) {
-    set_error("sor_variant_shape: variant %d outside 0..%d", variant, kNumSorVariants - 1);
+    set_error("sor_variant_info: variant %d outside 0..%d", variant, kNumSorVariants - 1);
     return -1;
   }
-  *tile_i = kFused[variant].ti;
-  *tile_j = kFused[variant].tj;
-  *threads = kFused[variant].nt;
-  *depth = kFused[variant].depth;
-  *min_wg_per_cu = kFused[variant].minb;
+  const FusedVariant &v = kFused[variant];
+  info[0] = v.ti;
+  info[1] = v.tj;
+  info[2] = v.nt;
+  info[3] = v.depth;
+  info[4] = v.minb;
+  info[5] = v.nit;
   return 0;
 }
 

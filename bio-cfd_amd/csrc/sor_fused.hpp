@@ -1,25 +1,30 @@
-// One full red-black SOR iteration (REDBLACKSOR_linear, PcorVcor.F90:313-349)
-// per launch, streaming HBM once: each workgroup owns a TI x TJ column of
-// interior cells over KS planes and marches it in k with the pc planes
-// staged in LDS.
+// NIT (1 or 2) full red-black SOR iterations (REDBLACKSOR_linear,
+// PcorVcor.F90:313-349) per launch, streaming HBM once: each workgroup owns a
+// TI x TJ column of interior cells over KS planes and marches it in k with the
+// pc planes staged in a ring of 2*NIT+2 LDS slots.
 //
-// Step s of the march (planes in a ring of four LDS slots), phases separated
-// by workgroup barriers:
+// Step s of the march, phases separated by workgroup barriers:
 //   A: store plane s+1 (loaded into registers D steps earlier), issue the
-//      loads of plane s+1+D, of the b/dt values step s+1 needs and of the
-//      Acz row of plane s+1 (Acx, Acy of the tile sit in LDS for the march);
-//   B: red update of plane s on the tile plus a one-cell halo (old black);
-//   C: black update of plane s-1 on the tile (new red of planes s-2..s) and
-//      write-back of plane s-1 of the tile, both colours, to the destination.
-// The halo red values are recomputed by every tile that needs them from the
-// same old black values, so they equal the owner's.  Source and destination
-// arrays alternate between iterations (the old black values of a tile's halo
-// must survive until every neighbour has read them).
+//      loads of plane s+1+D, of the rhs values step s+1 needs and of the Acz
+//      row of plane s+1 (Acx, Acy of the tile sit in LDS for the march);
+//   p = 0 .. 2*NIT-1: update colour red (p even) or black (p odd) of
+//      iteration p/2+1 on plane s-p over the tile plus a halo of
+//      2*NIT-1-p cells (x, y and z); the last phase (black of the last
+//      iteration, no halo) writes plane s-p of the tile, both colours, to
+//      the destination.
+// With NIT = 1: red on plane s (tile + 1), black on plane s-1 (tile).
+// Every phase reads the other colour's values of the stage it needs, in
+// place in LDS: a red update of iteration t reads black of iteration t-1 on
+// planes that black of iteration t has not reached yet, and so on.  Halo
+// values are recomputed by every tile that needs them from the same inputs,
+// so they equal the owner's.  Source and destination arrays alternate between
+// launches (the old values of a tile's halo must survive until every
+// neighbour has read them).
 //
-// Slot layout: per plane and colour a (TJ+4) x (TI+4)/2 array holding slot
-// row y = j - jL, half-column h = (i - iL) >> 1, where (iL, jL) is the tile
-// origin minus two.  TI and TJ are even, so the colour of slot cell (x, y) in
-// plane k is (x + y + k) & 1 (1 = red, as (i + j + k) odd).
+// Slot layout: per plane and colour a (TJ+2H) x (TI+2H)/2 array (H = 2*NIT)
+// holding slot row y = j - jL, half-column h = (i - iL) >> 1, where (iL, jL)
+// is the tile origin minus H.  TI, TJ and H are even, so the colour of slot
+// cell (x, y) in plane k is (x + y + k) & 1 (1 = red, as (i + j + k) odd).
 //
 // Index work per step is 32-bit: every thread keeps the slot positions of its
 // cells (computed once) and offsets within one plane; the plane's base
@@ -104,27 +109,36 @@ __host__ __device__ inline Coef3T<Real> coef_at(const double *A, int n, int q) {
   return Coef3T<Real>{Real(A[q - 2]), Real(A[(q - 2) + n]), Real(A[(q - 2) + 2 * n])};
 }
 
-template <typename Real, int TI, int TJ, int NT, int D>
+template <typename Real, int TI, int TJ, int NT, int D, int NIT = 1>
 struct SorTile {
   static_assert(TI % 2 == 0 && TJ % 2 == 0, "tile sides must be even");
   static_assert(TI <= 500 && TJ <= 250, "slot positions are packed in 8 bits");
   static_assert(D == 1 || D == 2, "prefetch depth is 1 or 2 planes");
-  static constexpr int PW = TI + 4;                 // slot row width in cells
-  static constexpr int SH = TJ + 4;                 // slot rows
+  static_assert(NIT == 1 || NIT == 2, "one or two iterations per pass");
+  static constexpr int H = 2 * NIT;                 // slot halo (cells) around the tile
+  static constexpr int NPH = 2 * NIT;               // update phases per step: red, black (, red, black)
+  static constexpr int R = 2 * NIT + 2;             // planes in the LDS ring
+  static constexpr int PW = TI + 2 * H;             // slot row width in cells
+  static constexpr int SH = TJ + 2 * H;             // slot rows
   static constexpr int SW = PW / 2;                 // half-row per colour
   static constexpr int PLANE = SH * PW;             // cells of one slot plane
   static constexpr int NL = (PLANE + NT - 1) / NT;  // plane loads per thread
-  static constexpr int RW = TI / 2 + 1;             // red cells per row of the tile + halo
-  static constexpr int NRC = (TJ + 2) * RW;         // red cells per plane
-  static constexpr int NR = (NRC + NT - 1) / NT;
-  static constexpr int HB = TI / 2;                 // red/black cell pairs per tile row
-  static constexpr int NBC = TJ * HB;               // pairs per plane
-  static constexpr int NB = (NBC + NT - 1) / NT;
-  static constexpr int SLOTS = 4 * 2 * SH * SW;
-  static constexpr int CXW = TI + 2, CYW = TJ + 2;  // staged coefficients per axis column
+  static constexpr int SLOTS = R * 2 * SH * SW;
+  static constexpr int CXW = PW - 2, CYW = SH - 2;  // staged coefficients: slot x (y) = 1 .. PW-2 (SH-2)
   static constexpr int LDS_DOUBLES = SLOTS + 3 * CXW + 3 * CYW;
 
-  // where a workgroup's tile sits
+  // phase p (0 .. NPH-1) updates colour red (p even) or black (p odd) of
+  // iteration p/2 + 1 on plane s - p over the tile plus a halo of hp(p)
+  // cells in x, y and z
+  __host__ __device__ static constexpr int hp(int p) { return 2 * NIT - 1 - p; }
+  __host__ __device__ static constexpr int RWp(int p) { return TI / 2 + hp(p); }   // cells per row
+  __host__ __device__ static constexpr int NCp(int p) { return (TJ + 2 * hp(p)) * RWp(p); }
+  __host__ __device__ static constexpr int NQp(int p) { return (NCp(p) + NT - 1) / NT; }
+  static constexpr int NQ = (NCp(0) + NT - 1) / NT;  // the largest phase
+
+  // where a workgroup's tile sits: slot cell (x, y) of plane k is cell
+  // (iL + x, jL + y, k); the tile's own cells have x, y in [H, H + TI (TJ))
+  // and k in [k0, k1]
   struct Tile {
     int iL, jL, k0, k1;
   };
@@ -138,17 +152,18 @@ struct SorTile {
   // per-thread registers of the march
   struct Thread {
     Real pcn[D][NL];          // planes in flight
-    Real rn[NR], rc[NR];      // rhs of the red cells: next step, this step
-    Real bn[NB], bc[NB];      // rhs of the black cells: next step, this step
+    Real rc[NPH][NQ];         // rhs of each phase's cells for this step
+    Real rn[NIT == 1 ? NPH : 1][NIT == 1 ? NQ : 1];  // NIT = 1: the next step's, loaded in phase A
     int ld_off[NL];           // in-plane offset of the loaded slot cell, -1 outside the array
     int ld_lds[NL];           // 2 * (y * SW + (x >> 1)) + ((x + y) & 1), -1 past the slot
-    int red_yr[NR];           // y << 8 | r: row y, r-th red cell of the row; -1 past the end
-    int pair_yh[NB];          // y << 8 | h of a red/black pair; -1 past the end
-    C3 czm, cz0, czp;         // Acz of planes s-1, s, s+1
+    int pos[NPH][NQ];         // y << 8 | r: row y (from the phase's first row), r-th cell; -1 past the end
+    C3 cz[NPH];               // Acz of plane s - p for phase p
+    C3 czp;                   // Acz of plane s + 1
   };
 
+  __host__ __device__ static int slot_index(int k) { return (k + 2 * R) % R; }
   __host__ __device__ static Real *slot(Real *lds, int k, int colour) {
-    return lds + ((k & 3) * 2 + colour) * (SH * SW);
+    return lds + (slot_index(k) * 2 + colour) * (SH * SW);
   }
 
   // tile coordinates of linear workgroup L of n: consecutive tiles go to one
@@ -162,11 +177,19 @@ struct SorTile {
   __host__ __device__ static Tile tile_of(const Args &a, int t, int gx, int gy) {
     const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
     Tile T;
-    T.iL = TI * tx;
-    T.jL = TJ * ty;
+    T.iL = 2 + TI * tx - H;
+    T.jL = 2 + TJ * ty - H;
     T.k0 = 2 + a.KS * tz;
     T.k1 = T.k0 + a.KS - 1 < a.d.nz + 1 ? T.k0 + a.KS - 1 : a.d.nz + 1;
     return T;
+  }
+
+  // the march runs steps s = first_step .. last_step
+  __host__ __device__ static int first_step(const Tile &T) { return T.k0 - (2 * NIT - 1); }
+  __host__ __device__ static int last_step(const Tile &T) { return T.k1 + (2 * NIT - 1); }
+  // phase p works on plane s - p at step s when that plane is in its range
+  __host__ __device__ static bool active(const Tile &T, int p, int s) {
+    return s - p >= T.k0 - hp(p) && s - p <= T.k1 + hp(p);
   }
 
   __host__ __device__ static C3 cz_of(const Args &a, int k) {
@@ -186,28 +209,21 @@ struct SorTile {
         if (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY) th.ld_off[q] = (i - 1) + a.d.NX * (j - 1);
       }
     }
-    for (int q = 0; q < NR; ++q) {
-      const int e = tid + q * NT;
-      th.red_yr[q] = -1;
-      if (e < NRC) {
-        const int y = 1 + e / RW;
-        th.red_yr[q] = (y << 8) | (e - (y - 1) * RW);
+    for (int p = 0; p < NPH; ++p)
+      for (int q = 0; q < NQ; ++q) {
+        const int e = tid + q * NT;
+        th.pos[p][q] = -1;
+        if (q < NQp(p) && e < NCp(p)) {
+          const int yy = e / RWp(p);
+          th.pos[p][q] = (yy << 8) | (e - yy * RWp(p));
+        }
       }
-    }
-    for (int q = 0; q < NB; ++q) {
-      const int e = tid + q * NT;
-      th.pair_yh[q] = -1;
-      if (e < NBC) {
-        const int y = 2 + e / HB;
-        th.pair_yh[q] = (y << 8) | (1 + (e - (y - 2) * HB));
-      }
-    }
-    th.czm = cz_of(a, T.k0 - 2);
-    th.cz0 = cz_of(a, T.k0 - 1);
-    th.czp = th.cz0;
+    const int s0 = first_step(T);
+    for (int p = 0; p < NPH; ++p) th.cz[p] = cz_of(a, s0 - 1 - p);
+    th.czp = cz_of(a, s0);
   }
 
-  // Acx over slot columns x = 1..TI+2 and Acy over slot rows y = 1..TJ+2
+  // Acx over slot columns x = 1..PW-2 and Acy over slot rows y = 1..SH-2
   // (zero outside the interior, where no cell is updated)
   __host__ __device__ static void load_coef(const Args &a, const Tile &T, Real *lds, int tid) {
     Real *cx = lds + SLOTS, *cy = cx + 3 * CXW;
@@ -245,22 +261,26 @@ struct SorTile {
     }
   }
 
-  // red cell of packed position yr in plane s: slot row y, half-column h, x = 2h + off
-  __host__ __device__ static void red_cell(int yr, int s, int &y, int &h, int &x) {
-    y = yr >> 8;
-    const int off = (1 - y - s) & 1;
-    h = 1 - off + (yr & 255);
-    x = 2 * h + off;
+  // cell of packed position v in phase p on plane k: slot row y, column x
+  // (colour red for even p, black for odd p)
+  __host__ __device__ static void cell_of(int p, int v, int k, int &y, int &x) {
+    const int lo = H - hp(p);  // first slot row and column of the phase's region
+    y = lo + (v >> 8);
+    const int colour = (p & 1) ? 0 : 1;
+    const int off = (colour - y - k) & 1;  // parity of the colour's columns in this row
+    x = lo + ((off - lo) & 1) + 2 * (v & 255);
   }
 
-  __host__ __device__ static void load_bdt_red(const Args &a, const Tile &T, const Thread &th, int s, Real *r) {
-    const bool kin = s >= 2 && s <= a.d.nz + 1;
-    const Real *P = a.bred + (int64_t)(kin ? s - 1 : 0) * ((int64_t)a.NXh * a.d.NY);
-    for (int q = 0; q < NR; ++q) {
+  __host__ __device__ static void load_rhs(const Args &a, const Tile &T, const Thread &th, int p, int k,
+                                           Real *r) {
+    const bool kin = k >= 2 && k <= a.d.nz + 1;
+    const Real *B = (p & 1) ? a.bblack : a.bred;
+    const Real *P = B + (int64_t)(kin ? k - 1 : 0) * ((int64_t)a.NXh * a.d.NY);
+    for (int q = 0; q < NQp(p); ++q) {
       Real v = not_fluid_marker<Real>();
-      if (kin && th.red_yr[q] >= 0) {
-        int y, h, x;
-        red_cell(th.red_yr[q], s, y, h, x);
+      if (kin && th.pos[p][q] >= 0) {
+        int y, x;
+        cell_of(p, th.pos[p][q], k, y, x);
         const int i = T.iL + x, j = T.jL + y;
         if (i >= 2 && i <= a.d.nx + 1 && j >= 2 && j <= a.d.ny + 1) v = P[((i - 1) >> 1) + a.NXh * (j - 1)];
       }
@@ -268,116 +288,96 @@ struct SorTile {
     }
   }
 
-  // pair of packed position yh in plane k: the black cell is at x = 2h + off,
-  // the red one at x = 2h + 1 - off
-  __host__ __device__ static void pair_cell(int yh, int k, int &y, int &h, int &off) {
-    y = yh >> 8;
-    h = yh & 255;
-    off = (0 - y - k) & 1;
-  }
-
-  __host__ __device__ static void load_bdt_black(const Args &a, const Tile &T, const Thread &th, int k, Real *r) {
-    const Real *P = a.bblack + (int64_t)(k - 1) * ((int64_t)a.NXh * a.d.NY);
-    for (int q = 0; q < NB; ++q) {
-      Real v = not_fluid_marker<Real>();
-      if (th.pair_yh[q] >= 0) {
-        int y, h, off;
-        pair_cell(th.pair_yh[q], k, y, h, off);
-        const int i = T.iL + 2 * h + off, j = T.jL + y;
-        if (i <= a.d.nx + 1 && j <= a.d.ny + 1) v = P[((i - 1) >> 1) + a.NXh * (j - 1)];
-      }
-      r[q] = v;
-    }
-  }
-
-  // red update of plane s (tile + halo); returns the largest |change|
-  __host__ __device__ static double red_phase(const Args &a, Real *lds, const Thread &th, int s) {
-    Real *R = slot(lds, s, 1);
-    const Real *B = slot(lds, s, 0), *Bm = slot(lds, s - 1, 0), *Bp = slot(lds, s + 1, 0);
-    double delta = 0.0;
-    for (int q = 0; q < NR; ++q) {
-      if (th.red_yr[q] < 0 || sor_skip(th.rc[q])) continue;
-      int y, h, x;
-      red_cell(th.red_yr[q], s, y, h, x);
-      const int c = y * SW + h;
-      const Real old = R[c];
-      const Real nw = sor_cell<Real>(Real(a.omega), cx_at(lds, x), cy_at(lds, y), th.cz0, th.rc[q], old,
-                                     B[y * SW + ((x - 1) >> 1)], B[y * SW + ((x + 1) >> 1)], B[c - SW], B[c + SW],
-                                     Bm[c], Bp[c]);
-      R[c] = nw;
-      if (kNorm) {
-        const double dd = fabs(double(nw - old));
-        delta = delta > dd ? delta : dd;
-      }
-    }
-    return delta;
-  }
-
-  // black update of plane k (tile) and write-back of the tile's plane k
-  __host__ __device__ static double black_phase(const Args &a, const Tile &T, Real *lds, const Thread &th, int k) {
-    const Real *R = slot(lds, k, 1), *Rm = slot(lds, k - 1, 1), *Rp = slot(lds, k + 1, 1);
-    const Real *B = slot(lds, k, 0);
+  // phase p at step s: update its colour on plane k = s - p (in LDS), and in
+  // the last phase write plane k of the tile, both colours, to dst; returns
+  // the largest |change| when norm is set
+  __host__ __device__ static double update(const Args &a, const Tile &T, Real *lds, const Thread &th, int p, int s,
+                                           bool norm) {
+    const int k = s - p;
+    const int own = (p & 1) ? 0 : 1, other = 1 - own;
+    Real *O = slot(lds, k, own);
+    const Real *N = slot(lds, k, other), *Nm = slot(lds, k - 1, other), *Np = slot(lds, k + 1, other);
+    const bool last = p == NPH - 1;
     Real *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
     double delta = 0.0;
-    for (int q = 0; q < NB; ++q) {
-      if (th.pair_yh[q] < 0) continue;
-      int y, h, off;
-      pair_cell(th.pair_yh[q], k, y, h, off);
-      const int j = T.jL + y;
-      if (j > a.d.ny + 1) continue;
-      const int x = 2 * h + off, i = T.iL + x, ir = T.iL + 2 * h + 1 - off;
-      const int c = y * SW + h;
-      const int row = a.d.NX * (j - 1) - 1;
-      if (i <= a.d.nx + 1) {
-        const Real old = B[c];
-        Real nw = old;
-        if (!sor_skip(th.bc[q])) {
-          nw = sor_cell<Real>(Real(a.omega), cx_at(lds, x), cy_at(lds, y), th.czm, th.bc[q], old,
-                              R[y * SW + ((x - 1) >> 1)], R[y * SW + ((x + 1) >> 1)], R[c - SW], R[c + SW], Rm[c],
-                              Rp[c]);
-          if (kNorm) {
-            const double dd = fabs(double(nw - old));
-            delta = delta > dd ? delta : dd;
-          }
+    for (int q = 0; q < NQp(p); ++q) {
+      if (th.pos[p][q] < 0) continue;
+      int y, x;
+      cell_of(p, th.pos[p][q], k, y, x);
+      const int c = y * SW + (x >> 1);
+      const Real old = O[c];
+      Real nw = old;
+      if (!sor_skip(th.rc[p][q])) {
+        nw = sor_cell<Real>(Real(a.omega), cx_at(lds, x), cy_at(lds, y), th.cz[p], th.rc[p][q], old,
+                            N[y * SW + ((x - 1) >> 1)], N[y * SW + ((x + 1) >> 1)], N[c - SW], N[c + SW], Nm[c],
+                            Np[c]);
+        if (kNorm && norm) {
+          const double dd = fabs(double(nw - old));
+          delta = delta > dd ? delta : dd;
         }
-        P[row + i] = nw;
       }
-      if (ir <= a.d.nx + 1) P[row + ir] = R[c];
+      if (!last) {
+        O[c] = nw;
+      } else {
+        // black cell x and its red partner x ^ 1 (same half-column)
+        const int i = T.iL + x, ir = T.iL + (x ^ 1), j = T.jL + y;
+        if (j <= a.d.ny + 1) {
+          const int row = a.d.NX * (j - 1) - 1;
+          if (i <= a.d.nx + 1) P[row + i] = nw;
+          if (ir <= a.d.nx + 1) P[row + ir] = N[c];
+        }
+      }
     }
     return delta;
   }
 
-  // ---- the march: prologue, then per step s = k0-1 .. k1+1 the phases A | B | C ----
+  // ---- the march: prologue, then per step s the phases A | 0 | 1 | ... | NPH-1 ----
   __host__ __device__ static void prologue(const Args &a, const Tile &T, Real *lds, Thread &th) {
+    const int s0 = first_step(T);
     Real *r = th.pcn[0];
-    load_plane(a, th, T.k0 - 2, r);
-    store_plane(lds, th, T.k0 - 2, r);
-    load_plane(a, th, T.k0 - 1, r);
-    store_plane(lds, th, T.k0 - 1, r);
-    for (int d = 0; d < D; ++d) load_plane(a, th, T.k0 + d, th.pcn[d]);
-    load_bdt_red(a, T, th, T.k0 - 1, th.rn);
-    for (int q = 0; q < NB; ++q) th.bn[q] = Real(0);
+    load_plane(a, th, s0 - 1, r);
+    store_plane(lds, th, s0 - 1, r);
+    load_plane(a, th, s0, r);
+    store_plane(lds, th, s0, r);
+    for (int d = 0; d < D; ++d) load_plane(a, th, s0 + 1 + d, th.pcn[d]);
+    // phase p first works on plane k0 - hp(p) (at step k0 - hp(p) + p)
+    if (NIT == 1) {
+      // the first step's rhs of phase p go to rn (phase A moves them to rc)
+      for (int p = 0; p < NPH; ++p) {
+        if (active(T, p, s0))
+          load_rhs(a, T, th, p, s0 - p, th.rn[p]);
+        else
+          for (int q = 0; q < NQ; ++q) th.rn[p][q] = Real(0);
+      }
+    } else {
+      for (int p = 0; p < NPH; ++p) load_rhs(a, T, th, p, T.k0 - hp(p), th.rc[p]);
+    }
   }
 
-  // buf holds plane s+1 on entry and plane s+1+D on exit
+  // phase A of step s: buf holds plane s+1 on entry and plane s+1+D on exit;
+  // on entry cz[p] = Acz(s-1-p) and czp = Acz(s), on exit cz[p] = Acz(s-p)
   __host__ __device__ static void phase_a(const Args &a, const Tile &T, Real *lds, Thread &th, int s, Real *buf) {
     store_plane(lds, th, s + 1, buf);
-    for (int q = 0; q < NR; ++q) th.rc[q] = th.rn[q];
-    for (int q = 0; q < NB; ++q) th.bc[q] = th.bn[q];
-    if (s + 1 + D <= T.k1 + 2) load_plane(a, th, s + 1 + D, buf);
-    if (s + 1 <= T.k1 + 1) load_bdt_red(a, T, th, s + 1, th.rn);
-    if (s >= T.k0 && s <= T.k1) load_bdt_black(a, T, th, s, th.bn);
+    if (NIT == 1)
+      for (int p = 0; p < NPH; ++p)
+        for (int q = 0; q < NQ; ++q) th.rc[p][q] = th.rn[NIT == 1 ? p : 0][NIT == 1 ? q : 0];
+    if (s + 1 + D <= last_step(T) + 1) load_plane(a, th, s + 1 + D, buf);
+    if (NIT == 1)
+      for (int p = 0; p < NPH; ++p)
+        if (s + 1 <= last_step(T) && active(T, p, s + 1)) load_rhs(a, T, th, p, s + 1 - p, th.rn[NIT == 1 ? p : 0]);
+    for (int p = NPH - 1; p > 0; --p) th.cz[p] = th.cz[p - 1];
+    th.cz[0] = th.czp;
     th.czp = cz_of(a, s + 1);
   }
 
-  __host__ __device__ static double phase_b(const Args &a, Real *lds, const Thread &th, int s) {
-    return red_phase(a, lds, th, s);
-  }
-
-  __host__ __device__ static double phase_c(const Args &a, const Tile &T, Real *lds, Thread &th, int s) {
-    const double d = (s - 1 >= T.k0) ? black_phase(a, T, lds, th, s - 1) : 0.0;
-    th.czm = th.cz0;
-    th.cz0 = th.czp;
+  // phase p of step s (after phase A and phases 0 .. p-1).  With NIT = 1 the
+  // next step's rhs were loaded in phase A (double-buffered); with NIT = 2
+  // (twice the rhs registers) each phase reloads its own a step ahead of use
+  __host__ __device__ static double phase_u(const Args &a, const Tile &T, Real *lds, Thread &th, int p, int s,
+                                            bool norm) {
+    if (!active(T, p, s)) return 0.0;
+    const double d = update(a, T, lds, th, p, s, norm);
+    if (NIT > 1 && active(T, p, s + 1)) load_rhs(a, T, th, p, s + 1 - p, th.rc[p]);
     return d;
   }
 };

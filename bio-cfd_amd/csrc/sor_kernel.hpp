@@ -9,16 +9,18 @@
 
 namespace bcfd {
 
-// One red-black iteration per launch on a 1-D grid of gx*gy*gz tiles; the
+// NIT red-black iterations per launch on a 1-D grid of gx*gy*gz tiles; the
 // tile schedule is described with SorTile.  Non-fluid cells of the tile are
-// copied through, so src and dst agree on every cell the iteration leaves.
-// On check iterations (every fifth, numbered by the host) each workgroup
-// posts its largest update to partials.
+// copied through, so src and dst agree on every cell the launch leaves.
+// When `check` is set (the launch's last iteration is a check iteration,
+// every fifth, numbered by the host) each workgroup posts the largest update
+// of that iteration to partials.
 // MINB: workgroups per CU the register allocation must leave room for
-template <typename Real, int TI, int TJ, int NT, int D, int MINB>
+template <typename Real, int TI, int TJ, int NT, int D, int NIT, int MINB>
 __global__ void __launch_bounds__(NT, MINB) k_sor_fused(FusedArgsT<Real> a, int gx, int gy, int check,
-                                                  const SorState *__restrict__ st, double *__restrict__ partials) {
-  using T = SorTile<Real, TI, TJ, NT, D>;
+                                                        const SorState *__restrict__ st,
+                                                        double *__restrict__ partials) {
+  using T = SorTile<Real, TI, TJ, NT, D, NIT>;
   if (st->done) return;
   __shared__ Real lds[T::LDS_DOUBLES];
   const int tid = threadIdx.x;
@@ -28,24 +30,19 @@ __global__ void __launch_bounds__(NT, MINB) k_sor_fused(FusedArgsT<Real> a, int 
   T::load_coef(a, tile, lds, tid);
   T::prologue(a, tile, lds, th);
   double delta = 0.0;
-  for (int s = tile.k0 - 1; s <= tile.k1 + 1; s += D) {
-    __syncthreads();
-    T::phase_a(a, tile, lds, th, s, th.pcn[0]);
-    __syncthreads();
-    const double dr = T::phase_b(a, lds, th, s);
-    delta = delta > dr ? delta : dr;
-    __syncthreads();
-    const double db = T::phase_c(a, tile, lds, th, s);
-    delta = delta > db ? delta : db;
-    if (D > 1 && s + 1 <= tile.k1 + 1) {
+  const int s1 = T::last_step(tile);
+  for (int s = T::first_step(tile); s <= s1; s += D) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (d > 0 && s + d > s1) break;
       __syncthreads();
-      T::phase_a(a, tile, lds, th, s + 1, th.pcn[D - 1]);
-      __syncthreads();
-      const double dr2 = T::phase_b(a, lds, th, s + 1);
-      delta = delta > dr2 ? delta : dr2;
-      __syncthreads();
-      const double db2 = T::phase_c(a, tile, lds, th, s + 1);
-      delta = delta > db2 ? delta : db2;
+      T::phase_a(a, tile, lds, th, s + d, th.pcn[d]);
+#pragma unroll
+      for (int p = 0; p < T::NPH; ++p) {
+        __syncthreads();
+        const double dp = T::phase_u(a, tile, lds, th, p, s + d, check && p >= T::NPH - 2);
+        delta = delta > dp ? delta : dp;
+      }
     }
   }
   if (check) wg_max_store(delta, partials);

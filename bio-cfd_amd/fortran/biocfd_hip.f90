@@ -315,15 +315,10 @@ module biocfd_hip
       integer(c_int), value :: planes_per_workgroup
     end function
 
-    integer(c_int) function biocfd_sor_variant_shape(variant, tile_i, tile_j, threads, depth, &
-        min_wg_per_cu) bind(C, name="biocfd_sor_variant_shape")
+    integer(c_int) function biocfd_sor_variant_info(variant, info) bind(C, name="biocfd_sor_variant_info")
       import :: c_int, c_int64_t, c_double, c_ptr
       integer(c_int), value :: variant
-      integer(c_int) :: tile_i(*)
-      integer(c_int) :: tile_j(*)
-      integer(c_int) :: threads(*)
-      integer(c_int) :: depth(*)
-      integer(c_int) :: min_wg_per_cu(*)
+      integer(c_int) :: info(*)
     end function
 
     integer(c_int) function biocfd_sor_time(blk, variant, planes_per_workgroup, iters, dt, &
@@ -351,15 +346,10 @@ module biocfd_hip
       real(c_double), value :: residual_tol
     end function
 
-    integer(c_int) function biocfd_mixed_variant_shape(variant, tile_i, tile_j, threads, depth, &
-        min_wg_per_cu) bind(C, name="biocfd_mixed_variant_shape")
+    integer(c_int) function biocfd_mixed_variant_info(variant, info) bind(C, name="biocfd_mixed_variant_info")
       import :: c_int, c_int64_t, c_double, c_ptr
       integer(c_int), value :: variant
-      integer(c_int) :: tile_i(*)
-      integer(c_int) :: tile_j(*)
-      integer(c_int) :: threads(*)
-      integer(c_int) :: depth(*)
-      integer(c_int) :: min_wg_per_cu(*)
+      integer(c_int) :: info(*)
     end function
 
     integer(c_int) function biocfd_mixed_time(blk, variant, planes_per_workgroup, iters, dt, &

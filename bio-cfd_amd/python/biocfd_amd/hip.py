@@ -75,11 +75,11 @@ SIGNATURES = {
     "biocfd_redblack_sor": [_I, _L, _D, _D],
     "biocfd_correct": [_I, _D],
     "biocfd_sor_config": [_I, _I],
-    "biocfd_sor_variant_shape": [_I, _P, _P, _P, _P, _P],
+    "biocfd_sor_variant_info": [_I, _P],
     "biocfd_sor_time": [_I, _I, _I, _I, _D, _P],
     "biocfd_set_precision": [_I],
     "biocfd_mixed_config": [_I, _I, _I, _D],
-    "biocfd_mixed_variant_shape": [_I, _P, _P, _P, _P, _P],
+    "biocfd_mixed_variant_info": [_I, _P],
     "biocfd_mixed_time": [_I, _I, _I, _I, _D, _P],
     "biocfd_residual_norm": [_I, _D, _P],
     "biocfd_compute_surface_norm": [_I, _P, _P, _P, _L],

@@ -17,19 +17,20 @@ from . import hip as H
 DEFAULT_PLANES = (8, 16, 32, 64)
 
 
-def _shapes(fn):
+def _infos(name):
     out = []
+    info = np.zeros(6, dtype=np.int32)
     v = 0
-    a, b, t, d, m = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
-    while fn(v, C.byref(a), C.byref(b), C.byref(t), C.byref(d), C.byref(m)) == 0:
-        out.append((v, a.value, b.value, t.value, d.value, m.value))
+    while getattr(H.lib(), name)(v, H.ptr(info)) == 0:
+        out.append((v,) + tuple(int(x) for x in info))
         v += 1
     return out
 
 
 def variants():
-    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu)] of the FP64 SOR kernels."""
-    return _shapes(H.lib().biocfd_sor_variant_shape)
+    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu, iters_per_pass)] of the FP64
+    SOR kernels."""
+    return _infos("biocfd_sor_variant_info")
 
 
 def time_variant(block: int, variant: int, planes: int, iters: int, dt: float, repeats: int = 2) -> float:
@@ -47,11 +48,11 @@ def autotune(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PL
     (best_variant, best_planes, table) with table rows
     {"variant", "tile", "threads", "planes", "us_per_iteration"}."""
     table = []
-    for v, ti, tj, nt, depth, minb in variants():
+    for v, ti, tj, nt, depth, minb, nit in variants():
         for p in (planes if v > 0 else (32,)):
             ms = time_variant(block, v, p, iters, dt)
             table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "min_wg_per_cu": minb,
-                          "planes": p, "us_per_iteration": round(1e3 * ms, 2)})
+                          "iters_per_pass": nit, "planes": p, "us_per_iteration": round(1e3 * ms, 2)})
     best = min(table, key=lambda r: r["us_per_iteration"])
     if install:
         H.call("biocfd_sor_config", best["variant"], best["planes"])
@@ -59,8 +60,9 @@ def autotune(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PL
 
 
 def mixed_variants():
-    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu)] of the FP32 smoothing kernels."""
-    return _shapes(H.lib().biocfd_mixed_variant_shape)
+    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu, sweeps_per_pass)] of the FP32
+    smoothing kernels."""
+    return _infos("biocfd_mixed_variant_info")
 
 
 def autotune_mixed(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PLANES, sweeps: int = 4,
@@ -69,14 +71,14 @@ def autotune_mixed(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFA
     current pc as rhs; pc is not changed); return (variant, planes, table)."""
     table = []
     ms = np.zeros(1)
-    for v, ti, tj, nt, depth, minb in mixed_variants():
+    for v, ti, tj, nt, depth, minb, nit in mixed_variants():
         for p in planes:
             best = float("inf")
             for _ in range(2):
                 H.call("biocfd_mixed_time", block, v, p, iters, dt, H.ptr(ms))
                 best = min(best, float(ms[0]) / iters)
             table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "min_wg_per_cu": minb,
-                          "planes": p, "us_per_sweep": round(1e3 * best, 2)})
+                          "sweeps_per_pass": nit, "planes": p, "us_per_sweep": round(1e3 * best, 2)})
     best = min(table, key=lambda r: r["us_per_sweep"])
     if install:
         H.call("biocfd_mixed_config", best["variant"], best["planes"], sweeps, residual_tol)

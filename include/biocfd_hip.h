@@ -123,13 +123,13 @@ int biocfd_correct(int blk, double dt);
  * reference has one loop nest).  variant 0: one launch per colour on
  * colour-split arrays; variant >= 1: one fused red+black launch per iteration
  * over tiles of tile_i x tile_j cells and `planes_per_workgroup` z-planes.
- * Every variant gives the same bits; the choice only changes speed. */
+ * A fused variant may run two iterations per pass (temporal blocking); every
+ * variant gives the same bits, the choice only changes speed. */
 int biocfd_sor_config(int variant, int planes_per_workgroup);
-/* tile shape, threads per workgroup, z-plane prefetch depth and the
- * workgroups per CU its register budget is compiled for (all 0 for
- * variant 0); fails past the last variant */
-int biocfd_sor_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth,
-                             int *min_wg_per_cu);
+/* info = {tile_i, tile_j, threads per workgroup, z-plane prefetch depth,
+ * workgroups per CU its register budget is compiled for, iterations per
+ * pass} of a variant (zeros and 1 for variant 0); fails past the last one */
+int biocfd_sor_variant_info(int variant, int info[6]);
 /* Autotuning probe (no reference counterpart): time `iters` SOR iterations
  * of a variant on block `blk`'s current pc and b (milliseconds, HIP events);
  * pc and the block's SOR state are restored afterwards. */
@@ -145,8 +145,7 @@ int biocfd_sor_time(int blk, int variant, int planes_per_workgroup, int iters, d
 int biocfd_set_precision(int mode);
 int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_correction,
                         double residual_tol);
-int biocfd_mixed_variant_shape(int variant, int *tile_i, int *tile_j, int *threads, int *depth,
-                               int *min_wg_per_cu);
+int biocfd_mixed_variant_info(int variant, int info[6]);
 /* time `iters` FP32 sweeps of a variant on the residual of block `blk`'s pc */
 int biocfd_mixed_time(int blk, int variant, int planes_per_workgroup, int iters, double dt, double *ms);
 /* max over fluid cells of |b/dt - A pc| for block `blk` (FP64 diagnostic) */

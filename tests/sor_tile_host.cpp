@@ -120,12 +120,13 @@ static double inplace_iteration(const Problem<Real> &p, std::vector<Real> &pc) {
   return delta;
 }
 
-// one fused iteration src -> dst, emulating every workgroup and thread: the
-// kernel's step loop with each barrier-separated phase run for all threads
-template <typename Real, int TI, int TJ, int NT, int D>
-static double fused_iteration(const Problem<Real> &p, int KS, const std::vector<Real> &src, std::vector<Real> &dst,
-                              bool zero_src) {
-  using T = SorTile<Real, TI, TJ, NT, D>;
+// one fused launch (NIT iterations) src -> dst, emulating every workgroup and
+// thread: the kernel's step loop with each barrier-separated phase run for all
+// threads; returns the update norm of the launch's last iteration
+template <typename Real, int TI, int TJ, int NT, int D, int NIT>
+static double fused_launch(const Problem<Real> &p, int KS, const std::vector<Real> &src, std::vector<Real> &dst,
+                           bool zero_src) {
+  using T = SorTile<Real, TI, TJ, NT, D, NIT>;
   FusedArgsT<Real> a = args_of(p, KS);
   a.src = src.data();
   a.dst = dst.data();
@@ -136,17 +137,6 @@ static double fused_iteration(const Problem<Real> &p, int KS, const std::vector<
   std::vector<typename T::Thread> th(NT);
   std::vector<int> seen(n, 0);
   double delta = 0.0;
-  auto step = [&](const typename T::Tile &tile, int s, int buf) {
-    for (int tid = 0; tid < NT; ++tid) T::phase_a(a, tile, lds.data(), th[tid], s, th[tid].pcn[buf]);
-    for (int tid = 0; tid < NT; ++tid) {
-      const double d = T::phase_b(a, lds.data(), th[tid], s);
-      delta = delta > d ? delta : d;
-    }
-    for (int tid = 0; tid < NT; ++tid) {
-      const double d = T::phase_c(a, tile, lds.data(), th[tid], s);
-      delta = delta > d ? delta : d;
-    }
-  };
   for (int L = 0; L < n; ++L) {
     const int t = T::xcd_tile(L, n);
     if (t < 0 || t >= n || seen[t]++) {
@@ -160,66 +150,74 @@ static double fused_iteration(const Problem<Real> &p, int KS, const std::vector<
       T::load_coef(a, tile, lds.data(), tid);
       T::prologue(a, tile, lds.data(), th[tid]);
     }
-    for (int s = tile.k0 - 1; s <= tile.k1 + 1; s += D) {
-      step(tile, s, 0);
-      if (D > 1 && s + 1 <= tile.k1 + 1) step(tile, s + 1, D - 1);
-    }
+    const int s1 = T::last_step(tile);
+    for (int s = T::first_step(tile); s <= s1; s += D)
+      for (int d = 0; d < D; ++d) {
+        if (d > 0 && s + d > s1) break;
+        for (int tid = 0; tid < NT; ++tid) T::phase_a(a, tile, lds.data(), th[tid], s + d, th[tid].pcn[d]);
+        for (int ph = 0; ph < T::NPH; ++ph)
+          for (int tid = 0; tid < NT; ++tid) {  // barrier before each phase
+            const double dd = T::phase_u(a, tile, lds.data(), th[tid], ph, s + d, ph >= T::NPH - 2);
+            delta = delta > dd ? delta : dd;
+          }
+      }
   }
   return delta;
 }
 
-// `iters` fused iterations against the in-place sweep; with zero_first the
-// first one starts from a zero iterate without reading it (the first
-// smoothing sweep of a mixed-precision correction)
-template <typename Real, int TI, int TJ, int NT, int D>
-static int run_case(int nx, int ny, int nz, int KS, int iters, bool zero_first) {
+// `launches` fused launches (NIT iterations each) against the in-place
+// sweep; with zero_first the first starts from a zero iterate without reading
+// it (the first smoothing pass of a mixed-precision correction)
+template <typename Real, int TI, int TJ, int NT, int D, int NIT>
+static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_first) {
   Problem<Real> p = make_problem<Real>(nx, ny, nz, 0.1);
   std::vector<Real> ref = p.pc, A = p.pc, B = p.pc;
   if (zero_first) {
     std::fill(ref.begin(), ref.end(), Real(0));
     std::fill(B.begin(), B.end(), Real(0));  // ghosts of the destination hold zeros
-    // A keeps garbage: the first iteration must not read it
+    // A keeps garbage: the first launch must not read it
   }
-  for (int it = 0; it < iters; ++it) {
+  for (int it = 0; it < launches; ++it) {
     const bool z = zero_first && it == 0;
-    const double dref = inplace_iteration(p, ref);
-    const double dfus = (it & 1) ? fused_iteration<Real, TI, TJ, NT, D>(p, KS, B, A, z)
-                                 : fused_iteration<Real, TI, TJ, NT, D>(p, KS, A, B, z);
+    double dref = 0.0;
+    for (int m = 0; m < NIT; ++m) dref = inplace_iteration(p, ref);
+    const double dfus = (it & 1) ? fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, B, A, z)
+                                 : fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, A, B, z);
     const std::vector<Real> &cur = (it & 1) ? A : B;
     const bool norm_ok = sizeof(Real) == sizeof(double) ? dref == dfus : dfus == 0.0;
     if (std::memcmp(cur.data(), ref.data(), sizeof(Real) * ref.size()) != 0 || !norm_ok) {
       int64_t bad = -1;
       for (size_t q = 0; q < ref.size(); ++q)
         if (std::memcmp(&cur[q], &ref[q], sizeof(Real))) { bad = (int64_t)q; break; }
-      std::printf("MISMATCH %s tile %dx%d/%d depth %d grid %dx%dx%d KS=%d iteration %d: first cell %lld, "
+      std::printf("MISMATCH %s tile %dx%d/%d depth %d x%d grid %dx%dx%d KS=%d launch %d: first cell %lld, "
                   "delta %.17g vs %.17g\n",
-                  sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, nx, ny, nz, KS, it + 1, (long long)bad, dref,
-                  dfus);
+                  sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, NIT, nx, ny, nz, KS, it + 1, (long long)bad,
+                  dref, dfus);
       return 1;
     }
-    if (z) std::fill(A.begin(), A.end(), Real(0));  // later sweeps read A: zero ghosts as the runtime keeps them
+    if (z) std::fill(A.begin(), A.end(), Real(0));  // later launches read A: zero ghosts as the runtime keeps them
   }
-  std::printf("ok %s tile %dx%d/%d depth %d grid %dx%dx%d KS=%d iterations=%d%s\n",
-              sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, nx, ny, nz, KS, iters,
+  std::printf("ok %s tile %dx%d/%d depth %d x%d grid %dx%dx%d KS=%d launches=%d%s\n",
+              sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, NIT, nx, ny, nz, KS, launches,
               zero_first ? " (zero first)" : "");
   return 0;
 }
 
-template <typename Real, int TI, int TJ, int NT, int D>
+template <typename Real, int TI, int TJ, int NT, int D, int NIT = 1>
 static int run_shape() {
   int bad = 0;
-  bad += run_case<Real, TI, TJ, NT, D>(37, 21, 13, 4, 3, false);
-  bad += run_case<Real, TI, TJ, NT, D>(TI, TJ, 9, 32, 2, false);
-  bad += run_case<Real, TI, TJ, NT, D>(TI + 6, 2 * TJ + 3, 11, 3, 4, false);
-  bad += run_case<Real, TI, TJ, NT, D>(5, 3, 7, 1, 3, false);
-  bad += run_case<Real, TI, TJ, NT, D>(19, 9, 8, 2, 3, false);
-  bad += run_case<Real, TI, TJ, NT, D>(TI + 6, 2 * TJ + 3, 11, 3, 4, true);
+  bad += run_case<Real, TI, TJ, NT, D, NIT>(37, 21, 13, 4, 3, false);
+  bad += run_case<Real, TI, TJ, NT, D, NIT>(TI, TJ, 9, 32, 2, false);
+  bad += run_case<Real, TI, TJ, NT, D, NIT>(TI + 6, 2 * TJ + 3, 11, 3, 4, false);
+  bad += run_case<Real, TI, TJ, NT, D, NIT>(5, 3, 7, 1, 3, false);
+  bad += run_case<Real, TI, TJ, NT, D, NIT>(19, 9, 8, 2, 3, false);
+  bad += run_case<Real, TI, TJ, NT, D, NIT>(TI + 6, 2 * TJ + 3, 11, 3, 4, true);
   return bad;
 }
 
 int main() {
   int bad = 0;
-  // the FP64 shapes the library compiles (k_pcor.hip)
+  // the FP64 shapes the library compiles (k_pcor.hip), one iteration per pass
   bad += run_shape<double, 64, 16, 256, 1>();
   bad += run_shape<double, 64, 8, 256, 1>();
   bad += run_shape<double, 64, 8, 256, 2>();
@@ -227,11 +225,20 @@ int main() {
   bad += run_shape<double, 128, 16, 512, 2>();
   bad += run_shape<double, 64, 32, 512, 1>();
   bad += run_shape<double, 64, 32, 512, 2>();
-  bad += run_shape<double, 32, 16, 256, 2>();
   bad += run_shape<double, 128, 8, 256, 2>();
-  bad += run_shape<double, 64, 16, 256, 2>();
+  bad += run_shape<double, 64, 16, 512, 1>();
   bad += run_shape<double, 32, 16, 256, 1>();
-  bad += run_shape<double, 128, 8, 256, 1>();
+  // ... and two iterations per pass
+  bad += run_shape<double, 64, 16, 256, 1, 2>();
+  bad += run_shape<double, 64, 16, 512, 1, 2>();
+  bad += run_shape<double, 64, 32, 512, 1, 2>();
+  bad += run_shape<double, 64, 8, 256, 1, 2>();
+  bad += run_shape<double, 32, 16, 256, 1, 2>();
+  bad += run_shape<double, 64, 32, 1024, 1>();
+  bad += run_shape<double, 64, 32, 1024, 1, 2>();
+  bad += run_shape<double, 128, 16, 1024, 1>();
+  bad += run_shape<double, 64, 16, 1024, 1>();
+  bad += run_shape<double, 128, 32, 1024, 1>();
   // the FP32 smoothing shapes (k_mixed.hip)
   bad += run_shape<float, 64, 16, 256, 1>();
   bad += run_shape<float, 64, 32, 512, 1>();
@@ -239,7 +246,16 @@ int main() {
   bad += run_shape<float, 128, 16, 512, 1>();
   bad += run_shape<float, 128, 16, 512, 2>();
   bad += run_shape<float, 128, 32, 512, 1>();
-  bad += run_shape<float, 64, 16, 256, 2>();
+  bad += run_shape<float, 64, 16, 512, 1>();
+  bad += run_shape<float, 64, 16, 256, 1, 2>();
+  bad += run_shape<float, 64, 32, 512, 1, 2>();
+  bad += run_shape<float, 128, 16, 512, 1, 2>();
+  bad += run_shape<float, 128, 32, 512, 1, 2>();
+  bad += run_shape<float, 64, 16, 512, 1, 2>();
+  bad += run_shape<float, 64, 32, 1024, 1>();
+  bad += run_shape<float, 64, 32, 1024, 1, 2>();
+  bad += run_shape<float, 128, 32, 1024, 1>();
+  bad += run_shape<float, 128, 16, 1024, 1>();
   std::printf(bad ? "FAILED %d\n" : "all cases bit-identical\n", bad);
   return bad ? 1 : 0;
 }

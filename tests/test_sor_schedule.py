@@ -31,14 +31,20 @@ def test_fused_sor_schedule_matches_inplace_sweep(tmp_path):
 
 
 def test_every_compiled_shape_is_emulated():
-    """The host emulation runs every (type, tile, threads, depth) the library instantiates."""
+    """The host emulation runs every (type, tile, threads, depth, iterations
+    per pass) the library instantiates, including the one-iteration kernel of
+    every multi-iteration variant."""
     import re
 
     csrc = os.path.join(ROOT, "bio-cfd_amd", "csrc")
     emu = open(os.path.join(ROOT, "tests", "sor_tile_host.cpp")).read()
-    run = set(re.findall(r"run_shape<(\w+), (\d+), (\d+), (\d+), (\d+)>\(\)", emu))
-    f64 = open(os.path.join(csrc, "k_pcor.hip")).read()
-    f32 = open(os.path.join(csrc, "k_mixed.hip")).read()
-    lib = {("double",) + t for t in re.findall(r"launch_fused<(\d+), (\d+), (\d+), (\d+), \d+>\}", f64)}
-    lib |= {("float",) + t for t in re.findall(r"launch_sweep32<(\d+), (\d+), (\d+), (\d+), \d+>\}", f32)}
-    assert len(lib) > 9 and lib <= run, sorted(lib - run)
+    run = set()
+    for t in re.findall(r"run_shape<(\w+), (\d+), (\d+), (\d+), (\d+)(?:, (\d+))?>\(\)", emu):
+        run.add(t[:5] + (t[5] or "1",))
+    lib = set()
+    for real, src, macro in (("double", "k_pcor.hip", "FV"), ("float", "k_mixed.hip", "SV")):
+        text = open(os.path.join(csrc, src)).read()
+        for ti, tj, nt, d, nit in re.findall(macro + r"\((\d+), (\d+), (\d+), (\d+), \d+, (\d+)\),", text):
+            lib.add((real, ti, tj, nt, d, nit))
+            lib.add((real, ti, tj, nt, d, "1"))
+    assert len(lib) > 15 and lib <= run, sorted(lib - run)
