@@ -24,15 +24,24 @@
 
 namespace bcfd {
 
-constexpr int kResPlanes = 16;  // z-planes a residual thread marches
+// tile of a residual workgroup: 64 x 8 columns (256 threads, rows ty and
+// ty + 4), marching kResPlanes planes
+constexpr int kResTI = 64, kResTJ = 8, kResPlanes = 16;
+constexpr int kResPW = kResTI + 2, kResSH = kResTJ + 2;       // plane footprint with a one-cell halo
+constexpr int kResHalo = 2 * kResPW + 2 * kResTJ;             // 148 halo cells, one per thread < 148
 
 // With kUpdate: pc' = pc + e (every interior cell; e is zero off the fluid)
 // is written to pc_out and the residual is taken of pc'; without, of pc.
 // r = b/dt - A pc' at interior fluid cells in FP64, stored as FP32 into the
 // colour-split arrays (marker at the other interior cells), with the
-// per-workgroup max |r|.  A thread owns one (i, j) column and marches
-// kResPlanes planes keeping the k-1, k, k+1 values in registers; its i and j
-// neighbours are its neighbours' own loads, served by L1/L2.
+// per-workgroup max |r|.
+//
+// The workgroup marches its tile in k with the values of plane k (tile and
+// halo) in one of two LDS planes: step k stores plane k+1 (loaded in step
+// k-1) into the other, issues the loads of plane k+2 and of the rhs of k+1,
+// and computes plane k; i and j neighbours come from LDS, k neighbours from
+// registers.  HBM traffic per cell: pc 8, e 4, b/dt 8 read, pc' 8, r 4
+// written; the halo is re-read from L2.
 template <bool kUpdate>
 __global__ void __launch_bounds__(256) k_mx_resupd(Dims d, int NXh, const double *__restrict__ pc,
                                                    const float *__restrict__ e, double *__restrict__ pc_out,
@@ -42,37 +51,121 @@ __global__ void __launch_bounds__(256) k_mx_resupd(Dims d, int NXh, const double
                                                    float *__restrict__ rblack, const SorState *__restrict__ st,
                                                    double *__restrict__ partials) {
   if (st->done) return;
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
+  __shared__ double pl[2][kResSH][kResPW];
+  const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+  const int iL = 2 + blockIdx.x * kResTI, jL = 2 + blockIdx.y * kResTJ;
   const int k0 = 2 + blockIdx.z * kResPlanes;
   const int k1 = k0 + kResPlanes - 1 < d.nz + 1 ? k0 + kResPlanes - 1 : d.nz + 1;
-  double m = 0.0;
-  if (i <= d.nx + 1 && j <= d.ny + 1) {
-    const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
-    auto val = [&](int64_t c) -> double { return kUpdate ? pc[c] + (double)e[c] : pc[c]; };
-    const Coef3 ax = coef_at(Ax, d.nx, i), ay = coef_at(Ay, d.ny, j);
-    int64_t c = ix3(d, i, j, k0);
-    double vm = val(c - sz), v0 = val(c);
-    for (int k = k0; k <= k1; ++k, c += sz) {
-      const double vp = val(c + sz);
-      const bool red = ((i + j + k) & 1) != 0;
-      const int64_t h = sidx(NXh, d.NY, (i - 1) >> 1, j, k);
-      const double rhs = red ? bred[h] : bblack[h];
-      if (kUpdate) pc_out[c] = v0;
+  const int64_t sz = (int64_t)d.NX * d.NY;
+
+  // own cells (i, j[r]) at slot (tx + 1, ty + 1 + 4r); loaded where inside
+  // the array (a ghost cell is a neighbour), computed where interior
+  const int i = iL + tx;
+  int off[2], y[2], hoff[2];
+  bool ld[2], cmp[2];
+  Coef3 ay[2];
+  for (int r = 0; r < 2; ++r) {
+    const int j = jL + ty + 4 * r;
+    y[r] = ty + 1 + 4 * r;
+    ld[r] = i <= d.NX && j <= d.NY;
+    cmp[r] = i <= d.nx + 1 && j <= d.ny + 1;
+    off[r] = ld[r] ? (i - 1) + d.NX * (j - 1) : 0;
+    hoff[r] = cmp[r] ? ((i - 1) >> 1) + NXh * (j - 1) : 0;
+    ay[r] = cmp[r] ? coef_at(Ay, d.ny, j) : Coef3{0.0, 0.0, 0.0};
+  }
+  const Coef3 ax = i <= d.nx + 1 ? coef_at(Ax, d.nx, i) : Coef3{0.0, 0.0, 0.0};
+  // halo cell of this thread: top row, bottom row, left and right columns
+  int hx = 0, hy = 0;
+  if (tid < kResPW) {
+    hx = tid;
+  } else if (tid < 2 * kResPW) {
+    hx = tid - kResPW, hy = kResSH - 1;
+  } else if (tid < kResHalo) {
+    const int q = tid - 2 * kResPW;
+    hx = (q & 1) ? kResPW - 1 : 0;
+    hy = 1 + (q >> 1);
+  }
+  const int hi = iL - 1 + hx, hj = jL - 1 + hy;
+  const bool hld = tid < kResHalo && hi >= 1 && hi <= d.NX && hj >= 1 && hj <= d.NY;
+  const int hof = hld ? (hi - 1) + d.NX * (hj - 1) : 0;
+
+  // raw loads of a plane (pc and e kept apart: the sum is formed when the
+  // plane is stored, a step later)
+  double rp[3];
+  float re[3];
+  auto load = [&](int k) {
+    const int64_t base = (int64_t)(k - 1) * sz;
+    for (int r = 0; r < 2; ++r) {
+      rp[r] = ld[r] ? pc[base + off[r]] : 0.0;
+      if (kUpdate) re[r] = ld[r] ? e[base + off[r]] : 0.0f;
+    }
+    rp[2] = hld ? pc[base + hof] : 0.0;
+    if (kUpdate) re[2] = hld ? e[base + hof] : 0.0f;
+  };
+  auto val = [&](int q) -> double { return kUpdate ? rp[q] + (double)re[q] : rp[q]; };
+  auto store = [&](int k, double *own) {
+    double(*P)[kResPW] = pl[k & 1];
+    for (int r = 0; r < 2; ++r) {
+      own[r] = val(r);
+      P[y[r]][tx + 1] = own[r];
+    }
+    if (tid < kResHalo) P[hy][hx] = val(2);
+  };
+  auto load_rhs = [&](int k, double *rh) {
+    const int64_t base = (int64_t)NXh * d.NY * (k - 1);
+    for (int r = 0; r < 2; ++r) {
+      const bool red = ((i + jL + ty + 4 * r + k) & 1) != 0;
+      rh[r] = cmp[r] ? (red ? bred : bblack)[base + hoff[r]] : not_fluid_marker();
+    }
+  };
+
+  double vm[2], v0[2], vp[2], m = 0.0;
+  // step k: rh holds the rhs of plane k, rhn receives those of plane k+1
+  // (two buffers used alternately, so no register waits on a load in flight)
+  auto step = [&](int k, const double *rh, double *rhn) {
+    store(k + 1, vp);  // plane k+1 (its loads were issued a step ago)
+    if (k + 1 <= k1) {
+      load(k + 2);
+      load_rhs(k + 1, rhn);
+    }
+    const double(*P)[kResPW] = pl[k & 1];
+    const Coef3 az = coef_at(Az, d.nz, k);
+    const int64_t base = (int64_t)(k - 1) * sz;
+    const int64_t hbase = (int64_t)NXh * d.NY * (k - 1);
+    for (int r = 0; r < 2; ++r) {
+      if (!cmp[r]) continue;
+      if (kUpdate) pc_out[base + off[r]] = v0[r];
       float rv = not_fluid_marker<float>();
-      if (!sor_skip(rhs)) {
-        const Coef3 az = coef_at(Az, d.nz, k);
-        const double r = (rhs - ax.m * val(c - 1) - ax.p * val(c + 1) - ay.m * val(c - sy) - ay.p * val(c + sy) -
-                          az.m * vm - az.p * vp) -
-                         (ax.c + ay.c + az.c) * v0;
-        rv = (float)r;
-        const double ar = fabs(r);
+      if (!sor_skip(rh[r])) {
+        const int x = tx + 1, yy = y[r];
+        const double res = (rh[r] - ax.m * P[yy][x - 1] - ax.p * P[yy][x + 1] - ay[r].m * P[yy - 1][x] -
+                            ay[r].p * P[yy + 1][x] - az.m * vm[r] - az.p * vp[r]) -
+                           (ax.c + ay[r].c + az.c) * v0[r];
+        rv = (float)res;
+        const double ar = fabs(res);
         m = m > ar ? m : ar;
       }
-      (red ? rred : rblack)[h] = rv;
-      vm = v0;
-      v0 = vp;
+      const bool red = ((i + jL + ty + 4 * r + k) & 1) != 0;
+      (red ? rred : rblack)[hbase + hoff[r]] = rv;
     }
+    for (int r = 0; r < 2; ++r) {
+      vm[r] = v0[r];
+      v0[r] = vp[r];
+    }
+    __syncthreads();
+  };
+
+  double rha[2], rhb[2];
+  load(k0 - 1);
+  for (int r = 0; r < 2; ++r) vm[r] = val(r);
+  load(k0);
+  store(k0, v0);
+  load(k0 + 1);
+  load_rhs(k0, rha);
+  __syncthreads();
+  for (int k = k0; k <= k1; k += 2) {
+    step(k, rha, rhb);
+    if (k + 1 <= k1) step(k + 1, rhb, rha);
   }
   wg_max_store(m, partials);
 }
@@ -194,7 +287,7 @@ static int ensure_mixed(Block &b) {
 }
 
 int64_t residual_workgroups(const Block &b) {
-  return (int64_t)div_up(b.d.nx, 64) * div_up(b.d.ny, 4) * div_up(b.d.nz, kResPlanes);
+  return (int64_t)div_up(b.d.nx, kResTI) * div_up(b.d.ny, kResTJ) * div_up(b.d.nz, kResPlanes);
 }
 
 // b/dt of the fluid cells into the colour-split FP64 arrays (marker elsewhere)
@@ -203,7 +296,7 @@ int split_bdt(Block &b, double dt);
 // residual of pc (src, with e added when e != nullptr, written to dst)
 static int64_t residual_launch(Block &b, const double *src, const float *e, double *dst, const SorState *st) {
   const int NXh = (b.d.NX + 1) / 2;
-  dim3 blk(64, 4, 1), grd(div_up(b.d.nx, 64), div_up(b.d.ny, 4), div_up(b.d.nz, kResPlanes));
+  dim3 blk(256, 1, 1), grd(div_up(b.d.nx, kResTI), div_up(b.d.ny, kResTJ), div_up(b.d.nz, kResPlanes));
   if (e)
     hipLaunchKernelGGL(k_mx_resupd<true>, grd, blk, 0, rt().stream, b.d, NXh, src, e, dst, b.bs[1], b.bs[0], b.A[0],
                        b.A[1], b.A[2], b.r32[1], b.r32[0], st, b.partials);

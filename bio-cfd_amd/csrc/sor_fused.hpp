@@ -4,9 +4,10 @@
 // pc planes staged in a ring of 2*NIT+2 LDS slots.
 //
 // Step s of the march, phases separated by workgroup barriers:
-//   A: store plane s+1 (loaded into registers D steps earlier), issue the
-//      loads of plane s+1+D, of the rhs values step s+1 needs and of the Acz
-//      row of plane s+1 (Acx, Acy of the tile sit in LDS for the march);
+//   A: store plane s+1 and its Acz row (loaded into registers D steps
+//      earlier), issue the loads of plane s+1+D with its Acz row and of the
+//      rhs values step s+1 needs (Acx, Acy of the tile sit in LDS for the
+//      march; the Acz rows in a ring of R rows beside the plane slots);
 //   p = 0 .. 2*NIT-1: update colour red (p even) or black (p odd) of
 //      iteration p/2+1 on plane s-p over the tile plus a halo of
 //      2*NIT-1-p cells (x, y and z); the last phase (black of the last
@@ -29,6 +30,10 @@
 // Index work per step is 32-bit: every thread keeps the slot positions of its
 // cells (computed once) and offsets within one plane; the plane's base
 // pointer is uniform.
+//
+// No value loaded from global memory is used in the step that loads it: the
+// loads of a step complete in order, so waiting on one (a uniform Acz row,
+// say) would also wait on the plane prefetched before it.
 //
 // The phase functions are the per-thread bodies of the kernel; a host build
 // of this header runs the same bodies thread by thread, phase by phase, to
@@ -125,7 +130,9 @@ struct SorTile {
   static constexpr int NL = (PLANE + NT - 1) / NT;  // plane loads per thread
   static constexpr int SLOTS = R * 2 * SH * SW;
   static constexpr int CXW = PW - 2, CYW = SH - 2;  // staged coefficients: slot x (y) = 1 .. PW-2 (SH-2)
-  static constexpr int LDS_DOUBLES = SLOTS + 3 * CXW + 3 * CYW;
+  static constexpr int CZ = SLOTS + 3 * CXW + 3 * CYW;  // Acz ring: 3 per slot plane
+  static constexpr int SINK = CZ + 3 * R;                // written by slots past a phase's region
+  static constexpr int LDS_DOUBLES = SINK + 1;
 
   // phase p (0 .. NPH-1) updates colour red (p even) or black (p odd) of
   // iteration p/2 + 1 on plane s - p over the tile plus a halo of hp(p)
@@ -152,13 +159,13 @@ struct SorTile {
   // per-thread registers of the march
   struct Thread {
     Real pcn[D][NL];          // planes in flight
+    double czn[D];            // their Acz rows: thread m < 3 carries column m
     Real rc[NPH][NQ];         // rhs of each phase's cells for this step
     Real rn[NIT == 1 ? NPH : 1][NIT == 1 ? NQ : 1];  // NIT = 1: the next step's, loaded in phase A
     int ld_off[NL];           // in-plane offset of the loaded slot cell, -1 outside the array
     int ld_lds[NL];           // 2 * (y * SW + (x >> 1)) + ((x + y) & 1), -1 past the slot
     int pos[NPH][NQ];         // y << 8 | r: row y (from the phase's first row), r-th cell; -1 past the end
-    C3 cz[NPH];               // Acz of plane s - p for phase p
-    C3 czp;                   // Acz of plane s + 1
+    int czm;                  // Acz column this thread carries (0 .. 2), -1 for none
   };
 
   __host__ __device__ static int slot_index(int k) { return (k + 2 * R) % R; }
@@ -192,11 +199,6 @@ struct SorTile {
     return s - p >= T.k0 - hp(p) && s - p <= T.k1 + hp(p);
   }
 
-  __host__ __device__ static C3 cz_of(const Args &a, int k) {
-    if (k < 2 || k > a.d.nz + 1) return C3{Real(0), Real(0), Real(0)};
-    return coef_at<Real>(a.Az, a.d.nz, k);
-  }
-
   __host__ __device__ static void init_thread(const Args &a, const Tile &T, int tid, Thread &th) {
     for (int q = 0; q < NL; ++q) {
       const int e = tid + q * NT;
@@ -218,9 +220,7 @@ struct SorTile {
           th.pos[p][q] = (yy << 8) | (e - yy * RWp(p));
         }
       }
-    const int s0 = first_step(T);
-    for (int p = 0; p < NPH; ++p) th.cz[p] = cz_of(a, s0 - 1 - p);
-    th.czp = cz_of(a, s0);
+    th.czm = tid < 3 ? tid : -1;
   }
 
   // Acx over slot columns x = 1..PW-2 and Acy over slot rows y = 1..SH-2
@@ -246,19 +246,27 @@ struct SorTile {
     return C3{cy[y - 1], cy[CYW + y - 1], cy[2 * CYW + y - 1]};
   }
 
-  // plane p of src over the slot footprint (zero outside the array)
-  __host__ __device__ static void load_plane(const Args &a, const Thread &th, int p, Real *r) {
+  // plane p of src over the slot footprint (zero outside the array) and
+  // column czm of its Acz row (zero off the interior, where no cell is
+  // updated)
+  __host__ __device__ static void load_plane(const Args &a, const Thread &th, int p, Real *r, double &cz) {
     const bool pin = p >= 1 && p <= a.d.NZ && !a.zero_src;
     const Real *P = a.src + (int64_t)(pin ? p - 1 : 0) * ((int64_t)a.d.NX * a.d.NY);
     for (int q = 0; q < NL; ++q) r[q] = (pin && th.ld_off[q] >= 0) ? P[th.ld_off[q]] : Real(0);
+    cz = (th.czm >= 0 && p >= 2 && p <= a.d.nz + 1) ? a.Az[(p - 2) + th.czm * a.d.nz] : 0.0;
   }
 
-  __host__ __device__ static void store_plane(Real *lds, const Thread &th, int p, const Real *r) {
+  __host__ __device__ static void store_plane(Real *lds, const Thread &th, int p, const Real *r, double cz) {
     Real *S = slot(lds, p, 0);
     for (int q = 0; q < NL; ++q) {
       const int l = th.ld_lds[q];
       if (l >= 0) S[((l & 1) ^ (p & 1)) * (SH * SW) + (l >> 1)] = r[q];
     }
+    if (th.czm >= 0) lds[CZ + 3 * slot_index(p) + th.czm] = Real(cz);
+  }
+  __host__ __device__ static C3 cz_at(const Real *lds, int k) {
+    const Real *c = lds + CZ + 3 * slot_index(k);
+    return C3{c[0], c[1], c[2]};
   }
 
   // cell of packed position v in phase p on plane k: slot row y, column x
@@ -290,7 +298,10 @@ struct SorTile {
 
   // phase p at step s: update its colour on plane k = s - p (in LDS), and in
   // the last phase write plane k of the tile, both colours, to dst; returns
-  // the largest |change| when norm is set
+  // the largest |change| when norm is set.  The cells of all slots are
+  // computed without branches (a slot past the region works on the region's
+  // first cell and writes to the sink; a skipped cell keeps its value), so
+  // their LDS reads are in flight together.
   __host__ __device__ static double update(const Args &a, const Tile &T, Real *lds, const Thread &th, int p, int s,
                                            bool norm) {
     const int k = s - p;
@@ -298,33 +309,42 @@ struct SorTile {
     Real *O = slot(lds, k, own);
     const Real *N = slot(lds, k, other), *Nm = slot(lds, k - 1, other), *Np = slot(lds, k + 1, other);
     const bool last = p == NPH - 1;
-    Real *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
-    double delta = 0.0;
+    const C3 cz = cz_at(lds, k);
+    constexpr int NQM = NQ;
+    Real old[NQM], nw[NQM];
+    int cs[NQM], xs[NQM], ys[NQM];
     for (int q = 0; q < NQp(p); ++q) {
-      if (th.pos[p][q] < 0) continue;
       int y, x;
-      cell_of(p, th.pos[p][q], k, y, x);
+      cell_of(p, th.pos[p][q] < 0 ? 0 : th.pos[p][q], k, y, x);
       const int c = y * SW + (x >> 1);
-      const Real old = O[c];
-      Real nw = old;
-      if (!sor_skip(th.rc[p][q])) {
-        nw = sor_cell<Real>(Real(a.omega), cx_at(lds, x), cy_at(lds, y), th.cz[p], th.rc[p][q], old,
-                            N[y * SW + ((x - 1) >> 1)], N[y * SW + ((x + 1) >> 1)], N[c - SW], N[c + SW], Nm[c],
-                            Np[c]);
-        if (kNorm && norm) {
-          const double dd = fabs(double(nw - old));
-          delta = delta > dd ? delta : dd;
-        }
+      cs[q] = c;
+      xs[q] = x;
+      ys[q] = y;
+      old[q] = O[c];
+      const Real g = sor_cell<Real>(Real(a.omega), cx_at(lds, x), cy_at(lds, y), cz, th.rc[p][q], old[q],
+                                    N[y * SW + ((x - 1) >> 1)], N[y * SW + ((x + 1) >> 1)], N[c - SW], N[c + SW],
+                                    Nm[c], Np[c]);
+      nw[q] = sor_skip(th.rc[p][q]) ? old[q] : g;
+    }
+    double delta = 0.0;
+    if (kNorm && norm)
+      for (int q = 0; q < NQp(p); ++q) {
+        const double dd = th.pos[p][q] < 0 ? 0.0 : fabs(double(nw[q] - old[q]));
+        delta = delta > dd ? delta : dd;
       }
-      if (!last) {
-        O[c] = nw;
-      } else {
+    if (!last) {
+      const int base = (int)(O - lds);
+      for (int q = 0; q < NQp(p); ++q) lds[th.pos[p][q] < 0 ? SINK : base + cs[q]] = nw[q];
+    } else {
+      Real *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
+      for (int q = 0; q < NQp(p); ++q) {
+        if (th.pos[p][q] < 0) continue;
         // black cell x and its red partner x ^ 1 (same half-column)
-        const int i = T.iL + x, ir = T.iL + (x ^ 1), j = T.jL + y;
+        const int i = T.iL + xs[q], ir = T.iL + (xs[q] ^ 1), j = T.jL + ys[q];
         if (j <= a.d.ny + 1) {
           const int row = a.d.NX * (j - 1) - 1;
-          if (i <= a.d.nx + 1) P[row + i] = nw;
-          if (ir <= a.d.nx + 1) P[row + ir] = N[c];
+          if (i <= a.d.nx + 1) P[row + i] = nw[q];
+          if (ir <= a.d.nx + 1) P[row + ir] = N[cs[q]];
         }
       }
     }
@@ -335,11 +355,12 @@ struct SorTile {
   __host__ __device__ static void prologue(const Args &a, const Tile &T, Real *lds, Thread &th) {
     const int s0 = first_step(T);
     Real *r = th.pcn[0];
-    load_plane(a, th, s0 - 1, r);
-    store_plane(lds, th, s0 - 1, r);
-    load_plane(a, th, s0, r);
-    store_plane(lds, th, s0, r);
-    for (int d = 0; d < D; ++d) load_plane(a, th, s0 + 1 + d, th.pcn[d]);
+    double &c = th.czn[0];
+    load_plane(a, th, s0 - 1, r, c);
+    store_plane(lds, th, s0 - 1, r, c);
+    load_plane(a, th, s0, r, c);
+    store_plane(lds, th, s0, r, c);
+    for (int d = 0; d < D; ++d) load_plane(a, th, s0 + 1 + d, th.pcn[d], th.czn[d]);
     // phase p first works on plane k0 - hp(p) (at step k0 - hp(p) + p)
     if (NIT == 1) {
       // the first step's rhs of phase p go to rn (phase A moves them to rc)
@@ -354,20 +375,18 @@ struct SorTile {
     }
   }
 
-  // phase A of step s: buf holds plane s+1 on entry and plane s+1+D on exit;
-  // on entry cz[p] = Acz(s-1-p) and czp = Acz(s), on exit cz[p] = Acz(s-p)
-  __host__ __device__ static void phase_a(const Args &a, const Tile &T, Real *lds, Thread &th, int s, Real *buf) {
-    store_plane(lds, th, s + 1, buf);
+  // phase A of step s: buffer d holds plane s+1 on entry and plane s+1+D on
+  // exit
+  __host__ __device__ static void phase_a(const Args &a, const Tile &T, Real *lds, Thread &th, int s, int d) {
+    Real *buf = th.pcn[d];
+    store_plane(lds, th, s + 1, buf, th.czn[d]);
     if (NIT == 1)
       for (int p = 0; p < NPH; ++p)
         for (int q = 0; q < NQ; ++q) th.rc[p][q] = th.rn[NIT == 1 ? p : 0][NIT == 1 ? q : 0];
-    if (s + 1 + D <= last_step(T) + 1) load_plane(a, th, s + 1 + D, buf);
+    if (s + 1 + D <= last_step(T) + 1) load_plane(a, th, s + 1 + D, buf, th.czn[d]);
     if (NIT == 1)
       for (int p = 0; p < NPH; ++p)
         if (s + 1 <= last_step(T) && active(T, p, s + 1)) load_rhs(a, T, th, p, s + 1 - p, th.rn[NIT == 1 ? p : 0]);
-    for (int p = NPH - 1; p > 0; --p) th.cz[p] = th.cz[p - 1];
-    th.cz[0] = th.czp;
-    th.czp = cz_of(a, s + 1);
   }
 
   // phase p of step s (after phase A and phases 0 .. p-1).  With NIT = 1 the

@@ -36,7 +36,7 @@ __global__ void __launch_bounds__(NT, MINB) k_sor_fused(FusedArgsT<Real> a, int 
     for (int d = 0; d < D; ++d) {
       if (d > 0 && s + d > s1) break;
       __syncthreads();
-      T::phase_a(a, tile, lds, th, s + d, th.pcn[d]);
+      T::phase_a(a, tile, lds, th, s + d, d);
 #pragma unroll
       for (int p = 0; p < T::NPH; ++p) {
         __syncthreads();

@@ -154,7 +154,7 @@ static double fused_launch(const Problem<Real> &p, int KS, const std::vector<Rea
     for (int s = T::first_step(tile); s <= s1; s += D)
       for (int d = 0; d < D; ++d) {
         if (d > 0 && s + d > s1) break;
-        for (int tid = 0; tid < NT; ++tid) T::phase_a(a, tile, lds.data(), th[tid], s + d, th[tid].pcn[d]);
+        for (int tid = 0; tid < NT; ++tid) T::phase_a(a, tile, lds.data(), th[tid], s + d, d);
         for (int ph = 0; ph < T::NPH; ++ph)
           for (int tid = 0; tid < NT; ++tid) {  // barrier before each phase
             const double dd = T::phase_u(a, tile, lds.data(), th[tid], ph, s + d, ph >= T::NPH - 2);
