@@ -12,7 +12,7 @@ all fields resident in HBM.
 
 Prints one JSON line.  `value` is the FP64 method's Mcell-steps/s (one GPU);
 `mixed` holds the mixed-precision run of the same steps on the same inputs
-(FP32 SOR smoothing, 4 sweeps per FP64 defect correction, the same sweep
+(FP32 SOR smoothing, 16 sweeps per FP64 defect correction, the same sweep
 budget), its speed-up and its per-field error against the FP64 run.
 `roofline` describes the FP64 red-black SOR iteration (the dominant kernel)
 measured with HIP events on the runtime's stream; `cpu_baseline` times the
@@ -42,13 +42,15 @@ ORC_RUN = os.path.join(ROOT, "oracle", "orc_run")
 # read pc once, read b/dt once, write pc once
 SOR_BYTES_PER_CELL_ITER = 24.0
 # HBM bytes per fused FP64 SOR launch measured with rocprofv3 (FETCH_SIZE x 2.00
-# + WRITE_SIZE x 1.00, calibrated on a 512 MiB copy), keyed by (n, tile_i,
-# tile_j, threads, depth, planes); quoted only for the configuration measured
+# + WRITE_SIZE x 1.00, calibrated on a 512 MiB copy); quoted only for the
+# configuration measured
 SOR_TRAFFIC_MEASURED = {
-    # (n, tile_i, tile_j, threads, depth, iterations per pass, planes): bytes per launch of the
-    # kernel with that many iterations per pass (profiles/sor_fp64_sphere256_hbm_traffic.md)
-    (256, 64, 32, 512, 1, 2, 32): 5.7318e+08,
-    (256, 64, 32, 512, 1, 1, 32): 4.2695e+08,
+    # (n, family, tile_i, tile_j, threads, depth, iterations per pass, planes): bytes per launch
+    # of the kernel with that many iterations per pass (profiles/sor_fp64_sphere256_hbm_traffic.md)
+    (256, "cell lists", 64, 32, 512, 1, 2, 32): 5.7318e+08,
+    (256, "cell lists", 64, 32, 512, 1, 1, 32): 4.2695e+08,
+    (256, "pair-owned", 64, 32, 512, 1, 2, 32): 5.2988e+08,
+    (256, "pair-owned", 64, 32, 512, 1, 1, 32): 4.2834e+08,
 }
 
 
@@ -89,11 +91,17 @@ def run_path(args, precision: str, keep_fields: bool):
             H.call("biocfd_sor_config", v, planes)
         shape = [t for t in tune.variants() if t[0] == v][0]
     else:
-        v, planes, table = tune.autotune_mixed(block=1, dt=case.dt, sweeps=args.sweeps, residual_tol=args.epsi)
-        print(json.dumps({"mixed_autotune": table}), file=sys.stderr)
+        if args.mixed_variant == "auto":
+            v, planes, table = tune.autotune_mixed(block=1, dt=case.dt, sweeps=args.sweeps, residual_tol=args.epsi)
+            print(json.dumps({"mixed_autotune": table}), file=sys.stderr)
+        else:
+            v, planes = int(args.mixed_variant), args.mixed_planes
+            H.call("biocfd_mixed_config", v, planes, args.sweeps, args.epsi)
         shape = [t for t in tune.mixed_variants() if t[0] == v][0]
-    kernel = {"variant": v, "tile": [shape[1], shape[2]], "threads": shape[3], "depth": shape[4],
-              "min_wg_per_cu": shape[5], "iters_per_pass": shape[6], "planes": planes, "autotuned": precision == "mixed" or args.sor_variant == "auto"}
+    kernel = {"variant": v, "family": ["cell lists", "pair-owned", "pair-owned, LDS coefficients"][shape[7]], "tile": [shape[1], shape[2]],
+              "threads": shape[3], "depth": shape[4],
+              "min_wg_per_cu": shape[5], "iters_per_pass": shape[6], "planes": planes,
+              "autotuned": (args.mixed_variant if precision == "mixed" else args.sor_variant) == "auto"}
     for _ in range(args.warmup):
         sim.step()
     H.call("biocfd_synchronize")
@@ -196,10 +204,12 @@ def main():
     ap.add_argument("--epsi", type=float, default=1e-8,
                     help="SOR tolerance: update norm (FP64 method) / residual max-norm (mixed)")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--sweeps", type=int, default=4, help="FP32 sweeps per defect correction (mixed)")
+    ap.add_argument("--sweeps", type=int, default=16, help="FP32 sweeps per defect correction (mixed)")
     ap.add_argument("--sor-variant", default="auto",
                     help="FP64 SOR kernel variant (0 = colour-split sweeps, >=1 fused tiles) or 'auto'")
     ap.add_argument("--sor-planes", type=int, default=32, help="planes per workgroup of a fused variant")
+    ap.add_argument("--mixed-variant", default="auto", help="FP32 sweep kernel variant of the mixed solve or 'auto'")
+    ap.add_argument("--mixed-planes", type=int, default=16, help="planes per workgroup of the FP32 sweep kernel")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (byte-counter calibration)")
@@ -240,8 +250,8 @@ def main():
     if g is not None:
         achieved = SOR_BYTES_PER_CELL_ITER * g["sor_cell_iterations"] / (g["sor_gpu_ms"] * 1e-3) / 1e9
         kk = g["kernel"]
-        traffic = SOR_TRAFFIC_MEASURED.get((args.n, kk["tile"][0], kk["tile"][1], kk["threads"], kk["depth"],
-                                            kk["iters_per_pass"], kk["planes"]))
+        traffic = SOR_TRAFFIC_MEASURED.get((args.n, kk["family"], kk["tile"][0], kk["tile"][1], kk["threads"],
+                                            kk["depth"], kk["iters_per_pass"], kk["planes"]))
         out["roofline"] = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -249,8 +259,8 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": (f"red-black SOR (k_sor_fused<double>: {kk['iters_per_pass']} red+black iteration(s) per "
-                       f"launch, tile {kk['tile'][0]}x{kk['tile'][1]})"
+            "kernel": (f"red-black SOR ({'k_sor_fused' if kk['family'] == 'cell lists' else 'k_sor_pair'}<double>: "
+                       f"{kk['iters_per_pass']} red+black iteration(s) per launch, tile {kk['tile'][0]}x{kk['tile'][1]})"
                        if kk["variant"] > 0 else
                        "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
                       + ", 24 B/cell/iteration algorithmic; traffic = HBM bytes per launch",

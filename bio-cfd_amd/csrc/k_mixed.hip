@@ -217,6 +217,7 @@ using Sweep32Launch = void (*)(dim3, dim3, hipStream_t, const FusedArgsT<float> 
 struct Sweep32Variant {
   int ti, tj, nt, depth, minb, nit;
   Sweep32Launch launch, launch1;
+  int family;  // 0: SorTile, 1, 2: SorPairTile (coefficients in registers, LDS)
 };
 
 template <int TI, int TJ, int NT, int D, int MINB, int NIT>
@@ -225,10 +226,20 @@ static void launch_sweep32(dim3 g, dim3 blk, hipStream_t s, const FusedArgsT<flo
   hipLaunchKernelGGL((k_sor_fused<float, TI, TJ, NT, D, NIT, MINB>), g, blk, 0, s, a, gx, gy, 0, st, partials);
 }
 
-// SV(tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers
-// must allow, sweeps per pass)
+template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG>
+static void launch_pair32(dim3 g, dim3 blk, hipStream_t s, const FusedArgsT<float> &a, int gx, int gy,
+                          const SorState *st, double *partials) {
+  hipLaunchKernelGGL((k_sor_pair<float, TI, TJ, NT, NIT, MINW, CREG>), g, blk, 0, s, a, gx, gy, 0, st, partials);
+}
+
+// SV(tile_i, tile_j, threads, prefetch depth, waves per SIMD the registers
+// must allow, sweeps per pass): SorTile; PS(tile_i, tile_j, threads, waves
+// per SIMD, sweeps per pass, coefficients in registers): SorPairTile
 #define SV(TI, TJ, NT, D, MINB, NIT) \
-  { TI, TJ, NT, D, MINB, NIT, launch_sweep32<TI, TJ, NT, D, MINB, NIT>, launch_sweep32<TI, TJ, NT, D, MINB, 1> }
+  { TI, TJ, NT, D, MINB, NIT, launch_sweep32<TI, TJ, NT, D, MINB, NIT>, launch_sweep32<TI, TJ, NT, D, MINB, 1>, 0 }
+#define PS(TI, TJ, NT, MINW, NIT, CREG) \
+  { TI, TJ, NT, 1, MINW, NIT, launch_pair32<TI, TJ, NT, MINW, NIT, CREG>, launch_pair32<TI, TJ, NT, MINW, 1, CREG>, \
+    CREG ? 1 : 2 }
 static const Sweep32Variant kSweep32[] = {
     SV(64, 16, 256, 1, 1, 1),
     SV(64, 32, 512, 1, 1, 1),
@@ -245,15 +256,22 @@ static const Sweep32Variant kSweep32[] = {
     SV(64, 32, 1024, 1, 1, 2),
     SV(128, 32, 1024, 1, 1, 1),
     SV(128, 16, 1024, 1, 1, 1),
+    PS(64, 32, 512, 1, 1, 1),
+    PS(64, 32, 512, 1, 2, 1),
+    PS(64, 28, 512, 4, 1, 1),
+    PS(64, 28, 512, 4, 2, 0),
+    PS(64, 32, 512, 4, 2, 0),
+    PS(128, 16, 512, 4, 1, 1),
 };
 #undef SV
+#undef PS
 constexpr int kNumSweep32 = (int)(sizeof(kSweep32) / sizeof(kSweep32[0]));
 
 struct MixedConfig {
   int precision = 0;        // 0: FP64 method, 1: mixed
-  int variant = 1;
-  int planes = 32;
-  int sweeps = 4;           // FP32 sweeps per correction
+  int variant = 19;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 2 sweeps per pass, 2 WG/CU)
+  int planes = 16;
+  int sweeps = 16;          // FP32 sweeps per correction
   double tol = 1e-8;        // max-norm of the FP64 residual
 };
 static MixedConfig g_mx;
@@ -455,7 +473,7 @@ int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_co
   return 0;
 }
 
-int biocfd_mixed_variant_info(int variant, int info[6]) {
+int biocfd_mixed_variant_info(int variant, int info[7]) {
   if (variant < 0 || variant >= kNumSweep32) {
     set_error("mixed_variant_info: variant %d outside 0..%d", variant, kNumSweep32 - 1);
     return -1;
@@ -467,6 +485,7 @@ int biocfd_mixed_variant_info(int variant, int info[6]) {
   info[3] = v.depth;
   info[4] = v.minb;
   info[5] = v.nit;
+  info[6] = v.family;
   return 0;
 }
 

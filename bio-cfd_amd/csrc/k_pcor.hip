@@ -392,6 +392,7 @@ using FusedLaunch = void (*)(dim3, dim3, hipStream_t, const FusedArgs &, int, in
 struct FusedVariant {
   int ti, tj, nt, depth, minb, nit;
   FusedLaunch launch, launch1;
+  int family;  // 0: SorTile (cell lists per phase), 1, 2: SorPairTile (pair-owned; coefficients in registers, LDS)
 };
 
 template <int TI, int TJ, int NT, int D, int MINB, int NIT>
@@ -401,12 +402,25 @@ static void launch_fused(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, in
                      partials);
 }
 
-// FV(tile_i, tile_j, threads, prefetch depth, workgroups per CU the registers
-// must allow, iterations per pass)
+template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG>
+static void launch_pair(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, int check,
+                        const SorState *st, double *partials) {
+  hipLaunchKernelGGL((k_sor_pair<double, TI, TJ, NT, NIT, MINW, CREG>), g, blk, 0, s, a, gx, gy, check, st,
+                     partials);
+}
+
+// FV(tile_i, tile_j, threads, prefetch depth, waves per SIMD the registers
+// must allow, iterations per pass): SorTile; PV(tile_i, tile_j, threads,
+// waves per SIMD, iterations per pass, coefficients in registers):
+// SorPairTile (prefetch depth 1; family 1 with register coefficients, 2 with
+// LDS coefficients)
 #define FV(TI, TJ, NT, D, MINB, NIT) \
-  { TI, TJ, NT, D, MINB, NIT, launch_fused<TI, TJ, NT, D, MINB, NIT>, launch_fused<TI, TJ, NT, D, MINB, 1> }
+  { TI, TJ, NT, D, MINB, NIT, launch_fused<TI, TJ, NT, D, MINB, NIT>, launch_fused<TI, TJ, NT, D, MINB, 1>, 0 }
+#define PV(TI, TJ, NT, MINW, NIT, CREG) \
+  { TI, TJ, NT, 1, MINW, NIT, launch_pair<TI, TJ, NT, MINW, NIT, CREG>, launch_pair<TI, TJ, NT, MINW, 1, CREG>, \
+    CREG ? 1 : 2 }
 static const FusedVariant kFused[] = {
-    {0, 0, 0, 0, 0, 1, nullptr, nullptr},
+    {0, 0, 0, 0, 0, 1, nullptr, nullptr, 0},
     FV(64, 16, 256, 1, 1, 1),
     FV(64, 8, 256, 1, 1, 1),
     FV(64, 8, 256, 2, 1, 1),
@@ -427,10 +441,17 @@ static const FusedVariant kFused[] = {
     FV(128, 16, 1024, 1, 1, 1),
     FV(64, 16, 1024, 1, 1, 1),
     FV(128, 32, 1024, 1, 1, 1),
+    PV(64, 32, 512, 1, 2, 1),
+    PV(64, 32, 512, 1, 1, 1),
+    PV(64, 32, 512, 4, 1, 0),
+    PV(64, 28, 512, 4, 1, 0),
+    PV(64, 24, 512, 4, 1, 0),
+    PV(64, 32, 512, 1, 2, 0),
 };
 #undef FV
+#undef PV
 constexpr int kNumSorVariants = (int)(sizeof(kFused) / sizeof(kFused[0]));
-static int g_sor_variant = 7;
+static int g_sor_variant = 21;  // the fastest at 256^3 on MI355X (pair-owned 64x32, 2 iterations per pass)
 static int g_sor_planes = 32;
 
 static int enqueue_split_iterations(Block &b, SorState *st, long long n, int64_t pc_ita_max, double epsi) {
@@ -773,7 +794,7 @@ int biocfd_sor_config(int variant, int planes_per_workgroup) {
   return 0;
 }
 
-int biocfd_sor_variant_info(int variant, int info[6]) {
+int biocfd_sor_variant_info(int variant, int info[7]) {
   if (variant < 0 || variant >= kNumSorVariants) {
     set_error("sor_variant_info: variant %d outside 0..%d", variant, kNumSorVariants - 1);
     return -1;
@@ -785,6 +806,7 @@ int biocfd_sor_variant_info(int variant, int info[6]) {
   info[3] = v.depth;
   info[4] = v.minb;
   info[5] = v.nit;
+  info[6] = v.family;
   return 0;
 }
 

@@ -6,6 +6,7 @@
 #include "biocfd_rt.hpp"
 #include "reduce.hpp"
 #include "sor_fused.hpp"
+#include "sor_pair.hpp"
 
 namespace bcfd {
 
@@ -44,6 +45,44 @@ __global__ void __launch_bounds__(NT, MINB) k_sor_fused(FusedArgsT<Real> a, int 
         delta = delta > dp ? delta : dp;
       }
     }
+  }
+  if (check) wg_max_store(delta, partials);
+}
+
+// one step of the pair-owned tile march; B: the step's rhs buffer
+template <class T, int B, typename Real>
+__device__ inline void pair_step(const FusedArgsT<Real> &a, const typename T::Tile &tile, Real *lds,
+                                 typename T::Thread &th, int s, int check, double &delta) {
+  __syncthreads();
+  T::template phase_a<B>(a, tile, lds, th, s);
+#pragma unroll
+  for (int p = 0; p < T::NPH; ++p) {
+    __syncthreads();
+    const double dp = T::template phase_u<B>(a, tile, lds, th, p, s, check && p >= T::NPH - 2);
+    delta = delta > dp ? delta : dp;
+  }
+}
+
+// NIT red-black iterations per launch with the pair-owned tile (SorPairTile);
+// same grid, arguments and check protocol as k_sor_fused.  MINW: waves per
+// SIMD the register allocation must leave room for (4 = two 512-thread
+// workgroups per CU)
+template <typename Real, int TI, int TJ, int NT, int NIT, int MINW, bool CREG>
+__global__ void __launch_bounds__(NT, MINW) k_sor_pair(FusedArgsT<Real> a, int gx, int gy, int check,
+                                                       const SorState *__restrict__ st,
+                                                       double *__restrict__ partials) {
+  using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
+  if (st->done) return;
+  __shared__ Real lds[T::LDS_DOUBLES];
+  const typename T::Tile tile = T::tile_of(a, T::xcd_tile(blockIdx.x, gridDim.x), gx, gy);
+  typename T::Thread th;
+  T::init_thread(a, tile, threadIdx.x, th);
+  T::prologue(a, tile, lds, th, threadIdx.x);
+  double delta = 0.0;
+  const int s1 = T::last_step(tile);
+  for (int s = T::first_step(tile); s <= s1; s += 2) {
+    pair_step<T, 0>(a, tile, lds, th, s, check, delta);
+    if (s + 1 <= s1) pair_step<T, 1>(a, tile, lds, th, s + 1, check, delta);
   }
   if (check) wg_max_store(delta, partials);
 }

@@ -1,0 +1,301 @@
+// Pair-owned variant of the fused red-black SOR tile (NIT = 1 or 2 full
+// iterations of REDBLACKSOR_linear, PcorVcor.F90:313-349, per launch): the
+// same march, LDS ring, phases and halos as SorTile (sor_fused.hpp), with a
+// thread-to-cell mapping that keeps per-cell work to the stencil itself.
+//
+// Every thread owns fixed pairs of cells (slot columns 2h and 2h+1 of one
+// slot row y, 1 <= y <= SH-2) for the whole launch.  In a phase of colour c
+// on plane k exactly one cell of each pair has that colour, the one at
+// column 2h + par with par = (c ^ y ^ k) & 1, and both colours of a pair
+// share the colour-split slot index y*SW + h.  So per phase a pair costs one
+// parity bit: its slot address, its neighbours (W, E at index - 1 + par and
+// + par of the other colour, S, N at -SW, +SW, B, T on the neighbour planes)
+// and its coefficients follow from it.  With CREG the coefficients (Acx of
+// both columns, Acy of the row and the partial sums Acx(c) + Acy(c)) sit in
+// registers from the start; without, Acx and Acy of the tile are staged in
+// LDS (fewer registers, so more workgroups per CU).
+// Whether the cell is updated (inside the phase's region and the interior
+// in i and j) is a precomputed bit per (phase, parity); a pair whose cell is
+// not updated writes to a sink word, so the phase has no branches.
+//
+// All loads of step s+1 (the plane s+2 of the iterate with its Acz row, the
+// rhs of every phase) are issued in phase A of step s into registers that
+// alternate with the step's parity, and first used in step s+1: the only
+// wait for global memory is the one phase A makes for the loads a step
+// earlier.
+//
+// Real = double is the reference's SOR (bit-identical: the operand order of
+// sor_cell, Acx(c) + Acy(c) + Acz(c) summed left to right); Real = float the
+// smoothing sweeps of the mixed-precision solve.
+#pragma once
+
+#include "sor_fused.hpp"
+
+namespace bcfd {
+
+template <typename Real, int TI, int TJ, int NT, int NIT, bool CREG = true>
+struct SorPairTile {
+  static_assert(TI % 2 == 0 && TJ % 2 == 0, "tile sides must be even");
+  static_assert(NIT == 1 || NIT == 2, "one or two iterations per pass");
+  static constexpr int H = 2 * NIT;                 // slot halo (cells) around the tile
+  static constexpr int NPH = 2 * NIT;               // update phases per step
+  static constexpr int R = 2 * NIT + 2;             // planes in the LDS ring
+  static constexpr int PW = TI + 2 * H, SH = TJ + 2 * H, SW = PW / 2;
+  static constexpr int PLANE = SH * PW;             // cells of one slot plane
+  static constexpr int NL = (PLANE + NT - 1) / NT;  // plane loads per thread
+  static constexpr int NPAIR = (SH - 2) * SW;       // pairs of slot rows 1 .. SH-2
+  static constexpr int NQ = (NPAIR + NT - 1) / NT;  // pairs per thread
+  static constexpr int SLOTS = R * 2 * SH * SW;
+  static constexpr int CZ = SLOTS;                  // Acz ring: 3 per slot plane
+  static constexpr int SINK = CZ + 3 * R;           // written by pairs whose cell is not updated
+  static constexpr int CXW = PW - 2, CYW = SH - 2;  // staged Acx (Acy) of slot columns (rows) 1 .. PW-2 (SH-2)
+  static constexpr int CX = SINK + 1, CY = CX + 3 * CXW;
+  static constexpr int LDS_DOUBLES = CREG ? SINK + 1 : CY + 3 * CYW;
+  // mask bits of a pair: 2p + par = the cell at column 2h + par is updated by
+  // phase p; kW0, kW1 = the last phase writes cell 2h, 2h+1 to the
+  // destination; kYOdd = slot row odd; bits 16-23: h, 24-31: y
+  static constexpr int kW0 = 2 * NPH, kW1 = 2 * NPH + 1, kYOdd = 2 * NPH + 2;
+  static_assert(kYOdd < 16 && SW < 256 && SH < 256, "pair mask layout");
+  static constexpr int NQC = CREG ? NQ : 1;         // pairs with coefficient registers
+
+  struct Tile {
+    int iL, jL, k0, k1;
+  };
+  using Args = FusedArgsT<Real>;
+  static constexpr bool kNorm = sizeof(Real) == sizeof(double);
+
+  struct Thread {
+    Real pcn[NL];            // the plane in flight
+    double czn;              // its Acz row: thread m < 3 carries column m
+    int ld_off[NL];          // in-plane offset of the loaded slot cell, -1 outside the array
+    int ld_lds[NL];          // LDS index of the slot cell in the ring slot of an even plane, -1 past the slot
+    int czm;                 // Acz column this thread carries (0 .. 2), -1 for none
+    int c[NQ];               // slot index y*SW + h of each pair
+    unsigned mask[NQ];
+    int roff[NQ];            // rhs in-plane offset of column 2h (column 2h+1: + 1)
+    int goff[NQ];            // iterate in-plane offset of column 2h
+    Real cxm[NQC][2], cxp[NQC][2], sxy[NQC][2], cym[NQC], cyp[NQC];
+    Real rc[2][NPH][NQ];     // rhs of each phase's cells, by step parity
+  };
+
+  __host__ __device__ static int slot_index(int k) { return (k + 2 * R) % R; }
+  __host__ __device__ static int slot_off(int k, int colour) { return (slot_index(k) * 2 + colour) * (SH * SW); }
+
+  __host__ __device__ static int xcd_tile(int L, int n) {
+    const int per = n / 8, rem = n % 8, xcd = L % 8, q = L / 8;
+    return xcd < rem ? xcd * (per + 1) + q : rem * (per + 1) + (xcd - rem) * per + q;
+  }
+
+  __host__ __device__ static Tile tile_of(const Args &a, int t, int gx, int gy) {
+    const int tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
+    Tile T;
+    T.iL = 2 + TI * tx - H;
+    T.jL = 2 + TJ * ty - H;
+    T.k0 = 2 + a.KS * tz;
+    T.k1 = T.k0 + a.KS - 1 < a.d.nz + 1 ? T.k0 + a.KS - 1 : a.d.nz + 1;
+    return T;
+  }
+
+  __host__ __device__ static constexpr int hp(int p) { return 2 * NIT - 1 - p; }
+  __host__ __device__ static int first_step(const Tile &T) { return T.k0 - (2 * NIT - 1); }
+  __host__ __device__ static int last_step(const Tile &T) { return T.k1 + (2 * NIT - 1); }
+  __host__ __device__ static bool active(const Tile &T, int p, int s) {
+    return s - p >= T.k0 - hp(p) && s - p <= T.k1 + hp(p);
+  }
+
+  __host__ __device__ static void init_thread(const Args &a, const Tile &T, int tid, Thread &th) {
+    for (int q = 0; q < NL; ++q) {
+      const int e = tid + q * NT;
+      th.ld_off[q] = -1;
+      th.ld_lds[q] = -1;
+      if (e < PLANE) {
+        const int y = e / PW, x = e - y * PW;
+        const int i = T.iL + x, j = T.jL + y;
+        // colour of (x, y) in an even plane: (x + y) & 1
+        th.ld_lds[q] = ((x + y) & 1) * (SH * SW) + y * SW + (x >> 1);
+        if (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY) th.ld_off[q] = (i - 1) + a.d.NX * (j - 1);
+      }
+    }
+    th.czm = tid < 3 ? tid : -1;
+    const int nxh = a.NXh * a.d.NY;
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + q * NT;
+      const bool real = e < NPAIR;
+      const int y = real ? 1 + e / SW : 1, h = real ? e % SW : 0;
+      const int j = T.jL + y, i0 = T.iL + 2 * h;
+      const bool jin = j >= 2 && j <= a.d.ny + 1;
+      unsigned m = ((y & 1) ? 1u << kYOdd : 0u) | (unsigned)h << 16 | (unsigned)y << 24;
+      for (int par = 0; par < 2; ++par) {
+        const int x = 2 * h + par, i = i0 + par;
+        const bool iin = i >= 2 && i <= a.d.nx + 1;
+        for (int p = 0; p < NPH; ++p) {
+          const int lo = p + 1, hi = lo + 2 * hp(p);  // region: [lo, lo + T + 2 hp)
+          if (real && iin && jin && x >= lo && x < hi + TI && y >= lo && y < hi + TJ) m |= 1u << (2 * p + par);
+        }
+        if constexpr (CREG) {
+          const Coef3T<Real> cx = iin ? coef_at<Real>(a.Ax, a.d.nx, i) : Coef3T<Real>{Real(0), Real(0), Real(0)};
+          const Coef3T<Real> cy = jin ? coef_at<Real>(a.Ay, a.d.ny, j) : Coef3T<Real>{Real(0), Real(0), Real(0)};
+          th.cxm[q][par] = cx.m;
+          th.cxp[q][par] = cx.p;
+          th.sxy[q][par] = cx.c + cy.c;
+          th.cym[q] = cy.m;
+          th.cyp[q] = cy.p;
+        }
+      }
+      // the tile's own pairs: the last phase writes both cells
+      const bool own = real && y >= H && y < H + TJ && h >= H / 2 && h < H / 2 + TI / 2 && j <= a.d.ny + 1;
+      if (own && i0 <= a.d.nx + 1) m |= 1u << kW0;
+      if (own && i0 + 1 <= a.d.nx + 1) m |= 1u << kW1;
+      th.mask[q] = m;
+      th.c[q] = y * SW + h;
+      // rhs of column 2h + par: half-column (i0 + par - 1) >> 1 = i0/2 - 1 + par
+      int ro = a.NXh * (j - 1) + i0 / 2 - 1;
+      ro = ro < 0 ? 0 : (ro > nxh - 2 ? nxh - 2 : ro);
+      th.roff[q] = ro;
+      th.goff[q] = (i0 - 1) + a.d.NX * (j - 1);
+    }
+  }
+
+  __host__ __device__ static bool plane_in(const Args &a, int p) { return p >= 1 && p <= a.d.NZ && !a.zero_src; }
+
+  // plane p of src over the slot footprint, raw (zeros outside the array are
+  // applied when it is stored), and column czm of its Acz row
+  __host__ __device__ static void load_plane(const Args &a, Thread &th, int p) {
+    if (plane_in(a, p)) {
+      const Real *P = a.src + (int64_t)(p - 1) * ((int64_t)a.d.NX * a.d.NY);
+      for (int q = 0; q < NL; ++q) th.pcn[q] = P[th.ld_off[q] < 0 ? 0 : th.ld_off[q]];
+    }
+    th.czn = (th.czm >= 0 && p >= 2 && p <= a.d.nz + 1) ? a.Az[(p - 2) + th.czm * a.d.nz] : 0.0;
+  }
+
+  __host__ __device__ static void store_plane(const Args &a, Real *lds, const Thread &th, int p) {
+    const bool pin = plane_in(a, p);
+    const int base = slot_off(p, 0), flip = (p & 1) * (SH * SW);
+    for (int q = 0; q < NL; ++q) {
+      const int l = th.ld_lds[q];
+      // an odd plane swaps the colour halves
+      const int w = l < 0 ? SINK : base + (l >= SH * SW ? l - flip : l + flip);
+      lds[w] = (pin && th.ld_off[q] >= 0) ? th.pcn[q] : Real(0);
+    }
+    if (th.czm >= 0) lds[CZ + 3 * slot_index(p) + th.czm] = Real(th.czn);
+  }
+
+  // the rhs of every phase active at step s, into buffer rb
+  __host__ __device__ static void load_rhs(const Args &a, const Tile &T, Thread &th, int s, Real (*rb)[NQ]) {
+    const int64_t plane = (int64_t)a.NXh * a.d.NY;
+    for (int p = 0; p < NPH; ++p) {
+      if (!active(T, p, s)) continue;
+      const int k = s - p, colour = (p & 1) ? 0 : 1;
+      const bool kin = k >= 2 && k <= a.d.nz + 1;
+      const Real *B = (colour ? a.bred : a.bblack) + (kin ? k - 1 : 1) * plane;
+      const unsigned cbit = (unsigned)((colour ^ k) & 1);
+      for (int q = 0; q < NQ; ++q) {
+        const unsigned par = ((th.mask[q] >> kYOdd) ^ cbit) & 1u;
+        rb[p][q] = B[th.roff[q] + (int)par];
+      }
+    }
+  }
+
+  // without CREG: Acx over slot columns 1..PW-2 and Acy over slot rows
+  // 1..SH-2 into LDS (zero outside the interior, where no cell is updated)
+  __host__ __device__ static void load_coef(const Args &a, const Tile &T, Real *lds, int tid) {
+    if constexpr (!CREG) {
+      for (int e = tid; e < 3 * CXW + 3 * CYW; e += NT) {
+        if (e < 3 * CXW) {
+          const int m = e / CXW, i = T.iL + 1 + (e - m * CXW);
+          lds[CX + e] = (i >= 2 && i <= a.d.nx + 1) ? Real(a.Ax[(i - 2) + m * a.d.nx]) : Real(0);
+        } else {
+          const int f = e - 3 * CXW, m = f / CYW, j = T.jL + 1 + (f - m * CYW);
+          lds[CY + f] = (j >= 2 && j <= a.d.ny + 1) ? Real(a.Ay[(j - 2) + m * a.d.ny]) : Real(0);
+        }
+      }
+    }
+  }
+
+  // planes s0-1 and s0 into LDS, plane s0+1 and the rhs of step s0 in flight
+  // (tid: the thread's index, for the coefficient staging)
+  __host__ __device__ static void prologue(const Args &a, const Tile &T, Real *lds, Thread &th, int tid) {
+    load_coef(a, T, lds, tid);
+    const int s0 = first_step(T);
+    load_plane(a, th, s0 - 1);
+    store_plane(a, lds, th, s0 - 1);
+    load_plane(a, th, s0);
+    store_plane(a, lds, th, s0);
+    load_plane(a, th, s0 + 1);
+    load_rhs(a, T, th, s0, th.rc[0]);
+  }
+
+  // phase A of step s (rhs buffer B holds step s's): store plane s+1, issue
+  // the loads of plane s+2 and of step s+1's rhs into buffer 1-B
+  template <int B>
+  __host__ __device__ static void phase_a(const Args &a, const Tile &T, Real *lds, Thread &th, int s) {
+    store_plane(a, lds, th, s + 1);
+    if (s + 2 <= last_step(T) + 1) load_plane(a, th, s + 2);
+    if (s + 1 <= last_step(T)) load_rhs(a, T, th, s + 1, th.rc[1 - B]);
+  }
+
+  // phase p of step s: update its colour on plane k = s - p (in LDS; the last
+  // phase writes plane k of the tile, both colours, to dst); returns the
+  // largest |change| when norm is set
+  template <int B>
+  __host__ __device__ static double phase_u(const Args &a, const Tile &T, Real *lds, const Thread &th, int p, int s,
+                                            bool norm) {
+    if (!active(T, p, s)) return 0.0;
+    const int k = s - p, colour = (p & 1) ? 0 : 1;
+    const bool kin = k >= 2 && k <= a.d.nz + 1;
+    const bool last = p == NPH - 1;
+    const int oO = slot_off(k, colour), oN = slot_off(k, 1 - colour);
+    const int oNm = slot_off(k - 1, 1 - colour), oNp = slot_off(k + 1, 1 - colour);
+    const Real *cz = lds + CZ + 3 * slot_index(k);
+    const Real czm = cz[0], czc = cz[1], czp = cz[2];
+    const Real omega = Real(a.omega);
+    const unsigned cbit = (unsigned)((colour ^ k) & 1);
+    Real *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
+    double delta = 0.0;
+    for (int q = 0; q < NQ; ++q) {
+      const unsigned m = th.mask[q];
+      const int par = (int)(((m >> kYOdd) ^ cbit) & 1u);
+      const int c = th.c[q];
+      const Real *N = lds + oN + c;
+      const Real old = lds[oO + c];
+      const Real pw = N[par - 1], pe = N[par], ps = N[-SW], pn = N[SW];
+      const Real pb = lds[oNm + c], pt = lds[oNp + c];
+      const Real rhs = th.rc[B][p][q];
+      Real axm, axp, aym, ayp, sxy;
+      if constexpr (CREG) {
+        axm = par ? th.cxm[q][1] : th.cxm[q][0];
+        axp = par ? th.cxp[q][1] : th.cxp[q][0];
+        sxy = par ? th.sxy[q][1] : th.sxy[q][0];
+        aym = th.cym[q];
+        ayp = th.cyp[q];
+      } else {
+        const Real *cx = lds + CX + 2 * (int)((m >> 16) & 255u) + par - 1;
+        const Real *cy = lds + CY + (int)(m >> 24) - 1;
+        axm = cx[0];
+        axp = cx[2 * CXW];
+        aym = cy[0];
+        ayp = cy[2 * CYW];
+        sxy = cx[CXW] + cy[CYW];
+      }
+      // sor_cell's operand order; the centre sum (Acx + Acy) + Acz
+      const Real gs = (rhs - axm * pw - axp * pe - aym * ps - ayp * pn - czm * pb - czp * pt) / (sxy + czc);
+      const Real nw = (Real(1) - omega) * old + omega * gs;
+      const bool upd = kin && ((m >> (2 * p + par)) & 1u) && !sor_skip(rhs);
+      if (kNorm && norm) {
+        const double dd = upd ? fabs(double(nw - old)) : 0.0;
+        delta = delta > dd ? delta : dd;
+      }
+      if (!last) {
+        lds[upd ? oO + c : SINK] = nw;
+      } else {
+        const Real v = upd ? nw : old;
+        const Real partner = par ? pw : pe;  // the other colour in the same half-column
+        if (m & (1u << kW0)) P[th.goff[q]] = par ? partner : v;
+        if (m & (1u << kW1)) P[th.goff[q] + 1] = par ? v : partner;
+      }
+    }
+    return delta;
+  }
+};
+
+}  // namespace bcfd

@@ -19,7 +19,7 @@ DEFAULT_PLANES = (8, 16, 32, 64)
 
 def _infos(name):
     out = []
-    info = np.zeros(6, dtype=np.int32)
+    info = np.zeros(7, dtype=np.int32)
     v = 0
     while getattr(H.lib(), name)(v, H.ptr(info)) == 0:
         out.append((v,) + tuple(int(x) for x in info))
@@ -28,8 +28,8 @@ def _infos(name):
 
 
 def variants():
-    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu, iters_per_pass)] of the FP64
-    SOR kernels."""
+    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu, iters_per_pass, family)] of
+    the FP64 SOR kernels (family 0: per-phase cell lists, 1: pair-owned cells)."""
     return _infos("biocfd_sor_variant_info")
 
 
@@ -48,11 +48,12 @@ def autotune(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PL
     (best_variant, best_planes, table) with table rows
     {"variant", "tile", "threads", "planes", "us_per_iteration"}."""
     table = []
-    for v, ti, tj, nt, depth, minb, nit in variants():
+    for v, ti, tj, nt, depth, minb, nit, fam in variants():
         for p in (planes if v > 0 else (32,)):
             ms = time_variant(block, v, p, iters, dt)
-            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "min_wg_per_cu": minb,
-                          "iters_per_pass": nit, "planes": p, "us_per_iteration": round(1e3 * ms, 2)})
+            table.append({"variant": v, "family": fam, "tile": [ti, tj], "threads": nt, "depth": depth,
+                          "min_wg_per_cu": minb, "iters_per_pass": nit, "planes": p,
+                          "us_per_iteration": round(1e3 * ms, 2)})
     best = min(table, key=lambda r: r["us_per_iteration"])
     if install:
         H.call("biocfd_sor_config", best["variant"], best["planes"])
@@ -60,8 +61,8 @@ def autotune(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PL
 
 
 def mixed_variants():
-    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu, sweeps_per_pass)] of the FP32
-    smoothing kernels."""
+    """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu, sweeps_per_pass, family)] of
+    the FP32 smoothing kernels."""
     return _infos("biocfd_mixed_variant_info")
 
 
@@ -71,14 +72,15 @@ def autotune_mixed(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFA
     current pc as rhs; pc is not changed); return (variant, planes, table)."""
     table = []
     ms = np.zeros(1)
-    for v, ti, tj, nt, depth, minb, nit in mixed_variants():
+    for v, ti, tj, nt, depth, minb, nit, fam in mixed_variants():
         for p in planes:
             best = float("inf")
             for _ in range(2):
                 H.call("biocfd_mixed_time", block, v, p, iters, dt, H.ptr(ms))
                 best = min(best, float(ms[0]) / iters)
-            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "depth": depth, "min_wg_per_cu": minb,
-                          "sweeps_per_pass": nit, "planes": p, "us_per_sweep": round(1e3 * best, 2)})
+            table.append({"variant": v, "family": fam, "tile": [ti, tj], "threads": nt, "depth": depth,
+                          "min_wg_per_cu": minb, "sweeps_per_pass": nit, "planes": p,
+                          "us_per_sweep": round(1e3 * best, 2)})
     best = min(table, key=lambda r: r["us_per_sweep"])
     if install:
         H.call("biocfd_mixed_config", best["variant"], best["planes"], sweeps, residual_tol)

@@ -128,8 +128,9 @@ int biocfd_correct(int blk, double dt);
 int biocfd_sor_config(int variant, int planes_per_workgroup);
 /* info = {tile_i, tile_j, threads per workgroup, z-plane prefetch depth,
  * workgroups per CU its register budget is compiled for, iterations per
- * pass} of a variant (zeros and 1 for variant 0); fails past the last one */
-int biocfd_sor_variant_info(int variant, int info[6]);
+ * pass, tile family (0: per-phase cell lists, 1: pair-owned cells)} of a
+ * variant (zeros and 1 for variant 0); fails past the last one */
+int biocfd_sor_variant_info(int variant, int info[7]);
 /* Autotuning probe (no reference counterpart): time `iters` SOR iterations
  * of a variant on block `blk`'s current pc and b (milliseconds, HIP events);
  * pc and the block's SOR state are restored afterwards. */
@@ -145,7 +146,7 @@ int biocfd_sor_time(int blk, int variant, int planes_per_workgroup, int iters, d
 int biocfd_set_precision(int mode);
 int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_correction,
                         double residual_tol);
-int biocfd_mixed_variant_info(int variant, int info[6]);
+int biocfd_mixed_variant_info(int variant, int info[7]);
 /* time `iters` FP32 sweeps of a variant on the residual of block `blk`'s pc */
 int biocfd_mixed_time(int blk, int variant, int planes_per_workgroup, int iters, double dt, double *ms);
 /* max over fluid cells of |b/dt - A pc| for block `blk` (FP64 diagnostic) */

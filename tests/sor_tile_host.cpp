@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "sor_fused.hpp"
+#include "sor_pair.hpp"
 
 using namespace bcfd;
 
@@ -165,10 +166,54 @@ static double fused_launch(const Problem<Real> &p, int KS, const std::vector<Rea
   return delta;
 }
 
+// the same launch with the pair-owned tile (sor_pair.hpp): the rhs buffer of
+// a step alternates with the step's parity from the first step
+template <typename Real, int TI, int TJ, int NT, int NIT, bool CREG>
+static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real> &src, std::vector<Real> &dst,
+                          bool zero_src) {
+  using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
+  FusedArgsT<Real> a = args_of(p, KS);
+  a.src = src.data();
+  a.dst = dst.data();
+  a.zero_src = zero_src ? 1 : 0;
+  const int gx = (p.d.nx + TI - 1) / TI, gy = (p.d.ny + TJ - 1) / TJ, gz = (p.d.nz + KS - 1) / KS;
+  const int n = gx * gy * gz;
+  std::vector<Real> lds(T::LDS_DOUBLES);
+  std::vector<typename T::Thread> th(NT);
+  double delta = 0.0;
+  for (int L = 0; L < n; ++L) {
+    const typename T::Tile tile = T::tile_of(a, T::xcd_tile(L, n), gx, gy);
+    for (auto &v : lds) v = Real(std::nan(""));
+    for (int tid = 0; tid < NT; ++tid) {
+      T::init_thread(a, tile, tid, th[tid]);
+      T::prologue(a, tile, lds.data(), th[tid], tid);
+    }
+    const int s0 = T::first_step(tile), s1 = T::last_step(tile);
+    for (int s = s0; s <= s1; ++s) {
+      const bool odd = ((s - s0) & 1) != 0;
+      for (int tid = 0; tid < NT; ++tid) {
+        if (odd)
+          T::template phase_a<1>(a, tile, lds.data(), th[tid], s);
+        else
+          T::template phase_a<0>(a, tile, lds.data(), th[tid], s);
+      }
+      for (int ph = 0; ph < T::NPH; ++ph)
+        for (int tid = 0; tid < NT; ++tid) {
+          const double dd = odd ? T::template phase_u<1>(a, tile, lds.data(), th[tid], ph, s, ph >= T::NPH - 2)
+                                : T::template phase_u<0>(a, tile, lds.data(), th[tid], ph, s, ph >= T::NPH - 2);
+          delta = delta > dd ? delta : dd;
+        }
+    }
+  }
+  return delta;
+}
+
 // `launches` fused launches (NIT iterations each) against the in-place
 // sweep; with zero_first the first starts from a zero iterate without reading
 // it (the first smoothing pass of a mixed-precision correction)
-template <typename Real, int TI, int TJ, int NT, int D, int NIT>
+// Pair: 0 = SorTile, 1 = SorPairTile with coefficient registers, 2 = with
+// coefficients in LDS
+template <typename Real, int TI, int TJ, int NT, int D, int NIT, int Pair = 0>
 static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_first) {
   Problem<Real> p = make_problem<Real>(nx, ny, nz, 0.1);
   std::vector<Real> ref = p.pc, A = p.pc, B = p.pc;
@@ -181,24 +226,31 @@ static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_firs
     const bool z = zero_first && it == 0;
     double dref = 0.0;
     for (int m = 0; m < NIT; ++m) dref = inplace_iteration(p, ref);
-    const double dfus = (it & 1) ? fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, B, A, z)
-                                 : fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, A, B, z);
+    double dfus;
+    if (Pair)
+      dfus = (it & 1) ? pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, B, A, z)
+                      : pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, A, B, z);
+    else
+      dfus = (it & 1) ? fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, B, A, z)
+                      : fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, A, B, z);
     const std::vector<Real> &cur = (it & 1) ? A : B;
     const bool norm_ok = sizeof(Real) == sizeof(double) ? dref == dfus : dfus == 0.0;
     if (std::memcmp(cur.data(), ref.data(), sizeof(Real) * ref.size()) != 0 || !norm_ok) {
       int64_t bad = -1;
       for (size_t q = 0; q < ref.size(); ++q)
         if (std::memcmp(&cur[q], &ref[q], sizeof(Real))) { bad = (int64_t)q; break; }
-      std::printf("MISMATCH %s tile %dx%d/%d depth %d x%d grid %dx%dx%d KS=%d launch %d: first cell %lld, "
+      std::printf("MISMATCH %s%s tile %dx%d/%d depth %d x%d grid %dx%dx%d KS=%d launch %d: first cell %lld, "
                   "delta %.17g vs %.17g\n",
-                  sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, NIT, nx, ny, nz, KS, it + 1, (long long)bad,
+                  sizeof(Real) == 8 ? "fp64" : "fp32", Pair ? (Pair == 1 ? " pair" : " pair-lds") : "", TI, TJ, NT,
+                  D, NIT, nx, ny, nz, KS, it + 1,
+                  (long long)bad,
                   dref, dfus);
       return 1;
     }
     if (z) std::fill(A.begin(), A.end(), Real(0));  // later launches read A: zero ghosts as the runtime keeps them
   }
-  std::printf("ok %s tile %dx%d/%d depth %d x%d grid %dx%dx%d KS=%d launches=%d%s\n",
-              sizeof(Real) == 8 ? "fp64" : "fp32", TI, TJ, NT, D, NIT, nx, ny, nz, KS, launches,
+  std::printf("ok %s%s tile %dx%d/%d depth %d x%d grid %dx%dx%d KS=%d launches=%d%s\n",
+              sizeof(Real) == 8 ? "fp64" : "fp32", Pair ? (Pair == 1 ? " pair" : " pair-lds") : "", TI, TJ, NT, D, NIT, nx, ny, nz, KS, launches,
               zero_first ? " (zero first)" : "");
   return 0;
 }
@@ -212,6 +264,20 @@ static int run_shape() {
   bad += run_case<Real, TI, TJ, NT, D, NIT>(5, 3, 7, 1, 3, false);
   bad += run_case<Real, TI, TJ, NT, D, NIT>(19, 9, 8, 2, 3, false);
   bad += run_case<Real, TI, TJ, NT, D, NIT>(TI + 6, 2 * TJ + 3, 11, 3, 4, true);
+  return bad;
+}
+
+// the pair-owned tile (k_sor_pair)
+template <typename Real, int TI, int TJ, int NT, int NIT, int CREG>
+static int run_pair() {
+  constexpr int P = CREG ? 1 : 2;
+  int bad = 0;
+  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(37, 21, 13, 4, 3, false);
+  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI, TJ, 9, 32, 2, false);
+  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, false);
+  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(5, 3, 7, 1, 3, false);
+  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(19, 9, 8, 2, 3, false);
+  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, true);
   return bad;
 }
 
@@ -256,6 +322,23 @@ int main() {
   bad += run_shape<float, 64, 32, 1024, 1, 2>();
   bad += run_shape<float, 128, 32, 1024, 1>();
   bad += run_shape<float, 128, 16, 1024, 1>();
+  // pair-owned tiles
+  bad += run_pair<double, 64, 32, 512, 2, 1>();
+  bad += run_pair<double, 64, 32, 512, 1, 1>();
+  bad += run_pair<double, 64, 32, 512, 1, 0>();
+  bad += run_pair<double, 64, 28, 512, 1, 0>();
+  bad += run_pair<double, 64, 28, 512, 1, 1>();
+  bad += run_pair<double, 64, 24, 512, 1, 0>();
+  bad += run_pair<double, 64, 32, 512, 2, 0>();
+  bad += run_pair<double, 64, 16, 256, 1, 1>();
+  bad += run_pair<float, 64, 32, 512, 1, 1>();
+  bad += run_pair<float, 64, 32, 512, 2, 1>();
+  bad += run_pair<float, 64, 28, 512, 1, 1>();
+  bad += run_pair<float, 64, 28, 512, 2, 0>();
+  bad += run_pair<float, 64, 28, 512, 1, 0>();
+  bad += run_pair<float, 64, 32, 512, 1, 0>();
+  bad += run_pair<float, 64, 32, 512, 2, 0>();
+  bad += run_pair<float, 128, 16, 512, 1, 1>();
   std::printf(bad ? "FAILED %d\n" : "all cases bit-identical\n", bad);
   return bad ? 1 : 0;
 }

@@ -9,7 +9,7 @@ from parity import compare, run_oracle, small_cases
 pytestmark = pytest.mark.gpu
 
 # the library's default SOR variant, restored after each test
-DEFAULT_VARIANT = 7
+DEFAULT_VARIANT = 21
 
 
 def n_variants():

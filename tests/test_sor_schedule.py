@@ -40,11 +40,16 @@ def test_every_compiled_shape_is_emulated():
     emu = open(os.path.join(ROOT, "tests", "sor_tile_host.cpp")).read()
     run = set()
     for t in re.findall(r"run_shape<(\w+), (\d+), (\d+), (\d+), (\d+)(?:, (\d+))?>\(\)", emu):
-        run.add(t[:5] + (t[5] or "1",))
+        run.add(("cells",) + t[:5] + (t[5] or "1",))
+    for t in re.findall(r"run_pair<(\w+), (\d+), (\d+), (\d+), (\d+), (\d)>\(\)", emu):
+        run.add(("pair",) + t)
     lib = set()
-    for real, src, macro in (("double", "k_pcor.hip", "FV"), ("float", "k_mixed.hip", "SV")):
+    for real, src, macro, pmacro in (("double", "k_pcor.hip", "FV", "PV"), ("float", "k_mixed.hip", "SV", "PS")):
         text = open(os.path.join(csrc, src)).read()
         for ti, tj, nt, d, nit in re.findall(macro + r"\((\d+), (\d+), (\d+), (\d+), \d+, (\d+)\),", text):
-            lib.add((real, ti, tj, nt, d, nit))
-            lib.add((real, ti, tj, nt, d, "1"))
+            lib.add(("cells", real, ti, tj, nt, d, nit))
+            lib.add(("cells", real, ti, tj, nt, d, "1"))
+        for ti, tj, nt, nit, creg in re.findall(pmacro + r"\((\d+), (\d+), (\d+), \d+, (\d+), (\d)\),", text):
+            lib.add(("pair", real, ti, tj, nt, nit, creg))
+            lib.add(("pair", real, ti, tj, nt, "1", creg))
     assert len(lib) > 15 and lib <= run, sorted(lib - run)
