@@ -16,7 +16,10 @@
 // LDS (fewer registers, so more workgroups per CU).
 // Whether the cell is updated (inside the phase's region and the interior
 // in i and j) is a precomputed bit per (phase, parity); a pair whose cell is
-// not updated writes to a sink word, so the phase has no branches.
+// not updated writes to a sink word, so the phase has no branches.  The
+// pairs are numbered from the tile outwards, ring by ring of the shrinking
+// phase regions, so the pairs a phase works on are a prefix of the
+// numbering: a thread's pair slots past it drop out at compile time.
 //
 // All loads of step s+1 (the plane s+2 of the iterate with its Acz row, the
 // rhs of every phase) are issued in phase A of step s into registers that
@@ -26,12 +29,49 @@
 //
 // Real = double is the reference's SOR (bit-identical: the operand order of
 // sor_cell, Acx(c) + Acy(c) + Acz(c) summed left to right); Real = float the
-// smoothing sweeps of the mixed-precision solve.
+// smoothing sweeps of the mixed-precision solve, which accumulate the
+// stencil with fused multiply-adds and divide by the diagonal through the
+// hardware reciprocal (v_rcp_f32, about 1 ulp): rounding-level changes of
+// the FP32 smoother, whose corrections the FP64 residual of the defect
+// correction checks.
 #pragma once
 
 #include "sor_fused.hpp"
 
 namespace bcfd {
+
+// 1/den in FP32: the hardware reciprocal on the GPU, 1/den on the host (the
+// host run checks the schedule, not the reciprocal's last bit)
+__host__ __device__ inline float sor_rcp(float den) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(den);
+#else
+  return 1.0f / den;
+#endif
+}
+
+// the SOR update of one cell; den = (Acx + Acy) + Acz of the cell.  FP64:
+// the operand order of PcorVcor.F90:318-327 (sor_cell), bit-identical.
+// FP32: the stencil as a chain of fused multiply-adds (IEEE fma, the same
+// bits on host and GPU), the reciprocal, and old + omega (gs - old).
+__host__ __device__ inline double sor_update(double omega, double rhs, double old, double axm, double pw, double axp,
+                                             double pe, double aym, double ps, double ayp, double pn, double azm,
+                                             double pb, double azp, double pt, double den) {
+  const double gs = (rhs - axm * pw - axp * pe - aym * ps - ayp * pn - azm * pb - azp * pt) / den;
+  return (1.0 - omega) * old + omega * gs;
+}
+__host__ __device__ inline float sor_update(float omega, float rhs, float old, float axm, float pw, float axp,
+                                            float pe, float aym, float ps, float ayp, float pn, float azm, float pb,
+                                            float azp, float pt, float den) {
+  float num = fmaf(-axm, pw, rhs);
+  num = fmaf(-axp, pe, num);
+  num = fmaf(-aym, ps, num);
+  num = fmaf(-ayp, pn, num);
+  num = fmaf(-azm, pb, num);
+  num = fmaf(-azp, pt, num);
+  const float gs = num * sor_rcp(den);
+  return fmaf(omega, gs - old, old);
+}
 
 template <typename Real, int TI, int TJ, int NT, int NIT, bool CREG = true>
 struct SorPairTile {
@@ -58,6 +98,46 @@ struct SorPairTile {
   static_assert(kYOdd < 16 && SW < 256 && SH < 256, "pair mask layout");
   static constexpr int NQC = CREG ? NQ : 1;         // pairs with coefficient registers
 
+  // the pairs with a cell in phase p's region form the rectangle of slot
+  // rows [ry0, ry1] and pairs [rh0, rh1]; numbered innermost (phase NPH-1,
+  // the tile itself) first, so phase p works on pairs 0 .. cnt(p)-1, in
+  // slots q < used(p) of its threads (slot q holds pair tid + q*NT)
+  __host__ __device__ static constexpr int ry0(int p) { return p + 1; }
+  __host__ __device__ static constexpr int ry1(int p) { return TJ + 4 * NIT - 2 - p; }
+  __host__ __device__ static constexpr int rh0(int p) { return (p + 1) / 2; }
+  __host__ __device__ static constexpr int rh1(int p) { return (TI + 4 * NIT - 2 - p) / 2; }
+  __host__ __device__ static constexpr int cnt(int p) { return (ry1(p) - ry0(p) + 1) * (rh1(p) - rh0(p) + 1); }
+  __host__ __device__ static constexpr int used(int p) { return (cnt(p) + NT - 1) / NT; }
+  __host__ __device__ static constexpr bool full(int p, int q) { return (q + 1) * NT <= cnt(p); }
+  static_assert(cnt(0) == NPAIR, "phase 0 covers every pair");
+
+  // pair number e (0 <= e < NPAIR) -> pair h of slot row y
+  __host__ __device__ static void pair_of(int e, int &h, int &y) {
+    int p = NPH - 1;
+    while (e >= cnt(p)) --p;  // the ring of level p: cnt(p+1) <= e < cnt(p)
+    const int oy0 = ry0(p), oy1 = ry1(p), ox0 = rh0(p), ox1 = rh1(p), W = ox1 - ox0 + 1;
+    if (p == NPH - 1) {
+      y = oy0 + e / W;
+      h = ox0 + e % W;
+      return;
+    }
+    const int r = e - cnt(p + 1);
+    const int iy0 = ry0(p + 1), iy1 = ry1(p + 1), ix0 = rh0(p + 1), ix1 = rh1(p + 1);
+    const int nTop = (iy0 - oy0) * W, mL = ix0 - ox0, m = mL + (ox1 - ix1), nMid = (iy1 - iy0 + 1) * m;
+    if (r < nTop) {
+      y = oy0 + r / W;
+      h = ox0 + r % W;
+    } else if (r < nTop + nMid) {
+      const int r2 = r - nTop, c = r2 % m;
+      y = iy0 + r2 / m;
+      h = c < mL ? ox0 + c : ix1 + 1 + (c - mL);
+    } else {
+      const int r3 = r - nTop - nMid;
+      y = iy1 + 1 + r3 / W;
+      h = ox0 + r3 % W;
+    }
+  }
+
   struct Tile {
     int iL, jL, k0, k1;
   };
@@ -70,6 +150,7 @@ struct SorPairTile {
     int ld_off[NL];          // in-plane offset of the loaded slot cell, -1 outside the array
     int ld_lds[NL];          // LDS index of the slot cell in the ring slot of an even plane, -1 past the slot
     int czm;                 // Acz column this thread carries (0 .. 2), -1 for none
+    int tid;
     int c[NQ];               // slot index y*SW + h of each pair
     unsigned mask[NQ];
     int roff[NQ];            // rhs in-plane offset of column 2h (column 2h+1: + 1)
@@ -117,11 +198,13 @@ struct SorPairTile {
       }
     }
     th.czm = tid < 3 ? tid : -1;
+    th.tid = tid;
     const int nxh = a.NXh * a.d.NY;
     for (int q = 0; q < NQ; ++q) {
       const int e = tid + q * NT;
       const bool real = e < NPAIR;
-      const int y = real ? 1 + e / SW : 1, h = real ? e % SW : 0;
+      int y = 1, h = 0;
+      if (real) pair_of(e, h, y);
       const int j = T.jL + y, i0 = T.iL + 2 * h;
       const bool jin = j >= 2 && j <= a.d.ny + 1;
       unsigned m = ((y & 1) ? 1u << kYOdd : 0u) | (unsigned)h << 16 | (unsigned)y << 24;
@@ -190,6 +273,7 @@ struct SorPairTile {
       const Real *B = (colour ? a.bred : a.bblack) + (kin ? k - 1 : 1) * plane;
       const unsigned cbit = (unsigned)((colour ^ k) & 1);
       for (int q = 0; q < NQ; ++q) {
+        if (q >= used(p)) break;
         const unsigned par = ((th.mask[q] >> kYOdd) ^ cbit) & 1u;
         rb[p][q] = B[th.roff[q] + (int)par];
       }
@@ -253,6 +337,8 @@ struct SorPairTile {
     Real *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
     double delta = 0.0;
     for (int q = 0; q < NQ; ++q) {
+      if (q >= used(p)) break;
+      if (!full(p, q) && th.tid + q * NT >= cnt(p)) continue;
       const unsigned m = th.mask[q];
       const int par = (int)(((m >> kYOdd) ^ cbit) & 1u);
       const int c = th.c[q];
@@ -277,9 +363,8 @@ struct SorPairTile {
         ayp = cy[2 * CYW];
         sxy = cx[CXW] + cy[CYW];
       }
-      // sor_cell's operand order; the centre sum (Acx + Acy) + Acz
-      const Real gs = (rhs - axm * pw - axp * pe - aym * ps - ayp * pn - czm * pb - czp * pt) / (sxy + czc);
-      const Real nw = (Real(1) - omega) * old + omega * gs;
+      // the centre sum (Acx + Acy) + Acz, as sor_cell's
+      const Real nw = sor_update(omega, rhs, old, axm, pw, axp, pe, aym, ps, ayp, pn, czm, pb, czp, pt, sxy + czc);
       const bool upd = kin && ((m >> (2 * p + par)) & 1u) && !sor_skip(rhs);
       if (kNorm && norm) {
         const double dd = upd ? fabs(double(nw - old)) : 0.0;

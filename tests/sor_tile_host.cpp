@@ -94,8 +94,10 @@ static FusedArgsT<Real> args_of(const Problem<Real> &p, int KS) {
   return a;
 }
 
-// textbook in-place red-black iteration; returns max |change|
-template <typename Real>
+// textbook in-place red-black iteration; returns max |change|.  With RCP the
+// update is sor_update's (the pair-owned tiles': FP64 identical to
+// sor_cell, FP32 with fused multiply-adds and the reciprocal)
+template <typename Real, bool RCP = false>
 static double inplace_iteration(const Problem<Real> &p, std::vector<Real> &pc) {
   const FusedArgsT<Real> a = args_of(p, 1);
   const Dims &d = p.d;
@@ -111,9 +113,16 @@ static double inplace_iteration(const Problem<Real> &p, std::vector<Real> &pc) {
           if (sor_skip(rhs)) continue;
           const int64_t c = ix3(d, i, j, k);
           const Real old = pc[c];
-          const Real nw = sor_cell<Real>(Real(a.omega), coef_at<Real>(a.Ax, d.nx, i), coef_at<Real>(a.Ay, d.ny, j),
-                                         coef_at<Real>(a.Az, d.nz, k), rhs, old, pc[c - 1], pc[c + 1], pc[c - sy],
-                                         pc[c + sy], pc[c - sz], pc[c + sz]);
+          const Coef3T<Real> ax = coef_at<Real>(a.Ax, d.nx, i), ay = coef_at<Real>(a.Ay, d.ny, j),
+                             az = coef_at<Real>(a.Az, d.nz, k);
+          Real nw;
+          if (RCP) {
+            nw = sor_update(Real(a.omega), rhs, old, ax.m, pc[c - 1], ax.p, pc[c + 1], ay.m, pc[c - sy], ay.p,
+                            pc[c + sy], az.m, pc[c - sz], az.p, pc[c + sz], ax.c + ay.c + az.c);
+          } else {
+            nw = sor_cell<Real>(Real(a.omega), ax, ay, az, rhs, old, pc[c - 1], pc[c + 1], pc[c - sy], pc[c + sy],
+                                pc[c - sz], pc[c + sz]);
+          }
           pc[c] = nw;
           const double dd = std::fabs(double(nw - old));
           delta = delta > dd ? delta : dd;
@@ -172,6 +181,23 @@ template <typename Real, int TI, int TJ, int NT, int NIT, bool CREG>
 static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real> &src, std::vector<Real> &dst,
                           bool zero_src) {
   using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
+  {
+    // the pair numbering is a bijection onto slot rows 1..SH-2 x pairs
+    // 0..SW-1, and the pairs of phase p (a prefix) have a cell in its region
+    std::vector<int> hit((size_t)T::SH * T::SW, 0);
+    for (int e = 0; e < T::NPAIR; ++e) {
+      int h, y;
+      T::pair_of(e, h, y);
+      int lvl = T::NPH - 1;
+      while (e >= T::cnt(lvl)) --lvl;
+      const bool in = y >= 1 && y <= T::SH - 2 && h >= 0 && h < T::SW && y >= T::ry0(lvl) && y <= T::ry1(lvl) &&
+                      h >= T::rh0(lvl) && h <= T::rh1(lvl);
+      if (!in || hit[(size_t)y * T::SW + h]++) {
+        std::printf("pair numbering broken at %d -> (%d, %d)\n", e, h, y);
+        std::exit(2);
+      }
+    }
+  }
   FusedArgsT<Real> a = args_of(p, KS);
   a.src = src.data();
   a.dst = dst.data();
@@ -225,7 +251,7 @@ static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_firs
   for (int it = 0; it < launches; ++it) {
     const bool z = zero_first && it == 0;
     double dref = 0.0;
-    for (int m = 0; m < NIT; ++m) dref = inplace_iteration(p, ref);
+    for (int m = 0; m < NIT; ++m) dref = inplace_iteration<Real, Pair != 0>(p, ref);
     double dfus;
     if (Pair)
       dfus = (it & 1) ? pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, B, A, z)
