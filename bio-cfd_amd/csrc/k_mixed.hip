@@ -262,6 +262,8 @@ static const Sweep32Variant kSweep32[] = {
     PS(64, 28, 512, 4, 2, 0),
     PS(64, 32, 512, 4, 2, 0),
     PS(128, 16, 512, 4, 1, 1),
+    PS(64, 32, 1024, 1, 2, 1),
+    PS(64, 32, 1024, 1, 1, 1),
 };
 #undef SV
 #undef PS
@@ -269,9 +271,9 @@ constexpr int kNumSweep32 = (int)(sizeof(kSweep32) / sizeof(kSweep32[0]));
 
 struct MixedConfig {
   int precision = 0;        // 0: FP64 method, 1: mixed
-  int variant = 19;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 2 sweeps per pass, 2 WG/CU)
+  int variant = 15;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 512 threads, 2 WG/CU)
   int planes = 16;
-  int sweeps = 16;          // FP32 sweeps per correction
+  int sweeps = 32;          // FP32 sweeps per correction
   double tol = 1e-8;        // max-norm of the FP64 residual
 };
 static MixedConfig g_mx;

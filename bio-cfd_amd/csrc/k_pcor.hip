@@ -447,6 +447,10 @@ static const FusedVariant kFused[] = {
     PV(64, 28, 512, 4, 1, 0),
     PV(64, 24, 512, 4, 1, 0),
     PV(64, 32, 512, 1, 2, 0),
+    PV(64, 32, 1024, 1, 2, 0),
+    PV(64, 32, 1024, 1, 2, 1),
+    PV(128, 16, 1024, 1, 2, 0),
+    PV(64, 32, 1024, 1, 1, 0),
 };
 #undef FV
 #undef PV

@@ -357,6 +357,14 @@ int main() {
   bad += run_pair<double, 64, 24, 512, 1, 0>();
   bad += run_pair<double, 64, 32, 512, 2, 0>();
   bad += run_pair<double, 64, 16, 256, 1, 1>();
+  bad += run_pair<double, 64, 32, 1024, 2, 0>();
+  bad += run_pair<double, 64, 32, 1024, 1, 0>();
+  bad += run_pair<double, 64, 32, 1024, 2, 1>();
+  bad += run_pair<double, 64, 32, 1024, 1, 1>();
+  bad += run_pair<double, 128, 16, 1024, 2, 0>();
+  bad += run_pair<double, 128, 16, 1024, 1, 0>();
+  bad += run_pair<float, 64, 32, 1024, 2, 1>();
+  bad += run_pair<float, 64, 32, 1024, 1, 1>();
   bad += run_pair<float, 64, 32, 512, 1, 1>();
   bad += run_pair<float, 64, 32, 512, 2, 1>();
   bad += run_pair<float, 64, 28, 512, 1, 1>();

@@ -15,9 +15,20 @@ from parity import run_oracle, small_cases
 pytestmark = pytest.mark.gpu
 
 STEPS = 3
-# max |mixed - fp64| / max |fp64| allowed per field
-TOL = {"u": 1e-7, "v": 1e-7, "w": 1e-7, "ut": 1e-7, "vt": 1e-7, "wt": 1e-7, "p": 1e-7, "pc": 1e-6, "b": 1e-6}
-TOL_OTHER = 1e-6          # every other FP64 array (ghosts, normal distances, forces ...)
+# the library's mixed-precision defaults (variant, planes, sweeps per correction, residual tolerance),
+# restored after each test
+DEFAULT = (15, 16, 32, 1e-8)
+# (variant, planes, sweeps per correction): the per-phase cell-list FP32 kernel at 4 sweeps and the
+# pair-owned FP32 kernel (fused multiply-adds, hardware reciprocal) at the default 32
+CONFIGS = [(1, 32, 4), (15, 16, 32)]
+# max |mixed - fp64| / max |fp64| allowed per field.  The FP32 rounding the corrections carry grows
+# with the sweeps per correction: at 4 sweeps the fields stay within 1e-8 of the FP64 run, at 32 within
+# 2e-7 (velocities, p) on these cases, so the second set is the accuracy contract of the mixed path
+# (1e-6 relative velocity error) with the derived arrays a decade looser.
+TOL = {4: {"u": 1e-7, "v": 1e-7, "w": 1e-7, "ut": 1e-7, "vt": 1e-7, "wt": 1e-7, "p": 1e-7, "pc": 1e-6, "b": 1e-6},
+       32: {"u": 1e-6, "v": 1e-6, "w": 1e-6, "ut": 1e-6, "vt": 1e-6, "wt": 1e-6, "p": 1e-6, "pc": 1e-5,
+            "b": 1e-5}}
+TOL_OTHER = {4: 1e-6, 32: 1e-5}  # every other FP64 array (ghosts, normal distances, forces ...)
 SKIP = ("derr2", "pco")   # derr2 is the residual norm in the mixed solve, pco is not kept
 
 
@@ -28,8 +39,9 @@ def rel_err(ref, got):
     return float(np.max(np.abs(got - ref))) / scale if ref.size else 0.0
 
 
+@pytest.mark.parametrize("config", CONFIGS)
 @pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing"])
-def test_mixed_step_against_fp64_oracle(name):
+def test_mixed_step_against_fp64_oracle(name, config):
     from biocfd_amd import hip as H
     from biocfd_amd.driver import Simulation
 
@@ -37,13 +49,13 @@ def test_mixed_step_against_fp64_oracle(name):
     ref = run_oracle(case, (STEPS,))[STEPS]
     sim = Simulation(case, precision="mixed")
     try:
-        H.call("biocfd_mixed_config", 1, 32, 4, 0.0)
+        H.call("biocfd_mixed_config", *config, 0.0)
         for _ in range(STEPS):
             sim.step()
         got = sim.dump()
     finally:
         sim.close()
-        H.call("biocfd_mixed_config", 1, 32, 4, 1e-8)
+        H.call("biocfd_mixed_config", *DEFAULT)
     bad, report = [], []
     for g, (r, o) in enumerate(zip(ref, got)):
         for k, x in r.items():
@@ -58,16 +70,18 @@ def test_mixed_step_against_fp64_oracle(name):
                     bad.append(f"block {g + 1} {k}: {np.count_nonzero(x != y)} integer mismatches")
             else:
                 e = rel_err(x, y)
+                tol = TOL[config[2]].get(k, TOL_OTHER[config[2]])
                 report.append(f"{k}={e:.2e}")
-                if not e <= TOL.get(k, TOL_OTHER):
-                    bad.append(f"block {g + 1} {k}: relative error {e:.3e} > {TOL.get(k, TOL_OTHER):.0e}")
-    print(f"{name}: " + " ".join(report))
+                if not e <= tol:
+                    bad.append(f"block {g + 1} {k}: relative error {e:.3e} > {tol:.0e}")
+    print(f"{name} {config}: " + " ".join(report))
     assert not bad, "\n".join(bad[:40])
 
 
-def test_mixed_stops_at_residual_tolerance():
-    """With a residual tolerance the correction loop stops early and the
-    final FP64 residual is below it."""
+@pytest.mark.parametrize("config", CONFIGS)
+def test_mixed_stops_at_residual_tolerance(config):
+    """With a residual tolerance the correction loop stops early (at the
+    end of a correction) and the final FP64 residual is below it."""
     from biocfd_amd import cases as CS
     from biocfd_amd import hip as H
     from biocfd_amd.driver import Simulation
@@ -75,14 +89,14 @@ def test_mixed_stops_at_residual_tolerance():
     case = CS.config_sphere_64(n=24, markers=256, nsteps=2, pc_ita_max=400, epsi=0.0)
     sim = Simulation(case, precision="mixed")
     try:
-        H.call("biocfd_mixed_config", 1, 32, 4, 1e-6)
+        H.call("biocfd_mixed_config", *config, 1e-6)
         sim.step()
         d = sim.dump()[0]
         r = sim.residual_norm(1)
     finally:
         sim.close()
-        H.call("biocfd_mixed_config", 1, 32, 4, 1e-8)
+        H.call("biocfd_mixed_config", *DEFAULT)
     assert int(d["nIterPcor"]) < 401
-    assert int(d["nIterPcor"]) % 4 == 0
+    assert int(d["nIterPcor"]) % config[2] == 0
     assert r < 1e-6
     assert abs(float(d["derr2"]) - r) <= 1e-12 * max(1.0, r)
