@@ -77,6 +77,12 @@ struct IbData {
   double *node[3] = {nullptr, nullptr, nullptr};  // xnode1..znode1
   double *elem[BE_NELEM] = {};                // xcent..element_length
   std::vector<int> h_surf;
+  // uniform grid of the element centroids, counting-sorted by bin (rebuilt
+  // whenever the centroids are): bin grid, first entry of each bin, element
+  // of each entry, per-bin counters
+  void *bin_grid = nullptr;
+  int *bin_start = nullptr, *bin_elem = nullptr, *bin_fill = nullptr;
+  bool bins_ready = false;
   // per IB cell (capacity grows)
   int64_t cap_ib = 0;
   int64_t n_mapped = 0;                       // entries of ib_ijk currently in ib_map

@@ -8,8 +8,11 @@
 // reference's serial i-fastest order, so they are bit-identical to the CPU
 // build instead of following atomic-capture order.  Nearest-element searches
 // keep the reference's first-minimum tie-break (strict <, lowest index wins):
-// every candidate is scanned in element order by the thread that owns the
-// cell, with the element coordinates staged through LDS in tiles.
+// the set-up tagging of the search box scans every element in order (LDS
+// tiles of centroids); the per-step searches near the body (retagging,
+// normal distances) go through a uniform grid of the centroids, one
+// wavefront per cell (bin_search.hpp), with the same result.
+#include "bin_search.hpp"
 #include "biocfd_rt.hpp"
 
 namespace bcfd {
@@ -394,23 +397,129 @@ __global__ void k_reset_ib(Dims d, const int *ijk, int64_t cap, int64_t n, uint8
   if (q < n) cell[ix3(d, ijk[q], ijk[q + cap], ijk[q + 2 * cap])] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_retag(Geo g, const int *ijk, int64_t cap, int64_t n, uint8_t *cell,
-                                               uint8_t *node_tag) {
-  __shared__ double sx[kElemTile], sy[kElemTile], sz[kElemTile];
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = q < n * 27;
-  int i = 2, j = 2, k = 2;
-  if (active) {
-    const int64_t nn = q / 27;
-    const int r = (int)(q % 27);
-    i = ijk[nn] + (r % 3) - 1;
-    j = ijk[nn + cap] + ((r / 3) % 3) - 1;
-    k = ijk[nn + 2 * cap] + (r / 9) - 1;
+// ---- uniform grid of the centroids ----
+// bounding box of the centroids -> bin grid; bin counters to zero
+__global__ void __launch_bounds__(1024) k_bin_setup(const double *xc, const double *yc, const double *zc,
+                                                    int64_t ne, BinGrid *bg, int *fill) {
+  __shared__ double s_mn[3][16], s_mx[3][16];
+  double mn[3] = {1e308, 1e308, 1e308}, mx[3] = {-1e308, -1e308, -1e308};
+  for (int64_t e = threadIdx.x; e < ne; e += blockDim.x) {
+    const double v[3] = {xc[e], yc[e], zc[e]};
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = v[a] < mn[a] ? v[a] : mn[a];
+      mx[a] = v[a] > mx[a] ? v[a] : mx[a];
+    }
   }
-  int nc, np;
-  nearest_pair(g, active, g.xp[i - 1], g.yp[j - 1], g.zp[k - 1], g.x1[i - 1], g.y1[j - 1], g.z1[k - 1], nc, np,
-               sx, sy, sz);
-  if (active) tag_from(g, cell, node_tag, i, j, k, nc, np);
+  for (int a = 0; a < 3; ++a)
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double o1 = __shfl_xor(mn[a], off), o2 = __shfl_xor(mx[a], off);
+      mn[a] = o1 < mn[a] ? o1 : mn[a];
+      mx[a] = o2 > mx[a] ? o2 : mx[a];
+    }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int a = 0; a < 3; ++a) { s_mn[a][w] = mn[a]; s_mx[a][w] = mx[a]; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+      for (int a = 0; a < 3; ++a) {
+        mn[a] = s_mn[a][q] < mn[a] ? s_mn[a][q] : mn[a];
+        mx[a] = s_mx[a][q] > mx[a] ? s_mx[a][q] : mx[a];
+      }
+    *bg = bin_grid_of(mn, mx);
+  }
+  for (int q = threadIdx.x; q < kMaxBins; q += blockDim.x) fill[q] = 0;
+}
+
+__global__ void k_bin_count(const double *xc, const double *yc, const double *zc, int64_t ne, const BinGrid *bg,
+                            int *fill) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) atomicAdd(&fill[bin_of(*bg, xc[e], yc[e], zc[e])], 1);
+}
+
+// exclusive scan of the bin counts into start[0..nb]; the counters become
+// the bins' fill cursors
+__global__ void __launch_bounds__(1024) k_bin_scan(const BinGrid *bg, int *fill, int *start) {
+  __shared__ int s_sum[1024];
+  const int nb = bg->nx * bg->ny * bg->nz;
+  const int per = (nb + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int b0 = threadIdx.x * per, b1 = b0 + per < nb ? b0 + per : nb;
+  int sum = 0;
+  for (int b = b0; b < b1; ++b) sum += fill[b];
+  s_sum[threadIdx.x] = sum;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    const int v = threadIdx.x >= (unsigned)off ? s_sum[threadIdx.x - off] : 0;
+    __syncthreads();
+    s_sum[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = s_sum[threadIdx.x] - sum;  // exclusive prefix of this thread's chunk
+  for (int b = b0; b < b1; ++b) {
+    const int c = fill[b];
+    start[b] = run;
+    fill[b] = run;
+    run += c;
+  }
+  if (threadIdx.x == blockDim.x - 1) start[nb] = s_sum[threadIdx.x];
+}
+
+__global__ void k_bin_fill(const double *xc, const double *yc, const double *zc, int64_t ne, const BinGrid *bg,
+                           int *fill, int *belem) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < ne) belem[atomicAdd(&fill[bin_of(*bg, xc[e], yc[e], zc[e])], 1)] = (int)e;
+}
+
+struct BinView {
+  const BinGrid *bg;
+  const int *start, *elem;
+};
+
+// lexicographic (distance, element) minimum over the wavefront
+__device__ inline void wave_lexmin(double &d, int &e) {
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double od = __shfl_xor(d, off);
+    const int oe = __shfl_xor(e, off);
+    if (lex_less(od, oe, d, e)) {
+      d = od;
+      e = oe;
+    }
+  }
+}
+
+// nearest elements (1-based, 0 for none) of T points of one cell, by the
+// calling wavefront; every lane gets the results
+template <int T>
+__device__ void wave_nearest(const Geo &g, const BinView &bv, const double *px, const double *py, const double *pz,
+                             int *out) {
+  const BinGrid bg = *bv.bg;
+  const NearestQuery<T> Q = make_query<T>(bg, px, py, pz);
+  const int lane = threadIdx.x & 63;
+  double bd[T];
+  int be[T];
+  for (int s = 0; s < T; ++s) { bd[s] = kNoDist; be[s] = kNoElem; }
+  for (int r = 0;; ++r) {
+    shell_scan<T>(bg, bv.start, bv.elem, g.xc, g.yc, g.zc, Q, r, lane, kSearchLanes, bd, be);
+    for (int s = 0; s < T; ++s) wave_lexmin(bd[s], be[s]);
+    if (shell_done<T>(bg, Q, r, bd)) break;
+  }
+  for (int s = 0; s < T; ++s) out[s] = nearest_result(bd[s], be[s]);
+}
+
+// selectiveRetagging_th's re-tag of the 3x3x3 neighbourhoods (Search.F90:
+// 540-560 with tagging_th_core :305-338): one wavefront per neighbour cell
+__global__ void __launch_bounds__(256) k_retag(Geo g, BinView bv, const int *ijk, int64_t cap, int64_t n,
+                                               uint8_t *cell, uint8_t *node_tag) {
+  const int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (q >= n * 27) return;  // the whole wavefront
+  const int64_t nn = q / 27;
+  const int r = (int)(q % 27);
+  const int i = ijk[nn] + (r % 3) - 1, j = ijk[nn + cap] + ((r / 3) % 3) - 1, k = ijk[nn + 2 * cap] + (r / 9) - 1;
+  const double px[2] = {g.xp[i - 1], g.x1[i - 1]}, py[2] = {g.yp[j - 1], g.y1[j - 1]},
+               pz[2] = {g.zp[k - 1], g.z1[k - 1]};
+  int best[2];
+  wave_nearest<2>(g, bv, px, py, pz, best);
+  if ((threadIdx.x & 63) == 0) tag_from(g, cell, node_tag, i, j, k, best[0], best[1]);
 }
 
 __global__ void k_reclass(Dims d, const int *ijk, int64_t cap, int64_t n, uint8_t *cell, const uint8_t *node_tag) {
@@ -425,56 +534,30 @@ __global__ void k_reclass(Dims d, const int *ijk, int64_t cap, int64_t n, uint8_
 
 // computeNormDistance: seven nearest-element searches per IB cell and the
 // signed normal distances (Search.F90:762-858)
-__global__ void __launch_bounds__(256) k_norm_distance(Geo g, const int *ijk, int64_t cap, int64_t n,
+__global__ void __launch_bounds__(256) k_norm_distance(Geo g, BinView bv, const int *ijk, int64_t cap, int64_t n,
                                                        int *nel0, int *nel1, int *nel2, int *nel3, int *nel4,
                                                        int *nel5, int *nel6, double *nd0, double *nd1,
                                                        double *nd2, double *nd3, double *nd4, double *nd5,
                                                        double *nd6) {
-  __shared__ double sx[kElemTile], sy[kElemTile], sz[kElemTile];
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = q < n;
-  int i = 2, j = 2, k = 2;
-  if (active) { i = ijk[q]; j = ijk[q + cap]; k = ijk[q + 2 * cap]; }
+  const int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (q >= n) return;  // the whole wavefront
+  const int i = ijk[q], j = ijk[q + cap], k = ijk[q + 2 * cap];
   const double n1x = g.xp[i - 1], n2x = g.x1[i - 1], n3x = g.x1[i];
   const double n1y = g.yp[j - 1], n2y = g.y1[j - 1], n3y = g.y1[j];
   const double n1z = g.zp[k - 1], n2z = g.z1[k - 1], n3z = g.z1[k];
-  double md[7];
-  int mi[7];
-  for (int t = 0; t < 7; ++t) { md[t] = 1e14; mi[t] = 0; }
-  for (int64_t base = 0; base < g.ne; base += kElemTile) {
-    const int nt = (int)((g.ne - base) < kElemTile ? (g.ne - base) : kElemTile);
-    __syncthreads();
-    for (int t = threadIdx.x; t < nt; t += blockDim.x) {
-      sx[t] = g.xc[base + t]; sy[t] = g.yc[base + t]; sz[t] = g.zc[base + t];
-    }
-    __syncthreads();
-    if (!active) continue;
-    for (int t = 0; t < nt; ++t) {
-      const double ax1 = n1x - sx[t], ax2 = n2x - sx[t], ax3 = n3x - sx[t];
-      const double ay1 = n1y - sy[t], ay2 = n2y - sy[t], ay3 = n3y - sy[t];
-      const double az1 = n1z - sz[t], az2 = n2z - sz[t], az3 = n3z - sz[t];
-      const double X1 = ax1 * ax1, X2 = ax2 * ax2, X3 = ax3 * ax3;
-      const double Y1 = ay1 * ay1, Y2 = ay2 * ay2, Y3 = ay3 * ay3;
-      const double Z1 = az1 * az1, Z2 = az2 * az2, Z3 = az3 * az3;
-      const double dd[7] = {X1 + Y1 + Z1, X2 + Y1 + Z1, X3 + Y1 + Z1, X1 + Y2 + Z1,
-                            X1 + Y3 + Z1, X1 + Y1 + Z2, X1 + Y1 + Z3};
-      const int m = (int)(base + t) + 1;
-#pragma unroll
-      for (int s = 0; s < 7; ++s)
-        if (dd[s] < md[s]) { md[s] = dd[s]; mi[s] = m; }
-    }
-  }
-  if (!active) return;
-  int *nel[7] = {nel0, nel1, nel2, nel3, nel4, nel5, nel6};
-  double *nd[7] = {nd0, nd1, nd2, nd3, nd4, nd5, nd6};
+  // the p point and the six face points (u: x faces, v: y faces, w: z faces)
   const double px[7] = {n1x, n2x, n3x, n1x, n1x, n1x, n1x};
   const double py[7] = {n1y, n1y, n1y, n2y, n3y, n1y, n1y};
   const double pz[7] = {n1z, n1z, n1z, n1z, n1z, n2z, n3z};
-  for (int s = 0; s < 7; ++s) {
-    const int e = mi[s] - 1;
-    nel[s][q] = mi[s];
-    nd[s][q] = (px[s] - g.xc[e]) * g.ca[e] + (py[s] - g.yc[e]) * g.cb[e] + (pz[s] - g.zc[e]) * g.cg[e];
-  }
+  int mi[7];
+  wave_nearest<7>(g, bv, px, py, pz, mi);
+  const int lane = threadIdx.x & 63;
+  if (lane >= 7) return;
+  int *nel[7] = {nel0, nel1, nel2, nel3, nel4, nel5, nel6};
+  double *nd[7] = {nd0, nd1, nd2, nd3, nd4, nd5, nd6};
+  const int s = lane, e = mi[s] - 1;
+  nel[s][q] = mi[s];
+  nd[s][q] = e >= 0 ? (px[s] - g.xc[e]) * g.ca[e] + (py[s] - g.yc[e]) * g.cb[e] + (pz[s] - g.zc[e]) * g.cg[e] : 0.0;
 }
 
 // findTScells (Search.F90:434-518)
@@ -549,6 +632,30 @@ static Geo geo_of(Block &b) {
   g.ca = b.ib.elem[BE_COSA]; g.cb = b.ib.elem[BE_COSB]; g.cg = b.ib.elem[BE_COSG];
   g.ne = b.ib.n_elems;
   return g;
+}
+
+// the bin grid of the current centroids (after k_surface_norm)
+static int build_bins(Block &b) {
+  IbData &ib = b.ib;
+  hipStream_t s = rt().stream;
+  if (!ib.bin_grid && dev_alloc((char **)&ib.bin_grid, (int64_t)sizeof(BinGrid))) return -1;
+  if (!ib.bin_start && dev_alloc(&ib.bin_start, kMaxBins + 1)) return -1;
+  if (!ib.bin_fill && dev_alloc(&ib.bin_fill, kMaxBins)) return -1;
+  if (!ib.bin_elem && dev_alloc(&ib.bin_elem, ib.n_elems)) return -1;
+  const double *xc = ib.elem[BE_XCENT], *yc = ib.elem[BE_YCENT], *zc = ib.elem[BE_ZCENT];
+  BinGrid *bg = (BinGrid *)ib.bin_grid;
+  const dim3 ge(div_up(ib.n_elems, 256));
+  hipLaunchKernelGGL(k_bin_setup, dim3(1), dim3(1024), 0, s, xc, yc, zc, ib.n_elems, bg, ib.bin_fill);
+  hipLaunchKernelGGL(k_bin_count, ge, dim3(256), 0, s, xc, yc, zc, ib.n_elems, bg, ib.bin_fill);
+  hipLaunchKernelGGL(k_bin_scan, dim3(1), dim3(1024), 0, s, bg, ib.bin_fill, ib.bin_start);
+  hipLaunchKernelGGL(k_bin_fill, ge, dim3(256), 0, s, xc, yc, zc, ib.n_elems, bg, ib.bin_fill, ib.bin_elem);
+  BCFD_CHECK_LAUNCH();
+  ib.bins_ready = true;
+  return 0;
+}
+
+static BinView bins_of(const Block &b) {
+  return BinView{(const BinGrid *)b.ib.bin_grid, b.ib.bin_start, b.ib.bin_elem};
 }
 
 static Block *ib_block(int id) {
@@ -640,6 +747,7 @@ int biocfd_compute_surface_norm(int id, const double *x1, const double *y1, cons
                      ib.elem[BE_YCENT], ib.elem[BE_ZCENT], ib.elem[BE_COSA], ib.elem[BE_COSB],
                      ib.elem[BE_COSG], ib.elem[BE_ELEN], (double)inor);
   BCFD_CHECK_LAUNCH();
+  if (build_bins(*b)) return -1;
   BCFD_HIP(hipStreamSynchronize(s));  // host node buffers may be reused by the caller
   return 0;
 }
@@ -675,7 +783,8 @@ int biocfd_selective_retagging(int id, int64_t blk_mv_tag) {
     Geo g = geo_of(*b);
     hipLaunchKernelGGL(k_reset_ib, dim3(div_up(n, 256)), dim3(256), 0, s, b->d, ib.ib_ijk, ib.cap_ib, n,
                        b->tag[BT_CELL]);
-    hipLaunchKernelGGL(k_retag, dim3(div_up(n * 27, 256)), dim3(256), 0, s, g, ib.ib_ijk, ib.cap_ib, n,
+    if (!ib.bins_ready) { set_error("selective_retagging: surface not computed"); return -1; }
+    hipLaunchKernelGGL(k_retag, dim3(div_up(n * 27, 4)), dim3(256), 0, s, g, bins_of(*b), ib.ib_ijk, ib.cap_ib, n,
                        b->tag[BT_CELL], b->tag[BT_NODEIDTAG]);
     hipLaunchKernelGGL(k_reclass, dim3(div_up(n * 27, 256)), dim3(256), 0, s, b->d, ib.ib_ijk, ib.cap_ib, n,
                        b->tag[BT_CELL], b->tag[BT_NODEIDTAG]);
@@ -744,9 +853,10 @@ int biocfd_compute_norm_distance(int id) {
   const int64_t n = b->counts[BC_IB];
   if (n == 0) return 0;
   if (!ib.ib_ijk) { set_error("compute_norm_distance: no IB list"); return -1; }
+  if (!ib.bins_ready) { set_error("compute_norm_distance: surface not computed"); return -1; }
   Geo g = geo_of(*b);
-  hipLaunchKernelGGL(k_norm_distance, dim3(div_up(n, 256)), dim3(256), 0, rt().stream, g, ib.ib_ijk, ib.cap_ib,
-                     n, ib.nel[BN_P], ib.nel[BN_U1], ib.nel[BN_U2], ib.nel[BN_V1], ib.nel[BN_V2], ib.nel[BN_W1],
+  hipLaunchKernelGGL(k_norm_distance, dim3(div_up(n, 4)), dim3(256), 0, rt().stream, g, bins_of(*b), ib.ib_ijk,
+                     ib.cap_ib, n, ib.nel[BN_P], ib.nel[BN_U1], ib.nel[BN_U2], ib.nel[BN_V1], ib.nel[BN_V2], ib.nel[BN_W1],
                      ib.nel[BN_W2], ib.normdis[BN_P], ib.normdis[BN_U1], ib.normdis[BN_U2], ib.normdis[BN_V1],
                      ib.normdis[BN_V2], ib.normdis[BN_W1], ib.normdis[BN_W2]);
   BCFD_CHECK_LAUNCH();

@@ -231,6 +231,8 @@ static void free_block(Block &b) {
   if (ib.surf) (void)hipFree(ib.surf);
   for (auto &p : ib.node) if (p) (void)hipFree(p);
   for (auto &p : ib.elem) if (p) (void)hipFree(p);
+  for (void *p : {ib.bin_grid, (void *)ib.bin_start, (void *)ib.bin_elem, (void *)ib.bin_fill})
+    if (p) (void)hipFree(p);
   if (ib.ib_ijk) (void)hipFree(ib.ib_ijk);
   for (auto &p : ib.nel) if (p) (void)hipFree(p);
   for (auto &p : ib.normdis) if (p) (void)hipFree(p);
