@@ -8,6 +8,8 @@
 // (stable) scans so their order is the reference's serial i-fastest order.
 #pragma once
 
+#include <mutex>
+
 #include <hip/hip_runtime.h>
 #include <cstdint>
 #include <cstdio>
@@ -21,6 +23,13 @@ namespace bcfd {
 
 constexpr int kMaxBlocks = 64;
 constexpr int kMaxInterfaces = 64;
+
+// Every C entry point holds this lock while it runs: host threads may call
+// the library concurrently (the reference drives its blocks from OpenMP
+// threads), and their work is queued in call order on the library's one
+// in-order stream.  Recursive, as some entry points call others.
+std::recursive_mutex &api_mutex();
+#define BCFD_API_LOCK() std::lock_guard<std::recursive_mutex> bcfd_api_lock_(::bcfd::api_mutex())
 
 void set_error(const char *fmt, ...);
 

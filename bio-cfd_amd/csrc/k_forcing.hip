@@ -524,14 +524,21 @@ using namespace bcfd;
 
 extern "C" {
 
-int biocfd_velocity_forcing1(int id, const double kin[BK_N]) { return force(id, R_V1, kin); }
-int biocfd_pressure_forcing1(int id, const double kin[BK_N]) { return force(id, R_P1, kin); }
-int biocfd_velocity_forcing_field(int id, const double kin[BK_N]) { return force(id, R_VF, kin); }
-int biocfd_pressure_forcing_field(int id, const double kin[BK_N]) { return force(id, R_PF, kin); }
-int biocfd_velocity_forcing_ghost(int id, const double kin[BK_N]) { return force(id, R_VG, kin); }
-int biocfd_pressure_forcing_ghost(int id, const double kin[BK_N]) { return force(id, R_PG, kin); }
+int biocfd_velocity_forcing1(int id, const double kin[BK_N]) {
+  BCFD_API_LOCK(); return force(id, R_V1, kin); }
+int biocfd_pressure_forcing1(int id, const double kin[BK_N]) {
+  BCFD_API_LOCK(); return force(id, R_P1, kin); }
+int biocfd_velocity_forcing_field(int id, const double kin[BK_N]) {
+  BCFD_API_LOCK(); return force(id, R_VF, kin); }
+int biocfd_pressure_forcing_field(int id, const double kin[BK_N]) {
+  BCFD_API_LOCK(); return force(id, R_PF, kin); }
+int biocfd_velocity_forcing_ghost(int id, const double kin[BK_N]) {
+  BCFD_API_LOCK(); return force(id, R_VG, kin); }
+int biocfd_pressure_forcing_ghost(int id, const double kin[BK_N]) {
+  BCFD_API_LOCK(); return force(id, R_PG, kin); }
 
 int biocfd_stress_calculation(int id, double mu_f, double rho_f, const double kin[BK_N], double out[4]) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   IbData &ib = b->ib;

@@ -222,6 +222,7 @@ using namespace bcfd;
 extern "C" {
 
 int biocfd_interface_create(int itf, int a_blk, int b_blk, int64_t a_msh, int64_t b_msh, const double ext[6]) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   if (itf < 1 || itf > kMaxInterfaces) { set_error("interface id %d out of range", itf); return -1; }
   Block *a = get_block(a_blk);
@@ -255,6 +256,7 @@ int biocfd_interface_create(int itf, int a_blk, int b_blk, int64_t a_msh, int64_
 }
 
 int biocfd_interface_table(int itf, int comp, int axis, int64_t *host, int64_t *counter, int64_t cap) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   if (itf < 1 || itf > kMaxInterfaces || !rt().itf[itf].live || comp < 0 || comp > 3 || axis < 0 || axis > 2) {
     set_error("interface_table: bad request");
@@ -269,12 +271,14 @@ int biocfd_interface_table(int itf, int comp, int axis, int64_t *host, int64_t *
 }
 
 int biocfd_coarse_update(int which) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   if (which < 0 || which > 2) { set_error("coarse_update: bad selector %d", which); return -1; }
   return interface_coarse_update(which);
 }
 
 int biocfd_fine_update_bd(int which) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   if (which < 0 || which > 2) { set_error("fine_update_bd: bad selector %d", which); return -1; }
   return interface_fine_update(which);

@@ -450,6 +450,7 @@ using namespace bcfd;
 extern "C" {
 
 int biocfd_set_precision(int mode) {
+  BCFD_API_LOCK();
   if (mode != 0 && mode != 1) {
     set_error("set_precision: mode %d is neither 0 (FP64) nor 1 (mixed)", mode);
     return -1;
@@ -459,6 +460,7 @@ int biocfd_set_precision(int mode) {
 }
 
 int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_correction, double residual_tol) {
+  BCFD_API_LOCK();
   if (variant < 0 || variant >= kNumSweep32) {
     set_error("mixed_config: variant %d outside 0..%d", variant, kNumSweep32 - 1);
     return -1;
@@ -476,6 +478,7 @@ int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_co
 }
 
 int biocfd_mixed_variant_info(int variant, int info[7]) {
+  BCFD_API_LOCK();
   if (variant < 0 || variant >= kNumSweep32) {
     set_error("mixed_variant_info: variant %d outside 0..%d", variant, kNumSweep32 - 1);
     return -1;
@@ -492,6 +495,7 @@ int biocfd_mixed_variant_info(int variant, int info[7]) {
 }
 
 int biocfd_mixed_time(int id, int variant, int planes_per_workgroup, int iters, double dt, double *ms) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (!b->A[0]) { set_error("coefficientMatrix has not been called"); return -1; }
@@ -504,6 +508,7 @@ int biocfd_mixed_time(int id, int variant, int planes_per_workgroup, int iters, 
 }
 
 int biocfd_residual_norm(int id, double dt, double *out) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (!b->A[0]) { set_error("coefficientMatrix has not been called"); return -1; }

@@ -380,6 +380,7 @@ int compact_mask_lin(Block &b, const uint8_t *mask, int **out, int64_t *cap, int
 using namespace bcfd;
 
 extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   for (int t = 0; t < BTB_N; ++t)

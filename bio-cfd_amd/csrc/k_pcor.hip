@@ -722,6 +722,7 @@ using namespace bcfd;
 extern "C" {
 
 int biocfd_velocity_bc(int id, double dt, double uc) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (!b->g[BG_DX]) { set_error("grids not uploaded"); return -1; }
@@ -729,6 +730,7 @@ int biocfd_velocity_bc(int id, double dt, double uc) {
 }
 
 int biocfd_solid_cell_bc(int id) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   dim3 blk(64, 4, 1);
@@ -740,6 +742,7 @@ int biocfd_solid_cell_bc(int id) {
 }
 
 int biocfd_solid_cell_bc_move(int id, int64_t move_check) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (move_check != 1) return 0;
@@ -752,6 +755,7 @@ int biocfd_solid_cell_bc_move(int id, int64_t move_check) {
 }
 
 int biocfd_update_velocity_newv(int id, int64_t move_check) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (move_check != 1) return 0;
@@ -762,6 +766,7 @@ int biocfd_update_velocity_newv(int id, int64_t move_check) {
 }
 
 int biocfd_timing_enable(int on) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   for (auto &pr : g_tm.ev) { (void)hipEventDestroy(pr.first); (void)hipEventDestroy(pr.second); }
   g_tm = SorTiming();
@@ -770,6 +775,7 @@ int biocfd_timing_enable(int on) {
 }
 
 int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_iters) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   BCFD_HIP(hipStreamSynchronize(rt().stream));
   double ms = 0.0;
@@ -785,6 +791,7 @@ int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_ite
 }
 
 int biocfd_sor_config(int variant, int planes_per_workgroup) {
+  BCFD_API_LOCK();
   if (variant < 0 || variant >= kNumSorVariants) {
     set_error("sor_config: variant %d outside 0..%d", variant, kNumSorVariants - 1);
     return -1;
@@ -799,6 +806,7 @@ int biocfd_sor_config(int variant, int planes_per_workgroup) {
 }
 
 int biocfd_sor_variant_info(int variant, int info[7]) {
+  BCFD_API_LOCK();
   if (variant < 0 || variant >= kNumSorVariants) {
     set_error("sor_variant_info: variant %d outside 0..%d", variant, kNumSorVariants - 1);
     return -1;
@@ -815,6 +823,7 @@ int biocfd_sor_variant_info(int variant, int info[7]) {
 }
 
 int biocfd_sor_time(int id, int variant, int planes_per_workgroup, int iters, double dt, double *ms) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b || check_ready(*b)) return -1;
   if (variant < 0 || variant >= kNumSorVariants || planes_per_workgroup < 1 || planes_per_workgroup > 4096 ||
@@ -826,24 +835,28 @@ int biocfd_sor_time(int id, int variant, int planes_per_workgroup, int iters, do
 }
 
 int biocfd_compute_div(int id) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b || check_ready(*b)) return -1;
   return compute_div(*b);
 }
 
 int biocfd_redblack_sor(int id, int64_t pc_ita_max, double dt, double epsi) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b || check_ready(*b)) return -1;
   return redblack_sor(*b, pc_ita_max, dt, epsi);
 }
 
 int biocfd_correct(int id, double dt) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b || check_ready(*b)) return -1;
   return correct(*b, dt);
 }
 
 int biocfd_poisson_solver(int first, int last, int64_t pc_ita_max, double epsi, double dt, double uc) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   if (first < 1 || last < first || last > kMaxBlocks) {
     set_error("poisson_solver: bad block range %d..%d", first, last);

@@ -734,6 +734,7 @@ using namespace bcfd;
 extern "C" {
 
 int biocfd_compute_surface_norm(int id, const double *x1, const double *y1, const double *z1, int64_t inor) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   IbData &ib = b->ib;
@@ -753,6 +754,7 @@ int biocfd_compute_surface_norm(int id, const double *x1, const double *y1, cons
 }
 
 int biocfd_tagging(int id) {
+  BCFD_API_LOCK();
   Block *b = ib_block(id);
   if (!b) return -1;
   hipStream_t s = rt().stream;
@@ -773,6 +775,7 @@ int biocfd_tagging(int id) {
 }
 
 int biocfd_selective_retagging(int id, int64_t blk_mv_tag) {
+  BCFD_API_LOCK();
   Block *b = ib_block(id);
   if (!b) return -1;
   if (blk_mv_tag != 0) return 0;
@@ -807,12 +810,14 @@ int biocfd_selective_retagging(int id, int64_t blk_mv_tag) {
 }
 
 int biocfd_cell_count_solid(int id) {
+  BCFD_API_LOCK();
   Block *b = ib_block(id);
   if (!b) return -1;
   return build_cell_lists(*b, true);
 }
 
 int biocfd_cell_count_solid_coarse(int id) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   // only the fluid/red/black counts change (Search.F90:1468-1506)
@@ -824,6 +829,7 @@ int biocfd_cell_count_solid_coarse(int id) {
 }
 
 int biocfd_fine_block_cell(int coarse) {
+  BCFD_API_LOCK();
   Block *a = get_block(coarse);
   if (!a) return -1;
   hipStream_t s = rt().stream;
@@ -847,6 +853,7 @@ int biocfd_fine_block_cell(int coarse) {
 }
 
 int biocfd_compute_norm_distance(int id) {
+  BCFD_API_LOCK();
   Block *b = ib_block(id);
   if (!b) return -1;
   IbData &ib = b->ib;
@@ -864,6 +871,7 @@ int biocfd_compute_norm_distance(int id) {
 }
 
 int biocfd_find_ts_cells(int id) {
+  BCFD_API_LOCK();
   Block *b = ib_block(id);
   if (!b) return -1;
   if (b->counts[BC_IB] > 0 && !b->ib.ib_ijk) { set_error("find_ts_cells: no IB list"); return -1; }
@@ -871,6 +879,7 @@ int biocfd_find_ts_cells(int id) {
 }
 
 int biocfd_list_download(int id, int which, int64_t *host, int64_t count) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   IbData &ib = b->ib;
@@ -920,6 +929,7 @@ int biocfd_list_download(int id, int which, int64_t *host, int64_t count) {
 }
 
 int biocfd_nel_download(int id, int which, int64_t *host, int64_t count) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (which < 0 || which >= BN_NNEL || count != b->counts[BC_IB]) { set_error("nel_download: bad request"); return -1; }
@@ -927,6 +937,7 @@ int biocfd_nel_download(int id, int which, int64_t *host, int64_t count) {
 }
 
 int biocfd_normdis_download(int id, int which, double *host, int64_t count) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (which < 0 || which >= BN_NNEL || count != b->counts[BC_IB]) { set_error("normdis_download: bad request"); return -1; }
@@ -934,6 +945,7 @@ int biocfd_normdis_download(int id, int which, double *host, int64_t count) {
 }
 
 int biocfd_ghost_download(int id, int which, double *host, int64_t count) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (which < 0 || which >= BGH_N || count != b->counts[BC_TS]) { set_error("ghost_download: bad request"); return -1; }
@@ -941,6 +953,7 @@ int biocfd_ghost_download(int id, int which, double *host, int64_t count) {
 }
 
 int biocfd_elem_download(int id, int which, double *host, int64_t count) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (which < 0 || which >= BE_NELEM || count != b->ib.n_elems) { set_error("elem_download: bad request"); return -1; }
@@ -948,6 +961,7 @@ int biocfd_elem_download(int id, int which, double *host, int64_t count) {
 }
 
 int biocfd_index_ts_download(int id, int64_t *host, int64_t count) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (count != b->counts[BC_TS]) { set_error("index_ts_download: bad count"); return -1; }

@@ -13,6 +13,11 @@ static thread_local char g_err[1024] = "";
 static Runtime g_rt;
 static std::mutex g_mu;
 
+std::recursive_mutex &api_mutex() {
+  static std::recursive_mutex m;
+  return m;
+}
+
 void set_error(const char *fmt, ...) {
   va_list ap;
   va_start(ap, fmt);
@@ -165,6 +170,7 @@ extern "C" {
 const char *biocfd_last_error(void) { return g_err; }
 
 int biocfd_init(int device) {
+  BCFD_API_LOCK();
   std::lock_guard<std::mutex> lk(g_mu);
   if (g_rt.init) return 0;
   int n = 0;
@@ -183,6 +189,7 @@ int biocfd_init(int device) {
 }
 
 int biocfd_copy_probe(int64_t bytes, int reps) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   if (bytes <= 0 || reps <= 0) { set_error("copy_probe: bad size"); return -1; }
   void *a = nullptr, *b = nullptr;
@@ -198,6 +205,7 @@ int biocfd_copy_probe(int64_t bytes, int reps) {
 }
 
 int biocfd_synchronize(void) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   BCFD_HIP(hipStreamSynchronize(g_rt.stream));
   return 0;
@@ -247,6 +255,7 @@ static void free_block(Block &b) {
 }
 
 int biocfd_finalize(void) {
+  BCFD_API_LOCK();
   std::lock_guard<std::mutex> lk(g_mu);
   if (!g_rt.init) return 0;
   (void)hipStreamSynchronize(g_rt.stream);
@@ -268,6 +277,7 @@ int biocfd_finalize(void) {
 
 int biocfd_block_create(int id, int64_t nx, int64_t ny, int64_t nz, double omega, int64_t cintp,
                         double dx, double dy, double dz) {
+  BCFD_API_LOCK();
   if (ensure_init()) return -1;
   if (id < 1 || id > kMaxBlocks) {
     set_error("block id %d out of range 1..%d", id, kMaxBlocks);
@@ -305,6 +315,7 @@ int biocfd_block_create(int id, int64_t nx, int64_t ny, int64_t nz, double omega
 }
 
 int biocfd_block_destroy(int id) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   (void)hipStreamSynchronize(g_rt.stream);
@@ -325,6 +336,7 @@ static int64_t grid_len(const Block &b, int which) {
 }
 
 int biocfd_grid_upload(int id, int which, const double *host, int64_t n) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (which < 0 || which >= BG_NGRID || n != grid_len(*b, which)) {
@@ -340,6 +352,7 @@ int biocfd_grid_upload(int id, int which, const double *host, int64_t n) {
 }
 
 int biocfd_field_upload(int id, int field, const double *host) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (field < 0 || field >= BF_NFIELD) { set_error("bad field %d", field); return -1; }
@@ -350,6 +363,7 @@ int biocfd_field_upload(int id, int field, const double *host) {
 }
 
 int biocfd_field_download(int id, int field, double *host) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (field < 0 || field >= BF_NFIELD) { set_error("bad field %d", field); return -1; }
@@ -362,6 +376,7 @@ int biocfd_field_download(int id, int field, double *host) {
 }
 
 int biocfd_tag_upload(int id, int tag, const int64_t *host) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (tag < 0 || tag >= BT_NTAG) { set_error("bad tag %d", tag); return -1; }
@@ -377,6 +392,7 @@ int biocfd_tag_upload(int id, int tag, const int64_t *host) {
 }
 
 int biocfd_tag_download(int id, int tag, int64_t *host) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (tag < 0 || tag >= BT_NTAG) { set_error("bad tag %d", tag); return -1; }
@@ -389,6 +405,7 @@ int biocfd_tag_download(int id, int tag, int64_t *host) {
 }
 
 int biocfd_counts(int id, int64_t out[BC_N]) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   for (int q = 0; q < BC_N; ++q) out[q] = b->counts[q];
@@ -396,6 +413,7 @@ int biocfd_counts(int id, int64_t out[BC_N]) {
 }
 
 int biocfd_sor_state(int id, int64_t *n_iter, double *derr2, double *derr_std_st) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   SorState s;
@@ -408,6 +426,7 @@ int biocfd_sor_state(int id, int64_t *n_iter, double *derr2, double *derr_std_st
 }
 
 int biocfd_coefficient_matrix(int id, int is_coarse) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (b->hg[BG_XP].empty() || b->hg[BG_YP].empty() || b->hg[BG_ZP].empty()) {
@@ -428,6 +447,7 @@ int biocfd_coefficient_matrix(int id, int is_coarse) {
 }
 
 int biocfd_non_uni_coeff(int id) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (b->hg[BG_DX].empty() || b->hg[BG_DY].empty() || b->hg[BG_DZ].empty()) {
@@ -451,6 +471,7 @@ int biocfd_non_uni_coeff(int id) {
 }
 
 int biocfd_coef_download(int id, int axis, double *host, int64_t n) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (axis < 0 || axis > 2 || b->hA[axis].size() != (size_t)(3 * n)) {
@@ -462,6 +483,7 @@ int biocfd_coef_download(int id, int axis, double *host, int64_t n) {
 }
 
 int biocfd_table_download(int id, int which, int is_ck, double *host, int64_t ncols) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   if (which < 0 || which >= BTB_N) { set_error("bad table %d", which); return -1; }
@@ -474,6 +496,7 @@ int biocfd_table_download(int id, int which, int is_ck, double *host, int64_t nc
 int biocfd_body_create(int id, int64_t n_nodes, int64_t n_elems, const int64_t *p1,
                        const int64_t *p2, const int64_t *p3, const int64_t *surf,
                        const int64_t box[6]) {
+  BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
   IbData &ib = b->ib;
@@ -545,6 +568,7 @@ int biocfd_shift_surface_nodes_initial(int64_t nn, const double *xn, const doubl
                                        double piv_pt, int64_t ita, double dt, double dxmin,
                                        double totime, double *s, double *x1, double *y1,
                                        double *z1) {
+  BCFD_API_LOCK();
   const double aoa1 = aoa * kPi / 180.0;
   const double bdfr = 15.0, bdy = 15.0 * dxmin;
   s[S_XMOVE] = s[S_YMOVE] = s[S_ZMOVE] = 0.0;
@@ -573,6 +597,7 @@ int biocfd_compute_surface_variables(int64_t nn, const double *xn, const double 
                                      double piv_pt, int64_t ita, double dt, double dxmin,
                                      double freq, double totime, double *s, double *x1,
                                      double *y1, double *z1) {
+  BCFD_API_LOCK();
   const double ph = 2.0 * kPi * freq * (totime + dt) + phase;
   const double aoa1 = (s[S_A0]) * sin(ph);
   const double wtheta = 2.0 * kPi * freq;

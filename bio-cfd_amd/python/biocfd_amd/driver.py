@@ -54,13 +54,22 @@ def grid_arrays(st: BlockState) -> List[np.ndarray]:
 class Simulation:
     """One case on the GPU; the device holds all state between calls."""
 
-    def __init__(self, case: CS.Case, device: int = 0, precision: str = "fp64"):
+    def __init__(self, case: CS.Case, device: int = 0, precision: str = "fp64", threads: int = 1):
         """precision: "fp64" (the reference's method) or "mixed" (FP32 SOR
-        smoothing inside FP64 defect correction, include/biocfd_hip.h)."""
+        smoothing inside FP64 defect correction, include/biocfd_hip.h).
+        threads > 1 runs the per-block loops of a time step on that many host
+        threads, as the reference's OpenMP loops over blocks do
+        (main.F90:133-148, 232-263); the library serialises the calls."""
         if precision not in ("fp64", "mixed"):
             raise ValueError(f"precision must be 'fp64' or 'mixed', not {precision!r}")
         self.case = case
         self.precision = precision
+        self.threads = max(1, int(threads))
+        self._pool = None
+        if self.threads > 1:
+            from concurrent.futures import ThreadPoolExecutor
+
+            self._pool = ThreadPoolExecutor(max_workers=self.threads)
         H.call("biocfd_init", device)
         H.call("biocfd_set_precision", 0 if precision == "fp64" else 1)
         self.blocks = [BlockState(g + 1, spec) for g, spec in enumerate(case.blocks)]
@@ -154,11 +163,22 @@ class Simulation:
         H.call("biocfd_tag_upload", st.g, H.BT["cell_pr"], H.ptr(zero_pr))
 
     # ---------------------------------------------------------------- time step
+    def _each(self, fn, bodies_only: bool = False):
+        """fn(st) for every block (with a body), one after the other or on the
+        thread pool; exceptions propagate."""
+        blocks = [st for st in self.blocks if st.body is not None] if bodies_only else self.blocks
+        if self._pool is None or len(blocks) < 2:
+            for st in blocks:
+                fn(st)
+        else:
+            list(self._pool.map(fn, blocks))
+
     def step(self):
         c = self.case
         self.ita += 1
         self.totime = self.totime + c.dt
-        for st in self.blocks:
+
+        def momentum(st):
             H.call("biocfd_ns_momentum", st.g, c.dt, c.re)
             if st.g == 1:
                 H.call("biocfd_velocity_bc", st.g, c.dt, c.uc)
@@ -168,24 +188,21 @@ class Simulation:
                 H.call("biocfd_velocity_forcing1", st.g, H.ptr(kv))
             if st.g == 1:
                 H.call("biocfd_velocity_bc", st.g, c.dt, c.uc)
-        H.call("biocfd_poisson_solver", 1, len(self.blocks), c.pc_ita_max, c.epsi, c.dt, c.uc)
-        for st in self.blocks:
-            if st.body is not None:
-                kv = st.kin_vec()
-                H.call("biocfd_pressure_forcing1", st.g, H.ptr(kv))
-        for st in self.blocks:
-            if st.body is None:
-                continue
+
+        def pressure_forcing(st):
+            kv = st.kin_vec()
+            H.call("biocfd_pressure_forcing1", st.g, H.ptr(kv))
+
+        def surface_variables(st):
             H.call("biocfd_compute_surface_variables", st.nn, H.ptr(st.xn), H.ptr(st.yn), H.ptr(st.zn),
                    H.ptr(st.nid), self.phase, c.piv_pt, self.ita, c.dt, c.dxmin, c.freq, self.totime,
                    H.ptr(st.kin), H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.znode1))
-        for st in self.blocks:
-            if st.body is not None:
-                H.call("biocfd_compute_surface_norm", st.g, H.ptr(st.xnode1), H.ptr(st.ynode1),
-                       H.ptr(st.znode1), c.inor)
-        for st in self.blocks:
-            if st.body is None:
-                continue
+
+        def surface_norm(st):
+            H.call("biocfd_compute_surface_norm", st.g, H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.znode1),
+                   c.inor)
+
+        def retag_and_force(st):
             g = st.g
             kv = st.kin_vec()
             H.call("biocfd_selective_retagging", g, 0)
@@ -198,6 +215,13 @@ class Simulation:
             H.call("biocfd_pressure_forcing_field", g, H.ptr(kv))
             H.call("biocfd_velocity_forcing_ghost", g, H.ptr(kv))
             H.call("biocfd_pressure_forcing_ghost", g, H.ptr(kv))
+
+        self._each(momentum)
+        H.call("biocfd_poisson_solver", 1, len(self.blocks), c.pc_ita_max, c.epsi, c.dt, c.uc)
+        self._each(pressure_forcing, bodies_only=True)
+        self._each(surface_variables, bodies_only=True)
+        self._each(surface_norm, bodies_only=True)
+        self._each(retag_and_force, bodies_only=True)
 
     # ---------------------------------------------------------------- data access
     def upload(self, g: int, name: str, a: np.ndarray):
@@ -311,5 +335,8 @@ class Simulation:
         return float(out[0])
 
     def close(self):
+        if self._pool is not None:
+            self._pool.shutdown()
+            self._pool = None
         H.call("biocfd_set_precision", 0)
         H.call("biocfd_finalize")
