@@ -1,12 +1,19 @@
-! Fortran host driver of the gfx950 hot path: reads a case file, runs the
-! set-up and the time steps through the C ABI (module biocfd_hip) in the call
-! order of the reference driver (app/main.F90:43-265), and writes record
-! dumps of the block state at the requested steps.
+! Fortran host driver of the gfx950 hot path: reads a case file or the
+! reference's input deck, runs the set-up and the time steps through the C ABI
+! (module biocfd_hip) in the call order of the reference driver
+! (app/main.F90:43-265), and writes record dumps of the block state.
 !
 !   biocfd_run CASE_FILE OUT_DIR [--time]
+!   biocfd_run --deck DECK_DIR OUT_DIR [--time]
+!   biocfd_run --deck-to-case DECK_DIR CASE_FILE
 !
 ! Case file: the stream written by biocfd_amd.cases.write_case (magic
-! 20261203, version 1).  Dumps: OUT_DIR/dump_s<step>_b<block>.bin, records of
+! 20261203, version 1).  Deck: the files the reference's readInput,
+! readBlockInterface and readSurfaceMeshGmsh read (input_data.nml, the .dat
+! files, the grid files, geometries/butterflyMedium.msh), with a body in every
+! block g /= 1; deck runs dump after the set-up and after the last step, and
+! --deck-to-case writes the deck as read in the case-file layout without
+! touching the GPU.  Dumps: OUT_DIR/dump_s<step>_b<block>.bin, records of
 ! (name(24), kind, ndim, shape, data) as read by biocfd_amd.dumps.read_dump.
 ! --time: no dumps; writes OUT_DIR/timing.txt ("steps N seconds S") for the
 ! time-step loop, set-up excluded.
@@ -43,6 +50,13 @@ module biocfd_host
   type :: host_interface
     integer(int64) :: a = 0, b = 0, a_msh = 1, b_msh = 1
     real(real64) :: ext(6) = 0
+  end type
+
+  ! the input_data namelist of a deck, as read (before unit scaling)
+  type :: deck_params
+    integer(int64) :: nblocks, intflines, itamax, pcitamax, surgeopoints, inor
+    real(real64) :: epsi, omega1, omega2, omega3, omega4, re, rho_f, mu_f, l_c, phase_angle, freq, aoa, piv_pt, &
+                    dt_order, dxmin
   end type
 
   integer(int64) :: nblocks, nintf, ib_block1, pc_ita_max, nsteps, inor, seed
@@ -121,6 +135,326 @@ contains
       b%ynode1 = 0
       b%znode1 = 0
     end if
+  end subroutine
+
+  ! ------------------------------------------------------------ input deck
+  ! The reference's input deck in directory `dir` (ReadInput.f90:25-456), with
+  ! the reference's unit scaling (millimetres and 1e-5 viscosity units) and its
+  ! derived values, computed with the same expressions in the same order so
+  ! that every double equals the reference's: u0 = re mu_f / (rho_f l_c),
+  ! re = rho_f / mu_f, freq = freq u0 / l_c, dt = 1 / (4 freq dt_order)
+  ! (:128-157), heave amplitude = dxmin yamp (:178), dx = (1/dx) 0.001
+  ! (:226-231), grid faces 0.001 x + shift with extrapolated end faces and
+  ! mid-point centres (:248-350), node IDs from the element surface IDs
+  ! (:396-418).  main.F90 gives every block g /= 1 the one surface mesh
+  ! (:63-68); the deck carries no noise.
+  subroutine read_deck(dir)
+    character(len=*), intent(in) :: dir
+    type(deck_params) :: prm
+    integer :: u, g
+    integer(int64) :: i, n, i1, i2, i3, i5, nn, ne
+    real(real64) :: l_c, u0, dt_order, gsh(3)
+    real(real64) :: ext(6)
+    real(real64), allocatable :: xs(:), ys(:), zs(:), omg(:)
+    integer(int64), allocatable :: box(:, :), el(:, :), sid(:), nid(:)
+    real(real64), allocatable :: sh(:, :), a0(:), ym(:), bf(:), gshift(:, :), xn(:), yn(:), zn(:)
+    character(len=32) :: grid_name
+
+    call read_deck_namelist(dir // '/input_data.nml', prm)
+    nblocks = prm%nblocks
+    nintf = prm%intflines
+    nsteps = prm%itamax
+    epsi = prm%epsi
+    pc_ita_max = prm%pcitamax
+    inor = prm%inor
+    aoa = prm%aoa
+    phase_deg = prm%phase_angle
+    piv_pt = prm%piv_pt
+    if (nblocks < 2) then
+      write(*, '(a)') 'biocfd_run: a deck needs a coarse block and a body block (nblocks >= 2)'
+      error stop 3
+    end if
+    allocate(blk(nblocks), itf(nintf), box(6, nblocks), sh(3, nblocks), gshift(3, nblocks))
+    allocate(a0(nblocks), ym(nblocks), bf(nblocks), omg(nblocks))
+
+    open(newunit=u, file=dir // '/body_search.dat', status='old', action='read')
+    do g = 1, int(nblocks)
+      read(u, *) box(:, g)
+    end do
+    close(u)
+    open(newunit=u, file=dir // '/grid_shift.dat', status='old', action='read')
+    do g = 1, int(nblocks)
+      read(u, *) gshift(:, g)
+      gshift(:, g) = gshift(:, g) * 0.001_real64
+    end do
+    close(u)
+    open(newunit=u, file=dir // '/shift.dat', status='old', action='read')
+    do g = 1, int(nblocks)
+      read(u, *) sh(:, g)
+      sh(:, g) = sh(:, g) * 0.001_real64
+    end do
+    close(u)
+    dxmin = 0.001_real64 * prm%dxmin
+    a0 = 0
+    open(newunit=u, file=dir // '/flap_amp.dat', status='old', action='read')
+    do g = 2, int(nblocks)
+      read(u, *) a0(g)
+    end do
+    close(u)
+    mu_f = prm%mu_f * 1e-5_real64
+    rho_f = prm%rho_f
+    l_c = 0.001_real64 * prm%l_c
+    u0 = prm%re * mu_f / (rho_f * l_c)
+    uc = u0
+    re = rho_f / mu_f
+    freq = prm%freq * u0 / l_c
+    dt_order = prm%dt_order
+    dt = 1.0_real64 / (4.0_real64 * freq * dt_order)
+    ym = 0
+    bf = 0
+    open(newunit=u, file=dir // '/butter_move.dat', status='old', action='read')
+    do g = 2, int(nblocks)
+      read(u, *) ym(g), bf(g)
+      ym(g) = dxmin * ym(g)
+    end do
+    close(u)
+
+    open(newunit=u, file=dir // '/block_details.dat', status='old', action='read')
+    do g = 1, int(nblocks)
+      read(u, *) ext, blk(g)%nx, blk(g)%ny, blk(g)%nz, blk(g)%dx, blk(g)%dy, blk(g)%dz
+      blk(g)%dx = (1.0_real64 / blk(g)%dx) * 0.001_real64
+      blk(g)%dy = (1.0_real64 / blk(g)%dy) * 0.001_real64
+      blk(g)%dz = (1.0_real64 / blk(g)%dz) * 0.001_real64
+    end do
+    close(u)
+    omg = prm%omega4
+    omg(1) = prm%omega1
+    omg(2) = prm%omega2
+    if (nblocks >= 3) omg(3) = prm%omega3
+
+    do g = 1, int(nblocks)
+      associate (b => blk(g))
+        b%cintp = 3
+        b%omega = omg(g)
+        b%has_ib = merge(1_int64, 0_int64, g /= 1)
+        b%moving = b%has_ib
+        allocate(b%x1(b%nx + 3), b%y1(b%ny + 3), b%z1(b%nz + 3))
+        allocate(b%deltax(b%nx + 2), b%deltay(b%ny + 2), b%deltaz(b%nz + 2))
+        write(grid_name, '(a,i3.3,a,i0,a)') '/xgrid_bk', g, '_', b%nx + 1, '.txt'
+        call read_faces(dir // trim(grid_name), b%nx, gshift(1, g), b%x1, b%deltax)
+        write(grid_name, '(a,i3.3,a,i0,a)') '/ygrid_bk', g, '_', b%ny + 1, '.txt'
+        call read_faces(dir // trim(grid_name), b%ny, gshift(2, g), b%y1, b%deltay)
+        write(grid_name, '(a,i3.3,a,i0,a)') '/zgrid_bk', g, '_', b%nz + 1, '.txt'
+        call read_faces(dir // trim(grid_name), b%nz, gshift(3, g), b%z1, b%deltaz)
+        b%xu = b%x1
+        b%yv = b%y1
+        b%zw = b%z1
+        b%xp = centres(b%x1)
+        b%yp = centres(b%y1)
+        b%zp = centres(b%z1)
+        b%xv = b%xp
+        b%xw = b%xp
+        b%yu = b%yp
+        b%yw = b%yp
+        b%zu = b%zp
+        b%zv = b%zp
+      end associate
+    end do
+
+    open(newunit=u, file=dir // '/interface_details.dat', status='old', action='read')
+    do g = 1, int(nintf)
+      read(u, *) itf(g)%a, itf(g)%a_msh, i1, itf(g)%b, itf(g)%b_msh, i2, itf(g)%ext
+      itf(g)%ext = itf(g)%ext * 0.001_real64
+    end do
+    close(u)
+
+    ! Gmsh 2.2 ASCII: 4 lines, the node count, "id x y z" (mm), 2 lines, the
+    ! element count, surGeoPoints skipped lines, "id type ntags surface
+    ! elementary n1 n2 n3"
+    open(newunit=u, file=dir // '/geometries/butterflyMedium.msh', status='old', action='read')
+    do i = 1, 4
+      read(u, *)
+    end do
+    read(u, *) nn
+    allocate(xn(nn), yn(nn), zn(nn), nid(nn))
+    do i = 1, nn
+      read(u, *) i1, xn(i), yn(i), zn(i)
+      xn(i) = xn(i) * 0.001_real64
+      yn(i) = yn(i) * 0.001_real64
+      zn(i) = zn(i) * 0.001_real64
+    end do
+    do i = 1, 2
+      read(u, *)
+    end do
+    read(u, *) ne
+    do i = 1, prm%surgeopoints
+      read(u, *)
+    end do
+    ne = ne - prm%surgeopoints
+    allocate(el(3, ne), sid(ne))
+    do n = 1, ne
+      read(u, *) i1, i2, i3, sid(n), i5, el(:, n)
+    end do
+    close(u)
+    nid = 50
+    do n = 1, ne
+      if (sid(n) == 51) then
+        nid(el(:, n)) = 51
+      else if (sid(n) == 52) then
+        nid(el(:, n)) = 52
+      end if
+    end do
+    do n = 1, ne
+      if (sid(n) == 50) nid(el(:, n)) = 50
+    end do
+
+    do g = 2, int(nblocks)
+      associate (b => blk(g))
+        b%nn = nn
+        b%ne = ne
+        b%box = box(:, g)
+        b%centre = sh(:, g)
+        b%a0 = a0(g)
+        b%yamp = ym(g)
+        b%bfreq = bf(g)
+        b%xnode = xn
+        b%ynode = yn
+        b%znode = zn
+        b%node_id = nid
+        b%el1 = el(1, :)
+        b%el2 = el(2, :)
+        b%el3 = el(3, :)
+        b%surf = sid
+        allocate(b%xnode1(nn), b%ynode1(nn), b%znode1(nn))
+        b%xnode1 = 0
+        b%ynode1 = 0
+        b%znode1 = 0
+      end associate
+    end do
+    ib_block1 = 0
+    noise_amp = 0
+    seed = 0
+    allocate(dump_steps(2))
+    dump_steps = [0_int64, nsteps]
+  end subroutine
+
+  ! the input_data namelist (ReadInput.f90:86-94); the group's variable
+  ! names are the file format's keys
+  subroutine read_deck_namelist(path, prm)
+    character(len=*), intent(in) :: path
+    type(deck_params), intent(out) :: prm
+    integer(int64) :: nblocks, intflines, itamax, pcItaMax, surGeoPoints, inor
+    real(real64) :: epsi, omega1, omega2, omega3, omega4, re, rho_f, mu_f, l_c, u0, phase_angle, freq, aoa, &
+                    piv_pt, alpha_m, theta_m, dt_order, dxmin
+    integer :: istart, u
+    namelist /input_data/ nblocks, intflines, itamax, epsi, pcItaMax, omega1, omega2, omega3, omega4, re, &
+                          rho_f, mu_f, l_c, u0, surGeoPoints, phase_angle, freq, aoa, piv_pt, alpha_m, theta_m, &
+                          istart, dt_order, inor, dxmin
+    nblocks = 0
+    intflines = 0
+    itamax = 0
+    pcItaMax = 0
+    surGeoPoints = 0
+    inor = 1
+    istart = 0
+    epsi = 0
+    omega1 = 1
+    omega2 = 1
+    omega3 = 1
+    omega4 = 1
+    re = 0
+    rho_f = 1
+    mu_f = 0
+    l_c = 0
+    u0 = 0
+    phase_angle = 0
+    freq = 0
+    aoa = 0
+    piv_pt = 0
+    alpha_m = 0
+    theta_m = 0
+    dt_order = 1
+    dxmin = 0
+    open(newunit=u, file=path, status='old', action='read')
+    read(u, nml=input_data)
+    close(u)
+    if (istart /= 0) then
+      write(*, '(a)') 'biocfd_run: istart = 1 (restart from a checkpoint) is not supported'
+      error stop 3
+    end if
+    prm = deck_params(nblocks, intflines, itamax, pcItaMax, surGeoPoints, inor, epsi, omega1, omega2, omega3, &
+                      omega4, re, rho_f, mu_f, l_c, phase_angle, freq, aoa, piv_pt, dt_order, dxmin)
+  end subroutine
+
+  ! faces 2 .. n+2 of one axis from a grid file (mm), the end faces and the
+  ! spacings as ReadInput.f90:248-268 extends them
+  subroutine read_faces(path, n, shift, x1, delta)
+    character(len=*), intent(in) :: path
+    integer(int64), intent(in) :: n
+    real(real64), intent(in) :: shift
+    real(real64), intent(out) :: x1(:), delta(:)
+    integer :: u
+    integer(int64) :: i
+    open(newunit=u, file=path, status='old', action='read')
+    do i = 2, n + 2
+      read(u, *) x1(i)
+      x1(i) = 0.001_real64 * x1(i)
+      x1(i) = x1(i) + shift
+    end do
+    close(u)
+    do i = 2, n + 1
+      delta(i) = x1(i + 1) - x1(i)
+    end do
+    delta(1) = delta(2)
+    delta(n + 2) = delta(n + 1)
+    x1(1) = x1(2) - delta(1)
+    x1(n + 3) = x1(n + 2) + delta(n + 2)
+  end subroutine
+
+  ! cell centres 0.5 (x1(i) + x1(i+1)), i = 1 .. n+2
+  pure function centres(x1) result(c)
+    real(real64), intent(in) :: x1(:)
+    real(real64) :: c(size(x1) - 1)
+    integer :: i
+    do i = 1, size(x1) - 1
+      c(i) = 0.5_real64 * (x1(i) + x1(i + 1))
+    end do
+  end function
+
+  ! the loaded case in the case-file layout (biocfd_amd.cases.write_case)
+  subroutine write_case(path)
+    character(len=*), intent(in) :: path
+    integer :: u, g
+    open(newunit=u, file=path, access='stream', form='unformatted', status='replace')
+    write(u) 20261203_int64, 1_int64, nblocks, nintf, ib_block1, pc_ita_max, nsteps, inor, seed
+    write(u) dt, re, uc, epsi, noise_amp, aoa, phase_deg, piv_pt, dxmin, freq, mu_f, rho_f
+    do g = 1, int(nblocks)
+      associate (b => blk(g))
+        write(u) b%nx, b%ny, b%nz, b%cintp, b%has_ib, b%moving
+        write(u) b%dx, b%dy, b%dz, b%omega
+        write(u) b%x1, b%y1, b%z1
+        write(u) b%deltax, b%deltay, b%deltaz
+        write(u) b%xu, b%yu, b%zu
+        write(u) b%xv, b%yv, b%zv
+        write(u) b%xw, b%yw, b%zw
+        write(u) b%xp, b%yp, b%zp
+        if (b%has_ib == 1) then
+          write(u) b%nn, b%ne
+          write(u) b%box
+          write(u) b%centre, b%a0, b%yamp, b%bfreq
+          write(u) b%xnode, b%ynode, b%znode
+          write(u) b%node_id
+          write(u) b%el1, b%el2, b%el3, b%surf
+        end if
+      end associate
+    end do
+    do g = 1, int(nintf)
+      write(u) itf(g)%a, itf(g)%b, itf(g)%a_msh, itf(g)%b_msh
+      write(u) itf(g)%ext
+    end do
+    write(u) int(size(dump_steps), int64)
+    if (size(dump_steps) > 0) write(u) dump_steps
+    close(u)
   end subroutine
 
   ! the forcing routines' kinematic scalars (enum biocfd_kin order)
@@ -299,6 +633,8 @@ contains
                         piv_pt, ita, dt, dxmin, freq, totime, b%kin, b%xnode1, b%ynode1, b%znode1), &
                         'compute_surface_variables')
     end do
+    ! the coarse block's interface masks, every step (main.F90:195-200)
+    if (blk(1)%has_ib /= 1) call biocfd_check(biocfd_fine_block_cell(1_c_int), 'fine_block_cell')
     do g = 1, int(nblocks)
       b => blk(g)
       if (b%has_ib /= 1) cycle
@@ -541,20 +877,37 @@ program biocfd_run
   character(len=512) :: case_path, out_dir, arg
   logical :: do_time
   integer(int64) :: s, t0, t1, rate
-  integer :: u
+  integer :: u, na
 
   if (command_argument_count() < 2) then
     write(*, '(a)') 'usage: biocfd_run CASE_FILE OUT_DIR [--time]'
+    write(*, '(a)') '       biocfd_run --deck DECK_DIR OUT_DIR [--time]'
+    write(*, '(a)') '       biocfd_run --deck-to-case DECK_DIR CASE_FILE'
     error stop 2
   end if
   call get_command_argument(1, case_path)
   call get_command_argument(2, out_dir)
+  na = 2
+  if (trim(case_path) == '--deck' .or. trim(case_path) == '--deck-to-case') then
+    if (command_argument_count() < 3) then
+      write(*, '(a)') 'usage: biocfd_run --deck DECK_DIR OUT_DIR [--time]'
+      error stop 2
+    end if
+    call read_deck(trim(out_dir))
+    call get_command_argument(3, out_dir)
+    na = 3
+    if (trim(case_path) == '--deck-to-case') then
+      call write_case(trim(out_dir))
+      stop
+    end if
+  else
+    call read_case(trim(case_path))
+  end if
   do_time = .false.
-  if (command_argument_count() >= 3) then
-    call get_command_argument(3, arg)
+  if (command_argument_count() > na) then
+    call get_command_argument(na + 1, arg)
     do_time = trim(arg) == '--time'
   end if
-  call read_case(trim(case_path))
   call setup()
   if (.not. do_time .and. any(dump_steps == 0)) call dump_all(trim(out_dir), 0_int64)
   call biocfd_check(biocfd_synchronize(), 'synchronize')

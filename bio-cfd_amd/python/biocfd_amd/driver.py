@@ -220,6 +220,9 @@ class Simulation:
         H.call("biocfd_poisson_solver", 1, len(self.blocks), c.pc_ita_max, c.epsi, c.dt, c.uc)
         self._each(pressure_forcing, bodies_only=True)
         self._each(surface_variables, bodies_only=True)
+        if self.blocks[0].body is None:
+            # the coarse block's interface masks, every step (main.F90:195-200)
+            H.call("biocfd_fine_block_cell", 1)
         self._each(surface_norm, bodies_only=True)
         self._each(retag_and_force, bodies_only=True)
 

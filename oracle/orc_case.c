@@ -258,6 +258,8 @@ int orc_case_step(orc_case *c, int64_t ita, double *totime, double *totalTime)
             orc_compute_surface_variables(b, g_phase, c->piv_pt, ita, c->dt, c->dxmin, c->freq, *totime);
         }
     }
+    /* the coarse block's interface masks, every step (main.F90:195-200) */
+    if (!c->blk[0].has_ib) orc_fine_block_cell(c);
     for (int64_t g = 1; g <= nb; ++g)
         if (c->blk[g - 1].has_ib) orc_compute_surface_norm(&c->blk[g - 1], c->inor);
     for (int64_t g = 1; g <= nb; ++g) {

@@ -6,9 +6,20 @@ and a few steps, the arrays the reference driver dumps after that step.  The
 case itself is rebuilt from the parameters in GOLDEN_CASES, so the tests can
 run the oracle (or the GPU path) on identical inputs.
 
+The input-deck fixture: deck_wing/ is the reference input deck that
+biocfd_amd.deck writes for config_deck_wing(), deck_wing.case is that deck
+as the reference's own readers (readInput, readBlockInterface,
+readSurfaceMeshGmsh) load it, in the case-file layout, and
+deck_wing_digests.json holds SHA-256 digests of the arrays the reference
+driver dumps when it runs the deck (the arrays themselves are too large to
+keep).
+
     python tests/golden/make_golden.py
 """
+import hashlib
+import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -20,6 +31,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "bio-cfd_amd", "python"))
 
 from biocfd_amd import cases as CS  # noqa: E402
+from biocfd_amd import deck as DK  # noqa: E402
 from biocfd_amd.dumps import read_dump  # noqa: E402
 
 REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
@@ -54,6 +66,44 @@ def fixture_path(name):
     return os.path.join(HERE, f"{name}.npz")
 
 
+DECK_DIR = os.path.join(HERE, "deck_wing")
+DECK_CASE = os.path.join(HERE, "deck_wing.case")
+DECK_DIGESTS = os.path.join(HERE, "deck_wing_digests.json")
+DECK_STEPS = (0, 2)   # the reference driver's deck dumps: after the set-up and the last step
+
+
+def array_digest(a) -> str:
+    """SHA-256 of an array's dtype, shape and Fortran-order bytes."""
+    a = np.asarray(a)
+    h = hashlib.sha256(f"{a.dtype.str}{a.shape}".encode())
+    h.update(np.asfortranarray(a).tobytes(order="F"))
+    return h.hexdigest()
+
+
+def make_deck_fixture():
+    if os.path.isdir(DECK_DIR):
+        shutil.rmtree(DECK_DIR)
+    case = DK.config_deck_wing()
+    DK.write_deck(case, DECK_DIR)
+    wd = tempfile.mkdtemp(prefix="golden_deck_")
+    run_dir = os.path.join(wd, "deck")
+    shutil.copytree(DECK_DIR, run_dir)
+    subprocess.run([REF_BIN, "--deck-to-case", DECK_CASE], check=True, capture_output=True, cwd=run_dir)
+    out = os.path.join(wd, "out")
+    os.makedirs(out)
+    subprocess.run([REF_BIN, "--deck", out], check=True, capture_output=True, cwd=run_dir)
+    digests = {}
+    for s in DECK_STEPS:
+        for g in range(len(case.blocks)):
+            d = read_dump(os.path.join(out, f"dump_s{s}_b{g + 1}.bin"))
+            for k in KEEP:
+                if k in d:
+                    digests[f"s{s}_b{g + 1}_{k}"] = array_digest(d[k])
+    with open(DECK_DIGESTS, "w") as f:
+        json.dump(digests, f, indent=1, sort_keys=True)
+    print("deck_wing", len(digests), "digests")
+
+
 def main():
     if not os.path.exists(REF_BIN):
         sys.exit(f"{REF_BIN} missing: run `make -C oracle ref` with the reference present")
@@ -72,6 +122,7 @@ def main():
                         arrays[f"s{s}_b{g + 1}_{k}"] = np.asarray(d[k])
         np.savez_compressed(fixture_path(name), **arrays)
         print(name, os.path.getsize(fixture_path(name)), "bytes")
+    make_deck_fixture()
 
 
 if __name__ == "__main__":
