@@ -159,6 +159,17 @@ def plate_body(nc: int, ns: int, chord: float, span: float, thickness: float,
                 np.full(t.shape[0], surf_id, np.int64))
 
 
+def flap_marked(body: Body) -> Body:
+    """The body with its rigid (ID 50) elements and nodes marked 51.  With
+    flap amplitude a0 = 0 and pivot 0 an ID-51 node is rotated by a zero
+    angle, z cos 0 + y sin 0 + 0 - 0 cos 0 = z and -z sin 0 + y cos 0 = y
+    (Search.F90:186-195), so the body stays where an ID-50 body would be, and
+    main.F90's block-move check has the marker nodes it reads."""
+    sid = np.where(body.surf_id == 50, 51, body.surf_id)
+    nid = np.where(body.node_id == 50, 51, body.node_id)
+    return Body(body.nodes.copy(), nid, body.elems.copy(), sid)
+
+
 # --------------------------------------------------------------------------
 # blocks and cases
 # --------------------------------------------------------------------------
@@ -354,3 +365,24 @@ def config_channel(n=(16, 12, 10), nsteps: int = 3, **kw) -> Case:
     args = dict(dt=0.002, re=500.0, nsteps=nsteps, pc_ita_max=40, epsi=0.0)
     args.update(kw)
     return Case("channel", [blk], **args)
+
+
+def config_nested_move(nc: int = 24, nf: int = 24, markers: int = 256, nsteps: int = 2, **kw) -> Case:
+    """Coarse block on [0,6]^3 plus a refinement-ratio-2 block on [1.5,4.5]^3
+    carrying a heaving sphere d=0.5 at (3,3,3) whose nodes are ID 51 (the
+    marker nodes of the reference's block-move check) with flap amplitude 0.
+    At these sizes the sphere is within the move margins of the fine block's
+    faces (4.5, 5 and 1.52 times cintp coarse cells), so the fine block moves
+    every step: one coarse cell in -x and in +y.  The fine block is wide
+    enough that two moves keep the stress probes (one cell diagonal off the
+    surface) on its grid, and the search box is widened to keep the sphere
+    inside it."""
+    coarse = BlockSpec((0.0, 0.0, 0.0), (6.0, 6.0, 6.0), (nc, nc, nc), omega=1.7)
+    lo, hi = (1.5, 1.5, 1.5), (4.5, 4.5, 4.5)
+    body = BodyPlacement(flap_marked(sphere_body(markers, 0.5)), (3.0, 3.0, 3.0), yamp=0.125, bfreq=0.2,
+                         margin_cells=6)
+    fine = BlockSpec(lo, hi, (nf, nf, nf), omega=1.7, body=body, moving=True)
+    itf = InterfaceSpec(1, 2, 1, 2, lo, hi)
+    args = dict(dt=0.001, re=re_code(150.0, 1.0, 0.5), nsteps=nsteps, pc_ita_max=10, epsi=0.0, dxmin=1e-6)
+    args.update(kw)
+    return Case(f"move{nc}", [coarse, fine], [itf], **args)

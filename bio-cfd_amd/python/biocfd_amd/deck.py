@@ -62,7 +62,7 @@ def write_deck(case: CS.Case, path: str, dxmin_mm: Optional[float] = None, l_c_m
             raise DeckError("the deck has one surface mesh for every body block (ReadInput.f90:370)")
     if not np.any(mesh.surf_id == 51):
         raise DeckError("main.F90's block-move check reads the marker nodes that only ID-51 nodes define "
-                        "(Search.F90:29-44, :1018-1021): the body needs ID-51 elements (see flap_marked)")
+                        "(Search.F90:29-44, :1018-1021): the body needs ID-51 elements (see cases.flap_marked)")
     if any(int(b.cintp) != 3 for b in blocks):
         raise DeckError("readInput sets cintp = 3 for every block (ReadInput.f90:227)")
     if len({float(b.omega) for b in blocks[3:]}) > 1:
@@ -152,17 +152,6 @@ def write_deck(case: CS.Case, path: str, dxmin_mm: Optional[float] = None, l_c_m
     return written
 
 
-def flap_marked(body: CS.Body) -> CS.Body:
-    """The body with its rigid (ID 50) elements and nodes marked 51.  With
-    flap amplitude a0 = 0 and pivot 0 an ID-51 node is rotated by a zero
-    angle, z cos 0 + y sin 0 + 0 - 0 cos 0 = z and -z sin 0 + y cos 0 = y
-    (Search.F90:186-195), so the body stays where an ID-50 body would be, and
-    main.F90's block-move check has the marker nodes it reads."""
-    sid = np.where(body.surf_id == 50, 51, body.surf_id)
-    nid = np.where(body.node_id == 50, 51, body.node_id)
-    return CS.Body(body.nodes.copy(), nid, body.elems.copy(), sid)
-
-
 def config_deck_wing(nsteps: int = 2, pc_ita_max: int = 10, markers: int = 512) -> CS.Case:
     """Coarse block (32, 44, 20) on [0,4]x[0,5.5]x[0,2.5] (h = 1/8) and a
     refinement-ratio-2 block (44, 72, 24) on [0.5,3.25]x[0.5,5]x[0.5,2]
@@ -180,5 +169,7 @@ def config_deck_wing(nsteps: int = 2, pc_ita_max: int = 10, markers: int = 512) 
     return CS.Case("deck_wing", [coarse, fine], [itf], dt=0.002, re=300.0, nsteps=nsteps, pc_ita_max=pc_ita_max,
                    epsi=0.0, freq=0.5, phase_angle_deg=90.0, dxmin=1e-6, noise_amp=0.0)
 
+
+flap_marked = CS.flap_marked
 
 __all__ = ["write_deck", "DeckError", "flap_marked", "config_deck_wing"]

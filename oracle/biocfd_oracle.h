@@ -54,6 +54,10 @@ typedef struct orc_block {
     double xpth1, xpth2, ypth2, xchg, ychg, xmove, ymove, zmove;
     double inity_cent, nxty_cent, initx_cent, nxtx_cent, initz_cent, nxtz_cent;
     int64_t nIterPcor, mk, mkx1, mkx2, move_check, blk_mv_tag;
+    /* block move (Blocks.f90:64-71): shift in cells, the copied index ranges
+     * before (cpy[0..5]) and after (cpy_mv[0..5]) the move, x/y/z start/end */
+    int64_t move_amtx, move_amty, move_amtz, cpy[6], cpy_mv[6];
+    double *u_dum, *v_dum, *w_dum, *p_dum;
     double derr1, derr2, derrStdSt;
     double viscous_drag, pressure_drag, viscous_lift, pressure_lift;
 } orc_block;
@@ -61,6 +65,7 @@ typedef struct orc_block {
 typedef struct orc_interface {
     int64_t a_blk, b_blk, a_msh, b_msh;     /* 1-based block numbers */
     double xs, xe, ys, ye, zs, ze;
+    double xs_new, xe_new, ys_new, ye_new, zs_new, ze_new;   /* overlap kept by a move */
     /* det tables (3, cap) column-major and counters, order p,u,v,w x,y,z */
     int64_t *det[4][3];
     int64_t counter[4][3];
@@ -127,6 +132,16 @@ void orc_coarse_update_pc(orc_case *c);
 void orc_fine_update_pc_bd(orc_case *c);
 void orc_fine_update_newv_bd(orc_case *c);
 void orc_fine_update_bd_mv(orc_case *c, int64_t g);
+void orc_fine_update_mv(orc_case *c, int64_t g);               /* fineUpdate_mv */
+
+/* -------- block move (Search.F90:916-1461) -------- */
+int orc_move_markers_defined(const orc_case *c);
+void orc_block_move_check(orc_case *c, int64_t *coarse_flcnt_check);
+void orc_change_block_coords(orc_block *b);
+void orc_change_block_coords_interfaces(orc_interface *it, orc_block *b);
+void orc_change_block_interface(orc_interface *it, const orc_block *b);
+void orc_cell_count_solid_coarse_mv(orc_case *c, int64_t *coarse_flcnt_check);
+void orc_tagging_move(orc_case *c, int64_t g);                 /* tagging_th_move */
 
 /* scalar helpers (Interpolation.f90, Forcing.f90:1080-1183) */
 double orc_linear_interpolation(double x, double x0, double x1, double y0, double y1);

@@ -144,7 +144,8 @@ static void free_block(orc_block *b)
                      b->u1NormDis, b->u2NormDis, b->v1NormDis, b->v2NormDis, b->w1NormDis,
                      b->w2NormDis, b->ibNodeId, b->elP1, b->elP2, b->elP3, b->surfId, b->xnode,
                      b->ynode, b->znode, b->xnode1, b->ynode1, b->znode1, b->xcent, b->ycent,
-                     b->zcent, b->cosA, b->cosB, b->cosG, b->elen };
+                     b->zcent, b->cosA, b->cosB, b->cosG, b->elen, b->u_dum, b->v_dum, b->w_dum,
+                     b->p_dum };
     for (size_t q = 0; q < sizeof(ptrs) / sizeof(ptrs[0]); ++q) free(ptrs[q]);
     for (int q = 0; q < ORC_NGHOST; ++q) free(b->ghost[q]);
     for (int t = 0; t < T_COUNT; ++t) { free(b->ca[t]); free(b->ck[t]); }
@@ -258,10 +259,24 @@ int orc_case_step(orc_case *c, int64_t ita, double *totime, double *totalTime)
             orc_compute_surface_variables(b, g_phase, c->piv_pt, ita, c->dt, c->dxmin, c->freq, *totime);
         }
     }
-    /* the coarse block's interface masks, every step (main.F90:195-200) */
-    if (!c->blk[0].has_ib) orc_fine_block_cell(c);
+    /* block move (main.F90:178-203): only where the fine blocks carry the
+     * marker nodes the move check reads */
+    int64_t flcnt = 0;
+    if (c->nintf > 0 && orc_move_markers_defined(c)) orc_block_move_check(c, &flcnt);
+    for (int64_t g = 2; g <= nb; ++g) orc_change_block_coords(&c->blk[g - 1]);
+    for (int64_t q = 1; q <= c->nintf; ++q)
+        orc_change_block_coords_interfaces(&c->intf[q - 1], &c->blk[c->intf[q - 1].b_blk - 1]);
+    for (int64_t q = 1; q <= c->nintf; ++q)
+        orc_change_block_interface(&c->intf[q - 1], &c->blk[c->intf[q - 1].b_blk - 1]);
+    /* the coarse block's interface masks and lists (main.F90:195-200) */
+    if (!c->blk[0].has_ib) {
+        orc_fine_block_cell(c);
+        orc_cell_count_solid_coarse_mv(c, &flcnt);
+    }
     for (int64_t g = 1; g <= nb; ++g)
         if (c->blk[g - 1].has_ib) orc_compute_surface_norm(&c->blk[g - 1], c->inor);
+    for (int64_t g = 1; g <= nb; ++g)
+        if (c->blk[g - 1].has_ib) orc_tagging_move(c, g);
     for (int64_t g = 1; g <= nb; ++g) {
         orc_block *b = &c->blk[g - 1];
         if (!b->has_ib) continue;
@@ -338,6 +353,7 @@ int orc_case_dump(const orc_case *c, const char *out_dir, int64_t step)
         w_f3(f, b, "u", b->u); w_f3(f, b, "v", b->v); w_f3(f, b, "w", b->w); w_f3(f, b, "p", b->p);
         w_f3(f, b, "ut", b->ut); w_f3(f, b, "vt", b->vt); w_f3(f, b, "wt", b->wt);
         w_f3(f, b, "pc", b->pc); w_f3(f, b, "pco", b->pco); w_f3(f, b, "b", b->b);
+        w_f1(f, "x1", b->x1, b->nx + 3); w_f1(f, "y1", b->y1, b->ny + 3); w_f1(f, "z1", b->z1, b->nz + 3);
         w_i3(f, b, "cell", b->cell, 0); w_i3(f, b, "cell2", b->cell2, 1);
         w_i3(f, b, "nodeIdTag", b->nodeIdTag, 1); w_i3(f, b, "cell_n", b->cell_n, 0);
         w_i3(f, b, "cell_pr", b->cell_pr, 1);

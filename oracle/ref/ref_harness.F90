@@ -26,7 +26,9 @@ program ref_harness
   use global, only: block, intfr, pi
   use biocfd_search, only: findDistnode, shiftSurfaceNodesInitial, computeSurfaceNorm, &
        tagging_th_move, cellcount_solid, cellcount_solid_coarse, computenormdistance, &
-       computesurfacevariables, findtscells, fine_block_cell, selectiveretagging_th
+       computesurfacevariables, findtscells, fine_block_cell, selectiveretagging_th, &
+       block_move_check, change_block_coords, change_block_coords_interfaces, &
+       change_block_interface, cellcount_solid_coarse_mv
   use biocfd_pcor_vcor, only: poissonSolver, updateVelocity_newv
   use biocfd_boundary_conditions, only: velocityBC, solidcellbc, solidcellbc_move
   use biocfd_allocate_arrays, only: allocateArrays
@@ -47,7 +49,7 @@ program ref_harness
   integer(int64), allocatable :: has_ib(:), moving(:), dump_steps(:)
   real(dp) :: deltat, re, uc, epsi, noise_amp, aoa, phase_angle, piv_pt, dxmin, freq
   real(dp) :: totime, totalTime, mu_f, rho_f
-  integer(int64) :: count0, count1, count_rate
+  integer(int64) :: count0, count1, count_rate, flcnt
   logical :: do_time, do_mid
   character(len=8) :: dump_prefix
 
@@ -160,10 +162,30 @@ program ref_harness
                 deltat, dxmin, freq, re, totime)
         end if
      end do
-     ! the coarse block's interface masks, every step (main.F90:195-200)
-     if (has_ib(1) == 0) call fine_block_cell
+     ! block move (main.F90:178-203); the move check reads the marker nodes
+     ! findDistnode sets only from ID-51 nodes, so it runs only where every
+     ! fine block has them
+     flcnt = 0
+     if (nintf > 0 .and. markers_defined()) call block_move_check(flcnt)
+     do g = 2, nblocks
+        call change_block_coords(block(g))
+     end do
+     do g = 1, nintf
+        call change_block_coords_interfaces(intfr(g), block(intfr(g)%b_blk))
+     end do
+     do g = 1, nintf
+        call change_block_interface(intfr(g), block(intfr(g)%a_blk), block(intfr(g)%b_blk))
+     end do
+     ! the coarse block's interface masks and lists (main.F90:195-200)
+     if (has_ib(1) == 0) then
+        call fine_block_cell
+        call cellCount_solid_coarse_mv(flcnt)
+     end if
      do g = 1, nblocks
         if (has_ib(g) == 1) call computeSurfaceNorm(block(g), inor)
+     end do
+     do g = 1, nblocks
+        if (has_ib(g) == 1) call tagging_th_move(block(g), g)
      end do
      do g = 1, nblocks
         if (has_ib(g) /= 1) cycle
@@ -310,9 +332,22 @@ contains
     close(u)
   end subroutine write_case_file
 
+  !> Every interface's fine block has the marker nodes of findDistnode.
+  logical function markers_defined()
+    integer(int64) :: q, b
+    markers_defined = .true.
+    do q = 1, nintf
+       b = intfr(q)%b_blk
+       if (block(b)%mk < 1 .or. block(b)%mkx1 < 1 .or. block(b)%mkx2 < 1) markers_defined = .false.
+    end do
+  end function markers_defined
+
   !> The block scalars a case file does not carry.
   subroutine reset_scalars(gb)
     integer(int64), intent(in) :: gb
+    block(gb)%mk = 0
+    block(gb)%mkx1 = 0
+    block(gb)%mkx2 = 0
     block(gb)%nIterPcor = 0
     block(gb)%move_check = 0
     block(gb)%blk_mv_tag = 0
@@ -480,6 +515,9 @@ contains
        call wr3('pc', block(gb)%pc)
        call wr3('pco', block(gb)%pco)
        call wr3('b', block(gb)%b)
+       call wr1('x1', block(gb)%x1)
+       call wr1('y1', block(gb)%y1)
+       call wr1('z1', block(gb)%z1)
        call wi3('cell', block(gb)%cell)
        call wi3('cell2', block(gb)%cell2)
        call wi3('nodeIdTag', block(gb)%nodeIdTag)

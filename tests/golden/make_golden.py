@@ -38,7 +38,7 @@ REF_BIN = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
 STEPS = (0, 2)
 
 # arrays kept in the fixtures (everything the parity bar covers)
-KEEP = ("u", "v", "w", "p", "ut", "vt", "wt", "pc", "b", "cell", "cell2", "nodeIdTag", "cell_n", "nIterPcor",
+KEEP = ("u", "v", "w", "p", "ut", "vt", "wt", "pc", "b", "x1", "y1", "z1", "cell", "cell2", "nodeIdTag", "cell_n", "nIterPcor",
         "derr2", "derrStdSt", "fluidCellCount", "solidCellCount", "ibCellCount", "redCellCount",
         "blackCellCount", "TSCellCount", "fluidIndexPtr", "redCellIndexPtr", "blackCellIndexPtr", "Acx", "Acy",
         "Acz", "ca_uu", "ck_uu", "ca_vu", "ck_vu", "ca_uw", "ck_uw", "interceptedIndexPtr", "solidIndexPtr",
@@ -59,6 +59,7 @@ def golden_cases():
     blk = CS.BlockSpec((0, 0, 0), (1, 1, 1), (16, 16, 16), omega=1.7, body=body, moving=True)
     out["wing"] = CS.Case("wing", [blk], dt=0.002, re=300.0, nsteps=max(STEPS), pc_ita_max=10, epsi=0.0,
                           freq=0.5, phase_angle_deg=90.0)
+    out["move"] = CS.config_nested_move(nsteps=max(STEPS))
     return out
 
 

@@ -63,7 +63,8 @@ def compare(ref, got, label="", field_rtol=FIELD_RTOL, skip=()):
 
 
 def small_cases():
-    """The parity cases: single sphere, channel, nested moving sphere, flapping body."""
+    """The parity cases: single sphere, channel, nested heaving sphere, flapping body, and a
+    nested case whose fine block moves every step."""
     sphere = CS.config_sphere_64(n=24, markers=256, nsteps=3, pc_ita_max=20, epsi=0.0)
     channel = CS.config_channel(n=(16, 12, 10), nsteps=3)
     nested = CS.config_nested(nc=24, nf=24, markers=256, nsteps=3, pc_ita_max=15, epsi=0.0)
@@ -71,4 +72,5 @@ def small_cases():
     blk = CS.BlockSpec((0, 0, 0), (1, 1, 1), (24, 24, 24), omega=1.7, body=body, moving=True)
     wing = CS.Case("wing", [blk], dt=0.002, re=300.0, nsteps=3, pc_ita_max=15, epsi=0.0, freq=0.5,
                    phase_angle_deg=90.0)
-    return {"sphere": sphere, "channel": channel, "nested": nested, "wing": wing}
+    move = CS.config_nested_move(nsteps=3)
+    return {"sphere": sphere, "channel": channel, "nested": nested, "wing": wing, "move": move}

@@ -145,25 +145,38 @@ def parse_field_file(path):
     return np.array([float(v) for v in tok]).reshape(-1, 11)
 
 
+def deck_source(name):
+    """The committed deck, or the moving case written as a deck."""
+    if name == "deck_wing":
+        return make_golden.DECK_DIR
+    d = os.path.join(tempfile.mkdtemp(prefix="deckmv_"), "deck")
+    DK.write_deck(CS.config_nested_move(), d)
+    return d
+
+
 @pytest.mark.skipif(not (os.path.exists(REF_MAIN) and os.path.exists(REF_HARNESS)),
                     reason="reference binaries not built (make -C oracle ref)")
-def test_unchanged_main_runs_the_deck():
+@pytest.mark.parametrize("name,moves", [("deck_wing", False), ("move", True)])
+def test_unchanged_main_runs_the_deck(name, moves):
     """The reference's own main.F90, unchanged, runs the deck; the field
     files it writes after the set-up and in steps 1 and 2 equal, bit for bit,
     the state the harness (the oracle's reference) reaches on the same deck
-    at the same points."""
+    at the same points: cell centres, u, v, w (face averages), p and the
+    tags.  In the moving case the fine block moves in both steps (main.F90's
+    block_move_check, change_block_* and tagging_th_move)."""
+    src = deck_source(name)
     wd = tempfile.mkdtemp(prefix="deckmain_")
     main_dir = os.path.join(wd, "main")
     h_dir = os.path.join(wd, "harness")
     h_out = os.path.join(wd, "harness_out")
-    shutil.copytree(make_golden.DECK_DIR, main_dir)
-    shutil.copytree(make_golden.DECK_DIR, h_dir)
+    shutil.copytree(src, main_dir)
+    shutil.copytree(src, h_dir)
     os.makedirs(os.path.join(main_dir, "out", "Chkpt"))
     os.makedirs(h_out)
     env = dict(os.environ, OMP_NUM_THREADS="4")
     r = subprocess.run([REF_MAIN], cwd=main_dir, capture_output=True, text=True, timeout=600, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    assert "blk_movey" not in r.stdout          # the block never moves
+    assert r.stdout.count("blk_movex") == (2 if moves else 0)
     r = subprocess.run([REF_HARNESS, "--deck", h_out, "--mid"], cwd=h_dir, capture_output=True, text=True,
                        timeout=600, env=env)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
@@ -177,9 +190,12 @@ def test_unchanged_main_runs_the_deck():
             u, v, w = d["u"], d["v"], d["w"]
             nx, ny, nz = (n - 2 for n in u.shape)
             I, J, K = slice(1, nx + 1), slice(1, ny + 1), slice(1, nz + 1)
-            cols = {3: 0.5 * (u[1:nx + 1, J, K] + u[0:nx, J, K]), 4: 0.5 * (v[I, 1:ny + 1, K] + v[I, 0:ny, K]),
-                    5: 0.5 * (w[I, J, 1:nz + 1] + w[I, J, 0:nz]), 6: d["p"][I, J, K], 8: d["cell"][I, J, K],
-                    9: d["cell_n"][I, J, K], 10: d["cell_pr"][I, J, K]}
+            cen = [0.5 * (d[a][1:n + 1] + d[a][2:n + 2]) for a, n in (("x1", nx), ("y1", ny), ("z1", nz))]
+            X, Y, Z = np.meshgrid(*cen, indexing="ij")
+            cols = {0: X, 1: Y, 2: Z, 3: 0.5 * (u[1:nx + 1, J, K] + u[0:nx, J, K]),
+                    4: 0.5 * (v[I, 1:ny + 1, K] + v[I, 0:ny, K]), 5: 0.5 * (w[I, J, 1:nz + 1] + w[I, J, 0:nz]),
+                    6: d["p"][I, J, K], 8: d["cell"][I, J, K], 9: d["cell_n"][I, J, K],
+                    10: d["cell_pr"][I, J, K]}
             assert m.shape[0] == nx * ny * nz
             for c, a in cols.items():
                 n_bad = int(np.count_nonzero(m[:, c] != a.reshape(-1, order="F").astype(np.float64)))

@@ -42,7 +42,7 @@ def run_case(case, steps):
     return wd
 
 
-@pytest.mark.parametrize("name", ["sphere", "channel", "nested", "wing"])
+@pytest.mark.parametrize("name", ["sphere", "channel", "nested", "wing", "move"])
 def test_oracle_matches_reference_fixtures(name):
     """Bit-identical to the reference CPU build: every dumped array equal."""
     case = make_golden.golden_cases()[name]
@@ -68,6 +68,14 @@ def test_fixtures_exercise_every_path():
     assert "s2_b2_u" in n.files and float(n["s2_b2_ydot"]) != 0.0
     s = np.load(make_golden.fixture_path("sphere"))
     assert int(s["s2_b1_ibCellCount"]) > 0 and int(s["s2_b1_nIterPcor"]) == 13
+    # the moving case: the fine block's grid moved by one coarse cell per
+    # step in -x and +y (two steps), not in z
+    m = np.load(make_golden.fixture_path("move"))
+    h = 3.0 / 24
+    assert np.allclose(m["s2_b2_x1"] - m["s0_b2_x1"], -4 * h, rtol=0, atol=1e-12)
+    assert np.allclose(m["s2_b2_y1"] - m["s0_b2_y1"], 4 * h, rtol=0, atol=1e-12)
+    assert np.array_equal(m["s2_b2_z1"], m["s0_b2_z1"])
+    assert not np.array_equal(m["s2_b1_cell"], m["s0_b1_cell"])
 
 
 def test_reference_interpolation_vectors():
