@@ -115,6 +115,18 @@ struct IbData {
   bool present = false;
 };
 
+// The last block move of a fine block (Blocks.f90:64-71): the shift in
+// cells per axis, the index ranges of the kept overlap before the move (cpy)
+// and after it (cpy_mv), x start/end, y start/end, z start/end, and the
+// fields u, v, w, p kept for the shifted copy (device, allocated at the
+// first move).
+struct MoveState {
+  int64_t amt[3] = {0, 0, 0};
+  int64_t cpy[6] = {0, 0, 0, 0, 0, 0};
+  int64_t cpy_mv[6] = {0, 0, 0, 0, 0, 0};
+  double *dum[4] = {nullptr, nullptr, nullptr, nullptr};
+};
+
 struct Block {
   bool live = false;
   Dims d{};
@@ -148,6 +160,7 @@ struct Block {
   int64_t mx_n = 0;
   double *partials = nullptr;         // per-workgroup maxima of the reductions
   int64_t cap_partials = 0;
+  MoveState mv;
 };
 
 struct Interface {
@@ -155,6 +168,7 @@ struct Interface {
   int a = 0, b = 0;                    // 1-based block numbers
   int64_t a_msh = 1, b_msh = 1;
   double ext[6] = {0, 0, 0, 0, 0, 0};
+  double ext_new[6] = {0, 0, 0, 0, 0, 0};  // the overlap a block move keeps
   // det tables [comp p,u,v,w][axis x,y,z]: host (3,cap) and device copies
   std::vector<int64_t> hdet[4][3];
   int64_t counter[4][3] = {};

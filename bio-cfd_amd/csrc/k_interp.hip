@@ -134,6 +134,45 @@ __global__ void k_prolong(Dims cd, Dims fd, ITab t, int comp, int axis, GridSet 
   if (fvar2) fvar2[dst] = out;
 }
 
+// coarse -> fine volume fill after a block move (FineInterp.f90:10-281):
+// each fine cell of a table entry (entries 2 .. counter-1 per axis) whose
+// cell_n is not 1 takes the bilinear value, in x and y, of the coarse planes
+// below and above the entry's coarse cell, blended linearly in z.  The
+// staggered component's coarse values and its fine target sit one index
+// lower along its axis (u(i) at xu(i+1)).  One thread per (entry, fine cell).
+__global__ void k_fine_mv(Dims cd, Dims fd, ITab t, int comp, GridSet cg, GridSet fg,
+                          const uint8_t *__restrict__ cell_n, const double *__restrict__ cvar,
+                          double *__restrict__ fvar, int factor) {
+  const int ni = t.cnt[0] - 2, nj = t.cnt[1] - 2, nk = t.cnt[2] - 2;
+  if (ni <= 0 || nj <= 0 || nk <= 0) return;
+  const int per = factor * factor * factor;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)ni * nj * nk * per) return;
+  const int64_t e = q / per;
+  const int sub = (int)(q % per);
+  const int i = 2 + (int)(e % ni), j = 2 + (int)((e / ni) % nj), k = 2 + (int)(e / ((int64_t)ni * nj));
+  const int s = DET(t, 0, 2, i) + sub % factor, l = DET(t, 1, 2, j) + (sub / factor) % factor,
+            r = DET(t, 2, 2, k) + sub / (factor * factor);
+  if (s > DET(t, 0, 3, i) || l > DET(t, 1, 3, j) || r > DET(t, 2, 3, k)) return;
+  if (cell_n[ix3(fd, s, l, r)] == 1) return;
+  const int lx = DET(t, 0, 1, i), ly = DET(t, 1, 1, j), lz = DET(t, 2, 1, k);
+  const int ox = comp == 1, oy = comp == 2, oz = comp == 3;
+  const int i0 = lx - 1 - ox, i1 = lx + 1 - ox, j0 = ly - 1 - oy, j1 = ly + 1 - oy, k0 = lz - 1 - oz,
+            k1 = lz + 1 - oz;
+  const double vx = fg.x[s - 1], vy = fg.y[l - 1], vz = fg.z[r - 1];
+  const double x1 = cg.x[lx - 2], x2 = cg.x[lx], y1 = cg.y[ly - 2], y2 = cg.y[ly], z1 = cg.z[lz - 2],
+               z2 = cg.z[lz];
+  const double deno = (x2 - x1) * (y2 - y1);
+  const double xtx = (x2 - vx), xxo = (vx - x1), yty = (y2 - vy), yyo = (vy - y1);
+  double first = (cvar[ix3(cd, i0, j0, k0)] * xtx + cvar[ix3(cd, i1, j0, k0)] * xxo) * yty;
+  double second = (cvar[ix3(cd, i0, j1, k0)] * xtx + cvar[ix3(cd, i1, j1, k0)] * xxo) * yyo;
+  const double ans1 = (first + second) / deno;
+  first = (cvar[ix3(cd, i0, j0, k1)] * xtx + cvar[ix3(cd, i1, j0, k1)] * xxo) * yty;
+  second = (cvar[ix3(cd, i0, j1, k1)] * xtx + cvar[ix3(cd, i1, j1, k1)] * xxo) * yyo;
+  const double ans2 = (first + second) / deno;
+  fvar[ix3(fd, s - ox, l - oy, r - oz)] = ans1 + ((vz - z1) * ((ans2 - ans1) / (z2 - z1)));
+}
+
 static GridSet grids_of(const Block &b, int comp) {
   switch (comp) {
     case 0: return {b.g[BG_XP], b.g[BG_YP], b.g[BG_ZP]};
@@ -181,11 +220,10 @@ int interface_coarse_update(int which) {
   return 0;
 }
 
-int interface_fine_update(int which) {
+int interface_fine_update_one(int q, int which) {
   hipStream_t s = rt().stream;
-  for (int q = 1; q <= kMaxInterfaces; ++q) {
+  {
     Interface &it = rt().itf[q];
-    if (!it.live) continue;
     Block &a = rt().blk[it.a], &b = rt().blk[it.b];
     const int factor = (int)(it.b_msh / it.a_msh);
     auto run = [&](int comp, const double *cv, double *fv) {
@@ -211,6 +249,50 @@ int interface_fine_update(int which) {
       run(0, a.f[BF_PC], b.f[BF_PC]);
     }
   }
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+int interface_fine_update(int which) {
+  for (int q = 1; q <= kMaxInterfaces; ++q)
+    if (rt().itf[q].live && interface_fine_update_one(q, which)) return -1;
+  return 0;
+}
+
+// fineUpdate_mv of interface q.  The reference reads the coarse planes and
+// rows at loc-1 .. loc+1 (loc-2 for the staggered component) without a
+// check; here every table entry is checked on the host first, and an entry
+// whose stencil leaves either block is an error.
+int interface_fine_update_mv(int q) {
+  Interface &it = rt().itf[q];
+  Block &a = rt().blk[it.a], &b = rt().blk[it.b];
+  const int factor = (int)(it.b_msh / it.a_msh);
+  const int cn[3] = {a.d.NX, a.d.NY, a.d.NZ}, fn[3] = {b.d.NX, b.d.NY, b.d.NZ};
+  for (int comp = 0; comp < 4; ++comp)
+    for (int ax = 0; ax < 3; ++ax) {
+      const std::vector<int64_t> &h = it.hdet[comp][ax];
+      const int64_t cnt = it.counter[comp][ax];
+      const int off = comp == ax + 1;
+      for (int64_t n = 2; n <= cnt - 1; ++n) {
+        const int64_t loc = h[3 * (n - 1)], lo = h[3 * (n - 1) + 1], hi = h[3 * (n - 1) + 2];
+        if (loc - 1 - off < 1 || loc + 1 > cn[ax] || lo - off < 1 || hi > fn[ax]) {
+          set_error("fine_update_mv: interface %d entry %lld reaches outside the blocks", q, (long long)n);
+          return -1;
+        }
+      }
+    }
+  hipStream_t s = rt().stream;
+  auto run = [&](int comp, const double *cv, double *fv) {
+    ITab t = tab_of(it, comp);
+    const int64_t total = (int64_t)(t.cnt[0] - 2) * (t.cnt[1] - 2) * (t.cnt[2] - 2) * factor * factor * factor;
+    if (t.cnt[0] <= 2 || t.cnt[1] <= 2 || t.cnt[2] <= 2) return;
+    hipLaunchKernelGGL(k_fine_mv, dim3(div_up(total, 256)), dim3(256), 0, s, a.d, b.d, t, comp, grids_of(a, comp),
+                       grids_of(b, comp), b.tag[BT_CELL_N], cv, fv, factor);
+  };
+  run(0, a.f[BF_P], b.f[BF_P]);
+  run(1, a.f[BF_U], b.f[BF_U]);
+  run(2, a.f[BF_V], b.f[BF_V]);
+  run(3, a.f[BF_W], b.f[BF_W]);
   BCFD_CHECK_LAUNCH();
   return 0;
 }
@@ -275,6 +357,16 @@ int biocfd_coarse_update(int which) {
   if (ensure_init()) return -1;
   if (which < 0 || which > 2) { set_error("coarse_update: bad selector %d", which); return -1; }
   return interface_coarse_update(which);
+}
+
+int biocfd_fine_update_mv(int itf) {
+  BCFD_API_LOCK();
+  if (ensure_init()) return -1;
+  if (itf < 1 || itf > kMaxInterfaces || !rt().itf[itf].live) {
+    set_error("fine_update_mv: no interface %d", itf);
+    return -1;
+  }
+  return interface_fine_update_mv(itf);
 }
 
 int biocfd_fine_update_bd(int which) {

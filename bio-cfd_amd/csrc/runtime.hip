@@ -234,6 +234,7 @@ static void free_block(Block &b) {
     if (b.r32[q]) (void)hipFree(b.r32[q]);
   }
   if (b.partials) (void)hipFree(b.partials);
+  for (auto &p : b.mv.dum) if (p) (void)hipFree(p);
   IbData &ib = b.ib;
   for (auto &p : ib.el) if (p) (void)hipFree(p);
   if (ib.surf) (void)hipFree(ib.surf);

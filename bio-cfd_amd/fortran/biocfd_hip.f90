@@ -547,6 +547,75 @@ module biocfd_hip
       type(c_ptr), value :: s
       integer(c_size_t) :: c_strlen
     end function
+
+    ! ---- block move (Search.F90:21-47, :415-432, :916-1461; FineInterp.f90:10-281)
+    integer(c_int) function biocfd_find_dist_node(n_nodes, xnode, znode, node_id, markers) bind(C, &
+        name="biocfd_find_dist_node")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int64_t), value :: n_nodes
+      real(c_double) :: xnode(*)
+      real(c_double) :: znode(*)
+      integer(c_int64_t) :: node_id(*)
+      integer(c_int64_t) :: markers(3)
+    end function
+
+    integer(c_int) function biocfd_block_move_check(itf, xnode1, ynode1, markers, state, moved, &
+        coarse_flcnt_check) bind(C, name="biocfd_block_move_check")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: itf
+      real(c_double) :: xnode1(*)
+      real(c_double) :: ynode1(*)
+      integer(c_int64_t) :: markers(3)
+      real(c_double) :: state(*)
+      integer(c_int64_t) :: moved
+      integer(c_int64_t) :: coarse_flcnt_check
+    end function
+
+    integer(c_int) function biocfd_change_block_coords(blk, move_check) bind(C, name="biocfd_change_block_coords")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: move_check
+    end function
+
+    integer(c_int) function biocfd_change_block_coords_interfaces(itf, move_check) bind(C, &
+        name="biocfd_change_block_coords_interfaces")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: itf
+      integer(c_int64_t), value :: move_check
+    end function
+
+    integer(c_int) function biocfd_change_block_interface(itf, move_check) bind(C, &
+        name="biocfd_change_block_interface")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: itf
+      integer(c_int64_t), value :: move_check
+    end function
+
+    integer(c_int) function biocfd_cell_count_solid_coarse_mv(coarse_blk, coarse_flcnt_check) bind(C, &
+        name="biocfd_cell_count_solid_coarse_mv")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: coarse_blk
+      integer(c_int64_t) :: coarse_flcnt_check
+    end function
+
+    integer(c_int) function biocfd_fine_update_mv(itf) bind(C, name="biocfd_fine_update_mv")
+      import :: c_int
+      integer(c_int), value :: itf
+    end function
+
+    integer(c_int) function biocfd_tagging_move(blk, move_check) bind(C, name="biocfd_tagging_move")
+      import :: c_int, c_int64_t
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: move_check
+    end function
+
+    integer(c_int) function biocfd_grid_download(blk, which, host, n) bind(C, name="biocfd_grid_download")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int), value :: which
+      real(c_double) :: host(*)
+      integer(c_int64_t), value :: n
+    end function
   end interface
 
 contains

@@ -45,6 +45,8 @@ module biocfd_host
     real(real64), allocatable :: xnode(:), ynode(:), znode(:), xnode1(:), ynode1(:), znode1(:)
     integer(int64), allocatable :: node_id(:), el1(:), el2(:), el3(:), surf(:)
     real(real64) :: kin(0:BKS_N - 1) = 0
+    ! block move: marker nodes of findDistnode, move_check, blk_mv_tag
+    integer(int64) :: mk(3) = 0, move_check = 0, blk_mv_tag = 0
   end type
 
   type :: host_interface
@@ -559,6 +561,7 @@ contains
       b => blk(g)
       if (b%has_ib /= 1) cycle
       call biocfd_check(biocfd_body_create(g, b%nn, b%ne, b%el1, b%el2, b%el3, b%surf, b%box), 'body_create')
+      call biocfd_check(biocfd_find_dist_node(b%nn, b%xnode, b%znode, b%node_id, b%mk), 'find_dist_node')
       b%kin = 0
       b%kin(0:5) = [b%centre(1), b%centre(2), b%centre(3), b%a0, b%yamp, b%bfreq]
       call biocfd_check(biocfd_shift_surface_nodes_initial(b%nn, b%xnode, b%ynode, b%znode, b%node_id, aoa, &
@@ -600,9 +603,19 @@ contains
     call biocfd_check(biocfd_grid_upload(g, which, a, int(size(a), int64)), 'grid_upload')
   end subroutine
 
+  ! every interface's fine block has the marker nodes of findDistnode
+  logical function markers_defined()
+    integer(int64) :: q
+    markers_defined = .true.
+    do q = 1, nintf
+      if (any(blk(itf(q)%b)%mk < 1)) markers_defined = .false.
+    end do
+  end function
+
   ! one time step, main.F90:133-263
   subroutine time_step()
     integer :: g
+    integer(int64) :: q, flcnt, moved
     type(host_block), pointer :: b
     real(real64) :: kv(0:BK_N - 1)
     ita = ita + 1
@@ -629,12 +642,42 @@ contains
     do g = 1, int(nblocks)
       b => blk(g)
       if (b%has_ib /= 1) cycle
+      b%blk_mv_tag = 0
       call biocfd_check(biocfd_compute_surface_variables(b%nn, b%xnode, b%ynode, b%znode, b%node_id, phase, &
                         piv_pt, ita, dt, dxmin, freq, totime, b%kin, b%xnode1, b%ynode1, b%znode1), &
                         'compute_surface_variables')
     end do
-    ! the coarse block's interface masks, every step (main.F90:195-200)
-    if (blk(1)%has_ib /= 1) call biocfd_check(biocfd_fine_block_cell(1_c_int), 'fine_block_cell')
+    ! block move (main.F90:178-203); the move check reads the marker nodes
+    ! findDistnode sets only from ID-51 nodes, so it runs only where every
+    ! fine block has them
+    flcnt = 0
+    if (nintf > 0 .and. markers_defined()) then
+      do q = 1, nintf
+        b => blk(itf(q)%b)
+        call biocfd_check(biocfd_block_move_check(int(q, c_int), b%xnode1, b%ynode1, b%mk, b%kin, moved, flcnt), &
+                          'block_move_check')
+        if (moved == 1) then
+          b%move_check = 1
+          b%blk_mv_tag = 1
+        end if
+      end do
+    end if
+    do g = 2, int(nblocks)
+      call biocfd_check(biocfd_change_block_coords(g, blk(g)%move_check), 'change_block_coords')
+    end do
+    do q = 1, nintf
+      call biocfd_check(biocfd_change_block_coords_interfaces(int(q, c_int), blk(itf(q)%b)%move_check), &
+                        'change_block_coords_interfaces')
+    end do
+    do q = 1, nintf
+      call biocfd_check(biocfd_change_block_interface(int(q, c_int), blk(itf(q)%b)%move_check), &
+                        'change_block_interface')
+    end do
+    ! the coarse block's interface masks and lists (main.F90:195-200)
+    if (blk(1)%has_ib /= 1) then
+      call biocfd_check(biocfd_fine_block_cell(1_c_int), 'fine_block_cell')
+      call biocfd_check(biocfd_cell_count_solid_coarse_mv(1_c_int, flcnt), 'cell_count_solid_coarse_mv')
+    end if
     do g = 1, int(nblocks)
       b => blk(g)
       if (b%has_ib /= 1) cycle
@@ -643,11 +686,18 @@ contains
     do g = 1, int(nblocks)
       b => blk(g)
       if (b%has_ib /= 1) cycle
+      call biocfd_check(biocfd_tagging_move(g, b%move_check), 'tagging_move')
+    end do
+    do g = 1, int(nblocks)
+      b => blk(g)
+      if (b%has_ib /= 1) cycle
       kv = kin_vec(b)
-      call biocfd_check(biocfd_selective_retagging(g, 0_int64), 'selective_retagging')
+      call biocfd_check(biocfd_selective_retagging(g, b%blk_mv_tag), 'selective_retagging')
+      b%blk_mv_tag = 0
       call biocfd_check(biocfd_cell_count_solid(g), 'cell_count_solid')
-      call biocfd_check(biocfd_solid_cell_bc_move(g, 0_int64), 'solid_cell_bc_move')
-      call biocfd_check(biocfd_update_velocity_newv(g, 0_int64), 'update_velocity_newv')
+      call biocfd_check(biocfd_solid_cell_bc_move(g, b%move_check), 'solid_cell_bc_move')
+      call biocfd_check(biocfd_update_velocity_newv(g, b%move_check), 'update_velocity_newv')
+      b%move_check = 0
       call biocfd_check(biocfd_compute_norm_distance(g), 'compute_norm_distance')
       call biocfd_check(biocfd_find_ts_cells(g), 'find_ts_cells')
       call biocfd_check(biocfd_velocity_forcing_field(g, kv), 'velocity_forcing_field')
@@ -696,6 +746,12 @@ contains
         call rec_r3(u, fields(q), f3)
       end do
       deallocate(f3)
+      call biocfd_check(biocfd_grid_download(g, BG_X1, b%x1, b%nx + 3), 'grid_download')
+      call biocfd_check(biocfd_grid_download(g, BG_Y1, b%y1, b%ny + 3), 'grid_download')
+      call biocfd_check(biocfd_grid_download(g, BG_Z1, b%z1, b%nz + 3), 'grid_download')
+      call rec_r1(u, 'x1', b%x1)
+      call rec_r1(u, 'y1', b%y1)
+      call rec_r1(u, 'z1', b%z1)
       do q = 1, 4
         if (q == 1 .or. q == 4) then
           allocate(t3(nxg, nyg, nzg))

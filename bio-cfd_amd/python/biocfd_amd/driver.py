@@ -32,6 +32,10 @@ class BlockState:
         self.kin = np.zeros(H.BKS_N)
         self.body = spec.body
         self.xnode1 = self.ynode1 = self.znode1 = None
+        # block move: marker nodes of findDistnode, move_check, blk_mv_tag
+        self.markers = np.zeros(3, dtype=np.int64)
+        self.move_check = 0
+        self.blk_mv_tag = 0
 
     @property
     def shape(self):
@@ -100,6 +104,7 @@ class Simulation:
             box = np.array(CS.search_box(st.spec, st.grids), dtype=np.int64)
             H.call("biocfd_body_create", st.g, b.nodes.shape[0], b.elems.shape[0], H.ptr(st.e1), H.ptr(st.e2),
                    H.ptr(st.e3), H.ptr(st.sid), H.ptr(box))
+            H.call("biocfd_find_dist_node", st.nn, H.ptr(st.xn), H.ptr(st.zn), H.ptr(st.nid), H.ptr(st.markers))
             st.kin[:6] = [st.body.centre[0], st.body.centre[1], st.body.centre[2], st.body.a0_deg,
                           st.body.yamp, st.body.bfreq]
             H.call("biocfd_shift_surface_nodes_initial", st.nn, H.ptr(st.xn), H.ptr(st.yn), H.ptr(st.zn),
@@ -194,6 +199,7 @@ class Simulation:
             H.call("biocfd_pressure_forcing1", st.g, H.ptr(kv))
 
         def surface_variables(st):
+            st.blk_mv_tag = 0
             H.call("biocfd_compute_surface_variables", st.nn, H.ptr(st.xn), H.ptr(st.yn), H.ptr(st.zn),
                    H.ptr(st.nid), self.phase, c.piv_pt, self.ita, c.dt, c.dxmin, c.freq, self.totime,
                    H.ptr(st.kin), H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.znode1))
@@ -202,13 +208,18 @@ class Simulation:
             H.call("biocfd_compute_surface_norm", st.g, H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.znode1),
                    c.inor)
 
+        def tagging_move(st):
+            H.call("biocfd_tagging_move", st.g, st.move_check)
+
         def retag_and_force(st):
             g = st.g
             kv = st.kin_vec()
-            H.call("biocfd_selective_retagging", g, 0)
+            H.call("biocfd_selective_retagging", g, st.blk_mv_tag)
+            st.blk_mv_tag = 0
             H.call("biocfd_cell_count_solid", g)
-            H.call("biocfd_solid_cell_bc_move", g, 0)
-            H.call("biocfd_update_velocity_newv", g, 0)
+            H.call("biocfd_solid_cell_bc_move", g, st.move_check)
+            H.call("biocfd_update_velocity_newv", g, st.move_check)
+            st.move_check = 0
             H.call("biocfd_compute_norm_distance", g)
             H.call("biocfd_find_ts_cells", g)
             H.call("biocfd_velocity_forcing_field", g, H.ptr(kv))
@@ -220,11 +231,36 @@ class Simulation:
         H.call("biocfd_poisson_solver", 1, len(self.blocks), c.pc_ita_max, c.epsi, c.dt, c.uc)
         self._each(pressure_forcing, bodies_only=True)
         self._each(surface_variables, bodies_only=True)
-        if self.blocks[0].body is None:
-            # the coarse block's interface masks, every step (main.F90:195-200)
-            H.call("biocfd_fine_block_cell", 1)
+        self._block_move()
         self._each(surface_norm, bodies_only=True)
+        self._each(tagging_move, bodies_only=True)
         self._each(retag_and_force, bodies_only=True)
+
+    def _block_move(self):
+        """block_move_check and the moves it decides (main.F90:178-203).  The
+        move check reads the marker nodes findDistnode sets only from ID-51
+        nodes, so it runs only where every fine block has them."""
+        c = self.case
+        flcnt = np.zeros(1, dtype=np.int64)
+        fine = [self.blocks[itf.b_blk - 1] for itf in c.interfaces]
+        if fine and all(np.all(st.markers > 0) for st in fine):
+            moved = np.zeros(1, dtype=np.int64)
+            for q, st in enumerate(fine, start=1):
+                H.call("biocfd_block_move_check", q, H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.markers),
+                       H.ptr(st.kin), H.ptr(moved), H.ptr(flcnt))
+                if moved[0] == 1:
+                    st.move_check = 1
+                    st.blk_mv_tag = 1
+        for st in self.blocks[1:]:
+            H.call("biocfd_change_block_coords", st.g, st.move_check)
+        for q, st in enumerate(fine, start=1):
+            H.call("biocfd_change_block_coords_interfaces", q, st.move_check)
+        for q, st in enumerate(fine, start=1):
+            H.call("biocfd_change_block_interface", q, st.move_check)
+        if self.blocks[0].body is None:
+            # the coarse block's interface masks and lists (main.F90:195-200)
+            H.call("biocfd_fine_block_cell", 1)
+            H.call("biocfd_cell_count_solid_coarse_mv", 1, H.ptr(flcnt))
 
     # ---------------------------------------------------------------- data access
     def upload(self, g: int, name: str, a: np.ndarray):
@@ -259,6 +295,10 @@ class Simulation:
                 d[name] = self.field(g, name)
             for name in ("cell", "cell2", "nodeIdTag", "cell_n", "cell_pr"):
                 d[name] = self.tag(g, name)
+            for name, w, n in (("x1", 0, st.n[0] + 3), ("y1", 1, st.n[1] + 3), ("z1", 2, st.n[2] + 3)):
+                a = np.zeros(n)
+                H.call("biocfd_grid_download", g, w, H.ptr(a), n)
+                d[name] = a
             it = np.zeros(1, dtype=np.int64)
             e2 = np.zeros(1)
             es = np.zeros(1)

@@ -103,6 +103,15 @@ SIGNATURES = {
     "biocfd_interface_table": [_I, _I, _I, _P, _P, _L],
     "biocfd_shift_surface_nodes_initial": [_L, _P, _P, _P, _P, _D, _D, _L, _D, _D, _D, _P, _P, _P, _P],
     "biocfd_compute_surface_variables": [_L, _P, _P, _P, _P, _D, _D, _L, _D, _D, _D, _D, _P, _P, _P, _P],
+    "biocfd_find_dist_node": [_L, _P, _P, _P, _P],
+    "biocfd_block_move_check": [_I, _P, _P, _P, _P, _P, _P],
+    "biocfd_change_block_coords": [_I, _L],
+    "biocfd_change_block_coords_interfaces": [_I, _L],
+    "biocfd_change_block_interface": [_I, _L],
+    "biocfd_cell_count_solid_coarse_mv": [_I, _P],
+    "biocfd_fine_update_mv": [_I],
+    "biocfd_tagging_move": [_I, _L],
+    "biocfd_grid_download": [_I, _I, _P, _L],
 }
 
 

@@ -216,6 +216,43 @@ int biocfd_compute_surface_variables(int64_t n_nodes, const double *xnode, const
                                      double dxmin, double freq, double totime, double *state,
                                      double *xnode1, double *ynode1, double *znode1);
 
+/* ---- block move (Search.F90:21-47, :415-432, :916-1461; FineInterp.f90:10-281) ---- */
+/* findDistnode :21-47: the 1-based marker nodes {mk, mkx1, mkx2} among the
+ * ID-51 nodes (body-frame coordinates), 0 where no ID-51 node qualifies;
+ * block_move_check needs all three */
+int biocfd_find_dist_node(int64_t n_nodes, const double *xnode, const double *znode,
+                          const int64_t *node_id, int64_t markers[3]);
+/* block_move_check :997-1203 for interface itf: when the body of its fine
+ * block (nodes xnode1/ynode1 at the markers, kinematic state) is within the
+ * margins of the interface, the block moves: *moved = 1 (the caller sets the
+ * block's move_check and blk_mv_tag to 1), *coarse_flcnt_check = 1, the
+ * state's init*_cent are updated, the interface extents shift and the fine
+ * block's u, v, w, p are kept for the shifted copy; else *moved = 0 */
+int biocfd_block_move_check(int itf, const double *xnode1, const double *ynode1,
+                            const int64_t markers[3], double *state, int64_t *moved,
+                            int64_t *coarse_flcnt_check);
+/* change_block_coords :1208-1264: the block's grids shifted by its last move
+ * (nothing unless move_check == 1) */
+int biocfd_change_block_coords(int blk, int64_t move_check);
+/* change_block_coords_interfaces :1269-1370: interior mask cell_n of the
+ * moved fine block, its kept overlap copied to the new indices, zero
+ * elsewhere (move_check of the interface's fine block) */
+int biocfd_change_block_coords_interfaces(int itf, int64_t move_check);
+/* change_block_interface :1372-1461: coarse indices of the tables shifted */
+int biocfd_change_block_interface(int itf, int64_t move_check);
+/* cellCount_solid_coarse_mv :916-995: the coarse block's lists rebuilt when
+ * *coarse_flcnt_check == 1; the flag is cleared */
+int biocfd_cell_count_solid_coarse_mv(int coarse_blk, int64_t *coarse_flcnt_check);
+/* fineUpdate_mv, FineInterp.f90:10-281: coarse -> fine volume fill of the
+ * cells of interface itf's fine block outside its interior mask */
+int biocfd_fine_update_mv(int itf);
+/* tagging_th_move :415-432: nothing unless move_check == 1; else the search
+ * box's tagging, then fineUpdate_mv and fineUpdate_bd_mv of every interface
+ * whose fine block is blk */
+int biocfd_tagging_move(int blk, int64_t move_check);
+/* a block's grid array (enum biocfd_grid) back to the host, n entries */
+int biocfd_grid_download(int blk, int which, double *host, int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 STEPS = (0, 1, 3)
 
 
-@pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing"])
+@pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing", "move"])
 def test_time_step_parity(name):
     from biocfd_amd.driver import Simulation
 

@@ -8,7 +8,7 @@ from parity import compare, run_oracle, small_cases
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["nested", "sphere"])
+@pytest.mark.parametrize("name", ["nested", "sphere", "move"])
 def test_threaded_blocks_match_oracle(name):
     from biocfd_amd.driver import Simulation
 
