@@ -161,6 +161,9 @@ struct Block {
   double *partials = nullptr;         // per-workgroup maxima of the reductions
   int64_t cap_partials = 0;
   MoveState mv;
+  // output view staging (cell-centre velocities, p, tags of the interior)
+  double *out_stage = nullptr;
+  int64_t out_cap = 0;
 };
 
 struct Interface {

@@ -235,6 +235,7 @@ static void free_block(Block &b) {
   }
   if (b.partials) (void)hipFree(b.partials);
   for (auto &p : b.mv.dum) if (p) (void)hipFree(p);
+  if (b.out_stage) (void)hipFree(b.out_stage);
   IbData &ib = b.ib;
   for (auto &p : ib.el) if (p) (void)hipFree(p);
   if (ib.surf) (void)hipFree(ib.surf);

@@ -616,6 +616,20 @@ module biocfd_hip
       real(c_double) :: host(*)
       integer(c_int64_t), value :: n
     end function
+
+    ! ---- output sync (WriteOutput_corner1.F90:80-139)
+    integer(c_int) function biocfd_output_fields(blk, u1, v1, w1, p, cell, cell_n, cell_pr) bind(C, &
+        name="biocfd_output_fields")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      real(c_double) :: u1(*)
+      real(c_double) :: v1(*)
+      real(c_double) :: w1(*)
+      real(c_double) :: p(*)
+      integer(c_int64_t) :: cell(*)
+      integer(c_int64_t) :: cell_n(*)
+      integer(c_int64_t) :: cell_pr(*)
+    end function
   end interface
 
 contains

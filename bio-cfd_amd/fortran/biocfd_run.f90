@@ -3,8 +3,8 @@
 ! (module biocfd_hip) in the call order of the reference driver
 ! (app/main.F90:43-265), and writes record dumps of the block state.
 !
-!   biocfd_run CASE_FILE OUT_DIR [--time]
-!   biocfd_run --deck DECK_DIR OUT_DIR [--time]
+!   biocfd_run CASE_FILE OUT_DIR [--time] [--output] [--checkpoint-every N]
+!   biocfd_run --deck DECK_DIR OUT_DIR [--time] [--output] [--checkpoint-every N]
 !   biocfd_run --deck-to-case DECK_DIR CASE_FILE
 !
 ! Case file: the stream written by biocfd_amd.cases.write_case (magic
@@ -17,6 +17,15 @@
 ! (name(24), kind, ndim, shape, data) as read by biocfd_amd.dumps.read_dump.
 ! --time: no dumps; writes OUT_DIR/timing.txt ("steps N seconds S") for the
 ! time-step loop, set-up excluded.
+! --output: the reference driver's own output files under OUT_DIR/out (the
+! directories OUT_DIR/out and OUT_DIR/out/Chkpt must exist, as main.F90
+! needs out/ and out/Chkpt/): the field files after the set-up and at steps 1,
+! 2 and every 200th, the checkpoint every 500th step (or every N with
+! --checkpoint-every N) and the body plot at step 1 (WriteOutput_corner1.F90).
+! A deck with istart = 1 restarts from the checkpoint of every block, read
+! from OUT_DIR/out/Chkpt/ under the name the checkpoint writer gives it, or
+! else under the name the reference's lastConditions reads
+! (out/aorta_chkpt.<block>.<re>.dat, LastConditions.f90:24-39).
 !
 ! The single-block-with-body configuration runs the IB routines on block 1
 ! together with the domain boundary conditions; a coarse block 1 without a
@@ -56,9 +65,9 @@ module biocfd_host
 
   ! the input_data namelist of a deck, as read (before unit scaling)
   type :: deck_params
-    integer(int64) :: nblocks, intflines, itamax, pcitamax, surgeopoints, inor
+    integer(int64) :: nblocks, intflines, itamax, pcitamax, surgeopoints, inor, istart
     real(real64) :: epsi, omega1, omega2, omega3, omega4, re, rho_f, mu_f, l_c, phase_angle, freq, aoa, piv_pt, &
-                    dt_order, dxmin
+                    dt_order, dxmin, alpha_m, theta_m
   end type
 
   integer(int64) :: nblocks, nintf, ib_block1, pc_ita_max, nsteps, inor, seed
@@ -68,6 +77,12 @@ module biocfd_host
   integer(int64), allocatable :: dump_steps(:)
   real(real64) :: totime, phase
   integer(int64) :: ita
+  ! output and restart (main.F90:100-127, 156-166; WriteOutput_corner1.F90,
+  ! LastConditions.f90)
+  logical :: do_output = .false.
+  character(len=3) :: char_f = 'amp'
+  character(len=512) :: out_root = '.'
+  integer(int64) :: istart = 0, ita1 = 0, chkpt_every = 500
 
 contains
 
@@ -163,6 +178,12 @@ contains
     character(len=32) :: grid_name
 
     call read_deck_namelist(dir // '/input_data.nml', prm)
+    ! the output files' tag (ReadInput.f90:129-138; with both amplitudes 0
+    ! the reference leaves it undefined, "amp" here)
+    char_f = 'amp'
+    if (prm%alpha_m /= 0 .and. prm%theta_m /= 0) char_f = 'bot'
+    if (prm%alpha_m /= 0 .and. prm%theta_m == 0) char_f = 'ang'
+    istart = prm%istart
     nblocks = prm%nblocks
     nintf = prm%intflines
     nsteps = prm%itamax
@@ -380,12 +401,9 @@ contains
     open(newunit=u, file=path, status='old', action='read')
     read(u, nml=input_data)
     close(u)
-    if (istart /= 0) then
-      write(*, '(a)') 'biocfd_run: istart = 1 (restart from a checkpoint) is not supported'
-      error stop 3
-    end if
-    prm = deck_params(nblocks, intflines, itamax, pcItaMax, surGeoPoints, inor, epsi, omega1, omega2, omega3, &
-                      omega4, re, rho_f, mu_f, l_c, phase_angle, freq, aoa, piv_pt, dt_order, dxmin)
+    prm = deck_params(nblocks, intflines, itamax, pcItaMax, surGeoPoints, inor, int(istart, int64), epsi, omega1, &
+                      omega2, omega3, omega4, re, rho_f, mu_f, l_c, phase_angle, freq, aoa, piv_pt, dt_order, dxmin, &
+                      alpha_m, theta_m)
   end subroutine
 
   ! faces 2 .. n+2 of one axis from a grid file (mm), the end faces and the
@@ -578,8 +596,15 @@ contains
       call biocfd_check(biocfd_fine_block_cell(1_c_int), 'fine_block_cell')
       call biocfd_check(biocfd_cell_count_solid_coarse(1_c_int), 'cell_count_solid_coarse')
     end if
+    totime = 0
+    ita = 0
+    ita1 = 0
     do g = 1, int(nblocks)
-      call initial_fields(g)
+      if (istart == 1) then
+        call restart_fields(g)
+      else
+        call initial_fields(g)
+      end if
     end do
     do g = 1, int(nblocks)
       if (blk(g)%has_ib == 1) then
@@ -592,8 +617,151 @@ contains
       call biocfd_check(biocfd_non_uni_coeff(g), 'non_uni_coeff')
     end do
     phase = phase_deg * acos(-1.0_real64) / 180.0_real64
-    totime = dt
-    ita = 0
+    totime = totime + dt
+    if (do_output) call write_outputs(.true.)
+  end subroutine
+
+  ! lastConditions (LastConditions.f90:10-49): u, v, w, p of block g and
+  ! totime, ita, ita1 from a checkpoint (the last block read sets the
+  ! scalars); ut = u, vt = v, wt = w; cell_pr keeps what fine_block_cell set
+  subroutine restart_fields(g)
+    integer, intent(in) :: g
+    type(host_block), pointer :: b
+    real(real64), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), p(:, :, :)
+    character(len=700) :: name_w, name_r
+    logical :: there
+    integer :: io
+    integer(int64) :: i, j, k
+    b => blk(g)
+    allocate(u(b%nx + 2, b%ny + 2, b%nz + 2), v(b%nx + 2, b%ny + 2, b%nz + 2), w(b%nx + 2, b%ny + 2, b%nz + 2), &
+             p(b%nx + 2, b%ny + 2, b%nz + 2))
+    ! the checkpoint writer's name, else the name lastConditions reads
+    write(name_w, '(a,"/out/Chkpt/",a3,"_butter_chkpt.",i3.3,".",f6.1,".",f8.6,".dat")') trim(out_root), char_f, &
+      g, re, b%dx
+    write(name_r, '(a,"/out/aorta_chkpt.",i3.3,".",f6.1,".dat")') trim(out_root), g, re
+    inquire(file=trim(name_w), exist=there)
+    if (.not. there) then
+      inquire(file=trim(name_r), exist=there)
+      if (.not. there) then
+        write(*, '(a)') 'biocfd_run: no checkpoint for block: '//trim(name_w)//' or '//trim(name_r)
+        error stop 3
+      end if
+      name_w = name_r
+    end if
+    open(newunit=io, file=trim(name_w), status='old', action='read')
+    do k = 1, b%nz + 2
+      do j = 1, b%ny + 2
+        do i = 1, b%nx + 2
+          read(io, *) u(i, j, k), v(i, j, k), w(i, j, k), p(i, j, k), totime, ita, ita1
+        end do
+      end do
+    end do
+    close(io)
+    call biocfd_check(biocfd_field_upload(g, BF_U, u), 'field_upload u')
+    call biocfd_check(biocfd_field_upload(g, BF_UT, u), 'field_upload ut')
+    call biocfd_check(biocfd_field_upload(g, BF_V, v), 'field_upload v')
+    call biocfd_check(biocfd_field_upload(g, BF_VT, v), 'field_upload vt')
+    call biocfd_check(biocfd_field_upload(g, BF_W, w), 'field_upload w')
+    call biocfd_check(biocfd_field_upload(g, BF_WT, w), 'field_upload wt')
+    call biocfd_check(biocfd_field_upload(g, BF_P, p), 'field_upload p')
+  end subroutine
+
+  ! the reference driver's outputs at this point of the step (main.F90:122-127
+  ! after the set-up, :156-166 after pressureForcing1): field files at steps
+  ! 0, 1, 2 and every 200th, the checkpoint every chkpt_every steps, the body
+  ! plot at step 1 (WriteOutput_corner1.F90:80-162)
+  subroutine write_outputs(at_setup)
+    logical, intent(in) :: at_setup
+    integer :: g
+    do g = 1, int(nblocks)
+      call write_fields(g)
+      if (at_setup) cycle
+      call write_checkpoint(g)
+      call write_body(g)
+    end do
+  end subroutine
+
+  subroutine write_fields(g)
+    integer, intent(in) :: g
+    type(host_block), pointer :: b
+    real(real64), allocatable :: u1(:), v1(:), w1(:), p(:)
+    integer(int64), allocatable :: c(:), cn(:), cp(:)
+    character(len=700) :: fname
+    integer :: io
+    integer(int64) :: i, j, k, q, n
+    if (.not. (mod(ita, 200_int64) == 0 .or. ita <= 2)) return
+    b => blk(g)
+    n = b%nx * b%ny * b%nz
+    allocate(u1(n), v1(n), w1(n), p(n), c(n), cn(n), cp(n))
+    call biocfd_check(biocfd_output_fields(g, u1, v1, w1, p, c, cn, cp), 'output_fields')
+    call biocfd_check(biocfd_grid_download(g, BG_XP, b%xp, b%nx + 2), 'grid_download')
+    call biocfd_check(biocfd_grid_download(g, BG_YP, b%yp, b%ny + 2), 'grid_download')
+    call biocfd_check(biocfd_grid_download(g, BG_ZP, b%zp, b%nz + 2), 'grid_download')
+    write(fname, '(a,"/out/",a3,"_butter_fielddata.",i9.9,".",i3.3,".",f7.1,".",f8.6,".",i3.3,".dat")') &
+      trim(out_root), char_f, ita, g, re, b%dx, nblocks
+    open(newunit=io, file=trim(fname), status='replace')
+    write(io, *) 'variables="x","y","z","u","v","w","p","totime","cellid","cell_n","cell_pr"'
+    write(io, *) "zone, ", "i = ", b%nx, " j = ", b%ny, " k = ", b%nz
+    q = 0
+    do k = 2, b%nz + 1
+      do j = 2, b%ny + 1
+        do i = 2, b%nx + 1
+          q = q + 1
+          write(io, *) b%xp(i), b%yp(j), b%zp(k), u1(q), v1(q), w1(q), p(q), totime, c(q), cn(q), cp(q)
+        end do
+      end do
+    end do
+    close(io)
+  end subroutine
+
+  subroutine write_checkpoint(g)
+    integer, intent(in) :: g
+    type(host_block), pointer :: b
+    real(real64), allocatable :: u(:, :, :), v(:, :, :), w(:, :, :), p(:, :, :)
+    character(len=700) :: fname
+    integer :: io
+    integer(int64) :: i, j, k
+    if (mod(ita, chkpt_every) /= 0) return
+    b => blk(g)
+    allocate(u(b%nx + 2, b%ny + 2, b%nz + 2), v(b%nx + 2, b%ny + 2, b%nz + 2), w(b%nx + 2, b%ny + 2, b%nz + 2), &
+             p(b%nx + 2, b%ny + 2, b%nz + 2))
+    call biocfd_check(biocfd_field_download(g, BF_U, u), 'field_download')
+    call biocfd_check(biocfd_field_download(g, BF_V, v), 'field_download')
+    call biocfd_check(biocfd_field_download(g, BF_W, w), 'field_download')
+    call biocfd_check(biocfd_field_download(g, BF_P, p), 'field_download')
+    write(fname, '(a,"/out/Chkpt/",a3,"_butter_chkpt.",i3.3,".",f6.1,".",f8.6,".dat")') trim(out_root), char_f, &
+      g, re, b%dx
+    open(newunit=io, file=trim(fname), status='replace')
+    do k = 1, b%nz + 2
+      do j = 1, b%ny + 2
+        do i = 1, b%nx + 2
+          write(io, *) u(i, j, k), v(i, j, k), w(i, j, k), p(i, j, k), totime, ita, ita1
+        end do
+      end do
+    end do
+    close(io)
+  end subroutine
+
+  subroutine write_body(g)
+    integer, intent(in) :: g
+    type(host_block), pointer :: b
+    character(len=700) :: fname
+    integer :: io
+    integer(int64) :: n
+    if (ita /= 1) return
+    b => blk(g)
+    write(fname, '(a,"/out/butterfly_",i3.3,".dat")') trim(out_root), g
+    open(newunit=io, file=trim(fname), status='replace')
+    write(io, *) 'TITLE = "FEstressplot"'
+    write(io, *) 'VARIABLES= "x", "y", "z","bd_n"'
+    write(io, *) "ZONE NODES= ", b%nn, ",ELEMENTS=", b%ne, ",DATAPACKING=POINT, ZONETYPE=FETRIANGLE"
+    do n = 1, b%nn
+      write(io, *) b%xnode1(n), b%ynode1(n), b%znode1(n), 99
+    end do
+    do n = 1, b%ne
+      write(io, *) b%el1(n), b%el2(n), b%el3(n)
+    end do
+    close(io)
   end subroutine
 
   subroutine upload_grid(g, which, a)
@@ -639,6 +807,7 @@ contains
       kv = kin_vec(b)
       call biocfd_check(biocfd_pressure_forcing1(g, kv), 'pressure_forcing1')
     end do
+    if (do_output) call write_outputs(.false.)
     do g = 1, int(nblocks)
       b => blk(g)
       if (b%has_ib /= 1) cycle
@@ -932,12 +1101,12 @@ program biocfd_run
   implicit none
   character(len=512) :: case_path, out_dir, arg
   logical :: do_time
-  integer(int64) :: s, t0, t1, rate
-  integer :: u, na
+  integer(int64) :: t0, t1, rate, ita0
+  integer :: u, na, q
 
   if (command_argument_count() < 2) then
-    write(*, '(a)') 'usage: biocfd_run CASE_FILE OUT_DIR [--time]'
-    write(*, '(a)') '       biocfd_run --deck DECK_DIR OUT_DIR [--time]'
+    write(*, '(a)') 'usage: biocfd_run CASE_FILE OUT_DIR [--time] [--output] [--checkpoint-every N]'
+    write(*, '(a)') '       biocfd_run --deck DECK_DIR OUT_DIR [--time] [--output] [--checkpoint-every N]'
     write(*, '(a)') '       biocfd_run --deck-to-case DECK_DIR CASE_FILE'
     error stop 2
   end if
@@ -960,23 +1129,42 @@ program biocfd_run
     call read_case(trim(case_path))
   end if
   do_time = .false.
-  if (command_argument_count() > na) then
-    call get_command_argument(na + 1, arg)
-    do_time = trim(arg) == '--time'
-  end if
+  out_root = out_dir
+  q = na + 1
+  do while (q <= command_argument_count())
+    call get_command_argument(q, arg)
+    select case (trim(arg))
+    case ('--time')
+      do_time = .true.
+    case ('--output')
+      do_output = .true.
+    case ('--checkpoint-every')
+      q = q + 1
+      call get_command_argument(q, arg)
+      read(arg, *) chkpt_every
+    case default
+      write(*, '(a)') 'biocfd_run: unknown option '//trim(arg)
+      error stop 2
+    end select
+    q = q + 1
+  end do
   call setup()
   if (.not. do_time .and. any(dump_steps == 0)) call dump_all(trim(out_dir), 0_int64)
   call biocfd_check(biocfd_synchronize(), 'synchronize')
   call system_clock(t0, rate)
-  do s = 1, nsteps
+  ! the step loop of main.F90:128-265: at least one step, up to itamax
+  ! (a restart continues from the checkpoint's step)
+  ita0 = ita
+  do
     call time_step()
-    if (.not. do_time .and. any(dump_steps == s)) call dump_all(trim(out_dir), s)
+    if (.not. do_time .and. any(dump_steps == ita)) call dump_all(trim(out_dir), ita)
+    if (ita >= nsteps) exit
   end do
   call biocfd_check(biocfd_synchronize(), 'synchronize')
   call system_clock(t1)
   if (do_time) then
     open(newunit=u, file=trim(out_dir)//'/timing.txt', status='replace')
-    write(u, '(a,i0,a,es23.15)') 'steps ', nsteps, ' seconds ', real(t1 - t0, real64) / real(rate, real64)
+    write(u, '(a,i0,a,es23.15)') 'steps ', ita - ita0, ' seconds ', real(t1 - t0, real64) / real(rate, real64)
     close(u)
   end if
   call biocfd_check(biocfd_finalize(), 'finalize')

@@ -112,6 +112,7 @@ SIGNATURES = {
     "biocfd_fine_update_mv": [_I],
     "biocfd_tagging_move": [_I, _L],
     "biocfd_grid_download": [_I, _I, _P, _L],
+    "biocfd_output_fields": [_I, _P, _P, _P, _P, _P, _P, _P],
 }
 
 

@@ -253,6 +253,14 @@ int biocfd_tagging_move(int blk, int64_t move_check);
 /* a block's grid array (enum biocfd_grid) back to the host, n entries */
 int biocfd_grid_download(int blk, int which, double *host, int64_t n);
 
+/* ---- output sync (WriteOutput_corner1.F90:80-139) ---- */
+/* the state the field output writes, per interior cell (nx*ny*nz entries,
+ * i fastest): cell-centre velocities u1 = 0.5 (u(i) + u(i-1)), v1, w1, the
+ * pressure and the tags cell, cell_n, cell_pr (write_output_ascii :97-107),
+ * assembled on the device and copied to the host in one transfer */
+int biocfd_output_fields(int blk, double *u1, double *v1, double *w1, double *p, int64_t *cell,
+                         int64_t *cell_n, int64_t *cell_pr);
+
 #ifdef __cplusplus
 }
 #endif
