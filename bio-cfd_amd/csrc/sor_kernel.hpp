@@ -80,9 +80,13 @@ __global__ void __launch_bounds__(NT, MINW) k_sor_pair(FusedArgsT<Real> a, int g
   T::prologue(a, tile, lds, th, threadIdx.x);
   double delta = 0.0;
   const int s1 = T::last_step(tile);
-  for (int s = T::first_step(tile); s <= s1; s += 2) {
+  for (int s = T::first_step(tile); s <= s1; s += T::RB) {
     pair_step<T, 0>(a, tile, lds, th, s, check, delta);
     if (s + 1 <= s1) pair_step<T, 1>(a, tile, lds, th, s + 1, check, delta);
+    if constexpr (T::RB == 4) {
+      if (s + 2 <= s1) pair_step<T, 2 % T::RB>(a, tile, lds, th, s + 2, check, delta);
+      if (s + 3 <= s1) pair_step<T, 3 % T::RB>(a, tile, lds, th, s + 3, check, delta);
+    }
   }
   if (check) wg_max_store(delta, partials);
 }

@@ -22,10 +22,14 @@
 // numbering: a thread's pair slots past it drop out at compile time.
 //
 // All loads of step s+1 (the plane s+2 of the iterate with its Acz row, the
-// rhs of every phase) are issued in phase A of step s into registers that
-// alternate with the step's parity, and first used in step s+1: the only
+// rhs of its first two phases) are issued in phase A of step s into
+// registers indexed by the step number, and first used in step s+1: the only
 // wait for global memory is the one phase A makes for the loads a step
-// earlier.
+// earlier.  With two iterations per launch each cell's rhs is read once:
+// phase p + 2 of step s + 2 updates the cells phase p updated at step s
+// (same plane k = s - p, same colour, and, the pairs being numbered by
+// phase region, the same threads and slots), so it reuses the registers
+// loaded for phase p, which live four steps (buffers by step mod 4).
 //
 // Real = double is the reference's SOR (bit-identical: the operand order of
 // sor_cell, Acx(c) + Acy(c) + Acz(c) summed left to right); Real = float the
@@ -97,6 +101,7 @@ struct SorPairTile {
   static constexpr int kW0 = 2 * NPH, kW1 = 2 * NPH + 1, kYOdd = 2 * NPH + 2;
   static_assert(kYOdd < 16 && SW < 256 && SH < 256, "pair mask layout");
   static constexpr int NQC = CREG ? NQ : 1;         // pairs with coefficient registers
+  static constexpr int RB = NIT == 2 ? 4 : 2;       // rhs register buffers (steps)
 
   // the pairs with a cell in phase p's region form the rectangle of slot
   // rows [ry0, ry1] and pairs [rh0, rh1]; numbered innermost (phase NPH-1,
@@ -156,7 +161,7 @@ struct SorPairTile {
     int roff[NQ];            // rhs in-plane offset of column 2h (column 2h+1: + 1)
     int goff[NQ];            // iterate in-plane offset of column 2h
     Real cxm[NQC][2], cxp[NQC][2], sxy[NQC][2], cym[NQC], cyp[NQC];
-    Real rc[2][NPH][NQ];     // rhs of each phase's cells, by step parity
+    Real rc[RB][2][NQ];      // rhs of the cells of phases 0 and 1, by step mod RB
   };
 
   __host__ __device__ static int slot_index(int k) { return (k + 2 * R) % R; }
@@ -263,10 +268,11 @@ struct SorPairTile {
     if (th.czm >= 0) lds[CZ + 3 * slot_index(p) + th.czm] = Real(th.czn);
   }
 
-  // the rhs of every phase active at step s, into buffer rb
+  // the rhs of phases 0 and 1 active at step s, into buffer rb (phases 2
+  // and 3 of step s + 2 reuse them)
   __host__ __device__ static void load_rhs(const Args &a, const Tile &T, Thread &th, int s, Real (*rb)[NQ]) {
     const int64_t plane = (int64_t)a.NXh * a.d.NY;
-    for (int p = 0; p < NPH; ++p) {
+    for (int p = 0; p < 2; ++p) {
       if (!active(T, p, s)) continue;
       const int k = s - p, colour = (p & 1) ? 0 : 1;
       const bool kin = k >= 2 && k <= a.d.nz + 1;
@@ -310,12 +316,12 @@ struct SorPairTile {
   }
 
   // phase A of step s (rhs buffer B holds step s's): store plane s+1, issue
-  // the loads of plane s+2 and of step s+1's rhs into buffer 1-B
+  // the loads of plane s+2 and of step s+1's rhs into buffer B+1
   template <int B>
   __host__ __device__ static void phase_a(const Args &a, const Tile &T, Real *lds, Thread &th, int s) {
     store_plane(a, lds, th, s + 1);
     if (s + 2 <= last_step(T) + 1) load_plane(a, th, s + 2);
-    if (s + 1 <= last_step(T)) load_rhs(a, T, th, s + 1, th.rc[1 - B]);
+    if (s + 1 <= last_step(T)) load_rhs(a, T, th, s + 1, th.rc[(B + 1) % RB]);
   }
 
   // phase p of step s: update its colour on plane k = s - p (in LDS; the last
@@ -346,7 +352,8 @@ struct SorPairTile {
       const Real old = lds[oO + c];
       const Real pw = N[par - 1], pe = N[par], ps = N[-SW], pn = N[SW];
       const Real pb = lds[oNm + c], pt = lds[oNp + c];
-      const Real rhs = th.rc[B][p][q];
+      // phases 2, 3: the rhs phase p - 2 loaded two steps earlier
+      const Real rhs = p < 2 ? th.rc[B][p < 2 ? p : 0][q] : th.rc[(B + RB - 2) % RB][p >= 2 ? p - 2 : 0][q];
       Real axm, axp, aym, ayp, sxy;
       if constexpr (CREG) {
         axm = par ? th.cxm[q][1] : th.cxm[q][0];

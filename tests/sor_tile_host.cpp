@@ -216,17 +216,25 @@ static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real
     }
     const int s0 = T::first_step(tile), s1 = T::last_step(tile);
     for (int s = s0; s <= s1; ++s) {
-      const bool odd = ((s - s0) & 1) != 0;
+      const int b = (s - s0) % T::RB;  // the kernel's step unrolling
       for (int tid = 0; tid < NT; ++tid) {
-        if (odd)
-          T::template phase_a<1>(a, tile, lds.data(), th[tid], s);
-        else
-          T::template phase_a<0>(a, tile, lds.data(), th[tid], s);
+        switch (b) {
+          case 0: T::template phase_a<0>(a, tile, lds.data(), th[tid], s); break;
+          case 1: T::template phase_a<1>(a, tile, lds.data(), th[tid], s); break;
+          case 2: T::template phase_a<2 % T::RB>(a, tile, lds.data(), th[tid], s); break;
+          default: T::template phase_a<3 % T::RB>(a, tile, lds.data(), th[tid], s); break;
+        }
       }
       for (int ph = 0; ph < T::NPH; ++ph)
         for (int tid = 0; tid < NT; ++tid) {
-          const double dd = odd ? T::template phase_u<1>(a, tile, lds.data(), th[tid], ph, s, ph >= T::NPH - 2)
-                                : T::template phase_u<0>(a, tile, lds.data(), th[tid], ph, s, ph >= T::NPH - 2);
+          const bool nm = ph >= T::NPH - 2;
+          double dd;
+          switch (b) {
+            case 0: dd = T::template phase_u<0>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+            case 1: dd = T::template phase_u<1>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+            case 2: dd = T::template phase_u<2 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+            default: dd = T::template phase_u<3 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          }
           delta = delta > dd ? delta : dd;
         }
     }
