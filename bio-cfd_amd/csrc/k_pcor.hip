@@ -389,10 +389,13 @@ int ensure_split(Block &b) {
 // tile shapes and prefetch depths below (a pass shortened by a check
 // iteration or by the budget runs the same tile with one iteration)
 using FusedLaunch = void (*)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, int, const SorState *, double *);
+using BatchBlock = BatchBlockT<double>;
+using BatchLaunch = void (*)(dim3, dim3, hipStream_t, const BatchBlock *, int, int, int, double *);
 struct FusedVariant {
   int ti, tj, nt, depth, minb, nit;
   FusedLaunch launch, launch1;
   int family;  // 0: SorTile (cell lists per phase), 1, 2: SorPairTile (pair-owned; coefficients in registers, LDS)
+  BatchLaunch batch, batch1;  // several blocks in one launch (pair-owned tiles only)
 };
 
 template <int TI, int TJ, int NT, int D, int MINB, int NIT>
@@ -409,18 +412,26 @@ static void launch_pair(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int
                      partials);
 }
 
+template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG>
+static void launch_pair_batch(dim3 g, dim3 blk, hipStream_t s, const BatchBlock *tab, int nblk, int parity, int check,
+                              double *partials) {
+  hipLaunchKernelGGL((k_sor_pair_batch<double, TI, TJ, NT, NIT, MINW, CREG>), g, blk, 0, s, tab, nblk, parity, check,
+                     partials);
+}
+
 // FV(tile_i, tile_j, threads, prefetch depth, waves per SIMD the registers
 // must allow, iterations per pass): SorTile; PV(tile_i, tile_j, threads,
 // waves per SIMD, iterations per pass, coefficients in registers):
 // SorPairTile (prefetch depth 1; family 1 with register coefficients, 2 with
 // LDS coefficients)
 #define FV(TI, TJ, NT, D, MINB, NIT) \
-  { TI, TJ, NT, D, MINB, NIT, launch_fused<TI, TJ, NT, D, MINB, NIT>, launch_fused<TI, TJ, NT, D, MINB, 1>, 0 }
+  { TI, TJ, NT, D, MINB, NIT, launch_fused<TI, TJ, NT, D, MINB, NIT>, launch_fused<TI, TJ, NT, D, MINB, 1>, 0, \
+    nullptr, nullptr }
 #define PV(TI, TJ, NT, MINW, NIT, CREG) \
   { TI, TJ, NT, 1, MINW, NIT, launch_pair<TI, TJ, NT, MINW, NIT, CREG>, launch_pair<TI, TJ, NT, MINW, 1, CREG>, \
-    CREG ? 1 : 2 }
+    CREG ? 1 : 2, launch_pair_batch<TI, TJ, NT, MINW, NIT, CREG>, launch_pair_batch<TI, TJ, NT, MINW, 1, CREG> }
 static const FusedVariant kFused[] = {
-    {0, 0, 0, 0, 0, 1, nullptr, nullptr, 0},
+    {0, 0, 0, 0, 0, 1, nullptr, nullptr, 0, nullptr, nullptr},
     FV(64, 16, 256, 1, 1, 1),
     FV(64, 8, 256, 1, 1, 1),
     FV(64, 8, 256, 2, 1, 1),
@@ -644,6 +655,135 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   return sor_finish(b, b.sor, variant);
 }
 
+// Batching: the SOR calls of several blocks (the fine blocks of
+// poissonSolver, PcorVcor.F90:86-129, which the reference runs one block per
+// OpenMP thread) in one launch per pass over a table of the blocks, for the
+// pair-owned kernels; blocks whose calls start at the same iteration share a
+// pass schedule.  Off: one block after the other.
+static int g_sor_batch = 1;
+static BatchBlock *g_batch_tab = nullptr;
+static int64_t g_batch_cap = 0;
+static double *g_batch_partials = nullptr;
+static int64_t g_batch_pcap = 0;
+
+// one SOR call on each of `blocks` (all at start iteration `start`)
+static int batched_calls(const std::vector<Block *> &blocks, long long start, int64_t pc_ita_max, double epsi) {
+  hipStream_t s = rt().stream;
+  const int variant = g_sor_variant, planes = g_sor_planes;
+  const FusedVariant &v = kFused[variant];
+  const int nb = (int)blocks.size();
+  std::vector<BatchBlock> tab((size_t)nb);
+  int64_t total_tiles = 0;
+  for (int i = 0; i < nb; ++i) {
+    Block &b = *blocks[i];
+    int gx, gy, gz;
+    fused_grid(b, v, planes, gx, gy, gz);
+    BatchBlock &e = tab[(size_t)i];
+    for (int par = 0; par < 2; ++par) {
+      FusedArgs &a = e.a[par];
+      a.d = b.d;
+      a.NXh = (b.d.NX + 1) / 2;
+      a.KS = planes;
+      a.zero_src = 0;
+      a.bred = b.bs[1];
+      a.bblack = b.bs[0];
+      a.Ax = b.A[0];
+      a.Ay = b.A[1];
+      a.Az = b.A[2];
+      a.omega = b.omega;
+      a.src = par ? b.pc2 : b.f[BF_PC];
+      a.dst = par ? b.f[BF_PC] : b.pc2;
+    }
+    e.gx = gx;
+    e.gy = gy;
+    e.ntiles = gx * gy * gz;
+    e.first = (int)total_tiles;
+    e.st = b.sor;
+    total_tiles += e.ntiles;
+  }
+  if (grow(&g_batch_tab, &g_batch_cap, nb) || grow(&g_batch_partials, &g_batch_pcap, total_tiles)) return -1;
+  BCFD_HIP(hipMemcpyAsync(g_batch_tab, tab.data(), sizeof(BatchBlock) * nb, hipMemcpyHostToDevice, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  const long long total = pc_ita_max + 1 - start < 1 ? 1 : pc_ita_max + 1 - start;
+  SorState *h = (SorState *)rt().h_pinned;
+  long long done_iters = 0, q = 0, since_sync = 0;
+  while (done_iters < total) {
+    const long long it = start + done_iters + 1;
+    const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
+    long long L = v.nit;
+    if (L > total - done_iters) L = total - done_iters;
+    if (L > to_check) L = to_check;
+    const long long last_it = it + L - 1;
+    const int check = last_it % 5 == 0;
+    (L == v.nit ? v.batch : v.batch1)(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1),
+                                      check, g_batch_partials);
+    done_iters += L;
+    since_sync += L;
+    ++q;
+    if (check || done_iters == total)
+      for (int i = 0; i < nb; ++i)
+        hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, blocks[i]->sor, last_it, start, q, check,
+                           (long long)pc_ita_max, epsi, g_batch_partials + tab[(size_t)i].first,
+                           (int64_t)tab[(size_t)i].ntiles);
+    BCFD_CHECK_LAUNCH();
+    if (since_sync >= 50 && done_iters < total) {
+      // every block stopped: no more passes (PcorVcor.F90:365-375)
+      since_sync = 0;
+      bool all = true;
+      for (int i = 0; i < nb && all; ++i) {
+        BCFD_HIP(hipMemcpyAsync(h, blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+        BCFD_HIP(hipStreamSynchronize(s));
+        all = h->done != 0;
+      }
+      if (all) break;
+    }
+  }
+  for (Block *b : blocks)
+    if (sor_finish(*b, b->sor, variant)) return -1;
+  return 0;
+}
+
+// the SOR calls of `blocks` (REDBLACKSOR_linear on each), batched where the
+// kernel and the calls allow
+int redblack_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt, double epsi) {
+  const int variant = g_sor_variant;
+  const bool can = g_sor_batch && precision_mode() == 0 && !sor_timing_on() && kFused[variant].batch &&
+                   blocks.size() > 1;
+  if (!can) {
+    for (Block *b : blocks)
+      if (redblack_sor(*b, pc_ita_max, dt, epsi)) return -1;
+    return 0;
+  }
+  hipStream_t s = rt().stream;
+  // set-up of every call, then the start iteration of each (nIterPcor so far)
+  std::vector<long long> start(blocks.size());
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    Block &b = *blocks[i];
+    if (sor_prepare(b, variant, g_sor_planes, dt)) return -1;
+    hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
+    BCFD_CHECK_LAUNCH();
+  }
+  SorState *h = (SorState *)rt().h_pinned;
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    BCFD_HIP(hipMemcpyAsync(h, blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+    BCFD_HIP(hipStreamSynchronize(s));
+    start[i] = h->n_iter;
+  }
+  // blocks that start at the same iteration share a pass schedule
+  std::vector<bool> taken(blocks.size(), false);
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    if (taken[i]) continue;
+    std::vector<Block *> group;
+    for (size_t j = i; j < blocks.size(); ++j)
+      if (!taken[j] && start[j] == start[i]) {
+        group.push_back(blocks[j]);
+        taken[j] = true;
+      }
+    if (batched_calls(group, start[i], pc_ita_max, epsi)) return -1;
+  }
+  return 0;
+}
+
 // `iters` SOR iterations of the given variant on the block's current pc and
 // b, timed with HIP events; pc and the block's SOR state are left unchanged
 static SorState *g_probe_state = nullptr;
@@ -790,6 +930,12 @@ int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_ite
   return 0;
 }
 
+int biocfd_sor_batching(int on) {
+  BCFD_API_LOCK();
+  g_sor_batch = on ? 1 : 0;
+  return 0;
+}
+
 int biocfd_sor_config(int variant, int planes_per_workgroup) {
   BCFD_API_LOCK();
   if (variant < 0 || variant >= kNumSorVariants) {
@@ -874,16 +1020,17 @@ int biocfd_poisson_solver(int first, int last, int64_t pc_ita_max, double epsi, 
   }
   BCFD_CHECK_LAUNCH();
   if (interface_fine_update(1) || interface_coarse_update(1)) return -1;
+  std::vector<Block *> fine;
   for (int g = first; g <= last; ++g) {
     Block &b = rt().blk[g];
     if (compute_div(b)) return -1;
-    if (g != 1 && redblack_sor(b, pc_ita_max, dt, epsi)) return -1;
+    if (g != 1) fine.push_back(&b);
   }
+  if (redblack_sor_set(fine, pc_ita_max, dt, epsi)) return -1;
   if (interface_coarse_update(2)) return -1;
   if (first == 1 && redblack_sor(rt().blk[1], pc_ita_max, dt, epsi)) return -1;
   if (interface_fine_update(2)) return -1;
-  for (int g = first; g <= last; ++g)
-    if (g != 1 && redblack_sor(rt().blk[g], pc_ita_max, dt, epsi)) return -1;
+  if (redblack_sor_set(fine, pc_ita_max, dt, epsi)) return -1;
   if (interface_coarse_update(2)) return -1;
   for (int g = first; g <= last; ++g)
     if (correct(rt().blk[g], dt)) return -1;

@@ -63,18 +63,13 @@ __device__ inline void pair_step(const FusedArgsT<Real> &a, const typename T::Ti
   }
 }
 
-// NIT red-black iterations per launch with the pair-owned tile (SorPairTile);
-// same grid, arguments and check protocol as k_sor_fused.  MINW: waves per
-// SIMD the register allocation must leave room for (4 = two 512-thread
-// workgroups per CU)
-template <typename Real, int TI, int TJ, int NT, int NIT, int MINW, bool CREG>
-__global__ void __launch_bounds__(NT, MINW) k_sor_pair(FusedArgsT<Real> a, int gx, int gy, int check,
-                                                       const SorState *__restrict__ st,
-                                                       double *__restrict__ partials) {
-  using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
-  if (st->done) return;
+// the pair-owned tile march of one workgroup on tile `tile_index` (1-D,
+// gx*gy tiles per z chunk) of the block described by `a`
+template <class T, typename Real>
+__device__ inline void pair_run(const FusedArgsT<Real> &a, int tile_index, int gx, int gy, int check,
+                                double *__restrict__ partials) {
   __shared__ Real lds[T::LDS_DOUBLES];
-  const typename T::Tile tile = T::tile_of(a, T::xcd_tile(blockIdx.x, gridDim.x), gx, gy);
+  const typename T::Tile tile = T::tile_of(a, tile_index, gx, gy);
   typename T::Thread th;
   T::init_thread(a, tile, threadIdx.x, th);
   T::prologue(a, tile, lds, th, threadIdx.x);
@@ -89,6 +84,45 @@ __global__ void __launch_bounds__(NT, MINW) k_sor_pair(FusedArgsT<Real> a, int g
     }
   }
   if (check) wg_max_store(delta, partials);
+}
+
+// NIT red-black iterations per launch with the pair-owned tile (SorPairTile);
+// same grid, arguments and check protocol as k_sor_fused.  MINW: waves per
+// SIMD the register allocation must leave room for (4 = two 512-thread
+// workgroups per CU)
+template <typename Real, int TI, int TJ, int NT, int NIT, int MINW, bool CREG>
+__global__ void __launch_bounds__(NT, MINW) k_sor_pair(FusedArgsT<Real> a, int gx, int gy, int check,
+                                                       const SorState *__restrict__ st,
+                                                       double *__restrict__ partials) {
+  using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
+  if (st->done) return;
+  pair_run<T, Real>(a, T::xcd_tile(blockIdx.x, gridDim.x), gx, gy, check, partials);
+}
+
+// One block of a batched launch: its arguments for passes that read pc
+// (parity 0) and pc2 (parity 1), its tile grid, the first workgroup of the
+// launch that works on it, and its SOR state.
+template <typename Real>
+struct BatchBlockT {
+  FusedArgsT<Real> a[2];
+  int gx, gy, ntiles, first;
+  const SorState *st;
+};
+
+// The same iterations on every block of a set in one launch: workgroups
+// first .. first + ntiles - 1 work on block b of the table.  The blocks'
+// partials follow each other (workgroup index of the launch); a block whose
+// SOR call has stopped skips its tiles.
+template <typename Real, int TI, int TJ, int NT, int NIT, int MINW, bool CREG>
+__global__ void __launch_bounds__(NT, MINW) k_sor_pair_batch(const BatchBlockT<Real> *__restrict__ tab, int nblk,
+                                                             int parity, int check, double *__restrict__ partials) {
+  using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
+  int b = 0;
+  while (b + 1 < nblk && (int)blockIdx.x >= tab[b + 1].first) ++b;
+  if (tab[b].st->done) return;
+  const FusedArgsT<Real> a = tab[b].a[parity];
+  pair_run<T, Real>(a, T::xcd_tile((int)blockIdx.x - tab[b].first, tab[b].ntiles), tab[b].gx, tab[b].gy, check,
+                    partials);
 }
 
 }  // namespace bcfd

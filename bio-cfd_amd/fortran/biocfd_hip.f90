@@ -548,6 +548,11 @@ module biocfd_hip
       integer(c_size_t) :: c_strlen
     end function
 
+    integer(c_int) function biocfd_sor_batching(on) bind(C, name="biocfd_sor_batching")
+      import :: c_int
+      integer(c_int), value :: on
+    end function
+
     ! ---- block move (Search.F90:21-47, :415-432, :916-1461; FineInterp.f90:10-281)
     integer(c_int) function biocfd_find_dist_node(n_nodes, xnode, znode, node_id, markers) bind(C, &
         name="biocfd_find_dist_node")

@@ -596,6 +596,11 @@ contains
       call biocfd_check(biocfd_fine_block_cell(1_c_int), 'fine_block_cell')
       call biocfd_check(biocfd_cell_count_solid_coarse(1_c_int), 'cell_count_solid_coarse')
     end if
+    ! a fine block without a body: every interior cell fluid, the lists as
+    ! for the coarse block
+    do g = 2, int(nblocks)
+      if (blk(g)%has_ib /= 1) call biocfd_check(biocfd_cell_count_solid_coarse(g), 'cell_count_solid_coarse')
+    end do
     totime = 0
     ita = 0
     ita1 = 0

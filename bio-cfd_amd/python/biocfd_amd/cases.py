@@ -386,3 +386,35 @@ def config_nested_move(nc: int = 24, nf: int = 24, markers: int = 256, nsteps: i
     args = dict(dt=0.001, re=re_code(150.0, 1.0, 0.5), nsteps=nsteps, pc_ita_max=10, epsi=0.0, dxmin=1e-6)
     args.update(kw)
     return Case(f"move{nc}", [coarse, fine], [itf], **args)
+
+
+def config_batched(nc=(56, 32, 32), wake=(16, 20, 24), chord_cells: int = 8, span_cells: int = 16, nsteps: int = 3,
+                   **kw) -> Case:
+    """Uneven multi-block case: a coarse block on [0,14]x[0,8]x[0,8] and
+    refinement-ratio-2 fine blocks, the first ([2,4]x[2.5,5.5]x[2.5,5.5])
+    carrying a flat plate (chord 1 along x, span 2 along z, thickness 0.04;
+    ID 51) that heaves (0.5 at frequency 0.25) and pitches about x (30 deg,
+    phase 90 deg), then cubes of the given sides (fine cells) one after the
+    other along the wake, without bodies.  The reduced form of the batched
+    multi-block workload; the fine blocks' SOR calls run in one batched
+    launch per pass."""
+    nx, ny, nz = nc
+    coarse = BlockSpec((0.0, 0.0, 0.0), (14.0, 8.0, 8.0), (nx, ny, nz), omega=1.7)
+    hf = 14.0 / nx / 2          # fine spacing
+    plate = plate_body(chord_cells, span_cells, 1.0, 2.0, 0.04, surf_id=51)
+    body = BodyPlacement(plate, (3.0, 4.0, 4.0), a0_deg=30.0, yamp=0.5, bfreq=0.25, margin_cells=2)
+    lo, hi = (2.0, 2.5, 2.5), (4.0, 5.5, 5.5)
+    nb = [int(round((hi[d] - lo[d]) / hf)) for d in range(3)]
+    blocks = [coarse, BlockSpec(lo, hi, nb, omega=1.7, body=body, moving=True)]
+    itfs = [InterfaceSpec(1, 2, 1, 2, lo, hi)]
+    x = 4.5
+    for side in wake:
+        ext = side * hf
+        blo = (x, 4.0 - ext / 2, 4.0 - ext / 2)
+        bhi = (x + ext, 4.0 + ext / 2, 4.0 + ext / 2)
+        blocks.append(BlockSpec(blo, bhi, (side, side, side), omega=1.7))
+        itfs.append(InterfaceSpec(1, len(blocks), 1, 2, blo, bhi))
+        x += ext + 0.5
+    args = dict(dt=0.0005, re=200.0, nsteps=nsteps, pc_ita_max=20, epsi=0.0, freq=0.25, phase_angle_deg=90.0)
+    args.update(kw)
+    return Case("batched", blocks, itfs, **args)

@@ -119,6 +119,11 @@ class Simulation:
         if self.blocks[0].body is None:
             H.call("biocfd_fine_block_cell", 1)
             H.call("biocfd_cell_count_solid_coarse", 1)
+        for st in self.blocks[1:]:
+            if st.body is None:
+                # a fine block without a body: every interior cell fluid, the
+                # lists as for the coarse block
+                H.call("biocfd_cell_count_solid_coarse", st.g)
         for st in self.blocks:
             self._initial_fields(st)
         for st in self.blocks:

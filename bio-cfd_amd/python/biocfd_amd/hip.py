@@ -113,6 +113,7 @@ SIGNATURES = {
     "biocfd_tagging_move": [_I, _L],
     "biocfd_grid_download": [_I, _I, _P, _L],
     "biocfd_output_fields": [_I, _P, _P, _P, _P, _P, _P, _P],
+    "biocfd_sor_batching": [_I],
 }
 
 

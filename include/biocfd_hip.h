@@ -126,6 +126,13 @@ int biocfd_correct(int blk, double dt);
  * A fused variant may run two iterations per pass (temporal blocking); every
  * variant gives the same bits, the choice only changes speed. */
 int biocfd_sor_config(int variant, int planes_per_workgroup);
+/* Batching of the fine blocks' SOR calls in poisson_solver (no reference
+ * counterpart; the reference runs them one block per OpenMP thread,
+ * PcorVcor.F90:86-129): on (the default), the calls of all fine blocks that
+ * start at the same iteration advance together, one launch per pass over a
+ * table of the blocks (pair-owned kernels, FP64); off, one block after the
+ * other.  Both give the same bits. */
+int biocfd_sor_batching(int on);
 /* info = {tile_i, tile_j, threads per workgroup, z-plane prefetch depth,
  * workgroups per CU its register budget is compiled for, iterations per
  * pass, tile family (0: per-phase cell lists, 1: pair-owned cells)} of a

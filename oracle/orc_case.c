@@ -219,6 +219,13 @@ int orc_case_setup(orc_case *c)
         orc_cell_count_solid_coarse(&c->blk[0]);
         c->blk[0].tsCount = 0;
     }
+    /* a fine block without a body: every interior cell fluid, the lists
+     * built as for the coarse block */
+    for (int64_t g = 2; g <= nb; ++g)
+        if (!c->blk[g - 1].has_ib) {
+            orc_cell_count_solid_coarse(&c->blk[g - 1]);
+            c->blk[g - 1].tsCount = 0;
+        }
     for (int64_t g = 1; g <= nb; ++g) {
         initial_conditions(&c->blk[g - 1], c->uc);
         add_noise(c, g);

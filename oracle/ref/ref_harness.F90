@@ -114,6 +114,14 @@ program ref_harness
      block(1)%cell2 = 0
      block(1)%TSCellCount = 0
   end if
+  ! a fine block without a body (main.F90 gives every block g /= 1 one):
+  ! every interior cell fluid, the lists built as for the coarse block
+  do g = 2, nblocks
+     if (has_ib(g) == 0) then
+        call cellCount_solid_coarse(block(g))
+        block(g)%TSCellCount = 0
+     end if
+  end do
   do g = 1, nblocks
      call initialConditions(block(g), uc)
      call add_noise(g)

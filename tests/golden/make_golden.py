@@ -81,6 +81,36 @@ def array_digest(a) -> str:
     return h.hexdigest()
 
 
+BATCHED_DIGESTS = os.path.join(HERE, "batched_digests.json")
+BATCHED_STEPS = (0, 3)
+
+
+def batched_case():
+    """The uneven multi-block case (a coarse block, a fine block with a
+    heaving, pitching plate and three body-free fine blocks)."""
+    case = CS.config_batched(nsteps=max(BATCHED_STEPS))
+    case.dump_steps = BATCHED_STEPS
+    return case
+
+
+def make_batched_fixture():
+    case = batched_case()
+    wd = tempfile.mkdtemp(prefix="golden_batched_")
+    cpath = os.path.join(wd, "case.bin")
+    CS.write_case(case, cpath)
+    subprocess.run([REF_BIN, cpath, wd], check=True, capture_output=True, cwd=wd)
+    digests = {}
+    for s in BATCHED_STEPS:
+        for g in range(len(case.blocks)):
+            d = read_dump(os.path.join(wd, f"dump_s{s}_b{g + 1}.bin"))
+            for k in KEEP:
+                if k in d:
+                    digests[f"s{s}_b{g + 1}_{k}"] = array_digest(d[k])
+    with open(BATCHED_DIGESTS, "w") as f:
+        json.dump(digests, f, indent=1, sort_keys=True)
+    print("batched", len(digests), "digests")
+
+
 def make_deck_fixture():
     if os.path.isdir(DECK_DIR):
         shutil.rmtree(DECK_DIR)
@@ -124,6 +154,7 @@ def main():
         np.savez_compressed(fixture_path(name), **arrays)
         print(name, os.path.getsize(fixture_path(name)), "bytes")
     make_deck_fixture()
+    make_batched_fixture()
 
 
 if __name__ == "__main__":

@@ -63,8 +63,9 @@ def compare(ref, got, label="", field_rtol=FIELD_RTOL, skip=()):
 
 
 def small_cases():
-    """The parity cases: single sphere, channel, nested heaving sphere, flapping body, and a
-    nested case whose fine block moves every step."""
+    """The parity cases: single sphere, channel, nested heaving sphere, flapping body, a
+    nested case whose fine block moves every step, and five uneven blocks (a
+    pitching, heaving plate in one fine block, three body-free fine blocks)."""
     sphere = CS.config_sphere_64(n=24, markers=256, nsteps=3, pc_ita_max=20, epsi=0.0)
     channel = CS.config_channel(n=(16, 12, 10), nsteps=3)
     nested = CS.config_nested(nc=24, nf=24, markers=256, nsteps=3, pc_ita_max=15, epsi=0.0)
@@ -73,4 +74,5 @@ def small_cases():
     wing = CS.Case("wing", [blk], dt=0.002, re=300.0, nsteps=3, pc_ita_max=15, epsi=0.0, freq=0.5,
                    phase_angle_deg=90.0)
     move = CS.config_nested_move(nsteps=3)
-    return {"sphere": sphere, "channel": channel, "nested": nested, "wing": wing, "move": move}
+    batched = CS.config_batched(nsteps=3)
+    return {"sphere": sphere, "channel": channel, "nested": nested, "wing": wing, "move": move, "batched": batched}

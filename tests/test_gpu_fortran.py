@@ -18,7 +18,7 @@ RUN = os.path.join(ROOT, "bio-cfd_amd", "lib", "biocfd_run")
 STEPS = (0, 1, 3)
 
 
-@pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing", "move"])
+@pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing", "move", "batched"])
 def test_fortran_driver_parity(name):
     assert os.path.exists(RUN), "Fortran driver not built (bio-cfd_amd/build.py)"
     case = small_cases()[name]

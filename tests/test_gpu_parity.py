@@ -11,13 +11,18 @@ pytestmark = pytest.mark.gpu
 STEPS = (0, 1, 3)
 
 
-@pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing", "move"])
-def test_time_step_parity(name):
+@pytest.mark.parametrize("name,batching", [("channel", 1), ("sphere", 1), ("nested", 1), ("wing", 1), ("move", 1),
+                                          ("batched", 1), ("batched", 0)])
+def test_time_step_parity(name, batching):
+    """batching: the fine blocks' SOR calls in one launch per pass (1, the
+    default) or one block after the other (0)."""
+    from biocfd_amd import hip as H
     from biocfd_amd.driver import Simulation
 
     case = small_cases()[name]
     ref = run_oracle(case, STEPS)
     sim = Simulation(case)
+    H.call("biocfd_sor_batching", batching)
     try:
         bad = []
         for s in range(0, max(STEPS) + 1):
@@ -29,4 +34,5 @@ def test_time_step_parity(name):
                     bad += compare(r, o, label=f"{name} step {s} block {g + 1}: ")
         assert not bad, "\n".join(bad[:40])
     finally:
+        H.call("biocfd_sor_batching", 1)
         sim.close()
