@@ -9,6 +9,7 @@ pressure-correction projection, IB retagging / list rebuild / forcing), with
 all fields resident in HBM.
 
     python bench.py [--steps K] [--warmup W] [--precision both|fp64|mixed]
+    python bench.py --workload uneven [--steps K] [--warmup W]
 
 Prints one JSON line.  `value` is the FP64 method's Mcell-steps/s (one GPU);
 `mixed` holds the mixed-precision run of the same steps on the same inputs
@@ -18,6 +19,13 @@ budget), its speed-up and its per-field error against the FP64 run.
 measured with HIP events on the runtime's stream; `cpu_baseline` times the
 reference's own CPU build (oracle/_ref/ref_harness) on one step of the same
 workload.
+
+--workload uneven: the batched multi-block workload instead (a coarse block
+192x128x128 and 16 fine blocks of uneven sizes, one with a heaving, pitching
+plate; cases.config_uneven), FP64, run twice on the same inputs: the fine
+blocks' SOR calls batched in one launch per pass, then one block after the
+other; one JSON line with both throughputs, the speed-up and whether the two
+runs agree bit for bit.
 """
 from __future__ import annotations
 
@@ -142,6 +150,45 @@ def run_path(args, precision: str, keep_fields: bool):
     return out
 
 
+def run_uneven(args, batching: int):
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = CS.config_uneven(nsteps=args.warmup + args.steps)
+    sim = Simulation(case)
+    H.call("biocfd_sor_batching", batching)
+    H.call("biocfd_sor_config", 21, args.sor_planes)
+    for _ in range(args.warmup):
+        sim.step()
+    H.call("biocfd_synchronize")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.step()
+    H.call("biocfd_synchronize")
+    t1 = time.perf_counter()
+    fields = [sim.field(g, "p") for g in range(1, len(case.blocks) + 1)]
+    H.call("biocfd_sor_batching", 1)
+    sim.close()
+    cells = case.cells()
+    return {"ms_per_step": round(1e3 * (t1 - t0) / args.steps, 3),
+            "mcell_steps_per_s": round(cells * args.steps / (t1 - t0) / 1e6, 3)}, fields, case
+
+
+def uneven_main(args):
+    b, fb, case = run_uneven(args, 1)
+    p, fp, _ = run_uneven(args, 0)
+    same = all(np.array_equal(x, y) for x, y in zip(fb, fp))
+    sizes = [int(blk.n[0]) * int(blk.n[1]) * int(blk.n[2]) for blk in case.blocks]
+    print(json.dumps({
+        "workload": "uneven17: coarse 192x128x128 on [0,12]x[0,8]x[0,8] + 16 fine blocks (ratio 2, sides "
+                    + ",".join(str(int(blk.n[0])) for blk in case.blocks[1:])
+                    + "), heaving/pitching plate, Re=200, dt=5e-4, pcItaMax=500, epsi=1e-8, FP64",
+        "cells": case.cells(), "fine_cells": sum(sizes[1:]), "steps": args.steps, "warmup": args.warmup,
+        "batched": b, "per_block": p, "speedup": round(b["mcell_steps_per_s"] / p["mcell_steps_per_s"], 3),
+        "bit_identical": bool(same)}))
+
+
 def field_errors(ref, got):
     """max and L2 relative differences per field, and of the four body-force
     coefficients (viscous/pressure drag and lift)."""
@@ -215,11 +262,16 @@ def main():
                     help="first run a plain device copy of this many MiB (byte-counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-timeout", type=float, default=600.0)
+    ap.add_argument("--workload", choices=["sphere", "uneven"], default="sphere",
+                    help="sphere: the 256^3 headline workload; uneven: batched vs per-block multi-block run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the block-sharded multi-GPU "
               "path is a design note, not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "uneven":
+        uneven_main(args)
+        return
 
     both = args.precision == "both"
     g = run_path(args, "fp64", keep_fields=both) if args.precision in ("both", "fp64") else None

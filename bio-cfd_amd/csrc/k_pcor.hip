@@ -174,6 +174,39 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
   if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
 }
 
+// k_sor_fused_end for every block of a batched launch: workgroup b reduces
+// block b's partials (workgroups first .. first + ntiles - 1 of the launch)
+__global__ void __launch_bounds__(1024) k_sor_batch_end(const BatchBlockT<double> *__restrict__ tab,
+                                                        long long it, long long start, long long passes, int check,
+                                                        long long pc_ita_max, double epsi,
+                                                        const double *__restrict__ partials) {
+  const BatchBlockT<double> &e = tab[blockIdx.x];
+  SorState *st = e.st;
+  if (st->done) return;
+  __shared__ double s_m[16];
+  if (check) {
+    double m = 0.0;
+    for (int64_t q = threadIdx.x; q < e.ntiles; q += blockDim.x) {
+      const double v = partials[e.first + q];
+      m = m > v ? m : v;
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (check) {
+    double m = 0.0;
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
+    st->derr4 = m;
+  }
+  st->n_iter = it;
+  st->call_iters = it - start;
+  st->passes = (int)passes;
+  st->derr2 = st->derr4;
+  if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
+}
+
 // the passes of one SOR call alternate pc -> pc2 -> pc ...; after an odd
 // count the result is in pc2
 __global__ void k_sor_finish(const SorState *__restrict__ st, double *__restrict__ pc,
@@ -625,11 +658,42 @@ int sor_timing_end(hipEvent_t e0, const Block &b, long long start) {
   return 0;
 }
 
+// the device's compute units (workgroups one launch needs to fill it)
+static int device_cus() {
+  static int cus = 0;
+  if (!cus) {
+    int v = 0;
+    cus = hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, rt().device) == hipSuccess && v > 0 ? v
+                                                                                                           : 256;
+  }
+  return cus;
+}
+
+// planes per workgroup for a set of blocks: the configured count, halved
+// (down to 8) while their tiles would leave compute units without a
+// workgroup; the choice changes the speed, never the result
+static int planes_for(const std::vector<Block *> &blocks, int variant, int planes) {
+  if (variant <= 0) return planes;
+  const FusedVariant &v = kFused[variant];
+  int p = planes;
+  while (p > 8) {
+    int64_t tiles = 0;
+    for (const Block *b : blocks) {
+      int gx, gy, gz;
+      fused_grid(*b, v, p, gx, gy, gz);
+      tiles += (int64_t)gx * gy * gz;
+    }
+    if (tiles >= device_cus()) break;
+    p /= 2;
+  }
+  return p;
+}
+
 int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   if (precision_mode() == 1) return mixed_sor(b, pc_ita_max, dt);
   hipStream_t s = rt().stream;
   hipEvent_t e0 = nullptr;
-  const int variant = g_sor_variant, planes = g_sor_planes;
+  const int variant = g_sor_variant, planes = planes_for({&b}, variant, g_sor_planes);
   if (sor_prepare(b, variant, planes, dt)) return -1;
   hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
   BCFD_CHECK_LAUNCH();
@@ -669,7 +733,7 @@ static int64_t g_batch_pcap = 0;
 // one SOR call on each of `blocks` (all at start iteration `start`)
 static int batched_calls(const std::vector<Block *> &blocks, long long start, int64_t pc_ita_max, double epsi) {
   hipStream_t s = rt().stream;
-  const int variant = g_sor_variant, planes = g_sor_planes;
+  const int variant = g_sor_variant, planes = planes_for(blocks, variant, g_sor_planes);
   const FusedVariant &v = kFused[variant];
   const int nb = (int)blocks.size();
   std::vector<BatchBlock> tab((size_t)nb);
@@ -721,10 +785,8 @@ static int batched_calls(const std::vector<Block *> &blocks, long long start, in
     since_sync += L;
     ++q;
     if (check || done_iters == total)
-      for (int i = 0; i < nb; ++i)
-        hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, blocks[i]->sor, last_it, start, q, check,
-                           (long long)pc_ita_max, epsi, g_batch_partials + tab[(size_t)i].first,
-                           (int64_t)tab[(size_t)i].ntiles);
+      hipLaunchKernelGGL(k_sor_batch_end, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab, last_it, start, q,
+                         check, (long long)pc_ita_max, epsi, g_batch_partials);
     BCFD_CHECK_LAUNCH();
     if (since_sync >= 50 && done_iters < total) {
       // every block stopped: no more passes (PcorVcor.F90:365-375)
@@ -759,7 +821,7 @@ int redblack_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, dou
   std::vector<long long> start(blocks.size());
   for (size_t i = 0; i < blocks.size(); ++i) {
     Block &b = *blocks[i];
-    if (sor_prepare(b, variant, g_sor_planes, dt)) return -1;
+    if (sor_prepare(b, variant, planes_for(blocks, variant, g_sor_planes), dt)) return -1;
     hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
     BCFD_CHECK_LAUNCH();
   }

@@ -106,7 +106,7 @@ template <typename Real>
 struct BatchBlockT {
   FusedArgsT<Real> a[2];
   int gx, gy, ntiles, first;
-  const SorState *st;
+  SorState *st;
 };
 
 // The same iterations on every block of a set in one launch: workgroups

@@ -418,3 +418,57 @@ def config_batched(nc=(56, 32, 32), wake=(16, 20, 24), chord_cells: int = 8, spa
     args = dict(dt=0.0005, re=200.0, nsteps=nsteps, pc_ita_max=20, epsi=0.0, freq=0.25, phase_angle_deg=90.0)
     args.update(kw)
     return Case("batched", blocks, itfs, **args)
+
+
+def config_uneven(n_wake: int = 15, seed: int = 4, sides=(32, 48, 64, 80), markers_chord: int = 32,
+                  markers_span: int = 32, nsteps: int = 5, **kw) -> Case:
+    """The batched multi-block workload: a coarse block 192x128x128 on
+    [0,12]x[0,8]x[0,8] (h = 1/16) and 16 refinement-ratio-2 fine blocks:
+    one of 80^3 fine cells on [1,3.5]x[2.75,5.25]^2 carrying a flat plate
+    (chord 1, span 2, thickness 0.04, ID 51, 2 x chord x span quads per
+    face plus edge strips) that heaves (0.5 at 0.25) and pitches about x
+    (30 deg, phase 90 deg), and n_wake body-free cubes behind it whose sides
+    (fine cells) are drawn with the seed from `sides` and packed in slabs
+    along x (shelves in y and z, gaps of 1/4), so every block fits inside
+    the coarse block."""
+    rng = np.random.default_rng(seed)
+    coarse = BlockSpec((0.0, 0.0, 0.0), (12.0, 8.0, 8.0), (192, 128, 128), omega=1.7)
+    hf = 1.0 / 32
+    plate = plate_body(markers_chord, markers_span, 1.0, 2.0, 0.04, surf_id=51)
+    body = BodyPlacement(plate, (2.25, 4.0, 4.0), a0_deg=30.0, yamp=0.5, bfreq=0.25, margin_cells=3)
+    lo, hi = (1.0, 2.75, 2.75), (3.5, 5.25, 5.25)
+    blocks = [coarse, BlockSpec(lo, hi, (80, 80, 80), omega=1.7, body=body, moving=True)]
+    itfs = [InterfaceSpec(1, 2, 1, 2, lo, hi)]
+    want = sorted((int(rng.choice(sides)) for _ in range(n_wake)), reverse=True)
+    gap, x0, x_end, w_lo, w_hi = 0.25, 3.75, 11.75, 0.25, 7.75
+    placed = []
+    x = x0
+    while want:
+        slab = want[0] * hf                      # slab thickness: its largest block
+        if x + slab > x_end:
+            raise ValueError("the wake blocks do not fit the coarse block")
+        y, z, row = w_lo, w_lo, 0.0
+        rest = []
+        for side in want:
+            e = side * hf
+            if e > slab:
+                rest.append(side)
+                continue
+            if y + e > w_hi:                      # next shelf in z
+                y, z, row = w_lo, z + row + gap, 0.0
+            if z + e > w_hi:
+                rest.append(side)
+                continue
+            placed.append(((x, y, z), side))
+            y += e + gap
+            row = max(row, e)
+        want = rest
+        x += slab + gap
+    for (bx, by, bz), side in placed:
+        e = side * hf
+        blo, bhi = (bx, by, bz), (bx + e, by + e, bz + e)
+        blocks.append(BlockSpec(blo, bhi, (side, side, side), omega=1.7))
+        itfs.append(InterfaceSpec(1, len(blocks), 1, 2, blo, bhi))
+    args = dict(dt=0.0005, re=200.0, nsteps=nsteps, pc_ita_max=500, epsi=1e-8, freq=0.25, phase_angle_deg=90.0)
+    args.update(kw)
+    return Case("uneven17", blocks, itfs, **args)
