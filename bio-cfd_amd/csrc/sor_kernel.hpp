@@ -49,15 +49,16 @@ __global__ void __launch_bounds__(NT, MINB) k_sor_fused(FusedArgsT<Real> a, int 
   if (check) wg_max_store(delta, partials);
 }
 
-// one step of the pair-owned tile march; B: the step's rhs buffer
+// one step of the pair-owned tile march; B: the step's rhs buffer.  One
+// barrier: within a step no thread reads an LDS word another thread writes
+// (SorPairTile)
 template <class T, int B, typename Real>
 __device__ inline void pair_step(const FusedArgsT<Real> &a, const typename T::Tile &tile, Real *lds,
                                  typename T::Thread &th, int s, int check, double &delta) {
   __syncthreads();
-  T::template phase_a<B>(a, tile, lds, th, s);
+  T::template step_begin<B>(a, tile, lds, th, s);
 #pragma unroll
   for (int p = 0; p < T::NPH; ++p) {
-    __syncthreads();
     const double dp = T::template phase_u<B>(a, tile, lds, th, p, s, check && p >= T::NPH - 2);
     delta = delta > dp ? delta : dp;
   }

@@ -21,11 +21,19 @@
 // phase regions, so the pairs a phase works on are a prefix of the
 // numbering: a thread's pair slots past it drop out at compile time.
 //
-// All loads of step s+1 (the plane s+2 of the iterate with its Acz row, the
-// rhs of its first two phases) are issued in phase A of step s into
-// registers indexed by the step number, and first used in step s+1: the only
-// wait for global memory is the one phase A makes for the loads a step
-// earlier.  With two iterations per launch each cell's rhs is read once:
+// One barrier per plane step.  Each thread loads (and stores into the LDS
+// ring) the cells of its own pairs, plus a share of the two edge rows of the
+// slot plane, so within a step every LDS word a thread reads is one it wrote
+// itself or one written in an earlier step: the cell above a phase's cell
+// (plane s+1 for phase 0, the cell phase p-1 just updated for phase p) is
+// the same column of the same pair, kept in a register; the cell below is
+// the pair's own, updated by the thread's next phase; the four neighbours
+// in the plane were written a step earlier.  So the phases of a step need
+// no barrier between them, only the step boundary does (it orders a step's
+// writes before the next step's reads and the ring slot's reuse).
+// The rhs of step s+1's first two phases is loaded at the start of step s,
+// plane s+2 right after phase 0 (which still reads plane s+1 from the
+// registers), into registers indexed by the step number.  With two iterations per launch each cell's rhs is read once:
 // phase p + 2 of step s + 2 updates the cells phase p updated at step s
 // (same plane k = s - p, same colour, and, the pairs being numbered by
 // phase region, the same threads and slots), so it reuses the registers
@@ -86,9 +94,11 @@ struct SorPairTile {
   static constexpr int R = 2 * NIT + 2;             // planes in the LDS ring
   static constexpr int PW = TI + 2 * H, SH = TJ + 2 * H, SW = PW / 2;
   static constexpr int PLANE = SH * PW;             // cells of one slot plane
-  static constexpr int NL = (PLANE + NT - 1) / NT;  // plane loads per thread
+  static constexpr int NEDGE = 2 * PW;              // cells of slot rows 0 and SH-1 (no pairs)
   static constexpr int NPAIR = (SH - 2) * SW;       // pairs of slot rows 1 .. SH-2
   static constexpr int NQ = (NPAIR + NT - 1) / NT;  // pairs per thread
+  static constexpr int NE = (NEDGE + NT - 1) / NT;  // edge cells per thread
+  static constexpr int NL = 2 * NQ + NE;            // plane loads per thread: both cells of each pair, edges
   static constexpr int SLOTS = R * 2 * SH * SW;
   static constexpr int CZ = SLOTS;                  // Acz ring: 3 per slot plane
   static constexpr int SINK = CZ + 3 * R;           // written by pairs whose cell is not updated
@@ -150,7 +160,8 @@ struct SorPairTile {
   static constexpr bool kNorm = sizeof(Real) == sizeof(double);
 
   struct Thread {
-    Real pcn[NL];            // the plane in flight
+    Real pcn[NL];            // the plane in flight (entry 2q + c: column 2h + c of pair slot q)
+    Real tv[NQ];             // the cell above the next phase's cell of each pair
     double czn;              // its Acz row: thread m < 3 carries column m
     int ld_off[NL];          // in-plane offset of the loaded slot cell, -1 outside the array
     int ld_lds[NL];          // LDS index of the slot cell in the ring slot of an even plane, -1 past the slot
@@ -189,18 +200,31 @@ struct SorPairTile {
     return s - p >= T.k0 - hp(p) && s - p <= T.k1 + hp(p);
   }
 
+  // plane load entry: slot cell (x, y), its LDS index in the ring slot of an
+  // even plane (colour of (x, y) there: (x + y) & 1) and its in-plane offset,
+  // -1 outside the array
+  __host__ __device__ static void load_entry(const Args &a, const Tile &T, int x, int y, int &lds_i, int &off) {
+    const int i = T.iL + x, j = T.jL + y;
+    lds_i = ((x + y) & 1) * (SH * SW) + y * SW + (x >> 1);
+    off = (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY) ? (i - 1) + a.d.NX * (j - 1) : -1;
+  }
+
   __host__ __device__ static void init_thread(const Args &a, const Tile &T, int tid, Thread &th) {
     for (int q = 0; q < NL; ++q) {
-      const int e = tid + q * NT;
       th.ld_off[q] = -1;
       th.ld_lds[q] = -1;
-      if (e < PLANE) {
-        const int y = e / PW, x = e - y * PW;
-        const int i = T.iL + x, j = T.jL + y;
-        // colour of (x, y) in an even plane: (x + y) & 1
-        th.ld_lds[q] = ((x + y) & 1) * (SH * SW) + y * SW + (x >> 1);
-        if (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY) th.ld_off[q] = (i - 1) + a.d.NX * (j - 1);
-      }
+    }
+    for (int q = 0; q < NQ; ++q) {
+      const int e = tid + q * NT;
+      if (e >= NPAIR) continue;
+      int h, y;
+      pair_of(e, h, y);
+      for (int c = 0; c < 2; ++c) load_entry(a, T, 2 * h + c, y, th.ld_lds[2 * q + c], th.ld_off[2 * q + c]);
+    }
+    for (int r = 0; r < NE; ++r) {
+      const int e = tid + r * NT;
+      if (e >= NEDGE) continue;
+      load_entry(a, T, e % PW, e < PW ? 0 : SH - 1, th.ld_lds[2 * NQ + r], th.ld_off[2 * NQ + r]);
     }
     th.czm = tid < 3 ? tid : -1;
     th.tid = tid;
@@ -254,6 +278,11 @@ struct SorPairTile {
       for (int q = 0; q < NL; ++q) th.pcn[q] = P[th.ld_off[q] < 0 ? 0 : th.ld_off[q]];
     }
     th.czn = (th.czm >= 0 && p >= 2 && p <= a.d.nz + 1) ? a.Az[(p - 2) + th.czm * a.d.nz] : 0.0;
+  }
+
+  // the value plane p's load entry q stores: zero outside the array
+  __host__ __device__ static Real plane_value(const Args &a, const Thread &th, int p, int q) {
+    return (plane_in(a, p) && th.ld_off[q] >= 0) ? th.pcn[q] : Real(0);
   }
 
   __host__ __device__ static void store_plane(const Args &a, Real *lds, const Thread &th, int p) {
@@ -315,12 +344,11 @@ struct SorPairTile {
     load_rhs(a, T, th, s0, th.rc[0]);
   }
 
-  // phase A of step s (rhs buffer B holds step s's): store plane s+1, issue
-  // the loads of plane s+2 and of step s+1's rhs into buffer B+1
+  // start of step s (rhs buffer B holds step s's): store plane s+1 (the
+  // thread's own cells), issue the loads of step s+1's rhs into buffer B+1
   template <int B>
-  __host__ __device__ static void phase_a(const Args &a, const Tile &T, Real *lds, Thread &th, int s) {
+  __host__ __device__ static void step_begin(const Args &a, const Tile &T, Real *lds, Thread &th, int s) {
     store_plane(a, lds, th, s + 1);
-    if (s + 2 <= last_step(T) + 1) load_plane(a, th, s + 2);
     if (s + 1 <= last_step(T)) load_rhs(a, T, th, s + 1, th.rc[(B + 1) % RB]);
   }
 
@@ -328,14 +356,24 @@ struct SorPairTile {
   // phase writes plane k of the tile, both colours, to dst); returns the
   // largest |change| when norm is set
   template <int B>
-  __host__ __device__ static double phase_u(const Args &a, const Tile &T, Real *lds, const Thread &th, int p, int s,
+  __host__ __device__ static double phase_u(const Args &a, const Tile &T, Real *lds, Thread &th, int p, int s,
                                             bool norm) {
+    const double delta = phase_cells<B>(a, T, lds, th, p, s, norm);
+    // plane s+1 is read from the registers by phase 0 only: now the loads of
+    // plane s+2 may overwrite them
+    if (p == 0 && s + 2 <= last_step(T) + 1) load_plane(a, th, s + 2);
+    return delta;
+  }
+
+  template <int B>
+  __host__ __device__ static double phase_cells(const Args &a, const Tile &T, Real *lds, Thread &th, int p, int s,
+                                                bool norm) {
     if (!active(T, p, s)) return 0.0;
     const int k = s - p, colour = (p & 1) ? 0 : 1;
     const bool kin = k >= 2 && k <= a.d.nz + 1;
     const bool last = p == NPH - 1;
     const int oO = slot_off(k, colour), oN = slot_off(k, 1 - colour);
-    const int oNm = slot_off(k - 1, 1 - colour), oNp = slot_off(k + 1, 1 - colour);
+    const int oNm = slot_off(k - 1, 1 - colour);
     const Real *cz = lds + CZ + 3 * slot_index(k);
     const Real czm = cz[0], czc = cz[1], czp = cz[2];
     const Real omega = Real(a.omega);
@@ -351,7 +389,10 @@ struct SorPairTile {
       const Real *N = lds + oN + c;
       const Real old = lds[oO + c];
       const Real pw = N[par - 1], pe = N[par], ps = N[-SW], pn = N[SW];
-      const Real pb = lds[oNm + c], pt = lds[oNp + c];
+      // above: plane s+1 from the registers (phase 0), else the cell this
+      // thread's previous phase updated; below: the pair's own cell
+      const Real pb = lds[oNm + c];
+      const Real pt = p == 0 ? (par ? plane_value(a, th, s + 1, 2 * q + 1) : plane_value(a, th, s + 1, 2 * q)) : th.tv[q];
       // phases 2, 3: the rhs phase p - 2 loaded two steps earlier
       const Real rhs = p < 2 ? th.rc[B][p < 2 ? p : 0][q] : th.rc[(B + RB - 2) % RB][p >= 2 ? p - 2 : 0][q];
       Real axm, axp, aym, ayp, sxy;
@@ -379,6 +420,7 @@ struct SorPairTile {
       }
       if (!last) {
         lds[upd ? oO + c : SINK] = nw;
+        th.tv[q] = upd ? nw : old;
       } else {
         const Real v = upd ? nw : old;
         const Real partner = par ? pw : pe;  // the other colour in the same half-column

@@ -179,7 +179,7 @@ static double fused_launch(const Problem<Real> &p, int KS, const std::vector<Rea
 // a step alternates with the step's parity from the first step
 template <typename Real, int TI, int TJ, int NT, int NIT, bool CREG>
 static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real> &src, std::vector<Real> &dst,
-                          bool zero_src) {
+                          bool zero_src, int order) {
   using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
   {
     // the pair numbering is a bijection onto slot rows 1..SH-2 x pairs
@@ -217,26 +217,40 @@ static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real
     const int s0 = T::first_step(tile), s1 = T::last_step(tile);
     for (int s = s0; s <= s1; ++s) {
       const int b = (s - s0) % T::RB;  // the kernel's step unrolling
-      for (int tid = 0; tid < NT; ++tid) {
+      auto begin = [&](int tid) {
         switch (b) {
-          case 0: T::template phase_a<0>(a, tile, lds.data(), th[tid], s); break;
-          case 1: T::template phase_a<1>(a, tile, lds.data(), th[tid], s); break;
-          case 2: T::template phase_a<2 % T::RB>(a, tile, lds.data(), th[tid], s); break;
-          default: T::template phase_a<3 % T::RB>(a, tile, lds.data(), th[tid], s); break;
+          case 0: T::template step_begin<0>(a, tile, lds.data(), th[tid], s); break;
+          case 1: T::template step_begin<1>(a, tile, lds.data(), th[tid], s); break;
+          case 2: T::template step_begin<2 % T::RB>(a, tile, lds.data(), th[tid], s); break;
+          default: T::template step_begin<3 % T::RB>(a, tile, lds.data(), th[tid], s); break;
+        }
+      };
+      auto phase = [&](int tid, int ph) {
+        const bool nm = ph >= T::NPH - 2;
+        double dd;
+        switch (b) {
+          case 0: dd = T::template phase_u<0>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 1: dd = T::template phase_u<1>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 2: dd = T::template phase_u<2 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          default: dd = T::template phase_u<3 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+        }
+        delta = delta > dd ? delta : dd;
+      };
+      // the step's only barrier is at its start: any interleaving of the
+      // threads within a step must give the same result; order 0 runs the
+      // phases one after the other over all threads, 1 and 2 each thread's
+      // whole step, threads ascending / descending
+      if (order == 0) {
+        for (int tid = 0; tid < NT; ++tid) begin(tid);
+        for (int ph = 0; ph < T::NPH; ++ph)
+          for (int tid = 0; tid < NT; ++tid) phase(tid, ph);
+      } else {
+        for (int n = 0; n < NT; ++n) {
+          const int tid = order == 1 ? n : NT - 1 - n;
+          begin(tid);
+          for (int ph = 0; ph < T::NPH; ++ph) phase(tid, ph);
         }
       }
-      for (int ph = 0; ph < T::NPH; ++ph)
-        for (int tid = 0; tid < NT; ++tid) {
-          const bool nm = ph >= T::NPH - 2;
-          double dd;
-          switch (b) {
-            case 0: dd = T::template phase_u<0>(a, tile, lds.data(), th[tid], ph, s, nm); break;
-            case 1: dd = T::template phase_u<1>(a, tile, lds.data(), th[tid], ph, s, nm); break;
-            case 2: dd = T::template phase_u<2 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
-            default: dd = T::template phase_u<3 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
-          }
-          delta = delta > dd ? delta : dd;
-        }
     }
   }
   return delta;
@@ -248,7 +262,7 @@ static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real
 // Pair: 0 = SorTile, 1 = SorPairTile with coefficient registers, 2 = with
 // coefficients in LDS
 template <typename Real, int TI, int TJ, int NT, int D, int NIT, int Pair = 0>
-static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_first) {
+static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_first, int order = 0) {
   Problem<Real> p = make_problem<Real>(nx, ny, nz, 0.1);
   std::vector<Real> ref = p.pc, A = p.pc, B = p.pc;
   if (zero_first) {
@@ -262,8 +276,8 @@ static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_firs
     for (int m = 0; m < NIT; ++m) dref = inplace_iteration<Real, Pair != 0>(p, ref);
     double dfus;
     if (Pair)
-      dfus = (it & 1) ? pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, B, A, z)
-                      : pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, A, B, z);
+      dfus = (it & 1) ? pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, B, A, z, order)
+                      : pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, A, B, z, order);
     else
       dfus = (it & 1) ? fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, B, A, z)
                       : fused_launch<Real, TI, TJ, NT, D, NIT>(p, KS, A, B, z);
@@ -306,12 +320,16 @@ template <typename Real, int TI, int TJ, int NT, int NIT, int CREG>
 static int run_pair() {
   constexpr int P = CREG ? 1 : 2;
   int bad = 0;
-  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(37, 21, 13, 4, 3, false);
-  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI, TJ, 9, 32, 2, false);
-  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, false);
-  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(5, 3, 7, 1, 3, false);
-  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(19, 9, 8, 2, 3, false);
-  bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, true);
+  // thread orders within a step: phases in turn over all threads, then each
+  // thread's whole step, ascending and descending
+  for (int order = 0; order < 3; ++order) {
+    bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(37, 21, 13, 4, 3, false, order);
+    bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI, TJ, 9, 32, 2, false, order);
+    bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, false, order);
+    bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(5, 3, 7, 1, 3, false, order);
+    bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(19, 9, 8, 2, 3, false, order);
+    bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, true, order);
+  }
   return bad;
 }
 
