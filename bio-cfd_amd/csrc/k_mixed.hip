@@ -271,8 +271,8 @@ constexpr int kNumSweep32 = (int)(sizeof(kSweep32) / sizeof(kSweep32[0]));
 
 struct MixedConfig {
   int precision = 0;        // 0: FP64 method, 1: mixed
-  int variant = 15;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 512 threads, 2 WG/CU)
-  int planes = 16;
+  int variant = 21;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 1024 threads, 2 sweeps per pass)
+  int planes = 32;
   int sweeps = 32;          // FP32 sweeps per correction
   double tol = 1e-8;        // max-norm of the FP64 residual
 };

@@ -17,10 +17,11 @@ pytestmark = pytest.mark.gpu
 STEPS = 3
 # the library's mixed-precision defaults (variant, planes, sweeps per correction, residual tolerance),
 # restored after each test
-DEFAULT = (15, 16, 32, 1e-8)
+DEFAULT = (21, 32, 32, 1e-8)
 # (variant, planes, sweeps per correction): the per-phase cell-list FP32 kernel at 4 sweeps and the
-# pair-owned FP32 kernel (fused multiply-adds, hardware reciprocal) at the default 32
-CONFIGS = [(1, 32, 4), (15, 16, 32)]
+# pair-owned FP32 kernels (fused multiply-adds, hardware reciprocal; one and two sweeps per pass) at
+# the default 32
+CONFIGS = [(1, 32, 4), (15, 16, 32), (21, 32, 32)]
 # max |mixed - fp64| / max |fp64| allowed per field.  The FP32 rounding the corrections carry grows
 # with the sweeps per correction: at 4 sweeps the fields stay within 1e-8 of the FP64 run, at 32 within
 # 2e-7 (velocities, p) on these cases, so the second set is the accuracy contract of the mixed path
