@@ -264,6 +264,8 @@ static const Sweep32Variant kSweep32[] = {
     PS(128, 16, 512, 4, 1, 1),
     PS(64, 32, 1024, 1, 2, 1),
     PS(64, 32, 1024, 1, 1, 1),
+    PS(64, 22, 1024, 1, 2, 1),
+    PS(64, 22, 512, 1, 2, 1),
 };
 #undef SV
 #undef PS

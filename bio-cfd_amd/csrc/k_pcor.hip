@@ -495,6 +495,8 @@ static const FusedVariant kFused[] = {
     PV(64, 32, 1024, 1, 2, 1),
     PV(128, 16, 1024, 1, 2, 0),
     PV(64, 32, 1024, 1, 1, 0),
+    PV(64, 22, 512, 1, 2, 1),
+    PV(64, 22, 512, 1, 2, 0),
 };
 #undef FV
 #undef PV

@@ -49,25 +49,40 @@ __global__ void __launch_bounds__(NT, MINB) k_sor_fused(FusedArgsT<Real> a, int 
   if (check) wg_max_store(delta, partials);
 }
 
-// one step of the pair-owned tile march; B: the step's rhs buffer.  One
-// barrier: within a step no thread reads an LDS word another thread writes
-// (SorPairTile)
-template <class T, int B, typename Real>
-__device__ inline void pair_step(const FusedArgsT<Real> &a, const typename T::Tile &tile, Real *lds,
+// one step of the pair-owned tile march; B: the step's rhs buffer, S: a
+// step of a guard-free group.  One barrier: within a step no thread reads an
+// LDS word another thread writes (SorPairTile)
+template <class T, int B, bool S, typename Real>
+__device__ __forceinline__ void pair_step(const FusedArgsT<Real> &a, const typename T::Tile &tile, Real *lds,
                                  typename T::Thread &th, int s, int check, double &delta) {
   __syncthreads();
-  T::template step_begin<B>(a, tile, lds, th, s);
+  T::template step_begin<B, S>(a, tile, lds, th, s);
 #pragma unroll
   for (int p = 0; p < T::NPH; ++p) {
-    const double dp = T::template phase_u<B>(a, tile, lds, th, p, s, check && p >= T::NPH - 2);
+    // keep the phases apart in the schedule: hoisting a later phase's LDS
+    // reads above an earlier phase only lengthens register lifetimes
+    __builtin_amdgcn_sched_barrier(0);
+    const double dp = T::template phase_u<B, S>(a, tile, lds, th, p, s, check && p >= T::NPH - 2);
     delta = delta > dp ? delta : dp;
+  }
+}
+
+// RB steps from s (the last ones only while s + b <= s1 unless S)
+template <class T, bool S, typename Real>
+__device__ __forceinline__ void pair_group(const FusedArgsT<Real> &a, const typename T::Tile &tile, Real *lds,
+                                  typename T::Thread &th, int s, int s1, int check, double &delta) {
+  pair_step<T, 0, S>(a, tile, lds, th, s, check, delta);
+  if (S || s + 1 <= s1) pair_step<T, 1, S>(a, tile, lds, th, s + 1, check, delta);
+  if constexpr (T::RB == 4) {
+    if (S || s + 2 <= s1) pair_step<T, 2 % T::RB, S>(a, tile, lds, th, s + 2, check, delta);
+    if (S || s + 3 <= s1) pair_step<T, 3 % T::RB, S>(a, tile, lds, th, s + 3, check, delta);
   }
 }
 
 // the pair-owned tile march of one workgroup on tile `tile_index` (1-D,
 // gx*gy tiles per z chunk) of the block described by `a`
 template <class T, typename Real>
-__device__ inline void pair_run(const FusedArgsT<Real> &a, int tile_index, int gx, int gy, int check,
+__device__ __forceinline__ void pair_run(const FusedArgsT<Real> &a, int tile_index, int gx, int gy, int check,
                                 double *__restrict__ partials) {
   __shared__ Real lds[T::LDS_DOUBLES];
   const typename T::Tile tile = T::tile_of(a, tile_index, gx, gy);
@@ -76,14 +91,12 @@ __device__ inline void pair_run(const FusedArgsT<Real> &a, int tile_index, int g
   T::prologue(a, tile, lds, th, threadIdx.x);
   double delta = 0.0;
   const int s1 = T::last_step(tile);
-  for (int s = T::first_step(tile); s <= s1; s += T::RB) {
-    pair_step<T, 0>(a, tile, lds, th, s, check, delta);
-    if (s + 1 <= s1) pair_step<T, 1>(a, tile, lds, th, s + 1, check, delta);
-    if constexpr (T::RB == 4) {
-      if (s + 2 <= s1) pair_step<T, 2 % T::RB>(a, tile, lds, th, s + 2, check, delta);
-      if (s + 3 <= s1) pair_step<T, 3 % T::RB>(a, tile, lds, th, s + 3, check, delta);
-    }
-  }
+  // groups of RB steps: guarded ones until the first that fits the
+  // guard-free range, the guard-free ones, then guarded ones to the end
+  int s = T::first_step(tile);
+  for (; s <= s1 && !T::group_steady(a, tile, s); s += T::RB) pair_group<T, false>(a, tile, lds, th, s, s1, check, delta);
+  for (; s <= s1 && T::group_steady(a, tile, s); s += T::RB) pair_group<T, true>(a, tile, lds, th, s, s1, check, delta);
+  for (; s <= s1; s += T::RB) pair_group<T, false>(a, tile, lds, th, s, s1, check, delta);
   if (check) wg_max_store(delta, partials);
 }
 

@@ -16,7 +16,7 @@
 // LDS (fewer registers, so more workgroups per CU).
 // Whether the cell is updated (inside the phase's region and the interior
 // in i and j) is a precomputed bit per (phase, parity); a pair whose cell is
-// not updated writes to a sink word, so the phase has no branches.  The
+// not updated writes its own value back, so the phase has no branches.  The
 // pairs are numbered from the tile outwards, ring by ring of the shrinking
 // phase regions, so the pairs a phase works on are a prefix of the
 // numbering: a thread's pair slots past it drop out at compile time.
@@ -38,6 +38,19 @@
 // (same plane k = s - p, same colour, and, the pairs being numbered by
 // phase region, the same threads and slots), so it reuses the registers
 // loaded for phase p, which live four steps (buffers by step mod 4).
+//
+// Within a step the parity bit of every pair is the same for all phases (the
+// colour of phase p and its plane k = s - p change together), and it flips
+// from step to step.  So everything that depends on it is precomputed per
+// step class (b = (s - first step) & 1, known at compile time in the
+// unrolled step loop): the coefficients of the cell the class updates, the
+// update bit of each phase and the parity itself.  The steps in the middle
+// of the march, where every phase is active, every plane inside the block
+// and the next step inside the march, run a copy of the step with none of
+// these guards (groups of RB steps that fit entirely; the first and last
+// groups keep them).  Plane loads go through a raw buffer resource over the
+// plane: cells outside the array get an offset past its end, which the
+// hardware range check returns as zero.
 //
 // Real = double is the reference's SOR (bit-identical: the operand order of
 // sor_cell, Acx(c) + Acy(c) + Acz(c) summed left to right); Real = float the
@@ -101,15 +114,17 @@ struct SorPairTile {
   static constexpr int NL = 2 * NQ + NE;            // plane loads per thread: both cells of each pair, edges
   static constexpr int SLOTS = R * 2 * SH * SW;
   static constexpr int CZ = SLOTS;                  // Acz ring: 3 per slot plane
-  static constexpr int SINK = CZ + 3 * R;           // written by pairs whose cell is not updated
+  static constexpr int SINK = CZ + 3 * R;           // written by load entries past the slot plane
   static constexpr int CXW = PW - 2, CYW = SH - 2;  // staged Acx (Acy) of slot columns (rows) 1 .. PW-2 (SH-2)
   static constexpr int CX = SINK + 1, CY = CX + 3 * CXW;
   static constexpr int LDS_DOUBLES = CREG ? SINK + 1 : CY + 3 * CYW;
-  // mask bits of a pair: 2p + par = the cell at column 2h + par is updated by
-  // phase p; kW0, kW1 = the last phase writes cell 2h, 2h+1 to the
-  // destination; kYOdd = slot row odd; bits 16-23: h, 24-31: y
-  static constexpr int kW0 = 2 * NPH, kW1 = 2 * NPH + 1, kYOdd = 2 * NPH + 2;
-  static_assert(kYOdd < 16 && SW < 256 && SH < 256, "pair mask layout");
+  // mask bits of a pair: b * NPH + p = in steps of class b (b = (s - first
+  // step) & 1) phase p updates the pair's cell of that class; kW0, kW1 = the
+  // last phase writes cell 2h, 2h+1 to the destination; kPar + b = the column
+  // (2h + par) of the cell class b updates; bits 16-23: h, 24-31: y
+  static constexpr int kW0 = 2 * NPH, kW1 = 2 * NPH + 1, kPar = 2 * NPH + 2;
+  static_assert(kPar + 1 < 16 && SW < 256 && SH < 256, "pair mask layout");
+  static constexpr unsigned kOut = 0x80000000u;     // byte offset of a cell outside the array
   static constexpr int NQC = CREG ? NQ : 1;         // pairs with coefficient registers
   static constexpr int RB = NIT == 2 ? 4 : 2;       // rhs register buffers (steps)
 
@@ -163,7 +178,7 @@ struct SorPairTile {
     Real pcn[NL];            // the plane in flight (entry 2q + c: column 2h + c of pair slot q)
     Real tv[NQ];             // the cell above the next phase's cell of each pair
     double czn;              // its Acz row: thread m < 3 carries column m
-    int ld_off[NL];          // in-plane offset of the loaded slot cell, -1 outside the array
+    unsigned ld_off[NL];     // in-plane byte offset of the loaded slot cell, kOut outside the array
     int ld_lds[NL];          // LDS index of the slot cell in the ring slot of an even plane, -1 past the slot
     int czm;                 // Acz column this thread carries (0 .. 2), -1 for none
     int tid;
@@ -171,7 +186,7 @@ struct SorPairTile {
     unsigned mask[NQ];
     int roff[NQ];            // rhs in-plane offset of column 2h (column 2h+1: + 1)
     int goff[NQ];            // iterate in-plane offset of column 2h
-    Real cxm[NQC][2], cxp[NQC][2], sxy[NQC][2], cym[NQC], cyp[NQC];
+    Real cxm[NQC][2], cxp[NQC][2], sxy[NQC][2], cym[NQC], cyp[NQC];  // [b]: the cell of step class b
     Real rc[RB][2][NQ];      // rhs of the cells of phases 0 and 1, by step mod RB
   };
 
@@ -201,17 +216,18 @@ struct SorPairTile {
   }
 
   // plane load entry: slot cell (x, y), its LDS index in the ring slot of an
-  // even plane (colour of (x, y) there: (x + y) & 1) and its in-plane offset,
-  // -1 outside the array
-  __host__ __device__ static void load_entry(const Args &a, const Tile &T, int x, int y, int &lds_i, int &off) {
+  // even plane (colour of (x, y) there: (x + y) & 1) and its in-plane byte
+  // offset, kOut outside the array
+  __host__ __device__ static void load_entry(const Args &a, const Tile &T, int x, int y, int &lds_i, unsigned &off) {
     const int i = T.iL + x, j = T.jL + y;
     lds_i = ((x + y) & 1) * (SH * SW) + y * SW + (x >> 1);
-    off = (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY) ? (i - 1) + a.d.NX * (j - 1) : -1;
+    off = (i >= 1 && i <= a.d.NX && j >= 1 && j <= a.d.NY)
+              ? (unsigned)((i - 1) + a.d.NX * (j - 1)) * (unsigned)sizeof(Real) : kOut;
   }
 
   __host__ __device__ static void init_thread(const Args &a, const Tile &T, int tid, Thread &th) {
     for (int q = 0; q < NL; ++q) {
-      th.ld_off[q] = -1;
+      th.ld_off[q] = kOut;
       th.ld_lds[q] = -1;
     }
     for (int q = 0; q < NQ; ++q) {
@@ -229,6 +245,10 @@ struct SorPairTile {
     th.czm = tid < 3 ? tid : -1;
     th.tid = tid;
     const int nxh = a.NXh * a.d.NY;
+    // the colour of phase p and its plane k = s - p give (colour ^ k) & 1 =
+    // 1 ^ (s & 1): a pair updates column 2h + ((y ^ 1 ^ s) & 1) in every
+    // phase of step s, and s & 1 = (first step & 1) ^ b in class b
+    const int s0par = first_step(T) & 1;
     for (int q = 0; q < NQ; ++q) {
       const int e = tid + q * NT;
       const bool real = e < NPAIR;
@@ -236,20 +256,22 @@ struct SorPairTile {
       if (real) pair_of(e, h, y);
       const int j = T.jL + y, i0 = T.iL + 2 * h;
       const bool jin = j >= 2 && j <= a.d.ny + 1;
-      unsigned m = ((y & 1) ? 1u << kYOdd : 0u) | (unsigned)h << 16 | (unsigned)y << 24;
-      for (int par = 0; par < 2; ++par) {
+      unsigned m = (unsigned)h << 16 | (unsigned)y << 24;
+      for (int b = 0; b < 2; ++b) {
+        const int par = (y ^ 1 ^ s0par ^ b) & 1;
         const int x = 2 * h + par, i = i0 + par;
         const bool iin = i >= 2 && i <= a.d.nx + 1;
+        m |= (unsigned)par << (kPar + b);
         for (int p = 0; p < NPH; ++p) {
           const int lo = p + 1, hi = lo + 2 * hp(p);  // region: [lo, lo + T + 2 hp)
-          if (real && iin && jin && x >= lo && x < hi + TI && y >= lo && y < hi + TJ) m |= 1u << (2 * p + par);
+          if (real && iin && jin && x >= lo && x < hi + TI && y >= lo && y < hi + TJ) m |= 1u << (b * NPH + p);
         }
         if constexpr (CREG) {
           const Coef3T<Real> cx = iin ? coef_at<Real>(a.Ax, a.d.nx, i) : Coef3T<Real>{Real(0), Real(0), Real(0)};
           const Coef3T<Real> cy = jin ? coef_at<Real>(a.Ay, a.d.ny, j) : Coef3T<Real>{Real(0), Real(0), Real(0)};
-          th.cxm[q][par] = cx.m;
-          th.cxp[q][par] = cx.p;
-          th.sxy[q][par] = cx.c + cy.c;
+          th.cxm[q][b] = cx.m;
+          th.cxp[q][b] = cx.p;
+          th.sxy[q][b] = cx.c + cy.c;
           th.cym[q] = cy.m;
           th.cyp[q] = cy.p;
         }
@@ -270,47 +292,77 @@ struct SorPairTile {
 
   __host__ __device__ static bool plane_in(const Args &a, int p) { return p >= 1 && p <= a.d.NZ && !a.zero_src; }
 
-  // plane p of src over the slot footprint, raw (zeros outside the array are
-  // applied when it is stored), and column czm of its Acz row
+  // the steps that need no guard: every phase active (s >= k0 + 2 NIT - 1),
+  // every phase's plane inside the block (2 NIT + 1 <= s <= nz + 1), the
+  // planes s+1 and s+2 inside the array and step s+1 inside the march
+  // (s <= min(last step - 1, nz)); none when the source is not read
+  __host__ __device__ static int steady_lo(const Args &, const Tile &T) {
+    return T.k0 + 2 * NIT - 1 > 2 * NIT + 1 ? T.k0 + 2 * NIT - 1 : 2 * NIT + 1;
+  }
+  __host__ __device__ static int steady_hi(const Args &a, const Tile &T) {
+    if (a.zero_src) return -(1 << 30);
+    return last_step(T) - 1 < a.d.nz ? last_step(T) - 1 : a.d.nz;
+  }
+  // the group of RB steps starting at s runs without guards
+  __host__ __device__ static bool group_steady(const Args &a, const Tile &T, int s) {
+    return s >= steady_lo(a, T) && s + RB - 1 <= steady_hi(a, T);
+  }
+
+  // plane p of src over the slot footprint (cells outside the array read as
+  // zero) and column czm of its Acz row
+  template <bool S>
   __host__ __device__ static void load_plane(const Args &a, Thread &th, int p) {
-    if (plane_in(a, p)) {
+    if (S || plane_in(a, p)) {
       const Real *P = a.src + (int64_t)(p - 1) * ((int64_t)a.d.NX * a.d.NY);
-      for (int q = 0; q < NL; ++q) th.pcn[q] = P[th.ld_off[q] < 0 ? 0 : th.ld_off[q]];
+      const unsigned nbytes = (unsigned)a.d.NX * (unsigned)a.d.NY * (unsigned)sizeof(Real);
+#if defined(__HIP_DEVICE_COMPILE__)
+      const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(const_cast<Real *>(P), (short)0, (int)nbytes,
+                                                                         0x00020000);
+      for (int q = 0; q < NL; ++q) {
+        if constexpr (sizeof(Real) == 8)
+          th.pcn[q] = __builtin_bit_cast(Real, __builtin_amdgcn_raw_buffer_load_b64(r, (int)th.ld_off[q], 0, 0));
+        else
+          th.pcn[q] = __builtin_bit_cast(Real, __builtin_amdgcn_raw_buffer_load_b32(r, (int)th.ld_off[q], 0, 0));
+      }
+#else
+      for (int q = 0; q < NL; ++q) th.pcn[q] = th.ld_off[q] >= nbytes ? Real(0) : P[th.ld_off[q] / sizeof(Real)];
+#endif
     }
     th.czn = (th.czm >= 0 && p >= 2 && p <= a.d.nz + 1) ? a.Az[(p - 2) + th.czm * a.d.nz] : 0.0;
   }
 
-  // the value plane p's load entry q stores: zero outside the array
+  // the value plane p's load entry q stores
+  template <bool S>
   __host__ __device__ static Real plane_value(const Args &a, const Thread &th, int p, int q) {
-    return (plane_in(a, p) && th.ld_off[q] >= 0) ? th.pcn[q] : Real(0);
+    return (S || plane_in(a, p)) ? th.pcn[q] : Real(0);
   }
 
+  template <bool S>
   __host__ __device__ static void store_plane(const Args &a, Real *lds, const Thread &th, int p) {
-    const bool pin = plane_in(a, p);
     const int base = slot_off(p, 0), flip = (p & 1) * (SH * SW);
     for (int q = 0; q < NL; ++q) {
       const int l = th.ld_lds[q];
       // an odd plane swaps the colour halves
       const int w = l < 0 ? SINK : base + (l >= SH * SW ? l - flip : l + flip);
-      lds[w] = (pin && th.ld_off[q] >= 0) ? th.pcn[q] : Real(0);
+      lds[w] = plane_value<S>(a, th, p, q);
     }
     if (th.czm >= 0) lds[CZ + 3 * slot_index(p) + th.czm] = Real(th.czn);
   }
 
-  // the rhs of phases 0 and 1 active at step s, into buffer rb (phases 2
-  // and 3 of step s + 2 reuse them)
+  // the rhs of phases 0 and 1 active at step s (class CL), into buffer rb
+  // (phases 2 and 3 of step s + 2 reuse them)
+  template <int CL, bool S>
   __host__ __device__ static void load_rhs(const Args &a, const Tile &T, Thread &th, int s, Real (*rb)[NQ]) {
     const int64_t plane = (int64_t)a.NXh * a.d.NY;
     for (int p = 0; p < 2; ++p) {
-      if (!active(T, p, s)) continue;
+      if (!S && !active(T, p, s)) continue;
       const int k = s - p, colour = (p & 1) ? 0 : 1;
-      const bool kin = k >= 2 && k <= a.d.nz + 1;
+      const bool kin = S || (k >= 2 && k <= a.d.nz + 1);
       const Real *B = (colour ? a.bred : a.bblack) + (kin ? k - 1 : 1) * plane;
-      const unsigned cbit = (unsigned)((colour ^ k) & 1);
       for (int q = 0; q < NQ; ++q) {
         if (q >= used(p)) break;
-        const unsigned par = ((th.mask[q] >> kYOdd) ^ cbit) & 1u;
-        rb[p][q] = B[th.roff[q] + (int)par];
+        const int par = (int)((th.mask[q] >> (kPar + CL)) & 1u);
+        rb[p][q] = B[th.roff[q] + par];
       }
     }
   }
@@ -336,70 +388,72 @@ struct SorPairTile {
   __host__ __device__ static void prologue(const Args &a, const Tile &T, Real *lds, Thread &th, int tid) {
     load_coef(a, T, lds, tid);
     const int s0 = first_step(T);
-    load_plane(a, th, s0 - 1);
-    store_plane(a, lds, th, s0 - 1);
-    load_plane(a, th, s0);
-    store_plane(a, lds, th, s0);
-    load_plane(a, th, s0 + 1);
-    load_rhs(a, T, th, s0, th.rc[0]);
+    load_plane<false>(a, th, s0 - 1);
+    store_plane<false>(a, lds, th, s0 - 1);
+    load_plane<false>(a, th, s0);
+    store_plane<false>(a, lds, th, s0);
+    load_plane<false>(a, th, s0 + 1);
+    load_rhs<0, false>(a, T, th, s0, th.rc[0]);
   }
 
   // start of step s (rhs buffer B holds step s's): store plane s+1 (the
-  // thread's own cells), issue the loads of step s+1's rhs into buffer B+1
-  template <int B>
+  // thread's own cells), issue the loads of step s+1's rhs into buffer B+1.
+  // S: a step of a guard-free group
+  template <int B, bool S = false>
   __host__ __device__ static void step_begin(const Args &a, const Tile &T, Real *lds, Thread &th, int s) {
-    store_plane(a, lds, th, s + 1);
-    if (s + 1 <= last_step(T)) load_rhs(a, T, th, s + 1, th.rc[(B + 1) % RB]);
+    store_plane<S>(a, lds, th, s + 1);
+    if (S || s + 1 <= last_step(T)) load_rhs<(B + 1) & 1, S>(a, T, th, s + 1, th.rc[(B + 1) % RB]);
   }
 
   // phase p of step s: update its colour on plane k = s - p (in LDS; the last
   // phase writes plane k of the tile, both colours, to dst); returns the
   // largest |change| when norm is set
-  template <int B>
+  template <int B, bool S = false>
   __host__ __device__ static double phase_u(const Args &a, const Tile &T, Real *lds, Thread &th, int p, int s,
                                             bool norm) {
-    const double delta = phase_cells<B>(a, T, lds, th, p, s, norm);
-    // plane s+1 is read from the registers by phase 0 only: now the loads of
-    // plane s+2 may overwrite them
-    if (p == 0 && s + 2 <= last_step(T) + 1) load_plane(a, th, s + 2);
+    const double delta = phase_cells<B, S>(a, T, lds, th, p, s, norm);
+    // the loads of plane s+2 are issued after phase 0, not with the step's
+    // start, so that their registers are not held through phase 0
+    if (p == 0 && (S || s + 2 <= last_step(T) + 1)) load_plane<S>(a, th, s + 2);
     return delta;
   }
 
-  template <int B>
+  template <int B, bool S>
   __host__ __device__ static double phase_cells(const Args &a, const Tile &T, Real *lds, Thread &th, int p, int s,
                                                 bool norm) {
-    if (!active(T, p, s)) return 0.0;
+    if (!S && !active(T, p, s)) return 0.0;
+    constexpr int CL = B & 1;  // the step's class (RB is even)
     const int k = s - p, colour = (p & 1) ? 0 : 1;
-    const bool kin = k >= 2 && k <= a.d.nz + 1;
+    const bool kin = S || (k >= 2 && k <= a.d.nz + 1);
     const bool last = p == NPH - 1;
     const int oO = slot_off(k, colour), oN = slot_off(k, 1 - colour);
-    const int oNm = slot_off(k - 1, 1 - colour);
+    const int oNm = slot_off(k - 1, 1 - colour), oNp = slot_off(k + 1, 1 - colour);
     const Real *cz = lds + CZ + 3 * slot_index(k);
     const Real czm = cz[0], czc = cz[1], czp = cz[2];
     const Real omega = Real(a.omega);
-    const unsigned cbit = (unsigned)((colour ^ k) & 1);
     Real *P = a.dst + (int64_t)(k - 1) * ((int64_t)a.d.NX * a.d.NY);
     double delta = 0.0;
     for (int q = 0; q < NQ; ++q) {
       if (q >= used(p)) break;
       if (!full(p, q) && th.tid + q * NT >= cnt(p)) continue;
       const unsigned m = th.mask[q];
-      const int par = (int)(((m >> kYOdd) ^ cbit) & 1u);
+      const int par = (int)((m >> (kPar + CL)) & 1u);
       const int c = th.c[q];
       const Real *N = lds + oN + c;
       const Real old = lds[oO + c];
       const Real pw = N[par - 1], pe = N[par], ps = N[-SW], pn = N[SW];
-      // above: plane s+1 from the registers (phase 0), else the cell this
-      // thread's previous phase updated; below: the pair's own cell
+      // above: plane s+1 as this thread stored it at the step's start (phase
+      // 0), else the cell this thread's previous phase updated; below: the
+      // pair's own cell
       const Real pb = lds[oNm + c];
-      const Real pt = p == 0 ? (par ? plane_value(a, th, s + 1, 2 * q + 1) : plane_value(a, th, s + 1, 2 * q)) : th.tv[q];
+      const Real pt = p == 0 ? lds[oNp + c] : th.tv[q];
       // phases 2, 3: the rhs phase p - 2 loaded two steps earlier
       const Real rhs = p < 2 ? th.rc[B][p < 2 ? p : 0][q] : th.rc[(B + RB - 2) % RB][p >= 2 ? p - 2 : 0][q];
       Real axm, axp, aym, ayp, sxy;
       if constexpr (CREG) {
-        axm = par ? th.cxm[q][1] : th.cxm[q][0];
-        axp = par ? th.cxp[q][1] : th.cxp[q][0];
-        sxy = par ? th.sxy[q][1] : th.sxy[q][0];
+        axm = th.cxm[q][CL];
+        axp = th.cxp[q][CL];
+        sxy = th.sxy[q][CL];
         aym = th.cym[q];
         ayp = th.cyp[q];
       } else {
@@ -413,16 +467,18 @@ struct SorPairTile {
       }
       // the centre sum (Acx + Acy) + Acz, as sor_cell's
       const Real nw = sor_update(omega, rhs, old, axm, pw, axp, pe, aym, ps, ayp, pn, czm, pb, czp, pt, sxy + czc);
-      const bool upd = kin && ((m >> (2 * p + par)) & 1u) && !sor_skip(rhs);
+      const bool upd = kin && ((m >> (CL * NPH + p)) & 1u) && !sor_skip(rhs);
       if (kNorm && norm) {
         const double dd = upd ? fabs(double(nw - old)) : 0.0;
         delta = delta > dd ? delta : dd;
       }
+      const Real v = upd ? nw : old;
       if (!last) {
-        lds[upd ? oO + c : SINK] = nw;
-        th.tv[q] = upd ? nw : old;
+        // a cell that is not updated gets its own value back: only this
+        // thread writes it, and it holds that value since an earlier step
+        lds[oO + c] = v;
+        th.tv[q] = v;
       } else {
-        const Real v = upd ? nw : old;
         const Real partner = par ? pw : pe;  // the other colour in the same half-column
         if (m & (1u << kW0)) P[th.goff[q]] = par ? partner : v;
         if (m & (1u << kW1)) P[th.goff[q] + 1] = par ? v : partner;

@@ -32,6 +32,7 @@ struct Problem {
 };
 
 static unsigned long long g_rng = 88172645463325252ull;
+static long long steady_steps = 0;  // pair-tile steps emulated in guard-free groups
 static double urand() {
   g_rng ^= g_rng << 13;
   g_rng ^= g_rng >> 7;
@@ -217,22 +218,33 @@ static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real
     const int s0 = T::first_step(tile), s1 = T::last_step(tile);
     for (int s = s0; s <= s1; ++s) {
       const int b = (s - s0) % T::RB;  // the kernel's step unrolling
+      // the kernel's guard-free groups of RB steps
+      const bool st = T::group_steady(a, tile, s - b);
+      steady_steps += st;
       auto begin = [&](int tid) {
-        switch (b) {
-          case 0: T::template step_begin<0>(a, tile, lds.data(), th[tid], s); break;
-          case 1: T::template step_begin<1>(a, tile, lds.data(), th[tid], s); break;
-          case 2: T::template step_begin<2 % T::RB>(a, tile, lds.data(), th[tid], s); break;
-          default: T::template step_begin<3 % T::RB>(a, tile, lds.data(), th[tid], s); break;
+        switch (b + (st ? 4 : 0)) {
+          case 0: T::template step_begin<0, false>(a, tile, lds.data(), th[tid], s); break;
+          case 1: T::template step_begin<1, false>(a, tile, lds.data(), th[tid], s); break;
+          case 2: T::template step_begin<2 % T::RB, false>(a, tile, lds.data(), th[tid], s); break;
+          case 3: T::template step_begin<3 % T::RB, false>(a, tile, lds.data(), th[tid], s); break;
+          case 4: T::template step_begin<0, true>(a, tile, lds.data(), th[tid], s); break;
+          case 5: T::template step_begin<1, true>(a, tile, lds.data(), th[tid], s); break;
+          case 6: T::template step_begin<2 % T::RB, true>(a, tile, lds.data(), th[tid], s); break;
+          default: T::template step_begin<3 % T::RB, true>(a, tile, lds.data(), th[tid], s); break;
         }
       };
       auto phase = [&](int tid, int ph) {
         const bool nm = ph >= T::NPH - 2;
         double dd;
-        switch (b) {
-          case 0: dd = T::template phase_u<0>(a, tile, lds.data(), th[tid], ph, s, nm); break;
-          case 1: dd = T::template phase_u<1>(a, tile, lds.data(), th[tid], ph, s, nm); break;
-          case 2: dd = T::template phase_u<2 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
-          default: dd = T::template phase_u<3 % T::RB>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+        switch (b + (st ? 4 : 0)) {
+          case 0: dd = T::template phase_u<0, false>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 1: dd = T::template phase_u<1, false>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 2: dd = T::template phase_u<2 % T::RB, false>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 3: dd = T::template phase_u<3 % T::RB, false>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 4: dd = T::template phase_u<0, true>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 5: dd = T::template phase_u<1, true>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          case 6: dd = T::template phase_u<2 % T::RB, true>(a, tile, lds.data(), th[tid], ph, s, nm); break;
+          default: dd = T::template phase_u<3 % T::RB, true>(a, tile, lds.data(), th[tid], ph, s, nm); break;
         }
         delta = delta > dd ? delta : dd;
       };
@@ -320,15 +332,22 @@ template <typename Real, int TI, int TJ, int NT, int NIT, int CREG>
 static int run_pair() {
   constexpr int P = CREG ? 1 : 2;
   int bad = 0;
+  const long long steady0 = steady_steps;
   // thread orders within a step: phases in turn over all threads, then each
   // thread's whole step, ascending and descending
   for (int order = 0; order < 3; ++order) {
+    // deep slabs: the middle of each march runs in guard-free groups
+    bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(21, 13, 40, 16, 2, false, order);
     bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(37, 21, 13, 4, 3, false, order);
     bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI, TJ, 9, 32, 2, false, order);
     bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, false, order);
     bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(5, 3, 7, 1, 3, false, order);
     bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(19, 9, 8, 2, 3, false, order);
     bad += run_case<Real, TI, TJ, NT, 1, NIT, P>(TI + 6, 2 * TJ + 3, 11, 3, 4, true, order);
+  }
+  if (steady_steps == steady0) {
+    std::printf("no guard-free step group emulated for tile %dx%d/%d x%d\n", TI, TJ, NT, NIT);
+    ++bad;
   }
   return bad;
 }
@@ -389,6 +408,14 @@ int main() {
   bad += run_pair<double, 64, 32, 1024, 1, 1>();
   bad += run_pair<double, 128, 16, 1024, 2, 0>();
   bad += run_pair<double, 128, 16, 1024, 1, 0>();
+  bad += run_pair<double, 64, 22, 512, 2, 1>();
+  bad += run_pair<double, 64, 22, 512, 1, 1>();
+  bad += run_pair<double, 64, 22, 512, 2, 0>();
+  bad += run_pair<double, 64, 22, 512, 1, 0>();
+  bad += run_pair<float, 64, 22, 1024, 2, 1>();
+  bad += run_pair<float, 64, 22, 1024, 1, 1>();
+  bad += run_pair<float, 64, 22, 512, 2, 1>();
+  bad += run_pair<float, 64, 22, 512, 1, 1>();
   bad += run_pair<float, 64, 32, 1024, 2, 1>();
   bad += run_pair<float, 64, 32, 1024, 1, 1>();
   bad += run_pair<float, 64, 32, 512, 1, 1>();
