@@ -158,7 +158,8 @@ def run_uneven(args, batching: int):
     case = CS.config_uneven(nsteps=args.warmup + args.steps)
     sim = Simulation(case)
     H.call("biocfd_sor_batching", batching)
-    H.call("biocfd_sor_config", 21, args.sor_planes)
+    # the pair-owned 64x32 tile with LDS coefficients unless a variant is named
+    H.call("biocfd_sor_config", 26 if args.sor_variant == "auto" else int(args.sor_variant), args.sor_planes)
     for _ in range(args.warmup):
         sim.step()
     H.call("biocfd_synchronize")

@@ -501,7 +501,7 @@ static const FusedVariant kFused[] = {
 #undef FV
 #undef PV
 constexpr int kNumSorVariants = (int)(sizeof(kFused) / sizeof(kFused[0]));
-static int g_sor_variant = 21;  // the fastest at 256^3 on MI355X (pair-owned 64x32, 2 iterations per pass)
+static int g_sor_variant = 26;  // the fastest at 256^3 on MI355X (pair-owned 64x32, LDS coefficients, 2 iterations per pass)
 static int g_sor_planes = 32;
 
 static int enqueue_split_iterations(Block &b, SorState *st, long long n, int64_t pc_ita_max, double epsi) {

@@ -59,9 +59,6 @@ __device__ __forceinline__ void pair_step(const FusedArgsT<Real> &a, const typen
   T::template step_begin<B, S>(a, tile, lds, th, s);
 #pragma unroll
   for (int p = 0; p < T::NPH; ++p) {
-    // keep the phases apart in the schedule: hoisting a later phase's LDS
-    // reads above an earlier phase only lengthens register lifetimes
-    __builtin_amdgcn_sched_barrier(0);
     const double dp = T::template phase_u<B, S>(a, tile, lds, th, p, s, check && p >= T::NPH - 2);
     delta = delta > dp ? delta : dp;
   }
@@ -91,12 +88,14 @@ __device__ __forceinline__ void pair_run(const FusedArgsT<Real> &a, int tile_ind
   T::prologue(a, tile, lds, th, threadIdx.x);
   double delta = 0.0;
   const int s1 = T::last_step(tile);
-  // groups of RB steps: guarded ones until the first that fits the
-  // guard-free range, the guard-free ones, then guarded ones to the end
-  int s = T::first_step(tile);
-  for (; s <= s1 && !T::group_steady(a, tile, s); s += T::RB) pair_group<T, false>(a, tile, lds, th, s, s1, check, delta);
-  for (; s <= s1 && T::group_steady(a, tile, s); s += T::RB) pair_group<T, true>(a, tile, lds, th, s, s1, check, delta);
-  for (; s <= s1; s += T::RB) pair_group<T, false>(a, tile, lds, th, s, s1, check, delta);
+  // groups of RB steps, guard-free where the whole group fits the range
+  // that needs no guard
+  for (int s = T::first_step(tile); s <= s1; s += T::RB) {
+    if (T::group_steady(a, tile, s))
+      pair_group<T, true>(a, tile, lds, th, s, s1, check, delta);
+    else
+      pair_group<T, false>(a, tile, lds, th, s, s1, check, delta);
+  }
   if (check) wg_max_store(delta, partials);
 }
 

@@ -9,7 +9,7 @@ from parity import compare, run_oracle, small_cases
 pytestmark = pytest.mark.gpu
 
 # the library's default SOR variant, restored after each test
-DEFAULT_VARIANT = 21
+DEFAULT_VARIANT = 26
 
 
 def n_variants():
