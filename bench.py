@@ -57,8 +57,8 @@ SOR_TRAFFIC_MEASURED = {
     # of the kernel with that many iterations per pass (profiles/sor_sphere256_hbm_traffic.md)
     (256, "cell lists", 64, 32, 512, 1, 2, 32): 5.7318e+08,
     (256, "cell lists", 64, 32, 512, 1, 1, 32): 4.2695e+08,
-    (256, "pair-owned", 64, 32, 512, 1, 2, 32): 4.7125e+08,
-    (256, "pair-owned", 64, 32, 512, 1, 1, 32): 4.2817e+08,
+    (256, "pair-owned, LDS coefficients", 64, 32, 512, 1, 2, 32): 4.7172e+08,
+    (256, "pair-owned, LDS coefficients", 64, 32, 512, 1, 1, 32): 4.2833e+08,
 }
 
 
