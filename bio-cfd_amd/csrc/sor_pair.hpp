@@ -115,7 +115,10 @@ struct SorPairTile {
   static constexpr int SLOTS = R * 2 * SH * SW;
   static constexpr int CZ = SLOTS;                  // Acz ring: 3 per slot plane
   static constexpr int SINK = CZ + 3 * R;           // written by load entries past the slot plane
-  static constexpr int CXW = PW - 2, CYW = SH - 2;  // staged Acx (Acy) of slot columns (rows) 1 .. PW-2 (SH-2)
+  // staged Acx of every slot column, the even columns then the odd ones (a
+  // wavefront's pairs, consecutive in h, read consecutive words), and Acy
+  // of slot rows 1 .. SH-2
+  static constexpr int CXW = PW, CYW = SH - 2;
   static constexpr int CX = SINK + 1, CY = CX + 3 * CXW;
   static constexpr int LDS_DOUBLES = CREG ? SINK + 1 : CY + 3 * CYW;
   // mask bits of a pair: b * NPH + p = in steps of class b (b = (s - first
@@ -367,13 +370,14 @@ struct SorPairTile {
     }
   }
 
-  // without CREG: Acx over slot columns 1..PW-2 and Acy over slot rows
-  // 1..SH-2 into LDS (zero outside the interior, where no cell is updated)
+  // without CREG: Acx over every slot column (even columns, then odd) and
+  // Acy over slot rows 1..SH-2 into LDS (zero outside the interior, where no
+  // cell is updated)
   __host__ __device__ static void load_coef(const Args &a, const Tile &T, Real *lds, int tid) {
     if constexpr (!CREG) {
       for (int e = tid; e < 3 * CXW + 3 * CYW; e += NT) {
         if (e < 3 * CXW) {
-          const int m = e / CXW, i = T.iL + 1 + (e - m * CXW);
+          const int m = e / CXW, r = e - m * CXW, i = T.iL + (r < SW ? 2 * r : 2 * (r - SW) + 1);
           lds[CX + e] = (i >= 2 && i <= a.d.nx + 1) ? Real(a.Ax[(i - 2) + m * a.d.nx]) : Real(0);
         } else {
           const int f = e - 3 * CXW, m = f / CYW, j = T.jL + 1 + (f - m * CYW);
@@ -457,7 +461,7 @@ struct SorPairTile {
         aym = th.cym[q];
         ayp = th.cyp[q];
       } else {
-        const Real *cx = lds + CX + 2 * (int)((m >> 16) & 255u) + par - 1;
+        const Real *cx = lds + CX + par * SW + (int)((m >> 16) & 255u);  // column 2h + par
         const Real *cy = lds + CY + (int)(m >> 24) - 1;
         axm = cx[0];
         axp = cx[2 * CXW];
