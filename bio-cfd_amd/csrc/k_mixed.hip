@@ -24,11 +24,13 @@
 
 namespace bcfd {
 
-// tile of a residual workgroup: 64 x 8 columns (256 threads, rows ty and
-// ty + 4), marching kResPlanes planes
-constexpr int kResTI = 64, kResTJ = 8, kResPlanes = 16;
+// tile of a residual workgroup: 256 x 8 columns (1024 threads, rows ty and
+// ty + 4), marching kResPlanes planes.  Wide in i: each side-halo cell of a
+// row costs a whole cache line of pc and of e
+constexpr int kResTI = 256, kResTJ = 8, kResPlanes = 32, kResNT = 4 * kResTI;
 constexpr int kResPW = kResTI + 2, kResSH = kResTJ + 2;       // plane footprint with a one-cell halo
-constexpr int kResHalo = 2 * kResPW + 2 * kResTJ;             // 148 halo cells, one per thread < 148
+constexpr int kResHalo = 2 * kResPW + 2 * kResTJ;             // halo cells, one per thread < kResHalo
+static_assert(kResHalo <= kResNT, "one halo cell per thread");
 
 // With kUpdate: pc' = pc + e (every interior cell; e is zero off the fluid)
 // is written to pc_out and the residual is taken of pc'; without, of pc.
@@ -43,7 +45,7 @@ constexpr int kResHalo = 2 * kResPW + 2 * kResTJ;             // 148 halo cells,
 // registers.  HBM traffic per cell: pc 8, e 4, b/dt 8 read, pc' 8, r 4
 // written; the halo is re-read from L2.
 template <bool kUpdate>
-__global__ void __launch_bounds__(256) k_mx_resupd(Dims d, int NXh, const double *__restrict__ pc,
+__global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const double *__restrict__ pc,
                                                    const float *__restrict__ e, double *__restrict__ pc_out,
                                                    const double *__restrict__ bred, const double *__restrict__ bblack,
                                                    const double *__restrict__ Ax, const double *__restrict__ Ay,
@@ -52,7 +54,7 @@ __global__ void __launch_bounds__(256) k_mx_resupd(Dims d, int NXh, const double
                                                    double *__restrict__ partials) {
   if (st->done) return;
   __shared__ double pl[2][kResSH][kResPW];
-  const int tid = threadIdx.x, tx = tid & 63, ty = tid >> 6;
+  const int tid = threadIdx.x, tx = tid % kResTI, ty = tid / kResTI;
   const int iL = 2 + blockIdx.x * kResTI, jL = 2 + blockIdx.y * kResTJ;
   const int k0 = 2 + blockIdx.z * kResPlanes;
   const int k1 = k0 + kResPlanes - 1 < d.nz + 1 ? k0 + kResPlanes - 1 : d.nz + 1;
@@ -318,7 +320,7 @@ int split_bdt(Block &b, double dt);
 // residual of pc (src, with e added when e != nullptr, written to dst)
 static int64_t residual_launch(Block &b, const double *src, const float *e, double *dst, const SorState *st) {
   const int NXh = (b.d.NX + 1) / 2;
-  dim3 blk(256, 1, 1), grd(div_up(b.d.nx, kResTI), div_up(b.d.ny, kResTJ), div_up(b.d.nz, kResPlanes));
+  dim3 blk(kResNT, 1, 1), grd(div_up(b.d.nx, kResTI), div_up(b.d.ny, kResTJ), div_up(b.d.nz, kResPlanes));
   if (e)
     hipLaunchKernelGGL(k_mx_resupd<true>, grd, blk, 0, rt().stream, b.d, NXh, src, e, dst, b.bs[1], b.bs[0], b.A[0],
                        b.A[1], b.A[2], b.r32[1], b.r32[0], st, b.partials);
