@@ -301,49 +301,106 @@ __global__ void k_ts_defer(Dims d, const uint8_t *__restrict__ cell, const uint8
   deferred[c] = r;
 }
 
+// The ghost entry each target of a TS-adjacent deferred cell takes: the
+// last TS cell (in TS order) whose element test at the target's position is
+// negative, 0 for none (best[36 q + 6 dq + t] for list entry q, direction
+// dq, target t).  It depends on positions and the surface only, not on the
+// field values, so all cells are done at once, one workgroup per entry.
+__global__ void __launch_bounds__(256) k_ts_best(MomArgs a, const int *list, int n_list, TsArgs ts,
+                                                 int *__restrict__ best) {
+  __shared__ int s_best[36];
+  const Dims &d = a.d;
+  const int q = blockIdx.x;
+  if (q >= n_list) return;
+  const int lin = list[q];
+  const int i = 2 + lin % d.nx;
+  const int j = 2 + (lin / d.nx) % d.ny;
+  const int k = 2 + lin / (d.nx * d.ny);
+  unsigned m = 0;
+  if (!ts_adjacent(d, a.cell2, i, j, k, &m)) return;
+  if (threadIdx.x < 36) s_best[threadIdx.x] = 0;
+  __syncthreads();
+  const double xs[7] = {ts.x1[i], ts.x1[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1]};
+  const double ys[7] = {ts.yp[j - 1], ts.yp[j - 1], ts.y1[j], ts.y1[j - 1], ts.yp[j - 1], ts.yp[j - 1], ts.yp[j - 1]};
+  const double zs[7] = {ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.z1[k], ts.z1[k - 1], ts.zp[k - 1]};
+  // the seven tests in the order u2,u1,v2,v1,w2,w1,p mapped to BN_* slots
+  const int test_nel[7] = {BN_U2, BN_U1, BN_V2, BN_V1, BN_W2, BN_W1, BN_P};
+  for (int nn = 1 + threadIdx.x; nn <= ts.n_ts; nn += blockDim.x) {
+    const int bt = ts.index_ts[nn - 1];
+    bool neg[BN_NNEL];
+    for (int t = 0; t < 7; ++t) {
+      const int e = ts.nel[test_nel[t]][bt - 1];
+      const double dn = (xs[t] - ts.elem[BE_XCENT][e - 1]) * ts.elem[BE_COSA][e - 1] +
+                        (ys[t] - ts.elem[BE_YCENT][e - 1]) * ts.elem[BE_COSB][e - 1] +
+                        (zs[t] - ts.elem[BE_ZCENT][e - 1]) * ts.elem[BE_COSG][e - 1];
+      neg[test_nel[t]] = dn < 0;
+    }
+    for (int dq = 0; dq < 6; ++dq) {
+      if (!(m & (1u << dq))) continue;
+      for (int t = 0; t < 6; ++t)
+        if (neg[kTargets[dq][t].test]) atomicMax(&s_best[dq * 6 + t], nn);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 36) best[36 * (int64_t)q + threadIdx.x] = s_best[threadIdx.x];
+}
+
 // Ordered pass over the deferred cells (list in fluid order), one workgroup.
+// Only a TS-adjacent cell changes u, v, w, p (its ghost substitution and the
+// restore after its stencil); the cells between two TS-adjacent ones read
+// the same field state and write only their own ut, vt, wt, so such a run
+// is evaluated in parallel, a window of up to blockDim cells at a time, and
+// each TS-adjacent cell in turn as in the reference's loop.
 __global__ void __launch_bounds__(256) k_ts_ordered(MomArgs a, double *u, double *v, double *w, double *p,
                                                     double *ut, double *vt, double *wt, const int *list,
-                                                    int n_list, TsArgs ts) {
-  __shared__ int s_best[36];
-  __shared__ double s_val[36];
+                                                    int n_list, TsArgs ts, const int *__restrict__ best) {
+  __shared__ int s_first;
   const Dims &d = a.d;
   double *fld[4] = {u, v, w, p};
-  for (int q = 0; q < n_list; ++q) {
+  auto decode = [&](int q, int &i, int &j, int &k) {
     const int lin = list[q];
-    const int i = 2 + lin % d.nx;
-    const int j = 2 + (lin / d.nx) % d.ny;
-    const int k = 2 + lin / (d.nx * d.ny);
+    i = 2 + lin % d.nx;
+    j = 2 + (lin / d.nx) % d.ny;
+    k = 2 + lin / (d.nx * d.ny);
+  };
+  int q = 0;
+  while (q < n_list) {
+    // the first TS-adjacent cell in the window [q, q + blockDim)
+    if (threadIdx.x == 0) s_first = n_list;
+    __syncthreads();
+    const int qq = q + (int)threadIdx.x;
+    int ci = 0, cj = 0, ck = 0;
+    unsigned cm = 0;
+    if (qq < n_list) {
+      decode(qq, ci, cj, ck);
+      if (ts_adjacent(d, a.cell2, ci, cj, ck, &cm)) atomicMin(&s_first, qq);
+    }
+    __syncthreads();
+    const int first = s_first;
+    const int wend = q + (int)blockDim.x < n_list ? q + (int)blockDim.x : n_list;
+    const int run_end = first < wend ? first : wend;
+    // the run of cells that are not TS-adjacent
+    if (qq < run_end) {
+      Fields F{u, v, w, p};
+      double ou, ov, ow;
+      momentum_cell(a, F, ci, cj, ck, ou, ov, ow);
+      const int64_t c = ix3(d, ci, cj, ck);
+      ut[c] = ou;
+      vt[c] = ov;
+      wt[c] = ow;
+    }
+    __syncthreads();
+    q = run_end;
+    if (q != first || q >= n_list) continue;
+    // the TS-adjacent cell q
+    int i, j, k;
+    decode(q, i, j, k);
     unsigned m = 0;
     const bool adj = ts_adjacent(d, a.cell2, i, j, k, &m);
     if (adj) {
-      if (threadIdx.x < 36) s_best[threadIdx.x] = 0;
-      __syncthreads();
-      const double xs[7] = {ts.x1[i], ts.x1[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1], ts.xp[i - 1]};
-      const double ys[7] = {ts.yp[j - 1], ts.yp[j - 1], ts.y1[j], ts.y1[j - 1], ts.yp[j - 1], ts.yp[j - 1], ts.yp[j - 1]};
-      const double zs[7] = {ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.z1[k], ts.z1[k - 1], ts.zp[k - 1]};
-      // the seven tests in the order u2,u1,v2,v1,w2,w1,p mapped to BN_* slots
-      const int test_nel[7] = {BN_U2, BN_U1, BN_V2, BN_V1, BN_W2, BN_W1, BN_P};
-      for (int nn = 1 + threadIdx.x; nn <= ts.n_ts; nn += blockDim.x) {
-        const int bt = ts.index_ts[nn - 1];
-        bool neg[BN_NNEL];
-        for (int t = 0; t < 7; ++t) {
-          const int e = ts.nel[test_nel[t]][bt - 1];
-          const double dn = (xs[t] - ts.elem[BE_XCENT][e - 1]) * ts.elem[BE_COSA][e - 1] +
-                            (ys[t] - ts.elem[BE_YCENT][e - 1]) * ts.elem[BE_COSB][e - 1] +
-                            (zs[t] - ts.elem[BE_ZCENT][e - 1]) * ts.elem[BE_COSG][e - 1];
-          neg[test_nel[t]] = dn < 0;
-        }
-        for (int dq = 0; dq < 6; ++dq) {
-          if (!(m & (1u << dq))) continue;
-          for (int t = 0; t < 6; ++t)
-            if (neg[kTargets[dq][t].test]) atomicMax(&s_best[dq * 6 + t], nn);
-        }
-      }
-      __syncthreads();
       if (threadIdx.x < 36) {
         const int dq = threadIdx.x / 6, t = threadIdx.x % 6;
-        const int nn = s_best[threadIdx.x];
+        const int nn = best[36 * (int64_t)q + threadIdx.x];
         if ((m & (1u << dq)) && nn > 0) {
           const Target T = kTargets[dq][t];
           fld[T.f][ix3(d, i + T.di, j + T.dj, k + T.dk)] = ts.ghost[T.g][nn - 1];
@@ -370,6 +427,7 @@ __global__ void __launch_bounds__(256) k_ts_ordered(MomArgs a, double *u, double
       }
     }
     __syncthreads();
+    ++q;
   }
 }
 
@@ -427,12 +485,17 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   ts.x1 = b->g[BG_X1]; ts.y1 = b->g[BG_Y1]; ts.z1 = b->g[BG_Z1];
   ts.xp = b->g[BG_XP]; ts.yp = b->g[BG_YP]; ts.zp = b->g[BG_ZP];
   ts.n_ts = n_ts;
-  if (cnt > 0)
+  int *best = nullptr;
+  if (cnt > 0) {
+    if (dev_alloc(&best, 36 * cnt)) return -1;
+    hipLaunchKernelGGL(k_ts_best, dim3((unsigned)cnt), dim3(256), 0, s, a, list, (int)cnt, ts, best);
     hipLaunchKernelGGL(k_ts_ordered, dim3(1), dim3(256), 0, s, a, b->f[BF_U], b->f[BF_V], b->f[BF_W],
-                       b->f[BF_P], b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, ts);
+                       b->f[BF_P], b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, ts, best);
+  }
   BCFD_CHECK_LAUNCH();
   BCFD_HIP(hipStreamSynchronize(s));
   (void)hipFree(list);
+  (void)hipFree(best);
   (void)hipFree(touched);
   (void)hipFree(deferred);
   return 0;
