@@ -13,7 +13,7 @@ all fields resident in HBM.
 
 Prints one JSON line.  `value` is the FP64 method's Mcell-steps/s (one GPU);
 `mixed` holds the mixed-precision run of the same steps on the same inputs
-(FP32 SOR smoothing, 32 sweeps per FP64 defect correction, the same sweep
+(FP32 SOR smoothing, 64 sweeps per FP64 defect correction, the same sweep
 budget), its speed-up and its per-field error against the FP64 run.
 `roofline` describes the FP64 red-black SOR iteration (the dominant kernel)
 measured with HIP events on the runtime's stream; `cpu_baseline` times the
@@ -57,7 +57,7 @@ SOR_TRAFFIC_MEASURED = {
     # of the kernel with that many iterations per pass (profiles/sor_sphere256_hbm_traffic.md)
     (256, "cell lists", 64, 32, 512, 1, 2, 32): 5.7318e+08,
     (256, "cell lists", 64, 32, 512, 1, 1, 32): 4.2695e+08,
-    (256, "pair-owned, LDS coefficients", 64, 32, 512, 1, 2, 32): 4.7172e+08,
+    (256, "pair-owned, LDS coefficients", 64, 32, 512, 1, 2, 32): 4.6074e+08,
     (256, "pair-owned, LDS coefficients", 64, 32, 512, 1, 1, 32): 4.2833e+08,
 }
 
@@ -252,7 +252,7 @@ def main():
     ap.add_argument("--epsi", type=float, default=1e-8,
                     help="SOR tolerance: update norm (FP64 method) / residual max-norm (mixed)")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--sweeps", type=int, default=32, help="FP32 sweeps per defect correction (mixed)")
+    ap.add_argument("--sweeps", type=int, default=64, help="FP32 sweeps per defect correction (mixed)")
     ap.add_argument("--sor-variant", default="auto",
                     help="FP64 SOR kernel variant (0 = colour-split sweeps, >=1 fused tiles) or 'auto'")
     ap.add_argument("--sor-planes", type=int, default=32, help="planes per workgroup of a fused variant")
