@@ -11,12 +11,13 @@
 // parity bit: its slot address, its neighbours (W, E at index - 1 + par and
 // + par of the other colour, S, N at -SW, +SW, B, T on the neighbour planes)
 // and its coefficients follow from it.  With CREG the coefficients (Acx of
-// both columns, Acy of the row and the partial sums Acx(c) + Acy(c)) sit in
-// registers from the start; without, Acx and Acy of the tile are staged in
-// LDS (fewer registers, so more workgroups per CU).
-// Whether the cell is updated (inside the phase's region and the interior
-// in i and j) is a precomputed bit per (phase, parity); a pair whose cell is
-// not updated writes its own value back, so the phase has no branches.  The
+// the cell each step class updates, Acy of the row and the partial sums
+// Acx(c) + Acy(c)) sit in registers from the start; without, Acx (even
+// columns, then odd ones) and Acy of the tile are staged in LDS (fewer
+// registers).  Whether the cell is updated (inside the phase's region and
+// the interior in i and j) is a precomputed bit per (phase, step class); a
+// pair whose cell is not updated writes its own value back, so the phase
+// has no branches.  The
 // pairs are numbered from the tile outwards, ring by ring of the shrinking
 // phase regions, so the pairs a phase works on are a prefix of the
 // numbering: a thread's pair slots past it drop out at compile time.
@@ -26,14 +27,15 @@
 // slot plane, so within a step every LDS word a thread reads is one it wrote
 // itself or one written in an earlier step: the cell above a phase's cell
 // (plane s+1 for phase 0, the cell phase p-1 just updated for phase p) is
-// the same column of the same pair, kept in a register; the cell below is
+// the same column of the same pair, read back from the thread's own store
+// at the step's start (phase 0) or kept in a register; the cell below is
 // the pair's own, updated by the thread's next phase; the four neighbours
 // in the plane were written a step earlier.  So the phases of a step need
 // no barrier between them, only the step boundary does (it orders a step's
 // writes before the next step's reads and the ring slot's reuse).
 // The rhs of step s+1's first two phases is loaded at the start of step s,
-// plane s+2 right after phase 0 (which still reads plane s+1 from the
-// registers), into registers indexed by the step number.  With two iterations per launch each cell's rhs is read once:
+// plane s+2 right after phase 0, into registers indexed by the step
+// number.  With two iterations per launch each cell's rhs is read once:
 // phase p + 2 of step s + 2 updates the cells phase p updated at step s
 // (same plane k = s - p, same colour, and, the pairs being numbered by
 // phase region, the same threads and slots), so it reuses the registers
