@@ -25,7 +25,8 @@ workload.
 plate; cases.config_uneven), FP64, run twice on the same inputs: the fine
 blocks' SOR calls batched in one launch per pass, then one block after the
 other; one JSON line with both throughputs, the speed-up and whether the two
-runs agree bit for bit.
+runs agree bit for bit.  With --sor-variant auto (the default) the SOR
+variant is picked by timing one batched step of the workload per candidate.
 """
 from __future__ import annotations
 
@@ -150,7 +151,36 @@ def run_path(args, precision: str, keep_fields: bool):
     return out
 
 
-def run_uneven(args, batching: int):
+def tune_uneven(args):
+    """The FP64 SOR variant for the uneven workload: one batched time step
+    with each two-iteration pair-owned variant (after one set-up step), the
+    fastest wins.  Every variant gives the same bits, so the steps only
+    differ in time."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd import tune
+    from biocfd_amd.driver import Simulation
+
+    cands = [t for t in tune.variants() if t[7] in (1, 2) and t[6] == 2]
+    case = CS.config_uneven(nsteps=1 + len(cands))
+    sim = Simulation(case)
+    H.call("biocfd_sor_batching", 1)
+    sim.step()
+    table = []
+    for t in cands:
+        H.call("biocfd_sor_config", t[0], args.sor_planes)
+        H.call("biocfd_synchronize")
+        t0 = time.perf_counter()
+        sim.step()
+        H.call("biocfd_synchronize")
+        table.append({"variant": t[0], "tile": [t[1], t[2]], "threads": t[3],
+                      "ms_per_step": round(1e3 * (time.perf_counter() - t0), 2)})
+    sim.close()
+    best = min(table, key=lambda r: r["ms_per_step"])
+    return best["variant"], table
+
+
+def run_uneven(args, batching: int, variant: int):
     from biocfd_amd import cases as CS
     from biocfd_amd import hip as H
     from biocfd_amd.driver import Simulation
@@ -158,8 +188,7 @@ def run_uneven(args, batching: int):
     case = CS.config_uneven(nsteps=args.warmup + args.steps)
     sim = Simulation(case)
     H.call("biocfd_sor_batching", batching)
-    # the pair-owned 64x32 tile with LDS coefficients unless a variant is named
-    H.call("biocfd_sor_config", 26 if args.sor_variant == "auto" else int(args.sor_variant), args.sor_planes)
+    H.call("biocfd_sor_config", variant, args.sor_planes)
     for _ in range(args.warmup):
         sim.step()
     H.call("biocfd_synchronize")
@@ -177,8 +206,13 @@ def run_uneven(args, batching: int):
 
 
 def uneven_main(args):
-    b, fb, case = run_uneven(args, 1)
-    p, fp, _ = run_uneven(args, 0)
+    if args.sor_variant == "auto":
+        variant, table = tune_uneven(args)
+        print(json.dumps({"uneven_autotune": table}), file=sys.stderr)
+    else:
+        variant = int(args.sor_variant)
+    b, fb, case = run_uneven(args, 1, variant)
+    p, fp, _ = run_uneven(args, 0, variant)
     same = all(np.array_equal(x, y) for x, y in zip(fb, fp))
     sizes = [int(blk.n[0]) * int(blk.n[1]) * int(blk.n[2]) for blk in case.blocks]
     print(json.dumps({
@@ -186,6 +220,7 @@ def uneven_main(args):
                     + ",".join(str(int(blk.n[0])) for blk in case.blocks[1:])
                     + "), heaving/pitching plate, Re=200, dt=5e-4, pcItaMax=500, epsi=1e-8, FP64",
         "cells": case.cells(), "fine_cells": sum(sizes[1:]), "steps": args.steps, "warmup": args.warmup,
+        "sor_variant": variant, "autotuned": args.sor_variant == "auto",
         "batched": b, "per_block": p, "speedup": round(b["mcell_steps_per_s"] / p["mcell_steps_per_s"], 3),
         "bit_identical": bool(same)}))
 

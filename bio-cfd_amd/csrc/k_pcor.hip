@@ -497,6 +497,8 @@ static const FusedVariant kFused[] = {
     PV(64, 32, 1024, 1, 1, 0),
     PV(64, 22, 512, 1, 2, 1),
     PV(64, 22, 512, 1, 2, 0),
+    PV(32, 32, 512, 1, 2, 1),
+    PV(32, 32, 512, 1, 2, 0),
 };
 #undef FV
 #undef PV

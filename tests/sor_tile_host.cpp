@@ -408,6 +408,10 @@ int main() {
   bad += run_pair<double, 64, 32, 1024, 1, 1>();
   bad += run_pair<double, 128, 16, 1024, 2, 0>();
   bad += run_pair<double, 128, 16, 1024, 1, 0>();
+  bad += run_pair<double, 32, 32, 512, 2, 1>();
+  bad += run_pair<double, 32, 32, 512, 1, 1>();
+  bad += run_pair<double, 32, 32, 512, 2, 0>();
+  bad += run_pair<double, 32, 32, 512, 1, 0>();
   bad += run_pair<double, 64, 22, 512, 2, 1>();
   bad += run_pair<double, 64, 22, 512, 1, 1>();
   bad += run_pair<double, 64, 22, 512, 2, 0>();
