@@ -13,7 +13,7 @@ all fields resident in HBM.
 
 Prints one JSON line.  `value` is the FP64 method's Mcell-steps/s (one GPU);
 `mixed` holds the mixed-precision run of the same steps on the same inputs
-(FP32 SOR smoothing, 64 sweeps per FP64 defect correction, the same sweep
+(FP32 SOR smoothing, 128 sweeps per FP64 defect correction, the same sweep
 budget), its speed-up and its per-field error against the FP64 run.
 `roofline` describes the FP64 red-black SOR iteration (the dominant kernel)
 measured with HIP events on the runtime's stream; `cpu_baseline` times the
@@ -91,6 +91,7 @@ def run_path(args, precision: str, keep_fields: bool):
         H.call("biocfd_init", 0)
         H.call("biocfd_copy_probe", int(args.copy_probe_mb) << 20, 4)
     sim = Simulation(case, precision=precision)
+    H.call("biocfd_momentum_config", args.momentum_variant, 0 if args.momentum_divisions else 1)
     if precision == "fp64":
         if args.sor_variant == "auto":
             v, planes, table = tune.autotune(block=1, dt=case.dt)
@@ -287,12 +288,17 @@ def main():
     ap.add_argument("--epsi", type=float, default=1e-8,
                     help="SOR tolerance: update norm (FP64 method) / residual max-norm (mixed)")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--sweeps", type=int, default=64, help="FP32 sweeps per defect correction (mixed)")
+    ap.add_argument("--sweeps", type=int, default=128, help="FP32 sweeps per defect correction (mixed)")
     ap.add_argument("--sor-variant", default="auto",
                     help="FP64 SOR kernel variant (0 = colour-split sweeps, >=1 fused tiles) or 'auto'")
     ap.add_argument("--sor-planes", type=int, default=32, help="planes per workgroup of a fused variant")
     ap.add_argument("--mixed-variant", default="auto", help="FP32 sweep kernel variant of the mixed solve or 'auto'")
     ap.add_argument("--mixed-planes", type=int, default=16, help="planes per workgroup of the FP32 sweep kernel")
+    ap.add_argument("--momentum-variant", type=int, default=0,
+                    help="dense momentum kernel: 1 = one launch per velocity component, 0 = all three per thread, "
+                         "2 = as 0 with four waves per SIMD")
+    ap.add_argument("--momentum-divisions", action="store_true",
+                    help="keep every grid division of the momentum stencil a division (no power-of-two products)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a plain device copy of this many MiB (byte-counter calibration)")

@@ -92,6 +92,10 @@ struct IbData {
   void *bin_grid = nullptr;
   int *bin_start = nullptr, *bin_elem = nullptr, *bin_fill = nullptr;
   bool bins_ready = false;
+  // one bit per cell of the block: the neighbour cells selective retagging
+  // has claimed in the current call (all clear between calls)
+  unsigned *retag_bits = nullptr;
+  int64_t retag_words = 0;
   // per IB cell (capacity grows)
   int64_t cap_ib = 0;
   int64_t n_mapped = 0;                       // entries of ib_ijk currently in ib_map

@@ -277,7 +277,7 @@ struct MixedConfig {
   int precision = 0;        // 0: FP64 method, 1: mixed
   int variant = 21;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 1024 threads, 2 sweeps per pass)
   int planes = 32;
-  int sweeps = 64;          // FP32 sweeps per correction (u, v, w within 2.1e-7 of FP64 after 100 steps at 256^3)
+  int sweeps = 128;         // FP32 sweeps per correction (u, v, w within 4.2e-7 of FP64 after 100 steps at 256^3)
   double tol = 1e-8;        // max-norm of the FP64 residual
 };
 static MixedConfig g_mx;
@@ -470,7 +470,7 @@ int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_co
     return -1;
   }
   if (planes_per_workgroup < 1 || planes_per_workgroup > 4096 || sweeps_per_correction < 1 ||
-      sweeps_per_correction > 64) {
+      sweeps_per_correction > 1024) {
     set_error("mixed_config: planes %d / sweeps %d out of range", planes_per_workgroup, sweeps_per_correction);
     return -1;
   }

@@ -12,6 +12,8 @@
 // a position that any TS-adjacent cell writes are processed afterwards, one
 // by one in list order, by a single workgroup; every other fluid cell reads
 // only untouched values and runs in the dense parallel kernel.
+#include <cmath>
+
 #include "biocfd_rt.hpp"
 
 namespace bcfd {
@@ -28,17 +30,6 @@ struct Fields {
   const double *u, *v, *w, *p;
 };
 
-__device__ inline bool adj_ok(const Dims &d, const uint8_t *t, bool tall, int i, int j, int k, int s) {
-  auto at = [&](int a, int b, int c) { return tall ? t[ixt(d, a, b, c)] : t[ix3(d, a, b, c)]; };
-  if (at(i - s, j, k) == 2) return false;
-  if (at(i + s, j, k) == 2) return false;
-  if (at(i, j - s, k) == 2) return false;
-  if (at(i, j + s, k) == 2) return false;
-  if (at(i, j, k - s) == 2) return false;
-  if (at(i, j, k + s) == 2) return false;
-  return true;
-}
-
 // vel*(ca1 f+2 + ca2 f+1 + ca3 f0 + ca4 f-1 + ca5 f-2)/(ca6 hA)
 //   + |vel|*(ck1 f+2 + ... + ck5 f-2)/(2 ck6 hB)       (NavierStokes.f90:619-647)
 __device__ inline double upwind3(double vel, const double *ca, const double *ck, int col, double f2,
@@ -49,172 +40,285 @@ __device__ inline double upwind3(double vel, const double *ca, const double *ck,
          fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2) / (2.0 * c[5] * hB);
 }
 
-__device__ void momentum_cell(const MomArgs &a, const Fields &F, int i, int j, int k, double &out_u,
-                              double &out_v, double &out_w) {
-  const Dims &d = a.d;
-  const int64_t c = ix3(d, i, j, k);
-  const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
-  const double *u = F.u, *v = F.v, *w = F.w, *p = F.p;
-  const double alpha = 1.0;
-#define U(di, dj, dk) u[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
-#define V(di, dj, dk) v[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
-#define W(di, dj, dk) w[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
-#define P(di, dj, dk) p[c + (di) + (int64_t)(dj) * sy + (int64_t)(dk) * sz]
-  const double dxr = a.dx[i], dx = a.dx[i - 1], dxl = a.dx[i - 2];
-  const double dye = a.dy[j], dy = a.dy[j - 1], dyw = a.dy[j - 2];
-  const double dzt = a.dz[k], dz = a.dz[k - 1], dzb = a.dz[k - 2];
+// Grid spacings around cell (i,j,k) and its linear offset, shared by the
+// three component evaluations.
+struct MomCell {
+  int64_t c, sy, sz;
+  double dxr, dx, dxl, dye, dy, dyw, dzt, dz, dzb;
+};
 
+__device__ inline MomCell mom_cell(const MomArgs &a, int i, int j, int k) {
+  const Dims &d = a.d;
+  MomCell m;
+  m.c = ix3(d, i, j, k);
+  m.sy = d.NX;
+  m.sz = (int64_t)d.NX * d.NY;
+  m.dxr = a.dx[i]; m.dx = a.dx[i - 1]; m.dxl = a.dx[i - 2];
+  m.dye = a.dy[j]; m.dy = a.dy[j - 1]; m.dyw = a.dy[j - 2];
+  m.dzt = a.dz[k]; m.dz = a.dz[k - 1]; m.dzb = a.dz[k - 2];
+  return m;
+}
+
+// no IB cell (cell == 2) at distance 1 and no TS cell (cell2 == 2) at
+// distance 2 (check_adjacent_cell, NavierStokes.f90:1002-1022) inside the
+// index windows.  The reference tests the twelve tags one after the other;
+// here all twelve are loaded before any is tested (independent loads, one
+// wait instead of twelve dependent round trips), which gives the same value.
+// The cell2 reads two cells below are clamped to index 1: where i, j or
+// k == 2 the window test fails whatever they hold.
+__device__ inline bool mom_near_ok(const MomArgs &a, int i, int j, int k) {
+  const Dims &d = a.d;
+  const uint8_t *t1 = a.cell, *t2 = a.cell2;
+  const int im = i > 2 ? i - 2 : 1, jm = j > 2 ? j - 2 : 1, km = k > 2 ? k - 2 : 1;
+  const unsigned hit1 = (t1[ix3(d, i - 1, j, k)] == 2) | (t1[ix3(d, i + 1, j, k)] == 2) |
+                        (t1[ix3(d, i, j - 1, k)] == 2) | (t1[ix3(d, i, j + 1, k)] == 2) |
+                        (t1[ix3(d, i, j, k - 1)] == 2) | (t1[ix3(d, i, j, k + 1)] == 2);
+  const unsigned hit2 = (t2[ixt(d, im, j, k)] == 2) | (t2[ixt(d, i + 2, j, k)] == 2) |
+                        (t2[ixt(d, i, jm, k)] == 2) | (t2[ixt(d, i, j + 2, k)] == 2) |
+                        (t2[ixt(d, i, j, km)] == 2) | (t2[ixt(d, i, j, k + 2)] == 2);
+  return (i > 2) & (j > 2) & (k > 2) & ((hit1 | hit2) == 0u);
+}
+
+// x / y for a grid-derived divisor y.  P2: every axis of the block has one
+// power-of-two spacing, so every grid divisor here (a spacing, the sum of
+// two, or half of such a sum) is a power of two.  Then 1/y is exact and
+// x * (1/y) is the same real number as x / y, rounded once: the same bits,
+// subnormal results included, for about a tenth of the instructions.
+template <bool P2>
+__device__ inline double gdiv(double x, double y) {
+  if constexpr (P2) {
+    // y = 2^e with biased exponent E: 1/y has biased exponent 2046 - E
+    const long long ey = __double_as_longlong(y) & 0x7FF0000000000000ll;
+    return x * __longlong_as_double(0x7FE0000000000000ll - ey);
+  } else {
+    return x / y;
+  }
+}
+
+#define U(di, dj, dk) F.u[m.c + (di) + (int64_t)(dj) * m.sy + (int64_t)(dk) * m.sz]
+#define V(di, dj, dk) F.v[m.c + (di) + (int64_t)(dj) * m.sy + (int64_t)(dk) * m.sz]
+#define W(di, dj, dk) F.w[m.c + (di) + (int64_t)(dj) * m.sy + (int64_t)(dk) * m.sz]
+#define P(di, dj, dk) F.p[m.c + (di) + (int64_t)(dj) * m.sy + (int64_t)(dk) * m.sz]
+
+// ut (NavierStokes.f90:600-688)
+template <bool P2>
+__device__ inline double mom_u(const MomArgs &a, const Fields &F, const MomCell &m, int i, int j, int k,
+                               bool near_ok) {
+  const double alpha = 1.0;
+  const double dxr = m.dxr, dx = m.dx, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
   const double u1a = U(-1, 0, 0) + U(0, 0, 0), u22 = U(-1, 0, 0) - U(0, 0, 0);
   const double u3 = U(0, 0, 0) + U(1, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
   const double u5 = U(0, -1, 0) + U(0, 0, 0), u6 = U(0, -1, 0) - U(0, 0, 0);
   const double u7 = U(0, 0, 0) + U(0, 1, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
   const double u9 = U(0, 0, -1) + U(0, 0, 0), u10 = U(0, 0, -1) - U(0, 0, 0);
   const double u11 = U(0, 0, 0) + U(0, 0, 1), u12 = U(0, 0, 0) - U(0, 0, 1);
-  const double u13 = U(-1, 0, 0) + U(-1, 1, 0), u14 = u7;
-  const double u15 = U(-1, 0, 0) + U(-1, 0, 1), u16 = u11;
-
   const double v1a = V(0, -1, 0) + V(1, -1, 0), v22 = V(0, 0, 0) + V(1, 0, 0);
+  const double w1a = W(0, 0, -1) + W(1, 0, -1), w22 = W(0, 0, 0) + W(1, 0, 0);
+  const double dpdx = gdiv<P2>(P(0, 0, 0) - P(1, 0, 0), 0.5 * (dxr + dx));
+  const double dx2xr = dx + dxr, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt, dz2zb = dz + dzb;
+  const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+  double duudx, dvudy, dwudz;
+  const double vu_e = V(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, 0, 0) - V(1, 0, 0));
+  const double vu_w = V(1, -1, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, -1, 0) - V(1, -1, 0));
+  const double v_in = 0.5 * (vu_e + vu_w);
+  const double wu_n = W(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, 0) - W(1, 0, 0));
+  const double wu_s = W(1, 0, -1) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, -1) - W(1, 0, -1));
+  const double w_in = 0.5 * (wu_n + wu_s);
+  if (i > 2 && i < nx && j > 2 && j < ny + 1 && k > 2 && k < nz + 1 && near_ok) {
+    const double ddy = 0.5 * (a.dy[j - 1] + a.dy[j - 2]), ddye = 0.5 * (a.dy[j - 1] + a.dy[j]);
+    const double ddz = 0.5 * (a.dz[k - 1] + a.dz[k - 2]), ddzr = 0.5 * (a.dz[k - 1] + a.dz[k]);
+    duudx = upwind3(U(0, 0, 0), a.ca[BTB_UU], a.ck[BTB_UU], i, U(2, 0, 0), U(1, 0, 0), U(0, 0, 0),
+                    U(-1, 0, 0), U(-2, 0, 0), a.dx[i], a.dx[i - 1]);
+    dvudy = upwind3(v_in, a.ca[BTB_VU], a.ck[BTB_VU], j, U(0, 2, 0), U(0, 1, 0), U(0, 0, 0),
+                    U(0, -1, 0), U(0, -2, 0), ddye, ddy);
+    dwudz = upwind3(w_in, a.ca[BTB_WU], a.ck[BTB_WU], k, U(0, 0, 2), U(0, 0, 1), U(0, 0, 0),
+                    U(0, 0, -1), U(0, 0, -2), ddzr, ddz);
+  } else {
+    duudx = gdiv<P2>(-0.25 * (u1a * u1a + alpha * fabs(u1a) * u22 - u3 * u3 - alpha * fabs(u3) * u4), a.dx[i - 1]);
+    dvudy = gdiv<P2>(-0.25 * (v1a * u5 + alpha * fabs(v1a) * u6 - v22 * u7 - alpha * fabs(v22) * u8), a.dy[j - 1]);
+    dwudz = gdiv<P2>(-0.25 * (w1a * u9 + alpha * fabs(w1a) * u10 - w22 * u11 - alpha * fabs(w22) * u12), a.dz[k - 1]);
+  }
+  const double d2x = gdiv<P2>(2.0, dx2xr) * (gdiv<P2>(-u4, dxr) + gdiv<P2>(u22, dx));
+  const double d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-u8, 0.5 * dy2ye) + gdiv<P2>(u6, 0.5 * dy2yw));
+  const double d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-u12, 0.5 * dz2zt) + gdiv<P2>(u10, 0.5 * dz2zb));
+  const double visc = (d2x + d2y + d2z) / a.re;
+  const double res = (-duudx - dvudy - dwudz + visc);
+  return U(0, 0, 0) + a.dt * (res + dpdx);
+}
+
+// vt (NavierStokes.f90:690-780)
+template <bool P2>
+__device__ inline double mom_v(const MomArgs &a, const Fields &F, const MomCell &m, int i, int j, int k,
+                               bool near_ok) {
+  const double alpha = 1.0;
+  const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
+  const double u13 = U(-1, 0, 0) + U(-1, 1, 0), u14 = U(0, 0, 0) + U(0, 1, 0);
   const double v3 = V(-1, 0, 0) + V(0, 0, 0), v4 = V(-1, 0, 0) - V(0, 0, 0);
   const double v5 = V(0, 0, 0) + V(1, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
   const double v7 = V(0, -1, 0) + V(0, 0, 0), v8 = V(0, -1, 0) - V(0, 0, 0);
   const double v9 = V(0, 0, 0) + V(0, 1, 0), v10 = V(0, 0, 0) - V(0, 1, 0);
   const double v11 = V(0, 0, -1) + V(0, 0, 0), v12 = V(0, 0, -1) - V(0, 0, 0);
   const double v13 = V(0, 0, 0) + V(0, 0, 1), v14 = V(0, 0, 0) - V(0, 0, 1);
-  const double v15 = V(0, -1, 0) + V(0, -1, 1), v16 = v13;
-
-  const double w1a = W(0, 0, -1) + W(1, 0, -1), w22 = W(0, 0, 0) + W(1, 0, 0);
   const double w3 = W(0, 0, -1) + W(0, 1, -1), w4 = W(0, 0, 0) + W(0, 1, 0);
+  const double dpdy = gdiv<P2>(P(0, 0, 0) - P(0, 1, 0), 0.5 * (dye + dy));
+  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dz2zt = dz + dzt, dz2zb = dz + dzb;
+  const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+  const double uv_e = U(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(0, 0, 0) - U(0, 1, 0));
+  const double uv_w = U(-1, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(-1, 0, 0) - U(-1, 1, 0));
+  const double u_in = 0.5 * (uv_e + uv_w);
+  const double wv_n = W(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (W(0, 0, 0) - W(0, 1, 0));
+  const double wv_s = W(0, 1, -1) + gdiv<P2>(dye, dy2ye) * (W(0, 0, -1) - W(0, 1, -1));
+  const double w_in = 0.5 * (wv_n + wv_s);
+  double duvdx, dvvdy, dwvdz;
+  if (i > 2 && i < nx + 1 && j > 2 && j < ny && k > 2 && k < nz + 1 && near_ok) {
+    const double ddx = 0.5 * (a.dx[i - 1] + a.dx[i - 2]), ddxr = 0.5 * (a.dx[i - 1] + a.dx[i]);
+    const double ddz = 0.5 * (a.dz[k - 1] + a.dz[k - 2]), ddzr = 0.5 * (a.dz[k - 1] + a.dz[k]);
+    duvdx = upwind3(u_in, a.ca[BTB_UW], a.ck[BTB_UW], i, V(2, 0, 0), V(1, 0, 0), V(0, 0, 0),
+                    V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx);
+    dvvdy = upwind3(V(0, 0, 0), a.ca[BTB_VV], a.ck[BTB_VV], j, V(0, 2, 0), V(0, 1, 0), V(0, 0, 0),
+                    V(0, -1, 0), V(0, -2, 0), a.dy[j], a.dy[j - 1]);
+    dwvdz = upwind3(w_in, a.ca[BTB_WU], a.ck[BTB_WU], k, V(0, 0, 2), V(0, 0, 1), V(0, 0, 0),
+                    V(0, 0, -1), V(0, 0, -2), ddzr, ddz);
+  } else {
+    duvdx = gdiv<P2>(-0.25 * (u13 * v3 + alpha * fabs(u13) * v4 - u14 * v5 - alpha * fabs(u14) * v6), a.dx[i - 1]);
+    dvvdy = gdiv<P2>(-0.25 * (v7 * v7 + alpha * fabs(v7) * v8 - v9 * v9 - alpha * fabs(v9) * v10), a.dy[j - 1]);
+    dwvdz = gdiv<P2>(-0.25 * (w3 * v11 + alpha * fabs(w3) * v12 - w4 * v13 - alpha * fabs(w4) * v14), a.dz[k - 1]);
+  }
+  const double d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-v6, 0.5 * dx2xr) + gdiv<P2>(v4, 0.5 * dx2xl));
+  const double d2y = gdiv<P2>(2.0, dy2ye) * (gdiv<P2>(-v10, dye) + gdiv<P2>(v8, dy));
+  const double d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-v14, 0.5 * dz2zt) + gdiv<P2>(v12, 0.5 * dz2zb));
+  const double visc = (d2x + d2y + d2z) / a.re;
+  const double res = (-duvdx - dvvdy - dwvdz + visc);
+  return V(0, 0, 0) + a.dt * (res + dpdy);
+}
+
+// wt (NavierStokes.f90:782-858)
+template <bool P2>
+__device__ inline double mom_w(const MomArgs &a, const Fields &F, const MomCell &m, int i, int j, int k,
+                               bool near_ok) {
+  const double alpha = 1.0;
+  const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz;
+  const double u15 = U(-1, 0, 0) + U(-1, 0, 1), u16 = U(0, 0, 0) + U(0, 0, 1);
+  const double v15 = V(0, -1, 0) + V(0, -1, 1), v16 = V(0, 0, 0) + V(0, 0, 1);
   const double w5 = W(-1, 0, 0) + W(0, 0, 0), w6 = W(-1, 0, 0) - W(0, 0, 0);
   const double w7 = W(0, 0, 0) + W(1, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
   const double w9 = W(0, -1, 0) + W(0, 0, 0), w10 = W(0, -1, 0) - W(0, 0, 0);
   const double w11 = W(0, 0, 0) + W(0, 1, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
   const double w13 = W(0, 0, -1) + W(0, 0, 0), w14 = W(0, 0, -1) - W(0, 0, 0);
   const double w15 = W(0, 0, 0) + W(0, 0, 1), w16 = W(0, 0, 0) - W(0, 0, 1);
-
-  const double dpdx = (P(0, 0, 0) - P(1, 0, 0)) / (0.5 * (dxr + dx));
-  const double dpdy = (P(0, 0, 0) - P(0, 1, 0)) / (0.5 * (dye + dy));
-  const double dpdz = (P(0, 0, 0) - P(0, 0, 1)) / (0.5 * (dzt + dz));
-  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw;
-  const double dz2zt = dz + dzt, dz2zb = dz + dzb;
-  const int nx = d.nx, ny = d.ny, nz = d.nz;
-  // the near-body test is shared by the three components; only the index
-  // window differs (NavierStokes.f90:602, 690, 782)
-  // (evaluated only inside the windows, as the reference's nested ifs do)
-  const bool near_ok = i > 2 && j > 2 && k > 2 && adj_ok(d, a.cell, false, i, j, k, 1) &&
-                       adj_ok(d, a.cell2, true, i, j, k, 2);
-
-  // ---- u ----
-  double duudx, dvudy, dwudz;
-  {
-    const double vu_e = V(1, 0, 0) + (dxr / (dx2xr)) * (V(0, 0, 0) - V(1, 0, 0));
-    const double vu_w = V(1, -1, 0) + (dxr / (dx2xr)) * (V(0, -1, 0) - V(1, -1, 0));
-    const double v_in = 0.5 * (vu_e + vu_w);
-    const double wu_n = W(1, 0, 0) + (dxr / (dx2xr)) * (W(0, 0, 0) - W(1, 0, 0));
-    const double wu_s = W(1, 0, -1) + (dxr / (dx2xr)) * (W(0, 0, -1) - W(1, 0, -1));
-    const double w_in = 0.5 * (wu_n + wu_s);
-    if (i > 2 && i < nx && j > 2 && j < ny + 1 && k > 2 && k < nz + 1 && near_ok) {
-      const double ddy = 0.5 * (a.dy[j - 1] + a.dy[j - 2]), ddye = 0.5 * (a.dy[j - 1] + a.dy[j]);
-      const double ddz = 0.5 * (a.dz[k - 1] + a.dz[k - 2]), ddzr = 0.5 * (a.dz[k - 1] + a.dz[k]);
-      duudx = upwind3(U(0, 0, 0), a.ca[BTB_UU], a.ck[BTB_UU], i, U(2, 0, 0), U(1, 0, 0), U(0, 0, 0),
-                      U(-1, 0, 0), U(-2, 0, 0), a.dx[i], a.dx[i - 1]);
-      dvudy = upwind3(v_in, a.ca[BTB_VU], a.ck[BTB_VU], j, U(0, 2, 0), U(0, 1, 0), U(0, 0, 0),
-                      U(0, -1, 0), U(0, -2, 0), ddye, ddy);
-      dwudz = upwind3(w_in, a.ca[BTB_WU], a.ck[BTB_WU], k, U(0, 0, 2), U(0, 0, 1), U(0, 0, 0),
-                      U(0, 0, -1), U(0, 0, -2), ddzr, ddz);
-    } else {
-      duudx = -0.25 * (u1a * u1a + alpha * fabs(u1a) * u22 - u3 * u3 - alpha * fabs(u3) * u4) / a.dx[i - 1];
-      dvudy = -0.25 * (v1a * u5 + alpha * fabs(v1a) * u6 - v22 * u7 - alpha * fabs(v22) * u8) / a.dy[j - 1];
-      dwudz = -0.25 * (w1a * u9 + alpha * fabs(w1a) * u10 - w22 * u11 - alpha * fabs(w22) * u12) / a.dz[k - 1];
-    }
-    const double d2x = (2.0 / dx2xr) * ((-u4 / dxr) + (u22 / dx));
-    const double d2y = (2.0 / (0.5 * (dy2ye + dy2yw))) * ((-u8 / (0.5 * dy2ye)) + (u6 / (0.5 * dy2yw)));
-    const double d2z = (2.0 / (0.5 * (dz2zt + dz2zb))) * ((-u12 / (0.5 * dz2zt)) + (u10 / (0.5 * dz2zb)));
-    const double visc = (d2x + d2y + d2z) / a.re;
-    const double res = (-duudx - dvudy - dwudz + visc);
-    out_u = U(0, 0, 0) + a.dt * (res + dpdx);
+  const double dpdz = gdiv<P2>(P(0, 0, 0) - P(0, 0, 1), 0.5 * (dzt + dz));
+  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt;
+  const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+  const double uw_e = U(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(0, 0, 0) - U(0, 0, 1));
+  const double uw_w = U(-1, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(-1, 0, 0) - U(-1, 0, 1));
+  const double u_in = 0.5 * (uw_e + uw_w);
+  const double vw_n = V(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, 0, 0) - V(0, 0, 1));
+  const double vw_s = V(0, -1, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, -1, 0) - V(0, -1, 1));
+  const double v_in = 0.5 * (vw_n + vw_s);
+  double duwdx, dvwdy, dwwdz;
+  if (i > 2 && i < nx + 1 && j > 2 && j < ny + 1 && k > 2 && k < nz && near_ok) {
+    const double ddx = 0.5 * (a.dx[i - 1] + a.dx[i - 2]), ddxr = 0.5 * (a.dx[i - 1] + a.dx[i]);
+    const double ddy = 0.5 * (a.dy[j - 1] + a.dy[j - 2]), ddye = 0.5 * (a.dy[j - 1] + a.dy[j]);
+    duwdx = upwind3(u_in, a.ca[BTB_UW], a.ck[BTB_UW], i, W(2, 0, 0), W(1, 0, 0), W(0, 0, 0),
+                    W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx);
+    dvwdy = upwind3(v_in, a.ca[BTB_VW], a.ck[BTB_VW], j, W(0, 2, 0), W(0, 1, 0), W(0, 0, 0),
+                    W(0, -1, 0), W(0, -2, 0), ddye, ddy);
+    dwwdz = upwind3(W(0, 0, 0), a.ca[BTB_WW], a.ck[BTB_WW], k, W(0, 0, 2), W(0, 0, 1), W(0, 0, 0),
+                    W(0, 0, -1), W(0, 0, -2), a.dz[k], a.dz[k - 1]);
+  } else {
+    duwdx = gdiv<P2>(-0.5 * (u15 * w5 + alpha * fabs(u15) * w6 - u16 * w7 - alpha * fabs(u16) * w8), a.dx[i - 1]);
+    dvwdy = gdiv<P2>(-0.5 * (v15 * w9 + alpha * fabs(v15) * w10 - v16 * w11 - alpha * fabs(v16) * w12), a.dy[j - 1]);
+    dwwdz = gdiv<P2>(-0.5 * (w13 * w13 + alpha * fabs(w13) * w14 - w15 * w15 - alpha * fabs(w15) * w16), a.dz[k - 1]);
   }
-  // ---- v ----
-  {
-    const double uv_e = U(0, 1, 0) + (dye / (dy2ye)) * (U(0, 0, 0) - U(0, 1, 0));
-    const double uv_w = U(-1, 1, 0) + (dye / (dy2ye)) * (U(-1, 0, 0) - U(-1, 1, 0));
-    const double u_in = 0.5 * (uv_e + uv_w);
-    const double wv_n = W(0, 1, 0) + (dye / (dy2ye)) * (W(0, 0, 0) - W(0, 1, 0));
-    const double wv_s = W(0, 1, -1) + (dye / (dy2ye)) * (W(0, 0, -1) - W(0, 1, -1));
-    const double w_in = 0.5 * (wv_n + wv_s);
-    double duvdx, dvvdy, dwvdz;
-    if (i > 2 && i < nx + 1 && j > 2 && j < ny && k > 2 && k < nz + 1 && near_ok) {
-      const double ddx = 0.5 * (a.dx[i - 1] + a.dx[i - 2]), ddxr = 0.5 * (a.dx[i - 1] + a.dx[i]);
-      const double ddz = 0.5 * (a.dz[k - 1] + a.dz[k - 2]), ddzr = 0.5 * (a.dz[k - 1] + a.dz[k]);
-      duvdx = upwind3(u_in, a.ca[BTB_UW], a.ck[BTB_UW], i, V(2, 0, 0), V(1, 0, 0), V(0, 0, 0),
-                      V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx);
-      dvvdy = upwind3(V(0, 0, 0), a.ca[BTB_VV], a.ck[BTB_VV], j, V(0, 2, 0), V(0, 1, 0), V(0, 0, 0),
-                      V(0, -1, 0), V(0, -2, 0), a.dy[j], a.dy[j - 1]);
-      dwvdz = upwind3(w_in, a.ca[BTB_WU], a.ck[BTB_WU], k, V(0, 0, 2), V(0, 0, 1), V(0, 0, 0),
-                      V(0, 0, -1), V(0, 0, -2), ddzr, ddz);
-    } else {
-      duvdx = -0.25 * (u13 * v3 + alpha * fabs(u13) * v4 - u14 * v5 - alpha * fabs(u14) * v6) / a.dx[i - 1];
-      dvvdy = -0.25 * (v7 * v7 + alpha * fabs(v7) * v8 - v9 * v9 - alpha * fabs(v9) * v10) / a.dy[j - 1];
-      dwvdz = -0.25 * (w3 * v11 + alpha * fabs(w3) * v12 - w4 * v13 - alpha * fabs(w4) * v14) / a.dz[k - 1];
-    }
-    const double d2x = (2.0 / (0.5 * (dx2xr + dx2xl))) * ((-v6 / (0.5 * dx2xr)) + (v4 / (0.5 * dx2xl)));
-    const double d2y = (2.0 / dy2ye) * ((-v10 / dye) + (v8 / dy));
-    const double d2z = (2.0 / (0.5 * (dz2zt + dz2zb))) * ((-v14 / (0.5 * dz2zt)) + (v12 / (0.5 * dz2zb)));
-    const double visc = (d2x + d2y + d2z) / a.re;
-    const double res = (-duvdx - dvvdy - dwvdz + visc);
-    out_v = V(0, 0, 0) + a.dt * (res + dpdy);
-  }
-  // ---- w ----
-  {
-    const double uw_e = U(0, 0, 1) + (dzt / (dz2zt)) * (U(0, 0, 0) - U(0, 0, 1));
-    const double uw_w = U(-1, 0, 1) + (dzt / (dz2zt)) * (U(-1, 0, 0) - U(-1, 0, 1));
-    const double u_in = 0.5 * (uw_e + uw_w);
-    const double vw_n = V(0, 0, 1) + (dzt / (dz2zt)) * (V(0, 0, 0) - V(0, 0, 1));
-    const double vw_s = V(0, -1, 1) + (dzt / (dz2zt)) * (V(0, -1, 0) - V(0, -1, 1));
-    const double v_in = 0.5 * (vw_n + vw_s);
-    double duwdx, dvwdy, dwwdz;
-    if (i > 2 && i < nx + 1 && j > 2 && j < ny + 1 && k > 2 && k < nz && near_ok) {
-      const double ddx = 0.5 * (a.dx[i - 1] + a.dx[i - 2]), ddxr = 0.5 * (a.dx[i - 1] + a.dx[i]);
-      const double ddy = 0.5 * (a.dy[j - 1] + a.dy[j - 2]), ddye = 0.5 * (a.dy[j - 1] + a.dy[j]);
-      duwdx = upwind3(u_in, a.ca[BTB_UW], a.ck[BTB_UW], i, W(2, 0, 0), W(1, 0, 0), W(0, 0, 0),
-                      W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx);
-      dvwdy = upwind3(v_in, a.ca[BTB_VW], a.ck[BTB_VW], j, W(0, 2, 0), W(0, 1, 0), W(0, 0, 0),
-                      W(0, -1, 0), W(0, -2, 0), ddye, ddy);
-      dwwdz = upwind3(W(0, 0, 0), a.ca[BTB_WW], a.ck[BTB_WW], k, W(0, 0, 2), W(0, 0, 1), W(0, 0, 0),
-                      W(0, 0, -1), W(0, 0, -2), a.dz[k], a.dz[k - 1]);
-    } else {
-      duwdx = -0.5 * (u15 * w5 + alpha * fabs(u15) * w6 - u16 * w7 - alpha * fabs(u16) * w8) / a.dx[i - 1];
-      dvwdy = -0.5 * (v15 * w9 + alpha * fabs(v15) * w10 - v16 * w11 - alpha * fabs(v16) * w12) / a.dy[j - 1];
-      dwwdz = -0.5 * (w13 * w13 + alpha * fabs(w13) * w14 - w15 * w15 - alpha * fabs(w15) * w16) / a.dz[k - 1];
-    }
-    const double d2x = (2.0 / (0.5 * (dx2xr + dx2xl))) * ((-w8 / (0.5 * dx2xr)) + (w6 / (0.5 * dx2xl)));
-    const double d2y = (2.0 / (0.5 * (dy2ye + dy2yw))) * ((-w12 / (0.5 * dy2ye)) + (w10 / (0.5 * dy2yw)));
-    const double d2z = (2.0 / dz2zt) * ((-w16 / dzt) + (w14 / dz));
-    const double visc = (d2x + d2y + d2z) / a.re;
-    const double res = (-duwdx - dvwdy - dwwdz + visc);
-    out_w = W(0, 0, 0) + a.dt * (res + dpdz);
-  }
+  const double d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-w8, 0.5 * dx2xr) + gdiv<P2>(w6, 0.5 * dx2xl));
+  const double d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-w12, 0.5 * dy2ye) + gdiv<P2>(w10, 0.5 * dy2yw));
+  const double d2z = gdiv<P2>(2.0, dz2zt) * (gdiv<P2>(-w16, dzt) + gdiv<P2>(w14, dz));
+  const double visc = (d2x + d2y + d2z) / a.re;
+  const double res = (-duwdx - dvwdy - dwwdz + visc);
+  return W(0, 0, 0) + a.dt * (res + dpdz);
+}
 #undef U
 #undef V
 #undef W
 #undef P
+
+template <bool P2 = false>
+__device__ void momentum_cell(const MomArgs &a, const Fields &F, int i, int j, int k, double &out_u,
+                              double &out_v, double &out_w) {
+  const MomCell m = mom_cell(a, i, j, k);
+  const bool near_ok = mom_near_ok(a, i, j, k);
+  out_u = mom_u<P2>(a, F, m, i, j, k, near_ok);
+  out_v = mom_v<P2>(a, F, m, i, j, k, near_ok);
+  out_w = mom_w<P2>(a, F, m, i, j, k, near_ok);
 }
 
-// Dense momentum kernel over fluid cells not deferred to the ordered pass.
-__global__ void __launch_bounds__(256) k_momentum(MomArgs a, Fields F, double *__restrict__ ut,
-                                                  double *__restrict__ vt, double *__restrict__ wt,
-                                                  const uint8_t *__restrict__ deferred) {
-  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = 2 + blockIdx.z;
+// Workgroup -> (tile along i, tile along j, plane k) over a one-dimensional
+// grid of gx*gy tiles per plane times nz planes.  Consecutive workgroups go
+// to the eight XCDs in turn; when a plane's tile count is a multiple of 8,
+// XCD e takes the e-th eighth of every plane's tiles in row order (a band of
+// j rows), so most of the j halo rows and k neighbour planes a tile reads
+// were brought into that XCD's L2 by its neighbours.
+__device__ inline void mom_tile(int gx, int gy, int &ti, int &tj, int &tk) {
+  const int L = blockIdx.x, T = gx * gy;
+  int t;
+  if ((T & 7) == 0) {
+    const int per = T >> 3, l = L >> 3;
+    tk = l / per;
+    t = (L & 7) * per + l % per;
+  } else {
+    tk = L / T;
+    t = L % T;
+  }
+  ti = t % gx;
+  tj = t / gx;
+}
+
+// Dense momentum kernels over fluid cells not deferred to the ordered pass.
+// A workgroup is 64 x 4 cells of one k plane, one row of i per wavefront, so
+// j is wave-uniform: the y and z coefficient rows and spacings are scalar
+// loads.  k_momentum evaluates the three components per cell; k_momentum_c
+// one component (launched once per component), which keeps fewer values
+// live per thread at the cost of re-reading the shared neighbours from the
+// caches.  WAVES = 4 caps k_momentum's registers for four waves per SIMD
+// (a few values spill); WAVES = 1 leaves the compiler's choice (three).  All
+// give the same bits.
+template <int WAVES, bool P2>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
+k_momentum(MomArgs a, Fields F, double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt,
+           const uint8_t *__restrict__ deferred, int gx, int gy) {
+  int ti, tj, tk;
+  mom_tile(gx, gy, ti, tj, tk);
+  const int i = 2 + ti * 64 + threadIdx.x;
+  const int j = 2 + tj * 4 + __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int k = 2 + tk;
   if (i > a.d.nx + 1 || j > a.d.ny + 1) return;
   const int64_t c = ix3(a.d, i, j, k);
-  if (a.cell[c] != 0) return;
-  if (deferred && deferred[c]) return;
+  const uint8_t tag = a.cell[c], dfr = deferred ? deferred[c] : 0;
+  if (tag != 0 || dfr != 0) return;
   double ou, ov, ow;
-  momentum_cell(a, F, i, j, k, ou, ov, ow);
+  momentum_cell<P2>(a, F, i, j, k, ou, ov, ow);
   ut[c] = ou;
   vt[c] = ov;
   wt[c] = ow;
+}
+
+template <int C, bool P2>
+__global__ void __launch_bounds__(256) k_momentum_c(MomArgs a, Fields F, double *__restrict__ out,
+                                                    const uint8_t *__restrict__ deferred, int gx, int gy) {
+  int ti, tj, tk;
+  mom_tile(gx, gy, ti, tj, tk);
+  const int i = 2 + ti * 64 + threadIdx.x;
+  const int j = 2 + tj * 4 + __builtin_amdgcn_readfirstlane(threadIdx.y);
+  const int k = 2 + tk;
+  if (i > a.d.nx + 1 || j > a.d.ny + 1) return;
+  const int64_t c = ix3(a.d, i, j, k);
+  const uint8_t tag = a.cell[c], dfr = deferred ? deferred[c] : 0;
+  if (tag != 0 || dfr != 0) return;
+  const MomCell m = mom_cell(a, i, j, k);
+  const bool near_ok = mom_near_ok(a, i, j, k);
+  if (C == 0) out[c] = mom_u<P2>(a, F, m, i, j, k, near_ok);
+  if (C == 1) out[c] = mom_v<P2>(a, F, m, i, j, k, near_ok);
+  if (C == 2) out[c] = mom_w<P2>(a, F, m, i, j, k, near_ok);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,6 +541,63 @@ int compact_mask_lin(Block &b, const uint8_t *mask, int **out, int64_t *cap, int
 
 using namespace bcfd;
 
+namespace {
+// dense momentum kernel: 0 = three components per thread, 1 = one launch per
+// component, 2 = as 0 with registers capped for four waves per SIMD
+int g_mom_variant = 0;
+// grid divisions as products by exact reciprocals where the block's spacings allow
+int g_mom_pow2 = 1;
+
+// every spacing of every axis is one and the same power of two per axis
+// (the grid divisors of the momentum stencil are then exact powers of two)
+bool pow2_grid(const Block &b) {
+  for (int g : {BG_DX, BG_DY, BG_DZ}) {
+    const std::vector<double> &h = b.hg[g];
+    if (h.empty()) return false;
+    int e = 0;
+    const double f = std::frexp(h[0], &e);
+    if (!(h[0] > 0.0) || f != 0.5 || e < -1000 || e > 1000) return false;
+    for (double x : h)
+      if (x != h[0]) return false;
+  }
+  return true;
+}
+
+template <bool P2>
+void launch_dense_p2(const MomArgs &a, const Fields &F, Block *b, hipStream_t s, const uint8_t *deferred) {
+  // 64 x 4 cells per workgroup, one workgroup per tile and plane
+  const dim3 blk(64, 4, 1);
+  const int gx = (int)div_up(b->d.nx, 64), gy = (int)div_up(b->d.ny, 4);
+  const dim3 grd((unsigned)((int64_t)gx * gy * b->d.nz));
+  if (g_mom_variant == 0 || g_mom_variant == 2) {
+    auto kern = g_mom_variant == 0 ? k_momentum<1, P2> : k_momentum<4, P2>;
+    hipLaunchKernelGGL(kern, grd, blk, 0, s, a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], deferred, gx, gy);
+    return;
+  }
+  hipLaunchKernelGGL((k_momentum_c<0, P2>), grd, blk, 0, s, a, F, b->f[BF_UT], deferred, gx, gy);
+  hipLaunchKernelGGL((k_momentum_c<1, P2>), grd, blk, 0, s, a, F, b->f[BF_VT], deferred, gx, gy);
+  hipLaunchKernelGGL((k_momentum_c<2, P2>), grd, blk, 0, s, a, F, b->f[BF_WT], deferred, gx, gy);
+}
+
+void launch_dense(const MomArgs &a, const Fields &F, Block *b, hipStream_t s, const uint8_t *deferred) {
+  if (g_mom_pow2 && pow2_grid(*b))
+    launch_dense_p2<true>(a, F, b, s, deferred);
+  else
+    launch_dense_p2<false>(a, F, b, s, deferred);
+}
+}  // namespace
+
+extern "C" int biocfd_momentum_config(int variant, int pow2_products) {
+  BCFD_API_LOCK();
+  if (variant < 0 || variant > 2) {
+    set_error("momentum_config: variant %d outside 0..2", variant);
+    return -1;
+  }
+  g_mom_variant = variant;
+  g_mom_pow2 = pow2_products != 0;
+  return 0;
+}
+
 extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   BCFD_API_LOCK();
   Block *b = get_block(id);
@@ -458,8 +619,7 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   dim3 grd(div_up(b->d.nx, 64), div_up(b->d.ny, 4), (unsigned)b->d.nz);
   const int n_ts = (int)b->counts[BC_TS];
   if (n_ts == 0) {
-    hipLaunchKernelGGL(k_momentum, grd, blk, 0, s, a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT],
-                       (const uint8_t *)nullptr);
+    launch_dense(a, F, b, s, nullptr);
     BCFD_CHECK_LAUNCH();
     return 0;
   }
@@ -470,8 +630,7 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   BCFD_HIP(hipMemsetAsync(deferred, 0, b->n2, s));
   hipLaunchKernelGGL(k_ts_touch, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched);
   hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched, deferred);
-  hipLaunchKernelGGL(k_momentum, grd, blk, 0, s, a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT],
-                     (const uint8_t *)deferred);
+  launch_dense(a, F, b, s, deferred);
   BCFD_CHECK_LAUNCH();
   int *list = nullptr;
   int64_t cap = 0, cnt = 0;

@@ -507,14 +507,25 @@ __device__ void wave_nearest(const Geo &g, const BinView &bv, const double *px, 
 }
 
 // selectiveRetagging_th's re-tag of the 3x3x3 neighbourhoods (Search.F90:
-// 540-560 with tagging_th_core :305-338): one wavefront per neighbour cell
+// 540-560 with tagging_th_core :305-338): one wavefront per neighbour entry.
+// Neighbourhoods of adjacent IB cells overlap; the tags of a cell depend on
+// its position and the surface only, so the reference's repeated visits
+// write the same values.  Here the first wavefront to claim a cell's bit
+// tags it and the others return; k_reclass clears the bits again.
 __global__ void __launch_bounds__(256) k_retag(Geo g, BinView bv, const int *ijk, int64_t cap, int64_t n,
-                                               uint8_t *cell, uint8_t *node_tag) {
+                                               uint8_t *cell, uint8_t *node_tag, unsigned *bits) {
   const int64_t q = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (q >= n * 27) return;  // the whole wavefront
   const int64_t nn = q / 27;
   const int r = (int)(q % 27);
   const int i = ijk[nn] + (r % 3) - 1, j = ijk[nn + cap] + ((r / 3) % 3) - 1, k = ijk[nn + 2 * cap] + (r / 9) - 1;
+  const int64_t c = ix3(g.d, i, j, k);
+  unsigned taken = 0;
+  if ((threadIdx.x & 63) == 0) {
+    const unsigned bit = 1u << (unsigned)(c & 31);
+    taken = atomicOr(&bits[c >> 5], bit) & bit;
+  }
+  if (__shfl(taken, 0, 64)) return;  // the whole wavefront: another one tags this cell
   const double px[2] = {g.xp[i - 1], g.x1[i - 1]}, py[2] = {g.yp[j - 1], g.y1[j - 1]},
                pz[2] = {g.zp[k - 1], g.z1[k - 1]};
   int best[2];
@@ -522,7 +533,8 @@ __global__ void __launch_bounds__(256) k_retag(Geo g, BinView bv, const int *ijk
   if ((threadIdx.x & 63) == 0) tag_from(g, cell, node_tag, i, j, k, best[0], best[1]);
 }
 
-__global__ void k_reclass(Dims d, const int *ijk, int64_t cap, int64_t n, uint8_t *cell, const uint8_t *node_tag) {
+__global__ void k_reclass(Dims d, const int *ijk, int64_t cap, int64_t n, uint8_t *cell, const uint8_t *node_tag,
+                          unsigned *bits) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n * 27) return;
   const int64_t nn = q / 27;
@@ -530,6 +542,7 @@ __global__ void k_reclass(Dims d, const int *ijk, int64_t cap, int64_t n, uint8_
   const int i = ijk[nn] + (r % 3) - 1, j = ijk[nn + cap] + ((r / 3) % 3) - 1, k = ijk[nn + 2 * cap] + (r / 9) - 1;
   const int64_t c = ix3(d, i, j, k);
   if (cell[c] != 2) cell[c] = (corner_sum(d, node_tag, i, j, k) == 8) ? 1 : 0;
+  atomicAnd(&bits[c >> 5], ~(1u << (unsigned)(c & 31)));
 }
 
 // computeNormDistance: seven nearest-element searches per IB cell and the
@@ -787,10 +800,16 @@ int biocfd_selective_retagging(int id, int64_t blk_mv_tag) {
     hipLaunchKernelGGL(k_reset_ib, dim3(div_up(n, 256)), dim3(256), 0, s, b->d, ib.ib_ijk, ib.cap_ib, n,
                        b->tag[BT_CELL]);
     if (!ib.bins_ready) { set_error("selective_retagging: surface not computed"); return -1; }
+    const int64_t words = div_up(b->n2, 32);
+    if (ib.retag_words < words) {
+      if (dev_alloc(&ib.retag_bits, words)) return -1;
+      ib.retag_words = words;
+      BCFD_HIP(hipMemsetAsync(ib.retag_bits, 0, sizeof(unsigned) * (size_t)words, s));
+    }
     hipLaunchKernelGGL(k_retag, dim3(div_up(n * 27, 4)), dim3(256), 0, s, g, bins_of(*b), ib.ib_ijk, ib.cap_ib, n,
-                       b->tag[BT_CELL], b->tag[BT_NODEIDTAG]);
+                       b->tag[BT_CELL], b->tag[BT_NODEIDTAG], ib.retag_bits);
     hipLaunchKernelGGL(k_reclass, dim3(div_up(n * 27, 256)), dim3(256), 0, s, b->d, ib.ib_ijk, ib.cap_ib, n,
-                       b->tag[BT_CELL], b->tag[BT_NODEIDTAG]);
+                       b->tag[BT_CELL], b->tag[BT_NODEIDTAG], ib.retag_bits);
     BCFD_CHECK_LAUNCH();
   }
   // new counts (Search.F90:623-647); the lists are rebuilt by cell_count_solid

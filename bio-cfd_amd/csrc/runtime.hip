@@ -244,6 +244,7 @@ static void free_block(Block &b) {
   for (void *p : {ib.bin_grid, (void *)ib.bin_start, (void *)ib.bin_elem, (void *)ib.bin_fill})
     if (p) (void)hipFree(p);
   if (ib.ib_ijk) (void)hipFree(ib.ib_ijk);
+  if (ib.retag_bits) (void)hipFree(ib.retag_bits);
   for (auto &p : ib.nel) if (p) (void)hipFree(p);
   for (auto &p : ib.normdis) if (p) (void)hipFree(p);
   if (ib.ib_map) (void)hipFree(ib.ib_map);

@@ -552,6 +552,11 @@ module biocfd_hip
       import :: c_int
       integer(c_int), value :: on
     end function
+    integer(c_int) function biocfd_momentum_config(variant, pow2_products) &
+        bind(C, name="biocfd_momentum_config")
+      import :: c_int
+      integer(c_int), value :: variant, pow2_products
+    end function
 
     ! ---- block move (Search.F90:21-47, :415-432, :916-1461; FineInterp.f90:10-281)
     integer(c_int) function biocfd_find_dist_node(n_nodes, xnode, znode, node_id, markers) bind(C, &

@@ -114,6 +114,7 @@ SIGNATURES = {
     "biocfd_grid_download": [_I, _I, _P, _L],
     "biocfd_output_fields": [_I, _P, _P, _P, _P, _P, _P, _P],
     "biocfd_sor_batching": [_I],
+    "biocfd_momentum_config": [_I, _I],
 }
 
 

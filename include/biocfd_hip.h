@@ -100,6 +100,14 @@ int biocfd_non_uni_coeff(int blk);
 /* ---- momentum and boundary conditions ---- */
 /* nsMomentum2order, NavierStokes.f90:252-912 */
 int biocfd_ns_momentum(int blk, double dt, double re);
+/* Dense momentum kernel of biocfd_ns_momentum (no reference counterpart):
+ * variant 0 (the default) = the three components per thread in one launch,
+ * 1 = one launch per velocity component, 2 = as 0 with its registers capped
+ * for four waves per SIMD.  pow2_products (default 1): on a block whose
+ * every axis has one power-of-two spacing, the divisions by grid spacings
+ * are products by their exact reciprocals (the same correctly rounded
+ * values).  All settings give the same bits. */
+int biocfd_momentum_config(int variant, int pow2_products);
 /* velocityBC, BoundaryConditions.f90:11-74 */
 int biocfd_velocity_bc(int blk, double dt, double uc);
 /* solidCellBC, BoundaryConditions.f90:76-92 */
@@ -151,6 +159,7 @@ int biocfd_sor_time(int blk, int variant, int planes_per_workgroup, int iters, d
  * budget (pcItaMax + 1 - nIterPcor) is spent; nIterPcor counts sweeps and
  * derr2 holds max|r|.  mode 0 (default) is the FP64 method. */
 int biocfd_set_precision(int mode);
+/* sweeps_per_correction: 1..1024 */
 int biocfd_mixed_config(int variant, int planes_per_workgroup, int sweeps_per_correction,
                         double residual_tol);
 int biocfd_mixed_variant_info(int variant, int info[7]);

@@ -75,4 +75,10 @@ def small_cases():
                    phase_angle_deg=90.0)
     move = CS.config_nested_move(nsteps=3)
     batched = CS.config_batched(nsteps=3)
-    return {"sphere": sphere, "channel": channel, "nested": nested, "wing": wing, "move": move, "batched": batched}
+    # power-of-two spacings (1/32 on every axis): the momentum kernels' exact-reciprocal path
+    sphere32 = CS.config_sphere_64(n=32, markers=256, nsteps=3, pc_ita_max=20, epsi=0.0)
+    blk32 = CS.BlockSpec((0, 0, 0), (1, 1, 1), (32, 32, 32), omega=1.7, body=body, moving=True)
+    wing32 = CS.Case("wing32", [blk32], dt=0.002, re=300.0, nsteps=3, pc_ita_max=15, epsi=0.0, freq=0.5,
+                     phase_angle_deg=90.0)
+    return {"sphere": sphere, "channel": channel, "nested": nested, "wing": wing, "move": move, "batched": batched,
+            "sphere32": sphere32, "wing32": wing32}

@@ -17,19 +17,19 @@ pytestmark = pytest.mark.gpu
 STEPS = 3
 # the library's mixed-precision defaults (variant, planes, sweeps per correction, residual tolerance),
 # restored after each test
-DEFAULT = (21, 32, 64, 1e-8)
+DEFAULT = (21, 32, 128, 1e-8)
 # (variant, planes, sweeps per correction): the per-phase cell-list FP32 kernel at 4 sweeps and the
 # pair-owned FP32 kernels (fused multiply-adds, hardware reciprocal; one and two sweeps per pass) at
-# 32 and at the default 64
-CONFIGS = [(1, 32, 4), (15, 16, 32), (21, 32, 32), (21, 32, 64)]
+# 32, 64 and the default 128 (these cases' SOR calls are shorter than 64 sweeps)
+CONFIGS = [(1, 32, 4), (15, 16, 32), (21, 32, 32), (21, 32, 64), (21, 32, 128)]
 # max |mixed - fp64| / max |fp64| allowed per field.  The FP32 rounding the corrections carry grows
 # with the sweeps per correction: at 4 sweeps the fields stay within 1e-8 of the FP64 run, at 32 within
 # 2e-7 (velocities, p) on these cases, so the set for 32 and 64 is the accuracy contract of the mixed
 # path (1e-6 relative velocity error) with the derived arrays a decade looser.
 CONTRACT = {"u": 1e-6, "v": 1e-6, "w": 1e-6, "ut": 1e-6, "vt": 1e-6, "wt": 1e-6, "p": 1e-6, "pc": 1e-5, "b": 1e-5}
 TOL = {4: {"u": 1e-7, "v": 1e-7, "w": 1e-7, "ut": 1e-7, "vt": 1e-7, "wt": 1e-7, "p": 1e-7, "pc": 1e-6, "b": 1e-6},
-       32: CONTRACT, 64: CONTRACT}
-TOL_OTHER = {4: 1e-6, 32: 1e-5, 64: 1e-5}  # every other FP64 array (ghosts, normal distances, forces ...)
+       32: CONTRACT, 64: CONTRACT, 128: CONTRACT}
+TOL_OTHER = {4: 1e-6, 32: 1e-5, 64: 1e-5, 128: 1e-5}  # every other FP64 array (ghosts, normal distances, forces ...)
 SKIP = ("derr2", "pco")   # derr2 is the residual norm in the mixed solve, pco is not kept
 
 

@@ -11,11 +11,20 @@ pytestmark = pytest.mark.gpu
 STEPS = (0, 1, 3)
 
 
-@pytest.mark.parametrize("name,batching", [("channel", 1), ("sphere", 1), ("nested", 1), ("wing", 1), ("move", 1),
-                                          ("batched", 1), ("batched", 0)])
-def test_time_step_parity(name, batching):
+@pytest.mark.parametrize("name,batching,mom,pow2", [
+    ("channel", 1, 1, 1), ("sphere", 1, 1, 1), ("nested", 1, 1, 1), ("wing", 1, 1, 1), ("move", 1, 1, 1),
+    ("batched", 1, 1, 1), ("batched", 0, 1, 1),
+    ("sphere", 1, 0, 1), ("wing", 1, 0, 1), ("sphere", 1, 2, 1), ("wing", 1, 2, 1),
+    ("sphere", 1, 1, 0), ("wing", 1, 1, 0), ("nested", 1, 0, 0),
+    ("sphere32", 1, 1, 1), ("sphere32", 1, 0, 1), ("sphere32", 1, 2, 1), ("sphere32", 1, 1, 0),
+    ("wing32", 1, 1, 1), ("wing32", 1, 2, 1), ("wing32", 1, 0, 0)])
+def test_time_step_parity(name, batching, mom, pow2):
     """batching: the fine blocks' SOR calls in one launch per pass (1, the
-    default) or one block after the other (0)."""
+    default) or one block after the other (0).  mom: the dense momentum
+    kernel, all three components per thread (0, the default), one launch per
+    velocity component (1), or 0 with four waves per SIMD (2).  pow2:
+    divisions by power-of-two grid spacings as products by the exact
+    reciprocal (1, the default) or as divisions (0)."""
     from biocfd_amd import hip as H
     from biocfd_amd.driver import Simulation
 
@@ -23,6 +32,7 @@ def test_time_step_parity(name, batching):
     ref = run_oracle(case, STEPS)
     sim = Simulation(case)
     H.call("biocfd_sor_batching", batching)
+    H.call("biocfd_momentum_config", mom, pow2)
     try:
         bad = []
         for s in range(0, max(STEPS) + 1):
@@ -35,4 +45,5 @@ def test_time_step_parity(name, batching):
         assert not bad, "\n".join(bad[:40])
     finally:
         H.call("biocfd_sor_batching", 1)
+        H.call("biocfd_momentum_config", 0, 1)
         sim.close()
