@@ -1,5 +1,5 @@
 """Throughput of the Bio-CFD time step on one MI355X, FP64 method and mixed
-precision side by side.
+precision side by side, both solved to the same FP64 Poisson residual.
 
 Workload: single block 256x256x256 on [0,8]x[0,4]x[0,4] with a rigid sphere
 of diameter 1 at (2,2,2) carried by 8192 surface triangles, Re = 300,
@@ -11,14 +11,20 @@ all fields resident in HBM.
     python bench.py [--steps K] [--warmup W] [--precision both|fp64|mixed]
     python bench.py --workload uneven [--steps K] [--warmup W]
 
-Prints one JSON line.  `value` is the FP64 method's Mcell-steps/s (one GPU);
-`mixed` holds the mixed-precision run of the same steps on the same inputs
-(FP32 SOR smoothing, 128 sweeps per FP64 defect correction, the same sweep
-budget), its speed-up and its per-field error against the FP64 run.
-`roofline` describes the FP64 red-black SOR iteration (the dominant kernel)
-measured with HIP events on the runtime's stream; `cpu_baseline` times the
-reference's own CPU build (oracle/_ref/ref_harness) on one step of the same
-workload.
+Prints one JSON line.  `value` is the FP64 method's Mcell-steps/s (one GPU)
+with every pressure solve run to max|b/dt - A pc| < 1e-8 over the fluid cells
+(the FP64 residual, tested every 256 iterations; pcItaMax raised so that the
+residual, not the budget, ends every solve).  `mixed` holds the mixed-precision
+run of the same steps on the same inputs to the same residual (FP32 SOR
+smoothing, 256 sweeps per FP64 defect correction, the residual tested after
+every correction), its speed-up and its per-field error against the FP64 run.
+`reference_rule` is the FP64 method under the reference's own stopping rule
+(update norm < 1e-8 tested every fifth iteration, pcItaMax = 500, which this
+workload reaches: 501 iterations per step).  `roofline` describes the FP64
+red-black SOR (the dominant kernel) of the residual run, measured with HIP
+events on the runtime's stream; `cpu_baseline` times the reference's own CPU
+build (oracle/_ref/ref_harness) on one step of the same workload under the
+reference's rule, on one core.
 
 --workload uneven: the batched multi-block workload instead (a coarse block
 192x128x128 and 16 fine blocks of uneven sizes, one with a heaving, pitching
@@ -72,26 +78,40 @@ def mixed_bytes_per_cell(sweeps_per_correction: int) -> float:
     return (12 * k - 4) + 32
 
 
-def make_case(args):
+def make_case(args, rule: str = "reference"):
+    """The 256^3 sphere workload; rule "residual": pcItaMax raised and the
+    residual tolerance as epsi, "reference": the reference's pcItaMax/epsi."""
     from biocfd_amd import cases as CS
 
     n = (args.n, args.n, args.n)
+    if rule == "residual":
+        return CS.config_sphere_256(n=n, markers=args.markers, nsteps=args.warmup + args.steps,
+                                    pc_ita_max=args.resid_ita_max, epsi=args.resid_tol)
     return CS.config_sphere_256(n=n, markers=args.markers, nsteps=args.warmup + args.steps,
                                 pc_ita_max=args.pc_ita_max, epsi=args.epsi)
 
 
-def run_path(args, precision: str, keep_fields: bool):
+def run_path(args, precision: str, keep_fields: bool, rule: str = "residual"):
+    """One run of the workload: `precision` fp64 or mixed, `rule` residual
+    (FP64 residual < resid_tol) or reference (the reference's update-norm
+    rule; FP64 only)."""
     from biocfd_amd import hip as H
     from biocfd_amd import tune
     from biocfd_amd.driver import Simulation
 
-    case = make_case(args)
+    case = make_case(args, rule)
     cells = case.cells()
     if args.copy_probe_mb > 0 and precision == "fp64":
         H.call("biocfd_init", 0)
         H.call("biocfd_copy_probe", int(args.copy_probe_mb) << 20, 4)
     sim = Simulation(case, precision=precision)
     H.call("biocfd_momentum_config", args.momentum_variant, 0 if args.momentum_divisions else 1)
+    if precision == "fp64":
+        if rule == "residual":
+            H.call("biocfd_sor_stop_rule", 1, args.check_every)
+        else:
+            H.call("biocfd_sor_stop_rule", 0, 5)
+    mixed_tol = args.resid_tol if rule == "residual" else args.epsi
     if precision == "fp64":
         if args.sor_variant == "auto":
             v, planes, table = tune.autotune(block=1, dt=case.dt)
@@ -102,11 +122,11 @@ def run_path(args, precision: str, keep_fields: bool):
         shape = [t for t in tune.variants() if t[0] == v][0]
     else:
         if args.mixed_variant == "auto":
-            v, planes, table = tune.autotune_mixed(block=1, dt=case.dt, sweeps=args.sweeps, residual_tol=args.epsi)
+            v, planes, table = tune.autotune_mixed(block=1, dt=case.dt, sweeps=args.sweeps, residual_tol=mixed_tol)
             print(json.dumps({"mixed_autotune": table}), file=sys.stderr)
         else:
             v, planes = int(args.mixed_variant), args.mixed_planes
-            H.call("biocfd_mixed_config", v, planes, args.sweeps, args.epsi)
+            H.call("biocfd_mixed_config", v, planes, args.sweeps, mixed_tol)
         shape = [t for t in tune.mixed_variants() if t[0] == v][0]
     kernel = {"variant": v, "family": ["cell lists", "pair-owned", "pair-owned, LDS coefficients"][shape[7]], "tile": [shape[1], shape[2]],
               "threads": shape[3], "depth": shape[4],
@@ -148,6 +168,7 @@ def run_path(args, precision: str, keep_fields: bool):
         kv = sim.blocks[0].kin_vec()
         H.call("biocfd_stress_calculation", 1, case.mu_f, case.rho_f, H.ptr(kv), H.ptr(res))
         out["forces"] = res.copy()
+    H.call("biocfd_sor_stop_rule", 0, 5)
     sim.close()
     return out
 
@@ -241,11 +262,13 @@ def field_errors(ref, got):
 
 
 def cpu_baseline(args):
-    """The reference CPU build (serial within one block) on one step of the
-    same 256^3 workload; setup excluded."""
+    """The reference CPU build (serial within one block) on `cpu_steps` steps
+    of the same 256^3 workload under the reference's stopping rule (501 SOR
+    iterations per step; the residual rule's ~9e4 iterations per step would
+    take hours on one core), setup excluded, on one OpenMP thread."""
     from biocfd_amd import cases as CS
 
-    case = make_case(args)
+    case = make_case(args, "reference")
     case.nsteps = args.cpu_steps
     case.dump_steps = ()
     wd = tempfile.mkdtemp(prefix="cpu_base_")
@@ -256,7 +279,10 @@ def cpu_baseline(args):
     else:
         kind, cmd = "port", [ORC_RUN, path, wd, "--time", str(args.cpu_steps)]
     env = dict(os.environ)
-    env.setdefault("OMP_NUM_THREADS", "16")
+    # the reference parallelises over blocks only (main.F90:133-148): one block
+    # runs on one thread whatever OMP_NUM_THREADS says; pin it to one
+    omp_env = env.get("OMP_NUM_THREADS")
+    env["OMP_NUM_THREADS"] = "1"
     r = subprocess.run(cmd, cwd=wd, capture_output=True, text=True, timeout=args.cpu_timeout, env=env)
     if r.returncode != 0:
         raise RuntimeError(f"cpu baseline failed: {r.stderr[-500:]}")
@@ -266,29 +292,79 @@ def cpu_baseline(args):
     else:
         parts = r.stdout.split()
         steps, sec = int(parts[1]), float(parts[3])
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
     return {
         "value": case.cells() * steps / sec / 1e6,
         "unit": "Mcell-steps/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"{steps} time step(s) of the same 256^3 sphere workload (pcItaMax={case.pc_ita_max}, "
-                  f"epsi={case.epsi}); serial within the block as the reference's CPU build is; "
-                  f"{sec:.1f} s",
+        "threads_used": 1,
+        "omp_num_threads_run": 1,
+        "omp_num_threads_env": omp_env,
+        "nproc": os.cpu_count(),
+        "cpus_allowed": affinity,
+        "seconds": round(sec, 2),
+        "sor_iterations_per_step": case.pc_ita_max + 1,
+        "sample": f"{steps} time step(s) of the same 256^3 sphere workload under the reference's stopping rule "
+                  f"(pcItaMax={case.pc_ita_max}, update norm < {case.epsi} every 5th iteration: "
+                  f"{case.pc_ita_max + 1} SOR iterations per step), OMP_NUM_THREADS=1 (the reference is serial "
+                  f"within a block), setup excluded; {sec:.1f} s; compare with reference_rule.value",
+    }
+
+
+def sor_roofline(args, run):
+    """Roofline object of an FP64 run's red-black SOR: algorithmic bytes
+    (24 B per cell and iteration, plus 16 B per cell per residual test of
+    the residual rule) over the GPU time of the SOR calls (HIP events)."""
+    tests = 0.0
+    if run.get("rule") == "residual" and run["sor_iterations"] > 0:
+        tests = run["sor_cell_iterations"] / args.check_every
+    bytes_ = SOR_BYTES_PER_CELL_ITER * run["sor_cell_iterations"] + 16.0 * tests
+    achieved = bytes_ / (run["sor_gpu_ms"] * 1e-3) / 1e9
+    kk = run["kernel"]
+    traffic = SOR_TRAFFIC_MEASURED.get((args.n, kk["family"], kk["tile"][0], kk["tile"][1], kk["threads"],
+                                        kk["depth"], kk["iters_per_pass"], kk["planes"]))
+    return {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic,
+        "kernel": (f"red-black SOR ({'k_sor_fused' if kk['family'] == 'cell lists' else 'k_sor_pair'}<double>: "
+                   f"{kk['iters_per_pass']} red+black iteration(s) per launch, tile {kk['tile'][0]}x{kk['tile'][1]})"
+                   if kk["variant"] > 0 else
+                   "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
+                  + ", 24 B/cell/iteration algorithmic"
+                  + (f" + 16 B/cell per residual test (every {args.check_every} iterations)"
+                     if run.get("rule") == "residual" else "")
+                  + "; achieved over the GPU time of the whole solve; traffic = HBM bytes per launch",
+        "us_per_iteration": round(1e3 * run["sor_gpu_ms"] / max(1, run["sor_iterations"]), 2),
     }
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--markers", type=int, default=8192)
-    ap.add_argument("--pc-ita-max", type=int, default=500)
-    ap.add_argument("--epsi", type=float, default=1e-8,
-                    help="SOR tolerance: update norm (FP64 method) / residual max-norm (mixed)")
+    ap.add_argument("--resid-tol", type=float, default=1e-8,
+                    help="max |b/dt - A pc| over the fluid cells that ends each pressure solve (both paths)")
+    ap.add_argument("--resid-ita-max", type=int, default=400000,
+                    help="pcItaMax of the residual-rule runs (out of reach: the residual ends every solve)")
+    ap.add_argument("--check-every", type=int, default=256,
+                    help="FP64 iterations between residual tests (the mixed path tests after every correction)")
+    ap.add_argument("--pc-ita-max", type=int, default=500, help="pcItaMax of the reference-rule run")
+    ap.add_argument("--epsi", type=float, default=1e-8, help="update-norm tolerance of the reference-rule run")
     ap.add_argument("--precision", choices=["both", "fp64", "mixed"], default="both")
-    ap.add_argument("--sweeps", type=int, default=128, help="FP32 sweeps per defect correction (mixed)")
+    ap.add_argument("--rule", choices=["both", "residual", "reference"], default="both",
+                    help="residual: the residual-rule runs (value, mixed); reference: the reference-rule run")
+    ap.add_argument("--sweeps", type=int, default=256, help="FP32 sweeps per defect correction (mixed)")
     ap.add_argument("--sor-variant", default="auto",
                     help="FP64 SOR kernel variant (0 = colour-split sweeps, >=1 fused tiles) or 'auto'")
     ap.add_argument("--sor-planes", type=int, default=32, help="planes per workgroup of a fused variant")
@@ -316,12 +392,28 @@ def main():
         return
 
     both = args.precision == "both"
-    g = run_path(args, "fp64", keep_fields=both) if args.precision in ("both", "fp64") else None
-    m = run_path(args, "mixed", keep_fields=both) if args.precision in ("both", "mixed") else None
-    main_run = g if g is not None else m
+    resid = args.rule in ("both", "residual")
+    g = m = None
+    if resid:
+        if args.precision in ("both", "fp64"):
+            g = run_path(args, "fp64", keep_fields=both, rule="residual")
+            g["rule"] = "residual"
+        if args.precision in ("both", "mixed"):
+            m = run_path(args, "mixed", keep_fields=both, rule="residual")
+    ref = None
+    if args.rule in ("both", "reference") and args.precision in ("both", "fp64"):
+        ref = run_path(args, "fp64", keep_fields=False, rule="reference")
+        ref["rule"] = "reference"
+    main_run = g if g is not None else (m if m is not None else ref)
+    rule_txt = (f"every pressure solve to max|b/dt - A pc| < {args.resid_tol} over the fluid cells (FP64 residual, "
+                f"tested every {args.check_every} FP64 iterations / after every {args.sweeps}-sweep mixed "
+                f"correction; pcItaMax={args.resid_ita_max} not reached)"
+                if main_run is not ref else
+                f"the reference's stopping rule (update norm < {args.epsi} every 5th iteration, "
+                f"pcItaMax={args.pc_ita_max})")
     out = {
         "metric": "Mcell-steps/s at 256^3 FP64 vs mixed-precision at 1e-8 residual; HBM %peak",
-        "value": round(main_run["mcell_steps_per_s"], 3),
+        "value": round(main_run["mcell_steps_per_s"], 4),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -330,42 +422,28 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if g is not None else "f32+f64",
+        "dtype": "f64" if main_run is not m else "f32+f64",
         "data": "synthetic (seeded 1e-3 noise on u=1 inflow; Fibonacci-hull sphere)",
         "config": {"workload": f"sphere{args.n}: single block {args.n}^3 on [0,8]x[0,4]x[0,4], sphere d=1, "
-                               f"{args.markers} markers, Re=300, dt=5e-4, omega=1.8, pcItaMax={args.pc_ita_max}, "
-                               f"epsi={args.epsi}",
-                   "value_is": "FP64 method" if g is not None else "mixed precision",
+                               f"{args.markers} markers, Re=300, dt=5e-4, omega=1.8",
+                   "stop_rule": rule_txt,
+                   "value_is": ("FP64 method" if main_run is not m else "mixed precision")
+                               + (" at the residual" if main_run is not ref else " under the reference's rule"),
                    "sor_iterations_per_step": main_run["sor_iters_per_step"],
+                   "residual_linf_last_step": main_run["residual_linf"],
                    "sor_kernel": main_run["kernel"]},
         "roofline": None,
         "cpu_baseline": None,
     }
     if g is not None:
-        achieved = SOR_BYTES_PER_CELL_ITER * g["sor_cell_iterations"] / (g["sor_gpu_ms"] * 1e-3) / 1e9
-        kk = g["kernel"]
-        traffic = SOR_TRAFFIC_MEASURED.get((args.n, kk["family"], kk["tile"][0], kk["tile"][1], kk["threads"],
-                                            kk["depth"], kk["iters_per_pass"], kk["planes"]))
-        out["roofline"] = {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "kernel": (f"red-black SOR ({'k_sor_fused' if kk['family'] == 'cell lists' else 'k_sor_pair'}<double>: "
-                       f"{kk['iters_per_pass']} red+black iteration(s) per launch, tile {kk['tile'][0]}x{kk['tile'][1]})"
-                       if kk["variant"] > 0 else
-                       "red-black SOR iteration (2 colour-sweep launches + norm reduction launch)")
-                      + ", 24 B/cell/iteration algorithmic; traffic = HBM bytes per launch",
-            "us_per_iteration": round(1e3 * g["sor_gpu_ms"] / max(1, g["sor_iterations"]), 2),
-        }
-        out["fp64_residual_linf"] = g["residual_linf"]
+        out["roofline"] = sor_roofline(args, g)
+    elif ref is not None:
+        out["roofline"] = sor_roofline(args, ref)
     if m is not None:
         kb = mixed_bytes_per_cell(args.sweeps)
         solver_s = m["sor_gpu_ms"] * 1e-3
         mixed = {
-            "value": round(m["mcell_steps_per_s"], 3),
+            "value": round(m["mcell_steps_per_s"], 4),
             "ms_per_step": round(m["ms_per_step"], 3),
             "sweeps_per_step": m["sor_iters_per_step"],
             "sweeps_per_correction": args.sweeps,
@@ -378,8 +456,21 @@ def main():
         }
         if g is not None:
             mixed["speedup_vs_fp64"] = round(m["mcell_steps_per_s"] / g["mcell_steps_per_s"], 3)
+            mixed["fp64_sor_iterations_per_step"] = g["sor_iters_per_step"]
+            mixed["fp64_residual_linf"] = g["residual_linf"]
             mixed["rel_err_vs_fp64"] = field_errors(g, m)
         out["mixed"] = mixed
+    if ref is not None and main_run is not ref:
+        out["reference_rule"] = {
+            "value": round(ref["mcell_steps_per_s"], 3),
+            "ms_per_step": round(ref["ms_per_step"], 3),
+            "stop_rule": f"update norm < {args.epsi} every 5th iteration, pcItaMax={args.pc_ita_max} "
+                         f"(PcorVcor.F90:351-375)",
+            "sor_iterations_per_step": ref["sor_iters_per_step"],
+            "residual_linf_last_step": ref["residual_linf"],
+            "sor_kernel": ref["kernel"],
+            "roofline": sor_roofline(args, ref),
+        }
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args)

@@ -35,8 +35,9 @@ static_assert(kResHalo <= kResNT, "one halo cell per thread");
 // With kUpdate: pc' = pc + e (every interior cell; e is zero off the fluid)
 // is written to pc_out and the residual is taken of pc'; without, of pc.
 // r = b/dt - A pc' at interior fluid cells in FP64, stored as FP32 into the
-// colour-split arrays (marker at the other interior cells), with the
-// per-workgroup max |r|.
+// colour-split arrays (marker at the other interior cells) when kStore, with
+// the per-workgroup max |r| (without kStore only the maxima are written: the
+// FP64 method's residual stopping test).
 //
 // The workgroup marches its tile in k with the values of plane k (tile and
 // halo) in one of two LDS planes: step k stores plane k+1 (loaded in step
@@ -44,7 +45,7 @@ static_assert(kResHalo <= kResNT, "one halo cell per thread");
 // and computes plane k; i and j neighbours come from LDS, k neighbours from
 // registers.  HBM traffic per cell: pc 8, e 4, b/dt 8 read, pc' 8, r 4
 // written; the halo is re-read from L2.
-template <bool kUpdate>
+template <bool kUpdate, bool kStore = true>
 __global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const double *__restrict__ pc,
                                                    const float *__restrict__ e, double *__restrict__ pc_out,
                                                    const double *__restrict__ bred, const double *__restrict__ bblack,
@@ -147,8 +148,10 @@ __global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const dou
         const double ar = fabs(res);
         m = m > ar ? m : ar;
       }
-      const bool red = ((i + jL + ty + 4 * r + k) & 1) != 0;
-      (red ? rred : rblack)[hbase + hoff[r]] = rv;
+      if (kStore) {
+        const bool red = ((i + jL + ty + 4 * r + k) & 1) != 0;
+        (red ? rred : rblack)[hbase + hoff[r]] = rv;
+      }
     }
     for (int r = 0; r < 2; ++r) {
       vm[r] = v0[r];
@@ -381,7 +384,10 @@ int mixed_sor(Block &b, int64_t pc_ita_max, double dt) {
   const int k = g_mx.sweeps;
   const long long cycles = (budget + k - 1) / k;
   int64_t nwg = residual_launch(b, b.f[BF_PC], nullptr, nullptr, b.sor);
-  hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, nwg, g_mx.tol, start, 0ll, budget, 0ll);
+  // the residual of the call's starting pc (the first correction's right-hand
+  // side) never stops the call: at least one correction runs, as the
+  // reference's loop runs at least one iteration (PcorVcor.F90:309)
+  hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, nwg, -1.0, start, 0ll, budget, 0ll);
   BCFD_CHECK_LAUNCH();
   const long long chunk = (48 + k - 1) / k;  // host polls about every 48 sweeps
   for (long long c = 0; c < cycles; ++c) {
@@ -429,6 +435,17 @@ int mixed_time(Block &b, int variant, int planes, int iters, double dt, double *
   BCFD_CHECK_LAUNCH();
   *ms = t;
   return 0;
+}
+
+// max |b/dt - A pc| over the fluid cells of `pc`, per workgroup into the
+// block's partials (the FP64 method's residual stopping test; the caller
+// reduces them); returns the number of workgroups
+int64_t residual_check_launch(Block &b, const double *pc, const SorState *st) {
+  const int NXh = (b.d.NX + 1) / 2;
+  dim3 blk(kResNT, 1, 1), grd(div_up(b.d.nx, kResTI), div_up(b.d.ny, kResTJ), div_up(b.d.nz, kResPlanes));
+  hipLaunchKernelGGL((k_mx_resupd<false, false>), grd, blk, 0, rt().stream, b.d, NXh, pc, nullptr, nullptr, b.bs[1],
+                     b.bs[0], b.A[0], b.A[1], b.A[2], nullptr, nullptr, st, b.partials);
+  return (int64_t)grd.x * grd.y * grd.z;
 }
 
 // max-norm of b/dt - A pc over the fluid cells (diagnostic, FP64)

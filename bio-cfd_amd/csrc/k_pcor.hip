@@ -506,6 +506,21 @@ constexpr int kNumSorVariants = (int)(sizeof(kFused) / sizeof(kFused[0]));
 static int g_sor_variant = 26;  // the fastest at 256^3 on MI355X (pair-owned 64x32, LDS coefficients, 2 iterations per pass)
 static int g_sor_planes = 32;
 
+// Stopping test of the FP64 SOR calls.  rule 0 (default): the reference's,
+// max |pc - pco| < epsi tested every fifth iteration (PcorVcor.F90:351-375).
+// rule 1: the FP64 residual, max |b/dt - A pc| over the fluid cells < epsi,
+// tested every `cadence` iterations (and after the last of the budget);
+// nIterPcor and the pcItaMax exit are the reference's in both.
+struct SorStop {
+  int rule = 0;
+  int cadence = 5;
+};
+static SorStop g_stop;
+static int check_period() { return g_stop.rule == 1 ? g_stop.cadence : 5; }
+
+int64_t residual_check_launch(Block &b, const double *pc, const SorState *st);
+int64_t residual_workgroups(const Block &b);
+
 static int enqueue_split_iterations(Block &b, SorState *st, long long n, int64_t pc_ita_max, double epsi) {
   hipStream_t s = rt().stream;
   const int NXh = (b.d.NX + 1) / 2;
@@ -556,22 +571,31 @@ static int enqueue_fused_passes(Block &b, SorState *st, int variant, int planes,
   a.Az = b.A[2];
   a.omega = b.omega;
   const long long stop = done_iters + chunk < total ? done_iters + chunk : total;
+  const long long C = check_period();
   while (done_iters < stop) {
     const long long it = start + done_iters + 1;
-    const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
+    const long long to_check = (C - it % C) % C + 1;  // up to and including the next check iteration
     long long L = v.nit;
     if (L > total - done_iters) L = total - done_iters;
     if (L > to_check) L = to_check;
     const long long last_it = it + L - 1;
-    const int check = last_it % 5 == 0;
+    const int check = last_it % C == 0;
     a.src = (q & 1) ? b.pc2 : b.f[BF_PC];
     a.dst = (q & 1) ? b.f[BF_PC] : b.pc2;
-    (L == v.nit ? v.launch : v.launch1)(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy, check, st, b.partials);
+    // rule 1 takes no update norm: the residual of the pass's result is
+    // reduced after it instead
+    (L == v.nit ? v.launch : v.launch1)(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy,
+                                        g_stop.rule == 0 ? check : 0, st, b.partials);
     done_iters += L;
     ++q;
-    if (check || done_iters == total)
+    if (g_stop.rule == 1 && (check || done_iters == total)) {
+      const int64_t nres = residual_check_launch(b, a.dst, st);
+      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, 1,
+                         (long long)pc_ita_max, epsi, b.partials, nres);
+    } else if (check || done_iters == total) {
       hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, check,
                          (long long)pc_ita_max, epsi, b.partials, nwg);
+    }
   }
   BCFD_CHECK_LAUNCH();
   return 0;
@@ -602,6 +626,7 @@ static int sor_prepare(Block &b, int variant, int planes, double dt) {
     int gx, gy, gz;
     fused_grid(b, kFused[variant], planes, gx, gy, gz);
     if (grow(&b.partials, &b.cap_partials, (int64_t)gx * gy * gz)) return -1;
+    if (g_stop.rule == 1 && grow(&b.partials, &b.cap_partials, residual_workgroups(b))) return -1;
     BCFD_HIP(hipMemcpyAsync(b.pc2, b.f[BF_PC], sizeof(double) * (size_t)b.n2, hipMemcpyDeviceToDevice, s));
   } else {
     hipLaunchKernelGGL(k_split, fg, fb, 0, s, b.d, NXh, b.f[BF_PC], b.pcs[1], b.pcs[0]);
@@ -698,6 +723,10 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   hipStream_t s = rt().stream;
   hipEvent_t e0 = nullptr;
   const int variant = g_sor_variant, planes = planes_for({&b}, variant, g_sor_planes);
+  if (g_stop.rule == 1 && variant == 0) {
+    set_error("redblack_sor: the residual stopping rule needs a fused SOR variant (>= 1), not variant 0");
+    return -1;
+  }
   if (sor_prepare(b, variant, planes, dt)) return -1;
   hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
   BCFD_CHECK_LAUNCH();
@@ -813,8 +842,8 @@ static int batched_calls(const std::vector<Block *> &blocks, long long start, in
 // kernel and the calls allow
 int redblack_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt, double epsi) {
   const int variant = g_sor_variant;
-  const bool can = g_sor_batch && precision_mode() == 0 && !sor_timing_on() && kFused[variant].batch &&
-                   blocks.size() > 1;
+  const bool can = g_sor_batch && precision_mode() == 0 && g_stop.rule == 0 && !sor_timing_on() &&
+                   kFused[variant].batch && blocks.size() > 1;
   if (!can) {
     for (Block *b : blocks)
       if (redblack_sor(*b, pc_ita_max, dt, epsi)) return -1;
@@ -999,6 +1028,23 @@ int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_ite
 int biocfd_sor_batching(int on) {
   BCFD_API_LOCK();
   g_sor_batch = on ? 1 : 0;
+  return 0;
+}
+
+int biocfd_sor_stop_rule(int rule, int cadence) {
+  BCFD_API_LOCK();
+  if (rule == 0) {
+    g_stop.rule = 0;
+    g_stop.cadence = 5;
+    return 0;
+  }
+  if (rule != 1 || cadence < 1) {
+    set_error("sor_stop_rule: rule %d cadence %d (rule 0 = update norm every 5th iteration, rule 1 = FP64 "
+              "residual every cadence >= 1 iterations)", rule, cadence);
+    return -1;
+  }
+  g_stop.rule = 1;
+  g_stop.cadence = cadence;
   return 0;
 }
 

@@ -315,6 +315,12 @@ module biocfd_hip
       integer(c_int), value :: planes_per_workgroup
     end function
 
+    integer(c_int) function biocfd_sor_stop_rule(rule, cadence) bind(C, name="biocfd_sor_stop_rule")
+      import :: c_int
+      integer(c_int), value :: rule
+      integer(c_int), value :: cadence
+    end function
+
     integer(c_int) function biocfd_sor_variant_info(variant, info) bind(C, name="biocfd_sor_variant_info")
       import :: c_int, c_int64_t, c_double, c_ptr
       integer(c_int), value :: variant

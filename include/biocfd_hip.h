@@ -134,6 +134,15 @@ int biocfd_correct(int blk, double dt);
  * A fused variant may run two iterations per pass (temporal blocking); every
  * variant gives the same bits, the choice only changes speed. */
 int biocfd_sor_config(int variant, int planes_per_workgroup);
+/* Stopping test of the FP64 SOR calls (rule 1 has no reference counterpart).
+ * rule 0 (default): the reference's, max |pc - pco| < epsi over the fluid
+ * cells tested every fifth iteration (PcorVcor.F90:351-375).  rule 1: the
+ * FP64 residual max |b/dt - A pc| < epsi over the fluid cells, tested every
+ * `cadence` (>= 1) iterations and after the last iteration of the budget;
+ * derr2 then holds the residual of the last test.  nIterPcor accumulation
+ * and the pcItaMax exit are the reference's under both rules.  Rule 1 needs
+ * a fused SOR variant and runs the fine blocks' calls one after the other. */
+int biocfd_sor_stop_rule(int rule, int cadence);
 /* Batching of the fine blocks' SOR calls in poisson_solver (no reference
  * counterpart; the reference runs them one block per OpenMP thread,
  * PcorVcor.F90:86-129): on (the default), the calls of all fine blocks that
