@@ -111,6 +111,15 @@ struct IbData {
   // solid list
   int64_t cap_solid = 0;
   int *solid_ijk = nullptr;
+  // nsMomentum2order's TS pass: touched and deferred masks (2 n2 bytes), the
+  // deferred cells and the ghost each of their targets takes
+  uint8_t *ts_mask = nullptr;
+  int64_t cap_ts_mask = 0;
+  int *ts_list = nullptr, *ts_best = nullptr;
+  int64_t cap_ts_list = 0, cap_ts_best = 0;
+  // stress_calculation: per-element terms, the coefficients and the miss counter
+  double *stress_work = nullptr;
+  int64_t cap_stress = 0;
   // forcing scratch (per task)
   int64_t cap_task = 0;
   double *task_out = nullptr;

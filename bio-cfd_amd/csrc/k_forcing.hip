@@ -505,17 +505,72 @@ __global__ void k_stress_terms(FArgs a, double mu_f, double rho_f, double *terms
   t[0] = sh0; t[1] = f0; t[2] = sh1; t[3] = f1; t[4] = area_xz; t[5] = area_yz;
 }
 
-// the reference sums element by element; one thread keeps that order
-__global__ void k_stress_sum(const double *terms, int64_t ne, double *out) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double s[6] = {0, 0, 0, 0, 0, 0};
-  for (int64_t e = 0; e < ne; ++e)
-    for (int q = 0; q < 6; ++q) s[q] = s[q] + terms[6 * e + q];
-  const double ax = 0.5 * s[4], ay = 0.5 * s[5];
-  out[0] = 2.0 * (s[0] / ax);
-  out[1] = 2.0 * (s[1] / ay);
-  out[2] = 2.0 * (s[2] / ax);
-  out[3] = 2.0 * (s[3] / ay);
+// The reference sums element by element (StressCalc.f90:75, serial in the
+// CPU build).  One wavefront keeps that order: chunks of 64 elements' terms
+// are staged through LDS with coalesced loads, then lanes 0..5 each add
+// their term of the chunk's elements in element order.
+constexpr int kStressChunk = 64;
+__global__ void __launch_bounds__(64) k_stress_sum(const double *__restrict__ terms, int64_t ne,
+                                                   double *__restrict__ out) {
+  __shared__ double buf[6 * kStressChunk];
+  const int lane = threadIdx.x;
+  double acc = 0.0;
+  for (int64_t e0 = 0; e0 < ne; e0 += kStressChunk) {
+    const int n = ne - e0 < kStressChunk ? (int)(ne - e0) : kStressChunk;
+    for (int q = lane; q < 6 * n; q += 64) buf[q] = terms[6 * e0 + q];
+    __syncthreads();
+    if (lane < 6)
+      for (int e = 0; e < n; ++e) acc = acc + buf[6 * e + lane];
+    __syncthreads();
+  }
+  __shared__ double sums[6];
+  if (lane < 6) sums[lane] = acc;
+  __syncthreads();
+  if (lane == 0) {
+    const double ax = 0.5 * sums[4], ay = 0.5 * sums[5];
+    out[0] = 2.0 * (sums[0] / ax);
+    out[1] = 2.0 * (sums[1] / ay);
+    out[2] = 2.0 * (sums[2] / ax);
+    out[3] = 2.0 * (sums[3] / ay);
+  }
+}
+
+__global__ void k_stress_clear(int *miss) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *miss = 0;
+}
+
+// stress_calculation of block b with its terms (6 per element), the four
+// coefficients and the miss counter in `work` (6 E + 8 doubles); leaves
+// the result in h_pinned
+static int stress_enqueue(Block &b, double mu_f, double rho_f, const double kin[BK_N], double *work) {
+  IbData &ib = b.ib;
+  hipStream_t s = rt().stream;
+  FArgs a{};
+  a.d = b.d;
+  for (int q = 0; q < BG_NGRID; ++q) a.g[q] = b.g[q];
+  a.n_cells = ib.n_elems;
+  for (int q = 0; q < BE_NELEM; ++q) a.el[q] = ib.elem[q];
+  a.surf = ib.surf;
+  for (int q = 0; q < BK_N; ++q) a.kin[q] = kin[q];
+  a.fld[0] = b.f[BF_P]; a.fld[1] = b.f[BF_U]; a.fld[2] = b.f[BF_V]; a.fld[3] = b.f[BF_W];
+  double *res = work + 6 * ib.n_elems;
+  a.miss = (int *)(res + 4);
+  hipLaunchKernelGGL(k_stress_clear, dim3(1), dim3(64), 0, s, a.miss);
+  hipLaunchKernelGGL(k_stress_terms, dim3(div_up(ib.n_elems, 256)), dim3(256), 0, s, a, mu_f, rho_f, work);
+  hipLaunchKernelGGL(k_stress_sum, dim3(1), dim3(64), 0, s, work, ib.n_elems, res);
+  BCFD_CHECK_LAUNCH();
+  BCFD_HIP(hipMemcpyAsync(rt().h_pinned, res, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+static int stress_result(double out[4]) {
+  const double *h = rt().h_pinned;
+  int miss;
+  memcpy(&miss, &h[4], sizeof(int));
+  for (int q = 0; q < 4; ++q) out[q] = h[q];
+  if (miss != 0) { set_error("stress_calculation: %d probe lookups missed", miss); return -2; }
+  return 0;
 }
 
 }  // namespace bcfd
@@ -543,31 +598,10 @@ int biocfd_stress_calculation(int id, double mu_f, double rho_f, const double ki
   if (!b) return -1;
   IbData &ib = b->ib;
   if (!ib.present) { set_error("block %d has no immersed body", id); return -1; }
-  hipStream_t s = rt().stream;
-  FArgs a{};
-  a.d = b->d;
-  for (int q = 0; q < BG_NGRID; ++q) a.g[q] = b->g[q];
-  a.n_cells = ib.n_elems;
-  for (int q = 0; q < BE_NELEM; ++q) a.el[q] = ib.elem[q];
-  a.surf = ib.surf;
-  for (int q = 0; q < BK_N; ++q) a.kin[q] = kin[q];
-  a.fld[0] = b->f[BF_P]; a.fld[1] = b->f[BF_U]; a.fld[2] = b->f[BF_V]; a.fld[3] = b->f[BF_W];
-  double *terms = nullptr;
-  if (dev_alloc(&terms, 6 * ib.n_elems + 8)) return -1;
-  a.miss = (int *)(terms + 6 * ib.n_elems + 4);
-  BCFD_HIP(hipMemsetAsync(a.miss, 0, sizeof(int), s));
-  hipLaunchKernelGGL(k_stress_terms, dim3(div_up(ib.n_elems, 256)), dim3(256), 0, s, a, mu_f, rho_f, terms);
-  hipLaunchKernelGGL(k_stress_sum, dim3(1), dim3(64), 0, s, terms, ib.n_elems, terms + 6 * ib.n_elems);
-  BCFD_CHECK_LAUNCH();
-  double h[5];
-  BCFD_HIP(hipMemcpyAsync(h, terms + 6 * ib.n_elems, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
-  BCFD_HIP(hipStreamSynchronize(s));
-  (void)hipFree(terms);
-  int miss;
-  memcpy(&miss, &h[4], sizeof(int));
-  for (int q = 0; q < 4; ++q) out[q] = h[q];
-  if (miss != 0) { set_error("stress_calculation: %d probe lookups missed", miss); return -2; }
-  return 0;
+  // per-element terms and the result, kept on the block between calls
+  if (grow(&ib.stress_work, &ib.cap_stress, 6 * ib.n_elems + 8)) return -1;
+  if (stress_enqueue(*b, mu_f, rho_f, kin, ib.stress_work)) return -1;
+  return stress_result(out);
 }
 
 }  // extern "C"

@@ -271,6 +271,8 @@ static const Sweep32Variant kSweep32[] = {
     PS(64, 32, 1024, 1, 1, 1),
     PS(64, 22, 1024, 1, 2, 1),
     PS(64, 22, 512, 1, 2, 1),
+    PS(64, 32, 1024, 1, 2, 0),
+    PS(64, 32, 768, 1, 2, 1),
 };
 #undef SV
 #undef PS
@@ -278,7 +280,7 @@ constexpr int kNumSweep32 = (int)(sizeof(kSweep32) / sizeof(kSweep32[0]));
 
 struct MixedConfig {
   int precision = 0;        // 0: FP64 method, 1: mixed
-  int variant = 21;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 1024 threads, 2 sweeps per pass)
+  int variant = 26;         // the fastest at 256^3 on MI355X (pair-owned 64x32, 768 threads, 2 sweeps per pass)
   int planes = 32;
   int sweeps = 128;         // FP32 sweeps per correction (u, v, w within 4.2e-7 of FP64 after 100 steps at 256^3)
   double tol = 1e-8;        // max-norm of the FP64 residual

@@ -623,20 +623,21 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
     BCFD_CHECK_LAUNCH();
     return 0;
   }
-  // TS path: touched positions -> deferred mask -> dense pass -> ordered pass
-  uint8_t *touched = nullptr, *deferred = nullptr;
-  if (dev_alloc(&touched, b->n2) || dev_alloc(&deferred, b->n2)) return -1;
+  // TS path: touched positions -> deferred mask -> dense pass -> ordered pass;
+  // the masks, the deferred list and the ghost choices are kept on the block
+  IbData &ib = b->ib;
+  if (grow(&ib.ts_mask, &ib.cap_ts_mask, 2 * b->n2)) return -1;
+  uint8_t *touched = ib.ts_mask, *deferred = ib.ts_mask + b->n2;
   BCFD_HIP(hipMemsetAsync(touched, 0, b->n2, s));
   BCFD_HIP(hipMemsetAsync(deferred, 0, b->n2, s));
   hipLaunchKernelGGL(k_ts_touch, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched);
   hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched, deferred);
   launch_dense(a, F, b, s, deferred);
   BCFD_CHECK_LAUNCH();
-  int *list = nullptr;
-  int64_t cap = 0, cnt = 0;
-  if (compact_mask_lin(*b, deferred, &list, &cap, &cnt)) return -1;
+  int64_t cnt = 0;
+  if (compact_mask_lin(*b, deferred, &ib.ts_list, &ib.cap_ts_list, &cnt)) return -1;
+  const int *list = ib.ts_list;
   TsArgs ts{};
-  IbData &ib = b->ib;
   ts.ts_ijk = ib.ts_ijk; ts.cap_ts = ib.cap_ts; ts.index_ts = ib.index_ts;
   for (int q = 0; q < BN_NNEL; ++q) ts.nel[q] = ib.nel[q];
   for (int q = 0; q < BE_NELEM; ++q) ts.elem[q] = ib.elem[q];
@@ -644,18 +645,13 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   ts.x1 = b->g[BG_X1]; ts.y1 = b->g[BG_Y1]; ts.z1 = b->g[BG_Z1];
   ts.xp = b->g[BG_XP]; ts.yp = b->g[BG_YP]; ts.zp = b->g[BG_ZP];
   ts.n_ts = n_ts;
-  int *best = nullptr;
   if (cnt > 0) {
-    if (dev_alloc(&best, 36 * cnt)) return -1;
+    if (grow(&ib.ts_best, &ib.cap_ts_best, 36 * cnt)) return -1;
+    int *best = ib.ts_best;
     hipLaunchKernelGGL(k_ts_best, dim3((unsigned)cnt), dim3(256), 0, s, a, list, (int)cnt, ts, best);
     hipLaunchKernelGGL(k_ts_ordered, dim3(1), dim3(256), 0, s, a, b->f[BF_U], b->f[BF_V], b->f[BF_W],
                        b->f[BF_P], b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, ts, best);
   }
   BCFD_CHECK_LAUNCH();
-  BCFD_HIP(hipStreamSynchronize(s));
-  (void)hipFree(list);
-  (void)hipFree(best);
-  (void)hipFree(touched);
-  (void)hipFree(deferred);
   return 0;
 }

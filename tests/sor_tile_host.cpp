@@ -430,6 +430,10 @@ int main() {
   bad += run_pair<float, 64, 32, 512, 1, 0>();
   bad += run_pair<float, 64, 32, 512, 2, 0>();
   bad += run_pair<float, 128, 16, 512, 1, 1>();
+  bad += run_pair<float, 64, 32, 1024, 2, 0>();
+  bad += run_pair<float, 64, 32, 1024, 1, 0>();
+  bad += run_pair<float, 64, 32, 768, 2, 1>();
+  bad += run_pair<float, 64, 32, 768, 1, 1>();
   std::printf(bad ? "FAILED %d\n" : "all cases bit-identical\n", bad);
   return bad ? 1 : 0;
 }

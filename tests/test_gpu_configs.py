@@ -29,7 +29,7 @@ from parity import compare, run_oracle
 pytestmark = pytest.mark.gpu
 
 # mixed precision: library defaults restored after each test
-MIXED_DEFAULT = (21, 32, 128, 1e-8)
+MIXED_DEFAULT = (26, 32, 128, 1e-8)
 MIXED_CONTRACT = {"u": 1e-6, "v": 1e-6, "w": 1e-6, "ut": 1e-6, "vt": 1e-6, "wt": 1e-6, "p": 1e-6, "pc": 1e-5,
                   "b": 1e-5}
 MIXED_OTHER = 1e-5

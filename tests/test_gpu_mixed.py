@@ -17,11 +17,12 @@ pytestmark = pytest.mark.gpu
 STEPS = 3
 # the library's mixed-precision defaults (variant, planes, sweeps per correction, residual tolerance),
 # restored after each test
-DEFAULT = (21, 32, 128, 1e-8)
+DEFAULT = (26, 32, 128, 1e-8)
 # (variant, planes, sweeps per correction): the per-phase cell-list FP32 kernel at 4 sweeps and the
 # pair-owned FP32 kernels (fused multiply-adds, hardware reciprocal; one and two sweeps per pass) at
 # 32, 64 and the default 128 (these cases' SOR calls are shorter than 64 sweeps)
-CONFIGS = [(1, 32, 4), (15, 16, 32), (21, 32, 32), (21, 32, 64), (21, 32, 128)]
+CONFIGS = [(1, 32, 4), (15, 16, 32), (21, 32, 32), (21, 32, 64), (21, 32, 128), (25, 32, 64), (26, 32, 32),
+           (26, 32, 128)]
 # max |mixed - fp64| / max |fp64| allowed per field.  The FP32 rounding the corrections carry grows
 # with the sweeps per correction: at 4 sweeps the fields stay within 1e-8 of the FP64 run, at 32 within
 # 2e-7 (velocities, p) on these cases, so the set for 32 and 64 is the accuracy contract of the mixed
@@ -101,3 +102,24 @@ def test_mixed_stops_at_residual_tolerance(config):
     assert int(d["nIterPcor"]) % config[2] == 0
     assert r < 1e-6
     assert abs(float(d["derr2"]) - r) <= 1e-12 * max(1.0, r)
+
+
+def test_mixed_runs_at_least_one_correction():
+    """A residual tolerance the starting pc already meets does not end the
+    call before its first correction: the reference's loop runs at least one
+    iteration per call (PcorVcor.F90:309), so nIterPcor advances by one
+    correction's sweeps."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = CS.config_sphere_64(n=24, markers=256, nsteps=1, pc_ita_max=400, epsi=0.0)
+    sim = Simulation(case, precision="mixed")
+    try:
+        H.call("biocfd_mixed_config", 26, 32, 32, 1e30)
+        sim.step()
+        d = sim.dump()[0]
+    finally:
+        sim.close()
+        H.call("biocfd_mixed_config", *DEFAULT)
+    assert int(d["nIterPcor"]) == 32
