@@ -236,6 +236,7 @@ int ib_list_capacity(Block &b, int64_t n_ib);
 int build_cell_lists(Block &b, bool ib_lists);
 int build_ts(Block &b);
 int precision_mode();
+int stress_scratch_init(int device);
 int mixed_sor(Block &b, int64_t pc_ita_max, double dt);
 
 }  // namespace bcfd

@@ -564,6 +564,34 @@ static int stress_enqueue(Block &b, double mu_f, double rho_f, const double kin[
   return 0;
 }
 
+// Scratch of biocfd_stress_calculation_noalloc: the per-element terms (6 per
+// element, up to kNoallocElems elements), then the four coefficients and the
+// miss counter.  A file-scope device array: each device the library runs on
+// holds its own copy (3 MB) until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the device.
+constexpr int64_t kNoallocElems = 65536;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[6 * kNoallocElems + 8];
+static double *g_scratch_ptr[kMaxDevices] = {};
+
+// the scratch address on `device` (current), once per device; called by
+// biocfd_init
+int stress_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("stress scratch: device %d outside 0..%d", device, kMaxDevices - 1);
+    return -1;
+  }
+  if (g_scratch_ptr[device]) return 0;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    set_error("hipGetSymbolAddress(stress scratch) on device %d: %s", device, hipGetErrorString(e));
+    return -1;
+  }
+  g_scratch_ptr[device] = (double *)p;
+  return 0;
+}
+
 static int stress_result(double out[4]) {
   const double *h = rt().h_pinned;
   int miss;
@@ -601,6 +629,27 @@ int biocfd_stress_calculation(int id, double mu_f, double rho_f, const double ki
   // per-element terms and the result, kept on the block between calls
   if (grow(&ib.stress_work, &ib.cap_stress, 6 * ib.n_elems + 8)) return -1;
   if (stress_enqueue(*b, mu_f, rho_f, kin, ib.stress_work)) return -1;
+  return stress_result(out);
+}
+
+// Calls on one device share its scratch: they are serialised by the library
+// lock and run in order on the library's stream; calls that overlapped on one
+// device (two processes, or another stream) would race on it.
+int biocfd_stress_calculation_noalloc(int id, double mu_f, double rho_f, const double kin[BK_N], double out[4]) {
+  BCFD_API_LOCK();
+  Block *b = get_block(id);
+  if (!b) return -1;
+  IbData &ib = b->ib;
+  if (!ib.present) { set_error("block %d has no immersed body", id); return -1; }
+  if (ib.n_elems > kNoallocElems) {
+    set_error("stress_calculation_noalloc: %lld elements exceed the scratch maximum of %lld",
+              (long long)ib.n_elems, (long long)kNoallocElems);
+    return -1;
+  }
+  const int dev = rt().device;
+  double *work = dev >= 0 && dev < kMaxDevices ? g_scratch_ptr[dev] : nullptr;
+  if (!work) { set_error("stress_calculation_noalloc: no scratch on device %d (biocfd_init)", dev); return -1; }
+  if (stress_enqueue(*b, mu_f, rho_f, kin, work)) return -1;
   return stress_result(out);
 }
 

@@ -180,6 +180,7 @@ int biocfd_init(int device) {
     return -1;
   }
   BCFD_HIP(hipSetDevice(device));
+  if (stress_scratch_init(device)) return -1;
   BCFD_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
   BCFD_HIP(hipHostMalloc((void **)&g_rt.h_pinned, 64 * sizeof(double), hipHostMallocDefault));
   BCFD_HIP(hipHostMalloc((void **)&g_rt.h_pinned_i, 64 * sizeof(int64_t), hipHostMallocDefault));

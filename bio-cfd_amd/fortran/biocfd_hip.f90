@@ -473,6 +473,16 @@ module biocfd_hip
       real(c_double) :: out(*)
     end function
 
+    integer(c_int) function biocfd_stress_calculation_noalloc(blk, mu_f, rho_f, kin, out) bind(C, &
+        name="biocfd_stress_calculation_noalloc")
+      import :: c_int, c_int64_t, c_double, c_ptr
+      integer(c_int), value :: blk
+      real(c_double), value :: mu_f
+      real(c_double), value :: rho_f
+      real(c_double) :: kin(*)
+      real(c_double) :: out(*)
+    end function
+
     integer(c_int) function biocfd_interface_create(itf, a_blk, b_blk, a_msh, b_msh, &
         extent) bind(C, name="biocfd_interface_create")
       import :: c_int, c_int64_t, c_double, c_ptr

@@ -98,6 +98,7 @@ SIGNATURES = {
     "biocfd_velocity_forcing_ghost": [_I, _P],
     "biocfd_pressure_forcing_ghost": [_I, _P],
     "biocfd_stress_calculation": [_I, _D, _D, _P, _P],
+    "biocfd_stress_calculation_noalloc": [_I, _D, _D, _P, _P],
     "biocfd_interface_create": [_I, _I, _I, _L, _L, _P],
     "biocfd_coarse_update": [_I],
     "biocfd_fine_update_bd": [_I],

@@ -207,6 +207,17 @@ int biocfd_pressure_forcing_ghost(int blk, const double kin[BK_N]); /* :362-448 
  * pressure_drag, viscous_lift, pressure_lift} coefficients */
 int biocfd_stress_calculation(int blk, double mu_f, double rho_f, const double kin[BK_N],
                               double out[4]);
+/* biocfd_stress_calculation without any device allocation (no counterpart in
+ * the reference): for callers that cannot allocate, such as code under HIP
+ * graph capture or a tight serving loop.  Its scratch (per-element terms,
+ * the coefficients) is a static device array of the library holding up to
+ * 65536 elements (3 MB per device, kept until the process ends); a body with
+ * more elements is refused with an error.  Same arguments, kernels and
+ * results as biocfd_stress_calculation.  Every call on one device shares
+ * that scratch: calls are serialised by the library, and calls that
+ * overlapped on one device would race. */
+int biocfd_stress_calculation_noalloc(int blk, double mu_f, double rho_f, const double kin[BK_N],
+                                      double out[4]);
 
 /* ---- coarse/fine interfaces ---- */
 /* register interface `itf` (1-based) between coarse block a and fine block b
