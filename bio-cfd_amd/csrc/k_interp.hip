@@ -376,4 +376,15 @@ int biocfd_fine_update_bd(int which) {
   return interface_fine_update(which);
 }
 
+int biocfd_fine_update_bd_one(int itf, int which) {
+  BCFD_API_LOCK();
+  if (ensure_init()) return -1;
+  if (which < 0 || which > 2) { set_error("fine_update_bd_one: bad selector %d", which); return -1; }
+  if (itf < 1 || itf > kMaxInterfaces || !rt().itf[itf].live) {
+    set_error("fine_update_bd_one: no interface %d", itf);
+    return -1;
+  }
+  return interface_fine_update_one(itf, which);
+}
+
 }  // extern "C"

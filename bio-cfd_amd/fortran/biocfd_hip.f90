@@ -504,6 +504,12 @@ module biocfd_hip
       integer(c_int), value :: which
     end function
 
+    integer(c_int) function biocfd_fine_update_bd_one(itf, which) bind(C, name="biocfd_fine_update_bd_one")
+      import :: c_int
+      integer(c_int), value :: itf
+      integer(c_int), value :: which
+    end function
+
     integer(c_int) function biocfd_interface_table(itf, comp, axis, host, counter, cap) bind(C, &
         name="biocfd_interface_table")
       import :: c_int, c_int64_t, c_double, c_ptr

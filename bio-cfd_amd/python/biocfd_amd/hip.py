@@ -102,6 +102,7 @@ SIGNATURES = {
     "biocfd_interface_create": [_I, _I, _I, _L, _L, _P],
     "biocfd_coarse_update": [_I],
     "biocfd_fine_update_bd": [_I],
+    "biocfd_fine_update_bd_one": [_I, _I],
     "biocfd_interface_table": [_I, _I, _I, _P, _P, _L],
     "biocfd_shift_surface_nodes_initial": [_L, _P, _P, _P, _P, _D, _D, _L, _D, _D, _D, _P, _P, _P, _P],
     "biocfd_compute_surface_variables": [_L, _P, _P, _P, _P, _D, _D, _L, _D, _D, _D, _D, _P, _P, _P, _P],

@@ -231,6 +231,9 @@ int biocfd_coarse_update(int which);
 /* fineUpdate_bd_mv (which 0) / fineUpdate_newv_bd (1) / fineUpdate_pc_bd (2),
  * FineInterp_bound.f90:17-544 */
 int biocfd_fine_update_bd(int which);
+/* the same on interface itf alone (fineUpdate_bd_mv(g) with which 0,
+ * FineInterp_bound.f90:311-544) */
+int biocfd_fine_update_bd_one(int itf, int which);
 int biocfd_interface_table(int itf, int comp, int axis, int64_t *host, int64_t *counter, int64_t cap);
 
 /* ---- body kinematics (host arithmetic on host arrays, Search.F90:21-208) ---- */
