@@ -66,6 +66,8 @@ SOR_TRAFFIC_MEASURED = {
     (256, "cell lists", 64, 32, 512, 1, 1, 32): 4.2695e+08,
     (256, "pair-owned, LDS coefficients", 64, 32, 512, 1, 2, 32): 4.6074e+08,
     (256, "pair-owned, LDS coefficients", 64, 32, 512, 1, 1, 32): 4.2833e+08,
+    (256, "pair-owned", 32, 32, 512, 1, 2, 64): 4.3925e+08,
+    (256, "pair-owned", 32, 32, 512, 1, 1, 64): 4.2003e+08,
 }
 
 

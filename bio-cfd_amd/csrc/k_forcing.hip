@@ -506,27 +506,48 @@ __global__ void k_stress_terms(FArgs a, double mu_f, double rho_f, double *terms
 }
 
 // The reference sums element by element (StressCalc.f90:75, serial in the
-// CPU build).  One wavefront keeps that order: chunks of 64 elements' terms
-// are staged through LDS with coalesced loads, then lanes 0..5 each add
-// their term of the chunk's elements in element order.
-constexpr int kStressChunk = 64;
-__global__ void __launch_bounds__(64) k_stress_sum(const double *__restrict__ terms, int64_t ne,
-                                                   double *__restrict__ out) {
-  __shared__ double buf[6 * kStressChunk];
-  const int lane = threadIdx.x;
-  double acc = 0.0;
-  for (int64_t e0 = 0; e0 < ne; e0 += kStressChunk) {
+// CPU build).  One workgroup keeps that order: the terms pass through two LDS
+// buffers of kStressChunk elements; while waves 1-3 load chunk c+1 into one,
+// lanes 0..5 of wave 0 add their term of chunk c's elements, in element
+// order, from the other (64 values at a time in registers, so the LDS reads
+// are in flight together and only the additions form a chain).
+constexpr int kStressChunk = 512;
+constexpr int kStressThreads = 256;
+__global__ void __launch_bounds__(kStressThreads) k_stress_sum(const double *__restrict__ terms, int64_t ne,
+                                                               double *__restrict__ out) {
+  __shared__ double buf[2][6 * kStressChunk];
+  __shared__ double sums[6];
+  const int t = threadIdx.x;
+  const int64_t nchunk = (ne + kStressChunk - 1) / kStressChunk;
+  auto load = [&](int64_t c, double *dst, int first, int stride) {
+    const int64_t e0 = c * kStressChunk;
     const int n = ne - e0 < kStressChunk ? (int)(ne - e0) : kStressChunk;
-    for (int q = lane; q < 6 * n; q += 64) buf[q] = terms[6 * e0 + q];
-    __syncthreads();
-    if (lane < 6)
-      for (int e = 0; e < n; ++e) acc = acc + buf[6 * e + lane];
+    for (int q = first; q < 6 * n; q += stride) dst[q] = terms[6 * e0 + q];
+  };
+  if (nchunk > 0) load(0, buf[0], t, kStressThreads);
+  __syncthreads();
+  double acc = 0.0;
+  for (int64_t c = 0; c < nchunk; ++c) {
+    if (t >= 64) {
+      if (c + 1 < nchunk) load(c + 1, buf[(c + 1) & 1], t - 64, kStressThreads - 64);
+    } else if (t < 6) {
+      const double *b = buf[c & 1];
+      const int64_t e0 = c * kStressChunk;
+      const int n = ne - e0 < kStressChunk ? (int)(ne - e0) : kStressChunk;
+      for (int g = 0; g < n; g += 64) {
+        double v[64];
+#pragma unroll
+        for (int q = 0; q < 64; ++q) v[q] = g + q < n ? b[6 * (g + q) + t] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 64; ++q)
+          if (g + q < n) acc = acc + v[q];
+      }
+    }
     __syncthreads();
   }
-  __shared__ double sums[6];
-  if (lane < 6) sums[lane] = acc;
+  if (t < 6) sums[t] = acc;
   __syncthreads();
-  if (lane == 0) {
+  if (t == 0) {
     const double ax = 0.5 * sums[4], ay = 0.5 * sums[5];
     out[0] = 2.0 * (sums[0] / ax);
     out[1] = 2.0 * (sums[1] / ay);
@@ -557,7 +578,7 @@ static int stress_enqueue(Block &b, double mu_f, double rho_f, const double kin[
   a.miss = (int *)(res + 4);
   hipLaunchKernelGGL(k_stress_clear, dim3(1), dim3(64), 0, s, a.miss);
   hipLaunchKernelGGL(k_stress_terms, dim3(div_up(ib.n_elems, 256)), dim3(256), 0, s, a, mu_f, rho_f, work);
-  hipLaunchKernelGGL(k_stress_sum, dim3(1), dim3(64), 0, s, work, ib.n_elems, res);
+  hipLaunchKernelGGL(k_stress_sum, dim3(1), dim3(kStressThreads), 0, s, work, ib.n_elems, res);
   BCFD_CHECK_LAUNCH();
   BCFD_HIP(hipMemcpyAsync(rt().h_pinned, res, 5 * sizeof(double), hipMemcpyDeviceToHost, s));
   BCFD_HIP(hipStreamSynchronize(s));

@@ -366,6 +366,7 @@ static float *sweeps_launch(Block &b, int variant, int planes, int k, const SorS
 }
 
 int sor_timing_on();
+int sor_poll(const SorState *st, long long &k, bool &stop);
 int sor_timing_begin(hipEvent_t *e0);
 int sor_timing_end(hipEvent_t e0, const Block &b, long long start);
 
@@ -392,6 +393,7 @@ int mixed_sor(Block &b, int64_t pc_ita_max, double dt) {
   hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, nwg, -1.0, start, 0ll, budget, 0ll);
   BCFD_CHECK_LAUNCH();
   const long long chunk = (48 + k - 1) / k;  // host polls about every 48 sweeps
+  long long polls = 0;
   for (long long c = 0; c < cycles; ++c) {
     // the last correction takes only the sweeps left in the budget
     const int kc = (int)((budget - c * k) < k ? (budget - c * k) : k);
@@ -405,9 +407,9 @@ int mixed_sor(Block &b, int64_t pc_ita_max, double dt) {
                        budget, c + 1);
     BCFD_CHECK_LAUNCH();
     if ((c + 1) % chunk == 0 && c + 1 < cycles) {
-      BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
-      BCFD_HIP(hipStreamSynchronize(s));
-      if (h->done) break;
+      bool stop = false;
+      if (sor_poll(b.sor, polls, stop)) return -1;
+      if (stop) break;
     }
   }
   if (sor_timing_end(e0, b, start)) return -1;

@@ -718,6 +718,29 @@ static int planes_for(const std::vector<Block *> &blocks, int variant, int plane
   return p;
 }
 
+// Host polls of a SOR call's state without draining the queue: the state
+// after chunk k is copied to a pinned slot and read only once chunk k+1 is
+// queued, so the GPU has work while the host waits.  A chunk queued past the
+// stop runs as launches that see the done flag and return at once.
+static hipEvent_t g_poll_ev[2] = {nullptr, nullptr};
+
+int sor_poll(const SorState *st, long long &k, bool &stop) {
+  hipStream_t s = rt().stream;
+  SorState *slots = (SorState *)rt().h_pinned + 1;  // slots 1 and 2 (slot 0: the synchronous reads)
+  for (auto &e : g_poll_ev)
+    if (!e) BCFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const int cur = (int)(k & 1);
+  BCFD_HIP(hipMemcpyAsync(&slots[cur], st, sizeof(SorState), hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipEventRecord(g_poll_ev[cur], s));
+  stop = false;
+  if (k > 0) {
+    BCFD_HIP(hipEventSynchronize(g_poll_ev[cur ^ 1]));
+    stop = slots[cur ^ 1].done != 0;
+  }
+  ++k;
+  return 0;
+}
+
 int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   if (precision_mode() == 1) return mixed_sor(b, pc_ita_max, dt);
   hipStream_t s = rt().stream;
@@ -739,14 +762,14 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   if (remaining < 1) remaining = 1;
   const long long total = remaining;
   const long long chunk = 50;
-  long long done_iters = 0, q = 0;
+  long long done_iters = 0, q = 0, polls = 0;
   while (done_iters < total) {
     if (enqueue_iterations(b, b.sor, variant, planes, start, total, chunk, done_iters, q, pc_ita_max, epsi))
       return -1;
     if (done_iters >= total) break;
-    BCFD_HIP(hipMemcpyAsync(h, b.sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
-    BCFD_HIP(hipStreamSynchronize(s));
-    if (h->done) break;
+    bool stop = false;
+    if (sor_poll(b.sor, polls, stop)) return -1;
+    if (stop) break;
   }
   if (sor_timing_end(e0, b, start)) return -1;
   return sor_finish(b, b.sor, variant);
@@ -1090,6 +1113,23 @@ int biocfd_sor_time(int id, int variant, int planes_per_workgroup, int iters, do
     return -1;
   }
   return sor_time(*b, variant, planes_per_workgroup, iters, dt, ms);
+}
+
+int biocfd_projection(int id, int64_t n_iter, double dt) {
+  BCFD_API_LOCK();
+  Block *b = get_block(id);
+  if (!b) return -1;
+  if (check_ready(*b)) return -1;
+  if (n_iter < 1) { set_error("projection: iteration count %lld < 1", (long long)n_iter); return -1; }
+  if (precision_mode() != 0) { set_error("projection: the FP64 method only (set_precision 0)"); return -1; }
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_zero_pcor, dim3(div_up(b->n2, 256)), dim3(256), 0, s, b->f[BF_B], b->f[BF_PC], b->n2);
+  hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b->sor, 1);
+  BCFD_CHECK_LAUNCH();
+  if (compute_div(*b)) return -1;
+  // a negative tolerance never stops the iterations: nIterPcor > n_iter - 1 ends them after n_iter
+  if (redblack_sor(*b, n_iter - 1, dt, -1.0)) return -1;
+  return correct(*b, dt);
 }
 
 int biocfd_compute_div(int id) {

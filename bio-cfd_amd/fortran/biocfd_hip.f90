@@ -376,6 +376,36 @@ module biocfd_hip
       real(c_double) :: out(*)
     end function
 
+    integer(c_int) function biocfd_projection(blk, n_iter, dt) bind(C, name="biocfd_projection")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: n_iter
+      real(c_double), value :: dt
+    end function
+
+    integer(c_int) function biocfd_projection_adjoint(blk, n_iter, dt, ubar, vbar, wbar, pbar, utbar, vtbar, &
+        wtbar) bind(C, name="biocfd_projection_adjoint")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: n_iter
+      real(c_double), value :: dt
+      real(c_double) :: ubar(*), vbar(*), wbar(*), pbar(*), utbar(*), vtbar(*), wtbar(*)
+    end function
+
+    integer(c_int) function biocfd_coarse_update_adjoint(itf, comp, lam_coarse, lam_fine) bind(C, &
+        name="biocfd_coarse_update_adjoint")
+      import :: c_int, c_double
+      integer(c_int), value :: itf, comp
+      real(c_double) :: lam_coarse(*), lam_fine(*)
+    end function
+
+    integer(c_int) function biocfd_fine_update_bd_adjoint(itf, comp, lam_fine, lam_coarse) bind(C, &
+        name="biocfd_fine_update_bd_adjoint")
+      import :: c_int, c_double
+      integer(c_int), value :: itf, comp
+      real(c_double) :: lam_fine(*), lam_coarse(*)
+    end function
+
     integer(c_int) function biocfd_compute_surface_norm(blk, xnode1, ynode1, znode1, inor) bind(C, &
         name="biocfd_compute_surface_norm")
       import :: c_int, c_int64_t, c_double, c_ptr

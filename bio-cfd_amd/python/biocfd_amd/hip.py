@@ -83,6 +83,10 @@ SIGNATURES = {
     "biocfd_mixed_variant_info": [_I, _P],
     "biocfd_mixed_time": [_I, _I, _I, _I, _D, _P],
     "biocfd_residual_norm": [_I, _D, _P],
+    "biocfd_projection": [_I, _L, _D],
+    "biocfd_projection_adjoint": [_I, _L, _D, _P, _P, _P, _P, _P, _P, _P],
+    "biocfd_coarse_update_adjoint": [_I, _I, _P, _P],
+    "biocfd_fine_update_bd_adjoint": [_I, _I, _P, _P],
     "biocfd_compute_surface_norm": [_I, _P, _P, _P, _L],
     "biocfd_tagging": [_I],
     "biocfd_selective_retagging": [_I, _L],

@@ -177,6 +177,34 @@ int biocfd_mixed_time(int blk, int variant, int planes_per_workgroup, int iters,
 /* max over fluid cells of |b/dt - A pc| for block `blk` (FP64 diagnostic) */
 int biocfd_residual_norm(int blk, double dt, double *out);
 
+/* ---- adjoint (no reference counterpart: the reference has no adjoint) ----
+ * FP64, geometry frozen (tags, index sets and interface tables as the
+ * forward's).  Host arrays in the block layout ((nx+2)(ny+2)(nz+2) doubles).
+ *
+ * biocfd_projection: on block blk, b = pc = 0, nIterPcor = 0, computeDiv,
+ * n_iter red-black SOR iterations of the FP64 method, correctPressure and
+ * correctVelocity (PcorVcor.F90:67-77, 222-377) in place on ut, vt, wt, p;
+ * no interface updates and no boundary conditions.
+ * biocfd_projection_adjoint: its transpose with respect to (ut, vt, wt):
+ * given the adjoints of (ut, vt, wt, p) after it, the adjoints of (ut, vt,
+ * wt) before it (the adjoint of p passes unchanged): the transposed
+ * correction, n_iter transposed red-black iterations in reverse order
+ * (black then red, each with the neighbours' coefficients) and the
+ * transposed divergence. */
+int biocfd_projection(int blk, int64_t n_iter, double dt);
+int biocfd_projection_adjoint(int blk, int64_t n_iter, double dt, const double *ubar, const double *vbar,
+                              const double *wbar, const double *pbar, double *utbar, double *vtbar,
+                              double *wtbar);
+/* Transposes of the trilinear interface updates of one component (0 p / pc,
+ * 1 u / ut, 2 v / vt, 3 w / wt) on interface itf (CoarseInterp.f90:447-502,
+ * FineInterp_bound.f90:554-603).  coarse: on entry the adjoint of the
+ * coarse field after coarseUpdate, on return before it (its targets
+ * zeroed); fine: the restriction's transpose added.  fine_bd: on entry the
+ * adjoint of the fine field after fineUpdate_*_bd, on return before it;
+ * coarse: the prolongation's transpose added. */
+int biocfd_coarse_update_adjoint(int itf, int comp, double *lam_coarse, double *lam_fine);
+int biocfd_fine_update_bd_adjoint(int itf, int comp, double *lam_fine, double *lam_coarse);
+
 /* ---- immersed-boundary geometry, search and tagging (Search.F90) ---- */
 /* computeSurfaceNorm :210-258 (nodes are the moved positions xnode1..) */
 int biocfd_compute_surface_norm(int blk, const double *xnode1, const double *ynode1,
