@@ -497,3 +497,18 @@ def config_uneven(n_wake: int = 15, seed: int = 1, sides=tuple(range(32, 129, 16
     args = dict(dt=0.0005, re=200.0, nsteps=nsteps, pc_ita_max=500, epsi=1e-8, freq=0.25, phase_angle_deg=90.0)
     args.update(kw)
     return Case("uneven17", blocks, itfs, **args)
+
+
+def config_adjoint(n: int = 96, markers: int = 2048, nsteps: int = 40, **kw) -> Case:
+    """The adjoint workload: single block on [0,3]^3, an ellipsoid (semi-axes
+    0.4, 0.2, 0.1; ID 51, so it pitches about x with the reference's flap
+    kinematics, Search.F90:186-195) at (1, 1.5, 1.5) heaving 0.2 in y,
+    pitch amplitude 20 deg, phase 90 deg, frequency 0.5, Re = 100 on the
+    major axis (re_code = 100 / 0.8), dt = 0.001."""
+    body = BodyPlacement(ellipsoid_body(markers, (0.4, 0.2, 0.1), surf_id=51), (1.0, 1.5, 1.5), a0_deg=20.0,
+                         yamp=0.2, bfreq=0.5)
+    blk = BlockSpec((0.0, 0.0, 0.0), (3.0, 3.0, 3.0), (n, n, n), omega=1.7, body=body, moving=True)
+    args = dict(dt=0.001, re=re_code(100.0, 1.0, 0.8), nsteps=nsteps, pc_ita_max=500, freq=0.5,
+                phase_angle_deg=90.0)
+    args.update(kw)
+    return Case(f"adjoint{n}", [blk], **args)
