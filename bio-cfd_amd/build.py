@@ -20,7 +20,8 @@ LIB = os.path.join(LIB_DIR, "libbiocfd_hip.so")
 OBJ_DIR = os.path.join(HERE, "lib", "obj")
 
 SOURCES = ["runtime.hip", "k_pcor.hip", "k_momentum.hip", "k_search.hip", "k_forcing.hip",
-           "k_interp.hip", "k_mixed.hip", "k_move.hip", "k_output.hip", "k_adjoint.hip"]
+           "k_interp.hip", "k_mixed.hip", "k_move.hip", "k_output.hip", "k_adjoint.hip",
+           "k_health.hip"]
 
 # FP64 parity needs no contraction; packed-FP32 instruction forms are kept off.
 HIP_FLAGS = [

@@ -76,6 +76,8 @@ struct SorState {
   double derr_std_st;            // steady-state norm
   unsigned long long std_bits;
   long long call_iters;          // iterations executed by the current SOR call
+  int exhausted;                 // SOR calls since the last nIterPcor reset that ended on their
+                                 // iteration budget, not their tolerance
 };
 
 struct IbData {
@@ -206,10 +208,18 @@ Runtime &rt();
 Block *get_block(int id);
 int ensure_init();
 
+// > 0: the device allocation that many calls from now fails as if hipMalloc
+// had (biocfd_inject_alloc_failure)
+extern int g_alloc_fail_in;
+
 template <typename T>
 int dev_alloc(T **p, int64_t n) {
   if (*p) { (void)hipFree(*p); *p = nullptr; }
   if (n <= 0) n = 1;
+  if (g_alloc_fail_in > 0 && --g_alloc_fail_in == 0) {
+    set_error("hipMalloc(%lld bytes): injected allocation failure", (long long)(sizeof(T) * n));
+    return -1;
+  }
   hipError_t e = hipMalloc((void **)p, sizeof(T) * (size_t)n);
   if (e != hipSuccess) {
     set_error("hipMalloc(%lld bytes): %s", (long long)(sizeof(T) * n), hipGetErrorString(e));
@@ -236,6 +246,7 @@ int ib_list_capacity(Block &b, int64_t n_ib);
 int build_cell_lists(Block &b, bool ib_lists);
 int build_ts(Block &b);
 int precision_mode();
+int health_after_solve(int first, int last);
 int stress_scratch_init(int device);
 int mixed_sor(Block &b, int64_t pc_ita_max, double dt);
 

@@ -193,7 +193,7 @@ __global__ void k_mx_finish(const SorState *__restrict__ st, double *__restrict_
 // call set-up of the mixed solve (reset_iter: also nIterPcor and derr2)
 __global__ void k_mx_reset(SorState *st, int reset_iter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (reset_iter) { st->n_iter = 0; st->derr2 = 0.0; st->derr_std_st = 0.0; }
+    if (reset_iter) { st->n_iter = 0; st->derr2 = 0.0; st->derr_std_st = 0.0; st->exhausted = 0; }
     st->derr4 = 11111.0;
     st->done = 0;
     st->call_iters = 0;
@@ -212,7 +212,10 @@ __global__ void __launch_bounds__(1024) k_mx_end(SorState *st, const double *par
   st->derr2 = s_r;
   st->n_iter = start + sweeps;
   st->call_iters = updates;
-  if (s_r < tol || sweeps >= budget) st->done = 1;
+  if (s_r < tol || sweeps >= budget) {
+    st->done = 1;
+    st->exhausted += !(s_r < tol);
+  }
 }
 
 // ---------------------------------------------------------------- host

@@ -31,7 +31,7 @@ __global__ void k_zero_pcor(double *__restrict__ b, double *__restrict__ pc, int
 
 __global__ void k_sor_reset_call(SorState *st, int reset_iter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (reset_iter) { st->n_iter = 0; st->derr2 = 0.0; st->derr_std_st = 0.0; }
+    if (reset_iter) { st->n_iter = 0; st->derr2 = 0.0; st->derr_std_st = 0.0; st->exhausted = 0; }
     st->derr4 = 11111.0;
     st->max_bits = 0ull;
     st->done = 0;
@@ -171,7 +171,10 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
   st->call_iters = it - start;
   st->passes = (int)passes;
   st->derr2 = st->derr4;
-  if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
+  if (st->derr4 < epsi || it > pc_ita_max) {
+    st->done = 1;
+    st->exhausted += !(st->derr4 < epsi);
+  }
 }
 
 // k_sor_fused_end for every block of a batched launch: workgroup b reduces
@@ -204,7 +207,10 @@ __global__ void __launch_bounds__(1024) k_sor_batch_end(const BatchBlockT<double
   st->call_iters = it - start;
   st->passes = (int)passes;
   st->derr2 = st->derr4;
-  if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
+  if (st->derr4 < epsi || it > pc_ita_max) {
+    st->done = 1;
+    st->exhausted += !(st->derr4 < epsi);
+  }
 }
 
 // the passes of one SOR call alternate pc -> pc2 -> pc ...; after an odd
@@ -239,7 +245,10 @@ __global__ void __launch_bounds__(1024) k_sor_iter_end(SorState *st, long long p
     st->derr4 = m;
   }
   st->derr2 = st->derr4;
-  if (st->derr4 < epsi || it > pc_ita_max) st->done = 1;
+  if (st->derr4 < epsi || it > pc_ita_max) {
+    st->done = 1;
+    st->exhausted += !(st->derr4 < epsi);
+  }
 }
 
 // correctPressure + correctVelocity (PcorVcor.F90:245-286)
@@ -1190,7 +1199,7 @@ int biocfd_poisson_solver(int first, int last, int64_t pc_ita_max, double epsi, 
   for (int g = first; g <= last; ++g)
     if (norm_copy(rt().blk[g], dt)) return -1;
   if (interface_fine_update(0) || interface_coarse_update(0)) return -1;
-  return 0;
+  return health_after_solve(first, last);
 }
 
 }  // extern "C"

@@ -12,6 +12,7 @@ namespace bcfd {
 static thread_local char g_err[1024] = "";
 static Runtime g_rt;
 static std::mutex g_mu;
+int g_alloc_fail_in = 0;
 
 std::recursive_mutex &api_mutex() {
   static std::recursive_mutex m;
@@ -169,6 +170,13 @@ extern "C" {
 
 const char *biocfd_last_error(void) { return g_err; }
 
+int biocfd_inject_alloc_failure(int nth) {
+  BCFD_API_LOCK();
+  if (nth < 0) { set_error("inject_alloc_failure: %d < 0", nth); return -1; }
+  g_alloc_fail_in = nth;
+  return 0;
+}
+
 int biocfd_init(int device) {
   BCFD_API_LOCK();
   std::lock_guard<std::mutex> lk(g_mu);
@@ -282,6 +290,31 @@ int biocfd_finalize(void) {
   return 0;
 }
 
+}  // extern "C"
+
+namespace bcfd {
+// the device arrays of a new block, cleared (ib_map: -1)
+static int block_alloc(Block &b) {
+  for (int q = 0; q < BF_NFIELD; ++q) {
+    if (q == BF_PCO) continue;
+    if (dev_alloc(&b.f[q], b.n2)) return -1;
+    BCFD_HIP(hipMemsetAsync(b.f[q], 0, sizeof(double) * b.n2, g_rt.stream));
+  }
+  for (int t = 0; t < BT_NTAG; ++t) {
+    int64_t n = tag_size(b, t);
+    if (dev_alloc(&b.tag[t], n)) return -1;
+    BCFD_HIP(hipMemsetAsync(b.tag[t], 0, n, g_rt.stream));
+  }
+  if (dev_alloc(&b.ib.ib_map, b.n2)) return -1;
+  BCFD_HIP(hipMemsetAsync(b.ib.ib_map, 0xff, sizeof(int) * b.n2, g_rt.stream));
+  if (dev_alloc(&b.sor, 1)) return -1;
+  BCFD_HIP(hipMemsetAsync(b.sor, 0, sizeof(SorState), g_rt.stream));
+  return 0;
+}
+}  // namespace bcfd
+
+extern "C" {
+
 int biocfd_block_create(int id, int64_t nx, int64_t ny, int64_t nz, double omega, int64_t cintp,
                         double dx, double dy, double dz) {
   BCFD_API_LOCK();
@@ -303,20 +336,12 @@ int biocfd_block_create(int id, int64_t nx, int64_t ny, int64_t nz, double omega
   b.n2 = (int64_t)b.d.NX * b.d.NY * b.d.NZ;
   b.n3 = (int64_t)(nx + 3) * (ny + 3) * (nz + 3);
   b.omega = omega; b.cintp = cintp; b.dx = dx; b.dy = dy; b.dz = dz;
-  for (int q = 0; q < BF_NFIELD; ++q) {
-    if (q == BF_PCO) continue;
-    if (dev_alloc(&b.f[q], b.n2)) return -1;
-    BCFD_HIP(hipMemsetAsync(b.f[q], 0, sizeof(double) * b.n2, g_rt.stream));
+  if (block_alloc(b)) {
+    // a failed create leaves no block and no device memory behind
+    (void)hipStreamSynchronize(g_rt.stream);
+    free_block(b);
+    return -1;
   }
-  for (int t = 0; t < BT_NTAG; ++t) {
-    int64_t n = tag_size(b, t);
-    if (dev_alloc(&b.tag[t], n)) return -1;
-    BCFD_HIP(hipMemsetAsync(b.tag[t], 0, n, g_rt.stream));
-  }
-  if (dev_alloc(&b.ib.ib_map, b.n2)) return -1;
-  BCFD_HIP(hipMemsetAsync(b.ib.ib_map, 0xff, sizeof(int) * b.n2, g_rt.stream));
-  if (dev_alloc(&b.sor, 1)) return -1;
-  BCFD_HIP(hipMemsetAsync(b.sor, 0, sizeof(SorState), g_rt.stream));
   b.live = true;
   return 0;
 }

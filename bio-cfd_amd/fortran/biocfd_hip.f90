@@ -221,6 +221,33 @@ module biocfd_hip
       real(c_double) :: derr_std_st(*)
     end function
 
+    integer(c_int) function biocfd_sor_exhausted(blk, count) bind(C, name="biocfd_sor_exhausted")
+      import :: c_int
+      integer(c_int), value :: blk
+      integer(c_int) :: count
+    end function
+
+    integer(c_int) function biocfd_health_check(blk, n_nonfinite, vel_max, where) bind(C, name="biocfd_health_check")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int64_t) :: n_nonfinite
+      real(c_double) :: vel_max
+      integer(c_int64_t) :: where(4)
+    end function
+
+    integer(c_int) function biocfd_health_config(check_after_solve, vel_limit, exhausted_is_error) &
+        bind(C, name="biocfd_health_config")
+      import :: c_int, c_double
+      integer(c_int), value :: check_after_solve
+      real(c_double), value :: vel_limit
+      integer(c_int), value :: exhausted_is_error
+    end function
+
+    integer(c_int) function biocfd_inject_alloc_failure(nth) bind(C, name="biocfd_inject_alloc_failure")
+      import :: c_int
+      integer(c_int), value :: nth
+    end function
+
     integer(c_int) function biocfd_body_create(blk, n_nodes, n_elems, el_p1, el_p2, el_p3, surf_id, &
         search_box) bind(C, name="biocfd_body_create")
       import :: c_int, c_int64_t, c_double, c_ptr

@@ -62,6 +62,10 @@ SIGNATURES = {
     "biocfd_coef_download": [_I, _I, _P, _L],
     "biocfd_table_download": [_I, _I, _I, _P, _L],
     "biocfd_sor_state": [_I, _P, _P, _P],
+    "biocfd_sor_exhausted": [_I, _P],
+    "biocfd_health_check": [_I, _P, _P, _P],
+    "biocfd_health_config": [_I, _D, _I],
+    "biocfd_inject_alloc_failure": [_I],
     "biocfd_body_create": [_I, _L, _L, _P, _P, _P, _P, _P],
     "biocfd_coefficient_matrix": [_I, _I],
     "biocfd_non_uni_coeff": [_I],
@@ -126,7 +130,13 @@ SIGNATURES = {
 
 
 class BiocfdError(RuntimeError):
-    pass
+    """A non-zero return of the library; code is that return value (-1
+    for most failures, -3 / -4 for the failure checks of the pressure
+    solve)."""
+
+    def __init__(self, msg: str, code: int | None = None):
+        super().__init__(msg)
+        self.code = code
 
 
 def load(path: str = LIB_PATH) -> C.CDLL:
@@ -161,5 +171,5 @@ def call(name: str, *args) -> int:
     rc = getattr(lib(), name)(*args)
     if rc != 0:
         msg = lib().biocfd_last_error().decode(errors="replace")
-        raise BiocfdError(f"{name} failed ({rc}): {msg}")
+        raise BiocfdError(f"{name} failed ({rc}): {msg}", rc)
     return rc

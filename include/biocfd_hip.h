@@ -86,6 +86,32 @@ int biocfd_table_download(int blk, int which, int is_ck, double *host, int64_t n
 /* SOR state of the last poissonSolver call: nIterPcor, derr2, derrStdSt */
 int biocfd_sor_state(int blk, int64_t *n_iter, double *derr2, double *derr_std_st);
 
+/* ---- failure detection (no reference counterpart: the reference checks
+ * nothing; an SOR call that reaches pcItaMax ends silently, PcorVcor.F90:181,
+ * 351-375).  All off by default, so the reference's behaviour is kept. ---- */
+/* Red-black SOR calls of the last poissonSolver call on this block that ended
+ * on their iteration budget (nIterPcor > pcItaMax) rather than their
+ * tolerance (counted on the device by the end-of-call kernels; a block is
+ * solved up to twice per poissonSolver call). */
+int biocfd_sor_exhausted(int blk, int *count);
+/* Scan u, v, w, p of a block (every cell, ghost layer included): the number
+ * of non-finite values, the largest |u|, |v|, |w| among the finite ones, and
+ * where[0..3] = field (0 u, 1 v, 2 w, 3 p), i, j, k (1-based) of the first
+ * non-finite value in (field, k, j, i) order, or all -1. */
+int biocfd_health_check(int blk, int64_t *n_nonfinite, double *vel_max, int64_t where[4]);
+/* check_after_solve != 0: biocfd_poisson_solver ends with the scan of its
+ * blocks and returns -3 if a value is non-finite or a velocity component
+ * exceeds vel_limit (vel_limit 0: no limit), and, with exhausted_is_error,
+ * -4 if an SOR call reached pcItaMax; the error text names the block (and
+ * the first non-finite cell).  The fields are left as the solve wrote them.
+ * Cost: one read of four fields per block and one stream synchronisation. */
+int biocfd_health_config(int check_after_solve, double vel_limit, int exhausted_is_error);
+/* Test hook: the nth device allocation from now (nth >= 1) fails as a failed
+ * hipMalloc would, with its error text marked as injected; 0 disarms it.  An
+ * entry point whose allocation fails returns -1 and leaves the library usable
+ * (a block whose creation fails is not created). */
+int biocfd_inject_alloc_failure(int nth);
+
 /* ---- immersed body (readSurfaceMeshGmsh target arrays, ReadInput.f90:375-420) ---- */
 int biocfd_body_create(int blk, int64_t n_nodes, int64_t n_elems, const int64_t *el_p1,
                        const int64_t *el_p2, const int64_t *el_p3, const int64_t *surf_id,
