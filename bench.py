@@ -372,9 +372,10 @@ def main():
     ap.add_argument("--sor-planes", type=int, default=32, help="planes per workgroup of a fused variant")
     ap.add_argument("--mixed-variant", default="auto", help="FP32 sweep kernel variant of the mixed solve or 'auto'")
     ap.add_argument("--mixed-planes", type=int, default=16, help="planes per workgroup of the FP32 sweep kernel")
-    ap.add_argument("--momentum-variant", type=int, default=0,
-                    help="dense momentum kernel: 1 = one launch per velocity component, 0 = all three per thread, "
-                         "2 = as 0 with four waves per SIMD")
+    ap.add_argument("--momentum-variant", type=int, default=4,
+                    help="dense momentum kernel (include/biocfd_hip.h, biocfd_momentum_config): 0 = all three "
+                         "components per thread per plane, 1 = one launch per component, 2 = as 0 with four waves "
+                         "per SIMD, 3..8 = plane march of 64 x TJ column tiles (4 = TJ 4, 32 planes, the default)")
     ap.add_argument("--momentum-divisions", action="store_true",
                     help="keep every grid division of the momentum stencil a division (no power-of-two products)")
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -127,9 +127,12 @@ int biocfd_non_uni_coeff(int blk);
 /* nsMomentum2order, NavierStokes.f90:252-912 */
 int biocfd_ns_momentum(int blk, double dt, double re);
 /* Dense momentum kernel of biocfd_ns_momentum (no reference counterpart):
- * variant 0 (the default) = the three components per thread in one launch,
- * 1 = one launch per velocity component, 2 = as 0 with its registers capped
- * for four waves per SIMD.  pow2_products (default 1): on a block whose
+ * variant 0 = the three components per thread, one workgroup per 64 x 4
+ * cells of a plane; 1 = one launch per velocity component; 2 = as 0 with its
+ * registers capped for four waves per SIMD; 3..8 = a workgroup marching a
+ * 64 x TJ column tile up KC planes with the planes staged in LDS, (TJ, KC) =
+ * (4, 16), (4, 32) (the default, 4), (8, 16), (8, 32), (4, 32) with
+ * registers capped for three waves per SIMD, (4, 8).  pow2_products (default 1): on a block whose
  * every axis has one power-of-two spacing, the divisions by grid spacings
  * are products by their exact reciprocals (the same correctly rounded
  * values).  All settings give the same bits. */

@@ -142,19 +142,20 @@ def _sphere256_reduced():
     return CS.config_sphere_256(n=(64, 32, 32), markers=512, nsteps=3, pc_ita_max=20, epsi=0.0)
 
 
-@pytest.mark.parametrize("pow2", [1, 0])
-def test_sphere256_reduced_fp64(pow2):
+@pytest.mark.parametrize("mom,pow2", [(4, 1), (4, 0), (0, 1)])
+def test_sphere256_reduced_fp64(mom, pow2):
     """The 256^3 workload's domain, body and parameters on a 64x32x32 grid
     (spacing 1/8 on every axis: the power-of-two reciprocal path, and with
-    pow2=0 the division path)."""
+    pow2=0 the division path), with the default plane-march momentum kernel
+    (4) and the per-plane one (0)."""
     from biocfd_amd import hip as H
 
     case = _sphere256_reduced()
     ref = _oracle(case, (0, 1, 3))
     try:
-        got = _run_gpu(case, (0, 1, 3), setup=lambda: H.call("biocfd_momentum_config", 0, pow2))
+        got = _run_gpu(case, (0, 1, 3), setup=lambda: H.call("biocfd_momentum_config", mom, pow2))
     finally:
-        H.call("biocfd_momentum_config", 0, 1)
+        H.call("biocfd_momentum_config", 4, 1)
     bad = _fp64_bad("sphere256 reduced", ref, got)
     assert not bad, "\n".join(bad[:40])
 

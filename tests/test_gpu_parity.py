@@ -11,20 +11,26 @@ pytestmark = pytest.mark.gpu
 STEPS = (0, 1, 3)
 
 
+MOM_DEFAULT = 4
+
+
 @pytest.mark.parametrize("name,batching,mom,pow2", [
-    ("channel", 1, 1, 1), ("sphere", 1, 1, 1), ("nested", 1, 1, 1), ("wing", 1, 1, 1), ("move", 1, 1, 1),
-    ("batched", 1, 1, 1), ("batched", 0, 1, 1),
-    ("sphere", 1, 0, 1), ("wing", 1, 0, 1), ("sphere", 1, 2, 1), ("wing", 1, 2, 1),
-    ("sphere", 1, 1, 0), ("wing", 1, 1, 0), ("nested", 1, 0, 0),
-    ("sphere32", 1, 1, 1), ("sphere32", 1, 0, 1), ("sphere32", 1, 2, 1), ("sphere32", 1, 1, 0),
-    ("wing32", 1, 1, 1), ("wing32", 1, 2, 1), ("wing32", 1, 0, 0)])
+    ("channel", 1, 4, 1), ("sphere", 1, 4, 1), ("nested", 1, 4, 1), ("wing", 1, 4, 1), ("move", 1, 4, 1),
+    ("batched", 1, 4, 1), ("batched", 0, 4, 1),
+    ("sphere", 1, 0, 1), ("wing", 1, 0, 1), ("sphere", 1, 1, 1), ("wing", 1, 1, 1), ("sphere", 1, 2, 1),
+    ("wing", 1, 2, 1), ("sphere", 1, 4, 0), ("wing", 1, 4, 0), ("nested", 1, 0, 0), ("wing", 1, 5, 1),
+    ("wing", 1, 7, 0), ("sphere", 1, 8, 1), ("channel", 1, 3, 0), ("nested", 1, 6, 1),
+    ("sphere32", 1, 4, 1), ("sphere32", 1, 0, 1), ("sphere32", 1, 1, 1), ("sphere32", 1, 2, 1), ("sphere32", 1, 4, 0),
+    ("sphere32", 1, 5, 0), ("wing32", 1, 4, 1), ("wing32", 1, 2, 1), ("wing32", 1, 0, 0), ("wing32", 1, 4, 0),
+    ("wing32", 1, 6, 1)])
 def test_time_step_parity(name, batching, mom, pow2):
     """batching: the fine blocks' SOR calls in one launch per pass (1, the
     default) or one block after the other (0).  mom: the dense momentum
-    kernel, all three components per thread (0, the default), one launch per
-    velocity component (1), or 0 with four waves per SIMD (2).  pow2:
-    divisions by power-of-two grid spacings as products by the exact
-    reciprocal (1, the default) or as divisions (0)."""
+    kernel (biocfd_momentum_config): 0 = all three components per thread per
+    plane, 1 = one launch per velocity component, 2 = 0 with four waves per
+    SIMD, 3..8 = the plane march (4, the default: 64 x 4 column tiles of 32
+    planes).  pow2: divisions by power-of-two grid spacings as products by
+    the exact reciprocal (1, the default) or as divisions (0)."""
     from biocfd_amd import hip as H
     from biocfd_amd.driver import Simulation
 
@@ -45,5 +51,5 @@ def test_time_step_parity(name, batching, mom, pow2):
         assert not bad, "\n".join(bad[:40])
     finally:
         H.call("biocfd_sor_batching", 1)
-        H.call("biocfd_momentum_config", 0, 1)
+        H.call("biocfd_momentum_config", MOM_DEFAULT, 1)
         sim.close()
