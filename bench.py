@@ -26,13 +26,18 @@ events on the runtime's stream; `cpu_baseline` times the reference's own CPU
 build (oracle/_ref/ref_harness) on one step of the same workload under the
 reference's rule, on one core.
 
---workload uneven: the batched multi-block workload instead (a coarse block
-192x128x128 and 16 fine blocks of uneven sizes, one with a heaving, pitching
-plate; cases.config_uneven), FP64, run twice on the same inputs: the fine
-blocks' SOR calls batched in one launch per pass, then one block after the
-other; one JSON line with both throughputs, the speed-up and whether the two
-runs agree bit for bit.  With --sor-variant auto (the default) the SOR
-variant is picked by timing one batched step of the workload per candidate.
+`uneven` is the batched multi-block workload (a coarse block 192x128x128 and
+16 fine blocks of uneven sizes, one with a heaving, pitching plate;
+cases.config_uneven) under the reference's rule, FP64 with the fine blocks'
+SOR calls batched: throughput, nIterPcor per block, and the roofline of its
+SOR calls (algorithmic bytes over their GPU time).
+
+--workload uneven: that workload alone, FP64 and mixed precision (equal sweep
+budget), each run twice on the same inputs: the fine blocks' SOR calls
+batched in one launch per pass, then one block after the other; one JSON
+line with the throughputs, the speed-ups and whether the runs agree bit for
+bit.  With --sor-variant auto (the default) the FP64 SOR variant is picked by
+timing one batched step of the workload per candidate.
 """
 from __future__ import annotations
 
@@ -204,29 +209,78 @@ def tune_uneven(args):
     return best["variant"], table
 
 
-def run_uneven(args, batching: int, variant: int):
+def run_uneven(args, batching: int, variant, precision: str = "fp64"):
+    """Warm-up plus timed steps of the uneven workload (the reference's own
+    rule: update norm < 1e-8 every fifth iteration, pcItaMax = 500).  In
+    mixed precision every solve gets the same sweep budget (no residual
+    tolerance), so the run measures the batching, not a convergence rate.
+    The SOR calls are timed with HIP events over the timed steps."""
     from biocfd_amd import cases as CS
     from biocfd_amd import hip as H
     from biocfd_amd.driver import Simulation
 
     case = CS.config_uneven(nsteps=args.warmup + args.steps)
-    sim = Simulation(case)
+    sim = Simulation(case, precision=precision)
     H.call("biocfd_sor_batching", batching)
-    H.call("biocfd_sor_config", variant, args.sor_planes)
+    H.call("biocfd_momentum_config", args.momentum_variant, 0 if args.momentum_divisions else 1)
+    if variant is not None:
+        H.call("biocfd_sor_config", variant, args.sor_planes)
+    if precision == "mixed":
+        H.call("biocfd_mixed_config", 26, 32, args.uneven_sweeps, 0.0)
     for _ in range(args.warmup):
         sim.step()
     H.call("biocfd_synchronize")
+    H.call("biocfd_timing_enable", 1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         sim.step()
     H.call("biocfd_synchronize")
     t1 = time.perf_counter()
+    sor_ms = np.zeros(1)
+    sor_it = np.zeros(1, dtype=np.int64)
+    sor_ci = np.zeros(1, dtype=np.int64)
+    H.call("biocfd_timing_read", H.ptr(sor_ms), H.ptr(sor_it), H.ptr(sor_ci))
+    H.call("biocfd_timing_enable", 0)
+    n_iter = []
+    for g in range(1, len(case.blocks) + 1):
+        it = np.zeros(1, dtype=np.int64)
+        H.call("biocfd_sor_state", g, H.ptr(it), H.ptr(np.zeros(1)), H.ptr(np.zeros(1)))
+        n_iter.append(int(it[0]))
     fields = [sim.field(g, "p") for g in range(1, len(case.blocks) + 1)]
     H.call("biocfd_sor_batching", 1)
+    if precision == "mixed":
+        H.call("biocfd_mixed_config", 26, 32, 128, 1e-8)
     sim.close()
     cells = case.cells()
-    return {"ms_per_step": round(1e3 * (t1 - t0) / args.steps, 3),
-            "mcell_steps_per_s": round(cells * args.steps / (t1 - t0) / 1e6, 3)}, fields, case
+    secs = t1 - t0
+    out = {"ms_per_step": round(1e3 * secs / args.steps, 3),
+           "mcell_steps_per_s": round(cells * args.steps / secs / 1e6, 3),
+           "nIterPcor_last_step": n_iter,
+           "sor_cell_iterations_per_step": int(sor_ci[0]) // args.steps,
+           "sor_gpu_ms_per_step": round(float(sor_ms[0]) / args.steps, 3),
+           "sor_share_of_step": round(float(sor_ms[0]) * 1e-3 / secs, 3)}
+    bpc = SOR_BYTES_PER_CELL_ITER if precision == "fp64" else mixed_bytes_per_cell(args.uneven_sweeps) / args.uneven_sweeps
+    if sor_ms[0] > 0:
+        achieved = bpc * float(sor_ci[0]) / (float(sor_ms[0]) * 1e-3) / 1e9
+        out["sor_roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(achieved / HBM_PEAK_GBS, 4),
+                               "bytes_per_cell_iteration": round(bpc, 3),
+                               "kernel": "every SOR call of the step (coarse block alone, fine blocks batched or "
+                                         "one after the other), algorithmic bytes over the GPU time of the calls"}
+    return out, fields, case
+
+
+UNEVEN_TXT = ("uneven17: coarse 192x128x128 on [0,12]x[0,8]x[0,8] + 16 fine blocks (refinement ratio 2, sides {sides}), "
+              "heaving/pitching plate with thin-surface cells, Re=200, dt=5e-4, the reference's rule (update norm < "
+              "1e-8 every 5th iteration, pcItaMax=500)")
+
+
+def uneven_summary(args):
+    """The secondary line of the default run: the uneven workload, FP64 with
+    the fine blocks batched, the library's default kernels."""
+    b, _, case = run_uneven(args, 1, None)
+    return {"workload": UNEVEN_TXT.format(sides=",".join(str(int(blk.n[0])) for blk in case.blocks[1:])),
+            "cells": case.cells(), "steps": args.steps, "warmup": args.warmup, "dtype": "f64", **b}
 
 
 def uneven_main(args):
@@ -237,16 +291,21 @@ def uneven_main(args):
         variant = int(args.sor_variant)
     b, fb, case = run_uneven(args, 1, variant)
     p, fp, _ = run_uneven(args, 0, variant)
-    same = all(np.array_equal(x, y) for x, y in zip(fb, fp))
+    mb, fmb, _ = run_uneven(args, 1, variant, "mixed")
+    mp, fmp, _ = run_uneven(args, 0, variant, "mixed")
     sizes = [int(blk.n[0]) * int(blk.n[1]) * int(blk.n[2]) for blk in case.blocks]
     print(json.dumps({
-        "workload": "uneven17: coarse 192x128x128 on [0,12]x[0,8]x[0,8] + 16 fine blocks (ratio 2, sides "
-                    + ",".join(str(int(blk.n[0])) for blk in case.blocks[1:])
-                    + "), heaving/pitching plate, Re=200, dt=5e-4, pcItaMax=500, epsi=1e-8, FP64",
+        "metric": "Mcell-steps/s of the uneven multi-block workload, fine blocks batched vs one after the other",
+        "value": b["mcell_steps_per_s"], "unit": "Mcell-steps/s", "n_gpus": 1, "higher_is_better": True,
+        "workload": UNEVEN_TXT.format(sides=",".join(str(int(blk.n[0])) for blk in case.blocks[1:])),
         "cells": case.cells(), "fine_cells": sum(sizes[1:]), "steps": args.steps, "warmup": args.warmup,
         "sor_variant": variant, "autotuned": args.sor_variant == "auto",
-        "batched": b, "per_block": p, "speedup": round(b["mcell_steps_per_s"] / p["mcell_steps_per_s"], 3),
-        "bit_identical": bool(same)}))
+        "fp64": {"batched": b, "per_block": p, "speedup": round(b["mcell_steps_per_s"] / p["mcell_steps_per_s"], 3),
+                 "bit_identical": bool(all(np.array_equal(x, y) for x, y in zip(fb, fp)))},
+        "mixed_equal_sweep_budget": {
+            "sweeps_per_correction": args.uneven_sweeps, "batched": mb, "per_block": mp,
+            "speedup": round(mb["mcell_steps_per_s"] / mp["mcell_steps_per_s"], 3),
+            "bit_identical": bool(all(np.array_equal(x, y) for x, y in zip(fmb, fmp)))}}))
 
 
 def field_errors(ref, got):
@@ -383,6 +442,9 @@ def main():
                     help="first run a plain device copy of this many MiB (byte-counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-timeout", type=float, default=600.0)
+    ap.add_argument("--uneven-sweeps", type=int, default=16,
+                    help="FP32 sweeps per correction of the uneven workload's mixed runs")
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven workload's secondary line")
     ap.add_argument("--workload", choices=["sphere", "uneven"], default="sphere",
                     help="sphere: the 256^3 headline workload; uneven: batched vs per-block multi-block run")
     args = ap.parse_args()
@@ -474,6 +536,8 @@ def main():
             "sor_kernel": ref["kernel"],
             "roofline": sor_roofline(args, ref),
         }
+    if not args.no_uneven:
+        out["uneven"] = uneven_summary(args)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args)

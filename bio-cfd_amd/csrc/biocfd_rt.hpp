@@ -249,5 +249,7 @@ int precision_mode();
 int health_after_solve(int first, int last);
 int stress_scratch_init(int device);
 int mixed_sor(Block &b, int64_t pc_ita_max, double dt);
+int mixed_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt);
+bool mixed_batchable();
 
 }  // namespace bcfd

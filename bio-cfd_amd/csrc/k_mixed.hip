@@ -222,10 +222,12 @@ __global__ void __launch_bounds__(1024) k_mx_end(SorState *st, const double *par
 
 using Sweep32Launch = void (*)(dim3, dim3, hipStream_t, const FusedArgsT<float> &, int, int, const SorState *,
                                double *);
+using Batch32Launch = void (*)(dim3, dim3, hipStream_t, const BatchBlockT<float> *, int, int);
 struct Sweep32Variant {
   int ti, tj, nt, depth, minb, nit;
   Sweep32Launch launch, launch1;
   int family;  // 0: SorTile, 1, 2: SorPairTile (coefficients in registers, LDS)
+  Batch32Launch batch, batch1;  // several blocks in one launch (pair-owned tiles only)
 };
 
 template <int TI, int TJ, int NT, int D, int MINB, int NIT>
@@ -240,14 +242,21 @@ static void launch_pair32(dim3 g, dim3 blk, hipStream_t s, const FusedArgsT<floa
   hipLaunchKernelGGL((k_sor_pair<float, TI, TJ, NT, NIT, MINW, CREG>), g, blk, 0, s, a, gx, gy, 0, st, partials);
 }
 
+template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG>
+static void launch_pair32_batch(dim3 g, dim3 blk, hipStream_t s, const BatchBlockT<float> *tab, int nblk, int parity) {
+  hipLaunchKernelGGL((k_sor_pair_batch<float, TI, TJ, NT, NIT, MINW, CREG>), g, blk, 0, s, tab, nblk, parity, 0,
+                     nullptr);
+}
+
 // SV(tile_i, tile_j, threads, prefetch depth, waves per SIMD the registers
 // must allow, sweeps per pass): SorTile; PS(tile_i, tile_j, threads, waves
 // per SIMD, sweeps per pass, coefficients in registers): SorPairTile
 #define SV(TI, TJ, NT, D, MINB, NIT) \
-  { TI, TJ, NT, D, MINB, NIT, launch_sweep32<TI, TJ, NT, D, MINB, NIT>, launch_sweep32<TI, TJ, NT, D, MINB, 1>, 0 }
+  { TI, TJ, NT, D, MINB, NIT, launch_sweep32<TI, TJ, NT, D, MINB, NIT>, launch_sweep32<TI, TJ, NT, D, MINB, 1>, 0, \
+    nullptr, nullptr }
 #define PS(TI, TJ, NT, MINW, NIT, CREG) \
   { TI, TJ, NT, 1, MINW, NIT, launch_pair32<TI, TJ, NT, MINW, NIT, CREG>, launch_pair32<TI, TJ, NT, MINW, 1, CREG>, \
-    CREG ? 1 : 2 }
+    CREG ? 1 : 2, launch_pair32_batch<TI, TJ, NT, MINW, NIT, CREG>, launch_pair32_batch<TI, TJ, NT, MINW, 1, CREG> }
 static const Sweep32Variant kSweep32[] = {
     SV(64, 16, 256, 1, 1, 1),
     SV(64, 32, 512, 1, 1, 1),
@@ -372,6 +381,8 @@ int sor_timing_on();
 int sor_poll(const SorState *st, long long &k, bool &stop);
 int sor_timing_begin(hipEvent_t *e0);
 int sor_timing_end(hipEvent_t e0, const Block &b, long long start);
+int sor_timing_end_set(hipEvent_t e0, const std::vector<Block *> &blocks, const std::vector<long long> &start);
+int set_poll(const std::vector<Block *> &blocks, long long &k, bool &stop);
 
 int mixed_sor(Block &b, int64_t pc_ita_max, double dt) {
   hipStream_t s = rt().stream;
@@ -417,6 +428,136 @@ int mixed_sor(Block &b, int64_t pc_ita_max, double dt) {
   }
   if (sor_timing_end(e0, b, start)) return -1;
   hipLaunchKernelGGL(k_mx_finish, dim3(1024), dim3(256), 0, s, b.sor, b.f[BF_PC], b.pc2, b.n2);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+// ------------------------------------------------ several blocks at once
+// The mixed solves of a set of blocks (poissonSolver's fine blocks,
+// PcorVcor.F90:86-129): blocks whose calls start at the same iteration share
+// one schedule of corrections, and the FP32 sweeps of a pass run for all of
+// them in one launch of the batched pair-owned kernel (a block whose call
+// has stopped skips its tiles); the residual updates and the end-of-correction
+// bookkeeping stay per block.  Each block sees the kernels, arguments and
+// order of mixed_sor: the same bits.
+static BatchBlockT<float> *g_mx_tab = nullptr;
+static int64_t g_mx_tab_cap = 0;
+bool mixed_batchable() { return kSweep32[g_mx.variant].batch != nullptr; }
+
+// one call on each of `blocks`, all at start iteration `start`
+static int mixed_group(const std::vector<Block *> &blocks, long long start, int64_t pc_ita_max) {
+  hipStream_t s = rt().stream;
+  const Sweep32Variant &v = kSweep32[g_mx.variant];
+  const int nb = (int)blocks.size();
+  long long budget = pc_ita_max + 1 - start;
+  if (budget < 1) budget = 1;
+  const int k = g_mx.sweeps;
+  const long long cycles = (budget + k - 1) / k;
+  // entries [0, nb): the first pass of a correction (zero source, into
+  // e32[0]); [nb, 2 nb): later passes, parity 0 e32[1] -> e32[0], parity 1
+  // e32[0] -> e32[1]
+  std::vector<BatchBlockT<float>> tab((size_t)(2 * nb));
+  int64_t total = 0;
+  for (int i = 0; i < nb; ++i) {
+    Block &b = *blocks[i];
+    const int gx = (int)div_up(b.d.nx, v.ti), gy = (int)div_up(b.d.ny, v.tj), gz = (int)div_up(b.d.nz, g_mx.planes);
+    for (int t = 0; t < 2; ++t) {
+      BatchBlockT<float> &e = tab[(size_t)(t * nb + i)];
+      for (int par = 0; par < 2; ++par) {
+        FusedArgsT<float> &a = e.a[par];
+        a.d = b.d;
+        a.NXh = (b.d.NX + 1) / 2;
+        a.KS = g_mx.planes;
+        a.bred = b.r32[1];
+        a.bblack = b.r32[0];
+        a.Ax = b.A[0];
+        a.Ay = b.A[1];
+        a.Az = b.A[2];
+        a.omega = b.omega;
+        a.zero_src = t == 0 && par == 0;
+        a.src = par ? b.e32[0] : b.e32[1];
+        a.dst = par ? b.e32[1] : b.e32[0];
+      }
+      e.gx = gx;
+      e.gy = gy;
+      e.ntiles = gx * gy * gz;
+      e.first = (int)total;
+      e.st = b.sor;
+      e.start = start;
+    }
+    total += (int64_t)gx * gy * gz;
+  }
+  if (grow(&g_mx_tab, &g_mx_tab_cap, 2 * nb)) return -1;
+  BCFD_HIP(hipMemcpyAsync(g_mx_tab, tab.data(), sizeof(BatchBlockT<float>) * 2 * nb, hipMemcpyHostToDevice, s));
+  // the host table is freed on return: wait for the copy
+  BCFD_HIP(hipStreamSynchronize(s));
+  hipEvent_t e0 = nullptr;
+  if (sor_timing_begin(&e0)) return -1;
+  std::vector<int64_t> nwg((size_t)nb);
+  for (int i = 0; i < nb; ++i) {
+    Block &b = *blocks[i];
+    nwg[i] = residual_launch(b, b.f[BF_PC], nullptr, nullptr, b.sor);
+    hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, nwg[i], -1.0, start, 0ll, budget, 0ll);
+  }
+  BCFD_CHECK_LAUNCH();
+  const long long chunk = (48 + k - 1) / k;
+  long long polls = 0;
+  for (long long c = 0; c < cycles; ++c) {
+    const int kc = (int)((budget - c * k) < k ? (budget - c * k) : k);
+    int q = 0;
+    for (int done = 0; done < kc; ++q) {
+      const int L = kc - done < v.nit ? kc - done : v.nit;
+      const BatchBlockT<float> *t = q == 0 ? g_mx_tab : g_mx_tab + nb;
+      (L == v.nit ? v.batch : v.batch1)(dim3((unsigned)total), dim3(v.nt), s, t, nb, q & 1);
+      done += L;
+    }
+    const bool in0 = ((q - 1) & 1) == 0;   // the last pass wrote e32[0]
+    for (int i = 0; i < nb; ++i) {
+      Block &b = *blocks[i];
+      const float *e = in0 ? b.e32[0] : b.e32[1];
+      const double *src = (c & 1) ? b.pc2 : b.f[BF_PC];
+      double *dst = (c & 1) ? b.f[BF_PC] : b.pc2;
+      nwg[i] = residual_launch(b, src, e, dst, b.sor);
+      hipLaunchKernelGGL(k_mx_end, dim3(1), dim3(1024), 0, s, b.sor, b.partials, nwg[i], g_mx.tol, start,
+                         c * k + kc, budget, c + 1);
+    }
+    BCFD_CHECK_LAUNCH();
+    if ((c + 1) % chunk == 0 && c + 1 < cycles) {
+      bool stop = false;
+      if (set_poll(blocks, polls, stop)) return -1;
+      if (stop) break;
+    }
+  }
+  return sor_timing_end_set(e0, blocks, std::vector<long long>(blocks.size(), start));
+}
+
+int mixed_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt) {
+  hipStream_t s = rt().stream;
+  for (Block *b : blocks) {
+    if (ensure_mixed(*b) || split_bdt(*b, dt)) return -1;
+    BCFD_HIP(hipMemcpyAsync(b->pc2, b->f[BF_PC], sizeof(double) * (size_t)b->n2, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(1), 0, s, b->sor, 0);
+  }
+  BCFD_CHECK_LAUNCH();
+  std::vector<SorState> st0(blocks.size());
+  for (size_t i = 0; i < blocks.size(); ++i)
+    BCFD_HIP(hipMemcpyAsync(&st0[i], blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  std::vector<long long> start(blocks.size());
+  for (size_t i = 0; i < blocks.size(); ++i) start[i] = st0[i].n_iter;
+  std::vector<bool> taken(blocks.size(), false);
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    if (taken[i]) continue;
+    std::vector<Block *> group;
+    for (size_t j = i; j < blocks.size(); ++j)
+      if (!taken[j] && start[j] == start[i]) {
+        group.push_back(blocks[j]);
+        taken[j] = true;
+      }
+    if (mixed_group(group, start[i], pc_ita_max)) return -1;
+  }
+  for (Block *b : blocks)
+    hipLaunchKernelGGL(k_mx_finish, dim3(1024), dim3(256), 0, s, b->sor, b->f[BF_PC], b->pc2, b->n2);
   BCFD_CHECK_LAUNCH();
   return 0;
 }

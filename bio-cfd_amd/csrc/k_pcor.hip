@@ -178,9 +178,11 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
 }
 
 // k_sor_fused_end for every block of a batched launch: workgroup b reduces
-// block b's partials (workgroups first .. first + ntiles - 1 of the launch)
+// block b's partials (workgroups first .. first + ntiles - 1 of the launch);
+// `done` iterations of the set have run, so block b is at iteration
+// start_b + done
 __global__ void __launch_bounds__(1024) k_sor_batch_end(const BatchBlockT<double> *__restrict__ tab,
-                                                        long long it, long long start, long long passes, int check,
+                                                        long long done, long long passes, int check,
                                                         long long pc_ita_max, double epsi,
                                                         const double *__restrict__ partials) {
   const BatchBlockT<double> &e = tab[blockIdx.x];
@@ -203,8 +205,9 @@ __global__ void __launch_bounds__(1024) k_sor_batch_end(const BatchBlockT<double
     for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
     st->derr4 = m;
   }
+  const long long it = e.start + done;
   st->n_iter = it;
-  st->call_iters = it - start;
+  st->call_iters = done;
   st->passes = (int)passes;
   st->derr2 = st->derr4;
   if (st->derr4 < epsi || it > pc_ita_max) {
@@ -696,6 +699,54 @@ int sor_timing_end(hipEvent_t e0, const Block &b, long long start) {
   return 0;
 }
 
+// the same for one call on each block of a set that ran together from
+// iteration `start`
+int sor_timing_end_set(hipEvent_t e0, const std::vector<Block *> &blocks, const std::vector<long long> &start) {
+  if (!g_tm.on) return 0;
+  hipStream_t s = rt().stream;
+  hipEvent_t e1 = nullptr;
+  BCFD_HIP(hipEventCreate(&e1));
+  BCFD_HIP(hipEventRecord(e1, s));
+  g_tm.ev.emplace_back(e0, e1);
+  SorState *h = (SorState *)rt().h_pinned;
+  for (size_t i = 0; i < blocks.size(); ++i) {
+    const Block *b = blocks[i];
+    BCFD_HIP(hipMemcpyAsync(h, b->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+    BCFD_HIP(hipStreamSynchronize(s));
+    g_tm.iters += h->n_iter - start[i];
+    g_tm.cells += (h->n_iter - start[i]) * (int64_t)b->d.nx * b->d.ny * b->d.nz;
+  }
+  return 0;
+}
+
+// Host polls of a set of SOR calls without draining the queue (as sor_poll
+// for one): the states after chunk k are copied to pinned set k & 1 and read
+// once chunk k+1 is queued; stop = every block had stopped at chunk k - 1.
+static SorState *g_set_poll_h = nullptr;   // two sets of kMaxBlocks + 1 states
+static hipEvent_t g_set_poll_ev[2] = {nullptr, nullptr};
+
+int set_poll(const std::vector<Block *> &blocks, long long &k, bool &stop) {
+  hipStream_t s = rt().stream;
+  if (!g_set_poll_h)
+    BCFD_HIP(hipHostMalloc((void **)&g_set_poll_h, sizeof(SorState) * 2 * (kMaxBlocks + 1), hipHostMallocDefault));
+  for (auto &e : g_set_poll_ev)
+    if (!e) BCFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  const int cur = (int)(k & 1);
+  SorState *set = g_set_poll_h + cur * (kMaxBlocks + 1);
+  for (size_t i = 0; i < blocks.size(); ++i)
+    BCFD_HIP(hipMemcpyAsync(&set[i], blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipEventRecord(g_set_poll_ev[cur], s));
+  stop = false;
+  if (k > 0) {
+    BCFD_HIP(hipEventSynchronize(g_set_poll_ev[cur ^ 1]));
+    const SorState *prev = g_set_poll_h + (cur ^ 1) * (kMaxBlocks + 1);
+    stop = true;
+    for (size_t i = 0; i < blocks.size(); ++i) stop = stop && prev[i].done != 0;
+  }
+  ++k;
+  return 0;
+}
+
 // the device's compute units (workgroups one launch needs to fill it)
 static int device_cus() {
   static int cus = 0;
@@ -795,13 +846,20 @@ static int64_t g_batch_cap = 0;
 static double *g_batch_partials = nullptr;
 static int64_t g_batch_pcap = 0;
 
-// one SOR call on each of `blocks` (all at start iteration `start`)
-static int batched_calls(const std::vector<Block *> &blocks, long long start, int64_t pc_ita_max, double epsi) {
+// one SOR call on each of `blocks`, block i from iteration starts[i]; the
+// starts agree modulo 5, so every block's check iterations (nIterPcor a
+// multiple of 5, PcorVcor.F90:351) fall on the same passes.  A pass never
+// spans the end of a block's budget: the set's schedule also breaks where a
+// block's budget ends, and the end-of-pass kernel stops that block there.
+static int batched_calls(const std::vector<Block *> &blocks, const std::vector<long long> &starts, int64_t pc_ita_max,
+                         double epsi) {
   hipStream_t s = rt().stream;
   const int variant = g_sor_variant, planes = planes_for(blocks, variant, g_sor_planes);
   const FusedVariant &v = kFused[variant];
   const int nb = (int)blocks.size();
   std::vector<BatchBlock> tab((size_t)nb);
+  std::vector<long long> budget((size_t)nb);
+  long long total = 1;
   int64_t total_tiles = 0;
   for (int i = 0; i < nb; ++i) {
     Block &b = *blocks[i];
@@ -828,43 +886,48 @@ static int batched_calls(const std::vector<Block *> &blocks, long long start, in
     e.ntiles = gx * gy * gz;
     e.first = (int)total_tiles;
     e.st = b.sor;
+    e.start = starts[(size_t)i];
     total_tiles += e.ntiles;
+    // at least one iteration per call, as the reference's loop (PcorVcor.F90:309)
+    budget[(size_t)i] = pc_ita_max + 1 - starts[(size_t)i] < 1 ? 1 : pc_ita_max + 1 - starts[(size_t)i];
+    total = budget[(size_t)i] > total ? budget[(size_t)i] : total;
   }
   if (grow(&g_batch_tab, &g_batch_cap, nb) || grow(&g_batch_partials, &g_batch_pcap, total_tiles)) return -1;
   BCFD_HIP(hipMemcpyAsync(g_batch_tab, tab.data(), sizeof(BatchBlock) * nb, hipMemcpyHostToDevice, s));
   BCFD_HIP(hipStreamSynchronize(s));
-  const long long total = pc_ita_max + 1 - start < 1 ? 1 : pc_ita_max + 1 - start;
-  SorState *h = (SorState *)rt().h_pinned;
-  long long done_iters = 0, q = 0, since_sync = 0;
+  hipEvent_t e0 = nullptr;
+  if (sor_timing_begin(&e0)) return -1;
+  const long long s0 = starts[0];
+  long long done_iters = 0, q = 0, since_poll = 0, polls = 0;
   while (done_iters < total) {
-    const long long it = start + done_iters + 1;
+    const long long it = s0 + done_iters + 1;
     const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
     long long L = v.nit;
     if (L > total - done_iters) L = total - done_iters;
     if (L > to_check) L = to_check;
-    const long long last_it = it + L - 1;
-    const int check = last_it % 5 == 0;
+    for (long long bt : budget)
+      if (bt > done_iters && L > bt - done_iters) L = bt - done_iters;
+    const int check = (it + L - 1) % 5 == 0;
     (L == v.nit ? v.batch : v.batch1)(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1),
                                       check, g_batch_partials);
     done_iters += L;
-    since_sync += L;
+    since_poll += L;
     ++q;
-    if (check || done_iters == total)
-      hipLaunchKernelGGL(k_sor_batch_end, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab, last_it, start, q,
-                         check, (long long)pc_ita_max, epsi, g_batch_partials);
+    bool ends = done_iters == total;
+    for (long long bt : budget) ends = ends || bt == done_iters;
+    if (check || ends)
+      hipLaunchKernelGGL(k_sor_batch_end, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab, done_iters, q, check,
+                         (long long)pc_ita_max, epsi, g_batch_partials);
     BCFD_CHECK_LAUNCH();
-    if (since_sync >= 50 && done_iters < total) {
+    if (since_poll >= 50 && done_iters < total) {
       // every block stopped: no more passes (PcorVcor.F90:365-375)
-      since_sync = 0;
-      bool all = true;
-      for (int i = 0; i < nb && all; ++i) {
-        BCFD_HIP(hipMemcpyAsync(h, blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
-        BCFD_HIP(hipStreamSynchronize(s));
-        all = h->done != 0;
-      }
-      if (all) break;
+      since_poll = 0;
+      bool stop = false;
+      if (set_poll(blocks, polls, stop)) return -1;
+      if (stop) break;
     }
   }
+  if (sor_timing_end_set(e0, blocks, starts)) return -1;
   for (Block *b : blocks)
     if (sor_finish(*b, b->sor, variant)) return -1;
   return 0;
@@ -873,9 +936,11 @@ static int batched_calls(const std::vector<Block *> &blocks, long long start, in
 // the SOR calls of `blocks` (REDBLACKSOR_linear on each), batched where the
 // kernel and the calls allow
 int redblack_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt, double epsi) {
+  if (precision_mode() == 1 && g_sor_batch && mixed_batchable() && blocks.size() > 1)
+    return mixed_sor_set(blocks, pc_ita_max, dt);
   const int variant = g_sor_variant;
-  const bool can = g_sor_batch && precision_mode() == 0 && g_stop.rule == 0 && !sor_timing_on() &&
-                   kFused[variant].batch && blocks.size() > 1;
+  const bool can = g_sor_batch && precision_mode() == 0 && g_stop.rule == 0 && kFused[variant].batch &&
+                   blocks.size() > 1;
   if (!can) {
     for (Block *b : blocks)
       if (redblack_sor(*b, pc_ita_max, dt, epsi)) return -1;
@@ -896,17 +961,20 @@ int redblack_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, dou
     BCFD_HIP(hipStreamSynchronize(s));
     start[i] = h->n_iter;
   }
-  // blocks that start at the same iteration share a pass schedule
+  // blocks whose calls start at iterations equal modulo 5 share a pass
+  // schedule (their check iterations coincide)
   std::vector<bool> taken(blocks.size(), false);
   for (size_t i = 0; i < blocks.size(); ++i) {
     if (taken[i]) continue;
     std::vector<Block *> group;
+    std::vector<long long> gstart;
     for (size_t j = i; j < blocks.size(); ++j)
-      if (!taken[j] && start[j] == start[i]) {
+      if (!taken[j] && start[j] % 5 == start[i] % 5) {
         group.push_back(blocks[j]);
+        gstart.push_back(start[j]);
         taken[j] = true;
       }
-    if (batched_calls(group, start[i], pc_ita_max, epsi)) return -1;
+    if (batched_calls(group, gstart, pc_ita_max, epsi)) return -1;
   }
   return 0;
 }

@@ -120,6 +120,7 @@ struct BatchBlockT {
   FusedArgsT<Real> a[2];
   int gx, gy, ntiles, first;
   SorState *st;
+  long long start;   // nIterPcor at the start of the block's call
 };
 
 // The same iterations on every block of a set in one launch: workgroups

@@ -123,3 +123,42 @@ def test_mixed_runs_at_least_one_correction():
         sim.close()
         H.call("biocfd_mixed_config", *DEFAULT)
     assert int(d["nIterPcor"]) == 32
+
+
+@pytest.mark.parametrize("tol", [0.0, 1e-7])
+def test_mixed_batched_fine_blocks_bit_identical(tol):
+    """The fine blocks' mixed solves batched (one FP32 sweep launch per pass
+    over every fine block whose call starts at the same iteration) give the
+    bits of one block after the other.  With a residual tolerance the blocks
+    stop after different numbers of corrections, so a stopped block's tiles
+    are skipped while the others go on."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = CS.config_batched(nsteps=2, pc_ita_max=300, epsi=0.0)
+    dumps = {}
+    for batching in (1, 0):
+        sim = Simulation(case, precision="mixed")
+        try:
+            H.call("biocfd_sor_batching", batching)
+            H.call("biocfd_mixed_config", 26, 32, 16, tol)
+            for _ in range(2):
+                sim.step()
+            dumps[batching] = sim.dump()
+        finally:
+            sim.close()
+            H.call("biocfd_sor_batching", 1)
+            H.call("biocfd_mixed_config", *DEFAULT)
+    iters = [int(b["nIterPcor"]) for b in dumps[1]]
+    print(f"tol {tol}: nIterPcor per block {iters}")
+    if tol > 0:
+        assert len(set(iters[1:])) > 1, iters
+    bad = []
+    for g, (a, b) in enumerate(zip(dumps[1], dumps[0])):
+        for k, x in a.items():
+            x = np.asarray(x)
+            y = np.asarray(b[k])
+            if x.shape != y.shape or x.dtype != y.dtype or x.tobytes() != y.tobytes():
+                bad.append(f"block {g + 1} {k} differs")
+    assert not bad, "\n".join(bad[:40])
