@@ -119,6 +119,12 @@ struct IbData {
   int64_t cap_ts_mask = 0;
   int *ts_list = nullptr, *ts_best = nullptr;
   int64_t cap_ts_list = 0, cap_ts_best = 0;
+  // the parallel form of that pass: list index of each deferred cell (n2,
+  // -1 elsewhere between calls) and the TS-adjacency mask of each entry
+  int *ts_qmap = nullptr;
+  int64_t cap_ts_qmap = 0;
+  uint8_t *ts_adjm = nullptr;
+  int64_t cap_ts_adjm = 0;
   // stress_calculation: per-element terms, the coefficients and the miss counter
   double *stress_work = nullptr;
   int64_t cap_stress = 0;

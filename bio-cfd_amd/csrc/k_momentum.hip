@@ -907,6 +907,150 @@ __global__ void __launch_bounds__(256) k_ts_ordered(MomArgs a, double *u, double
   }
 }
 
+// ---------------------------------------------------------------------------
+// The ordered pass in parallel
+// ---------------------------------------------------------------------------
+// In the ordered pass the only writes to u, v, w, p are a TS-adjacent cell's
+// ghost substitutions at its targets (kTargets: 36 distinct positions per
+// cell) and the restores that follow its own stencil; every restore of a
+// target kind stores the same value, ghost[gt][n_ts - 1].  So the value a
+// deferred cell q reads at a position is fixed by positions alone:
+//   * q's own substitution there, if q targets it and its ghost choice
+//     best[] is not 0;
+//   * else the restore of the last TS-adjacent cell before q in list order
+//     that targets the position (the last of its targets, in kTargets order);
+//   * else the value the pass started from.
+// Each deferred cell is then evaluated on its own, all at once, reading
+// through that rule, and the final state of the pass is the restore of the
+// last writer of every target position.  The writers of a position p are
+// found from p: a cell whose target (dq, t) is p sits at p - offset(dq, t),
+// so at most ten candidates per field; qmap maps a cell to its list index.
+
+__global__ void k_ts_prep(Dims d, const uint8_t *__restrict__ cell2, const int *__restrict__ list, int n_list,
+                          int *__restrict__ qmap, uint8_t *__restrict__ adjm, int clear) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_list) return;
+  const int lin = list[q];
+  const int i = 2 + lin % d.nx, j = 2 + (lin / d.nx) % d.ny, k = 2 + lin / (d.nx * d.ny);
+  const int64_t c = ix3(d, i, j, k);
+  if (clear) {
+    qmap[c] = -1;
+    return;
+  }
+  unsigned m = 0;
+  ts_adjacent(d, cell2, i, j, k, &m);
+  qmap[c] = q;
+  adjm[q] = (uint8_t)m;
+}
+
+struct TsView {   // passed by value: every pointer is a device address
+  const int *qmap;
+  const uint8_t *adjm;
+  const int *best;
+  const uint8_t *touched;
+  const double *ghost[BGH_N];
+  int n_ts;
+};
+
+// the writers of field f at cell (i,j,k): calls fn(q_w, slot) for each
+template <class Fn>
+__device__ inline void ts_writers(const Dims &d, const TsView &v, int f, int i, int j, int k, Fn fn) {
+  for (int dq = 0; dq < 6; ++dq)
+    for (int t = 0; t < 6; ++t) {
+      const Target T = kTargets[dq][t];
+      if (T.f != f) continue;
+      const int wi = i - T.di, wj = j - T.dj, wk = k - T.dk;
+      if (wi < 2 || wj < 2 || wk < 2 || wi > d.nx + 1 || wj > d.ny + 1 || wk > d.nz + 1) continue;
+      const int qw = v.qmap[ix3(d, wi, wj, wk)];
+      if (qw < 0 || !((v.adjm[qw] >> dq) & 1u)) continue;
+      fn(qw, dq * 6 + t);
+    }
+}
+
+// stencil operands of deferred cell q as the ordered pass would read them
+struct TsAcc {
+  const double *f[4];
+  const Dims *d;
+  const TsView *tv;
+  const MomArgs *a;
+  int q, i, j, k;
+  __device__ double rd(int fi, int di, int dj, int dk) const {
+    const int pi = i + di, pj = j + dj, pk = k + dk;
+    const int64_t p = ix3(*d, pi, pj, pk);
+    if (!tv->touched[p]) return f[fi][p];
+    int last_q = -1, last_slot = -1, own_slot = -1;
+    double own = 0.0;
+    ts_writers(*d, *tv, fi, pi, pj, pk, [&](int qw, int slot) {
+      if (qw < q) {
+        if (qw > last_q || (qw == last_q && slot > last_slot)) { last_q = qw; last_slot = slot; }
+      } else if (qw == q) {
+        const int nn = tv->best[36 * (int64_t)q + slot];
+        if (nn > 0 && slot > own_slot) {
+          own_slot = slot;
+          own = tv->ghost[kTargets[slot / 6][slot % 6].g][nn - 1];
+        }
+      }
+    });
+    if (own_slot >= 0) return own;
+    if (last_q >= 0) return tv->ghost[kTargets[last_slot / 6][last_slot % 6].gt][tv->n_ts - 1];
+    return f[fi][p];
+  }
+  template <int di, int dj, int dk> __device__ double u() const { return rd(0, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ double v() const { return rd(1, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ double w() const { return rd(2, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ double p() const { return rd(3, di, dj, dk); }
+  template <int o> __device__ double hx() const { return a->dx[i + o]; }
+  template <int o> __device__ double hy() const { return a->dy[j + o]; }
+  template <int o> __device__ double hz() const { return a->dz[k + o]; }
+  template <int T> __device__ const double *xa() const { return a->ca[T] + 6 * (i - 1); }
+  template <int T> __device__ const double *xk() const { return a->ck[T] + 6 * (i - 1); }
+  template <int T> __device__ const double *ya() const { return a->ca[T] + 6 * (j - 1); }
+  template <int T> __device__ const double *yk() const { return a->ck[T] + 6 * (j - 1); }
+  template <int T> __device__ const double *za() const { return a->ca[T] + 6 * (k - 1); }
+  template <int T> __device__ const double *zk() const { return a->ck[T] + 6 * (k - 1); }
+};
+
+// every deferred cell at once, reading the fields as the ordered pass would
+__global__ void __launch_bounds__(256) k_ts_parallel(MomArgs a, const double *u, const double *v, const double *w,
+                                                     const double *p, double *__restrict__ ut, double *__restrict__ vt,
+                                                     double *__restrict__ wt, const int *__restrict__ list, int n_list,
+                                                     TsView tv) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_list) return;
+  const Dims &d = a.d;
+  const int lin = list[q];
+  const int i = 2 + lin % d.nx, j = 2 + (lin / d.nx) % d.ny, k = 2 + lin / (d.nx * d.ny);
+  const MomCell m = mom_cell(a, i, j, k);
+  const bool near_ok = mom_near_ok(a, i, j, k);
+  const TsAcc A{{u, v, w, p}, &a.d, &tv, &a, q, i, j, k};
+  ut[m.c] = mom_u<false>(a, A, m, i, j, k, near_ok);
+  vt[m.c] = mom_v<false>(a, A, m, i, j, k, near_ok);
+  wt[m.c] = mom_w<false>(a, A, m, i, j, k, near_ok);
+}
+
+// the state the ordered pass leaves: each target position holds the restore
+// of its last writer
+__global__ void k_ts_restore(Dims d, double *u, double *v, double *w, double *p, const int *__restrict__ list,
+                             int n_list, TsView tv) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = gid / 36, slot = gid % 36;
+  if (q >= n_list) return;
+  const int dq = slot / 6;
+  if (!((tv.adjm[q] >> dq) & 1u)) return;
+  const int lin = list[q];
+  const int i = 2 + lin % d.nx, j = 2 + (lin / d.nx) % d.ny, k = 2 + lin / (d.nx * d.ny);
+  const Target T = kTargets[dq][slot % 6];
+  const int pi = i + T.di, pj = j + T.dj, pk = k + T.dk;
+  if (pi < 1 || pj < 1 || pk < 1 || pi > d.NX || pj > d.NY || pk > d.NZ) return;
+  bool last = true;
+  ts_writers(d, tv, T.f, pi, pj, pk, [&](int qw, int s2) {
+    if (qw > q || (qw == q && s2 > slot)) last = false;
+  });
+  if (!last) return;
+  double *fld[4] = {u, v, w, p};
+  fld[T.f][ix3(d, pi, pj, pk)] = tv.ghost[T.gt][tv.n_ts - 1];
+}
+
 int compact_mask_lin(Block &b, const uint8_t *mask, int **out, int64_t *cap, int64_t *count);
 
 }  // namespace bcfd
@@ -920,6 +1064,9 @@ namespace {
 // W waves per SIMD: (TJ, KC, W) = (4, 16, 2), (4, 32, 2), (8, 16, 2),
 // (8, 32, 2), (4, 32, 3), (4, 8, 2)
 constexpr int kMomVariants = 9;
+// the TS pass's deferred cells: 1 = all at once (k_ts_parallel), 0 = in list
+// order by one workgroup (k_ts_ordered); both give the same bits
+int g_ts_parallel = 1;
 int g_mom_variant = 4;   // the fastest at 256^3 on MI355X (0.72 ms against 1.02 ms for variant 0)
 // grid divisions as products by exact reciprocals where the block's spacings allow
 int g_mom_pow2 = 1;
@@ -990,6 +1137,17 @@ extern "C" int biocfd_momentum_config(int variant, int pow2_products) {
   return 0;
 }
 
+__global__ void k_fill_int(int *__restrict__ x, int64_t n, int value) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) x[q] = value;
+}
+
+extern "C" int biocfd_ts_pass_config(int parallel) {
+  BCFD_API_LOCK();
+  g_ts_parallel = parallel != 0;
+  return 0;
+}
+
 extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   BCFD_API_LOCK();
   Block *b = get_block(id);
@@ -1041,8 +1199,32 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
     if (grow(&ib.ts_best, &ib.cap_ts_best, 36 * cnt)) return -1;
     int *best = ib.ts_best;
     hipLaunchKernelGGL(k_ts_best, dim3((unsigned)cnt), dim3(256), 0, s, a, list, (int)cnt, ts, best);
-    hipLaunchKernelGGL(k_ts_ordered, dim3(1), dim3(256), 0, s, a, b->f[BF_U], b->f[BF_V], b->f[BF_W],
-                       b->f[BF_P], b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, ts, best);
+    if (!g_ts_parallel) {
+      hipLaunchKernelGGL(k_ts_ordered, dim3(1), dim3(256), 0, s, a, b->f[BF_U], b->f[BF_V], b->f[BF_W],
+                         b->f[BF_P], b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, ts, best);
+    } else {
+      const int *old_map = ib.ts_qmap;
+      if (grow(&ib.ts_qmap, &ib.cap_ts_qmap, b->n2) || grow(&ib.ts_adjm, &ib.cap_ts_adjm, cnt)) return -1;
+      if (ib.ts_qmap != old_map)  // a new map: no cell mapped
+        hipLaunchKernelGGL(k_fill_int, dim3(div_up(ib.cap_ts_qmap, 256)), dim3(256), 0, s, ib.ts_qmap,
+                           ib.cap_ts_qmap, -1);
+      TsView tv{};
+      tv.qmap = ib.ts_qmap;
+      tv.adjm = ib.ts_adjm;
+      tv.best = best;
+      tv.touched = touched;
+      for (int g = 0; g < BGH_N; ++g) tv.ghost[g] = ts.ghost[g];
+      tv.n_ts = n_ts;
+      const dim3 g1(div_up(cnt, 256)), g36(div_up(36 * cnt, 256));
+      hipLaunchKernelGGL(k_ts_prep, g1, dim3(256), 0, s, b->d, b->tag[BT_CELL2], list, (int)cnt, ib.ts_qmap,
+                         ib.ts_adjm, 0);
+      hipLaunchKernelGGL(k_ts_parallel, g1, dim3(256), 0, s, a, b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P],
+                         b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, tv);
+      hipLaunchKernelGGL(k_ts_restore, g36, dim3(256), 0, s, b->d, b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P],
+                         list, (int)cnt, tv);
+      hipLaunchKernelGGL(k_ts_prep, g1, dim3(256), 0, s, b->d, b->tag[BT_CELL2], list, (int)cnt, ib.ts_qmap,
+                         ib.ts_adjm, 1);
+    }
   }
   BCFD_CHECK_LAUNCH();
   return 0;

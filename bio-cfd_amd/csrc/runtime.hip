@@ -264,7 +264,7 @@ static void free_block(Block &b) {
   if (ib.task_out) (void)hipFree(ib.task_out);
   if (ib.task_done) (void)hipFree(ib.task_done);
   if (ib.stress_work) (void)hipFree(ib.stress_work);
-  for (void *p : {(void *)ib.ts_mask, (void *)ib.ts_list, (void *)ib.ts_best})
+  for (void *p : {(void *)ib.ts_mask, (void *)ib.ts_list, (void *)ib.ts_best, (void *)ib.ts_qmap, (void *)ib.ts_adjm})
     if (p) (void)hipFree(p);
   b = Block();
 }

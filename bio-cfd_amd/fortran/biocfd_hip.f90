@@ -637,6 +637,11 @@ module biocfd_hip
       integer(c_int), value :: variant, pow2_products
     end function
 
+    integer(c_int) function biocfd_ts_pass_config(parallel) bind(C, name="biocfd_ts_pass_config")
+      import :: c_int
+      integer(c_int), value :: parallel
+    end function
+
     ! ---- block move (Search.F90:21-47, :415-432, :916-1461; FineInterp.f90:10-281)
     integer(c_int) function biocfd_find_dist_node(n_nodes, xnode, znode, node_id, markers) bind(C, &
         name="biocfd_find_dist_node")

@@ -126,6 +126,7 @@ SIGNATURES = {
     "biocfd_output_fields": [_I, _P, _P, _P, _P, _P, _P, _P],
     "biocfd_sor_batching": [_I],
     "biocfd_momentum_config": [_I, _I],
+    "biocfd_ts_pass_config": [_I],
 }
 
 

@@ -137,6 +137,13 @@ int biocfd_ns_momentum(int blk, double dt, double re);
  * are products by their exact reciprocals (the same correctly rounded
  * values).  All settings give the same bits. */
 int biocfd_momentum_config(int variant, int pow2_products);
+/* The deferred cells of nsMomentum2order's thin-surface pass
+ * (NavierStokes.f90:310-490, 860-907) (no reference counterpart):
+ * parallel = 1 (the default) evaluates them all at once, each reading the
+ * ghost substitutions and restores the reference's in-order loop would have
+ * left at that point (fixed by positions alone), then leaves the loop's final
+ * state; 0 runs them in list order in one workgroup.  Same bits either way. */
+int biocfd_ts_pass_config(int parallel);
 /* velocityBC, BoundaryConditions.f90:11-74 */
 int biocfd_velocity_bc(int blk, double dt, double uc);
 /* solidCellBC, BoundaryConditions.f90:76-92 */

@@ -53,3 +53,29 @@ def test_time_step_parity(name, batching, mom, pow2):
         H.call("biocfd_sor_batching", 1)
         H.call("biocfd_momentum_config", MOM_DEFAULT, 1)
         sim.close()
+
+
+@pytest.mark.parametrize("name", ["wing", "wing32"])
+def test_ts_pass_in_list_order(name):
+    """The thin-surface pass's deferred cells one after the other in list
+    order (biocfd_ts_pass_config(0), the reference's loop order) against the
+    oracle; the default evaluates them all at once."""
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = small_cases()[name]
+    ref = run_oracle(case, STEPS)
+    sim = Simulation(case)
+    H.call("biocfd_ts_pass_config", 0)
+    try:
+        bad = []
+        for s in range(0, max(STEPS) + 1):
+            if s > 0:
+                sim.step()
+            if s in STEPS:
+                for g, (r, o) in enumerate(zip(ref[s], sim.dump())):
+                    bad += compare(r, o, label=f"{name} step {s} block {g + 1}: ")
+        assert not bad, "\n".join(bad[:40])
+    finally:
+        H.call("biocfd_ts_pass_config", 1)
+        sim.close()
