@@ -78,6 +78,8 @@ struct SorState {
   long long call_iters;          // iterations executed by the current SOR call
   int exhausted;                 // SOR calls since the last nIterPcor reset that ended on their
                                  // iteration budget, not their tolerance
+  int rollback;                  // the call stopped on the first iteration of a two-iteration pass:
+                                 // that iteration is redone alone from the pass's source
 };
 
 struct IbData {

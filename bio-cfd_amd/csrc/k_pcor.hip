@@ -37,6 +37,7 @@ __global__ void k_sor_reset_call(SorState *st, int reset_iter) {
     st->done = 0;
     st->call_iters = 0;
     st->passes = 0;
+    st->rollback = 0;
   }
 }
 
@@ -151,7 +152,8 @@ __global__ void __launch_bounds__(256) k_sor_split(Dims d, int NXh, int colour, 
 // the next of these.
 __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long it, long long start,
                                                         long long passes, int check, long long pc_ita_max,
-                                                        double epsi, const double *partials, int64_t n_partials) {
+                                                        double epsi, const double *partials, int64_t n_partials,
+                                                        int first = 0) {
   if (st->done) return;
   __shared__ double s_m[16];
   if (check) {
@@ -167,6 +169,17 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
     for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
     st->derr4 = m;
   }
+  if (first && check && st->derr4 < epsi) {
+    // stopped on the pass's first iteration (it - 1): that iteration is
+    // redone alone from the pass's source, which counts as the pass
+    st->n_iter = it - 1;
+    st->call_iters = it - 1 - start;
+    st->passes = (int)passes - 1;
+    st->derr2 = st->derr4;
+    st->done = 1;
+    st->rollback = 1;
+    return;
+  }
   st->n_iter = it;
   st->call_iters = it - start;
   st->passes = (int)passes;
@@ -177,6 +190,23 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
   }
 }
 
+// after the rollback pass: it counts as the stopping pass
+__global__ void k_sor_rollback_end(SorState *st) {
+  if (threadIdx.x == 0 && st->rollback) {
+    st->passes += 1;
+    st->rollback = 0;
+  }
+}
+__global__ void k_sor_rollback_end_set(const BatchBlockT<double> *__restrict__ tab, int nblk) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  SorState *st = tab[b].st;
+  if (st->rollback) {
+    st->passes += 1;
+    st->rollback = 0;
+  }
+}
+
 // k_sor_fused_end for every block of a batched launch: workgroup b reduces
 // block b's partials (workgroups first .. first + ntiles - 1 of the launch);
 // `done` iterations of the set have run, so block b is at iteration
@@ -184,7 +214,7 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
 __global__ void __launch_bounds__(1024) k_sor_batch_end(const BatchBlockT<double> *__restrict__ tab,
                                                         long long done, long long passes, int check,
                                                         long long pc_ita_max, double epsi,
-                                                        const double *__restrict__ partials) {
+                                                        const double *__restrict__ partials, int first) {
   const BatchBlockT<double> &e = tab[blockIdx.x];
   SorState *st = e.st;
   if (st->done) return;
@@ -206,6 +236,15 @@ __global__ void __launch_bounds__(1024) k_sor_batch_end(const BatchBlockT<double
     st->derr4 = m;
   }
   const long long it = e.start + done;
+  if (first && check && st->derr4 < epsi) {
+    st->n_iter = it - 1;
+    st->call_iters = done - 1;
+    st->passes = (int)passes - 1;
+    st->derr2 = st->derr4;
+    st->done = 1;
+    st->rollback = 1;
+    return;
+  }
   st->n_iter = it;
   st->call_iters = done;
   st->passes = (int)passes;
@@ -436,11 +475,19 @@ int ensure_split(Block &b) {
 using FusedLaunch = void (*)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, int, const SorState *, double *);
 using BatchBlock = BatchBlockT<double>;
 using BatchLaunch = void (*)(dim3, dim3, hipStream_t, const BatchBlock *, int, int, int, double *);
+using RollbackLaunch = void (*)(dim3, dim3, hipStream_t, const FusedArgs &, int, int, int, const SorState *);
+using BatchRollbackLaunch = void (*)(dim3, dim3, hipStream_t, const BatchBlock *, int);
 struct FusedVariant {
   int ti, tj, nt, depth, minb, nit;
   FusedLaunch launch, launch1;
   int family;  // 0: SorTile (cell lists per phase), 1, 2: SorPairTile (pair-owned; coefficients in registers, LDS)
   BatchLaunch batch, batch1;  // several blocks in one launch (pair-owned tiles only)
+  // two-iteration pair-owned passes whose check iteration is their first,
+  // and the rollback pass after a stop on such an iteration
+  FusedLaunch launch_first;
+  BatchLaunch batch_first;
+  RollbackLaunch rollback;
+  BatchRollbackLaunch batch_rollback;
 };
 
 template <int TI, int TJ, int NT, int D, int MINB, int NIT>
@@ -450,18 +497,29 @@ static void launch_fused(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, in
                      partials);
 }
 
-template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG>
+template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG, bool FIRST = false>
 static void launch_pair(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, int check,
                         const SorState *st, double *partials) {
-  hipLaunchKernelGGL((k_sor_pair<double, TI, TJ, NT, NIT, MINW, CREG>), g, blk, 0, s, a, gx, gy, check, st,
+  hipLaunchKernelGGL((k_sor_pair<double, TI, TJ, NT, NIT, MINW, CREG, FIRST>), g, blk, 0, s, a, gx, gy, check, st,
                      partials);
 }
 
-template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG>
+template <int TI, int TJ, int NT, int MINW, int NIT, bool CREG, bool FIRST = false>
 static void launch_pair_batch(dim3 g, dim3 blk, hipStream_t s, const BatchBlock *tab, int nblk, int parity, int check,
                               double *partials) {
-  hipLaunchKernelGGL((k_sor_pair_batch<double, TI, TJ, NT, NIT, MINW, CREG>), g, blk, 0, s, tab, nblk, parity, check,
-                     partials);
+  hipLaunchKernelGGL((k_sor_pair_batch<double, TI, TJ, NT, NIT, MINW, CREG, FIRST>), g, blk, 0, s, tab, nblk, parity,
+                     check, partials);
+}
+
+template <int TI, int TJ, int NT, int MINW, bool CREG>
+static void launch_pair_rollback(dim3 g, dim3 blk, hipStream_t s, const FusedArgs &a, int gx, int gy, int parity,
+                                 const SorState *st) {
+  hipLaunchKernelGGL((k_sor_pair_rollback<double, TI, TJ, NT, MINW, CREG>), g, blk, 0, s, a, gx, gy, parity, st);
+}
+
+template <int TI, int TJ, int NT, int MINW, bool CREG>
+static void launch_pair_batch_rollback(dim3 g, dim3 blk, hipStream_t s, const BatchBlock *tab, int nblk) {
+  hipLaunchKernelGGL((k_sor_pair_batch_rollback<double, TI, TJ, NT, MINW, CREG>), g, blk, 0, s, tab, nblk);
 }
 
 // FV(tile_i, tile_j, threads, prefetch depth, waves per SIMD the registers
@@ -471,12 +529,16 @@ static void launch_pair_batch(dim3 g, dim3 blk, hipStream_t s, const BatchBlock 
 // LDS coefficients)
 #define FV(TI, TJ, NT, D, MINB, NIT) \
   { TI, TJ, NT, D, MINB, NIT, launch_fused<TI, TJ, NT, D, MINB, NIT>, launch_fused<TI, TJ, NT, D, MINB, 1>, 0, \
-    nullptr, nullptr }
+    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr }
 #define PV(TI, TJ, NT, MINW, NIT, CREG) \
   { TI, TJ, NT, 1, MINW, NIT, launch_pair<TI, TJ, NT, MINW, NIT, CREG>, launch_pair<TI, TJ, NT, MINW, 1, CREG>, \
-    CREG ? 1 : 2, launch_pair_batch<TI, TJ, NT, MINW, NIT, CREG>, launch_pair_batch<TI, TJ, NT, MINW, 1, CREG> }
+    CREG ? 1 : 2, launch_pair_batch<TI, TJ, NT, MINW, NIT, CREG>, launch_pair_batch<TI, TJ, NT, MINW, 1, CREG>, \
+    NIT == 2 ? launch_pair<TI, TJ, NT, MINW, NIT, CREG, true> : nullptr, \
+    NIT == 2 ? launch_pair_batch<TI, TJ, NT, MINW, NIT, CREG, true> : nullptr, \
+    NIT == 2 ? launch_pair_rollback<TI, TJ, NT, MINW, CREG> : nullptr, \
+    NIT == 2 ? launch_pair_batch_rollback<TI, TJ, NT, MINW, CREG> : nullptr }
 static const FusedVariant kFused[] = {
-    {0, 0, 0, 0, 0, 1, nullptr, nullptr, 0, nullptr, nullptr},
+    {0, 0, 0, 0, 0, 1, nullptr, nullptr, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr},
     FV(64, 16, 256, 1, 1, 1),
     FV(64, 8, 256, 1, 1, 1),
     FV(64, 8, 256, 2, 1, 1),
@@ -563,6 +625,43 @@ static void fused_grid(const Block &b, const FusedVariant &v, int planes, int &g
 // the call's budget of `total` iterations, so the stopping test sees exactly
 // the reference's iterates.  Enqueues passes until `chunk` more iterations
 // are queued or the budget is spent; `done_iters` and `q` carry over.
+// The reference's rule (update norm every fifth iteration) with two-iteration
+// pair-owned passes: no pass is cut short for a check iteration.  A check
+// iteration that is a pass's first has its norm taken in that pass; if it
+// stops the call, the pass's second iteration is undone by redoing the first
+// alone from the pass's source (rollback_pass), which the pass did not
+// overwrite.  The result is that of one-iteration passes, bit for bit.
+static bool two_iteration_schedule(const FusedVariant &v) {
+  return g_stop.rule == 0 && v.nit == 2 && v.family != 0 && v.launch_first;
+}
+
+static int rollback_pass(Block &b, SorState *st, int variant, int planes) {
+  const FusedVariant &v = kFused[variant];
+  if (!two_iteration_schedule(v)) return 0;
+  hipStream_t s = rt().stream;
+  int gx, gy, gz;
+  fused_grid(b, v, planes, gx, gy, gz);
+  FusedArgs a;
+  a.d = b.d;
+  a.NXh = (b.d.NX + 1) / 2;
+  a.KS = planes;
+  a.zero_src = 0;
+  a.bred = b.bs[1];
+  a.bblack = b.bs[0];
+  a.Ax = b.A[0];
+  a.Ay = b.A[1];
+  a.Az = b.A[2];
+  a.omega = b.omega;
+  for (int par = 0; par < 2; ++par) {
+    a.src = par ? b.pc2 : b.f[BF_PC];
+    a.dst = par ? b.f[BF_PC] : b.pc2;
+    v.rollback(dim3((unsigned)((int64_t)gx * gy * gz)), dim3(v.nt), s, a, gx, gy, par, st);
+  }
+  hipLaunchKernelGGL(k_sor_rollback_end, dim3(1), dim3(64), 0, s, st);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
 static int enqueue_fused_passes(Block &b, SorState *st, int variant, int planes, long long start, long long total,
                                 long long chunk, long long &done_iters, long long &q, int64_t pc_ita_max,
                                 double epsi) {
@@ -584,29 +683,40 @@ static int enqueue_fused_passes(Block &b, SorState *st, int variant, int planes,
   a.omega = b.omega;
   const long long stop = done_iters + chunk < total ? done_iters + chunk : total;
   const long long C = check_period();
+  const bool first_checks = two_iteration_schedule(v);
   while (done_iters < stop) {
     const long long it = start + done_iters + 1;
-    const long long to_check = (C - it % C) % C + 1;  // up to and including the next check iteration
     long long L = v.nit;
     if (L > total - done_iters) L = total - done_iters;
-    if (L > to_check) L = to_check;
+    int check;
+    if (first_checks) {
+      // every pass two iterations (the budget's end aside); a check iteration
+      // may be a pass's first (check 2) or last (check 1)
+      check = (it + L - 1) % C == 0 ? 1 : (L == 2 && it % C == 0 ? 2 : 0);
+    } else {
+      const long long to_check = (C - it % C) % C + 1;  // up to and including the next check iteration
+      if (L > to_check) L = to_check;
+      check = (it + L - 1) % C == 0;
+    }
     const long long last_it = it + L - 1;
-    const int check = last_it % C == 0;
     a.src = (q & 1) ? b.pc2 : b.f[BF_PC];
     a.dst = (q & 1) ? b.f[BF_PC] : b.pc2;
     // rule 1 takes no update norm: the residual of the pass's result is
     // reduced after it instead
-    (L == v.nit ? v.launch : v.launch1)(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy,
-                                        g_stop.rule == 0 ? check : 0, st, b.partials);
+    if (check == 2)
+      v.launch_first(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy, 1, st, b.partials);
+    else
+      (L == v.nit ? v.launch : v.launch1)(dim3((unsigned)nwg), dim3(v.nt), s, a, gx, gy,
+                                          g_stop.rule == 0 ? check : 0, st, b.partials);
     done_iters += L;
     ++q;
     if (g_stop.rule == 1 && (check || done_iters == total)) {
       const int64_t nres = residual_check_launch(b, a.dst, st);
       hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, 1,
-                         (long long)pc_ita_max, epsi, b.partials, nres);
+                         (long long)pc_ita_max, epsi, b.partials, nres, 0);
     } else if (check || done_iters == total) {
-      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, check,
-                         (long long)pc_ita_max, epsi, b.partials, nwg);
+      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, check != 0,
+                         (long long)pc_ita_max, epsi, b.partials, nwg, check == 2 ? 1 : 0);
     }
   }
   BCFD_CHECK_LAUNCH();
@@ -831,6 +941,7 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
     if (sor_poll(b.sor, polls, stop)) return -1;
     if (stop) break;
   }
+  if (variant > 0 && rollback_pass(b, b.sor, variant, planes)) return -1;
   if (sor_timing_end(e0, b, start)) return -1;
   return sor_finish(b, b.sor, variant);
 }
@@ -899,25 +1010,34 @@ static int batched_calls(const std::vector<Block *> &blocks, const std::vector<l
   if (sor_timing_begin(&e0)) return -1;
   const long long s0 = starts[0];
   long long done_iters = 0, q = 0, since_poll = 0, polls = 0;
+  const bool first_checks = two_iteration_schedule(v);
   while (done_iters < total) {
     const long long it = s0 + done_iters + 1;
-    const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
     long long L = v.nit;
     if (L > total - done_iters) L = total - done_iters;
-    if (L > to_check) L = to_check;
     for (long long bt : budget)
       if (bt > done_iters && L > bt - done_iters) L = bt - done_iters;
-    const int check = (it + L - 1) % 5 == 0;
-    (L == v.nit ? v.batch : v.batch1)(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1),
-                                      check, g_batch_partials);
+    int check;
+    if (first_checks) {
+      check = (it + L - 1) % 5 == 0 ? 1 : (L == 2 && it % 5 == 0 ? 2 : 0);
+    } else {
+      const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
+      if (L > to_check) L = to_check;
+      check = (it + L - 1) % 5 == 0;
+    }
+    if (check == 2)
+      v.batch_first(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1), 1, g_batch_partials);
+    else
+      (L == v.nit ? v.batch : v.batch1)(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1),
+                                        check, g_batch_partials);
     done_iters += L;
     since_poll += L;
     ++q;
     bool ends = done_iters == total;
     for (long long bt : budget) ends = ends || bt == done_iters;
     if (check || ends)
-      hipLaunchKernelGGL(k_sor_batch_end, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab, done_iters, q, check,
-                         (long long)pc_ita_max, epsi, g_batch_partials);
+      hipLaunchKernelGGL(k_sor_batch_end, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab, done_iters, q,
+                         check != 0, (long long)pc_ita_max, epsi, g_batch_partials, check == 2 ? 1 : 0);
     BCFD_CHECK_LAUNCH();
     if (since_poll >= 50 && done_iters < total) {
       // every block stopped: no more passes (PcorVcor.F90:365-375)
@@ -926,6 +1046,12 @@ static int batched_calls(const std::vector<Block *> &blocks, const std::vector<l
       if (set_poll(blocks, polls, stop)) return -1;
       if (stop) break;
     }
+  }
+  if (first_checks) {
+    // blocks that stopped on a pass's first iteration redo it alone
+    v.batch_rollback(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb);
+    hipLaunchKernelGGL(k_sor_rollback_end_set, dim3(div_up(nb, 64)), dim3(64), 0, s, g_batch_tab, nb);
+    BCFD_CHECK_LAUNCH();
   }
   if (sor_timing_end_set(e0, blocks, starts)) return -1;
   for (Block *b : blocks)

@@ -180,7 +180,7 @@ static double fused_launch(const Problem<Real> &p, int KS, const std::vector<Rea
 // a step alternates with the step's parity from the first step
 template <typename Real, int TI, int TJ, int NT, int NIT, bool CREG>
 static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real> &src, std::vector<Real> &dst,
-                          bool zero_src, int order) {
+                          bool zero_src, int order, int check = 1) {
   using T = SorPairTile<Real, TI, TJ, NT, NIT, CREG>;
   {
     // the pair numbering is a bijection onto slot rows 1..SH-2 x pairs
@@ -234,7 +234,9 @@ static double pair_launch(const Problem<Real> &p, int KS, const std::vector<Real
         }
       };
       auto phase = [&](int tid, int ph) {
-        const bool nm = ph >= T::NPH - 2;
+        // the norm of the launch's last iteration (check 1) or of its first
+        // (check 2), as pair_step selects the phases
+        const bool nm = (check == 1 && ph >= T::NPH - 2) || (check == 2 && ph < 2);
         double dd;
         switch (b + (st ? 4 : 0)) {
           case 0: dd = T::template phase_u<0, false>(a, tile, lds.data(), th[tid], ph, s, nm); break;
@@ -284,8 +286,22 @@ static int run_case(int nx, int ny, int nz, int KS, int launches, bool zero_firs
   }
   for (int it = 0; it < launches; ++it) {
     const bool z = zero_first && it == 0;
-    double dref = 0.0;
-    for (int m = 0; m < NIT; ++m) dref = inplace_iteration<Real, Pair != 0>(p, ref);
+    double dref = 0.0, dfirst = 0.0;
+    for (int m = 0; m < NIT; ++m) {
+      dref = inplace_iteration<Real, Pair != 0>(p, ref);
+      if (m == 0) dfirst = dref;
+    }
+    if constexpr (Pair != 0 && NIT == 2 && sizeof(Real) == sizeof(double)) {
+      // the norm of the first iteration taken inside a two-iteration launch
+      // (the reference's check iteration falling on a pass's first)
+      std::vector<Real> scratch = (it & 1) ? A : B;
+      const double d1 = pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, (it & 1) ? B : A, scratch, z, order, 2);
+      if (d1 != dfirst) {
+        std::printf("MISMATCH first-iteration norm tile %dx%d/%d grid %dx%dx%d KS=%d launch %d: %.17g vs %.17g\n",
+                    TI, TJ, NT, nx, ny, nz, KS, it + 1, d1, dfirst);
+        return 1;
+      }
+    }
     double dfus;
     if (Pair)
       dfus = (it & 1) ? pair_launch<Real, TI, TJ, NT, NIT, Pair == 1>(p, KS, B, A, z, order)

@@ -80,3 +80,31 @@ def test_sor_time_leaves_state_unchanged():
     finally:
         sim.close()
         H.call("biocfd_sor_config", DEFAULT_VARIANT, 32)
+
+
+def test_stop_on_first_iteration_of_a_pass():
+    """Under the reference's rule every pass runs two iterations, so a check
+    iteration (every fifth) is a pass's first every other time; a call that
+    stops there redoes that iteration alone from the pass's source.  The
+    convergence-terminated case stops at 185 (a pass's first iteration) and
+    at 220 and 200 (a pass's last): every step matches the oracle."""
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = cases()["sphere_conv"]
+    steps = (1, 2, 3)
+    ref = run_oracle(case, steps)
+    iters = [int(ref[s][0]["nIterPcor"]) for s in steps]
+    print(f"nIterPcor per step {iters}")
+    assert any(n % 10 == 5 for n in iters) and any(n % 10 == 0 for n in iters), iters
+    H.call("biocfd_sor_config", DEFAULT_VARIANT, 32)
+    sim = Simulation(case)
+    bad = []
+    try:
+        for s in steps:
+            sim.step()
+            for g, (r, o) in enumerate(zip(ref[s], sim.dump())):
+                bad += compare(r, o, label=f"step {s} block {g + 1}: ")
+    finally:
+        sim.close()
+    assert not bad, "\n".join(bad[:40])
