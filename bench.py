@@ -295,6 +295,15 @@ def uneven_main(args):
         print(json.dumps({"uneven_autotune": table}), file=sys.stderr)
     else:
         variant = int(args.sor_variant)
+    runs = set(args.uneven_runs.split(","))
+    if runs != {"fp64-batched", "fp64-per-block", "mixed-batched", "mixed-per-block"}:
+        # a subset (profiling): the runs asked for, no comparison
+        out = {}
+        for r in sorted(runs):
+            prec, mode = r.split("-", 1)
+            out[r] = run_uneven(args, 1 if mode == "batched" else 0, variant, prec)[0]
+        print(json.dumps({"workload": "uneven17", "sor_variant": variant, **out}))
+        return
     b, fb, case = run_uneven(args, 1, variant)
     p, fp, _ = run_uneven(args, 0, variant)
     mb, fmb, _ = run_uneven(args, 1, variant, "mixed")
@@ -451,6 +460,8 @@ def main():
     ap.add_argument("--uneven-sweeps", type=int, default=16,
                     help="FP32 sweeps per correction of the uneven workload's mixed runs")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven workload's secondary line")
+    ap.add_argument("--uneven-runs", default="fp64-batched,fp64-per-block,mixed-batched,mixed-per-block",
+                    help="--workload uneven: the runs (a subset prints them without the comparison)")
     ap.add_argument("--workload", choices=["sphere", "uneven"], default="sphere",
                     help="sphere: the 256^3 headline workload; uneven: batched vs per-block multi-block run")
     args = ap.parse_args()
