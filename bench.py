@@ -323,6 +323,58 @@ def uneven_main(args):
             "bit_identical": bool(all(np.array_equal(x, y) for x, y in zip(fmb, fmp)))}}))
 
 
+def adjoint_summary(args, fd: bool = True):
+    """The adjoint workload (96^3 heaving, pitching ellipsoid, 40 steps, FP64,
+    fixed 501 SOR iterations per solve): J = mean body x-force over the last
+    20 steps and dJ/d(heave amplitude, pitch amplitude, phase, frequency) at
+    frozen geometry by the adjoint time steps with checkpoints every 8 steps;
+    wall time of the forward alone and of forward + backward, and central
+    differences of J (h = 1e-6) beside the gradient."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd.adjoint import AdjointRun, THETA_NAMES
+
+    case = CS.config_adjoint(n=args.adjoint_n, markers=args.adjoint_markers, nsteps=args.adjoint_steps,
+                             pc_ita_max=args.pc_ita_max, epsi=0.0)
+    run = AdjointRun(case, every=8, avg_steps=max(1, args.adjoint_steps // 2))
+    try:
+        run.forward()                       # warm-up (kernel loads, buffers)
+        H.call("biocfd_synchronize")
+        t0 = time.perf_counter()
+        J = run.forward()
+        H.call("biocfd_synchronize")
+        t_f = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        J2, grad = run.gradient()
+        H.call("biocfd_synchronize")
+        t_fb = time.perf_counter() - t0
+        out = {
+            "workload": f"adjoint{args.adjoint_n}: single block {args.adjoint_n}^3 on [0,3]^3, ellipsoid 0.4:0.2:0.1 "
+                        f"({args.adjoint_markers} markers, ID 51) at (1,1.5,1.5), heave 0.2, pitch 20 deg, phase "
+                        f"90 deg, f=0.5, Re=100, dt=1e-3, {args.adjoint_steps} steps, {run.n_iter} SOR iterations "
+                        f"per solve; J = mean x-force over the last {run.m} steps; geometry frozen",
+            "cells": case.cells(), "steps": args.adjoint_steps, "checkpoint_every": run.every, "dtype": "f64",
+            "J": J, "gradient": dict(zip(THETA_NAMES, (float(x) for x in grad))),
+            "forward_s": round(t_f, 3), "forward_backward_s": round(t_fb, 3),
+            "backward_over_forward": round(t_fb / t_f, 2),
+            "forward_mcell_steps_per_s": round(case.cells() * args.adjoint_steps / t_f / 1e6, 2),
+            "forward_bit_identical_in_gradient_run": bool(J == J2),
+        }
+        if fd:
+            h = 1e-6
+            fdv = []
+            for q in range(len(grad)):
+                e = np.zeros(len(grad))
+                e[q] = h
+                fdv.append((run.forward(run.theta0 + e) - run.forward(run.theta0 - e)) / (2 * h))
+            fdv = np.array(fdv)
+            out["central_differences"] = dict(zip(THETA_NAMES, (float(x) for x in fdv)))
+            out["max_rel_diff_vs_fd"] = float(np.max(np.abs(grad - fdv)) / max(np.max(np.abs(fdv)), 1e-300))
+        return out
+    finally:
+        run.close()
+
+
 def field_errors(ref, got):
     """max and L2 relative differences per field, and of the four body-force
     coefficients (viscous/pressure drag and lift)."""
@@ -462,8 +514,13 @@ def main():
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven workload's secondary line")
     ap.add_argument("--uneven-runs", default="fp64-batched,fp64-per-block,mixed-batched,mixed-per-block",
                     help="--workload uneven: the runs (a subset prints them without the comparison)")
-    ap.add_argument("--workload", choices=["sphere", "uneven"], default="sphere",
-                    help="sphere: the 256^3 headline workload; uneven: batched vs per-block multi-block run")
+    ap.add_argument("--workload", choices=["sphere", "uneven", "adjoint"], default="sphere",
+                    help="sphere: the 256^3 headline workload; uneven: batched vs per-block multi-block run; "
+                         "adjoint: the adjoint workload's forward and gradient")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint workload's secondary line")
+    ap.add_argument("--adjoint-n", type=int, default=96)
+    ap.add_argument("--adjoint-markers", type=int, default=2048)
+    ap.add_argument("--adjoint-steps", type=int, default=40)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the block-sharded multi-GPU "
@@ -471,6 +528,9 @@ def main():
         sys.exit(2)
     if args.workload == "uneven":
         uneven_main(args)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps({"adjoint": adjoint_summary(args)}))
         return
 
     both = args.precision == "both"
@@ -555,6 +615,8 @@ def main():
         }
     if not args.no_uneven:
         out["uneven"] = uneven_summary(args)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_summary(args, fd=False)
     if not args.no_cpu_baseline:
         try:
             out["cpu_baseline"] = cpu_baseline(args)

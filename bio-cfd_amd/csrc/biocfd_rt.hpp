@@ -150,6 +150,20 @@ struct MoveState {
   double *dum[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
+// Sensitivities of the time step for the tangent-linear and adjoint passes
+// (FP64, block layout): s[q] for the fields BF_U .. BF_P (u, v, w, ut, vt,
+// wt, p), the ghost arrays', two scratch arrays of the transposed projection, the per-task
+// adjoints of the forcing routines and a kinematic accumulator (BK_N).
+struct AdWork {
+  double *s[7] = {};
+  double *gh[BGH_N] = {};   // the thin-surface ghost arrays' sensitivities (cap_gh entries each)
+  int64_t cap_gh = 0;
+  double *scr[2] = {};
+  double *ybar = nullptr;
+  int64_t cap_ybar = 0;
+  double *kin = nullptr;
+};
+
 struct Block {
   bool live = false;
   Dims d{};
@@ -187,6 +201,7 @@ struct Block {
   // output view staging (cell-centre velocities, p, tags of the interior)
   double *out_stage = nullptr;
   int64_t out_cap = 0;
+  AdWork ad;
 };
 
 struct Interface {
@@ -248,6 +263,22 @@ int grow(T **p, int64_t *cap, int64_t need) {
 
 inline unsigned div_up(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
+// b.f <-> b.ad.s for the seven fields while it lives: the tangent-linear
+// passes run the forward routines, linear in the fields at frozen geometry,
+// on the tangents
+struct SwapFields {
+  Block &b;
+  explicit SwapFields(Block &blk) : b(blk) { flip(); }
+  ~SwapFields() { flip(); }
+  void flip() {
+    for (int q = 0; q < 7; ++q) {
+      double *t = b.f[q];
+      b.f[q] = b.ad.s[q];
+      b.ad.s[q] = t;
+    }
+  }
+};
+
 // internal entry points shared between translation units
 int poisson_block_setup(Block &b);
 int ib_list_capacity(Block &b, int64_t n_ib);
@@ -257,6 +288,24 @@ int precision_mode();
 int health_after_solve(int first, int last);
 int stress_scratch_init(int device);
 int mixed_sor(Block &b, int64_t pc_ita_max, double dt);
+// tangent-linear (tlm) and adjoint (adj) passes over b.ad (geometry frozen)
+int ad_ready(Block &b);
+int ad_ghosts(Block &b);
+int momentum_tlm(Block &b, double dt, double re);
+int momentum_adj(Block &b, double dt, double re);
+int forcing_tlm(Block &b, int routine, const double kin[BK_N], const double kin_dot[BK_N]);
+int forcing_adj(Block &b, int routine, const double kin[BK_N]);
+int stress_tlm(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
+               double out[4]);
+int stress_adj(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double cbar[4]);
+int compute_div(Block &b);
+int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi);
+int correct(Block &b, double dt);
+int velocity_bc(Block &b, double dt, double uc, double inlet);
+int norm_copy(Block &b, double dt);
+int projection(Block &b, int64_t n_iter, double dt);
+int stress_sums(Block &b, double mu_f, double rho_f, const double kin[BK_N], double out[4]);
+void ad_release(Block &b);
 int mixed_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt);
 bool mixed_batchable();
 

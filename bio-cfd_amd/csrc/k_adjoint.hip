@@ -137,11 +137,12 @@ __global__ void k_adj_sor_b(Dims d, const uint8_t *__restrict__ cell, const doub
 
 // utbar = ubar + D^T bbar; computeDiv: b(i) = (ut(i) - ut(i-1)) / dx(i) + ...
 // at fluid cells (PcorVcor.F90:229-242, k_div's expressions)
+// (ub and utb may be the same array: each thread reads and writes only its
+// own cell of them)
 __global__ void k_adj_div(Dims d, const uint8_t *__restrict__ cell, const double *__restrict__ bbar,
-                          const double *__restrict__ ub, const double *__restrict__ vb,
-                          const double *__restrict__ wb, const double *__restrict__ dxv,
-                          const double *__restrict__ dyv, const double *__restrict__ dzv,
-                          double *__restrict__ utb, double *__restrict__ vtb, double *__restrict__ wtb) {
+                          const double *ub, const double *vb, const double *wb, const double *__restrict__ dxv,
+                          const double *__restrict__ dyv, const double *__restrict__ dzv, double *utb, double *vtb,
+                          double *wtb) {
   Cell x;
   if (!cell_of(d, x)) return;
   const int i = x.i, j = x.j, k = x.k;
@@ -400,6 +401,590 @@ int biocfd_fine_update_bd_adjoint(int itf, int comp, double *lam_fine, double *l
   BCFD_CHECK_LAUNCH();
   if (download(lam_coarse, lc, nc) || download(lam_fine, lf, nf)) return -1;
   BCFD_HIP(hipStreamSynchronize(rt().stream));
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The adjoint time step (no counterpart in the reference): sensitivity
+// arrays on the device, the transposes of the boundary conditions, of
+// solidCellBC and of the projection stage of poissonSolver, block
+// snapshots for step checkpointing, and the C entry points of every
+// tangent-linear and adjoint piece.
+//
+// A time step of a single block with a body (app/main.F90:133-263 in the
+// order the drivers call it) is, at frozen geometry, x -> x' with
+// x = (u, v, w, p, ut, vt, wt):
+//   start (the geometry at the step's start, kinematics kin_a):
+//     nsMomentum2order; velocityBC; solidCellBC; velocityForcing1(kin_a);
+//     velocityBC; poissonSolver (projection with a fixed SOR count,
+//     velocityBC, u <- ut); pressureForcing1(kin_a)
+//   end (the geometry after the body moved, kinematics kin_b):
+//     velocityForcingField(kin_b); pressureForcingField(kin_b)
+// Only nsMomentum2order is non-linear in x; every other piece is linear in
+// x and in the surface terms.
+// ---------------------------------------------------------------------------
+namespace bcfd {
+namespace {
+
+__global__ void k_ad_fill(double *__restrict__ x, int64_t n, double v) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) x[q] = v;
+}
+
+// transpose of velocityBC's z pass (k_bc_z), statements in reverse order
+__global__ void k_adj_bc_z(Dims d, double *__restrict__ lu, double *__restrict__ lv, double *__restrict__ lw) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 2 + blockIdx.y;
+  if (i > d.nx + 1) return;
+  const int nz = d.nz;
+  const int64_t c1 = ix3(d, i, j, 1), c2 = ix3(d, i, j, 2), cn1 = ix3(d, i, j, nz + 1), cn2 = ix3(d, i, j, nz + 2);
+  lw[cn1] = 0.0;
+  lv[cn1] += lv[cn2]; lv[cn2] = 0.0;
+  lu[cn1] += lu[cn2]; lu[cn2] = 0.0;
+  lw[c1] = 0.0;
+  lv[c2] += lv[c1]; lv[c1] = 0.0;
+  lu[c2] += lu[c1]; lu[c1] = 0.0;
+}
+
+// transpose of the y pass (k_bc_y)
+__global__ void k_adj_bc_y(Dims d, double *__restrict__ lu, double *__restrict__ lv, double *__restrict__ lw) {
+  const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = 2 + blockIdx.y;
+  if (i > d.nx + 1) return;
+  const int ny = d.ny;
+  const int64_t c1 = ix3(d, i, 1, k), c2 = ix3(d, i, 2, k), cn1 = ix3(d, i, ny + 1, k), cn2 = ix3(d, i, ny + 2, k);
+  lv[cn1] = 0.0;
+  lw[cn1] += lw[cn2]; lw[cn2] = 0.0;
+  lu[cn1] += lu[cn2]; lu[cn2] = 0.0;
+  lv[c2] = 0.0;
+  lw[c2] += lw[c1]; lw[c1] = 0.0;
+  lu[c2] += lu[c1]; lu[c1] = 0.0;
+}
+
+// transpose of the x pass (k_bc_x): inflow, then the convective outflow,
+// which reads the fields u, v, w of the step's start
+__global__ void k_adj_bc_x(Dims d, double *__restrict__ lut, double *__restrict__ lvt, double *__restrict__ lwt,
+                           double *__restrict__ lu, double *__restrict__ lv, double *__restrict__ lw,
+                           const double *__restrict__ dxv, double dt, double uc) {
+  const int j = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = 2 + blockIdx.y;
+  if (j > d.ny + 1) return;
+  const int nx = d.nx;
+  const int64_t c1 = ix3(d, 1, j, k), cn = ix3(d, nx, j, k), cn1 = ix3(d, nx + 1, j, k), cn2 = ix3(d, nx + 2, j, k);
+  const double c2 = (2.0 * dt / dxv[nx + 1]) * uc, cu = (dt / dxv[nx]) * uc;
+  double a = lwt[cn2];
+  lwt[cn2] = 0.0;
+  lwt[cn1] -= a;
+  lw[cn2] += a * (1.0 - c2);
+  lw[cn1] += a * (1.0 + c2);
+  a = lvt[cn2];
+  lvt[cn2] = 0.0;
+  lvt[cn1] -= a;
+  lv[cn2] += a * (1.0 - c2);
+  lv[cn1] += a * (1.0 + c2);
+  a = lut[cn1];
+  lut[cn1] = 0.0;
+  lu[cn1] += a * (1.0 - cu);
+  lu[cn] += a * cu;
+  a = lwt[c1];
+  lwt[c1] = 0.0;
+  lwt[cn1] -= a;
+  a = lvt[c1];
+  lvt[c1] = 0.0;
+  lvt[cn1] -= a;
+  lut[c1] = 0.0;
+}
+
+// transpose of solidCellBC (k_solid_bc without the u copy): zero the
+// adjoints of ut, vt, wt, p at solid cells
+__global__ void k_adj_solid(Dims d, const uint8_t *__restrict__ cell, double *__restrict__ lut,
+                            double *__restrict__ lvt, double *__restrict__ lwt, double *__restrict__ lp) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)d.nx * d.ny * d.nz) return;
+  const int i = 2 + (int)(q % d.nx), j = 2 + (int)((q / d.nx) % d.ny), k = 2 + (int)(q / ((int64_t)d.nx * d.ny));
+  const int64_t c = ix3(d, i, j, k);
+  if (cell[c] != 1) return;
+  lut[c] = 0.0; lvt[c] = 0.0; lwt[c] = 0.0; lp[c] = 0.0;
+}
+
+// transpose of u, v, w <- ut, vt, wt (every cell)
+__global__ void k_adj_copy(int64_t n, double *__restrict__ lu, double *__restrict__ lv, double *__restrict__ lw,
+                           double *__restrict__ lut, double *__restrict__ lvt, double *__restrict__ lwt) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  lut[q] += lu[q]; lu[q] = 0.0;
+  lvt[q] += lv[q]; lv[q] = 0.0;
+  lwt[q] += lw[q]; lw[q] = 0.0;
+}
+
+// snapshot of the state a time step reads and writes: fields, tags, the IB,
+// TS and solid lists (in their capacity layouts), element and node data,
+// the counts
+struct Snapshot {
+  bool live = false;
+  std::vector<void *> bufs;
+  std::vector<size_t> bytes;
+  int64_t cap_ib = 0, cap_ts = 0, cap_solid = 0, n_mapped = 0;
+  int64_t counts[BC_N] = {};
+};
+std::vector<Snapshot> g_snaps[kMaxBlocks + 1];
+
+// every saved buffer of block b with its size at the current capacities
+template <class F>
+void snapshot_visit(Block &b, F f) {
+  IbData &ib = b.ib;
+  for (int q = 0; q < 7; ++q) f((void **)&b.f[q], sizeof(double) * (size_t)b.n2);
+  for (int t = 0; t < BT_NTAG; ++t)
+    f((void **)&b.tag[t], (size_t)((t == BT_CELL || t == BT_CELL_N) ? b.n2 : b.n3));
+  if (!ib.present) return;
+  f((void **)&ib.ib_map, sizeof(int) * (size_t)b.n2);
+  if (ib.ib_ijk) {
+    f((void **)&ib.ib_ijk, sizeof(int) * (size_t)(3 * ib.cap_ib));
+    for (int q = 0; q < BN_NNEL; ++q) {
+      f((void **)&ib.nel[q], sizeof(int) * (size_t)ib.cap_ib);
+      f((void **)&ib.normdis[q], sizeof(double) * (size_t)ib.cap_ib);
+    }
+  }
+  if (ib.ts_ijk) {
+    f((void **)&ib.ts_ijk, sizeof(int) * (size_t)(3 * ib.cap_ts));
+    f((void **)&ib.index_ts, sizeof(int) * (size_t)ib.cap_ts);
+    for (int q = 0; q < BGH_N; ++q) f((void **)&ib.ghost[q], sizeof(double) * (size_t)ib.cap_ts);
+  }
+  if (ib.solid_ijk) f((void **)&ib.solid_ijk, sizeof(int) * (size_t)ib.cap_solid);
+  for (int q = 0; q < 3; ++q) f((void **)&ib.node[q], sizeof(double) * (size_t)ib.n_nodes);
+  for (int q = 0; q < BE_NELEM; ++q) f((void **)&ib.elem[q], sizeof(double) * (size_t)ib.n_elems);
+}
+
+int snapshot_save(Block &b, int id, int slot) {
+  std::vector<Snapshot> &v = g_snaps[id];
+  if ((int)v.size() <= slot) v.resize(slot + 1);
+  Snapshot &sn = v[slot];
+  IbData &ib = b.ib;
+  size_t k = 0;
+  int rc = 0;
+  snapshot_visit(b, [&](void **p, size_t n) {
+    if (rc || !*p) return;
+    if (k >= sn.bufs.size()) { sn.bufs.push_back(nullptr); sn.bytes.push_back(0); }
+    if (sn.bytes[k] != n) {
+      if (sn.bufs[k]) (void)hipFree(sn.bufs[k]);
+      sn.bufs[k] = nullptr;
+      char *q = nullptr;
+      if (dev_alloc(&q, (int64_t)n)) { rc = -1; return; }
+      sn.bufs[k] = q;
+      sn.bytes[k] = n;
+    }
+    if (hipMemcpyAsync(sn.bufs[k], *p, n, hipMemcpyDeviceToDevice, rt().stream) != hipSuccess) {
+      set_error("snapshot: copy failed");
+      rc = -1;
+    }
+    ++k;
+  });
+  if (rc) return -1;
+  sn.cap_ib = ib.cap_ib; sn.cap_ts = ib.cap_ts; sn.cap_solid = ib.cap_solid; sn.n_mapped = ib.n_mapped;
+  for (int q = 0; q < BC_N; ++q) sn.counts[q] = b.counts[q];
+  sn.live = true;
+  return 0;
+}
+
+int snapshot_restore(Block &b, int id, int slot) {
+  std::vector<Snapshot> &v = g_snaps[id];
+  if (slot < 0 || slot >= (int)v.size() || !v[slot].live) { set_error("snapshot %d of block %d not saved", slot, id); return -1; }
+  Snapshot &sn = v[slot];
+  IbData &ib = b.ib;
+  // the list buffers at the saved capacities (their layouts depend on them)
+  if (ib.present) {
+    if (ib.cap_ib != sn.cap_ib) {
+      if (dev_alloc(&ib.ib_ijk, 3 * sn.cap_ib)) return -1;
+      for (int q = 0; q < BN_NNEL; ++q)
+        if (dev_alloc(&ib.nel[q], sn.cap_ib) || dev_alloc(&ib.normdis[q], sn.cap_ib)) return -1;
+      ib.cap_ib = sn.cap_ib;
+    }
+    if (ib.cap_ts != sn.cap_ts) {
+      if (dev_alloc(&ib.ts_ijk, 3 * sn.cap_ts) || dev_alloc(&ib.index_ts, sn.cap_ts)) return -1;
+      for (int q = 0; q < BGH_N; ++q)
+        if (dev_alloc(&ib.ghost[q], sn.cap_ts)) return -1;
+      ib.cap_ts = sn.cap_ts;
+    }
+    if (ib.cap_solid != sn.cap_solid) {
+      if (dev_alloc(&ib.solid_ijk, sn.cap_solid)) return -1;
+      ib.cap_solid = sn.cap_solid;
+    }
+  }
+  size_t k = 0;
+  int rc = 0;
+  snapshot_visit(b, [&](void **p, size_t n) {
+    if (rc || !*p) return;
+    if (k >= sn.bufs.size() || sn.bytes[k] != n) {
+      set_error("snapshot %d of block %d does not match the block's buffers", slot, id);
+      rc = -1;
+      return;
+    }
+    if (hipMemcpyAsync(*p, sn.bufs[k], n, hipMemcpyDeviceToDevice, rt().stream) != hipSuccess) {
+      set_error("snapshot: copy failed");
+      rc = -1;
+    }
+    ++k;
+  });
+  if (rc) return -1;
+  ib.n_mapped = sn.n_mapped;
+  for (int q = 0; q < BC_N; ++q) b.counts[q] = sn.counts[q];
+  return 0;
+}
+
+void snapshot_free(int id, int slot) {
+  std::vector<Snapshot> &v = g_snaps[id];
+  for (int s = 0; s < (int)v.size(); ++s) {
+    if (slot >= 0 && s != slot) continue;
+    for (void *p : v[s].bufs)
+      if (p) (void)hipFree(p);
+    v[s] = Snapshot();
+  }
+  if (slot < 0) v.clear();
+}
+
+Block *ad_block(int id) {
+  Block *b = get_block(id);
+  if (!b) return nullptr;
+  if (ad_ready(*b)) return nullptr;
+  return b;
+}
+
+int n_interior_grid(const Block &b, dim3 &g) {
+  g = dim3(div_up((int64_t)b.d.nx * b.d.ny * b.d.nz, 256));
+  return 0;
+}
+
+int velocity_bc_adj(Block &b, double dt, double uc) {
+  hipStream_t s = rt().stream;
+  double **l = b.ad.s;
+  hipLaunchKernelGGL(k_adj_bc_z, dim3(div_up(b.d.nx, 64), b.d.ny), dim3(64), 0, s, b.d, l[BF_UT], l[BF_VT], l[BF_WT]);
+  hipLaunchKernelGGL(k_adj_bc_y, dim3(div_up(b.d.nx, 64), b.d.nz), dim3(64), 0, s, b.d, l[BF_UT], l[BF_VT], l[BF_WT]);
+  hipLaunchKernelGGL(k_adj_bc_x, dim3(div_up(b.d.ny, 64), b.d.nz), dim3(64), 0, s, b.d, l[BF_UT], l[BF_VT], l[BF_WT],
+                     l[BF_U], l[BF_V], l[BF_W], b.g[BG_DX], dt, uc);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+int velocity_bc_tlm(Block &b, double dt, double uc) {
+  SwapFields sw(b);
+  return velocity_bc(b, dt, uc, 0.0);
+}
+
+int solid_bc_adj(Block &b) {
+  dim3 g;
+  n_interior_grid(b, g);
+  double **l = b.ad.s;
+  hipLaunchKernelGGL(k_adj_solid, g, dim3(256), 0, rt().stream, b.d, b.tag[BT_CELL], l[BF_UT], l[BF_VT], l[BF_WT],
+                     l[BF_P]);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+int solid_bc_tlm(Block &b) {
+  // the forward zeroes ut, vt, wt, p at solid cells: so do the tangents
+  return solid_bc_adj(b);
+}
+
+// tangent of poissonSolver on one block: the projection with n_iter SOR
+// iterations, velocityBC (tangent inflow 0), u <- ut
+int poisson_tlm(Block &b, int64_t n_iter, double dt, double uc) {
+  SwapFields sw(b);
+  if (projection(b, n_iter, dt)) return -1;
+  if (velocity_bc(b, dt, uc, 0.0)) return -1;
+  return norm_copy(b, dt);
+}
+
+int poisson_adj(Block &b, int64_t n_iter, double dt, double uc) {
+  if (!b.A[0] || !b.g[BG_DX]) { set_error("poisson_adj: block is not set up"); return -1; }
+  if (n_iter < 1) { set_error("poisson_adj: iteration count %lld < 1", (long long)n_iter); return -1; }
+  hipStream_t s = rt().stream;
+  double **l = b.ad.s;
+  const int64_t n = b.n2;
+  const dim3 blk(256), grd(div_up(n, 256));
+  hipLaunchKernelGGL(k_adj_copy, grd, blk, 0, s, n, l[BF_U], l[BF_V], l[BF_W], l[BF_UT], l[BF_VT], l[BF_WT]);
+  BCFD_CHECK_LAUNCH();
+  if (velocity_bc_adj(b, dt, uc)) return -1;
+  const Dims &d = b.d;
+  const uint8_t *cell = b.tag[BT_CELL];
+  double *lpc = b.ad.scr[0], *bbar = b.ad.scr[1];
+  hipLaunchKernelGGL(k_adj_grad, grd, blk, 0, s, d, cell, l[BF_UT], l[BF_VT], l[BF_WT], l[BF_P], b.g[BG_DX],
+                     b.g[BG_DY], b.g[BG_DZ], dt, lpc, bbar);
+  for (int64_t it = 0; it < n_iter; ++it) {
+    for (int colour : {0, 1}) {
+      hipLaunchKernelGGL(k_adj_sor_a, grd, blk, 0, s, d, cell, b.A[0], b.A[1], b.A[2], b.omega, colour, lpc);
+      hipLaunchKernelGGL(k_adj_sor_b, grd, blk, 0, s, d, cell, b.A[0], b.A[1], b.A[2], b.omega, dt, colour, lpc,
+                         bbar);
+    }
+  }
+  // each thread reads and writes only its own cell of the velocity adjoints
+  hipLaunchKernelGGL(k_adj_div, grd, blk, 0, s, d, cell, bbar, l[BF_UT], l[BF_VT], l[BF_WT], b.g[BG_DX], b.g[BG_DY],
+                     b.g[BG_DZ], l[BF_UT], l[BF_VT], l[BF_WT]);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+bool fp64_only() {
+  if (precision_mode() != 0) { set_error("the tangent-linear and adjoint passes run the FP64 method only"); return false; }
+  return true;
+}
+
+enum { AD_V1 = 0, AD_P1 = 1, AD_VF = 2, AD_PF = 3, AD_VG = 4, AD_PG = 5 };
+
+}  // namespace
+
+int ad_ready(Block &b) {
+  AdWork &w = b.ad;
+  if (w.s[0]) return 0;
+  hipStream_t s = rt().stream;
+  for (int q = 0; q < 7; ++q) {
+    if (dev_alloc(&w.s[q], b.n2)) return -1;
+    hipLaunchKernelGGL(k_ad_fill, dim3(div_up(b.n2, 256)), dim3(256), 0, s, w.s[q], b.n2, 0.0);
+  }
+  for (int q = 0; q < 2; ++q)
+    if (dev_alloc(&w.scr[q], b.n2)) return -1;
+  if (dev_alloc(&w.kin, 32)) return -1;
+  hipLaunchKernelGGL(k_ad_fill, dim3(1), dim3(64), 0, s, w.kin, (int64_t)32, 0.0);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+void ad_release(Block &b) {
+  AdWork &w = b.ad;
+  for (auto &p : w.s) if (p) { (void)hipFree(p); p = nullptr; }
+  for (auto &p : w.scr) if (p) { (void)hipFree(p); p = nullptr; }
+  for (auto &p : w.gh) if (p) { (void)hipFree(p); p = nullptr; }
+  if (w.ybar) (void)hipFree(w.ybar);
+  if (w.kin) (void)hipFree(w.kin);
+  w = AdWork();
+  for (int id = 1; id <= kMaxBlocks; ++id)
+    if (&rt().blk[id] == &b) snapshot_free(id, -1);
+}
+
+}  // namespace bcfd
+
+extern "C" {
+
+int biocfd_ad_reset(int id) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  hipStream_t s = rt().stream;
+  for (int q = 0; q < 7; ++q)
+    hipLaunchKernelGGL(k_ad_fill, dim3(div_up(b->n2, 256)), dim3(256), 0, s, b->ad.s[q], b->n2, 0.0);
+  for (int q = 0; q < BGH_N; ++q)
+    if (b->ad.gh[q])
+      hipLaunchKernelGGL(k_ad_fill, dim3(div_up(b->ad.cap_gh, 256)), dim3(256), 0, s, b->ad.gh[q], b->ad.cap_gh, 0.0);
+  hipLaunchKernelGGL(k_ad_fill, dim3(1), dim3(64), 0, s, b->ad.kin, (int64_t)BK_N, 0.0);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+int biocfd_ad_upload(int id, int field, const double *host) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (field < 0 || field > BF_P) { set_error("ad_upload: field %d outside u .. p", field); return -1; }
+  return upload(b->ad.s[field], host, b->n2);
+}
+
+int biocfd_ad_download(int id, int field, double *host) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (field < 0 || field > BF_P) { set_error("ad_download: field %d outside u .. p", field); return -1; }
+  if (download(host, b->ad.s[field], b->n2)) return -1;
+  BCFD_HIP(hipStreamSynchronize(rt().stream));
+  return 0;
+}
+
+int biocfd_ad_ghost_upload(int id, int which, const double *host, int64_t count) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || ad_ghosts(*b)) return -1;
+  if (which < 0 || which >= BGH_N || count < 0 || count > b->ad.cap_gh) {
+    set_error("ad_ghost_upload: ghost %d, %lld entries (capacity %lld)", which, (long long)count,
+              (long long)b->ad.cap_gh);
+    return -1;
+  }
+  return count ? upload(b->ad.gh[which], host, count) : 0;
+}
+
+int biocfd_ad_ghost_download(int id, int which, double *host, int64_t count) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || ad_ghosts(*b)) return -1;
+  if (which < 0 || which >= BGH_N || count < 0 || count > b->ad.cap_gh) {
+    set_error("ad_ghost_download: ghost %d, %lld entries (capacity %lld)", which, (long long)count,
+              (long long)b->ad.cap_gh);
+    return -1;
+  }
+  if (count && download(host, b->ad.gh[which], count)) return -1;
+  BCFD_HIP(hipStreamSynchronize(rt().stream));
+  return 0;
+}
+
+int biocfd_ad_kin(int id, double kin_bar[BK_N], int clear) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (download(rt().h_pinned, b->ad.kin, BK_N)) return -1;
+  BCFD_HIP(hipStreamSynchronize(rt().stream));
+  for (int q = 0; q < BK_N; ++q) kin_bar[q] = rt().h_pinned[q];
+  if (clear) {
+    hipLaunchKernelGGL(k_ad_fill, dim3(1), dim3(64), 0, rt().stream, b->ad.kin, (int64_t)BK_N, 0.0);
+    BCFD_CHECK_LAUNCH();
+  }
+  return 0;
+}
+
+int biocfd_tlm_momentum(int id, double dt, double re) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? momentum_tlm(*b, dt, re) : -1;
+}
+
+int biocfd_adj_momentum(int id, double dt, double re) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? momentum_adj(*b, dt, re) : -1;
+}
+
+int biocfd_tlm_velocity_bc(int id, double dt, double uc) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? velocity_bc_tlm(*b, dt, uc) : -1;
+}
+
+int biocfd_adj_velocity_bc(int id, double dt, double uc) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? velocity_bc_adj(*b, dt, uc) : -1;
+}
+
+int biocfd_tlm_solid_cell_bc(int id) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? solid_bc_tlm(*b) : -1;
+}
+
+int biocfd_adj_solid_cell_bc(int id) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? solid_bc_adj(*b) : -1;
+}
+
+int biocfd_tlm_forcing(int id, int routine, const double kin[BK_N], const double kin_dot[BK_N]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (routine < AD_V1 || routine > AD_PG) { set_error("tlm_forcing: routine %d outside 0..5", routine); return -1; }
+  return forcing_tlm(*b, routine, kin, kin_dot);
+}
+
+int biocfd_adj_forcing(int id, int routine, const double kin[BK_N]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (routine < AD_V1 || routine > AD_PG) { set_error("adj_forcing: routine %d outside 0..5", routine); return -1; }
+  return forcing_adj(*b, routine, kin);
+}
+
+int biocfd_tlm_poisson(int id, int64_t n_iter, double dt, double uc) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || !fp64_only()) return -1;
+  return poisson_tlm(*b, n_iter, dt, uc);
+}
+
+int biocfd_adj_poisson(int id, int64_t n_iter, double dt, double uc) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || !fp64_only()) return -1;
+  return poisson_adj(*b, n_iter, dt, uc);
+}
+
+int biocfd_stress_sums(int id, double mu_f, double rho_f, const double kin[BK_N], double out[4]) {
+  BCFD_API_LOCK();
+  Block *b = get_block(id);
+  return b ? stress_sums(*b, mu_f, rho_f, kin, out) : -1;
+}
+
+int biocfd_tlm_stress(int id, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
+                      double out[4]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? stress_tlm(*b, mu_f, rho_f, kin, kin_dot, out) : -1;
+}
+
+int biocfd_adj_stress(int id, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  return b ? stress_adj(*b, mu_f, rho_f, kin, sbar) : -1;
+}
+
+int biocfd_tlm_step_start(int id, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N],
+                          const double kin_dot[BK_N]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || !fp64_only()) return -1;
+  if (momentum_tlm(*b, dt, re) || velocity_bc_tlm(*b, dt, uc) || solid_bc_tlm(*b) ||
+      forcing_tlm(*b, AD_V1, kin, kin_dot) || velocity_bc_tlm(*b, dt, uc) || poisson_tlm(*b, n_iter, dt, uc) ||
+      forcing_tlm(*b, AD_P1, kin, kin_dot))
+    return -1;
+  return 0;
+}
+
+int biocfd_tlm_step_end(int id, const double kin[BK_N], const double kin_dot[BK_N]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (forcing_tlm(*b, AD_VF, kin, kin_dot) || forcing_tlm(*b, AD_PF, kin, kin_dot) ||
+      forcing_tlm(*b, AD_VG, kin, kin_dot) || forcing_tlm(*b, AD_PG, kin, kin_dot))
+    return -1;
+  return 0;
+}
+
+int biocfd_adj_step_end(int id, const double kin[BK_N]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (forcing_adj(*b, AD_PG, kin) || forcing_adj(*b, AD_VG, kin) || forcing_adj(*b, AD_PF, kin) ||
+      forcing_adj(*b, AD_VF, kin))
+    return -1;
+  return 0;
+}
+
+int biocfd_adj_step_start(int id, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N]) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || !fp64_only()) return -1;
+  if (forcing_adj(*b, AD_P1, kin) || poisson_adj(*b, n_iter, dt, uc) || velocity_bc_adj(*b, dt, uc) ||
+      forcing_adj(*b, AD_V1, kin) || solid_bc_adj(*b) || velocity_bc_adj(*b, dt, uc) || momentum_adj(*b, dt, re))
+    return -1;
+  return 0;
+}
+
+int biocfd_snapshot_save(int id, int slot) {
+  BCFD_API_LOCK();
+  Block *b = get_block(id);
+  if (!b) return -1;
+  if (slot < 0 || slot >= 4096) { set_error("snapshot slot %d outside 0..4095", slot); return -1; }
+  return snapshot_save(*b, id, slot);
+}
+
+int biocfd_snapshot_restore(int id, int slot) {
+  BCFD_API_LOCK();
+  Block *b = get_block(id);
+  if (!b) return -1;
+  return snapshot_restore(*b, id, slot);
+}
+
+int biocfd_snapshot_free(int id, int slot) {
+  BCFD_API_LOCK();
+  if (id < 1 || id > kMaxBlocks) { set_error("no block %d", id); return -1; }
+  snapshot_free(id, slot);
   return 0;
 }
 

@@ -38,6 +38,10 @@ struct FArgs {
   const double *fld[4];    // arrays interpolated: p, ut, vt, wt
   const double *plain[4];  // arrays copied into the "t" ghosts: p, u, v, w
   int *miss;
+  // tangent-linear mode: the surface values are the tangents of the surface
+  // terms along kin_dot (the fields passed are tangents too)
+  int tangent;
+  double kin_dot[BK_N];
 };
 
 struct PartSpec {
@@ -160,7 +164,8 @@ struct Reader {
 };
 
 // compute_value_and_derivatives (Forcing.f90:1091-1161)
-__device__ inline void value_derivs(const Reader &rd, double x, double y, double z, int i, int j, int k,
+template <class R>
+__device__ inline void value_derivs(const R &rd, double x, double y, double z, int i, int j, int k,
                                     const double *xg, const double *yg, const double *zg, int off,
                                     double &val, double dv[3]) {
   int ii = i, jj = j, kk = k;
@@ -215,9 +220,57 @@ __device__ inline double surface_term(const FArgs &a, int routine, int part, int
   return -td * (zc - k[BK_PIVZ]) + (-k[BK_ALPHADOT]) * (xc - k[BK_PIVX]) + k[BK_YDOT];
 }
 
-// one reconstruction; returns false when an ordered read is not available yet
-__device__ bool run_task(const FArgs &a, int routine, int64_t cell_ord, int part, const Reader &rd_in,
-                         double &out) {
+// gradient of surface_term with respect to the kinematic vector (the
+// tangent-linear and adjoint passes; element positions held fixed)
+__device__ inline void surface_term_grad(const FArgs &a, int routine, int comp, int e, double g[BK_N]) {
+  for (int q = 0; q < BK_N; ++q) g[q] = 0.0;
+  const int sid = a.surf[e];
+  const double *k = a.kin;
+  const bool flap = (sid == 51 || sid == 52);
+  const int std_ = (sid == 51) ? BK_THETADOT1 : BK_THETADOT2;
+  const int stdd = (sid == 51) ? BK_THETADDOT1 : BK_THETADDOT2;
+  const double td = k[std_], tdd = k[stdd];
+  const double xc = a.el[BE_XCENT][e], yc = a.el[BE_YCENT][e], zc = a.el[BE_ZCENT][e];
+  const double ry = yc - k[BK_PIVY], rz = zc - k[BK_PIVZ];
+  if (comp == 0) {
+    const double sg = (routine == R_PG) ? 1.0 : -1.0;
+    const double cb = a.el[BE_COSB][e], cg = a.el[BE_COSG][e];
+    // S = sg ((ac_z + at_z) cg + (ac_y + at_y) cb + yddot cb)
+    g[BK_YDDOT] = sg * cb;
+    if (flap) {
+      g[std_] = sg * (-2.0 * td * rz * cg - 2.0 * td * ry * cb);
+      g[stdd] = sg * (ry * cg - rz * cb);
+      g[BK_PIVZ] = sg * (td * td * cg + tdd * cb);
+      g[BK_PIVY] = sg * (-tdd * cg + td * td * cb);
+    }
+    return;
+  }
+  if (comp == 1) { g[BK_XDOT] = 1.0; return; }
+  if (comp == 3) {
+    if (flap) { g[std_] = ry; g[BK_PIVY] = -td; }
+    return;
+  }
+  g[BK_YDOT] = 1.0;
+  if (!flap) return;
+  g[std_] = -rz;
+  g[BK_PIVZ] = td;
+  if (routine != R_V1) {
+    g[BK_ALPHADOT] = -(xc - k[BK_PIVX]);
+    g[BK_PIVX] = k[BK_ALPHADOT];
+  }
+}
+
+// The geometry of one reconstruction: the written component and slot, the
+// image point and its interpolation cube, the normal and the distances.
+struct TaskGeom {
+  PartSpec s;
+  const double *gx, *gy, *gz;
+  int ix, iy, iz, e;
+  double px, py, pz, ca, cb, cg, sur2, n1;
+  bool ghost;
+};
+
+__device__ inline void task_geom(const FArgs &a, int routine, int64_t cell_ord, int part, TaskGeom &t) {
   const Dims &d = a.d;
   const int i = a.ijk[cell_ord], j = a.ijk[cell_ord + a.cap], k = a.ijk[cell_ord + 2 * a.cap];
   const bool ghost = (routine == R_VG || routine == R_PG);
@@ -262,31 +315,59 @@ __device__ bool run_task(const FArgs &a, int routine, int64_t cell_ord, int part
   }
   // keep the interpolation cube inside the arrays (a miss is reported)
   const int mx = (s.comp == 1) ? 2 : 1, my = (s.comp == 2) ? 2 : 1, mz = (s.comp == 3) ? 2 : 1;
-  ix = ix < mx ? mx : (ix > d.NX + mx - 2 ? d.NX + mx - 2 : ix);
-  iy = iy < my ? my : (iy > d.NY + my - 2 ? d.NY + my - 2 : iy);
-  iz = iz < mz ? mz : (iz > d.NZ + mz - 2 ? d.NZ + mz - 2 : iz);
-  bool blocked = false;
-  Reader rd = rd_in;
-  rd.comp = s.comp;
-  rd.blocked = &blocked;
-  double val, dv[3];
-  value_derivs(rd, px, py, pz, ix, iy, iz, gx, gy, gz, s.comp, val, dv);
-  if (blocked) return false;
-  double dn = dv[0] * ca + dv[1] * cb + dv[2] * cg;
-  if (ghost) dn = -1.0 * dn;
-  const double n1 = pt1 + sur2;
-  const double surfv = surface_term(a, routine, part, s.comp, e);
-  if (s.comp == 0) {
+  t.ix = ix < mx ? mx : (ix > d.NX + mx - 2 ? d.NX + mx - 2 : ix);
+  t.iy = iy < my ? my : (iy > d.NY + my - 2 ? d.NY + my - 2 : iy);
+  t.iz = iz < mz ? mz : (iz > d.NZ + mz - 2 ? d.NZ + mz - 2 : iz);
+  t.s = s;
+  t.gx = gx; t.gy = gy; t.gz = gz;
+  t.e = e;
+  t.px = px; t.py = py; t.pz = pz;
+  t.ca = ca; t.cb = cb; t.cg = cg;
+  t.sur2 = sur2;
+  t.n1 = pt1 + sur2;
+  t.ghost = ghost;
+}
+
+// the quadratic profile's value at the node from the image-point value and
+// derivatives and the surface term
+__device__ inline double task_value(const TaskGeom &t, double val, const double dv[3], double surfv) {
+  double dn = dv[0] * t.ca + dv[1] * t.cb + dv[2] * t.cg;
+  if (t.ghost) dn = -1.0 * dn;
+  const double n1 = t.n1, sur2 = t.sur2;
+  if (t.s.comp == 0) {
     const double bval = surfv;
     const double aval = (dn - surfv) / (2.0 * n1);
     const double cval = val - (dn + surfv) * n1 * 0.5;
-    out = aval * (sur2 * sur2) + bval * sur2 + cval;
-  } else {
-    const double cval = surfv;
-    const double bval = 2.0 / n1 * (val - surfv) - dn;
-    const double aval = dn / n1 - (val - surfv) / (n1 * n1);
-    out = aval * (sur2 * sur2) + bval * sur2 + cval;
+    return aval * (sur2 * sur2) + bval * sur2 + cval;
   }
+  const double cval = surfv;
+  const double bval = 2.0 / n1 * (val - surfv) - dn;
+  const double aval = dn / n1 - (val - surfv) / (n1 * n1);
+  return aval * (sur2 * sur2) + bval * sur2 + cval;
+}
+
+// one reconstruction; returns false when an ordered read is not available yet
+__device__ bool run_task(const FArgs &a, int routine, int64_t cell_ord, int part, const Reader &rd_in,
+                         double &out) {
+  TaskGeom t;
+  task_geom(a, routine, cell_ord, part, t);
+  bool blocked = false;
+  Reader rd = rd_in;
+  rd.comp = t.s.comp;
+  rd.blocked = &blocked;
+  double val, dv[3];
+  value_derivs(rd, t.px, t.py, t.pz, t.ix, t.iy, t.iz, t.gx, t.gy, t.gz, t.s.comp, val, dv);
+  if (blocked) return false;
+  double surfv;
+  if (a.tangent) {
+    double g[BK_N];
+    surface_term_grad(a, routine, t.s.comp, t.e, g);
+    surfv = 0.0;
+    for (int q = 0; q < BK_N; ++q) surfv += g[q] * a.kin_dot[q];
+  } else {
+    surfv = surface_term(a, routine, part, t.s.comp, t.e);
+  }
+  out = task_value(t, val, dv, surfv);
   return true;
 }
 
@@ -364,21 +445,22 @@ __global__ void k_force_direct(FArgs a, int routine, int nparts, double *f0, dou
   }
 }
 
-static int force(int id, int routine, const double *kin) {
-  Block *b = get_block(id);
-  if (!b) return -1;
+// The arguments of one forcing routine on block b (fields: the block's
+// current field pointers).  Fails for a stale IB list; n_cells 0 = nothing
+// to do.
+static int forcing_args(Block *b, int routine, const double *kin, const double *kin_dot, FArgs &a, int &nparts,
+                        int64_t &n_cells) {
   IbData &ib = b->ib;
-  if (!ib.present) { set_error("block %d has no immersed body", id); return -1; }
+  if (!ib.present) { set_error("block has no immersed body"); return -1; }
   const bool ts = (routine == R_VG || routine == R_PG);
-  const int64_t n_cells = ts ? b->counts[BC_TS] : b->counts[BC_IB];
+  n_cells = ts ? b->counts[BC_TS] : b->counts[BC_IB];
   if (n_cells == 0) return 0;
   if (!ts && ib.n_mapped != n_cells) {
     set_error("forcing: IB list is stale (cell_count_solid after retagging is required)");
     return -1;
   }
-  const int nparts = (routine == R_P1 || routine == R_PF || routine == R_PG) ? 1 : 6;
-  hipStream_t s = rt().stream;
-  FArgs a{};
+  nparts = (routine == R_P1 || routine == R_PF || routine == R_PG) ? 1 : 6;
+  a = FArgs{};
   a.d = b->d;
   for (int q = 0; q < BG_NGRID; ++q) {
     if (!b->g[q]) { set_error("grid %d not uploaded", q); return -1; }
@@ -392,15 +474,68 @@ static int force(int id, int routine, const double *kin) {
   for (int q = 0; q < BE_NELEM; ++q) a.el[q] = ib.elem[q];
   a.surf = ib.surf;
   for (int q = 0; q < BK_N; ++q) a.kin[q] = kin[q];
+  a.tangent = kin_dot != nullptr;
+  for (int q = 0; q < BK_N; ++q) a.kin_dot[q] = kin_dot ? kin_dot[q] : 0.0;
   a.ib_map = ib.ib_map;
   a.fld[0] = b->f[BF_P]; a.fld[1] = b->f[BF_UT]; a.fld[2] = b->f[BF_VT]; a.fld[3] = b->f[BF_WT];
   a.plain[0] = b->f[BF_P]; a.plain[1] = b->f[BF_U]; a.plain[2] = b->f[BF_V]; a.plain[3] = b->f[BF_W];
   const int64_t ntask = n_cells * nparts;
   if (grow(&ib.task_out, &ib.cap_task, ntask + 2)) return -1;
   if (grow(&ib.task_done, &ib.cap_done, ntask + 2)) return -1;
-  int *counters = (int *)(ib.task_out + ib.cap_task - 1);  // 2 ints in the last slot
-  a.miss = counters;
-  BCFD_HIP(hipMemsetAsync(counters, 0, 2 * sizeof(int), s));
+  a.miss = (int *)(ib.task_out + ib.cap_task - 1);  // 2 ints in the last slot
+  BCFD_HIP(hipMemsetAsync(a.miss, 0, 2 * sizeof(int), rt().stream));
+  return 0;
+}
+
+// The dependency waves of an in-place routine: task_done[t] = the wave that
+// computed task t (1, 2, ...), task_out its value; `levels` the last wave.
+static int force_waves(Block *b, const FArgs &a, int routine, int nparts, int &levels) {
+  IbData &ib = b->ib;
+  hipStream_t s = rt().stream;
+  const int64_t ntask = a.n_cells * nparts;
+  const unsigned g = div_up(ntask, 256);
+  int *counters = a.miss;
+  hipLaunchKernelGGL(k_force_init, dim3(g), dim3(256), 0, s, ib.task_done, ntask);
+  int *h = (int *)rt().h_pinned_i;
+  int pass = 1;
+  for (;; ++pass) {
+    BCFD_HIP(hipMemsetAsync(counters + 1, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_force_pass, dim3(g), dim3(256), 0, s, a, routine, nparts, ib.task_out, ib.task_done,
+                       pass, counters + 1);
+    BCFD_CHECK_LAUNCH();
+    BCFD_HIP(hipMemcpyAsync(h, counters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+    BCFD_HIP(hipStreamSynchronize(s));
+    if (h[1] == 0) break;
+    if (pass > ntask + 1) { set_error("forcing: dependency waves did not finish"); return -1; }
+  }
+  levels = pass;
+  return 0;
+}
+
+static int force_misses(const FArgs &a) {
+  int *h = (int *)rt().h_pinned_i;
+  BCFD_HIP(hipMemcpyAsync(h, a.miss, sizeof(int), hipMemcpyDeviceToHost, rt().stream));
+  BCFD_HIP(hipStreamSynchronize(rt().stream));
+  if (h[0] != 0) {
+    set_error("forcing: %d image-point lookups fell outside their search window", h[0]);
+    return -2;
+  }
+  return 0;
+}
+
+// Runs one forcing routine on block b.  kin_dot: null for the routine
+// itself; otherwise the tangent-linear pass, with the block's field pointers
+// set to the tangents by the caller.
+static int force_block(Block *b, int routine, const double *kin, const double *kin_dot) {
+  IbData &ib = b->ib;
+  FArgs a;
+  int nparts = 1;
+  int64_t n_cells = 0;
+  if (forcing_args(b, routine, kin, kin_dot, a, nparts, n_cells)) return -1;
+  if (n_cells == 0) return 0;
+  const bool ts = (routine == R_VG || routine == R_PG);
+  hipStream_t s = rt().stream;
+  const int64_t ntask = n_cells * nparts;
   const unsigned g = div_up(ntask, 256);
   if (routine == R_VF || ts) {
     double **gh = nullptr;
@@ -414,32 +549,21 @@ static int force(int id, int routine, const double *kin) {
     BCFD_HIP(hipStreamSynchronize(s));
     if (gh) (void)hipFree(gh);
   } else {
-    hipLaunchKernelGGL(k_force_init, dim3(g), dim3(256), 0, s, ib.task_done, ntask);
-    int *h = (int *)rt().h_pinned_i;
-    int pass = 1;
-    for (;; ++pass) {
-      BCFD_HIP(hipMemsetAsync(counters + 1, 0, sizeof(int), s));
-      hipLaunchKernelGGL(k_force_pass, dim3(g), dim3(256), 0, s, a, routine, nparts, ib.task_out, ib.task_done,
-                         pass, counters + 1);
-      BCFD_CHECK_LAUNCH();
-      BCFD_HIP(hipMemcpyAsync(h, counters, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
-      BCFD_HIP(hipStreamSynchronize(s));
-      if (h[1] == 0) break;
-      if (pass > ntask + 1) { set_error("forcing: dependency waves did not finish"); return -1; }
-    }
+    int levels = 0;
+    if (force_waves(b, a, routine, nparts, levels)) return -1;
     double *f0 = b->f[BF_P];
     double *f1 = b->f[BF_UT], *f2 = b->f[BF_VT], *f3 = b->f[BF_WT];
     hipLaunchKernelGGL(k_force_scatter, dim3(g), dim3(256), 0, s, a, routine, nparts, ib.task_out, f0, f1, f2, f3);
     BCFD_CHECK_LAUNCH();
   }
-  int *h = (int *)rt().h_pinned_i;
-  BCFD_HIP(hipMemcpyAsync(h, counters, sizeof(int), hipMemcpyDeviceToHost, s));
-  BCFD_HIP(hipStreamSynchronize(s));
-  if (h[0] != 0) {
-    set_error("forcing: %d image-point lookups fell outside their search window", h[0]);
-    return -2;
-  }
-  return 0;
+  return force_misses(a);
+}
+
+static int force(int id, int routine, const double *kin) {
+  Block *b = get_block(id);
+  if (!b) return -1;
+  if (!b->ib.present) { set_error("block %d has no immersed body", id); return -1; }
+  return force_block(b, routine, kin, nullptr);
 }
 
 // ---------------- stress_calculation (StressCalc.f90:14-208) ----------------
@@ -610,6 +734,466 @@ int stress_scratch_init(int device) {
     return -1;
   }
   g_scratch_ptr[device] = (double *)p;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Tangent-linear and adjoint passes of the forcing routines and of the
+// body-force sums (the adjoint time step; no counterpart in the reference).
+// Geometry frozen: IB cells, nearest elements, distances, normals and
+// image-point cubes are the forward's.  A reconstruction is then linear in
+// the eight values of its cube and in its surface term, and the surface
+// term is a function of the kinematic vector (surface_term_grad).
+// ---------------------------------------------------------------------------
+
+// 1 at one position, 0 elsewhere: value_derivs through it gives the
+// image-point value and derivatives per unit value at that cube corner
+struct UnitReader {
+  int ti, tj, tk;
+  __device__ double operator()(int ii, int jj, int kk) const {
+    return (ii == ti && jj == tj && kk == tk) ? 1.0 : 0.0;
+  }
+};
+
+// corner m (bit 0: i + 1, bit 1: j + 1, bit 2: k + 1) of the cube
+// value_derivs reads for component comp at cube index (i, j, k)
+__device__ inline void cube_corner(int comp, int i, int j, int k, int m, int &ii, int &jj, int &kk) {
+  ii = i - (comp == 1) + (m & 1);
+  jj = j - (comp == 2) + ((m >> 1) & 1);
+  kk = k - (comp == 3) + ((m >> 2) & 1);
+}
+
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// adds c[q] summed over the wavefront into acc[q] (one atomic per entry
+// and wavefront; every lane of the wavefront calls it)
+__device__ inline void wave_add_kin(const double c[BK_N], double *acc) {
+  const int lane = threadIdx.x & 63;
+  for (int q = 0; q < BK_N; ++q) {
+    const double v = wave_sum(c[q]);
+    if (lane == 0 && v != 0.0) atomicAdd(&acc[q], v);
+  }
+}
+
+// location written by task t and whether t is its last writer in serial
+// order (the opposite face of the neighbouring IB cell writes the same
+// location for the velocity routines, as k_force_scatter decides)
+__device__ inline int64_t task_target(const FArgs &a, int routine, int nparts, int64_t t, int &comp, bool &last) {
+  const int64_t n = t / nparts;
+  const int part = (int)(t % nparts);
+  const PartSpec s = part_spec(routine, part);
+  const int i = a.ijk[n] + s.tx, j = a.ijk[n + a.cap] + s.ty, k = a.ijk[n + 2 * a.cap] + s.tz;
+  const Dims &d = a.d;
+  comp = s.comp;
+  last = true;
+  if (routine == R_V1 || routine == R_VF) {
+    const int si = (s.comp == 1), sj = (s.comp == 2), sk = (s.comp == 3);
+    long long other = -1;
+    if ((part % 2) == 0) {
+      const int ni = i + si, nj = j + sj, nk = k + sk;
+      if (ni <= d.NX && nj <= d.NY && nk <= d.NZ) {
+        const int m = a.ib_map[ix3(d, ni, nj, nk)];
+        if (m >= 0) other = 6LL * m + part + 1;
+      }
+    } else {
+      const int m = a.ib_map[ix3(d, i, j, k)];
+      if (m >= 0) other = 6LL * m + part - 1;
+    }
+    last = !(other > t);
+  }
+  return ix3(d, i, j, k);
+}
+
+// ybar[t] = adjoint of the written location for its last writer, 0 for the
+// others (their values were overwritten)
+struct Lam4 {
+  double *f[4];   // p, u, v, w components
+};
+
+__global__ void k_force_adj_init(FArgs a, int routine, int nparts, double *ybar, Lam4 lam_out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_cells * nparts) return;
+  int comp;
+  bool last;
+  const int64_t loc = task_target(a, routine, nparts, t, comp, last);
+  ybar[t] = last ? lam_out.f[comp][loc] : 0.0;
+}
+
+__global__ void k_force_adj_clear(FArgs a, int routine, int nparts, Lam4 lam_out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_cells * nparts) return;
+  int comp;
+  bool last;
+  const int64_t loc = task_target(a, routine, nparts, t, comp, last);
+  lam_out.f[comp][loc] = 0.0;
+}
+
+// The tasks of one dependency wave (level; 0: all tasks, for the routines
+// without in-place reads), last wave first: each passes ybar[t] times its
+// cube weights to the earlier task that wrote a cube position (in-place
+// routines) or to the adjoint of the array read, and times its surface
+// weight through the surface term's gradient into kin_bar.
+__global__ void k_force_adj(FArgs a, int routine, int nparts, int level, const int *task_done, double *ybar,
+                            Lam4 lam_in, double *kin_bar) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double kc[BK_N];
+  for (int q = 0; q < BK_N; ++q) kc[q] = 0.0;
+  const bool active = t < a.n_cells * nparts && (level == 0 || task_done[t] == level);
+  const double yb = active ? ybar[t] : 0.0;
+  if (active && yb != 0.0) {
+    TaskGeom g;
+    task_geom(a, routine, t / nparts, (int)(t % nparts), g);
+    const double s2 = g.sur2, n1 = g.n1;
+    double dval, ddn, dsurf;
+    if (g.s.comp == 0) {
+      dval = 1.0;
+      ddn = s2 * s2 / (2.0 * n1) - 0.5 * n1;
+      dsurf = -(s2 * s2) / (2.0 * n1) + s2 - 0.5 * n1;
+    } else {
+      dval = -(s2 * s2) / (n1 * n1) + 2.0 * s2 / n1;
+      ddn = s2 * s2 / n1 - s2;
+      dsurf = s2 * s2 / (n1 * n1) - 2.0 * s2 / n1 + 1.0;
+    }
+    if (g.ghost) ddn = -ddn;
+    const bool ordered = (routine == R_V1 || routine == R_P1 || routine == R_PF);
+    const Reader rd{&a, g.s.comp, true, routine, (long long)t, nullptr, nullptr, 0, nullptr};
+    for (int m = 0; m < 8; ++m) {
+      int ii, jj, kk;
+      cube_corner(g.s.comp, g.ix, g.iy, g.iz, m, ii, jj, kk);
+      const UnitReader ur{ii, jj, kk};
+      double val, dv[3];
+      value_derivs(ur, g.px, g.py, g.pz, g.ix, g.iy, g.iz, g.gx, g.gy, g.gz, g.s.comp, val, dv);
+      const double w = dval * val + ddn * (dv[0] * g.ca + dv[1] * g.cb + dv[2] * g.cg);
+      if (w == 0.0) continue;
+      const int64_t loc = ix3(a.d, ii, jj, kk);
+      const long long wr = ordered ? rd.writer_before(loc, ii, jj, kk) : -1;
+      if (wr >= 0) atomicAdd(&ybar[wr], w * yb);
+      else atomicAdd(&lam_in.f[g.s.comp][loc], w * yb);
+    }
+    double gs[BK_N];
+    surface_term_grad(a, routine, g.s.comp, g.e, gs);
+    for (int q = 0; q < BK_N; ++q) kc[q] = dsurf * yb * gs[q];
+  }
+  wave_add_kin(kc, kin_bar);
+}
+
+// the ghost routines' adjoint seeds: ybar[t] = adjoint of the reconstructed
+// ghost; the adjoint of the copied ("t") ghost goes to the copied field
+// entry; both ghost adjoints are consumed
+struct Gh14 {
+  double *g[BGH_N];
+};
+
+__global__ void k_force_adj_ghost(FArgs a, int routine, int nparts, double *ybar, Lam4 lam_plain, Gh14 gh) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_cells * nparts) return;
+  const int64_t n = t / nparts;
+  const int part = (int)(t % nparts);
+  const PartSpec s = part_spec(routine, part);
+  const int i = a.ijk[n] + s.tx, j = a.ijk[n + a.cap] + s.ty, k = a.ijk[n + 2 * a.cap] + s.tz;
+  ybar[t] = gh.g[s.ghost][n];
+  gh.g[s.ghost][n] = 0.0;
+  const double c = gh.g[s.ghost_t][n];
+  gh.g[s.ghost_t][n] = 0.0;
+  if (c != 0.0) atomicAdd(&lam_plain.f[s.comp][ix3(a.d, i, j, k)], c);
+}
+
+int forcing_tlm(Block &b, int routine, const double kin[BK_N], const double kin_dot[BK_N]) {
+  SwapFields sw(b);
+  if (routine != R_VG && routine != R_PG) return force_block(&b, routine, kin, kin_dot);
+  // the ghost routines write the ghost arrays: their tangents
+  if (ad_ghosts(b)) return -1;
+  for (int q = 0; q < BGH_N; ++q) std::swap(b.ib.ghost[q], b.ad.gh[q]);
+  const int rc = force_block(&b, routine, kin, kin_dot);
+  for (int q = 0; q < BGH_N; ++q) std::swap(b.ib.ghost[q], b.ad.gh[q]);
+  return rc;
+}
+
+int forcing_adj(Block &b, int routine, const double kin[BK_N]) {
+  FArgs a;
+  int nparts = 1;
+  int64_t n_cells = 0;
+  if (forcing_args(&b, routine, kin, nullptr, a, nparts, n_cells)) return -1;
+  if (n_cells == 0) return 0;
+  IbData &ib = b.ib;
+  hipStream_t s = rt().stream;
+  const int64_t ntask = n_cells * nparts;
+  const unsigned g = div_up(ntask, 256);
+  int levels = 0;
+  const bool ordered = (routine == R_V1 || routine == R_P1 || routine == R_PF);
+  if (ordered && force_waves(&b, a, routine, nparts, levels)) return -1;
+  if (grow(&b.ad.ybar, &b.ad.cap_ybar, ntask)) return -1;
+  // adjoints of the written arrays and of the arrays the cubes read
+  Lam4 out{{b.ad.s[BF_P], b.ad.s[BF_UT], b.ad.s[BF_VT], b.ad.s[BF_WT]}};
+  Lam4 in = out;
+  if (routine == R_VF) { out.f[1] = b.ad.s[BF_U]; out.f[2] = b.ad.s[BF_V]; out.f[3] = b.ad.s[BF_W]; }
+  if (routine == R_VG || routine == R_PG) {
+    if (ad_ghosts(b)) return -1;
+    Gh14 gh;
+    for (int q = 0; q < BGH_N; ++q) gh.g[q] = b.ad.gh[q];
+    const Lam4 plain{{b.ad.s[BF_P], b.ad.s[BF_U], b.ad.s[BF_V], b.ad.s[BF_W]}};
+    hipLaunchKernelGGL(k_force_adj_ghost, dim3(g), dim3(256), 0, s, a, routine, nparts, b.ad.ybar, plain, gh);
+  } else {
+    hipLaunchKernelGGL(k_force_adj_init, dim3(g), dim3(256), 0, s, a, routine, nparts, b.ad.ybar, out);
+    hipLaunchKernelGGL(k_force_adj_clear, dim3(g), dim3(256), 0, s, a, routine, nparts, out);
+  }
+  if (routine == R_VF || routine == R_VG || routine == R_PG) {
+    hipLaunchKernelGGL(k_force_adj, dim3(g), dim3(256), 0, s, a, routine, nparts, 0, ib.task_done, b.ad.ybar, in,
+                       b.ad.kin);
+  } else {
+    for (int level = levels; level >= 1; --level)
+      hipLaunchKernelGGL(k_force_adj, dim3(g), dim3(256), 0, s, a, routine, nparts, level, ib.task_done, b.ad.ybar,
+                         in, b.ad.kin);
+  }
+  BCFD_CHECK_LAUNCH();
+  return force_misses(a);
+}
+
+// ---- body-force sums: tangent-linear and adjoint ----
+// S = (viscous x, pressure x, viscous y, pressure y) summed over elements,
+// the quantities stress_calculation divides by its projected areas.
+
+// surface velocities and dp/dn of element e and their gradients with respect
+// to the kinematic vector (StressCalc.f90:103-125)
+__device__ inline void stress_surface_grad(const FArgs &a, int e, double gs[3][BK_N], double gd[BK_N]) {
+  for (int q = 0; q < BK_N; ++q) { gs[0][q] = gs[1][q] = gs[2][q] = gd[q] = 0.0; }
+  const int sid = a.surf[e];
+  const double *k = a.kin;
+  const double n1 = a.el[BE_COSB][e], n2 = a.el[BE_COSG][e];
+  const double yc = a.el[BE_YCENT][e], zc = a.el[BE_ZCENT][e];
+  const double ry = yc - k[BK_PIVY], rz = zc - k[BK_PIVZ];
+  gs[0][BK_XDOT] = 1.0;
+  gs[1][BK_YDOT] = 1.0;
+  gd[BK_YDDOT] = -n1;
+  if (sid == 51 || sid == 52) {
+    const int std_ = (sid == 51) ? BK_THETADOT1 : BK_THETADOT2;
+    const int stdd = (sid == 51) ? BK_THETADDOT1 : BK_THETADDOT2;
+    const double td = k[std_], tdd = k[stdd];
+    gs[1][std_] = -rz;
+    gs[1][BK_PIVZ] = td;
+    gs[2][std_] = ry;
+    gs[2][BK_PIVY] = -td;
+    // dpdn = -((ac_z + at_z) n2 + (ac_y + at_y) n1) - yddot n1
+    gd[std_] = 2.0 * td * (rz * n2 + ry * n1);
+    gd[stdd] = -(ry * n2 - rz * n1);
+    gd[BK_PIVZ] = -(td * td * n2 + tdd * n1);
+    gd[BK_PIVY] = tdd * n2 - td * td * n1;
+  }
+}
+
+// the image point of element e and the cube of each component there
+struct StressGeom {
+  double nrm[3], px, py, pz, diag, area;
+  int cube[4][3];
+};
+
+__device__ inline void stress_geom(const FArgs &a, int64_t e, StressGeom &g) {
+  const Dims &d = a.d;
+  g.nrm[0] = a.el[BE_COSA][e]; g.nrm[1] = a.el[BE_COSB][e]; g.nrm[2] = a.el[BE_COSG][e];
+  const double xc = a.el[BE_XCENT][e], yc = a.el[BE_YCENT][e], zc = a.el[BE_ZCENT][e];
+  const int ic = find_index(xc, a.g[BG_X1], d.nx + 3, 2, d.nx + 1, a.miss);
+  const int jc = find_index(yc, a.g[BG_Y1], d.ny + 3, 2, d.ny + 1, a.miss);
+  const int kc = find_index(zc, a.g[BG_Z1], d.nz + 3, 2, d.nz + 1, a.miss);
+  const double dX = a.g[BG_X1][ic] - a.g[BG_X1][ic - 1];
+  const double dY = a.g[BG_Y1][jc] - a.g[BG_Y1][jc - 1];
+  const double dZ = a.g[BG_Z1][kc] - a.g[BG_Z1][kc - 1];
+  g.diag = sqrt(dX * dX + dY * dY + dZ * dZ);
+  g.px = xc + g.diag * g.nrm[0]; g.py = yc + g.diag * g.nrm[1]; g.pz = zc + g.diag * g.nrm[2];
+  g.area = a.el[BE_ELEN][e] / 2.0;
+  for (int c = 0; c < 4; ++c) {
+    const double *gx, *gy, *gz;
+    int lx, ly, lz;
+    comp_grids(a, c, gx, gy, gz, lx, ly, lz);
+    g.cube[c][0] = find_index(g.px, gx, lx, 2, d.nx + 1, a.miss);
+    g.cube[c][1] = find_index(g.py, gy, ly, 2, d.ny + 1, a.miss);
+    g.cube[c][2] = find_index(g.pz, gz, lz, 2, d.nz + 1, a.miss);
+  }
+}
+
+// The terms of the sums at element e, tangent-linear in the fields given
+// (the tangents) and in kin_dot: t = (viscous x, pressure x, viscous y,
+// pressure y) (StressCalc.f90:75-185 differentiated at frozen geometry)
+__global__ void k_stress_tlm_terms(FArgs a, double mu_f, double rho_f, double *terms) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n_cells) return;
+  StressGeom g;
+  stress_geom(a, e, g);
+  double gs[3][BK_N], gd[BK_N];
+  stress_surface_grad(a, (int)e, gs, gd);
+  double surf[3] = {0.0, 0.0, 0.0}, dpdn = 0.0;
+  for (int q = 0; q < BK_N; ++q) {
+    surf[0] += gs[0][q] * a.kin_dot[q]; surf[1] += gs[1][q] * a.kin_dot[q]; surf[2] += gs[2][q] * a.kin_dot[q];
+    dpdn += gd[q] * a.kin_dot[q];
+  }
+  double ddn[3];
+  for (int c = 0; c < 3; ++c) {
+    const double *gx, *gy, *gz;
+    int lx, ly, lz;
+    comp_grids(a, c + 1, gx, gy, gz, lx, ly, lz);
+    Reader rd{&a, c + 1, false, 0, 0, nullptr, nullptr, 0, nullptr};
+    double val, dv[3];
+    value_derivs(rd, g.px, g.py, g.pz, g.cube[c + 1][0], g.cube[c + 1][1], g.cube[c + 1][2], gx, gy, gz, c + 1,
+                 val, dv);
+    const double dn = dv[0] * g.nrm[0] + dv[1] * g.nrm[1] + dv[2] * g.nrm[2];
+    ddn[c] = (2.0 / g.diag) * (val - surf[c]) - dn;
+  }
+  const double dd = ddn[0] * g.nrm[0] + ddn[1] * g.nrm[1] + ddn[2] * g.nrm[2];
+  const double *gx, *gy, *gz;
+  int lx, ly, lz;
+  comp_grids(a, 0, gx, gy, gz, lx, ly, lz);
+  Reader rd{&a, 0, false, 0, 0, nullptr, nullptr, 0, nullptr};
+  double val, dv[3];
+  value_derivs(rd, g.px, g.py, g.pz, g.cube[0][0], g.cube[0][1], g.cube[0][2], gx, gy, gz, 0, val, dv);
+  const double dpdn_e = dv[0] * g.nrm[0] + dv[1] * g.nrm[1] + dv[2] * g.nrm[2];
+  const double cval = val - (dpdn_e + dpdn) * g.diag * 0.5;
+  double *t = terms + 4 * e;
+  t[0] = (ddn[0] - dd * g.nrm[0]) * mu_f * g.area;
+  t[1] = -cval * g.area * g.nrm[0] * rho_f;
+  t[2] = (ddn[1] - dd * g.nrm[1]) * mu_f * g.area;
+  t[3] = -cval * g.area * g.nrm[1] * rho_f;
+}
+
+// four sums over the elements (one workgroup, a fixed order)
+__global__ void __launch_bounds__(256) k_sum4(const double *__restrict__ terms, int64_t ne, int stride,
+                                              double *__restrict__ out) {
+  __shared__ double part[4][256];
+  const int t = threadIdx.x;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int64_t e = t; e < ne; e += 256)
+    for (int q = 0; q < 4; ++q) acc[q] += terms[stride * e + q];
+  for (int q = 0; q < 4; ++q) part[q][t] = acc[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+      for (int q = 0; q < 4; ++q) part[q][t] += part[q][t + w];
+    __syncthreads();
+  }
+  if (t < 4) out[t] = part[t][0];
+}
+
+// sbar (the adjoints of the four sums) into the adjoints of u, v, w, p at
+// the image-point cubes and into kin_bar
+struct Sum4 {
+  double v[4];
+};
+
+__global__ void k_stress_adj(FArgs a, double mu_f, double rho_f, Sum4 sbar, double *lu, double *lv, double *lw,
+                             double *lp, double *kin_bar) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double kc[BK_N];
+  for (int q = 0; q < BK_N; ++q) kc[q] = 0.0;
+  if (e < a.n_cells) {
+    const double *sb = sbar.v;
+    StressGeom g;
+    stress_geom(a, e, g);
+    const double *n = g.nrm;
+    // t0 = (ddn0 - dd n0) mu A, t2 = (ddn1 - dd n1) mu A, dd = sum ddn_c n_c
+    double bddn[3];
+    for (int c = 0; c < 3; ++c)
+      bddn[c] = mu_f * g.area * (sb[0] * ((c == 0 ? 1.0 : 0.0) - n[c] * n[0]) +
+                                 sb[2] * ((c == 1 ? 1.0 : 0.0) - n[c] * n[1]));
+    // t1 = -cval A n0 rho, t3 = -cval A n1 rho
+    const double bcval = -g.area * rho_f * (sb[1] * n[0] + sb[3] * n[1]);
+    double *lam[4] = {lp, lu, lv, lw};
+    for (int c = 0; c < 4; ++c) {
+      // ddn_c = (2/diag)(val - surf_c) - dn ; cval = val - (dn + dpdn) diag/2
+      const double bval = c == 0 ? bcval : (2.0 / g.diag) * bddn[c - 1];
+      const double bdn = c == 0 ? -0.5 * g.diag * bcval : -bddn[c - 1];
+      const double *gx, *gy, *gz;
+      int lx, ly, lz;
+      comp_grids(a, c, gx, gy, gz, lx, ly, lz);
+      for (int m = 0; m < 8; ++m) {
+        int ii, jj, kk;
+        cube_corner(c, g.cube[c][0], g.cube[c][1], g.cube[c][2], m, ii, jj, kk);
+        const UnitReader ur{ii, jj, kk};
+        double val, dv[3];
+        value_derivs(ur, g.px, g.py, g.pz, g.cube[c][0], g.cube[c][1], g.cube[c][2], gx, gy, gz, c, val, dv);
+        const double w = bval * val + bdn * (dv[0] * n[0] + dv[1] * n[1] + dv[2] * n[2]);
+        if (w != 0.0) atomicAdd(&lam[c][ix3(a.d, ii, jj, kk)], w);
+      }
+    }
+    double gs[3][BK_N], gd[BK_N];
+    stress_surface_grad(a, (int)e, gs, gd);
+    const double bdpdn = -0.5 * g.diag * bcval;
+    for (int q = 0; q < BK_N; ++q)
+      kc[q] = -(2.0 / g.diag) * (bddn[0] * gs[0][q] + bddn[1] * gs[1][q] + bddn[2] * gs[2][q]) + bdpdn * gd[q];
+  }
+  wave_add_kin(kc, kin_bar);
+}
+
+static int stress_args(Block &b, const double kin[BK_N], const double *kin_dot, FArgs &a) {
+  IbData &ib = b.ib;
+  if (!ib.present) { set_error("block has no immersed body"); return -1; }
+  a = FArgs{};
+  a.d = b.d;
+  for (int q = 0; q < BG_NGRID; ++q) a.g[q] = b.g[q];
+  a.n_cells = ib.n_elems;
+  for (int q = 0; q < BE_NELEM; ++q) a.el[q] = ib.elem[q];
+  a.surf = ib.surf;
+  for (int q = 0; q < BK_N; ++q) { a.kin[q] = kin[q]; a.kin_dot[q] = kin_dot ? kin_dot[q] : 0.0; }
+  a.tangent = kin_dot != nullptr;
+  a.fld[0] = b.f[BF_P]; a.fld[1] = b.f[BF_U]; a.fld[2] = b.f[BF_V]; a.fld[3] = b.f[BF_W];
+  if (grow(&ib.stress_work, &ib.cap_stress, 6 * ib.n_elems + 8)) return -1;
+  a.miss = (int *)(ib.stress_work + 6 * ib.n_elems + 4);
+  hipLaunchKernelGGL(k_stress_clear, dim3(1), dim3(64), 0, rt().stream, a.miss);
+  return 0;
+}
+
+static int stress_misses(const FArgs &a) {
+  int *h = (int *)rt().h_pinned_i;
+  BCFD_HIP(hipMemcpyAsync(h, a.miss, sizeof(int), hipMemcpyDeviceToHost, rt().stream));
+  BCFD_HIP(hipStreamSynchronize(rt().stream));
+  if (h[0] != 0) { set_error("stress: %d probe lookups missed", h[0]); return -2; }
+  return 0;
+}
+
+int stress_tlm(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
+               double out[4]) {
+  FArgs a;
+  {
+    SwapFields sw(b);
+    if (stress_args(b, kin, kin_dot, a)) return -1;
+  }
+  IbData &ib = b.ib;
+  double *terms = ib.stress_work, *res = ib.stress_work + 6 * ib.n_elems;
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_stress_tlm_terms, dim3(div_up(ib.n_elems, 256)), dim3(256), 0, s, a, mu_f, rho_f, terms);
+  hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, s, terms, ib.n_elems, 4, res);
+  BCFD_CHECK_LAUNCH();
+  BCFD_HIP(hipMemcpyAsync(rt().h_pinned, res, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (stress_misses(a)) return -2;
+  for (int q = 0; q < 4; ++q) out[q] = rt().h_pinned[q];
+  return 0;
+}
+
+int stress_adj(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4]) {
+  FArgs a;
+  if (stress_args(b, kin, nullptr, a)) return -1;
+  const Sum4 sb{{sbar[0], sbar[1], sbar[2], sbar[3]}};
+  double **l = b.ad.s;
+  hipLaunchKernelGGL(k_stress_adj, dim3(div_up(b.ib.n_elems, 256)), dim3(256), 0, rt().stream, a, mu_f, rho_f, sb,
+                     l[BF_U], l[BF_V], l[BF_W], l[BF_P], b.ad.kin);
+  BCFD_CHECK_LAUNCH();
+  return stress_misses(a);
+}
+
+// the four sums of the forward (a fixed order; the adjoint workload's
+// objective reads their x parts)
+int stress_sums(Block &b, double mu_f, double rho_f, const double kin[BK_N], double out[4]) {
+  FArgs a;
+  if (stress_args(b, kin, nullptr, a)) return -1;
+  IbData &ib = b.ib;
+  double *terms = ib.stress_work, *res = ib.stress_work + 6 * ib.n_elems;
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_stress_terms, dim3(div_up(ib.n_elems, 256)), dim3(256), 0, s, a, mu_f, rho_f, terms);
+  hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, s, terms, ib.n_elems, 6, res);
+  BCFD_CHECK_LAUNCH();
+  BCFD_HIP(hipMemcpyAsync(rt().h_pinned, res, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (stress_misses(a)) return -2;
+  for (int q = 0; q < 4; ++q) out[q] = rt().h_pinned[q];
   return 0;
 }
 

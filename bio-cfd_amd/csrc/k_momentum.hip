@@ -13,6 +13,7 @@
 // by one in list order, by a single workgroup; every other fluid cell reads
 // only untouched values and runs in the dense parallel kernel.
 #include <cmath>
+#include <type_traits>
 
 #include "biocfd_rt.hpp"
 
@@ -38,9 +39,8 @@ struct Fields {
 typedef const double __attribute__((address_space(4))) *cdp;
 
 // (a, c: the table rows of the stencil's column, ca/ck + 6 (col - 1))
-template <class TA, class TC>
-__device__ inline double upwind3r(double vel, TA a, TC c, double f2, double f1, double f0, double fm1, double fm2,
-                                  double hA, double hB) {
+template <class T, class TA, class TC>
+__device__ inline T upwind3r(T vel, TA a, TC c, T f2, T f1, T f0, T fm1, T fm2, double hA, double hB) {
   return vel * (a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2) / (a[5] * hA) +
          fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2) / (2.0 * c[5] * hB);
 }
@@ -100,6 +100,28 @@ __device__ inline double gdiv(double x, double y) {
   }
 }
 
+// A value and its directional derivative (forward-mode differentiation of
+// the stencil: the tangent-linear and adjoint passes below evaluate the same
+// component functions on these).  |x| takes the derivative sign(x), and 0 at
+// x = 0 (the value central differences give there: walls hold v or w at 0).
+struct Dual {
+  double v, t;
+};
+__device__ inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.t + b.t}; }
+__device__ inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.t - b.t}; }
+__device__ inline Dual operator-(Dual a) { return {-a.v, -a.t}; }
+__device__ inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.v * b.t + a.t * b.v}; }
+__device__ inline Dual operator*(double s, Dual a) { return {s * a.v, s * a.t}; }
+__device__ inline Dual operator/(Dual a, double s) { return {a.v / s, a.t / s}; }
+__device__ inline Dual fabs(Dual a) {
+  return a.v < 0.0 ? Dual{-a.v, -a.t} : (a.v > 0.0 ? a : Dual{0.0, 0.0});
+}
+using ::fabs;  // the double overload stays visible beside the Dual one
+template <bool P2>
+__device__ inline Dual gdiv(Dual x, double y) {
+  return {gdiv<P2>(x.v, y), gdiv<P2>(x.t, y)};
+}
+
 // Stencil operands straight from global memory: the cell's linear offset and
 // strides, and the upwind table rows of its column in each direction.
 struct GAcc {
@@ -134,28 +156,29 @@ struct GAcc {
 
 // ut (NavierStokes.f90:600-688)
 template <bool P2, class Acc>
-__device__ inline double mom_u(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+__device__ inline auto mom_u(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
                                bool near_ok) {
+  using T = std::decay_t<decltype(A.template u<0, 0, 0>())>;
   const double alpha = 1.0;
   const double dxr = m.dxr, dx = m.dx, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
-  const double u1a = U(-1, 0, 0) + U(0, 0, 0), u22 = U(-1, 0, 0) - U(0, 0, 0);
-  const double u3 = U(0, 0, 0) + U(1, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
-  const double u5 = U(0, -1, 0) + U(0, 0, 0), u6 = U(0, -1, 0) - U(0, 0, 0);
-  const double u7 = U(0, 0, 0) + U(0, 1, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
-  const double u9 = U(0, 0, -1) + U(0, 0, 0), u10 = U(0, 0, -1) - U(0, 0, 0);
-  const double u11 = U(0, 0, 0) + U(0, 0, 1), u12 = U(0, 0, 0) - U(0, 0, 1);
-  const double v1a = V(0, -1, 0) + V(1, -1, 0), v22 = V(0, 0, 0) + V(1, 0, 0);
-  const double w1a = W(0, 0, -1) + W(1, 0, -1), w22 = W(0, 0, 0) + W(1, 0, 0);
-  const double dpdx = gdiv<P2>(P(0, 0, 0) - P(1, 0, 0), 0.5 * (dxr + dx));
+  const T u1a = U(-1, 0, 0) + U(0, 0, 0), u22 = U(-1, 0, 0) - U(0, 0, 0);
+  const T u3 = U(0, 0, 0) + U(1, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
+  const T u5 = U(0, -1, 0) + U(0, 0, 0), u6 = U(0, -1, 0) - U(0, 0, 0);
+  const T u7 = U(0, 0, 0) + U(0, 1, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
+  const T u9 = U(0, 0, -1) + U(0, 0, 0), u10 = U(0, 0, -1) - U(0, 0, 0);
+  const T u11 = U(0, 0, 0) + U(0, 0, 1), u12 = U(0, 0, 0) - U(0, 0, 1);
+  const T v1a = V(0, -1, 0) + V(1, -1, 0), v22 = V(0, 0, 0) + V(1, 0, 0);
+  const T w1a = W(0, 0, -1) + W(1, 0, -1), w22 = W(0, 0, 0) + W(1, 0, 0);
+  const T dpdx = gdiv<P2>(P(0, 0, 0) - P(1, 0, 0), 0.5 * (dxr + dx));
   const double dx2xr = dx + dxr, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt, dz2zb = dz + dzb;
   const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
-  double duudx, dvudy, dwudz;
-  const double vu_e = V(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, 0, 0) - V(1, 0, 0));
-  const double vu_w = V(1, -1, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, -1, 0) - V(1, -1, 0));
-  const double v_in = 0.5 * (vu_e + vu_w);
-  const double wu_n = W(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, 0) - W(1, 0, 0));
-  const double wu_s = W(1, 0, -1) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, -1) - W(1, 0, -1));
-  const double w_in = 0.5 * (wu_n + wu_s);
+  T duudx, dvudy, dwudz;
+  const T vu_e = V(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, 0, 0) - V(1, 0, 0));
+  const T vu_w = V(1, -1, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, -1, 0) - V(1, -1, 0));
+  const T v_in = 0.5 * (vu_e + vu_w);
+  const T wu_n = W(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, 0) - W(1, 0, 0));
+  const T wu_s = W(1, 0, -1) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, -1) - W(1, 0, -1));
+  const T w_in = 0.5 * (wu_n + wu_s);
   if (i > 2 && i < nx && j > 2 && j < ny + 1 && k > 2 && k < nz + 1 && near_ok) {
     const double ddy = 0.5 * (A.template hy<-1>() + A.template hy<-2>()), ddye = 0.5 * (A.template hy<-1>() + A.template hy<0>());
     const double ddz = 0.5 * (A.template hz<-1>() + A.template hz<-2>()), ddzr = 0.5 * (A.template hz<-1>() + A.template hz<0>());
@@ -170,38 +193,39 @@ __device__ inline double mom_u(const MomArgs &a, const Acc &A, const MomCell &m,
     dvudy = gdiv<P2>(-0.25 * (v1a * u5 + alpha * fabs(v1a) * u6 - v22 * u7 - alpha * fabs(v22) * u8), A.template hy<-1>());
     dwudz = gdiv<P2>(-0.25 * (w1a * u9 + alpha * fabs(w1a) * u10 - w22 * u11 - alpha * fabs(w22) * u12), A.template hz<-1>());
   }
-  const double d2x = gdiv<P2>(2.0, dx2xr) * (gdiv<P2>(-u4, dxr) + gdiv<P2>(u22, dx));
-  const double d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-u8, 0.5 * dy2ye) + gdiv<P2>(u6, 0.5 * dy2yw));
-  const double d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-u12, 0.5 * dz2zt) + gdiv<P2>(u10, 0.5 * dz2zb));
-  const double visc = (d2x + d2y + d2z) / a.re;
-  const double res = (-duudx - dvudy - dwudz + visc);
+  const T d2x = gdiv<P2>(2.0, dx2xr) * (gdiv<P2>(-u4, dxr) + gdiv<P2>(u22, dx));
+  const T d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-u8, 0.5 * dy2ye) + gdiv<P2>(u6, 0.5 * dy2yw));
+  const T d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-u12, 0.5 * dz2zt) + gdiv<P2>(u10, 0.5 * dz2zb));
+  const T visc = (d2x + d2y + d2z) / a.re;
+  const T res = (-duudx - dvudy - dwudz + visc);
   return U(0, 0, 0) + a.dt * (res + dpdx);
 }
 
 // vt (NavierStokes.f90:690-780)
 template <bool P2, class Acc>
-__device__ inline double mom_v(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+__device__ inline auto mom_v(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
                                bool near_ok) {
+  using T = std::decay_t<decltype(A.template u<0, 0, 0>())>;
   const double alpha = 1.0;
   const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
-  const double u13 = U(-1, 0, 0) + U(-1, 1, 0), u14 = U(0, 0, 0) + U(0, 1, 0);
-  const double v3 = V(-1, 0, 0) + V(0, 0, 0), v4 = V(-1, 0, 0) - V(0, 0, 0);
-  const double v5 = V(0, 0, 0) + V(1, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
-  const double v7 = V(0, -1, 0) + V(0, 0, 0), v8 = V(0, -1, 0) - V(0, 0, 0);
-  const double v9 = V(0, 0, 0) + V(0, 1, 0), v10 = V(0, 0, 0) - V(0, 1, 0);
-  const double v11 = V(0, 0, -1) + V(0, 0, 0), v12 = V(0, 0, -1) - V(0, 0, 0);
-  const double v13 = V(0, 0, 0) + V(0, 0, 1), v14 = V(0, 0, 0) - V(0, 0, 1);
-  const double w3 = W(0, 0, -1) + W(0, 1, -1), w4 = W(0, 0, 0) + W(0, 1, 0);
-  const double dpdy = gdiv<P2>(P(0, 0, 0) - P(0, 1, 0), 0.5 * (dye + dy));
+  const T u13 = U(-1, 0, 0) + U(-1, 1, 0), u14 = U(0, 0, 0) + U(0, 1, 0);
+  const T v3 = V(-1, 0, 0) + V(0, 0, 0), v4 = V(-1, 0, 0) - V(0, 0, 0);
+  const T v5 = V(0, 0, 0) + V(1, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
+  const T v7 = V(0, -1, 0) + V(0, 0, 0), v8 = V(0, -1, 0) - V(0, 0, 0);
+  const T v9 = V(0, 0, 0) + V(0, 1, 0), v10 = V(0, 0, 0) - V(0, 1, 0);
+  const T v11 = V(0, 0, -1) + V(0, 0, 0), v12 = V(0, 0, -1) - V(0, 0, 0);
+  const T v13 = V(0, 0, 0) + V(0, 0, 1), v14 = V(0, 0, 0) - V(0, 0, 1);
+  const T w3 = W(0, 0, -1) + W(0, 1, -1), w4 = W(0, 0, 0) + W(0, 1, 0);
+  const T dpdy = gdiv<P2>(P(0, 0, 0) - P(0, 1, 0), 0.5 * (dye + dy));
   const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dz2zt = dz + dzt, dz2zb = dz + dzb;
   const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
-  const double uv_e = U(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(0, 0, 0) - U(0, 1, 0));
-  const double uv_w = U(-1, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(-1, 0, 0) - U(-1, 1, 0));
-  const double u_in = 0.5 * (uv_e + uv_w);
-  const double wv_n = W(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (W(0, 0, 0) - W(0, 1, 0));
-  const double wv_s = W(0, 1, -1) + gdiv<P2>(dye, dy2ye) * (W(0, 0, -1) - W(0, 1, -1));
-  const double w_in = 0.5 * (wv_n + wv_s);
-  double duvdx, dvvdy, dwvdz;
+  const T uv_e = U(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(0, 0, 0) - U(0, 1, 0));
+  const T uv_w = U(-1, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(-1, 0, 0) - U(-1, 1, 0));
+  const T u_in = 0.5 * (uv_e + uv_w);
+  const T wv_n = W(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (W(0, 0, 0) - W(0, 1, 0));
+  const T wv_s = W(0, 1, -1) + gdiv<P2>(dye, dy2ye) * (W(0, 0, -1) - W(0, 1, -1));
+  const T w_in = 0.5 * (wv_n + wv_s);
+  T duvdx, dvvdy, dwvdz;
   if (i > 2 && i < nx + 1 && j > 2 && j < ny && k > 2 && k < nz + 1 && near_ok) {
     const double ddx = 0.5 * (A.template hx<-1>() + A.template hx<-2>()), ddxr = 0.5 * (A.template hx<-1>() + A.template hx<0>());
     const double ddz = 0.5 * (A.template hz<-1>() + A.template hz<-2>()), ddzr = 0.5 * (A.template hz<-1>() + A.template hz<0>());
@@ -216,38 +240,39 @@ __device__ inline double mom_v(const MomArgs &a, const Acc &A, const MomCell &m,
     dvvdy = gdiv<P2>(-0.25 * (v7 * v7 + alpha * fabs(v7) * v8 - v9 * v9 - alpha * fabs(v9) * v10), A.template hy<-1>());
     dwvdz = gdiv<P2>(-0.25 * (w3 * v11 + alpha * fabs(w3) * v12 - w4 * v13 - alpha * fabs(w4) * v14), A.template hz<-1>());
   }
-  const double d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-v6, 0.5 * dx2xr) + gdiv<P2>(v4, 0.5 * dx2xl));
-  const double d2y = gdiv<P2>(2.0, dy2ye) * (gdiv<P2>(-v10, dye) + gdiv<P2>(v8, dy));
-  const double d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-v14, 0.5 * dz2zt) + gdiv<P2>(v12, 0.5 * dz2zb));
-  const double visc = (d2x + d2y + d2z) / a.re;
-  const double res = (-duvdx - dvvdy - dwvdz + visc);
+  const T d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-v6, 0.5 * dx2xr) + gdiv<P2>(v4, 0.5 * dx2xl));
+  const T d2y = gdiv<P2>(2.0, dy2ye) * (gdiv<P2>(-v10, dye) + gdiv<P2>(v8, dy));
+  const T d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-v14, 0.5 * dz2zt) + gdiv<P2>(v12, 0.5 * dz2zb));
+  const T visc = (d2x + d2y + d2z) / a.re;
+  const T res = (-duvdx - dvvdy - dwvdz + visc);
   return V(0, 0, 0) + a.dt * (res + dpdy);
 }
 
 // wt (NavierStokes.f90:782-858)
 template <bool P2, class Acc>
-__device__ inline double mom_w(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+__device__ inline auto mom_w(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
                                bool near_ok) {
+  using T = std::decay_t<decltype(A.template u<0, 0, 0>())>;
   const double alpha = 1.0;
   const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz;
-  const double u15 = U(-1, 0, 0) + U(-1, 0, 1), u16 = U(0, 0, 0) + U(0, 0, 1);
-  const double v15 = V(0, -1, 0) + V(0, -1, 1), v16 = V(0, 0, 0) + V(0, 0, 1);
-  const double w5 = W(-1, 0, 0) + W(0, 0, 0), w6 = W(-1, 0, 0) - W(0, 0, 0);
-  const double w7 = W(0, 0, 0) + W(1, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
-  const double w9 = W(0, -1, 0) + W(0, 0, 0), w10 = W(0, -1, 0) - W(0, 0, 0);
-  const double w11 = W(0, 0, 0) + W(0, 1, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
-  const double w13 = W(0, 0, -1) + W(0, 0, 0), w14 = W(0, 0, -1) - W(0, 0, 0);
-  const double w15 = W(0, 0, 0) + W(0, 0, 1), w16 = W(0, 0, 0) - W(0, 0, 1);
-  const double dpdz = gdiv<P2>(P(0, 0, 0) - P(0, 0, 1), 0.5 * (dzt + dz));
+  const T u15 = U(-1, 0, 0) + U(-1, 0, 1), u16 = U(0, 0, 0) + U(0, 0, 1);
+  const T v15 = V(0, -1, 0) + V(0, -1, 1), v16 = V(0, 0, 0) + V(0, 0, 1);
+  const T w5 = W(-1, 0, 0) + W(0, 0, 0), w6 = W(-1, 0, 0) - W(0, 0, 0);
+  const T w7 = W(0, 0, 0) + W(1, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
+  const T w9 = W(0, -1, 0) + W(0, 0, 0), w10 = W(0, -1, 0) - W(0, 0, 0);
+  const T w11 = W(0, 0, 0) + W(0, 1, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
+  const T w13 = W(0, 0, -1) + W(0, 0, 0), w14 = W(0, 0, -1) - W(0, 0, 0);
+  const T w15 = W(0, 0, 0) + W(0, 0, 1), w16 = W(0, 0, 0) - W(0, 0, 1);
+  const T dpdz = gdiv<P2>(P(0, 0, 0) - P(0, 0, 1), 0.5 * (dzt + dz));
   const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt;
   const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
-  const double uw_e = U(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(0, 0, 0) - U(0, 0, 1));
-  const double uw_w = U(-1, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(-1, 0, 0) - U(-1, 0, 1));
-  const double u_in = 0.5 * (uw_e + uw_w);
-  const double vw_n = V(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, 0, 0) - V(0, 0, 1));
-  const double vw_s = V(0, -1, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, -1, 0) - V(0, -1, 1));
-  const double v_in = 0.5 * (vw_n + vw_s);
-  double duwdx, dvwdy, dwwdz;
+  const T uw_e = U(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(0, 0, 0) - U(0, 0, 1));
+  const T uw_w = U(-1, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(-1, 0, 0) - U(-1, 0, 1));
+  const T u_in = 0.5 * (uw_e + uw_w);
+  const T vw_n = V(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, 0, 0) - V(0, 0, 1));
+  const T vw_s = V(0, -1, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, -1, 0) - V(0, -1, 1));
+  const T v_in = 0.5 * (vw_n + vw_s);
+  T duwdx, dvwdy, dwwdz;
   if (i > 2 && i < nx + 1 && j > 2 && j < ny + 1 && k > 2 && k < nz && near_ok) {
     const double ddx = 0.5 * (A.template hx<-1>() + A.template hx<-2>()), ddxr = 0.5 * (A.template hx<-1>() + A.template hx<0>());
     const double ddy = 0.5 * (A.template hy<-1>() + A.template hy<-2>()), ddye = 0.5 * (A.template hy<-1>() + A.template hy<0>());
@@ -262,11 +287,11 @@ __device__ inline double mom_w(const MomArgs &a, const Acc &A, const MomCell &m,
     dvwdy = gdiv<P2>(-0.5 * (v15 * w9 + alpha * fabs(v15) * w10 - v16 * w11 - alpha * fabs(v16) * w12), A.template hy<-1>());
     dwwdz = gdiv<P2>(-0.5 * (w13 * w13 + alpha * fabs(w13) * w14 - w15 * w15 - alpha * fabs(w15) * w16), A.template hz<-1>());
   }
-  const double d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-w8, 0.5 * dx2xr) + gdiv<P2>(w6, 0.5 * dx2xl));
-  const double d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-w12, 0.5 * dy2ye) + gdiv<P2>(w10, 0.5 * dy2yw));
-  const double d2z = gdiv<P2>(2.0, dz2zt) * (gdiv<P2>(-w16, dzt) + gdiv<P2>(w14, dz));
-  const double visc = (d2x + d2y + d2z) / a.re;
-  const double res = (-duwdx - dvwdy - dwwdz + visc);
+  const T d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-w8, 0.5 * dx2xr) + gdiv<P2>(w6, 0.5 * dx2xl));
+  const T d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-w12, 0.5 * dy2ye) + gdiv<P2>(w10, 0.5 * dy2yw));
+  const T d2z = gdiv<P2>(2.0, dz2zt) * (gdiv<P2>(-w16, dzt) + gdiv<P2>(w14, dz));
+  const T visc = (d2x + d2y + d2z) / a.re;
+  const T res = (-duwdx - dvwdy - dwwdz + visc);
   return W(0, 0, 0) + a.dt * (res + dpdz);
 }
 #undef U
@@ -974,12 +999,15 @@ struct TsAcc {
   const TsView *tv;
   const MomArgs *a;
   int q, i, j, k;
-  __device__ double rd(int fi, int di, int dj, int dk) const {
+  // where the read of field fi at offset (di, dj, dk) comes from: kind 0..3
+  // the field itself, 4 + g the ghost array g; idx the entry
+  __device__ void src(int fi, int di, int dj, int dk, int &kind, int64_t &idx) const {
     const int pi = i + di, pj = j + dj, pk = k + dk;
     const int64_t p = ix3(*d, pi, pj, pk);
-    if (!tv->touched[p]) return f[fi][p];
-    int last_q = -1, last_slot = -1, own_slot = -1;
-    double own = 0.0;
+    kind = fi;
+    idx = p;
+    if (!tv->touched[p]) return;
+    int last_q = -1, last_slot = -1, own_slot = -1, own_nn = 0;
     ts_writers(*d, *tv, fi, pi, pj, pk, [&](int qw, int slot) {
       if (qw < q) {
         if (qw > last_q || (qw == last_q && slot > last_slot)) { last_q = qw; last_slot = slot; }
@@ -987,13 +1015,23 @@ struct TsAcc {
         const int nn = tv->best[36 * (int64_t)q + slot];
         if (nn > 0 && slot > own_slot) {
           own_slot = slot;
-          own = tv->ghost[kTargets[slot / 6][slot % 6].g][nn - 1];
+          own_nn = nn;
         }
       }
     });
-    if (own_slot >= 0) return own;
-    if (last_q >= 0) return tv->ghost[kTargets[last_slot / 6][last_slot % 6].gt][tv->n_ts - 1];
-    return f[fi][p];
+    if (own_slot >= 0) {
+      kind = 4 + kTargets[own_slot / 6][own_slot % 6].g;
+      idx = own_nn - 1;
+    } else if (last_q >= 0) {
+      kind = 4 + kTargets[last_slot / 6][last_slot % 6].gt;
+      idx = tv->n_ts - 1;
+    }
+  }
+  __device__ double rd(int fi, int di, int dj, int dk) const {
+    int kind;
+    int64_t idx;
+    src(fi, di, dj, dk, kind, idx);
+    return kind < 4 ? f[kind][idx] : tv->ghost[kind - 4][idx];
   }
   template <int di, int dj, int dk> __device__ double u() const { return rd(0, di, dj, dk); }
   template <int di, int dj, int dk> __device__ double v() const { return rd(1, di, dj, dk); }
@@ -1229,3 +1267,417 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   BCFD_CHECK_LAUNCH();
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Tangent-linear and adjoint of the dense pass (the adjoint time step; no
+// counterpart in the reference).  Tags, spacings and tables are the
+// forward's; the operands are the forward fields u, v, w, p of the block,
+// which the derivatives are taken at.  The component functions above are
+// evaluated on Dual values: the tangent-linear pass carries the tangents of
+// u, v, w, p (b.ad.s) into those of ut, vt, wt; the adjoint pass seeds one
+// operand position at a time (every position the stencil reads, 49 of them)
+// and adds lambda . d(ut, vt, wt)/d(operand) into the operand's adjoint.
+// ---------------------------------------------------------------------------
+namespace bcfd {
+namespace {
+
+// operands with tangents read from the sensitivity arrays
+struct DAcc : GAcc {
+  const double *tu, *tv, *tw, *tp;
+  template <int di, int dj, int dk>
+  __device__ Dual u() const { return {this->template at<di, dj, dk>(fu), this->template at<di, dj, dk>(tu)}; }
+  template <int di, int dj, int dk>
+  __device__ Dual v() const { return {this->template at<di, dj, dk>(fv), this->template at<di, dj, dk>(tv)}; }
+  template <int di, int dj, int dk>
+  __device__ Dual w() const { return {this->template at<di, dj, dk>(fw), this->template at<di, dj, dk>(tw)}; }
+  template <int di, int dj, int dk>
+  __device__ Dual p() const { return {this->template at<di, dj, dk>(fp), this->template at<di, dj, dk>(tp)}; }
+};
+
+// operands with a unit tangent at one seeded position (field sf: 0 u, 1 v,
+// 2 w, 3 p; offset si, sj, sk from the cell) and zero elsewhere
+struct SAcc : GAcc {
+  int sf, si, sj, sk;
+  template <int F, int di, int dj, int dk>
+  __device__ Dual op(const double *f) const {
+    return {this->template at<di, dj, dk>(f), (F == sf && di == si && dj == sj && dk == sk) ? 1.0 : 0.0};
+  }
+  template <int di, int dj, int dk> __device__ Dual u() const { return op<0, di, dj, dk>(fu); }
+  template <int di, int dj, int dk> __device__ Dual v() const { return op<1, di, dj, dk>(fv); }
+  template <int di, int dj, int dk> __device__ Dual w() const { return op<2, di, dj, dk>(fw); }
+  template <int di, int dj, int dk> __device__ Dual p() const { return op<3, di, dj, dk>(fp); }
+};
+
+// every operand position of the three component functions: 15 of u, v and
+// w each (the own component's 13-point upwind cross and the two positions the
+// other components' interpolated velocities add), 4 of p
+struct Seed {
+  signed char f, i, j, k;
+};
+__constant__ Seed kSeeds[49] = {
+    {0, 0, 0, 0}, {0, 1, 0, 0}, {0, 2, 0, 0}, {0, -1, 0, 0}, {0, -2, 0, 0}, {0, 0, 1, 0}, {0, 0, 2, 0},
+    {0, 0, -1, 0}, {0, 0, -2, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 0, -1}, {0, 0, 0, -2}, {0, -1, 1, 0},
+    {0, -1, 0, 1},
+    {1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 0, 0}, {1, -1, 0, 0}, {1, -2, 0, 0}, {1, 0, 1, 0}, {1, 0, 2, 0},
+    {1, 0, -1, 0}, {1, 0, -2, 0}, {1, 0, 0, 1}, {1, 0, 0, 2}, {1, 0, 0, -1}, {1, 0, 0, -2}, {1, 1, -1, 0},
+    {1, 0, -1, 1},
+    {2, 0, 0, 0}, {2, 1, 0, 0}, {2, 2, 0, 0}, {2, -1, 0, 0}, {2, -2, 0, 0}, {2, 0, 1, 0}, {2, 0, 2, 0},
+    {2, 0, -1, 0}, {2, 0, -2, 0}, {2, 0, 0, 1}, {2, 0, 0, 2}, {2, 0, 0, -1}, {2, 0, 0, -2}, {2, 1, 0, -1},
+    {2, 0, 1, -1},
+    {3, 0, 0, 0}, {3, 1, 0, 0}, {3, 0, 1, 0}, {3, 0, 0, 1}};
+
+__device__ inline bool mom_fluid(const MomArgs &a, int64_t q, int &i, int &j, int &k, int64_t &c) {
+  const Dims &d = a.d;
+  if (q >= (int64_t)d.nx * d.ny * d.nz) return false;
+  i = 2 + (int)(q % d.nx);
+  j = 2 + (int)((q / d.nx) % d.ny);
+  k = 2 + (int)(q / ((int64_t)d.nx * d.ny));
+  c = ix3(d, i, j, k);
+  return a.cell[c] == 0;
+}
+
+template <bool P2>
+__global__ void __launch_bounds__(256) k_momentum_tlm(MomArgs a, Fields F, Fields T, double *__restrict__ tut,
+                                                      double *__restrict__ tvt, double *__restrict__ twt,
+                                                      const uint8_t *__restrict__ deferred) {
+  int i, j, k;
+  int64_t c;
+  if (!mom_fluid(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k, c)) return;
+  if (deferred && deferred[c]) return;
+  const MomCell m = mom_cell(a, i, j, k);
+  const bool near_ok = mom_near_ok(a, i, j, k);
+  DAcc A;
+  A.fu = F.u; A.fv = F.v; A.fw = F.w; A.fp = F.p;
+  A.c = m.c; A.sy = m.sy; A.sz = m.sz; A.a = &a; A.i = i; A.j = j; A.k = k;
+  A.tu = T.u; A.tv = T.v; A.tw = T.w; A.tp = T.p;
+  tut[c] = mom_u<P2>(a, A, m, i, j, k, near_ok).t;
+  tvt[c] = mom_v<P2>(a, A, m, i, j, k, near_ok).t;
+  twt[c] = mom_w<P2>(a, A, m, i, j, k, near_ok).t;
+}
+
+// lambda of ut, vt, wt at a fluid cell is consumed (the forward overwrote
+// them) and scattered, atomically, into the adjoints of the operands
+template <bool P2>
+__global__ void __launch_bounds__(256) k_momentum_adj(MomArgs a, Fields F, double *__restrict__ lut,
+                                                      double *__restrict__ lvt, double *__restrict__ lwt,
+                                                      double *__restrict__ lu, double *__restrict__ lv,
+                                                      double *__restrict__ lw, double *__restrict__ lp,
+                                                      const uint8_t *__restrict__ deferred) {
+  int i, j, k;
+  int64_t c;
+  if (!mom_fluid(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, i, j, k, c)) return;
+  if (deferred && deferred[c]) return;
+  const double gu = lut[c], gv = lvt[c], gw = lwt[c];
+  lut[c] = 0.0;
+  lvt[c] = 0.0;
+  lwt[c] = 0.0;
+  if (gu == 0.0 && gv == 0.0 && gw == 0.0) return;
+  const MomCell m = mom_cell(a, i, j, k);
+  const bool near_ok = mom_near_ok(a, i, j, k);
+  SAcc A;
+  A.fu = F.u; A.fv = F.v; A.fw = F.w; A.fp = F.p;
+  A.c = m.c; A.sy = m.sy; A.sz = m.sz; A.a = &a; A.i = i; A.j = j; A.k = k;
+  double *lam[4] = {lu, lv, lw, lp};
+  for (int q = 0; q < 49; ++q) {
+    const Seed s = kSeeds[q];
+    A.sf = s.f; A.si = s.i; A.sj = s.j; A.sk = s.k;
+    const double g = gu * mom_u<P2>(a, A, m, i, j, k, near_ok).t + gv * mom_v<P2>(a, A, m, i, j, k, near_ok).t +
+                     gw * mom_w<P2>(a, A, m, i, j, k, near_ok).t;
+    if (g != 0.0) atomicAdd(&lam[s.f][c + s.i + (int64_t)s.j * m.sy + (int64_t)s.k * m.sz], g);
+  }
+}
+
+__global__ void k_fill_u8(uint8_t *__restrict__ x, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) x[q] = 0;
+}
+
+__global__ void k_fill_f64(double *__restrict__ x, int64_t n, double v) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) x[q] = v;
+}
+
+// The thin-surface pass's deferred cells (k_ts_parallel): every operand is
+// read from where the ordered pass lets the cell read it, a field entry or a
+// ghost entry (TsAcc::src), and its tangent or adjoint goes to the same place.
+// Ghost tangents / adjoints: gh (BGH_N arrays indexed like the ghosts).
+struct TsSens {
+  double *f[4];
+  double *gh[BGH_N];
+};
+
+struct TsDAcc {
+  TsAcc base;
+  const TsSens *t;
+  __device__ Dual rd(int fi, int di, int dj, int dk) const {
+    int kind;
+    int64_t idx;
+    base.src(fi, di, dj, dk, kind, idx);
+    if (kind < 4) return {base.f[kind][idx], t->f[kind][idx]};
+    return {base.tv->ghost[kind - 4][idx], t->gh[kind - 4][idx]};
+  }
+  template <int di, int dj, int dk> __device__ Dual u() const { return rd(0, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ Dual v() const { return rd(1, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ Dual w() const { return rd(2, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ Dual p() const { return rd(3, di, dj, dk); }
+  template <int o> __device__ double hx() const { return base.template hx<o>(); }
+  template <int o> __device__ double hy() const { return base.template hy<o>(); }
+  template <int o> __device__ double hz() const { return base.template hz<o>(); }
+  template <int T> __device__ const double *xa() const { return base.template xa<T>(); }
+  template <int T> __device__ const double *xk() const { return base.template xk<T>(); }
+  template <int T> __device__ const double *ya() const { return base.template ya<T>(); }
+  template <int T> __device__ const double *yk() const { return base.template yk<T>(); }
+  template <int T> __device__ const double *za() const { return base.template za<T>(); }
+  template <int T> __device__ const double *zk() const { return base.template zk<T>(); }
+};
+
+struct TsSAcc {
+  TsAcc base;
+  int sf, si, sj, sk;
+  __device__ Dual rd(int fi, int di, int dj, int dk) const {
+    return {base.rd(fi, di, dj, dk), (fi == sf && di == si && dj == sj && dk == sk) ? 1.0 : 0.0};
+  }
+  template <int di, int dj, int dk> __device__ Dual u() const { return rd(0, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ Dual v() const { return rd(1, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ Dual w() const { return rd(2, di, dj, dk); }
+  template <int di, int dj, int dk> __device__ Dual p() const { return rd(3, di, dj, dk); }
+  template <int o> __device__ double hx() const { return base.template hx<o>(); }
+  template <int o> __device__ double hy() const { return base.template hy<o>(); }
+  template <int o> __device__ double hz() const { return base.template hz<o>(); }
+  template <int T> __device__ const double *xa() const { return base.template xa<T>(); }
+  template <int T> __device__ const double *xk() const { return base.template xk<T>(); }
+  template <int T> __device__ const double *ya() const { return base.template ya<T>(); }
+  template <int T> __device__ const double *yk() const { return base.template yk<T>(); }
+  template <int T> __device__ const double *za() const { return base.template za<T>(); }
+  template <int T> __device__ const double *zk() const { return base.template zk<T>(); }
+};
+
+__device__ inline void ts_cell(const Dims &d, const int *list, int q, int &i, int &j, int &k) {
+  const int lin = list[q];
+  i = 2 + lin % d.nx;
+  j = 2 + (lin / d.nx) % d.ny;
+  k = 2 + lin / (d.nx * d.ny);
+}
+
+__global__ void __launch_bounds__(256) k_ts_tlm(MomArgs a, Fields F, TsSens T, const int *__restrict__ list,
+                                                int n_list, TsView tv, double *__restrict__ tut,
+                                                double *__restrict__ tvt, double *__restrict__ twt) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_list) return;
+  int i, j, k;
+  ts_cell(a.d, list, q, i, j, k);
+  const MomCell m = mom_cell(a, i, j, k);
+  const bool near_ok = mom_near_ok(a, i, j, k);
+  const TsDAcc A{{{F.u, F.v, F.w, F.p}, &a.d, &tv, &a, q, i, j, k}, &T};
+  tut[m.c] = mom_u<false>(a, A, m, i, j, k, near_ok).t;
+  tvt[m.c] = mom_v<false>(a, A, m, i, j, k, near_ok).t;
+  twt[m.c] = mom_w<false>(a, A, m, i, j, k, near_ok).t;
+}
+
+__global__ void __launch_bounds__(256) k_ts_adj(MomArgs a, Fields F, TsSens L, const int *__restrict__ list,
+                                                int n_list, TsView tv, double *__restrict__ lut,
+                                                double *__restrict__ lvt, double *__restrict__ lwt) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_list) return;
+  int i, j, k;
+  ts_cell(a.d, list, q, i, j, k);
+  const MomCell m = mom_cell(a, i, j, k);
+  const double gu = lut[m.c], gv = lvt[m.c], gw = lwt[m.c];
+  lut[m.c] = 0.0;
+  lvt[m.c] = 0.0;
+  lwt[m.c] = 0.0;
+  if (gu == 0.0 && gv == 0.0 && gw == 0.0) return;
+  const bool near_ok = mom_near_ok(a, i, j, k);
+  TsSAcc A{{{F.u, F.v, F.w, F.p}, &a.d, &tv, &a, q, i, j, k}, 0, 0, 0, 0};
+  for (int s = 0; s < 49; ++s) {
+    const Seed sd = kSeeds[s];
+    A.sf = sd.f; A.si = sd.i; A.sj = sd.j; A.sk = sd.k;
+    const double g = gu * mom_u<false>(a, A, m, i, j, k, near_ok).t + gv * mom_v<false>(a, A, m, i, j, k, near_ok).t +
+                     gw * mom_w<false>(a, A, m, i, j, k, near_ok).t;
+    if (g == 0.0) continue;
+    int kind;
+    int64_t idx;
+    A.base.src(sd.f, sd.i, sd.j, sd.k, kind, idx);
+    atomicAdd(kind < 4 ? &L.f[kind][idx] : &L.gh[kind - 4][idx], g);
+  }
+}
+
+// transpose of k_ts_restore: the adjoint at every restored position goes to
+// the ghost entry its last writer stored there
+__global__ void k_ts_restore_adj(Dims d, TsSens L, const int *__restrict__ list, int n_list, TsView tv) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = gid / 36, slot = gid % 36;
+  if (q >= n_list) return;
+  const int dq = slot / 6;
+  if (!((tv.adjm[q] >> dq) & 1u)) return;
+  int i, j, k;
+  ts_cell(d, list, q, i, j, k);
+  const Target T = kTargets[dq][slot % 6];
+  const int pi = i + T.di, pj = j + T.dj, pk = k + T.dk;
+  if (pi < 1 || pj < 1 || pk < 1 || pi > d.NX || pj > d.NY || pk > d.NZ) return;
+  bool last = true;
+  ts_writers(d, tv, T.f, pi, pj, pk, [&](int qw, int s2) {
+    if (qw > q || (qw == q && s2 > slot)) last = false;
+  });
+  if (!last) return;
+  double *lf = &L.f[T.f][ix3(d, pi, pj, pk)];
+  const double g = *lf;
+  *lf = 0.0;
+  if (g != 0.0) atomicAdd(&L.gh[T.gt][tv.n_ts - 1], g);
+}
+
+}  // namespace
+
+int compact_mask_lin(Block &b, const uint8_t *mask, int **out, int64_t *cap, int64_t *count);
+
+// The deferred cells of the thin-surface pass and the structures of its
+// parallel form, as biocfd_ns_momentum builds them (geometry only); cnt 0
+// and deferred null without TS cells.  ts_release clears the cell map.
+static int ts_prepare(Block &b, const MomArgs &a, TsView &tv, const int *&list, int64_t &cnt,
+                      const uint8_t *&deferred) {
+  cnt = 0;
+  deferred = nullptr;
+  list = nullptr;
+  const int n_ts = (int)b.counts[BC_TS];
+  if (n_ts == 0) return 0;
+  IbData &ib = b.ib;
+  hipStream_t s = rt().stream;
+  if (grow(&ib.ts_mask, &ib.cap_ts_mask, 2 * b.n2)) return -1;
+  uint8_t *touched = ib.ts_mask, *dfr = ib.ts_mask + b.n2;
+  const dim3 blk(64, 4, 1), grd(div_up(b.d.nx, 64), div_up(b.d.ny, 4), (unsigned)b.d.nz);
+  hipLaunchKernelGGL(k_fill_u8, dim3(div_up(2 * b.n2, 256)), dim3(256), 0, s, ib.ts_mask, 2 * b.n2);
+  hipLaunchKernelGGL(k_ts_touch, grd, blk, 0, s, b.d, b.tag[BT_CELL], b.tag[BT_CELL2], touched);
+  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b.d, b.tag[BT_CELL], b.tag[BT_CELL2], touched, dfr);
+  BCFD_CHECK_LAUNCH();
+  if (compact_mask_lin(b, dfr, &ib.ts_list, &ib.cap_ts_list, &cnt)) return -1;
+  deferred = dfr;
+  list = ib.ts_list;
+  if (cnt == 0) return 0;
+  TsArgs ts{};
+  ts.ts_ijk = ib.ts_ijk; ts.cap_ts = ib.cap_ts; ts.index_ts = ib.index_ts;
+  for (int q = 0; q < BN_NNEL; ++q) ts.nel[q] = ib.nel[q];
+  for (int q = 0; q < BE_NELEM; ++q) ts.elem[q] = ib.elem[q];
+  for (int q = 0; q < BGH_N; ++q) ts.ghost[q] = ib.ghost[q];
+  ts.x1 = b.g[BG_X1]; ts.y1 = b.g[BG_Y1]; ts.z1 = b.g[BG_Z1];
+  ts.xp = b.g[BG_XP]; ts.yp = b.g[BG_YP]; ts.zp = b.g[BG_ZP];
+  ts.n_ts = n_ts;
+  if (grow(&ib.ts_best, &ib.cap_ts_best, 36 * cnt)) return -1;
+  hipLaunchKernelGGL(k_ts_best, dim3((unsigned)cnt), dim3(256), 0, s, a, list, (int)cnt, ts, ib.ts_best);
+  const int *old_map = ib.ts_qmap;
+  if (grow(&ib.ts_qmap, &ib.cap_ts_qmap, b.n2) || grow(&ib.ts_adjm, &ib.cap_ts_adjm, cnt)) return -1;
+  if (ib.ts_qmap != old_map)
+    hipLaunchKernelGGL(k_fill_int, dim3(div_up(ib.cap_ts_qmap, 256)), dim3(256), 0, s, ib.ts_qmap, ib.cap_ts_qmap, -1);
+  tv = TsView{};
+  tv.qmap = ib.ts_qmap;
+  tv.adjm = ib.ts_adjm;
+  tv.best = ib.ts_best;
+  tv.touched = touched;
+  for (int g = 0; g < BGH_N; ++g) tv.ghost[g] = ib.ghost[g];
+  tv.n_ts = n_ts;
+  hipLaunchKernelGGL(k_ts_prep, dim3(div_up(cnt, 256)), dim3(256), 0, s, b.d, b.tag[BT_CELL2], list, (int)cnt,
+                     ib.ts_qmap, ib.ts_adjm, 0);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+static int ts_release(Block &b, const int *list, int64_t cnt) {
+  if (cnt == 0) return 0;
+  hipLaunchKernelGGL(k_ts_prep, dim3(div_up(cnt, 256)), dim3(256), 0, rt().stream, b.d, b.tag[BT_CELL2], list,
+                     (int)cnt, b.ib.ts_qmap, b.ib.ts_adjm, 1);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+// the ghost sensitivities, at least as long as the TS arrays (new entries 0)
+int ad_ghosts(Block &b) {
+  AdWork &w = b.ad;
+  const int64_t need = b.ib.cap_ts > 0 ? b.ib.cap_ts : 1;
+  if (w.gh[0] && w.cap_gh >= need) return 0;
+  for (int q = 0; q < BGH_N; ++q) {
+    if (dev_alloc(&w.gh[q], need)) return -1;
+    hipLaunchKernelGGL(k_fill_f64, dim3(div_up(need, 256)), dim3(256), 0, rt().stream, w.gh[q], need, 0.0);
+  }
+  w.cap_gh = need;
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+static int momentum_args(Block &b, double dt, double re, MomArgs &a) {
+  for (int t = 0; t < BTB_N; ++t)
+    if (!b.ca[t]) { set_error("non_uni_coeff has not been called"); return -1; }
+  if (!b.g[BG_DX]) { set_error("grids not uploaded"); return -1; }
+  a.d = b.d;
+  a.cell = b.tag[BT_CELL];
+  a.cell2 = b.tag[BT_CELL2];
+  a.dx = b.g[BG_DX]; a.dy = b.g[BG_DY]; a.dz = b.g[BG_DZ];
+  for (int t = 0; t < BTB_N; ++t) { a.ca[t] = b.ca[t]; a.ck[t] = b.ck[t]; }
+  a.dt = dt;
+  a.re = re;
+  return 0;
+}
+
+int momentum_tlm(Block &b, double dt, double re) {
+  MomArgs a;
+  if (momentum_args(b, dt, re, a)) return -1;
+  TsView tv;
+  const int *list = nullptr;
+  const uint8_t *deferred = nullptr;
+  int64_t cnt = 0;
+  if (ts_prepare(b, a, tv, list, cnt, deferred) || ad_ghosts(b)) return -1;
+  hipStream_t s = rt().stream;
+  double **t = b.ad.s;
+  const Fields F{b.f[BF_U], b.f[BF_V], b.f[BF_W], b.f[BF_P]};
+  const Fields T{t[BF_U], t[BF_V], t[BF_W], t[BF_P]};
+  const unsigned g = div_up((int64_t)b.d.nx * b.d.ny * b.d.nz, 256);
+  if (g_mom_pow2 && pow2_grid(b))
+    hipLaunchKernelGGL(k_momentum_tlm<true>, dim3(g), dim3(256), 0, s, a, F, T, t[BF_UT], t[BF_VT], t[BF_WT], deferred);
+  else
+    hipLaunchKernelGGL(k_momentum_tlm<false>, dim3(g), dim3(256), 0, s, a, F, T, t[BF_UT], t[BF_VT], t[BF_WT],
+                       deferred);
+  if (cnt > 0) {
+    TsSens ts{{t[BF_U], t[BF_V], t[BF_W], t[BF_P]}, {}};
+    for (int q = 0; q < BGH_N; ++q) ts.gh[q] = b.ad.gh[q];
+    hipLaunchKernelGGL(k_ts_tlm, dim3(div_up(cnt, 256)), dim3(256), 0, s, a, F, ts, list, (int)cnt, tv, t[BF_UT],
+                       t[BF_VT], t[BF_WT]);
+    // the pass leaves each restored position holding its last writer's
+    // restore ghost: so do the tangents
+    TsView tt = tv;
+    for (int q = 0; q < BGH_N; ++q) tt.ghost[q] = b.ad.gh[q];
+    hipLaunchKernelGGL(k_ts_restore, dim3(div_up(36 * cnt, 256)), dim3(256), 0, s, b.d, t[BF_U], t[BF_V], t[BF_W],
+                       t[BF_P], list, (int)cnt, tt);
+  }
+  BCFD_CHECK_LAUNCH();
+  return ts_release(b, list, cnt);
+}
+
+int momentum_adj(Block &b, double dt, double re) {
+  MomArgs a;
+  if (momentum_args(b, dt, re, a)) return -1;
+  TsView tv;
+  const int *list = nullptr;
+  const uint8_t *deferred = nullptr;
+  int64_t cnt = 0;
+  if (ts_prepare(b, a, tv, list, cnt, deferred) || ad_ghosts(b)) return -1;
+  hipStream_t s = rt().stream;
+  double **l = b.ad.s;
+  const Fields F{b.f[BF_U], b.f[BF_V], b.f[BF_W], b.f[BF_P]};
+  TsSens ls{{l[BF_U], l[BF_V], l[BF_W], l[BF_P]}, {}};
+  for (int q = 0; q < BGH_N; ++q) ls.gh[q] = b.ad.gh[q];
+  // reverse order of the forward: the restores, the deferred cells, the dense pass
+  if (cnt > 0) {
+    hipLaunchKernelGGL(k_ts_restore_adj, dim3(div_up(36 * cnt, 256)), dim3(256), 0, s, b.d, ls, list, (int)cnt, tv);
+    hipLaunchKernelGGL(k_ts_adj, dim3(div_up(cnt, 256)), dim3(256), 0, s, a, F, ls, list, (int)cnt, tv, l[BF_UT],
+                       l[BF_VT], l[BF_WT]);
+  }
+  const unsigned g = div_up((int64_t)b.d.nx * b.d.ny * b.d.nz, 256);
+  if (g_mom_pow2 && pow2_grid(b))
+    hipLaunchKernelGGL(k_momentum_adj<true>, dim3(g), dim3(256), 0, s, a, F, l[BF_UT], l[BF_VT], l[BF_WT], l[BF_U],
+                       l[BF_V], l[BF_W], l[BF_P], deferred);
+  else
+    hipLaunchKernelGGL(k_momentum_adj<false>, dim3(g), dim3(256), 0, s, a, F, l[BF_UT], l[BF_VT], l[BF_WT], l[BF_U],
+                       l[BF_V], l[BF_W], l[BF_P], deferred);
+  BCFD_CHECK_LAUNCH();
+  return ts_release(b, list, cnt);
+}
+
+}  // namespace bcfd

@@ -351,17 +351,20 @@ __global__ void __launch_bounds__(1024) k_norm_end(SorState *st, const double *p
   }
 }
 
-// velocityBC (BoundaryConditions.f90:11-74): three face passes in order
+// velocityBC (BoundaryConditions.f90:11-74): three face passes in order.
+// `inlet` is the inflow value (uc; 0 for the tangent-linear pass, where uc
+// stays the convective speed of the outflow condition)
 __global__ void k_bc_x(Dims d, const double *__restrict__ u, const double *__restrict__ v,
                        const double *__restrict__ w, double *__restrict__ ut, double *__restrict__ vt,
-                       double *__restrict__ wt, const double *__restrict__ dxv, double dt, double uc) {
+                       double *__restrict__ wt, const double *__restrict__ dxv, double dt, double uc,
+                       double inlet) {
   const int j = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   const int k = 2 + blockIdx.y;
   if (j > d.ny + 1) return;
   const int nx = d.nx;
   const int64_t c1 = ix3(d, 1, j, k), cn = ix3(d, nx, j, k), cn1 = ix3(d, nx + 1, j, k),
                 cn2 = ix3(d, nx + 2, j, k);
-  ut[c1] = uc;
+  ut[c1] = inlet;
   vt[c1] = -vt[cn1];
   wt[c1] = -wt[cn1];
   ut[cn1] = u[cn1] - (dt / dxv[nx]) * uc * (u[cn1] - u[cn]);
@@ -1152,10 +1155,10 @@ int correct(Block &b, double dt) {
   return 0;
 }
 
-int velocity_bc(Block &b, double dt, double uc) {
+int velocity_bc(Block &b, double dt, double uc, double inlet) {
   hipStream_t s = rt().stream;
   hipLaunchKernelGGL(k_bc_x, dim3(div_up(b.d.ny, 64), b.d.nz), dim3(64), 0, s, b.d, b.f[BF_U],
-                     b.f[BF_V], b.f[BF_W], b.f[BF_UT], b.f[BF_VT], b.f[BF_WT], b.g[BG_DX], dt, uc);
+                     b.f[BF_V], b.f[BF_W], b.f[BF_UT], b.f[BF_VT], b.f[BF_WT], b.g[BG_DX], dt, uc, inlet);
   hipLaunchKernelGGL(k_bc_y, dim3(div_up(b.d.nx, 64), b.d.nz), dim3(64), 0, s, b.d, b.f[BF_UT],
                      b.f[BF_VT], b.f[BF_WT]);
   hipLaunchKernelGGL(k_bc_z, dim3(div_up(b.d.nx, 64), b.d.ny), dim3(64), 0, s, b.d, b.f[BF_UT],
@@ -1176,6 +1179,23 @@ int norm_copy(Block &b, double dt) {
   return 0;
 }
 
+// b = pc = 0, computeDiv, n_iter SOR iterations of the FP64 method,
+// correctPressure + correctVelocity (PcorVcor.F90:67-77, 222-377) on one
+// block, no interface updates or boundary conditions
+int projection(Block &b, int64_t n_iter, double dt) {
+  if (check_ready(b)) return -1;
+  if (n_iter < 1) { set_error("projection: iteration count %lld < 1", (long long)n_iter); return -1; }
+  if (precision_mode() != 0) { set_error("projection: the FP64 method only (set_precision 0)"); return -1; }
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_zero_pcor, dim3(div_up(b.n2, 256)), dim3(256), 0, s, b.f[BF_B], b.f[BF_PC], b.n2);
+  hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 1);
+  BCFD_CHECK_LAUNCH();
+  if (compute_div(b)) return -1;
+  // a negative tolerance never stops the iterations: nIterPcor > n_iter - 1 ends them after n_iter
+  if (redblack_sor(b, n_iter - 1, dt, -1.0)) return -1;
+  return correct(b, dt);
+}
+
 }  // namespace bcfd
 
 using namespace bcfd;
@@ -1187,7 +1207,7 @@ int biocfd_velocity_bc(int id, double dt, double uc) {
   Block *b = get_block(id);
   if (!b) return -1;
   if (!b->g[BG_DX]) { set_error("grids not uploaded"); return -1; }
-  return velocity_bc(*b, dt, uc);
+  return velocity_bc(*b, dt, uc, uc);
 }
 
 int biocfd_solid_cell_bc(int id) {
@@ -1322,17 +1342,7 @@ int biocfd_projection(int id, int64_t n_iter, double dt) {
   BCFD_API_LOCK();
   Block *b = get_block(id);
   if (!b) return -1;
-  if (check_ready(*b)) return -1;
-  if (n_iter < 1) { set_error("projection: iteration count %lld < 1", (long long)n_iter); return -1; }
-  if (precision_mode() != 0) { set_error("projection: the FP64 method only (set_precision 0)"); return -1; }
-  hipStream_t s = rt().stream;
-  hipLaunchKernelGGL(k_zero_pcor, dim3(div_up(b->n2, 256)), dim3(256), 0, s, b->f[BF_B], b->f[BF_PC], b->n2);
-  hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b->sor, 1);
-  BCFD_CHECK_LAUNCH();
-  if (compute_div(*b)) return -1;
-  // a negative tolerance never stops the iterations: nIterPcor > n_iter - 1 ends them after n_iter
-  if (redblack_sor(*b, n_iter - 1, dt, -1.0)) return -1;
-  return correct(*b, dt);
+  return projection(*b, n_iter, dt);
 }
 
 int biocfd_compute_div(int id) {
@@ -1389,7 +1399,7 @@ int biocfd_poisson_solver(int first, int last, int64_t pc_ita_max, double epsi, 
   if (interface_coarse_update(2)) return -1;
   for (int g = first; g <= last; ++g)
     if (correct(rt().blk[g], dt)) return -1;
-  if (first == 1 && velocity_bc(rt().blk[1], dt, uc)) return -1;
+  if (first == 1 && velocity_bc(rt().blk[1], dt, uc, uc)) return -1;
   for (int g = first; g <= last; ++g)
     if (norm_copy(rt().blk[g], dt)) return -1;
   if (interface_fine_update(0) || interface_coarse_update(0)) return -1;

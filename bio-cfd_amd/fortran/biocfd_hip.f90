@@ -419,6 +419,172 @@ module biocfd_hip
       real(c_double) :: ubar(*), vbar(*), wbar(*), pbar(*), utbar(*), vtbar(*), wtbar(*)
     end function
 
+    ! ---- tangent-linear and adjoint time step (geometry frozen) ----
+    integer(c_int) function biocfd_ad_reset(blk) bind(C, name="biocfd_ad_reset")
+      import :: c_int
+      integer(c_int), value :: blk
+    end function
+
+    integer(c_int) function biocfd_ad_upload(blk, field, host) bind(C, name="biocfd_ad_upload")
+      import :: c_int, c_double
+      integer(c_int), value :: blk, field
+      real(c_double) :: host(*)
+    end function
+
+    integer(c_int) function biocfd_ad_download(blk, field, host) bind(C, name="biocfd_ad_download")
+      import :: c_int, c_double
+      integer(c_int), value :: blk, field
+      real(c_double) :: host(*)
+    end function
+
+    integer(c_int) function biocfd_ad_kin(blk, kin_bar, clear) bind(C, name="biocfd_ad_kin")
+      import :: c_int, c_double
+      integer(c_int), value :: blk, clear
+      real(c_double) :: kin_bar(*)
+    end function
+
+    integer(c_int) function biocfd_ad_ghost_upload(blk, which, host, count) bind(C, name="biocfd_ad_ghost_upload")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk, which
+      real(c_double) :: host(*)
+      integer(c_int64_t), value :: count
+    end function
+
+    integer(c_int) function biocfd_ad_ghost_download(blk, which, host, count) bind(C, &
+        name="biocfd_ad_ghost_download")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk, which
+      real(c_double) :: host(*)
+      integer(c_int64_t), value :: count
+    end function
+
+    integer(c_int) function biocfd_tlm_momentum(blk, dt, re) bind(C, name="biocfd_tlm_momentum")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: dt, re
+    end function
+
+    integer(c_int) function biocfd_adj_momentum(blk, dt, re) bind(C, name="biocfd_adj_momentum")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: dt, re
+    end function
+
+    integer(c_int) function biocfd_tlm_velocity_bc(blk, dt, uc) bind(C, name="biocfd_tlm_velocity_bc")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: dt, uc
+    end function
+
+    integer(c_int) function biocfd_adj_velocity_bc(blk, dt, uc) bind(C, name="biocfd_adj_velocity_bc")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: dt, uc
+    end function
+
+    integer(c_int) function biocfd_tlm_solid_cell_bc(blk) bind(C, name="biocfd_tlm_solid_cell_bc")
+      import :: c_int
+      integer(c_int), value :: blk
+    end function
+
+    integer(c_int) function biocfd_adj_solid_cell_bc(blk) bind(C, name="biocfd_adj_solid_cell_bc")
+      import :: c_int
+      integer(c_int), value :: blk
+    end function
+
+    integer(c_int) function biocfd_tlm_forcing(blk, routine, kin, kin_dot) bind(C, name="biocfd_tlm_forcing")
+      import :: c_int, c_double
+      integer(c_int), value :: blk, routine
+      real(c_double) :: kin(*), kin_dot(*)
+    end function
+
+    integer(c_int) function biocfd_adj_forcing(blk, routine, kin) bind(C, name="biocfd_adj_forcing")
+      import :: c_int, c_double
+      integer(c_int), value :: blk, routine
+      real(c_double) :: kin(*)
+    end function
+
+    integer(c_int) function biocfd_tlm_poisson(blk, n_iter, dt, uc) bind(C, name="biocfd_tlm_poisson")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: n_iter
+      real(c_double), value :: dt, uc
+    end function
+
+    integer(c_int) function biocfd_adj_poisson(blk, n_iter, dt, uc) bind(C, name="biocfd_adj_poisson")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: n_iter
+      real(c_double), value :: dt, uc
+    end function
+
+    integer(c_int) function biocfd_stress_sums(blk, mu_f, rho_f, kin, out) bind(C, name="biocfd_stress_sums")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: mu_f, rho_f
+      real(c_double) :: kin(*), out(*)
+    end function
+
+    integer(c_int) function biocfd_tlm_stress(blk, mu_f, rho_f, kin, kin_dot, out) bind(C, &
+        name="biocfd_tlm_stress")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: mu_f, rho_f
+      real(c_double) :: kin(*), kin_dot(*), out(*)
+    end function
+
+    integer(c_int) function biocfd_adj_stress(blk, mu_f, rho_f, kin, sbar) bind(C, name="biocfd_adj_stress")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: mu_f, rho_f
+      real(c_double) :: kin(*), sbar(*)
+    end function
+
+    integer(c_int) function biocfd_tlm_step_start(blk, n_iter, dt, re, uc, kin, kin_dot) bind(C, &
+        name="biocfd_tlm_step_start")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: n_iter
+      real(c_double), value :: dt, re, uc
+      real(c_double) :: kin(*), kin_dot(*)
+    end function
+
+    integer(c_int) function biocfd_tlm_step_end(blk, kin, kin_dot) bind(C, name="biocfd_tlm_step_end")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double) :: kin(*), kin_dot(*)
+    end function
+
+    integer(c_int) function biocfd_adj_step_end(blk, kin) bind(C, name="biocfd_adj_step_end")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double) :: kin(*)
+    end function
+
+    integer(c_int) function biocfd_adj_step_start(blk, n_iter, dt, re, uc, kin) bind(C, &
+        name="biocfd_adj_step_start")
+      import :: c_int, c_int64_t, c_double
+      integer(c_int), value :: blk
+      integer(c_int64_t), value :: n_iter
+      real(c_double), value :: dt, re, uc
+      real(c_double) :: kin(*)
+    end function
+
+    integer(c_int) function biocfd_snapshot_save(blk, slot) bind(C, name="biocfd_snapshot_save")
+      import :: c_int
+      integer(c_int), value :: blk, slot
+    end function
+
+    integer(c_int) function biocfd_snapshot_restore(blk, slot) bind(C, name="biocfd_snapshot_restore")
+      import :: c_int
+      integer(c_int), value :: blk, slot
+    end function
+
+    integer(c_int) function biocfd_snapshot_free(blk, slot) bind(C, name="biocfd_snapshot_free")
+      import :: c_int
+      integer(c_int), value :: blk, slot
+    end function
+
     integer(c_int) function biocfd_coarse_update_adjoint(itf, comp, lam_coarse, lam_fine) bind(C, &
         name="biocfd_coarse_update_adjoint")
       import :: c_int, c_double

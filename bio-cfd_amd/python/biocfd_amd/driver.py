@@ -36,6 +36,8 @@ class BlockState:
         self.markers = np.zeros(3, dtype=np.int64)
         self.move_check = 0
         self.blk_mv_tag = 0
+        # the time the kinematic state was last computed at (None: the set-up's)
+        self.kin_t = None
 
     @property
     def shape(self):
@@ -77,6 +79,10 @@ class Simulation:
         H.call("biocfd_init", device)
         H.call("biocfd_set_precision", 0 if precision == "fp64" else 1)
         self.blocks = [BlockState(g + 1, spec) for g, spec in enumerate(case.blocks)]
+        # kin_hook(st, half) -> kinematic vector of the forcing calls ("start":
+        # momentum stage and pressureForcing1, "end": after the body moved);
+        # None: st.kin_vec() (the adjoint driver substitutes velocities)
+        self.kin_hook = None
         self.totime = 0.0
         self.ita = 0
         self.phase = case.phase_angle_deg * math.pi / 180.0
@@ -183,6 +189,9 @@ class Simulation:
         else:
             list(self._pool.map(fn, blocks))
 
+    def kin(self, st: BlockState, half: str) -> np.ndarray:
+        return st.kin_vec() if self.kin_hook is None else self.kin_hook(st, half)
+
     def step(self):
         c = self.case
         self.ita += 1
@@ -194,13 +203,13 @@ class Simulation:
                 H.call("biocfd_velocity_bc", st.g, c.dt, c.uc)
             if st.body is not None:
                 H.call("biocfd_solid_cell_bc", st.g)
-                kv = st.kin_vec()
+                kv = self.kin(st, "start")
                 H.call("biocfd_velocity_forcing1", st.g, H.ptr(kv))
             if st.g == 1:
                 H.call("biocfd_velocity_bc", st.g, c.dt, c.uc)
 
         def pressure_forcing(st):
-            kv = st.kin_vec()
+            kv = self.kin(st, "start")
             H.call("biocfd_pressure_forcing1", st.g, H.ptr(kv))
 
         def surface_variables(st):
@@ -208,6 +217,7 @@ class Simulation:
             H.call("biocfd_compute_surface_variables", st.nn, H.ptr(st.xn), H.ptr(st.yn), H.ptr(st.zn),
                    H.ptr(st.nid), self.phase, c.piv_pt, self.ita, c.dt, c.dxmin, c.freq, self.totime,
                    H.ptr(st.kin), H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.znode1))
+            st.kin_t = self.totime
 
         def surface_norm(st):
             H.call("biocfd_compute_surface_norm", st.g, H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.znode1),
@@ -218,7 +228,7 @@ class Simulation:
 
         def retag_and_force(st):
             g = st.g
-            kv = st.kin_vec()
+            kv = self.kin(st, "end")
             H.call("biocfd_selective_retagging", g, st.blk_mv_tag)
             st.blk_mv_tag = 0
             H.call("biocfd_cell_count_solid", g)

@@ -127,6 +127,32 @@ SIGNATURES = {
     "biocfd_sor_batching": [_I],
     "biocfd_momentum_config": [_I, _I],
     "biocfd_ts_pass_config": [_I],
+    "biocfd_ad_reset": [_I],
+    "biocfd_ad_upload": [_I, _I, _P],
+    "biocfd_ad_download": [_I, _I, _P],
+    "biocfd_ad_kin": [_I, _P, _I],
+    "biocfd_ad_ghost_upload": [_I, _I, _P, _L],
+    "biocfd_ad_ghost_download": [_I, _I, _P, _L],
+    "biocfd_tlm_momentum": [_I, _D, _D],
+    "biocfd_adj_momentum": [_I, _D, _D],
+    "biocfd_tlm_velocity_bc": [_I, _D, _D],
+    "biocfd_adj_velocity_bc": [_I, _D, _D],
+    "biocfd_tlm_solid_cell_bc": [_I],
+    "biocfd_adj_solid_cell_bc": [_I],
+    "biocfd_tlm_forcing": [_I, _I, _P, _P],
+    "biocfd_adj_forcing": [_I, _I, _P],
+    "biocfd_tlm_poisson": [_I, _L, _D, _D],
+    "biocfd_adj_poisson": [_I, _L, _D, _D],
+    "biocfd_stress_sums": [_I, _D, _D, _P, _P],
+    "biocfd_tlm_stress": [_I, _D, _D, _P, _P, _P],
+    "biocfd_adj_stress": [_I, _D, _D, _P, _P],
+    "biocfd_tlm_step_start": [_I, _L, _D, _D, _D, _P, _P],
+    "biocfd_tlm_step_end": [_I, _P, _P],
+    "biocfd_adj_step_end": [_I, _P],
+    "biocfd_adj_step_start": [_I, _L, _D, _D, _D, _P],
+    "biocfd_snapshot_save": [_I, _I],
+    "biocfd_snapshot_restore": [_I, _I],
+    "biocfd_snapshot_free": [_I, _I],
 }
 
 

@@ -241,6 +241,67 @@ int biocfd_projection_adjoint(int blk, int64_t n_iter, double dt, const double *
 int biocfd_coarse_update_adjoint(int itf, int comp, double *lam_coarse, double *lam_fine);
 int biocfd_fine_update_bd_adjoint(int itf, int comp, double *lam_fine, double *lam_coarse);
 
+/* The tangent-linear and adjoint time step of a single block with a body
+ * (no reference counterpart), FP64, geometry frozen: IB/solid cells, nearest
+ * elements, distances, normals and image-point cubes are the forward's at
+ * the moment of the call, and the momentum stencil is differentiated at the
+ * block's current u, v, w, p, including the thin-surface (TS) pass's ghost
+ * substitutions and restores.  The sensitivities live on the device per
+ * block: one array per field BF_U .. BF_P (u, v, w, ut, vt, wt, p) and per
+ * TS ghost array (Blocks.f90:55-58), tangents for the tlm calls, adjoints
+ * for the adj calls; the adj calls add the kinematic adjoint (the gradient with
+ * respect to the kin vector of the forcing and stress calls) into a device
+ * accumulator that biocfd_ad_kin reads.
+ *
+ * Forward pieces differentiated (a step of main.F90:133-263 in the drivers'
+ * order): nsMomentum2order (NavierStokes.f90:252-912), velocityBC
+ * (BoundaryConditions.f90:11-74), solidCellBC (:76-92), velocityForcing1,
+ * pressureForcing1, velocityForcingField, pressureForcingField
+ * (Forcing.f90:14-1075), poissonSolver on one block with a fixed SOR
+ * count n_iter = pcItaMax + 1 (epsi 0; PcorVcor.F90:22-220), and the force
+ * sums of stress_calculation (StressCalc.f90:14-208).  routine: 0
+ * velocityForcing1, 1 pressureForcing1, 2 velocityForcingField, 3
+ * pressureForcingField, 4 velocityForcingGhost, 5 pressureForcingGhost.
+ * Step halves: start = momentum, velocityBC, solidCellBC, velocityForcing1,
+ * velocityBC, poissonSolver, pressureForcing1 (at the geometry and
+ * kinematics of the step's start); end = velocityForcingField,
+ * pressureForcingField, velocityForcingGhost, pressureForcingGhost (after
+ * the body moved). */
+int biocfd_ad_reset(int blk);                      /* zero the sensitivities and the kin accumulator */
+int biocfd_ad_upload(int blk, int field, const double *host);
+int biocfd_ad_download(int blk, int field, double *host);
+int biocfd_ad_kin(int blk, double kin_bar[BK_N], int clear);
+/* the sensitivities of TS ghost array `which` (BGH_*), first `count` entries */
+int biocfd_ad_ghost_upload(int blk, int which, const double *host, int64_t count);
+int biocfd_ad_ghost_download(int blk, int which, double *host, int64_t count);
+int biocfd_tlm_momentum(int blk, double dt, double re);
+int biocfd_adj_momentum(int blk, double dt, double re);
+int biocfd_tlm_velocity_bc(int blk, double dt, double uc);
+int biocfd_adj_velocity_bc(int blk, double dt, double uc);
+int biocfd_tlm_solid_cell_bc(int blk);
+int biocfd_adj_solid_cell_bc(int blk);
+int biocfd_tlm_forcing(int blk, int routine, const double kin[BK_N], const double kin_dot[BK_N]);
+int biocfd_adj_forcing(int blk, int routine, const double kin[BK_N]);
+int biocfd_tlm_poisson(int blk, int64_t n_iter, double dt, double uc);
+int biocfd_adj_poisson(int blk, int64_t n_iter, double dt, double uc);
+/* the four element sums stress_calculation divides by its projected areas:
+ * viscous x, pressure x, viscous y, pressure y (StressCalc.f90:173-179) */
+int biocfd_stress_sums(int blk, double mu_f, double rho_f, const double kin[BK_N], double out[4]);
+int biocfd_tlm_stress(int blk, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
+                      double out[4]);
+int biocfd_adj_stress(int blk, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4]);
+int biocfd_tlm_step_start(int blk, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N],
+                          const double kin_dot[BK_N]);
+int biocfd_tlm_step_end(int blk, const double kin[BK_N], const double kin_dot[BK_N]);
+int biocfd_adj_step_end(int blk, const double kin[BK_N]);
+int biocfd_adj_step_start(int blk, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N]);
+/* Step checkpoints: the device state a time step reads and writes (fields
+ * u .. p, tags, IB / TS / solid lists, element and node data, counts) copied
+ * into slot `slot` (0..4095, device memory) and back; free: slot -1 = all. */
+int biocfd_snapshot_save(int blk, int slot);
+int biocfd_snapshot_restore(int blk, int slot);
+int biocfd_snapshot_free(int blk, int slot);
+
 /* ---- immersed-boundary geometry, search and tagging (Search.F90) ---- */
 /* computeSurfaceNorm :210-258 (nodes are the moved positions xnode1..) */
 int biocfd_compute_surface_norm(int blk, const double *xnode1, const double *ynode1,

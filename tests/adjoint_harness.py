@@ -1,7 +1,10 @@
 """Test-only torch.autograd wrappers of the library's FP64 projection and
 interface updates, with the library's hand-written transposes as their
 backward (include/biocfd_hip.h, biocfd_projection_adjoint,
-biocfd_coarse_update_adjoint, biocfd_fine_update_bd_adjoint).  Fields are
+biocfd_coarse_update_adjoint, biocfd_fine_update_bd_adjoint), and of the
+whole adjoint workload: the time-averaged body x-force as a function of the
+body's kinematic parameters, with the adjoint time steps as its backward
+(biocfd_amd.adjoint.AdjointRun).  Fields are
 flat float64 CPU tensors in the block layout (i fastest, one ghost layer);
 the work runs on the GPU through the C ABI."""
 import numpy as np
@@ -92,3 +95,21 @@ class Prolongation(torch.autograd.Function):
         lc = np.zeros(ctx.nc)
         H.call("biocfd_fine_update_bd_adjoint", ctx.itf, ctx.comp, H.ptr(lf), H.ptr(lc))
         return torch.from_numpy(lf), torch.from_numpy(lc), None, None, None, None
+
+
+class BodyForce(torch.autograd.Function):
+    """theta (heave amplitude, pitch amplitude, phase, frequency) -> J, the
+    mean body x-force over the run's averaging window at frozen geometry:
+    forward = AdjointRun.forward with checkpoints, backward = the adjoint
+    time steps replayed from them (AdjointRun.backward)."""
+
+    @staticmethod
+    def forward(ctx, theta, run):
+        J = run.forward(theta.detach().numpy(), checkpoints=True)
+        ctx.run = run
+        return torch.tensor(J, dtype=torch.float64)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        g = torch.from_numpy(ctx.run.backward())
+        return gJ * g, None
