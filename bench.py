@@ -326,10 +326,11 @@ def uneven_main(args):
 def adjoint_summary(args, fd: bool = True):
     """The adjoint workload (96^3 heaving, pitching ellipsoid, 40 steps, FP64,
     fixed 501 SOR iterations per solve): J = mean body x-force over the last
-    20 steps and dJ/d(heave amplitude, pitch amplitude, phase, frequency) at
-    frozen geometry by the adjoint time steps with checkpoints every 8 steps;
-    wall time of the forward alone and of forward + backward, and central
-    differences of J (h = 1e-6) beside the gradient."""
+    20 steps, dJ/d(heave amplitude, pitch amplitude, phase, frequency) and
+    dJ/d(body-frame marker nodes) of the run (the moving geometry included)
+    by the adjoint time steps with checkpoints every 8 steps; wall time of
+    the forward alone and of forward + backward; central differences of J
+    (h = 1e-6) for the four parameters and along a random node direction."""
     from biocfd_amd import cases as CS
     from biocfd_amd import hip as H
     from biocfd_amd.adjoint import AdjointRun, THETA_NAMES
@@ -345,31 +346,67 @@ def adjoint_summary(args, fd: bool = True):
         H.call("biocfd_synchronize")
         t_f = time.perf_counter() - t0
         t0 = time.perf_counter()
-        J2, grad = run.gradient()
+        J2, (grad, gnodes) = run.gradient()
         H.call("biocfd_synchronize")
         t_fb = time.perf_counter() - t0
         out = {
             "workload": f"adjoint{args.adjoint_n}: single block {args.adjoint_n}^3 on [0,3]^3, ellipsoid 0.4:0.2:0.1 "
                         f"({args.adjoint_markers} markers, ID 51) at (1,1.5,1.5), heave 0.2, pitch 20 deg, phase "
                         f"90 deg, f=0.5, Re=100, dt=1e-3, {args.adjoint_steps} steps, {run.n_iter} SOR iterations "
-                        f"per solve; J = mean x-force over the last {run.m} steps; geometry frozen",
+                        f"per solve; J = mean x-force over the last {run.m} steps; gradient of the run (moving "
+                        f"geometry included, discrete structure of the run)",
             "cells": case.cells(), "steps": args.adjoint_steps, "checkpoint_every": run.every, "dtype": "f64",
             "J": J, "gradient": dict(zip(THETA_NAMES, (float(x) for x in grad))),
+            "node_gradient_norm": float(np.linalg.norm(gnodes)), "nodes": int(run.st.nn),
             "forward_s": round(t_f, 3), "forward_backward_s": round(t_fb, 3),
             "backward_over_forward": round(t_fb / t_f, 2),
             "forward_mcell_steps_per_s": round(case.cells() * args.adjoint_steps / t_f / 1e6, 2),
             "forward_bit_identical_in_gradient_run": bool(J == J2),
         }
         if fd:
-            h = 1e-6
-            fdv = []
-            for q in range(len(grad)):
+            # the run is smooth between changes of its discrete structure (tags,
+            # IB cells, nearest elements): a difference is the derivative only
+            # where neither perturbed run changes it, so each difference reports
+            # the steps whose structure differs from the base run's
+            base = []
+            run.forward(states=base)
+
+            def central(h, theta_dir=None, node_dir=None):
+                sp, sm = [], []
+                if node_dir is None:
+                    jp = run.forward(run.theta0 + h * theta_dir, states=sp)
+                    jm = run.forward(run.theta0 - h * theta_dir, states=sm)
+                else:
+                    n0 = run.nodes0
+                    jp = run.forward(nodes=tuple(a + h * d for a, d in zip(n0, node_dir)), states=sp)
+                    jm = run.forward(nodes=tuple(a - h * d for a, d in zip(n0, node_dir)), states=sm)
+                changed = sum(int(a != b) for a, b in zip(sp, base)) + sum(int(a != b) for a, b in zip(sm, base))
+                return (jp - jm) / (2 * h), changed
+
+            rng = np.random.default_rng(7)
+            D = [rng.standard_normal(run.st.nn) * 0.1 for _ in range(3)]
+            adj_nodes = float(sum(np.dot(gnodes[:, q], D[q]) for q in range(3)))
+            checks = {}
+            for q, name in enumerate(THETA_NAMES):
                 e = np.zeros(len(grad))
-                e[q] = h
-                fdv.append((run.forward(run.theta0 + e) - run.forward(run.theta0 - e)) / (2 * h))
-            fdv = np.array(fdv)
-            out["central_differences"] = dict(zip(THETA_NAMES, (float(x) for x in fdv)))
-            out["max_rel_diff_vs_fd"] = float(np.max(np.abs(grad - fdv)) / max(np.max(np.abs(fdv)), 1e-300))
+                e[q] = 1.0
+                _, tl = run.tangent(e)
+                row = {"adjoint": float(grad[q]), "tangent_linear": tl}
+                for h in (1e-6, 1e-9):
+                    v, ch = central(h, theta_dir=e)
+                    row[f"central_h{h:g}"] = v
+                    row[f"steps_with_changed_structure_h{h:g}"] = ch
+                checks[name] = row
+            _, tl = run.tangent(np.zeros(len(grad)), dnodes=D)
+            row = {"adjoint": adj_nodes, "tangent_linear": tl}
+            for h in (1e-6, 1e-9):
+                v, ch = central(h, node_dir=D)
+                row[f"central_h{h:g}"] = v
+                row[f"steps_with_changed_structure_h{h:g}"] = ch
+            checks["random node direction"] = row
+            out["checks"] = checks
+            out["max_rel_diff_adjoint_vs_tangent"] = max(
+                abs(r["adjoint"] - r["tangent_linear"]) / max(abs(r["tangent_linear"]), 1e-300) for r in checks.values())
         return out
     finally:
         run.close()

@@ -154,6 +154,16 @@ struct MoveState {
 // (FP64, block layout): s[q] for the fields BF_U .. BF_P (u, v, w, ut, vt,
 // wt, p), the ghost arrays', two scratch arrays of the transposed projection, the per-task
 // adjoints of the forcing routines and a kinematic accumulator (BK_N).
+// What one forcing call read, kept while recording: the arrays it
+// interpolates at the call's start (p or ut, vt, wt) and, for the in-place
+// routines, its task outputs.
+struct ForceRec {
+  double *arr[4] = {};
+  double *tout = nullptr;
+  int64_t cap_tout = 0, ntask = 0;
+  bool live = false;
+};
+
 struct AdWork {
   double *s[7] = {};
   double *gh[BGH_N] = {};   // the thin-surface ghost arrays' sensitivities (cap_gh entries each)
@@ -162,6 +172,16 @@ struct AdWork {
   double *ybar = nullptr;
   int64_t cap_ybar = 0;
   double *kin = nullptr;
+  double *addend = nullptr;  // per-task geometric tangents of a forcing call
+  int64_t cap_addend = 0;
+  // geometric sensitivities per element, 7 x n_elems each (centroid x, y, z,
+  // normal x, y, z, element length): adjoints (gbar) and tangents (gdot);
+  // node adjoints, 3 x n_nodes
+  double *gbar = nullptr, *gdot = nullptr, *nbar = nullptr;
+  bool gdot_set = false;
+  // forcing-call records (slot * 6 + routine) and the slot being recorded
+  std::vector<ForceRec> recs;
+  int rec_slot = -1;
 };
 
 struct Block {
@@ -293,11 +313,15 @@ int ad_ready(Block &b);
 int ad_ghosts(Block &b);
 int momentum_tlm(Block &b, double dt, double re);
 int momentum_adj(Block &b, double dt, double re);
-int forcing_tlm(Block &b, int routine, const double kin[BK_N], const double kin_dot[BK_N]);
-int forcing_adj(Block &b, int routine, const double kin[BK_N]);
-int stress_tlm(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
+// rec: the record slot of the forward call whose values the geometric part
+// reads (-1: geometry frozen)
+int forcing_tlm(Block &b, int routine, const double kin[BK_N], const double kin_dot[BK_N], int rec);
+int forcing_adj(Block &b, int routine, const double kin[BK_N], int rec);
+int stress_tlm(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N], int geom,
                double out[4]);
-int stress_adj(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double cbar[4]);
+int stress_adj(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double cbar[4], int geom);
+int ad_geometry(Block &b);
+void ad_release_records(Block &b);
 int compute_div(Block &b);
 int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi);
 int correct(Block &b, double dt);

@@ -732,7 +732,85 @@ bool fp64_only() {
 
 enum { AD_V1 = 0, AD_P1 = 1, AD_VF = 2, AD_PF = 3, AD_VG = 4, AD_PG = 5 };
 
+// An element's centroid c = (p1 + p2 + p3) / 3, normal n = inor m / |m| and
+// length |m|, m = (p2 - p1) x (p3 - p1) (computeSurfaceNorm,
+// Search.F90:210-258): tangents from node tangents ...
+__global__ void k_geom_tlm(int64_t ne, int64_t nn, const int *e1, const int *e2, const int *e3, const double *x,
+                           const double *y, const double *z, const double *xd, const double *yd, const double *zd,
+                           double inor, double *gdot) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int a = e1[e] - 1, b = e2[e] - 1, c = e3[e] - 1;
+  const double p1[3] = {x[a], y[a], z[a]}, p2[3] = {x[b], y[b], z[b]}, p3[3] = {x[c], y[c], z[c]};
+  const double d1[3] = {xd[a], yd[a], zd[a]}, d2[3] = {xd[b], yd[b], zd[b]}, d3[3] = {xd[c], yd[c], zd[c]};
+  double u[3], v[3], du[3], dv[3], m[3], dm[3];
+  for (int q = 0; q < 3; ++q) {
+    u[q] = p2[q] - p1[q]; v[q] = p3[q] - p1[q];
+    du[q] = d2[q] - d1[q]; dv[q] = d3[q] - d1[q];
+    gdot[q * ne + e] = (d1[q] + d2[q] + d3[q]) / 3.0;
+  }
+  for (int q = 0; q < 3; ++q) {
+    const int r = (q + 1) % 3, t = (q + 2) % 3;
+    m[q] = u[r] * v[t] - u[t] * v[r];
+    dm[q] = du[r] * v[t] - du[t] * v[r] + u[r] * dv[t] - u[t] * dv[r];
+  }
+  const double L = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const double dL = (m[0] * dm[0] + m[1] * dm[1] + m[2] * dm[2]) / L;
+  for (int q = 0; q < 3; ++q) gdot[(3 + q) * ne + e] = inor * (dm[q] / L - m[q] * dL / (L * L));
+  gdot[6 * ne + e] = dL;
+}
+
+// ... and node adjoints from their adjoints (added atomically: nodes are shared)
+__global__ void k_geom_adj(int64_t ne, int64_t nn, const int *e1, const int *e2, const int *e3, const double *x,
+                           const double *y, const double *z, double inor, const double *gbar, double *nbar) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int a = e1[e] - 1, b = e2[e] - 1, c = e3[e] - 1;
+  const double p1[3] = {x[a], y[a], z[a]}, p2[3] = {x[b], y[b], z[b]}, p3[3] = {x[c], y[c], z[c]};
+  double u[3], v[3], m[3], cb[3], nb[3];
+  for (int q = 0; q < 3; ++q) {
+    u[q] = p2[q] - p1[q]; v[q] = p3[q] - p1[q];
+    cb[q] = gbar[q * ne + e];
+    nb[q] = gbar[(3 + q) * ne + e];
+  }
+  const double Lb = gbar[6 * ne + e];
+  for (int q = 0; q < 3; ++q) {
+    const int r = (q + 1) % 3, t = (q + 2) % 3;
+    m[q] = u[r] * v[t] - u[t] * v[r];
+  }
+  const double L = sqrt(m[0] * m[0] + m[1] * m[1] + m[2] * m[2]);
+  const double mn = m[0] * nb[0] + m[1] * nb[1] + m[2] * nb[2];
+  double mb[3], ub[3], vb[3];
+  for (int q = 0; q < 3; ++q) mb[q] = inor * (nb[q] / L - m[q] * mn / (L * L * L)) + m[q] * Lb / L;
+  // m = u x v: ubar = v x mbar, vbar = mbar x u
+  for (int q = 0; q < 3; ++q) {
+    const int r = (q + 1) % 3, t = (q + 2) % 3;
+    ub[q] = v[r] * mb[t] - v[t] * mb[r];
+    vb[q] = mb[r] * u[t] - mb[t] * u[r];
+  }
+  for (int q = 0; q < 3; ++q) {
+    const double c3 = cb[q] / 3.0;
+    atomicAdd(&nbar[q * nn + a], c3 - ub[q] - vb[q]);
+    atomicAdd(&nbar[q * nn + b], c3 + ub[q]);
+    atomicAdd(&nbar[q * nn + c], c3 + vb[q]);
+  }
+}
+
 }  // namespace
+
+int ad_geometry(Block &b) {
+  AdWork &w = b.ad;
+  if (w.gbar) return 0;
+  if (!b.ib.present) { set_error("block has no immersed body"); return -1; }
+  const int64_t ne = b.ib.n_elems, nn = b.ib.n_nodes;
+  if (dev_alloc(&w.gbar, 7 * ne) || dev_alloc(&w.gdot, 7 * ne) || dev_alloc(&w.nbar, 3 * nn)) return -1;
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_ad_fill, dim3(div_up(7 * ne, 256)), dim3(256), 0, s, w.gbar, 7 * ne, 0.0);
+  hipLaunchKernelGGL(k_ad_fill, dim3(div_up(7 * ne, 256)), dim3(256), 0, s, w.gdot, 7 * ne, 0.0);
+  hipLaunchKernelGGL(k_ad_fill, dim3(div_up(3 * nn, 256)), dim3(256), 0, s, w.nbar, 3 * nn, 0.0);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
 
 int ad_ready(Block &b) {
   AdWork &w = b.ad;
@@ -757,6 +835,9 @@ void ad_release(Block &b) {
   for (auto &p : w.gh) if (p) { (void)hipFree(p); p = nullptr; }
   if (w.ybar) (void)hipFree(w.ybar);
   if (w.kin) (void)hipFree(w.kin);
+  for (double *p : {w.addend, w.gbar, w.gdot, w.nbar})
+    if (p) (void)hipFree(p);
+  ad_release_records(b);
   w = AdWork();
   for (int id = 1; id <= kMaxBlocks; ++id)
     if (&rt().blk[id] == &b) snapshot_free(id, -1);
@@ -777,6 +858,9 @@ int biocfd_ad_reset(int id) {
     if (b->ad.gh[q])
       hipLaunchKernelGGL(k_ad_fill, dim3(div_up(b->ad.cap_gh, 256)), dim3(256), 0, s, b->ad.gh[q], b->ad.cap_gh, 0.0);
   hipLaunchKernelGGL(k_ad_fill, dim3(1), dim3(64), 0, s, b->ad.kin, (int64_t)BK_N, 0.0);
+  if (b->ad.gbar)
+    hipLaunchKernelGGL(k_ad_fill, dim3(div_up(7 * b->ib.n_elems, 256)), dim3(256), 0, s, b->ad.gbar,
+                       7 * b->ib.n_elems, 0.0);
   BCFD_CHECK_LAUNCH();
   return 0;
 }
@@ -875,20 +959,20 @@ int biocfd_adj_solid_cell_bc(int id) {
   return b ? solid_bc_adj(*b) : -1;
 }
 
-int biocfd_tlm_forcing(int id, int routine, const double kin[BK_N], const double kin_dot[BK_N]) {
+int biocfd_tlm_forcing(int id, int routine, const double kin[BK_N], const double kin_dot[BK_N], int rec) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
   if (!b) return -1;
   if (routine < AD_V1 || routine > AD_PG) { set_error("tlm_forcing: routine %d outside 0..5", routine); return -1; }
-  return forcing_tlm(*b, routine, kin, kin_dot);
+  return forcing_tlm(*b, routine, kin, kin_dot, rec);
 }
 
-int biocfd_adj_forcing(int id, int routine, const double kin[BK_N]) {
+int biocfd_adj_forcing(int id, int routine, const double kin[BK_N], int rec) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
   if (!b) return -1;
   if (routine < AD_V1 || routine > AD_PG) { set_error("adj_forcing: routine %d outside 0..5", routine); return -1; }
-  return forcing_adj(*b, routine, kin);
+  return forcing_adj(*b, routine, kin, rec);
 }
 
 int biocfd_tlm_poisson(int id, int64_t n_iter, double dt, double uc) {
@@ -912,57 +996,104 @@ int biocfd_stress_sums(int id, double mu_f, double rho_f, const double kin[BK_N]
 }
 
 int biocfd_tlm_stress(int id, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
-                      double out[4]) {
+                      int geom, double out[4]) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
-  return b ? stress_tlm(*b, mu_f, rho_f, kin, kin_dot, out) : -1;
+  return b ? stress_tlm(*b, mu_f, rho_f, kin, kin_dot, geom, out) : -1;
 }
 
-int biocfd_adj_stress(int id, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4]) {
+int biocfd_adj_stress(int id, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4], int geom) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
-  return b ? stress_adj(*b, mu_f, rho_f, kin, sbar) : -1;
+  return b ? stress_adj(*b, mu_f, rho_f, kin, sbar, geom) : -1;
 }
 
 int biocfd_tlm_step_start(int id, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N],
-                          const double kin_dot[BK_N]) {
+                          const double kin_dot[BK_N], int rec) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
   if (!b || !fp64_only()) return -1;
   if (momentum_tlm(*b, dt, re) || velocity_bc_tlm(*b, dt, uc) || solid_bc_tlm(*b) ||
-      forcing_tlm(*b, AD_V1, kin, kin_dot) || velocity_bc_tlm(*b, dt, uc) || poisson_tlm(*b, n_iter, dt, uc) ||
-      forcing_tlm(*b, AD_P1, kin, kin_dot))
+      forcing_tlm(*b, AD_V1, kin, kin_dot, rec) || velocity_bc_tlm(*b, dt, uc) || poisson_tlm(*b, n_iter, dt, uc) ||
+      forcing_tlm(*b, AD_P1, kin, kin_dot, rec))
     return -1;
   return 0;
 }
 
-int biocfd_tlm_step_end(int id, const double kin[BK_N], const double kin_dot[BK_N]) {
+int biocfd_tlm_step_end(int id, const double kin[BK_N], const double kin_dot[BK_N], int rec) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
   if (!b) return -1;
-  if (forcing_tlm(*b, AD_VF, kin, kin_dot) || forcing_tlm(*b, AD_PF, kin, kin_dot) ||
-      forcing_tlm(*b, AD_VG, kin, kin_dot) || forcing_tlm(*b, AD_PG, kin, kin_dot))
+  if (forcing_tlm(*b, AD_VF, kin, kin_dot, rec) || forcing_tlm(*b, AD_PF, kin, kin_dot, rec) ||
+      forcing_tlm(*b, AD_VG, kin, kin_dot, rec) || forcing_tlm(*b, AD_PG, kin, kin_dot, rec))
     return -1;
   return 0;
 }
 
-int biocfd_adj_step_end(int id, const double kin[BK_N]) {
+int biocfd_adj_step_end(int id, const double kin[BK_N], int rec) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
   if (!b) return -1;
-  if (forcing_adj(*b, AD_PG, kin) || forcing_adj(*b, AD_VG, kin) || forcing_adj(*b, AD_PF, kin) ||
-      forcing_adj(*b, AD_VF, kin))
+  if (forcing_adj(*b, AD_PG, kin, rec) || forcing_adj(*b, AD_VG, kin, rec) || forcing_adj(*b, AD_PF, kin, rec) ||
+      forcing_adj(*b, AD_VF, kin, rec))
     return -1;
   return 0;
 }
 
-int biocfd_adj_step_start(int id, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N]) {
+int biocfd_adj_step_start(int id, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N],
+                          int rec) {
   BCFD_API_LOCK();
   Block *b = ad_block(id);
   if (!b || !fp64_only()) return -1;
-  if (forcing_adj(*b, AD_P1, kin) || poisson_adj(*b, n_iter, dt, uc) || velocity_bc_adj(*b, dt, uc) ||
-      forcing_adj(*b, AD_V1, kin) || solid_bc_adj(*b) || velocity_bc_adj(*b, dt, uc) || momentum_adj(*b, dt, re))
+  if (forcing_adj(*b, AD_P1, kin, rec) || poisson_adj(*b, n_iter, dt, uc) || velocity_bc_adj(*b, dt, uc) ||
+      forcing_adj(*b, AD_V1, kin, rec) || solid_bc_adj(*b) || velocity_bc_adj(*b, dt, uc) ||
+      momentum_adj(*b, dt, re))
     return -1;
+  return 0;
+}
+
+int biocfd_ad_record(int id, int slot) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b) return -1;
+  if (slot < -1 || slot >= 4096) { set_error("ad_record: slot %d outside -1..4095", slot); return -1; }
+  b->ad.rec_slot = slot;
+  return 0;
+}
+
+int biocfd_ad_geometry_tangent(int id, double inor, const double *xdot, const double *ydot, const double *zdot) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || ad_geometry(*b)) return -1;
+  IbData &ib = b->ib;
+  const int64_t ne = ib.n_elems, nn = ib.n_nodes;
+  double *nd = b->ad.nbar;   // the node tangents travel through the node-adjoint buffer
+  if (upload(nd, xdot, nn) || upload(nd + nn, ydot, nn) || upload(nd + 2 * nn, zdot, nn)) return -1;
+  hipLaunchKernelGGL(k_geom_tlm, dim3(div_up(ne, 256)), dim3(256), 0, rt().stream, ne, nn, ib.el[0], ib.el[1],
+                     ib.el[2], ib.node[0], ib.node[1], ib.node[2], nd, nd + nn, nd + 2 * nn, inor, b->ad.gdot);
+  hipLaunchKernelGGL(k_ad_fill, dim3(div_up(3 * nn, 256)), dim3(256), 0, rt().stream, nd, 3 * nn, 0.0);
+  BCFD_CHECK_LAUNCH();
+  b->ad.gdot_set = true;
+  BCFD_HIP(hipStreamSynchronize(rt().stream));
+  return 0;
+}
+
+int biocfd_ad_geometry_adjoint(int id, double inor, double *xbar, double *ybar, double *zbar, int clear) {
+  BCFD_API_LOCK();
+  Block *b = ad_block(id);
+  if (!b || ad_geometry(*b)) return -1;
+  IbData &ib = b->ib;
+  const int64_t ne = ib.n_elems, nn = ib.n_nodes;
+  double *nb = b->ad.nbar;
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_ad_fill, dim3(div_up(3 * nn, 256)), dim3(256), 0, s, nb, 3 * nn, 0.0);
+  hipLaunchKernelGGL(k_geom_adj, dim3(div_up(ne, 256)), dim3(256), 0, s, ne, nn, ib.el[0], ib.el[1], ib.el[2],
+                     ib.node[0], ib.node[1], ib.node[2], inor, b->ad.gbar, nb);
+  BCFD_CHECK_LAUNCH();
+  if (download(xbar, nb, nn) || download(ybar, nb + nn, nn) || download(zbar, nb + 2 * nn, nn)) return -1;
+  if (clear) hipLaunchKernelGGL(k_ad_fill, dim3(div_up(7 * ne, 256)), dim3(256), 0, s, b->ad.gbar, 7 * ne, 0.0);
+  BCFD_CHECK_LAUNCH();
+  BCFD_HIP(hipStreamSynchronize(s));
   return 0;
 }
 

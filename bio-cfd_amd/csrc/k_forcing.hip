@@ -17,6 +17,7 @@
 // run in dependency waves until all are done; results are scattered at the
 // end, the last writer of each location winning.
 #include "biocfd_rt.hpp"
+#include "dual.hpp"
 
 namespace bcfd {
 
@@ -42,6 +43,7 @@ struct FArgs {
   // terms along kin_dot (the fields passed are tangents too)
   int tangent;
   double kin_dot[BK_N];
+  const double *addend;   // tangent mode: per-task geometric tangent added to the output (or null)
 };
 
 struct PartSpec {
@@ -102,22 +104,31 @@ __device__ inline int find_index(double v, const double *arr, int len, int s, in
   return s;
 }
 
-__device__ inline double bilin(double x, double y, double x1, double x2, double y1, double y2, double f1,
-                               double f2, double f3, double f4) {
-  const double numer = f1 * (x2 - x) * (y2 - y) + f3 * (x2 - x) * (y - y1) + f2 * (x - x1) * (y2 - y) +
-                       f4 * (x - x1) * (y - y1);
+// The interpolation helpers are generic in their scalar type (double, or
+// Dual for the derivatives with respect to the image point and the values).
+template <class T>
+__device__ inline T bilin(T x, T y, double x1, double x2, double y1, double y2, T f1, T f2, T f3, T f4) {
+  const T numer = f1 * (x2 - x) * (y2 - y) + f3 * (x2 - x) * (y - y1) + f2 * (x - x1) * (y2 - y) +
+                  f4 * (x - x1) * (y - y1);
   const double denom = (x2 - x1) * (y2 - y1);
   return numer / denom;
 }
 
-__device__ inline double lin(double x, double x0, double x1, double y0, double y1) {
+template <class T>
+__device__ inline T lin(T x, double x0, double x1, T y0, T y1) {
   return y0 + (x - x0) * (y1 - y0) / (x1 - x0);
 }
 
-__device__ inline double lagrange_d(double x, double x2, double x1, double px, double px2, double px1) {
-  const double h2 = fabs(x2 - x), h1 = fabs(x1 - x);
+template <class T>
+__device__ inline T lagrange_d(T x, double x2, double x1, T px, T px2, T px1) {
+  const T h2 = fabs(x2 - x), h1 = fabs(x1 - x);
   return (h1 * h1 * px2 - h2 * h2 * px1 + (h2 * h2 - h1 * h1) * px) / (h1 * h2 * (h1 + h2) + 1e-16);
 }
+
+template <class T> __device__ inline T as(double x);
+template <> __device__ inline double as<double>(double x) { return x; }
+template <> __device__ inline Dual as<Dual>(double x) { return {x, 0.0}; }
+template <class T> __device__ inline T as(Dual x) { return x; }
 
 // Reads of the interpolated array, serial-order aware for the in-place routines.
 struct Reader {
@@ -163,53 +174,57 @@ struct Reader {
   }
 };
 
-// compute_value_and_derivatives (Forcing.f90:1091-1161)
-template <class R>
-__device__ inline void value_derivs(const R &rd, double x, double y, double z, int i, int j, int k,
-                                    const double *xg, const double *yg, const double *zg, int off,
-                                    double &val, double dv[3]) {
+// compute_value_and_derivatives (Forcing.f90:1091-1161); the reader rd
+// returns the cube's values (double, or Dual where their derivatives are
+// wanted)
+template <class R, class T = double>
+__device__ inline void value_derivs(const R &rd, T x, T y, T z, int i, int j, int k, const double *xg,
+                                    const double *yg, const double *zg, int off, T &val, T dv[3]) {
   int ii = i, jj = j, kk = k;
   if (off == 1) ii = i - 1;
   if (off == 2) jj = j - 1;
   if (off == 3) kk = k - 1;
-  const double v000 = rd(ii, jj, kk), v100 = rd(ii + 1, jj, kk), v010 = rd(ii, jj + 1, kk);
-  const double v110 = rd(ii + 1, jj + 1, kk), v001 = rd(ii, jj, kk + 1), v101 = rd(ii + 1, jj, kk + 1);
-  const double v011 = rd(ii, jj + 1, kk + 1), v111 = rd(ii + 1, jj + 1, kk + 1);
+  const T v000 = as<T>(rd(ii, jj, kk)), v100 = as<T>(rd(ii + 1, jj, kk)), v010 = as<T>(rd(ii, jj + 1, kk));
+  const T v110 = as<T>(rd(ii + 1, jj + 1, kk)), v001 = as<T>(rd(ii, jj, kk + 1));
+  const T v101 = as<T>(rd(ii + 1, jj, kk + 1)), v011 = as<T>(rd(ii, jj + 1, kk + 1));
+  const T v111 = as<T>(rd(ii + 1, jj + 1, kk + 1));
   const double X0 = xg[i - 1], X1 = xg[i], Y0 = yg[j - 1], Y1 = yg[j], Z0 = zg[k - 1], Z1 = zg[k];
-  const double t11 = bilin(y, z, Y0, Y1, Z0, Z1, v000, v010, v001, v011);
-  const double t12 = bilin(y, z, Y0, Y1, Z0, Z1, v100, v110, v101, v111);
-  const double t21 = bilin(x, z, X0, X1, Z0, Z1, v000, v100, v001, v101);
-  const double t22 = bilin(x, z, X0, X1, Z0, Z1, v010, v110, v011, v111);
-  const double t31 = bilin(x, y, X0, X1, Y0, Y1, v000, v100, v010, v110);
-  const double t32 = bilin(x, y, X0, X1, Y0, Y1, v001, v101, v011, v111);
+  const T t11 = bilin(y, z, Y0, Y1, Z0, Z1, v000, v010, v001, v011);
+  const T t12 = bilin(y, z, Y0, Y1, Z0, Z1, v100, v110, v101, v111);
+  const T t21 = bilin(x, z, X0, X1, Z0, Z1, v000, v100, v001, v101);
+  const T t22 = bilin(x, z, X0, X1, Z0, Z1, v010, v110, v011, v111);
+  const T t31 = bilin(x, y, X0, X1, Y0, Y1, v000, v100, v010, v110);
+  const T t32 = bilin(x, y, X0, X1, Y0, Y1, v001, v101, v011, v111);
   val = lin(x, X0, X1, t11, t12);
   dv[0] = lagrange_d(x, X1, X0, val, t12, t11);
   dv[1] = lagrange_d(y, Y1, Y0, val, t22, t21);
   dv[2] = lagrange_d(z, Z1, Z0, val, t32, t31);
 }
 
-// surface value (velocity) or dp/dn (pressure) of element e
-__device__ inline double surface_term(const FArgs &a, int routine, int part, int comp, int e) {
-  const int sid = a.surf[e];
-  const double *k = a.kin;
+// surface value (velocity) or dp/dn (pressure) of an element of surface
+// sid with centroid c and normal nrm, from the kinematic vector k (generic:
+// the adjoint passes differentiate it in c, nrm and k)
+template <class T>
+__device__ inline T surface_term_T(int routine, int part, int comp, int sid, const T c[3], const T nrm[3],
+                                   const T k[BK_N]) {
   const bool flap = (sid == 51 || sid == 52);
-  const double td = (sid == 51) ? k[BK_THETADOT1] : k[BK_THETADOT2];
-  const double tdd = (sid == 51) ? k[BK_THETADDOT1] : k[BK_THETADDOT2];
-  const double xc = a.el[BE_XCENT][e], yc = a.el[BE_YCENT][e], zc = a.el[BE_ZCENT][e];
+  const T td = (sid == 51) ? k[BK_THETADOT1] : k[BK_THETADOT2];
+  const T tdd = (sid == 51) ? k[BK_THETADDOT1] : k[BK_THETADDOT2];
+  const T xc = c[0], yc = c[1], zc = c[2];
   if (comp == 0) {
-    double ac_z = 0.0, ac_y = 0.0, at_z = 0.0, at_y = 0.0;
+    T ac_z = as<T>(0.0), ac_y = as<T>(0.0), at_z = as<T>(0.0), at_y = as<T>(0.0);
     if (flap) {
       ac_z = -(td * td) * (zc - k[BK_PIVZ]);
       ac_y = -(td * td) * (yc - k[BK_PIVY]);
       at_z = tdd * (yc - k[BK_PIVY]);
       at_y = -tdd * (zc - k[BK_PIVZ]);
     }
-    const double cb = a.el[BE_COSB][e], cg = a.el[BE_COSG][e];
+    const T cb = nrm[1], cg = nrm[2];
     if (routine == R_PG) return ((ac_z + at_z) * cg + (ac_y + at_y) * cb) + (k[BK_YDDOT] * cb);
     return -((ac_z + at_z) * cg + (ac_y + at_y) * cb) - k[BK_YDDOT] * cb;
   }
   if (comp == 1) return k[BK_XDOT];
-  if (comp == 3) return flap ? td * (yc - k[BK_PIVY]) : 0.0;
+  if (comp == 3) return flap ? td * (yc - k[BK_PIVY]) : as<T>(0.0);
   // comp == 2
   const bool alpha = (routine != R_V1);
   if (!flap) {
@@ -218,6 +233,12 @@ __device__ inline double surface_term(const FArgs &a, int routine, int part, int
   }
   if (!alpha) return -td * (zc - k[BK_PIVZ]) + k[BK_YDOT];
   return -td * (zc - k[BK_PIVZ]) + (-k[BK_ALPHADOT]) * (xc - k[BK_PIVX]) + k[BK_YDOT];
+}
+
+__device__ inline double surface_term(const FArgs &a, int routine, int part, int comp, int e) {
+  const double c[3] = {a.el[BE_XCENT][e], a.el[BE_YCENT][e], a.el[BE_ZCENT][e]};
+  const double nrm[3] = {a.el[BE_COSA][e], a.el[BE_COSB][e], a.el[BE_COSG][e]};
+  return surface_term_T<double>(routine, part, comp, a.surf[e], c, nrm, a.kin);
 }
 
 // gradient of surface_term with respect to the kinematic vector (the
@@ -330,20 +351,54 @@ __device__ inline void task_geom(const FArgs &a, int routine, int64_t cell_ord, 
 
 // the quadratic profile's value at the node from the image-point value and
 // derivatives and the surface term
-__device__ inline double task_value(const TaskGeom &t, double val, const double dv[3], double surfv) {
-  double dn = dv[0] * t.ca + dv[1] * t.cb + dv[2] * t.cg;
-  if (t.ghost) dn = -1.0 * dn;
-  const double n1 = t.n1, sur2 = t.sur2;
-  if (t.s.comp == 0) {
-    const double bval = surfv;
-    const double aval = (dn - surfv) / (2.0 * n1);
-    const double cval = val - (dn + surfv) * n1 * 0.5;
+template <class T>
+__device__ inline T profile_T(bool ghost, int comp, T ca, T cb, T cg, T sur2, T n1, T val, const T dv[3], T surfv) {
+  T dn = dv[0] * ca + dv[1] * cb + dv[2] * cg;
+  if (ghost) dn = -1.0 * dn;
+  if (comp == 0) {
+    const T bval = surfv;
+    const T aval = (dn - surfv) / (2.0 * n1);
+    const T cval = val - (dn + surfv) * n1 * 0.5;
     return aval * (sur2 * sur2) + bval * sur2 + cval;
   }
-  const double cval = surfv;
-  const double bval = 2.0 / n1 * (val - surfv) - dn;
-  const double aval = dn / n1 - (val - surfv) / (n1 * n1);
+  const T cval = surfv;
+  const T bval = 2.0 / n1 * (val - surfv) - dn;
+  const T aval = dn / n1 - (val - surfv) / (n1 * n1);
   return aval * (sur2 * sur2) + bval * sur2 + cval;
+}
+
+__device__ inline double task_value(const TaskGeom &t, double val, const double dv[3], double surfv) {
+  return profile_T<double>(t.ghost, t.s.comp, t.ca, t.cb, t.cg, t.sur2, t.n1, val, dv, surfv);
+}
+
+// One reconstruction as a function of its element's centroid c and normal
+// nrm, the kinematic vector kin and the values the reader returns, with the
+// forward's image-point cube (g): the normal distance from the slot's node
+// (k_norm_distance's expression), the image point, the value and derivatives
+// there, the profile.  T = double reproduces run_task; Dual carries one
+// derivative.
+template <class T, class R>
+__device__ inline T task_eval(const FArgs &a, int routine, int part, int64_t cell_ord, const TaskGeom &g,
+                              const T c[3], const T nrm[3], const T kin[BK_N], const R &rd) {
+  const int i = a.ijk[cell_ord], j = a.ijk[cell_ord + a.cap], k = a.ijk[cell_ord + 2 * a.cap];
+  const int slot = g.s.slot;
+  const double ptx = slot == BN_U1 ? a.g[BG_X1][i - 1] : (slot == BN_U2 ? a.g[BG_X1][i] : a.g[BG_XP][i - 1]);
+  const double pty = slot == BN_V1 ? a.g[BG_Y1][j - 1] : (slot == BN_V2 ? a.g[BG_Y1][j] : a.g[BG_YP][j - 1]);
+  const double ptz = slot == BN_W1 ? a.g[BG_Z1][k - 1] : (slot == BN_W2 ? a.g[BG_Z1][k] : a.g[BG_ZP][k - 1]);
+  const T nd = (ptx - c[0]) * nrm[0] + (pty - c[1]) * nrm[1] + (ptz - c[2]) * nrm[2];
+  const T sur2 = g.ghost ? -nd : nd;
+  const double dxi = a.g[BG_DX][i - 1], dyj = a.g[BG_DY][j - 1], dzk = a.g[BG_DZ][k - 1];
+  const T pt1 = 1.5 * sqrt(dxi * dxi + dyj * dyj + dzk * dzk) + (fabs(sur2) - sur2) * 0.5;
+  const PartSpec &s = g.s;
+  const double bx = g.gx[i + s.bx - 1], by = g.gy[j + s.by - 1], bz = g.gz[k + s.bz - 1];
+  T px, py, pz;
+  if (!g.ghost) { px = bx + pt1 * nrm[0]; py = by + pt1 * nrm[1]; pz = bz + pt1 * nrm[2]; }
+  else { px = bx - pt1 * nrm[0]; py = by - pt1 * nrm[1]; pz = bz - pt1 * nrm[2]; }
+  T val, dv[3];
+  value_derivs<R, T>(rd, px, py, pz, g.ix, g.iy, g.iz, g.gx, g.gy, g.gz, s.comp, val, dv);
+  const T n1 = pt1 + sur2;
+  const T surfv = surface_term_T<T>(routine, part, s.comp, a.surf[g.e], c, nrm, kin);
+  return profile_T<T>(g.ghost, s.comp, nrm[0], nrm[1], nrm[2], sur2, n1, val, dv, surfv);
 }
 
 // one reconstruction; returns false when an ordered read is not available yet
@@ -368,6 +423,10 @@ __device__ bool run_task(const FArgs &a, int routine, int64_t cell_ord, int part
     surfv = surface_term(a, routine, part, t.s.comp, t.e);
   }
   out = task_value(t, val, dv, surfv);
+  if (a.tangent && a.addend) {
+    const int nparts = (routine == R_P1 || routine == R_PF || routine == R_PG) ? 1 : 6;
+    out += a.addend[cell_ord * nparts + part];
+  }
   return true;
 }
 
@@ -443,6 +502,43 @@ __global__ void k_force_direct(FArgs a, int routine, int nparts, double *f0, dou
     ghost[s.ghost][n] = out;
     ghost[s.ghost_t][n] = a.plain[s.comp][loc];
   }
+}
+
+// the components a routine interpolates: p (0) or ut, vt, wt (1..3)
+__host__ __device__ inline bool reads_comp(int routine, int comp) {
+  const bool pressure = (routine == R_P1 || routine == R_PF || routine == R_PG);
+  return pressure ? comp == 0 : comp != 0;
+}
+
+// Recording (the adjoint driver replays a step with it): a copy of the
+// arrays the routine interpolates, taken before it runs, in the record of
+// the current slot
+static ForceRec *record_inputs(Block &b, int routine) {
+  AdWork &w = b.ad;
+  const size_t idx = (size_t)w.rec_slot * 6 + routine;
+  if (w.recs.size() <= idx) w.recs.resize(idx + 1);
+  ForceRec &r = w.recs[idx];
+  const double *src[4] = {b.f[BF_P], b.f[BF_UT], b.f[BF_VT], b.f[BF_WT]};
+  for (int c = 0; c < 4; ++c) {
+    if (!reads_comp(routine, c)) continue;
+    if (!r.arr[c] && dev_alloc(&r.arr[c], b.n2)) return nullptr;
+    if (hipMemcpyAsync(r.arr[c], src[c], sizeof(double) * b.n2, hipMemcpyDeviceToDevice, rt().stream) !=
+        hipSuccess) {
+      set_error("forcing record: copy failed");
+      return nullptr;
+    }
+  }
+  r.live = false;
+  return &r;
+}
+
+void ad_release_records(Block &b) {
+  for (ForceRec &r : b.ad.recs) {
+    for (auto &p : r.arr) if (p) (void)hipFree(p);
+    if (r.tout) (void)hipFree(r.tout);
+  }
+  b.ad.recs.clear();
+  b.ad.rec_slot = -1;
 }
 
 // The arguments of one forcing routine on block b (fields: the block's
@@ -526,17 +622,23 @@ static int force_misses(const FArgs &a) {
 // Runs one forcing routine on block b.  kin_dot: null for the routine
 // itself; otherwise the tangent-linear pass, with the block's field pointers
 // set to the tangents by the caller.
-static int force_block(Block *b, int routine, const double *kin, const double *kin_dot) {
+static int force_block(Block *b, int routine, const double *kin, const double *kin_dot,
+                       const double *addend = nullptr) {
   IbData &ib = b->ib;
   FArgs a;
   int nparts = 1;
   int64_t n_cells = 0;
   if (forcing_args(b, routine, kin, kin_dot, a, nparts, n_cells)) return -1;
   if (n_cells == 0) return 0;
+  a.addend = kin_dot ? addend : nullptr;
   const bool ts = (routine == R_VG || routine == R_PG);
   hipStream_t s = rt().stream;
   const int64_t ntask = n_cells * nparts;
   const unsigned g = div_up(ntask, 256);
+  ForceRec *rec = nullptr;
+  if (!kin_dot && b->ad.rec_slot >= 0) {
+    if (!(rec = record_inputs(*b, routine))) return -1;
+  }
   if (routine == R_VF || ts) {
     double **gh = nullptr;
     if (ts) {
@@ -553,8 +655,16 @@ static int force_block(Block *b, int routine, const double *kin, const double *k
     if (force_waves(b, a, routine, nparts, levels)) return -1;
     double *f0 = b->f[BF_P];
     double *f1 = b->f[BF_UT], *f2 = b->f[BF_VT], *f3 = b->f[BF_WT];
+    if (rec) {
+      if (grow(&rec->tout, &rec->cap_tout, ntask)) return -1;
+      BCFD_HIP(hipMemcpyAsync(rec->tout, ib.task_out, sizeof(double) * ntask, hipMemcpyDeviceToDevice, s));
+    }
     hipLaunchKernelGGL(k_force_scatter, dim3(g), dim3(256), 0, s, a, routine, nparts, ib.task_out, f0, f1, f2, f3);
     BCFD_CHECK_LAUNCH();
+  }
+  if (rec) {
+    rec->ntask = ntask;
+    rec->live = true;
   }
   return force_misses(a);
 }
@@ -567,10 +677,13 @@ static int force(int id, int routine, const double *kin) {
 }
 
 // ---------------- stress_calculation (StressCalc.f90:14-208) ----------------
-__global__ void k_stress_terms(FArgs a, double mu_f, double rho_f, double *terms) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t ne = a.n_cells;
-  if (e >= ne) return;
+// the image point of element e and the cube of each component there
+struct StressGeom {
+  double diag;
+  int cube[4][3];
+};
+
+__device__ inline void stress_geom(const FArgs &a, int64_t e, StressGeom &g) {
   const Dims &d = a.d;
   const double nrm[3] = {a.el[BE_COSA][e], a.el[BE_COSB][e], a.el[BE_COSG][e]};
   const double xc = a.el[BE_XCENT][e], yc = a.el[BE_YCENT][e], zc = a.el[BE_ZCENT][e];
@@ -580,53 +693,81 @@ __global__ void k_stress_terms(FArgs a, double mu_f, double rho_f, double *terms
   const double dX = a.g[BG_X1][ic] - a.g[BG_X1][ic - 1];
   const double dY = a.g[BG_Y1][jc] - a.g[BG_Y1][jc - 1];
   const double dZ = a.g[BG_Z1][kc] - a.g[BG_Z1][kc - 1];
-  const double diag = sqrt(dX * dX + dY * dY + dZ * dZ);
-  const double px = xc + diag * nrm[0], py = yc + diag * nrm[1], pz = zc + diag * nrm[2];
-  const int sid = a.surf[e];
-  const double *k = a.kin;
-  double td = 0.0, tdd = 0.0;
-  if (sid == 51) { td = k[BK_THETADOT1]; tdd = k[BK_THETADDOT1]; }
-  else if (sid == 52) { td = k[BK_THETADOT2]; tdd = k[BK_THETADDOT2]; }
-  const double surf[3] = {k[BK_XDOT], -td * (zc - k[BK_PIVZ]) + k[BK_YDOT], td * (yc - k[BK_PIVY])};
-  const double ac_z = -(td * td) * (zc - k[BK_PIVZ]), ac_y = -(td * td) * (yc - k[BK_PIVY]);
-  const double at_z = tdd * (yc - k[BK_PIVY]), at_y = -tdd * (zc - k[BK_PIVZ]);
-  const double dpdn = -((ac_z + at_z) * nrm[2] + (ac_y + at_y) * nrm[1]) - k[BK_YDDOT] * nrm[1];
-  double ddn[3];
-  for (int c = 0; c < 3; ++c) {
+  g.diag = sqrt(dX * dX + dY * dY + dZ * dZ);
+  const double px = xc + g.diag * nrm[0], py = yc + g.diag * nrm[1], pz = zc + g.diag * nrm[2];
+  for (int c = 0; c < 4; ++c) {
     const double *gx, *gy, *gz;
     int lx, ly, lz;
-    comp_grids(a, c + 1, gx, gy, gz, lx, ly, lz);
-    const int ix = find_index(px, gx, lx, 2, d.nx + 1, a.miss);
-    const int iy = find_index(py, gy, ly, 2, d.ny + 1, a.miss);
-    const int iz = find_index(pz, gz, lz, 2, d.nz + 1, a.miss);
-    Reader rd{&a, c + 1, false, 0, 0, nullptr, nullptr, 0, nullptr};
-    double val, dv[3];
-    value_derivs(rd, px, py, pz, ix, iy, iz, gx, gy, gz, c + 1, val, dv);
-    const double dn = dv[0] * nrm[0] + dv[1] * nrm[1] + dv[2] * nrm[2];
-    ddn[c] = (2.0 / diag) * (val - surf[c]) - dn;
+    comp_grids(a, c, gx, gy, gz, lx, ly, lz);
+    g.cube[c][0] = find_index(px, gx, lx, 2, d.nx + 1, a.miss);
+    g.cube[c][1] = find_index(py, gy, ly, 2, d.ny + 1, a.miss);
+    g.cube[c][2] = find_index(pz, gz, lz, 2, d.nz + 1, a.miss);
   }
-  const double alen = a.el[BE_ELEN][e];
-  const double area = alen / 2.0;
-  const double area_yz = 0.5 * fabs(alen * nrm[0]);
-  const double area_xz = 0.5 * fabs(alen * nrm[1]);
-  const double dd = ddn[0] * nrm[0] + ddn[1] * nrm[1] + ddn[2] * nrm[2];
-  const double sh0 = (ddn[0] - dd * nrm[0]) * mu_f * area;
-  const double sh1 = (ddn[1] - dd * nrm[1]) * mu_f * area;
+}
+
+// The six terms of element e (viscous x, pressure x, viscous y, pressure y,
+// projected areas xz and yz) from its centroid c, normal nrm, length alen,
+// the kinematic vector and the fields the reader returns (rd.comp selects
+// the component: 0 p, 1 u, 2 v, 3 w), at the forward's cubes; generic in the
+// scalar type (Dual for the derivatives).
+template <class T, class R>
+__device__ inline void stress_terms_T(const FArgs &a, int e, const T c[3], const T nrm[3], T alen,
+                                      const StressGeom &sg, const T k[BK_N], R &rd, double mu_f, double rho_f,
+                                      T out[6]) {
+  const T xc = c[0], yc = c[1], zc = c[2];
+  const T px = xc + sg.diag * nrm[0], py = yc + sg.diag * nrm[1], pz = zc + sg.diag * nrm[2];
+  const int sid = a.surf[e];
+  T td = as<T>(0.0), tdd = as<T>(0.0);
+  if (sid == 51) { td = k[BK_THETADOT1]; tdd = k[BK_THETADDOT1]; }
+  else if (sid == 52) { td = k[BK_THETADOT2]; tdd = k[BK_THETADDOT2]; }
+  const T surf[3] = {k[BK_XDOT], -td * (zc - k[BK_PIVZ]) + k[BK_YDOT], td * (yc - k[BK_PIVY])};
+  const T ac_z = -(td * td) * (zc - k[BK_PIVZ]), ac_y = -(td * td) * (yc - k[BK_PIVY]);
+  const T at_z = tdd * (yc - k[BK_PIVY]), at_y = -tdd * (zc - k[BK_PIVZ]);
+  const T dpdn = -((ac_z + at_z) * nrm[2] + (ac_y + at_y) * nrm[1]) - k[BK_YDDOT] * nrm[1];
+  T ddn[3];
+  for (int q = 0; q < 3; ++q) {
+    const double *gx, *gy, *gz;
+    int lx, ly, lz;
+    comp_grids(a, q + 1, gx, gy, gz, lx, ly, lz);
+    rd.comp = q + 1;
+    T val, dv[3];
+    value_derivs<R, T>(rd, px, py, pz, sg.cube[q + 1][0], sg.cube[q + 1][1], sg.cube[q + 1][2], gx, gy, gz, q + 1,
+                       val, dv);
+    const T dn = dv[0] * nrm[0] + dv[1] * nrm[1] + dv[2] * nrm[2];
+    ddn[q] = (2.0 / sg.diag) * (val - surf[q]) - dn;
+  }
+  const T area = alen / 2.0;
+  const T area_yz = 0.5 * fabs(alen * nrm[0]);
+  const T area_xz = 0.5 * fabs(alen * nrm[1]);
+  const T dd = ddn[0] * nrm[0] + ddn[1] * nrm[1] + ddn[2] * nrm[2];
+  const T sh0 = (ddn[0] - dd * nrm[0]) * mu_f * area;
+  const T sh1 = (ddn[1] - dd * nrm[1]) * mu_f * area;
   const double *gx, *gy, *gz;
   int lx, ly, lz;
   comp_grids(a, 0, gx, gy, gz, lx, ly, lz);
-  const int ix = find_index(px, gx, lx, 2, d.nx + 1, a.miss);
-  const int iy = find_index(py, gy, ly, 2, d.ny + 1, a.miss);
-  const int iz = find_index(pz, gz, lz, 2, d.nz + 1, a.miss);
+  rd.comp = 0;
+  T val, dv[3];
+  value_derivs<R, T>(rd, px, py, pz, sg.cube[0][0], sg.cube[0][1], sg.cube[0][2], gx, gy, gz, 0, val, dv);
+  const T dpdn_e = dv[0] * nrm[0] + dv[1] * nrm[1] + dv[2] * nrm[2];
+  const T cval = val - (dpdn_e + dpdn) * sg.diag * 0.5;
+  out[0] = sh0;
+  out[1] = -cval * area * nrm[0] * rho_f;
+  out[2] = sh1;
+  out[3] = -cval * area * nrm[1] * rho_f;
+  out[4] = area_xz;
+  out[5] = area_yz;
+}
+
+__global__ void k_stress_terms(FArgs a, double mu_f, double rho_f, double *terms) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.n_cells) return;
+  StressGeom sg;
+  stress_geom(a, e, sg);
+  double c[3], nrm[3];
+  c[0] = a.el[BE_XCENT][e]; c[1] = a.el[BE_YCENT][e]; c[2] = a.el[BE_ZCENT][e];
+  nrm[0] = a.el[BE_COSA][e]; nrm[1] = a.el[BE_COSB][e]; nrm[2] = a.el[BE_COSG][e];
   Reader rd{&a, 0, false, 0, 0, nullptr, nullptr, 0, nullptr};
-  double val, dv[3];
-  value_derivs(rd, px, py, pz, ix, iy, iz, gx, gy, gz, 0, val, dv);
-  const double dpdn_e = dv[0] * nrm[0] + dv[1] * nrm[1] + dv[2] * nrm[2];
-  const double cval = val - (dpdn_e + dpdn) * diag * 0.5;
-  const double f0 = -cval * area * nrm[0] * rho_f;
-  const double f1 = -cval * area * nrm[1] * rho_f;
-  double *t = terms + 6 * e;
-  t[0] = sh0; t[1] = f0; t[2] = sh1; t[3] = f1; t[4] = area_xz; t[5] = area_yz;
+  stress_terms_T<double>(a, (int)e, c, nrm, a.el[BE_ELEN][e], sg, a.kin, rd, mu_f, rho_f, terms + 6 * e);
 }
 
 // The reference sums element by element (StressCalc.f90:75, serial in the
@@ -831,13 +972,61 @@ __global__ void k_force_adj_clear(FArgs a, int routine, int nparts, Lam4 lam_out
   lam_out.f[comp][loc] = 0.0;
 }
 
+// The values a recorded forcing call read: the arrays at its start, and for
+// the in-place routines the earlier tasks' outputs where they wrote first
+// (the forward's reads, Reader).
+struct RecView {
+  const double *arr[4];
+  const double *tout;
+  bool live;
+};
+
+struct RecReader {
+  const FArgs *a;
+  const RecView *rv;
+  Reader w;   // writer lookup of the in-place routines (ordered) or none
+  __device__ double operator()(int ii, int jj, int kk) const {
+    const int64_t loc = ix3(a->d, ii, jj, kk);
+    if (w.ordered) {
+      const long long wr = w.writer_before(loc, ii, jj, kk);
+      if (wr >= 0) return rv->tout[wr];
+    }
+    return rv->arr[w.comp][loc];
+  }
+};
+
+__device__ inline void elem_geom(const FArgs &a, int e, double c[3], double nrm[3]) {
+  c[0] = a.el[BE_XCENT][e]; c[1] = a.el[BE_YCENT][e]; c[2] = a.el[BE_ZCENT][e];
+  nrm[0] = a.el[BE_COSA][e]; nrm[1] = a.el[BE_COSB][e]; nrm[2] = a.el[BE_COSG][e];
+}
+
+// d(task output) / d(centroid, normal of its element) . (dc, dn) with the
+// recorded values: one Dual evaluation of task_eval
+__device__ inline double task_geom_derivative(const FArgs &a, int routine, int part, int64_t cell_ord,
+                                              const TaskGeom &g, const RecView &rv, long long key,
+                                              const double dg[6]) {
+  double c0[3], n0[3];
+  elem_geom(a, g.e, c0, n0);
+  Dual c[3], nrm[3], kin[BK_N];
+  for (int q = 0; q < 3; ++q) {
+    c[q] = {c0[q], dg[q]};
+    nrm[q] = {n0[q], dg[3 + q]};
+  }
+  for (int q = 0; q < BK_N; ++q) kin[q] = {a.kin[q], 0.0};
+  const bool ordered = (routine == R_V1 || routine == R_P1 || routine == R_PF);
+  const RecReader rd{&a, &rv, Reader{&a, g.s.comp, ordered, routine, key, nullptr, nullptr, 0, nullptr}};
+  return task_eval<Dual>(a, routine, part, cell_ord, g, c, nrm, kin, rd).t;
+}
+
 // The tasks of one dependency wave (level; 0: all tasks, for the routines
 // without in-place reads), last wave first: each passes ybar[t] times its
 // cube weights to the earlier task that wrote a cube position (in-place
 // routines) or to the adjoint of the array read, and times its surface
-// weight through the surface term's gradient into kin_bar.
+// weight through the surface term's gradient into kin_bar; with a record
+// (rv.live), ybar[t] times d(output)/d(centroid, normal) into the element's
+// geometric adjoint gbar (7 x ne).
 __global__ void k_force_adj(FArgs a, int routine, int nparts, int level, const int *task_done, double *ybar,
-                            Lam4 lam_in, double *kin_bar) {
+                            Lam4 lam_in, double *kin_bar, RecView rv, double *gbar, int64_t ne) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double kc[BK_N];
   for (int q = 0; q < BK_N; ++q) kc[q] = 0.0;
@@ -845,7 +1034,9 @@ __global__ void k_force_adj(FArgs a, int routine, int nparts, int level, const i
   const double yb = active ? ybar[t] : 0.0;
   if (active && yb != 0.0) {
     TaskGeom g;
-    task_geom(a, routine, t / nparts, (int)(t % nparts), g);
+    const int64_t n = t / nparts;
+    const int part = (int)(t % nparts);
+    task_geom(a, routine, n, part, g);
     const double s2 = g.sur2, n1 = g.n1;
     double dval, ddn, dsurf;
     if (g.s.comp == 0) {
@@ -876,8 +1067,31 @@ __global__ void k_force_adj(FArgs a, int routine, int nparts, int level, const i
     double gs[BK_N];
     surface_term_grad(a, routine, g.s.comp, g.e, gs);
     for (int q = 0; q < BK_N; ++q) kc[q] = dsurf * yb * gs[q];
+    if (rv.live) {
+      for (int q = 0; q < 6; ++q) {
+        double dg[6] = {0, 0, 0, 0, 0, 0};
+        dg[q] = 1.0;
+        const double d = task_geom_derivative(a, routine, part, n, g, rv, (long long)t, dg);
+        if (d != 0.0) atomicAdd(&gbar[q * ne + g.e], yb * d);
+      }
+    }
   }
   wave_add_kin(kc, kin_bar);
+}
+
+// the geometric tangent of every task: d(output)/d(centroid, normal) . the
+// element tangents (gdot, 7 x ne), with the recorded values
+__global__ void k_force_geom_tlm(FArgs a, int routine, int nparts, RecView rv, const double *gdot, int64_t ne,
+                                 double *addend) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.n_cells * nparts) return;
+  TaskGeom g;
+  const int64_t n = t / nparts;
+  const int part = (int)(t % nparts);
+  task_geom(a, routine, n, part, g);
+  double dg[6];
+  for (int q = 0; q < 6; ++q) dg[q] = gdot[q * ne + g.e];
+  addend[t] = task_geom_derivative(a, routine, part, n, g, rv, (long long)t, dg);
 }
 
 // the ghost routines' adjoint seeds: ybar[t] = adjoint of the reconstructed
@@ -901,18 +1115,56 @@ __global__ void k_force_adj_ghost(FArgs a, int routine, int nparts, double *ybar
   if (c != 0.0) atomicAdd(&lam_plain.f[s.comp][ix3(a.d, i, j, k)], c);
 }
 
-int forcing_tlm(Block &b, int routine, const double kin[BK_N], const double kin_dot[BK_N]) {
+static int rec_view(Block &b, int routine, int rec, RecView &rv) {
+  rv = RecView{{nullptr, nullptr, nullptr, nullptr}, nullptr, false};
+  if (rec < 0) return 0;
+  const size_t idx = (size_t)rec * 6 + routine;
+  if (idx >= b.ad.recs.size() || !b.ad.recs[idx].live) {
+    set_error("forcing: no record of routine %d in slot %d", routine, rec);
+    return -1;
+  }
+  const ForceRec &r = b.ad.recs[idx];
+  for (int c = 0; c < 4; ++c) rv.arr[c] = r.arr[c];
+  rv.tout = r.tout;
+  rv.live = true;
+  return 0;
+}
+
+int forcing_tlm(Block &b, int routine, const double kin[BK_N], const double kin_dot[BK_N], int rec) {
+  const double *addend = nullptr;
+  if (rec >= 0) {
+    // the geometric tangents, from the recorded values and the element tangents
+    RecView rv;
+    if (rec_view(b, routine, rec, rv) || ad_geometry(b)) return -1;
+    if (!b.ad.gdot_set) { set_error("forcing_tlm: no geometric tangent set"); return -1; }
+    FArgs a;
+    int nparts = 1;
+    int64_t n_cells = 0;
+    if (forcing_args(&b, routine, kin, nullptr, a, nparts, n_cells)) return -1;
+    if (n_cells > 0) {
+      const int64_t ntask = n_cells * nparts;
+      if (grow(&b.ad.addend, &b.ad.cap_addend, ntask)) return -1;
+      hipLaunchKernelGGL(k_force_geom_tlm, dim3(div_up(ntask, 256)), dim3(256), 0, rt().stream, a, routine, nparts,
+                         rv, b.ad.gdot, b.ib.n_elems, b.ad.addend);
+      BCFD_CHECK_LAUNCH();
+      if (force_misses(a)) return -2;
+      addend = b.ad.addend;
+    }
+  }
   SwapFields sw(b);
-  if (routine != R_VG && routine != R_PG) return force_block(&b, routine, kin, kin_dot);
+  if (routine != R_VG && routine != R_PG) return force_block(&b, routine, kin, kin_dot, addend);
   // the ghost routines write the ghost arrays: their tangents
   if (ad_ghosts(b)) return -1;
   for (int q = 0; q < BGH_N; ++q) std::swap(b.ib.ghost[q], b.ad.gh[q]);
-  const int rc = force_block(&b, routine, kin, kin_dot);
+  const int rc = force_block(&b, routine, kin, kin_dot, addend);
   for (int q = 0; q < BGH_N; ++q) std::swap(b.ib.ghost[q], b.ad.gh[q]);
   return rc;
 }
 
-int forcing_adj(Block &b, int routine, const double kin[BK_N]) {
+int forcing_adj(Block &b, int routine, const double kin[BK_N], int rec) {
+  RecView rv;
+  if (rec_view(b, routine, rec, rv)) return -1;
+  if (rec >= 0 && ad_geometry(b)) return -1;
   FArgs a;
   int nparts = 1;
   int64_t n_cells = 0;
@@ -940,13 +1192,14 @@ int forcing_adj(Block &b, int routine, const double kin[BK_N]) {
     hipLaunchKernelGGL(k_force_adj_init, dim3(g), dim3(256), 0, s, a, routine, nparts, b.ad.ybar, out);
     hipLaunchKernelGGL(k_force_adj_clear, dim3(g), dim3(256), 0, s, a, routine, nparts, out);
   }
-  if (routine == R_VF || routine == R_VG || routine == R_PG) {
+  const int64_t ne = ib.n_elems;
+  if (!ordered) {
     hipLaunchKernelGGL(k_force_adj, dim3(g), dim3(256), 0, s, a, routine, nparts, 0, ib.task_done, b.ad.ybar, in,
-                       b.ad.kin);
+                       b.ad.kin, rv, b.ad.gbar, ne);
   } else {
     for (int level = levels; level >= 1; --level)
       hipLaunchKernelGGL(k_force_adj, dim3(g), dim3(256), 0, s, a, routine, nparts, level, ib.task_done, b.ad.ybar,
-                         in, b.ad.kin);
+                         in, b.ad.kin, rv, b.ad.gbar, ne);
   }
   BCFD_CHECK_LAUNCH();
   return force_misses(a);
@@ -954,106 +1207,101 @@ int forcing_adj(Block &b, int routine, const double kin[BK_N]) {
 
 // ---- body-force sums: tangent-linear and adjoint ----
 // S = (viscous x, pressure x, viscous y, pressure y) summed over elements,
-// the quantities stress_calculation divides by its projected areas.
+// the quantities stress_calculation divides by its projected areas; the
+// terms are stress_terms_T on Dual values: fields (value, tangent), the
+// kinematic vector (value, kin_dot) and, with geometry, the element's
+// centroid, normal and length (value, tangent).
 
-// surface velocities and dp/dn of element e and their gradients with respect
-// to the kinematic vector (StressCalc.f90:103-125)
-__device__ inline void stress_surface_grad(const FArgs &a, int e, double gs[3][BK_N], double gd[BK_N]) {
-  for (int q = 0; q < BK_N; ++q) { gs[0][q] = gs[1][q] = gs[2][q] = gd[q] = 0.0; }
-  const int sid = a.surf[e];
-  const double *k = a.kin;
-  const double n1 = a.el[BE_COSB][e], n2 = a.el[BE_COSG][e];
-  const double yc = a.el[BE_YCENT][e], zc = a.el[BE_ZCENT][e];
-  const double ry = yc - k[BK_PIVY], rz = zc - k[BK_PIVZ];
-  gs[0][BK_XDOT] = 1.0;
-  gs[1][BK_YDOT] = 1.0;
-  gd[BK_YDDOT] = -n1;
-  if (sid == 51 || sid == 52) {
-    const int std_ = (sid == 51) ? BK_THETADOT1 : BK_THETADOT2;
-    const int stdd = (sid == 51) ? BK_THETADDOT1 : BK_THETADDOT2;
-    const double td = k[std_], tdd = k[stdd];
-    gs[1][std_] = -rz;
-    gs[1][BK_PIVZ] = td;
-    gs[2][std_] = ry;
-    gs[2][BK_PIVY] = -td;
-    // dpdn = -((ac_z + at_z) n2 + (ac_y + at_y) n1) - yddot n1
-    gd[std_] = 2.0 * td * (rz * n2 + ry * n1);
-    gd[stdd] = -(ry * n2 - rz * n1);
-    gd[BK_PIVZ] = -(td * td * n2 + tdd * n1);
-    gd[BK_PIVY] = tdd * n2 - td * td * n1;
+// field values with their tangents (tangent-linear) or with a unit seed at
+// one cube position (adjoint)
+struct DualFieldReader {
+  const FArgs *a;
+  const double *tan[4];   // null: no tangents
+  int comp;
+  int scomp;               // seeded component (-1: none)
+  int64_t sloc;
+  __device__ Dual operator()(int ii, int jj, int kk) const {
+    const int64_t loc = ix3(a->d, ii, jj, kk);
+    const double v = a->fld[comp][loc];
+    if (tan[0]) return {v, tan[comp][loc]};
+    return {v, (comp == scomp && loc == sloc) ? 1.0 : 0.0};
   }
-}
-
-// the image point of element e and the cube of each component there
-struct StressGeom {
-  double nrm[3], px, py, pz, diag, area;
-  int cube[4][3];
 };
 
-__device__ inline void stress_geom(const FArgs &a, int64_t e, StressGeom &g) {
-  const Dims &d = a.d;
-  g.nrm[0] = a.el[BE_COSA][e]; g.nrm[1] = a.el[BE_COSB][e]; g.nrm[2] = a.el[BE_COSG][e];
-  const double xc = a.el[BE_XCENT][e], yc = a.el[BE_YCENT][e], zc = a.el[BE_ZCENT][e];
-  const int ic = find_index(xc, a.g[BG_X1], d.nx + 3, 2, d.nx + 1, a.miss);
-  const int jc = find_index(yc, a.g[BG_Y1], d.ny + 3, 2, d.ny + 1, a.miss);
-  const int kc = find_index(zc, a.g[BG_Z1], d.nz + 3, 2, d.nz + 1, a.miss);
-  const double dX = a.g[BG_X1][ic] - a.g[BG_X1][ic - 1];
-  const double dY = a.g[BG_Y1][jc] - a.g[BG_Y1][jc - 1];
-  const double dZ = a.g[BG_Z1][kc] - a.g[BG_Z1][kc - 1];
-  g.diag = sqrt(dX * dX + dY * dY + dZ * dZ);
-  g.px = xc + g.diag * g.nrm[0]; g.py = yc + g.diag * g.nrm[1]; g.pz = zc + g.diag * g.nrm[2];
-  g.area = a.el[BE_ELEN][e] / 2.0;
-  for (int c = 0; c < 4; ++c) {
-    const double *gx, *gy, *gz;
-    int lx, ly, lz;
-    comp_grids(a, c, gx, gy, gz, lx, ly, lz);
-    g.cube[c][0] = find_index(g.px, gx, lx, 2, d.nx + 1, a.miss);
-    g.cube[c][1] = find_index(g.py, gy, ly, 2, d.ny + 1, a.miss);
-    g.cube[c][2] = find_index(g.pz, gz, lz, 2, d.nz + 1, a.miss);
+__device__ inline void stress_dual_inputs(const FArgs &a, int64_t e, const double *gdot, int64_t ne, int gseed,
+                                          const double *kdot, int kseed, Dual c[3], Dual nrm[3], Dual &alen,
+                                          Dual kin[BK_N]) {
+  const double cv[3] = {a.el[BE_XCENT][e], a.el[BE_YCENT][e], a.el[BE_ZCENT][e]};
+  const double nv[3] = {a.el[BE_COSA][e], a.el[BE_COSB][e], a.el[BE_COSG][e]};
+  for (int q = 0; q < 3; ++q) {
+    c[q] = {cv[q], gdot ? gdot[q * ne + e] : (gseed == q ? 1.0 : 0.0)};
+    nrm[q] = {nv[q], gdot ? gdot[(3 + q) * ne + e] : (gseed == 3 + q ? 1.0 : 0.0)};
   }
+  alen = {a.el[BE_ELEN][e], gdot ? gdot[6 * ne + e] : (gseed == 6 ? 1.0 : 0.0)};
+  for (int q = 0; q < BK_N; ++q) kin[q] = {a.kin[q], kdot ? kdot[q] : (kseed == q ? 1.0 : 0.0)};
 }
 
-// The terms of the sums at element e, tangent-linear in the fields given
-// (the tangents) and in kin_dot: t = (viscous x, pressure x, viscous y,
-// pressure y) (StressCalc.f90:75-185 differentiated at frozen geometry)
-__global__ void k_stress_tlm_terms(FArgs a, double mu_f, double rho_f, double *terms) {
+// tangents of the four terms: fields (tangent arrays), kin_dot and, if gdot,
+// the geometric tangents
+__global__ void k_stress_tlm_terms(FArgs a, double mu_f, double rho_f, Lam4 tan, const double *gdot, int64_t ne,
+                                   double *terms) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.n_cells) return;
-  StressGeom g;
-  stress_geom(a, e, g);
-  double gs[3][BK_N], gd[BK_N];
-  stress_surface_grad(a, (int)e, gs, gd);
-  double surf[3] = {0.0, 0.0, 0.0}, dpdn = 0.0;
-  for (int q = 0; q < BK_N; ++q) {
-    surf[0] += gs[0][q] * a.kin_dot[q]; surf[1] += gs[1][q] * a.kin_dot[q]; surf[2] += gs[2][q] * a.kin_dot[q];
-    dpdn += gd[q] * a.kin_dot[q];
+  StressGeom sg;
+  stress_geom(a, e, sg);
+  Dual c[3], nrm[3], alen, kin[BK_N];
+  stress_dual_inputs(a, e, gdot, ne, -1, a.kin_dot, -1, c, nrm, alen, kin);
+  if (!gdot) {
+    for (int q = 0; q < 3; ++q) { c[q].t = 0.0; nrm[q].t = 0.0; }
+    alen.t = 0.0;
   }
-  double ddn[3];
-  for (int c = 0; c < 3; ++c) {
-    const double *gx, *gy, *gz;
-    int lx, ly, lz;
-    comp_grids(a, c + 1, gx, gy, gz, lx, ly, lz);
-    Reader rd{&a, c + 1, false, 0, 0, nullptr, nullptr, 0, nullptr};
-    double val, dv[3];
-    value_derivs(rd, g.px, g.py, g.pz, g.cube[c + 1][0], g.cube[c + 1][1], g.cube[c + 1][2], gx, gy, gz, c + 1,
-                 val, dv);
-    const double dn = dv[0] * g.nrm[0] + dv[1] * g.nrm[1] + dv[2] * g.nrm[2];
-    ddn[c] = (2.0 / g.diag) * (val - surf[c]) - dn;
-  }
-  const double dd = ddn[0] * g.nrm[0] + ddn[1] * g.nrm[1] + ddn[2] * g.nrm[2];
-  const double *gx, *gy, *gz;
-  int lx, ly, lz;
-  comp_grids(a, 0, gx, gy, gz, lx, ly, lz);
-  Reader rd{&a, 0, false, 0, 0, nullptr, nullptr, 0, nullptr};
-  double val, dv[3];
-  value_derivs(rd, g.px, g.py, g.pz, g.cube[0][0], g.cube[0][1], g.cube[0][2], gx, gy, gz, 0, val, dv);
-  const double dpdn_e = dv[0] * g.nrm[0] + dv[1] * g.nrm[1] + dv[2] * g.nrm[2];
-  const double cval = val - (dpdn_e + dpdn) * g.diag * 0.5;
+  DualFieldReader rd{&a, {tan.f[0], tan.f[1], tan.f[2], tan.f[3]}, 0, -1, 0};
+  Dual out[6];
+  stress_terms_T<Dual>(a, (int)e, c, nrm, alen, sg, kin, rd, mu_f, rho_f, out);
   double *t = terms + 4 * e;
-  t[0] = (ddn[0] - dd * g.nrm[0]) * mu_f * g.area;
-  t[1] = -cval * g.area * g.nrm[0] * rho_f;
-  t[2] = (ddn[1] - dd * g.nrm[1]) * mu_f * g.area;
-  t[3] = -cval * g.area * g.nrm[1] * rho_f;
+  for (int q = 0; q < 4; ++q) t[q] = out[q].t;
+}
+
+struct Sum4 {
+  double v[4];
+};
+
+// sbar (the adjoints of the four sums) into the adjoints of u, v, w, p at
+// the image-point cubes, into kin_bar and, with geometry, into gbar: one
+// seeded Dual evaluation per cube position, kinematic entry and geometric
+// input
+__global__ void k_stress_adj(FArgs a, double mu_f, double rho_f, Sum4 sbar, Lam4 lam, double *kin_bar, double *gbar,
+                             int64_t ne) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double kc[BK_N];
+  for (int q = 0; q < BK_N; ++q) kc[q] = 0.0;
+  if (e < a.n_cells) {
+    StressGeom sg;
+    stress_geom(a, e, sg);
+    auto adj_of = [&](int gseed, int kseed, int scomp, int64_t sloc) {
+      Dual c[3], nrm[3], alen, kin[BK_N];
+      stress_dual_inputs(a, e, nullptr, ne, gseed, nullptr, kseed, c, nrm, alen, kin);
+      DualFieldReader rd{&a, {nullptr, nullptr, nullptr, nullptr}, 0, scomp, sloc};
+      Dual out[6];
+      stress_terms_T<Dual>(a, (int)e, c, nrm, alen, sg, kin, rd, mu_f, rho_f, out);
+      return sbar.v[0] * out[0].t + sbar.v[1] * out[1].t + sbar.v[2] * out[2].t + sbar.v[3] * out[3].t;
+    };
+    for (int comp = 0; comp < 4; ++comp)
+      for (int m = 0; m < 8; ++m) {
+        int ii, jj, kk;
+        cube_corner(comp, sg.cube[comp][0], sg.cube[comp][1], sg.cube[comp][2], m, ii, jj, kk);
+        const int64_t loc = ix3(a.d, ii, jj, kk);
+        const double g = adj_of(-1, -1, comp, loc);
+        if (g != 0.0) atomicAdd(&lam.f[comp][loc], g);
+      }
+    for (int q = 0; q < BK_N; ++q) kc[q] = adj_of(-1, q, -1, -1);
+    if (gbar)
+      for (int q = 0; q < 7; ++q) {
+        const double g = adj_of(q, -1, -1, -1);
+        if (g != 0.0) atomicAdd(&gbar[q * ne + e], g);
+      }
+  }
+  wave_add_kin(kc, kin_bar);
 }
 
 // four sums over the elements (one workgroup, a fixed order)
@@ -1072,56 +1320,6 @@ __global__ void __launch_bounds__(256) k_sum4(const double *__restrict__ terms, 
     __syncthreads();
   }
   if (t < 4) out[t] = part[t][0];
-}
-
-// sbar (the adjoints of the four sums) into the adjoints of u, v, w, p at
-// the image-point cubes and into kin_bar
-struct Sum4 {
-  double v[4];
-};
-
-__global__ void k_stress_adj(FArgs a, double mu_f, double rho_f, Sum4 sbar, double *lu, double *lv, double *lw,
-                             double *lp, double *kin_bar) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  double kc[BK_N];
-  for (int q = 0; q < BK_N; ++q) kc[q] = 0.0;
-  if (e < a.n_cells) {
-    const double *sb = sbar.v;
-    StressGeom g;
-    stress_geom(a, e, g);
-    const double *n = g.nrm;
-    // t0 = (ddn0 - dd n0) mu A, t2 = (ddn1 - dd n1) mu A, dd = sum ddn_c n_c
-    double bddn[3];
-    for (int c = 0; c < 3; ++c)
-      bddn[c] = mu_f * g.area * (sb[0] * ((c == 0 ? 1.0 : 0.0) - n[c] * n[0]) +
-                                 sb[2] * ((c == 1 ? 1.0 : 0.0) - n[c] * n[1]));
-    // t1 = -cval A n0 rho, t3 = -cval A n1 rho
-    const double bcval = -g.area * rho_f * (sb[1] * n[0] + sb[3] * n[1]);
-    double *lam[4] = {lp, lu, lv, lw};
-    for (int c = 0; c < 4; ++c) {
-      // ddn_c = (2/diag)(val - surf_c) - dn ; cval = val - (dn + dpdn) diag/2
-      const double bval = c == 0 ? bcval : (2.0 / g.diag) * bddn[c - 1];
-      const double bdn = c == 0 ? -0.5 * g.diag * bcval : -bddn[c - 1];
-      const double *gx, *gy, *gz;
-      int lx, ly, lz;
-      comp_grids(a, c, gx, gy, gz, lx, ly, lz);
-      for (int m = 0; m < 8; ++m) {
-        int ii, jj, kk;
-        cube_corner(c, g.cube[c][0], g.cube[c][1], g.cube[c][2], m, ii, jj, kk);
-        const UnitReader ur{ii, jj, kk};
-        double val, dv[3];
-        value_derivs(ur, g.px, g.py, g.pz, g.cube[c][0], g.cube[c][1], g.cube[c][2], gx, gy, gz, c, val, dv);
-        const double w = bval * val + bdn * (dv[0] * n[0] + dv[1] * n[1] + dv[2] * n[2]);
-        if (w != 0.0) atomicAdd(&lam[c][ix3(a.d, ii, jj, kk)], w);
-      }
-    }
-    double gs[3][BK_N], gd[BK_N];
-    stress_surface_grad(a, (int)e, gs, gd);
-    const double bdpdn = -0.5 * g.diag * bcval;
-    for (int q = 0; q < BK_N; ++q)
-      kc[q] = -(2.0 / g.diag) * (bddn[0] * gs[0][q] + bddn[1] * gs[1][q] + bddn[2] * gs[2][q]) + bdpdn * gd[q];
-  }
-  wave_add_kin(kc, kin_bar);
 }
 
 static int stress_args(Block &b, const double kin[BK_N], const double *kin_dot, FArgs &a) {
@@ -1150,17 +1348,20 @@ static int stress_misses(const FArgs &a) {
   return 0;
 }
 
-int stress_tlm(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
+int stress_tlm(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N], int geom,
                double out[4]) {
   FArgs a;
-  {
-    SwapFields sw(b);
-    if (stress_args(b, kin, kin_dot, a)) return -1;
+  if (stress_args(b, kin, kin_dot, a)) return -1;
+  if (geom) {
+    if (ad_geometry(b)) return -1;
+    if (!b.ad.gdot_set) { set_error("stress_tlm: no geometric tangent set"); return -1; }
   }
   IbData &ib = b.ib;
   double *terms = ib.stress_work, *res = ib.stress_work + 6 * ib.n_elems;
   hipStream_t s = rt().stream;
-  hipLaunchKernelGGL(k_stress_tlm_terms, dim3(div_up(ib.n_elems, 256)), dim3(256), 0, s, a, mu_f, rho_f, terms);
+  const Lam4 tan{{b.ad.s[BF_P], b.ad.s[BF_U], b.ad.s[BF_V], b.ad.s[BF_W]}};
+  hipLaunchKernelGGL(k_stress_tlm_terms, dim3(div_up(ib.n_elems, 256)), dim3(256), 0, s, a, mu_f, rho_f, tan,
+                     geom ? b.ad.gdot : nullptr, ib.n_elems, terms);
   hipLaunchKernelGGL(k_sum4, dim3(1), dim3(256), 0, s, terms, ib.n_elems, 4, res);
   BCFD_CHECK_LAUNCH();
   BCFD_HIP(hipMemcpyAsync(rt().h_pinned, res, 4 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -1169,13 +1370,14 @@ int stress_tlm(Block &b, double mu_f, double rho_f, const double kin[BK_N], cons
   return 0;
 }
 
-int stress_adj(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4]) {
+int stress_adj(Block &b, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4], int geom) {
   FArgs a;
   if (stress_args(b, kin, nullptr, a)) return -1;
+  if (geom && ad_geometry(b)) return -1;
   const Sum4 sb{{sbar[0], sbar[1], sbar[2], sbar[3]}};
-  double **l = b.ad.s;
+  const Lam4 lam{{b.ad.s[BF_P], b.ad.s[BF_U], b.ad.s[BF_V], b.ad.s[BF_W]}};
   hipLaunchKernelGGL(k_stress_adj, dim3(div_up(b.ib.n_elems, 256)), dim3(256), 0, rt().stream, a, mu_f, rho_f, sb,
-                     l[BF_U], l[BF_V], l[BF_W], l[BF_P], b.ad.kin);
+                     lam, b.ad.kin, geom ? b.ad.gbar : nullptr, b.ib.n_elems);
   BCFD_CHECK_LAUNCH();
   return stress_misses(a);
 }

@@ -16,6 +16,7 @@
 #include <type_traits>
 
 #include "biocfd_rt.hpp"
+#include "dual.hpp"
 
 namespace bcfd {
 
@@ -100,23 +101,8 @@ __device__ inline double gdiv(double x, double y) {
   }
 }
 
-// A value and its directional derivative (forward-mode differentiation of
-// the stencil: the tangent-linear and adjoint passes below evaluate the same
-// component functions on these).  |x| takes the derivative sign(x), and 0 at
-// x = 0 (the value central differences give there: walls hold v or w at 0).
-struct Dual {
-  double v, t;
-};
-__device__ inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.t + b.t}; }
-__device__ inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.t - b.t}; }
-__device__ inline Dual operator-(Dual a) { return {-a.v, -a.t}; }
-__device__ inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.v * b.t + a.t * b.v}; }
-__device__ inline Dual operator*(double s, Dual a) { return {s * a.v, s * a.t}; }
-__device__ inline Dual operator/(Dual a, double s) { return {a.v / s, a.t / s}; }
-__device__ inline Dual fabs(Dual a) {
-  return a.v < 0.0 ? Dual{-a.v, -a.t} : (a.v > 0.0 ? a : Dual{0.0, 0.0});
-}
-using ::fabs;  // the double overload stays visible beside the Dual one
+// Dual (dual.hpp): the tangent-linear and adjoint passes below evaluate the
+// component functions on values with a directional derivative.
 template <bool P2>
 __device__ inline Dual gdiv(Dual x, double y) {
   return {gdiv<P2>(x.v, y), gdiv<P2>(x.t, y)};

@@ -492,15 +492,15 @@ module biocfd_hip
       integer(c_int), value :: blk
     end function
 
-    integer(c_int) function biocfd_tlm_forcing(blk, routine, kin, kin_dot) bind(C, name="biocfd_tlm_forcing")
+    integer(c_int) function biocfd_tlm_forcing(blk, routine, kin, kin_dot, rec) bind(C, name="biocfd_tlm_forcing")
       import :: c_int, c_double
-      integer(c_int), value :: blk, routine
+      integer(c_int), value :: blk, routine, rec
       real(c_double) :: kin(*), kin_dot(*)
     end function
 
-    integer(c_int) function biocfd_adj_forcing(blk, routine, kin) bind(C, name="biocfd_adj_forcing")
+    integer(c_int) function biocfd_adj_forcing(blk, routine, kin, rec) bind(C, name="biocfd_adj_forcing")
       import :: c_int, c_double
-      integer(c_int), value :: blk, routine
+      integer(c_int), value :: blk, routine, rec
       real(c_double) :: kin(*)
     end function
 
@@ -525,49 +525,70 @@ module biocfd_hip
       real(c_double) :: kin(*), out(*)
     end function
 
-    integer(c_int) function biocfd_tlm_stress(blk, mu_f, rho_f, kin, kin_dot, out) bind(C, &
+    integer(c_int) function biocfd_tlm_stress(blk, mu_f, rho_f, kin, kin_dot, geom, out) bind(C, &
         name="biocfd_tlm_stress")
       import :: c_int, c_double
-      integer(c_int), value :: blk
+      integer(c_int), value :: blk, geom
       real(c_double), value :: mu_f, rho_f
       real(c_double) :: kin(*), kin_dot(*), out(*)
     end function
 
-    integer(c_int) function biocfd_adj_stress(blk, mu_f, rho_f, kin, sbar) bind(C, name="biocfd_adj_stress")
+    integer(c_int) function biocfd_adj_stress(blk, mu_f, rho_f, kin, sbar, geom) bind(C, name="biocfd_adj_stress")
       import :: c_int, c_double
-      integer(c_int), value :: blk
+      integer(c_int), value :: blk, geom
       real(c_double), value :: mu_f, rho_f
       real(c_double) :: kin(*), sbar(*)
     end function
 
-    integer(c_int) function biocfd_tlm_step_start(blk, n_iter, dt, re, uc, kin, kin_dot) bind(C, &
+    integer(c_int) function biocfd_tlm_step_start(blk, n_iter, dt, re, uc, kin, kin_dot, rec) bind(C, &
         name="biocfd_tlm_step_start")
       import :: c_int, c_int64_t, c_double
-      integer(c_int), value :: blk
+      integer(c_int), value :: blk, rec
       integer(c_int64_t), value :: n_iter
       real(c_double), value :: dt, re, uc
       real(c_double) :: kin(*), kin_dot(*)
     end function
 
-    integer(c_int) function biocfd_tlm_step_end(blk, kin, kin_dot) bind(C, name="biocfd_tlm_step_end")
+    integer(c_int) function biocfd_tlm_step_end(blk, kin, kin_dot, rec) bind(C, name="biocfd_tlm_step_end")
       import :: c_int, c_double
-      integer(c_int), value :: blk
+      integer(c_int), value :: blk, rec
       real(c_double) :: kin(*), kin_dot(*)
     end function
 
-    integer(c_int) function biocfd_adj_step_end(blk, kin) bind(C, name="biocfd_adj_step_end")
+    integer(c_int) function biocfd_adj_step_end(blk, kin, rec) bind(C, name="biocfd_adj_step_end")
       import :: c_int, c_double
-      integer(c_int), value :: blk
+      integer(c_int), value :: blk, rec
       real(c_double) :: kin(*)
     end function
 
-    integer(c_int) function biocfd_adj_step_start(blk, n_iter, dt, re, uc, kin) bind(C, &
+    integer(c_int) function biocfd_adj_step_start(blk, n_iter, dt, re, uc, kin, rec) bind(C, &
         name="biocfd_adj_step_start")
       import :: c_int, c_int64_t, c_double
-      integer(c_int), value :: blk
+      integer(c_int), value :: blk, rec
       integer(c_int64_t), value :: n_iter
       real(c_double), value :: dt, re, uc
       real(c_double) :: kin(*)
+    end function
+
+    integer(c_int) function biocfd_ad_record(blk, slot) bind(C, name="biocfd_ad_record")
+      import :: c_int
+      integer(c_int), value :: blk, slot
+    end function
+
+    integer(c_int) function biocfd_ad_geometry_tangent(blk, inor, xdot, ydot, zdot) bind(C, &
+        name="biocfd_ad_geometry_tangent")
+      import :: c_int, c_double
+      integer(c_int), value :: blk
+      real(c_double), value :: inor
+      real(c_double) :: xdot(*), ydot(*), zdot(*)
+    end function
+
+    integer(c_int) function biocfd_ad_geometry_adjoint(blk, inor, xbar, ybar, zbar, clear) bind(C, &
+        name="biocfd_ad_geometry_adjoint")
+      import :: c_int, c_double
+      integer(c_int), value :: blk, clear
+      real(c_double), value :: inor
+      real(c_double) :: xbar(*), ybar(*), zbar(*)
     end function
 
     integer(c_int) function biocfd_snapshot_save(blk, slot) bind(C, name="biocfd_snapshot_save")

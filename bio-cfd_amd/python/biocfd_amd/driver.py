@@ -86,6 +86,7 @@ class Simulation:
         self.totime = 0.0
         self.ita = 0
         self.phase = case.phase_angle_deg * math.pi / 180.0
+        self.freq = case.freq
         self._setup()
 
     # ---------------------------------------------------------------- set-up
@@ -215,7 +216,7 @@ class Simulation:
         def surface_variables(st):
             st.blk_mv_tag = 0
             H.call("biocfd_compute_surface_variables", st.nn, H.ptr(st.xn), H.ptr(st.yn), H.ptr(st.zn),
-                   H.ptr(st.nid), self.phase, c.piv_pt, self.ita, c.dt, c.dxmin, c.freq, self.totime,
+                   H.ptr(st.nid), self.phase, c.piv_pt, self.ita, c.dt, c.dxmin, self.freq, self.totime,
                    H.ptr(st.kin), H.ptr(st.xnode1), H.ptr(st.ynode1), H.ptr(st.znode1))
             st.kin_t = self.totime
 

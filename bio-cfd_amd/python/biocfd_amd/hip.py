@@ -139,17 +139,20 @@ SIGNATURES = {
     "biocfd_adj_velocity_bc": [_I, _D, _D],
     "biocfd_tlm_solid_cell_bc": [_I],
     "biocfd_adj_solid_cell_bc": [_I],
-    "biocfd_tlm_forcing": [_I, _I, _P, _P],
-    "biocfd_adj_forcing": [_I, _I, _P],
+    "biocfd_tlm_forcing": [_I, _I, _P, _P, _I],
+    "biocfd_adj_forcing": [_I, _I, _P, _I],
     "biocfd_tlm_poisson": [_I, _L, _D, _D],
     "biocfd_adj_poisson": [_I, _L, _D, _D],
     "biocfd_stress_sums": [_I, _D, _D, _P, _P],
-    "biocfd_tlm_stress": [_I, _D, _D, _P, _P, _P],
-    "biocfd_adj_stress": [_I, _D, _D, _P, _P],
-    "biocfd_tlm_step_start": [_I, _L, _D, _D, _D, _P, _P],
-    "biocfd_tlm_step_end": [_I, _P, _P],
-    "biocfd_adj_step_end": [_I, _P],
-    "biocfd_adj_step_start": [_I, _L, _D, _D, _D, _P],
+    "biocfd_tlm_stress": [_I, _D, _D, _P, _P, _I, _P],
+    "biocfd_adj_stress": [_I, _D, _D, _P, _P, _I],
+    "biocfd_tlm_step_start": [_I, _L, _D, _D, _D, _P, _P, _I],
+    "biocfd_tlm_step_end": [_I, _P, _P, _I],
+    "biocfd_adj_step_end": [_I, _P, _I],
+    "biocfd_adj_step_start": [_I, _L, _D, _D, _D, _P, _I],
+    "biocfd_ad_record": [_I, _I],
+    "biocfd_ad_geometry_tangent": [_I, _D, _P, _P, _P],
+    "biocfd_ad_geometry_adjoint": [_I, _D, _P, _P, _P, _I],
     "biocfd_snapshot_save": [_I, _I],
     "biocfd_snapshot_restore": [_I, _I],
     "biocfd_snapshot_free": [_I, _I],

@@ -280,21 +280,38 @@ int biocfd_tlm_velocity_bc(int blk, double dt, double uc);
 int biocfd_adj_velocity_bc(int blk, double dt, double uc);
 int biocfd_tlm_solid_cell_bc(int blk);
 int biocfd_adj_solid_cell_bc(int blk);
-int biocfd_tlm_forcing(int blk, int routine, const double kin[BK_N], const double kin_dot[BK_N]);
-int biocfd_adj_forcing(int blk, int routine, const double kin[BK_N]);
+int biocfd_tlm_forcing(int blk, int routine, const double kin[BK_N], const double kin_dot[BK_N], int rec);
+int biocfd_adj_forcing(int blk, int routine, const double kin[BK_N], int rec);
 int biocfd_tlm_poisson(int blk, int64_t n_iter, double dt, double uc);
 int biocfd_adj_poisson(int blk, int64_t n_iter, double dt, double uc);
 /* the four element sums stress_calculation divides by its projected areas:
  * viscous x, pressure x, viscous y, pressure y (StressCalc.f90:173-179) */
 int biocfd_stress_sums(int blk, double mu_f, double rho_f, const double kin[BK_N], double out[4]);
 int biocfd_tlm_stress(int blk, double mu_f, double rho_f, const double kin[BK_N], const double kin_dot[BK_N],
-                      double out[4]);
-int biocfd_adj_stress(int blk, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4]);
+                      int geom, double out[4]);
+int biocfd_adj_stress(int blk, double mu_f, double rho_f, const double kin[BK_N], const double sbar[4], int geom);
 int biocfd_tlm_step_start(int blk, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N],
-                          const double kin_dot[BK_N]);
-int biocfd_tlm_step_end(int blk, const double kin[BK_N], const double kin_dot[BK_N]);
-int biocfd_adj_step_end(int blk, const double kin[BK_N]);
-int biocfd_adj_step_start(int blk, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N]);
+                          const double kin_dot[BK_N], int rec);
+int biocfd_tlm_step_end(int blk, const double kin[BK_N], const double kin_dot[BK_N], int rec);
+int biocfd_adj_step_end(int blk, const double kin[BK_N], int rec);
+int biocfd_adj_step_start(int blk, int64_t n_iter, double dt, double re, double uc, const double kin[BK_N],
+                          int rec);
+/* The moving geometry (the same calls with rec >= 0 / geom 1): the element
+ * centroids, normals and lengths (computeSurfaceNorm, Search.F90:210-258)
+ * enter every reconstruction (its normal distance, image point, cube
+ * weights and surface term) and the force sums; the discrete structure (IB
+ * cells, nearest elements, cubes, TS ghost choices) stays the forward's.
+ * biocfd_ad_record(blk, slot): the forcing calls that follow keep what they
+ * read (the interpolated arrays at the call's start, the in-place routines'
+ * task outputs) in record `slot` (-1 stops); the tlm/adj calls with rec =
+ * slot use them.  biocfd_ad_geometry_tangent sets the element tangents from
+ * node-position tangents (n_nodes each, at the block's current node
+ * positions; inor the normal orientation of computeSurfaceNorm); the adj
+ * calls add element adjoints, which biocfd_ad_geometry_adjoint turns into
+ * node-position adjoints (and clears with clear = 1). */
+int biocfd_ad_record(int blk, int slot);
+int biocfd_ad_geometry_tangent(int blk, double inor, const double *xdot, const double *ydot, const double *zdot);
+int biocfd_ad_geometry_adjoint(int blk, double inor, double *xbar, double *ybar, double *zbar, int clear);
 /* Step checkpoints: the device state a time step reads and writes (fields
  * u .. p, tags, IB / TS / solid lists, element and node data, counts) copied
  * into slot `slot` (0..4095, device memory) and back; free: slot -1 = all. */

@@ -98,18 +98,24 @@ class Prolongation(torch.autograd.Function):
 
 
 class BodyForce(torch.autograd.Function):
-    """theta (heave amplitude, pitch amplitude, phase, frequency) -> J, the
-    mean body x-force over the run's averaging window at frozen geometry:
-    forward = AdjointRun.forward with checkpoints, backward = the adjoint
-    time steps replayed from them (AdjointRun.backward)."""
+    """(theta, body nodes) -> J, the mean body x-force over the run's
+    averaging window; theta = (heave amplitude, pitch amplitude, phase,
+    frequency), nodes (nn, 3) the body-frame marker nodes: forward =
+    AdjointRun.forward with checkpoints, backward = the adjoint time steps
+    replayed from them (AdjointRun.backward).  A frozen-geometry run has no
+    node gradient (None)."""
 
     @staticmethod
-    def forward(ctx, theta, run):
-        J = run.forward(theta.detach().numpy(), checkpoints=True)
+    def forward(ctx, theta, nodes, run):
+        nd = nodes.detach().numpy()
+        J = run.forward(theta.detach().numpy(), nodes=(nd[:, 0].copy(), nd[:, 1].copy(), nd[:, 2].copy()),
+                        checkpoints=True)
         ctx.run = run
         return torch.tensor(J, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, gJ):
-        g = torch.from_numpy(ctx.run.backward())
-        return gJ * g, None
+        out = ctx.run.backward()
+        if isinstance(out, tuple):
+            return gJ * torch.from_numpy(out[0]), gJ * torch.from_numpy(out[1]), None
+        return gJ * torch.from_numpy(out), None, None

@@ -1,7 +1,8 @@
 """The tangent-linear and adjoint time step (include/biocfd_hip.h,
 biocfd_tlm_* / biocfd_adj_*; bio-cfd_amd/python/biocfd_amd/adjoint.py) on the
 reduced adjoint workload: a heaving, pitching ellipsoid (ID 51, so every IB
-cell is a thin-surface cell) in [0,3]^3 at 32^3, geometry frozen.
+cell is a thin-surface cell) in [0,3]^3 at 32^3, geometry frozen (the moving
+geometry's part: tests/test_gpu_adjoint_geometry.py).
 
 * every piece: dot products <T x, y> = <x, A y> between its tangent-linear
   pass T and its adjoint pass A over all seven fields, the fourteen ghost
@@ -38,7 +39,7 @@ def _case(nsteps=4):
 def run():
     from biocfd_amd.adjoint import AdjointRun
 
-    r = AdjointRun(_case(), every=2, avg_steps=2)
+    r = AdjointRun(_case(), every=2, avg_steps=2, frozen=True)
     yield r
     r.close()
 
@@ -125,13 +126,13 @@ def _tlm(run, piece, kv, kd):
     elif piece == "solid_cell_bc":
         H.call("biocfd_tlm_solid_cell_bc", g)
     elif piece.startswith("forcing"):
-        H.call("biocfd_tlm_forcing", g, int(piece[-1]), H.ptr(kv), H.ptr(kd))
+        H.call("biocfd_tlm_forcing", g, int(piece[-1]), H.ptr(kv), H.ptr(kd), -1)
     elif piece == "poisson":
         H.call("biocfd_tlm_poisson", g, run.n_iter, c.dt, c.uc)
     elif piece == "step_start":
-        H.call("biocfd_tlm_step_start", g, run.n_iter, c.dt, c.re, c.uc, H.ptr(kv), H.ptr(kd))
+        H.call("biocfd_tlm_step_start", g, run.n_iter, c.dt, c.re, c.uc, H.ptr(kv), H.ptr(kd), -1)
     elif piece == "step_end":
-        H.call("biocfd_tlm_step_end", g, H.ptr(kv), H.ptr(kd))
+        H.call("biocfd_tlm_step_end", g, H.ptr(kv), H.ptr(kd), -1)
 
 
 def _adj(run, piece, kv):
@@ -145,13 +146,13 @@ def _adj(run, piece, kv):
     elif piece == "solid_cell_bc":
         H.call("biocfd_adj_solid_cell_bc", g)
     elif piece.startswith("forcing"):
-        H.call("biocfd_adj_forcing", g, int(piece[-1]), H.ptr(kv))
+        H.call("biocfd_adj_forcing", g, int(piece[-1]), H.ptr(kv), -1)
     elif piece == "poisson":
         H.call("biocfd_adj_poisson", g, run.n_iter, c.dt, c.uc)
     elif piece == "step_start":
-        H.call("biocfd_adj_step_start", g, run.n_iter, c.dt, c.re, c.uc, H.ptr(kv))
+        H.call("biocfd_adj_step_start", g, run.n_iter, c.dt, c.re, c.uc, H.ptr(kv), -1)
     elif piece == "step_end":
-        H.call("biocfd_adj_step_end", g, H.ptr(kv))
+        H.call("biocfd_adj_step_end", g, H.ptr(kv), -1)
 
 
 @pytest.mark.parametrize("piece", PIECES)
@@ -190,10 +191,10 @@ def test_stress_dot_product(run):
     kd = rng.standard_normal(H.BK_N)
     S.put(x)
     out = np.zeros(4)
-    H.call("biocfd_tlm_stress", run.g, c.mu_f, c.rho_f, H.ptr(kv), H.ptr(kd), H.ptr(out))
+    H.call("biocfd_tlm_stress", run.g, c.mu_f, c.rho_f, H.ptr(kv), H.ptr(kd), 0, H.ptr(out))
     sbar = rng.standard_normal(4)
     S.put([np.zeros_like(a) for a in x])
-    H.call("biocfd_adj_stress", run.g, c.mu_f, c.rho_f, H.ptr(kv), H.ptr(sbar))
+    H.call("biocfd_adj_stress", run.g, c.mu_f, c.rho_f, H.ptr(kv), H.ptr(sbar), 0)
     ay = S.get()
     kb = S.kin()
     lhs = float(np.dot(out, sbar))
@@ -275,7 +276,7 @@ def test_forcing_tangent_matches_central_differences_in_kin(run, routine):
     fd = [(a - b) / (2 * h) for a, b in zip(*outs)]
     _put_fields(run, x0)
     S.put([np.zeros(S.n2) for _ in FIELDS] + [np.zeros(S.n_ts) for _ in H.BGH_NAMES])
-    H.call("biocfd_tlm_forcing", run.g, routine, H.ptr(kv), H.ptr(kd))
+    H.call("biocfd_tlm_forcing", run.g, routine, H.ptr(kv), H.ptr(kd), -1)
     t = S.get()
     worst = 0.0
     scale = max(max(float(np.max(np.abs(a))) for a in t), 1e-300)
@@ -331,7 +332,8 @@ def test_autograd_wrapper(run):
     from adjoint_harness import BodyForce
 
     theta = torch.tensor(run.theta0, dtype=torch.float64, requires_grad=True)
-    J = BodyForce.apply(theta, run)
+    nodes = torch.from_numpy(np.stack(run.nodes0, axis=1))
+    J = BodyForce.apply(theta, nodes, run)
     J.backward()
     _, ref = run.gradient()
     assert np.allclose(theta.grad.numpy(), ref, rtol=1e-13, atol=0.0)
