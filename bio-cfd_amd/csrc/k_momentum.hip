@@ -1260,9 +1260,10 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
 // forward's; the operands are the forward fields u, v, w, p of the block,
 // which the derivatives are taken at.  The component functions above are
 // evaluated on Dual values: the tangent-linear pass carries the tangents of
-// u, v, w, p (b.ad.s) into those of ut, vt, wt; the adjoint pass seeds one
-// operand position at a time (every position the stencil reads, 49 of them)
-// and adds lambda . d(ut, vt, wt)/d(operand) into the operand's adjoint.
+// u, v, w, p (b.ad.s) into those of ut, vt, wt.  The adjoint pass is the
+// hand-written reverse of the component functions (mom_u_adj, ...): the
+// adjoint of every operand position the stencil reads (49 of them) is
+// accumulated in registers and added once.
 // ---------------------------------------------------------------------------
 namespace bcfd {
 namespace {
@@ -1278,20 +1279,6 @@ struct DAcc : GAcc {
   __device__ Dual w() const { return {this->template at<di, dj, dk>(fw), this->template at<di, dj, dk>(tw)}; }
   template <int di, int dj, int dk>
   __device__ Dual p() const { return {this->template at<di, dj, dk>(fp), this->template at<di, dj, dk>(tp)}; }
-};
-
-// operands with a unit tangent at one seeded position (field sf: 0 u, 1 v,
-// 2 w, 3 p; offset si, sj, sk from the cell) and zero elsewhere
-struct SAcc : GAcc {
-  int sf, si, sj, sk;
-  template <int F, int di, int dj, int dk>
-  __device__ Dual op(const double *f) const {
-    return {this->template at<di, dj, dk>(f), (F == sf && di == si && dj == sj && dk == sk) ? 1.0 : 0.0};
-  }
-  template <int di, int dj, int dk> __device__ Dual u() const { return op<0, di, dj, dk>(fu); }
-  template <int di, int dj, int dk> __device__ Dual v() const { return op<1, di, dj, dk>(fv); }
-  template <int di, int dj, int dk> __device__ Dual w() const { return op<2, di, dj, dk>(fw); }
-  template <int di, int dj, int dk> __device__ Dual p() const { return op<3, di, dj, dk>(fp); }
 };
 
 // every operand position of the three component functions: 15 of u, v and
@@ -1311,6 +1298,267 @@ __constant__ Seed kSeeds[49] = {
     {2, 0, -1, 0}, {2, 0, -2, 0}, {2, 0, 0, 1}, {2, 0, 0, 2}, {2, 0, 0, -1}, {2, 0, 0, -2}, {2, 1, 0, -1},
     {2, 0, 1, -1},
     {3, 0, 0, 0}, {3, 1, 0, 0}, {3, 0, 1, 0}, {3, 0, 0, 1}};
+
+// ---- hand-written reverse of the component functions ----
+// bar[s] accumulates d(lambda . (ut, vt, wt)) / d(operand at seed s)
+// (kSeeds order); the reverse follows mom_u / mom_v / mom_w statement by
+// statement.  |x| has the derivative sg(x), 0 at 0 (as Dual's fabs).
+__device__ inline double sg(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
+
+template <int F, int di, int dj, int dk>
+__device__ constexpr int sidx() {
+  if constexpr (F == 3) {
+    static_assert((di == 0 && dj == 0 && dk == 0) || (di == 1 && dj == 0 && dk == 0) ||
+                      (di == 0 && dj == 1 && dk == 0) || (di == 0 && dj == 0 && dk == 1),
+                  "p operand");
+    return 45 + (di == 1 ? 1 : dj == 1 ? 2 : dk == 1 ? 3 : 0);
+  } else {
+    constexpr int base = 15 * F;
+    if constexpr (di == 0 && dj == 0 && dk == 0) return base;
+    else if constexpr (dj == 0 && dk == 0) return base + (di == 1 ? 1 : di == 2 ? 2 : di == -1 ? 3 : 4);
+    else if constexpr (di == 0 && dk == 0) return base + (dj == 1 ? 5 : dj == 2 ? 6 : dj == -1 ? 7 : 8);
+    else if constexpr (di == 0 && dj == 0) return base + (dk == 1 ? 9 : dk == 2 ? 10 : dk == -1 ? 11 : 12);
+    else if constexpr (F == 0 && di == -1 && dj == 1 && dk == 0) return 13;
+    else if constexpr (F == 0 && di == -1 && dj == 0 && dk == 1) return 14;
+    else if constexpr (F == 1 && di == 1 && dj == -1 && dk == 0) return 28;
+    else if constexpr (F == 1 && di == 0 && dj == -1 && dk == 1) return 29;
+    else if constexpr (F == 2 && di == 1 && dj == 0 && dk == -1) return 43;
+    else {
+      static_assert(F == 2 && di == 0 && dj == 1 && dk == -1, "operand position");
+      return 44;
+    }
+  }
+}
+
+#define BU(di, dj, dk) bar[sidx<0, di, dj, dk>()]
+#define BV(di, dj, dk) bar[sidx<1, di, dj, dk>()]
+#define BW(di, dj, dk) bar[sidx<2, di, dj, dk>()]
+#define BP(di, dj, dk) bar[sidx<3, di, dj, dk>()]
+#define U(di, dj, dk) A.template u<di, dj, dk>()
+#define V(di, dj, dk) A.template v<di, dj, dk>()
+#define W(di, dj, dk) A.template w<di, dj, dk>()
+#define P(di, dj, dk) A.template p<di, dj, dk>()
+
+// reverse of upwind3r: b = adjoint of its value; returns the velocity's
+// adjoint, f's adjoints into bf (f2, f1, f0, f-1, f-2)
+template <class TA, class TC>
+__device__ inline double upwind3r_adj(double vel, TA a, TC c, double f2, double f1, double f0, double fm1,
+                                      double fm2, double hA, double hB, double b, double bf[5]) {
+  const double sa = a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2;
+  const double sk = c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2;
+  const double qa = 1.0 / (a[5] * hA), qk = 1.0 / (2.0 * c[5] * hB);
+  const double wa = b * vel * qa, wk = b * fabs(vel) * qk;
+  for (int m = 0; m < 5; ++m) bf[m] = wa * a[m] + wk * c[m];
+  return b * (sa * qa + sg(vel) * sk * qk);
+}
+
+// reverse of k (p q + |p| r - s t - |s| w): K = k * adjoint
+__device__ inline void donor_adj(double K, double p, double q, double r, double s, double t, double w, double &bp,
+                                 double &bq, double &br, double &bs, double &bt, double &bw) {
+  bp += K * (q + sg(p) * r);
+  bq += K * p;
+  br += K * fabs(p);
+  bs -= K * (t + sg(s) * w);
+  bt -= K * s;
+  bw -= K * fabs(s);
+}
+
+template <bool P2, class Acc>
+__device__ inline void mom_u_adj(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+                                 bool near_ok, double lam, double bar[49]) {
+  const double dxr = m.dxr, dx = m.dx, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
+  const double dx2xr = dx + dxr, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt, dz2zb = dz + dzb;
+  const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+  const double g = a.dt * lam;
+  BU(0, 0, 0) += lam;
+  // dpdx
+  const double qp = g / (0.5 * (dxr + dx));
+  BP(0, 0, 0) += qp;
+  BP(1, 0, 0) -= qp;
+  // viscous terms: d2x = A (-u4/dxr + u22/dx), ...
+  const double bv = g / a.re;
+  double bu4 = -gdiv<P2>(2.0, dx2xr) / dxr * bv, bu22 = gdiv<P2>(2.0, dx2xr) / dx * bv;
+  const double ay = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)), az = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb));
+  double bu8 = -ay / (0.5 * dy2ye) * bv, bu6 = ay / (0.5 * dy2yw) * bv;
+  double bu12 = -az / (0.5 * dz2zt) * bv, bu10 = az / (0.5 * dz2zb) * bv;
+  double bu1a = 0, bu3 = 0, bu5 = 0, bu7 = 0, bu9 = 0, bu11 = 0, bv1a = 0, bv22 = 0, bw1a = 0, bw22 = 0;
+  double bvin = 0, bwin = 0;
+  const double rx = gdiv<P2>(dxr, dx2xr);
+  if (i > 2 && i < nx && j > 2 && j < ny + 1 && k > 2 && k < nz + 1 && near_ok) {
+    const double ddy = 0.5 * (A.template hy<-1>() + A.template hy<-2>()), ddye = 0.5 * (A.template hy<-1>() + A.template hy<0>());
+    const double ddz = 0.5 * (A.template hz<-1>() + A.template hz<-2>()), ddzr = 0.5 * (A.template hz<-1>() + A.template hz<0>());
+    const double vu_e = V(1, 0, 0) + rx * (V(0, 0, 0) - V(1, 0, 0));
+    const double vu_w = V(1, -1, 0) + rx * (V(0, -1, 0) - V(1, -1, 0));
+    const double v_in = 0.5 * (vu_e + vu_w);
+    const double wu_n = W(1, 0, 0) + rx * (W(0, 0, 0) - W(1, 0, 0));
+    const double wu_s = W(1, 0, -1) + rx * (W(0, 0, -1) - W(1, 0, -1));
+    const double w_in = 0.5 * (wu_n + wu_s);
+    double bf[5];
+    BU(0, 0, 0) += upwind3r_adj(U(0, 0, 0), A.template xa<BTB_UU>(), A.template xk<BTB_UU>(), U(2, 0, 0), U(1, 0, 0),
+                                U(0, 0, 0), U(-1, 0, 0), U(-2, 0, 0), A.template hx<0>(), A.template hx<-1>(), -g, bf);
+    BU(2, 0, 0) += bf[0]; BU(1, 0, 0) += bf[1]; BU(0, 0, 0) += bf[2]; BU(-1, 0, 0) += bf[3]; BU(-2, 0, 0) += bf[4];
+    bvin = upwind3r_adj(v_in, A.template ya<BTB_VU>(), A.template yk<BTB_VU>(), U(0, 2, 0), U(0, 1, 0), U(0, 0, 0),
+                        U(0, -1, 0), U(0, -2, 0), ddye, ddy, -g, bf);
+    BU(0, 2, 0) += bf[0]; BU(0, 1, 0) += bf[1]; BU(0, 0, 0) += bf[2]; BU(0, -1, 0) += bf[3]; BU(0, -2, 0) += bf[4];
+    bwin = upwind3r_adj(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), U(0, 0, 2), U(0, 0, 1), U(0, 0, 0),
+                        U(0, 0, -1), U(0, 0, -2), ddzr, ddz, -g, bf);
+    BU(0, 0, 2) += bf[0]; BU(0, 0, 1) += bf[1]; BU(0, 0, 0) += bf[2]; BU(0, 0, -1) += bf[3]; BU(0, 0, -2) += bf[4];
+  } else {
+    const double u1a = U(-1, 0, 0) + U(0, 0, 0), u22 = U(-1, 0, 0) - U(0, 0, 0);
+    const double u3 = U(0, 0, 0) + U(1, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
+    const double u5 = U(0, -1, 0) + U(0, 0, 0), u6 = U(0, -1, 0) - U(0, 0, 0);
+    const double u7 = U(0, 0, 0) + U(0, 1, 0), u8 = U(0, 0, 0) - U(0, 1, 0);
+    const double u9 = U(0, 0, -1) + U(0, 0, 0), u10 = U(0, 0, -1) - U(0, 0, 0);
+    const double u11 = U(0, 0, 0) + U(0, 0, 1), u12 = U(0, 0, 0) - U(0, 0, 1);
+    const double v1a = V(0, -1, 0) + V(1, -1, 0), v22 = V(0, 0, 0) + V(1, 0, 0);
+    const double w1a = W(0, 0, -1) + W(1, 0, -1), w22 = W(0, 0, 0) + W(1, 0, 0);
+    donor_adj(-0.25 / A.template hx<-1>() * -g, u1a, u1a, u22, u3, u3, u4, bu1a, bu1a, bu22, bu3, bu3, bu4);
+    donor_adj(-0.25 / A.template hy<-1>() * -g, v1a, u5, u6, v22, u7, u8, bv1a, bu5, bu6, bv22, bu7, bu8);
+    donor_adj(-0.25 / A.template hz<-1>() * -g, w1a, u9, u10, w22, u11, u12, bw1a, bu9, bu10, bw22, bu11, bu12);
+  }
+  // v_in, w_in (third order) and the sums / differences
+  BV(1, 0, 0) += 0.5 * (1.0 - rx) * bvin; BV(0, 0, 0) += 0.5 * rx * bvin;
+  BV(1, -1, 0) += 0.5 * (1.0 - rx) * bvin; BV(0, -1, 0) += 0.5 * rx * bvin;
+  BW(1, 0, 0) += 0.5 * (1.0 - rx) * bwin; BW(0, 0, 0) += 0.5 * rx * bwin;
+  BW(1, 0, -1) += 0.5 * (1.0 - rx) * bwin; BW(0, 0, -1) += 0.5 * rx * bwin;
+  BU(-1, 0, 0) += bu1a + bu22; BU(0, 0, 0) += bu1a - bu22 + bu3 + bu4; BU(1, 0, 0) += bu3 - bu4;
+  BU(0, -1, 0) += bu5 + bu6; BU(0, 0, 0) += bu5 - bu6 + bu7 + bu8; BU(0, 1, 0) += bu7 - bu8;
+  BU(0, 0, -1) += bu9 + bu10; BU(0, 0, 0) += bu9 - bu10 + bu11 + bu12; BU(0, 0, 1) += bu11 - bu12;
+  BV(0, -1, 0) += bv1a; BV(1, -1, 0) += bv1a; BV(0, 0, 0) += bv22; BV(1, 0, 0) += bv22;
+  BW(0, 0, -1) += bw1a; BW(1, 0, -1) += bw1a; BW(0, 0, 0) += bw22; BW(1, 0, 0) += bw22;
+}
+
+template <bool P2, class Acc>
+__device__ inline void mom_v_adj(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+                                 bool near_ok, double lam, double bar[49]) {
+  const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
+  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dz2zt = dz + dzt, dz2zb = dz + dzb;
+  const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+  const double g = a.dt * lam;
+  BV(0, 0, 0) += lam;
+  const double qp = g / (0.5 * (dye + dy));
+  BP(0, 0, 0) += qp;
+  BP(0, 1, 0) -= qp;
+  const double bv = g / a.re;
+  const double ax = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)), az = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb));
+  double bv6 = -ax / (0.5 * dx2xr) * bv, bv4 = ax / (0.5 * dx2xl) * bv;
+  double bv10 = -gdiv<P2>(2.0, dy2ye) / dye * bv, bv8 = gdiv<P2>(2.0, dy2ye) / dy * bv;
+  double bv14 = -az / (0.5 * dz2zt) * bv, bv12 = az / (0.5 * dz2zb) * bv;
+  double bu13 = 0, bu14 = 0, bv3 = 0, bv5 = 0, bv7 = 0, bv9 = 0, bv11 = 0, bv13 = 0, bw3 = 0, bw4 = 0;
+  double buin = 0, bwin = 0;
+  const double ry = gdiv<P2>(dye, dy2ye);
+  if (i > 2 && i < nx + 1 && j > 2 && j < ny && k > 2 && k < nz + 1 && near_ok) {
+    const double ddx = 0.5 * (A.template hx<-1>() + A.template hx<-2>()), ddxr = 0.5 * (A.template hx<-1>() + A.template hx<0>());
+    const double ddz = 0.5 * (A.template hz<-1>() + A.template hz<-2>()), ddzr = 0.5 * (A.template hz<-1>() + A.template hz<0>());
+    const double uv_e = U(0, 1, 0) + ry * (U(0, 0, 0) - U(0, 1, 0));
+    const double uv_w = U(-1, 1, 0) + ry * (U(-1, 0, 0) - U(-1, 1, 0));
+    const double u_in = 0.5 * (uv_e + uv_w);
+    const double wv_n = W(0, 1, 0) + ry * (W(0, 0, 0) - W(0, 1, 0));
+    const double wv_s = W(0, 1, -1) + ry * (W(0, 0, -1) - W(0, 1, -1));
+    const double w_in = 0.5 * (wv_n + wv_s);
+    double bf[5];
+    buin = upwind3r_adj(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), V(2, 0, 0), V(1, 0, 0), V(0, 0, 0),
+                        V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx, -g, bf);
+    BV(2, 0, 0) += bf[0]; BV(1, 0, 0) += bf[1]; BV(0, 0, 0) += bf[2]; BV(-1, 0, 0) += bf[3]; BV(-2, 0, 0) += bf[4];
+    BV(0, 0, 0) += upwind3r_adj(V(0, 0, 0), A.template ya<BTB_VV>(), A.template yk<BTB_VV>(), V(0, 2, 0), V(0, 1, 0),
+                                V(0, 0, 0), V(0, -1, 0), V(0, -2, 0), A.template hy<0>(), A.template hy<-1>(), -g, bf);
+    BV(0, 2, 0) += bf[0]; BV(0, 1, 0) += bf[1]; BV(0, 0, 0) += bf[2]; BV(0, -1, 0) += bf[3]; BV(0, -2, 0) += bf[4];
+    bwin = upwind3r_adj(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), V(0, 0, 2), V(0, 0, 1), V(0, 0, 0),
+                        V(0, 0, -1), V(0, 0, -2), ddzr, ddz, -g, bf);
+    BV(0, 0, 2) += bf[0]; BV(0, 0, 1) += bf[1]; BV(0, 0, 0) += bf[2]; BV(0, 0, -1) += bf[3]; BV(0, 0, -2) += bf[4];
+  } else {
+    const double u13 = U(-1, 0, 0) + U(-1, 1, 0), u14 = U(0, 0, 0) + U(0, 1, 0);
+    const double v3 = V(-1, 0, 0) + V(0, 0, 0), v4 = V(-1, 0, 0) - V(0, 0, 0);
+    const double v5 = V(0, 0, 0) + V(1, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
+    const double v7 = V(0, -1, 0) + V(0, 0, 0), v8 = V(0, -1, 0) - V(0, 0, 0);
+    const double v9 = V(0, 0, 0) + V(0, 1, 0), v10 = V(0, 0, 0) - V(0, 1, 0);
+    const double v11 = V(0, 0, -1) + V(0, 0, 0), v12 = V(0, 0, -1) - V(0, 0, 0);
+    const double v13 = V(0, 0, 0) + V(0, 0, 1), v14 = V(0, 0, 0) - V(0, 0, 1);
+    const double w3 = W(0, 0, -1) + W(0, 1, -1), w4 = W(0, 0, 0) + W(0, 1, 0);
+    donor_adj(-0.25 / A.template hx<-1>() * -g, u13, v3, v4, u14, v5, v6, bu13, bv3, bv4, bu14, bv5, bv6);
+    donor_adj(-0.25 / A.template hy<-1>() * -g, v7, v7, v8, v9, v9, v10, bv7, bv7, bv8, bv9, bv9, bv10);
+    donor_adj(-0.25 / A.template hz<-1>() * -g, w3, v11, v12, w4, v13, v14, bw3, bv11, bv12, bw4, bv13, bv14);
+  }
+  BU(0, 1, 0) += 0.5 * (1.0 - ry) * buin; BU(0, 0, 0) += 0.5 * ry * buin;
+  BU(-1, 1, 0) += 0.5 * (1.0 - ry) * buin; BU(-1, 0, 0) += 0.5 * ry * buin;
+  BW(0, 1, 0) += 0.5 * (1.0 - ry) * bwin; BW(0, 0, 0) += 0.5 * ry * bwin;
+  BW(0, 1, -1) += 0.5 * (1.0 - ry) * bwin; BW(0, 0, -1) += 0.5 * ry * bwin;
+  BV(-1, 0, 0) += bv3 + bv4; BV(0, 0, 0) += bv3 - bv4 + bv5 + bv6; BV(1, 0, 0) += bv5 - bv6;
+  BV(0, -1, 0) += bv7 + bv8; BV(0, 0, 0) += bv7 - bv8 + bv9 + bv10; BV(0, 1, 0) += bv9 - bv10;
+  BV(0, 0, -1) += bv11 + bv12; BV(0, 0, 0) += bv11 - bv12 + bv13 + bv14; BV(0, 0, 1) += bv13 - bv14;
+  BU(-1, 0, 0) += bu13; BU(-1, 1, 0) += bu13; BU(0, 0, 0) += bu14; BU(0, 1, 0) += bu14;
+  BW(0, 0, -1) += bw3; BW(0, 1, -1) += bw3; BW(0, 0, 0) += bw4; BW(0, 1, 0) += bw4;
+}
+
+template <bool P2, class Acc>
+__device__ inline void mom_w_adj(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+                                 bool near_ok, double lam, double bar[49]) {
+  const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz;
+  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt;
+  const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
+  const double g = a.dt * lam;
+  BW(0, 0, 0) += lam;
+  const double qp = g / (0.5 * (dzt + dz));
+  BP(0, 0, 0) += qp;
+  BP(0, 0, 1) -= qp;
+  const double bv = g / a.re;
+  const double ax = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)), ay = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw));
+  double bw8 = -ax / (0.5 * dx2xr) * bv, bw6 = ax / (0.5 * dx2xl) * bv;
+  double bw12 = -ay / (0.5 * dy2ye) * bv, bw10 = ay / (0.5 * dy2yw) * bv;
+  double bw16 = -gdiv<P2>(2.0, dz2zt) / dzt * bv, bw14 = gdiv<P2>(2.0, dz2zt) / dz * bv;
+  double bu15 = 0, bu16 = 0, bv15 = 0, bv16 = 0, bw5 = 0, bw7 = 0, bw9 = 0, bw11 = 0, bw13 = 0, bw15 = 0;
+  double buin = 0, bvin = 0;
+  const double rz = gdiv<P2>(dzt, dz2zt);
+  if (i > 2 && i < nx + 1 && j > 2 && j < ny + 1 && k > 2 && k < nz && near_ok) {
+    const double ddx = 0.5 * (A.template hx<-1>() + A.template hx<-2>()), ddxr = 0.5 * (A.template hx<-1>() + A.template hx<0>());
+    const double ddy = 0.5 * (A.template hy<-1>() + A.template hy<-2>()), ddye = 0.5 * (A.template hy<-1>() + A.template hy<0>());
+    const double uw_e = U(0, 0, 1) + rz * (U(0, 0, 0) - U(0, 0, 1));
+    const double uw_w = U(-1, 0, 1) + rz * (U(-1, 0, 0) - U(-1, 0, 1));
+    const double u_in = 0.5 * (uw_e + uw_w);
+    const double vw_n = V(0, 0, 1) + rz * (V(0, 0, 0) - V(0, 0, 1));
+    const double vw_s = V(0, -1, 1) + rz * (V(0, -1, 0) - V(0, -1, 1));
+    const double v_in = 0.5 * (vw_n + vw_s);
+    double bf[5];
+    buin = upwind3r_adj(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), W(2, 0, 0), W(1, 0, 0), W(0, 0, 0),
+                        W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx, -g, bf);
+    BW(2, 0, 0) += bf[0]; BW(1, 0, 0) += bf[1]; BW(0, 0, 0) += bf[2]; BW(-1, 0, 0) += bf[3]; BW(-2, 0, 0) += bf[4];
+    bvin = upwind3r_adj(v_in, A.template ya<BTB_VW>(), A.template yk<BTB_VW>(), W(0, 2, 0), W(0, 1, 0), W(0, 0, 0),
+                        W(0, -1, 0), W(0, -2, 0), ddye, ddy, -g, bf);
+    BW(0, 2, 0) += bf[0]; BW(0, 1, 0) += bf[1]; BW(0, 0, 0) += bf[2]; BW(0, -1, 0) += bf[3]; BW(0, -2, 0) += bf[4];
+    BW(0, 0, 0) += upwind3r_adj(W(0, 0, 0), A.template za<BTB_WW>(), A.template zk<BTB_WW>(), W(0, 0, 2), W(0, 0, 1),
+                                W(0, 0, 0), W(0, 0, -1), W(0, 0, -2), A.template hz<0>(), A.template hz<-1>(), -g, bf);
+    BW(0, 0, 2) += bf[0]; BW(0, 0, 1) += bf[1]; BW(0, 0, 0) += bf[2]; BW(0, 0, -1) += bf[3]; BW(0, 0, -2) += bf[4];
+  } else {
+    const double u15 = U(-1, 0, 0) + U(-1, 0, 1), u16 = U(0, 0, 0) + U(0, 0, 1);
+    const double v15 = V(0, -1, 0) + V(0, -1, 1), v16 = V(0, 0, 0) + V(0, 0, 1);
+    const double w5 = W(-1, 0, 0) + W(0, 0, 0), w6 = W(-1, 0, 0) - W(0, 0, 0);
+    const double w7 = W(0, 0, 0) + W(1, 0, 0), w8 = W(0, 0, 0) - W(1, 0, 0);
+    const double w9 = W(0, -1, 0) + W(0, 0, 0), w10 = W(0, -1, 0) - W(0, 0, 0);
+    const double w11 = W(0, 0, 0) + W(0, 1, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
+    const double w13 = W(0, 0, -1) + W(0, 0, 0), w14 = W(0, 0, -1) - W(0, 0, 0);
+    const double w15 = W(0, 0, 0) + W(0, 0, 1), w16 = W(0, 0, 0) - W(0, 0, 1);
+    donor_adj(-0.5 / A.template hx<-1>() * -g, u15, w5, w6, u16, w7, w8, bu15, bw5, bw6, bu16, bw7, bw8);
+    donor_adj(-0.5 / A.template hy<-1>() * -g, v15, w9, w10, v16, w11, w12, bv15, bw9, bw10, bv16, bw11, bw12);
+    donor_adj(-0.5 / A.template hz<-1>() * -g, w13, w13, w14, w15, w15, w16, bw13, bw13, bw14, bw15, bw15, bw16);
+  }
+  BU(0, 0, 1) += 0.5 * (1.0 - rz) * buin; BU(0, 0, 0) += 0.5 * rz * buin;
+  BU(-1, 0, 1) += 0.5 * (1.0 - rz) * buin; BU(-1, 0, 0) += 0.5 * rz * buin;
+  BV(0, 0, 1) += 0.5 * (1.0 - rz) * bvin; BV(0, 0, 0) += 0.5 * rz * bvin;
+  BV(0, -1, 1) += 0.5 * (1.0 - rz) * bvin; BV(0, -1, 0) += 0.5 * rz * bvin;
+  BW(-1, 0, 0) += bw5 + bw6; BW(0, 0, 0) += bw5 - bw6 + bw7 + bw8; BW(1, 0, 0) += bw7 - bw8;
+  BW(0, -1, 0) += bw9 + bw10; BW(0, 0, 0) += bw9 - bw10 + bw11 + bw12; BW(0, 1, 0) += bw11 - bw12;
+  BW(0, 0, -1) += bw13 + bw14; BW(0, 0, 0) += bw13 - bw14 + bw15 + bw16; BW(0, 0, 1) += bw15 - bw16;
+  BU(-1, 0, 0) += bu15; BU(-1, 0, 1) += bu15; BU(0, 0, 0) += bu16; BU(0, 0, 1) += bu16;
+  BV(0, -1, 0) += bv15; BV(0, -1, 1) += bv15; BV(0, 0, 0) += bv16; BV(0, 0, 1) += bv16;
+}
+#undef BU
+#undef BV
+#undef BW
+#undef BP
+#undef U
+#undef V
+#undef W
+#undef P
 
 __device__ inline bool mom_fluid(const MomArgs &a, int64_t q, int &i, int &j, int &k, int64_t &c) {
   const Dims &d = a.d;
@@ -1342,7 +1590,8 @@ __global__ void __launch_bounds__(256) k_momentum_tlm(MomArgs a, Fields F, Field
 }
 
 // lambda of ut, vt, wt at a fluid cell is consumed (the forward overwrote
-// them) and scattered, atomically, into the adjoints of the operands
+// them) and the reverse of the component functions scatters it, atomically,
+// into the adjoints of the operand positions
 template <bool P2>
 __global__ void __launch_bounds__(256) k_momentum_adj(MomArgs a, Fields F, double *__restrict__ lut,
                                                       double *__restrict__ lvt, double *__restrict__ lwt,
@@ -1360,16 +1609,18 @@ __global__ void __launch_bounds__(256) k_momentum_adj(MomArgs a, Fields F, doubl
   if (gu == 0.0 && gv == 0.0 && gw == 0.0) return;
   const MomCell m = mom_cell(a, i, j, k);
   const bool near_ok = mom_near_ok(a, i, j, k);
-  SAcc A;
-  A.fu = F.u; A.fv = F.v; A.fw = F.w; A.fp = F.p;
-  A.c = m.c; A.sy = m.sy; A.sz = m.sz; A.a = &a; A.i = i; A.j = j; A.k = k;
+  const GAcc A{F.u, F.v, F.w, F.p, m.c, m.sy, m.sz, &a, i, j, k};
+  double bar[49];
+#pragma unroll
+  for (int q = 0; q < 49; ++q) bar[q] = 0.0;
+  mom_u_adj<P2>(a, A, m, i, j, k, near_ok, gu, bar);
+  mom_v_adj<P2>(a, A, m, i, j, k, near_ok, gv, bar);
+  mom_w_adj<P2>(a, A, m, i, j, k, near_ok, gw, bar);
   double *lam[4] = {lu, lv, lw, lp};
+#pragma unroll
   for (int q = 0; q < 49; ++q) {
     const Seed s = kSeeds[q];
-    A.sf = s.f; A.si = s.i; A.sj = s.j; A.sk = s.k;
-    const double g = gu * mom_u<P2>(a, A, m, i, j, k, near_ok).t + gv * mom_v<P2>(a, A, m, i, j, k, near_ok).t +
-                     gw * mom_w<P2>(a, A, m, i, j, k, near_ok).t;
-    if (g != 0.0) atomicAdd(&lam[s.f][c + s.i + (int64_t)s.j * m.sy + (int64_t)s.k * m.sz], g);
+    if (bar[q] != 0.0) atomicAdd(&lam[s.f][c + s.i + (int64_t)s.j * m.sy + (int64_t)s.k * m.sz], bar[q]);
   }
 }
 
@@ -1401,27 +1652,6 @@ struct TsDAcc {
     base.src(fi, di, dj, dk, kind, idx);
     if (kind < 4) return {base.f[kind][idx], t->f[kind][idx]};
     return {base.tv->ghost[kind - 4][idx], t->gh[kind - 4][idx]};
-  }
-  template <int di, int dj, int dk> __device__ Dual u() const { return rd(0, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ Dual v() const { return rd(1, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ Dual w() const { return rd(2, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ Dual p() const { return rd(3, di, dj, dk); }
-  template <int o> __device__ double hx() const { return base.template hx<o>(); }
-  template <int o> __device__ double hy() const { return base.template hy<o>(); }
-  template <int o> __device__ double hz() const { return base.template hz<o>(); }
-  template <int T> __device__ const double *xa() const { return base.template xa<T>(); }
-  template <int T> __device__ const double *xk() const { return base.template xk<T>(); }
-  template <int T> __device__ const double *ya() const { return base.template ya<T>(); }
-  template <int T> __device__ const double *yk() const { return base.template yk<T>(); }
-  template <int T> __device__ const double *za() const { return base.template za<T>(); }
-  template <int T> __device__ const double *zk() const { return base.template zk<T>(); }
-};
-
-struct TsSAcc {
-  TsAcc base;
-  int sf, si, sj, sk;
-  __device__ Dual rd(int fi, int di, int dj, int dk) const {
-    return {base.rd(fi, di, dj, dk), (fi == sf && di == si && dj == sj && dk == sk) ? 1.0 : 0.0};
   }
   template <int di, int dj, int dk> __device__ Dual u() const { return rd(0, di, dj, dk); }
   template <int di, int dj, int dk> __device__ Dual v() const { return rd(1, di, dj, dk); }
@@ -1474,17 +1704,21 @@ __global__ void __launch_bounds__(256) k_ts_adj(MomArgs a, Fields F, TsSens L, c
   lwt[m.c] = 0.0;
   if (gu == 0.0 && gv == 0.0 && gw == 0.0) return;
   const bool near_ok = mom_near_ok(a, i, j, k);
-  TsSAcc A{{{F.u, F.v, F.w, F.p}, &a.d, &tv, &a, q, i, j, k}, 0, 0, 0, 0};
+  const TsAcc A{{F.u, F.v, F.w, F.p}, &a.d, &tv, &a, q, i, j, k};
+  double bar[49];
+#pragma unroll
+  for (int s = 0; s < 49; ++s) bar[s] = 0.0;
+  mom_u_adj<false>(a, A, m, i, j, k, near_ok, gu, bar);
+  mom_v_adj<false>(a, A, m, i, j, k, near_ok, gv, bar);
+  mom_w_adj<false>(a, A, m, i, j, k, near_ok, gw, bar);
+#pragma unroll
   for (int s = 0; s < 49; ++s) {
+    if (bar[s] == 0.0) continue;
     const Seed sd = kSeeds[s];
-    A.sf = sd.f; A.si = sd.i; A.sj = sd.j; A.sk = sd.k;
-    const double g = gu * mom_u<false>(a, A, m, i, j, k, near_ok).t + gv * mom_v<false>(a, A, m, i, j, k, near_ok).t +
-                     gw * mom_w<false>(a, A, m, i, j, k, near_ok).t;
-    if (g == 0.0) continue;
     int kind;
     int64_t idx;
-    A.base.src(sd.f, sd.i, sd.j, sd.k, kind, idx);
-    atomicAdd(kind < 4 ? &L.f[kind][idx] : &L.gh[kind - 4][idx], g);
+    A.src(sd.f, sd.i, sd.j, sd.k, kind, idx);
+    atomicAdd(kind < 4 ? &L.f[kind][idx] : &L.gh[kind - 4][idx], bar[s]);
   }
 }
 

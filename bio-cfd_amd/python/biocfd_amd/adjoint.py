@@ -117,7 +117,12 @@ class AdjointRun:
     """The adjoint workload on the GPU: forward, tangent-linear and adjoint
     passes of `case` (one block with a body, epsi 0)."""
 
-    def __init__(self, case: CS.Case, every: int = 8, avg_steps: Optional[int] = None, frozen: bool = False):
+    def __init__(self, case: CS.Case, every: int = 8, avg_steps: Optional[int] = None, frozen: bool = False,
+                 tune: bool = False):
+        """every: steps per checkpoint; avg_steps: the averaging window M
+        (default half the steps); tune: pick the SOR kernel variant for the
+        block by timing the candidates (biocfd_amd.tune; every variant gives
+        the same bits)."""
         if len(case.blocks) != 1 or case.blocks[0].body is None:
             raise ValueError("the adjoint driver runs a single block with a body")
         if case.epsi != 0.0:
@@ -144,6 +149,12 @@ class AdjointRun:
         self.nodes0 = (self.st.xn.copy(), self.st.yn.copy(), self.st.zn.copy())
         self.nodes = self.nodes0
         self.sim.kin_hook = self._hook if self.frozen else None
+        self.sor = None
+        if tune:
+            from . import tune as T
+
+            v, planes, _ = T.autotune(block=self.g, dt=case.dt, planes=(4, 8, 16, 32))
+            self.sor = {"variant": int(v), "planes": int(planes)}
         self._host = {}
         self._save(0)
         self._ckpt = {0: 0}
