@@ -168,7 +168,7 @@ struct AdWork {
   double *s[7] = {};
   double *gh[BGH_N] = {};   // the thin-surface ghost arrays' sensitivities (cap_gh entries each)
   int64_t cap_gh = 0;
-  double *scr[2] = {};
+  double *scr[3] = {};
   double *ybar = nullptr;
   int64_t cap_ybar = 0;
   double *kin = nullptr;

@@ -95,44 +95,69 @@ __global__ void k_adj_grad(Dims d, const uint8_t *__restrict__ cell, const doubl
   lpc[x.c] = l;
 }
 
-// transpose of a colour-C sweep, step (a): fluid cells of the other colour
-// gather from their fluid neighbours of colour C
-__global__ void k_adj_sor_a(Dims d, const uint8_t *__restrict__ cell, const double *__restrict__ Ax,
-                            const double *__restrict__ Ay, const double *__restrict__ Az, double omega,
-                            int colour_c, double *__restrict__ lpc) {
+// wd = w / D at fluid cells, 0 elsewhere (the ghost ring included)
+__global__ void k_adj_wd(Dims d, const uint8_t *__restrict__ cell, const double *__restrict__ Ax,
+                         const double *__restrict__ Ay, const double *__restrict__ Az, double omega,
+                         double *__restrict__ wd) {
   Cell x;
   if (!cell_of(d, x)) return;
-  const int i = x.i, j = x.j, k = x.k;
-  if ((int)red(i, j, k) == colour_c || !fluid(d, cell, i, j, k)) return;
-  const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
-  double acc = 0.0;
-  // neighbour y west of x: x is y's east neighbour (column 2), and so on
-  if (fluid(d, cell, i - 1, j, k))
-    acc += omega * coef(Ax, d.nx, i - 1, 2) / diag(Ax, Ay, Az, d, i - 1, j, k) * lpc[x.c - 1];
-  if (fluid(d, cell, i + 1, j, k))
-    acc += omega * coef(Ax, d.nx, i + 1, 0) / diag(Ax, Ay, Az, d, i + 1, j, k) * lpc[x.c + 1];
-  if (fluid(d, cell, i, j - 1, k))
-    acc += omega * coef(Ay, d.ny, j - 1, 2) / diag(Ax, Ay, Az, d, i, j - 1, k) * lpc[x.c - sy];
-  if (fluid(d, cell, i, j + 1, k))
-    acc += omega * coef(Ay, d.ny, j + 1, 0) / diag(Ax, Ay, Az, d, i, j + 1, k) * lpc[x.c + sy];
-  if (fluid(d, cell, i, j, k - 1))
-    acc += omega * coef(Az, d.nz, k - 1, 2) / diag(Ax, Ay, Az, d, i, j, k - 1) * lpc[x.c - sz];
-  if (fluid(d, cell, i, j, k + 1))
-    acc += omega * coef(Az, d.nz, k + 1, 0) / diag(Ax, Ay, Az, d, i, j, k + 1) * lpc[x.c + sz];
-  lpc[x.c] -= acc;
+  wd[x.c] = fluid(d, cell, x.i, x.j, x.k) ? omega / diag(Ax, Ay, Az, d, x.i, x.j, x.k) : 0.0;
 }
 
-// step (b): fluid cells of colour C
-__global__ void k_adj_sor_b(Dims d, const uint8_t *__restrict__ cell, const double *__restrict__ Ax,
-                            const double *__restrict__ Ay, const double *__restrict__ Az, double omega, double dt,
-                            int colour_c, double *__restrict__ lpc, double *__restrict__ bbar) {
-  Cell x;
-  if (!cell_of(d, x)) return;
-  const int i = x.i, j = x.j, k = x.k;
-  if ((int)red(i, j, k) != colour_c || !fluid(d, cell, i, j, k)) return;
-  const double l = lpc[x.c];
-  bbar[x.c] += omega / (diag(Ax, Ay, Az, d, i, j, k) * dt) * l;
-  lpc[x.c] = (1.0 - omega) * l;
+// The transpose of one iteration (black sweep B after red sweep R) is
+// B^T(a), B^T(b), R^T(a), R^T(b), and step (b) of one colour and step (a)
+// of the other touch the same cells: one launch per colour does, at its
+// fluid cells, the pending (b) (unless first) and then the gather (a) from
+// the other colour's cells, which this launch does not write (unless last).
+// n iterations are 2 n + 1 launches, red first and last.  One thread per
+// interior cell of the launch's colour: thread (ii, j, k) takes i = 2 + 2 ii
+// + p, p the parity that gives (i + j + k) odd for red, even for black.
+// Non-fluid neighbours have wd = 0; a fluid cell's neighbours lie inside the
+// ghost ring, and the coefficient tables are read only for interior rows.
+__global__ void k_adj_sor_colour(Dims d, const uint8_t *__restrict__ cell, const double *__restrict__ Ax,
+                                 const double *__restrict__ Ay, const double *__restrict__ Az,
+                                 const double *__restrict__ wd, double dt, double omega, int red_cells, int first,
+                                 int last, double *__restrict__ lpc, double *__restrict__ bbar) {
+  const int W = (d.nx + 1) / 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)W * d.ny * d.nz) return;
+  const int ii = (int)(q % W);
+  const int j = 2 + (int)((q / W) % d.ny);
+  const int k = 2 + (int)(q / ((int64_t)W * d.ny));
+  const int i = 2 + 2 * ii + ((red_cells + j + k) & 1);
+  if (i > d.nx + 1) return;
+  const int64_t c = ix3(d, i, j, k);
+  if (cell[c] != 0) return;
+  double l = lpc[c];
+  if (!first) {
+    bbar[c] += wd[c] / dt * l;
+    l = (1.0 - omega) * l;
+  }
+  if (!last) {
+    const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
+    double acc = 0.0;
+    // neighbour y west of x: x is y's east neighbour (column 2), and so on
+    if (i > 2) acc += coef(Ax, d.nx, i - 1, 2) * wd[c - 1] * lpc[c - 1];
+    if (i <= d.nx) acc += coef(Ax, d.nx, i + 1, 0) * wd[c + 1] * lpc[c + 1];
+    if (j > 2) acc += coef(Ay, d.ny, j - 1, 2) * wd[c - sy] * lpc[c - sy];
+    if (j <= d.ny) acc += coef(Ay, d.ny, j + 1, 0) * wd[c + sy] * lpc[c + sy];
+    if (k > 2) acc += coef(Az, d.nz, k - 1, 2) * wd[c - sz] * lpc[c - sz];
+    if (k <= d.nz) acc += coef(Az, d.nz, k + 1, 0) * wd[c + sz] * lpc[c + sz];
+    l -= acc;
+  }
+  lpc[c] = l;
+}
+
+// lpc <- (B_1^T R_1^T ... B_N^T R_N^T applied backwards), bbar accumulated
+void adj_sor(const Block &b, int64_t n_iter, double dt, double *wd, double *lpc, double *bbar) {
+  hipStream_t s = rt().stream;
+  const dim3 blk(256), grd(div_up(b.n2, 256));
+  const uint8_t *cell = b.tag[BT_CELL];
+  hipLaunchKernelGGL(k_adj_wd, grd, blk, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], b.omega, wd);
+  const dim3 grc(div_up((int64_t)((b.d.nx + 1) / 2) * b.d.ny * b.d.nz, 256));
+  for (int64_t q = 0; q <= 2 * n_iter; ++q)
+    hipLaunchKernelGGL(k_adj_sor_colour, grc, blk, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], wd, dt, b.omega,
+                       q % 2 == 0 ? 1 : 0, q == 0 ? 1 : 0, q == 2 * n_iter ? 1 : 0, lpc, bbar);
 }
 
 // utbar = ubar + D^T bbar; computeDiv: b(i) = (ut(i) - ut(i-1)) / dx(i) + ...
@@ -342,14 +367,7 @@ int biocfd_projection_adjoint(int id, int64_t n_iter, double dt, const double *u
   const uint8_t *cell = b->tag[BT_CELL];
   hipLaunchKernelGGL(k_adj_grad, grd, blk, 0, s, d, cell, ub, vb, wb, pb, b->g[BG_DX], b->g[BG_DY], b->g[BG_DZ], dt,
                      lpc, bbar);
-  for (int64_t it = 0; it < n_iter; ++it) {
-    // black sweep transposed, then red (the forward's last sweep first)
-    for (int colour : {0, 1}) {
-      hipLaunchKernelGGL(k_adj_sor_a, grd, blk, 0, s, d, cell, b->A[0], b->A[1], b->A[2], b->omega, colour, lpc);
-      hipLaunchKernelGGL(k_adj_sor_b, grd, blk, 0, s, d, cell, b->A[0], b->A[1], b->A[2], b->omega, dt, colour,
-                         lpc, bbar);
-    }
-  }
+  adj_sor(*b, n_iter, dt, out, lpc, bbar);
   // utbar into out, vtbar into lpc, wtbar into pb (neither is read any more)
   hipLaunchKernelGGL(k_adj_div, grd, blk, 0, s, d, cell, bbar, ub, vb, wb, b->g[BG_DX], b->g[BG_DY], b->g[BG_DZ],
                      out, lpc, pb);
@@ -711,13 +729,7 @@ int poisson_adj(Block &b, int64_t n_iter, double dt, double uc) {
   double *lpc = b.ad.scr[0], *bbar = b.ad.scr[1];
   hipLaunchKernelGGL(k_adj_grad, grd, blk, 0, s, d, cell, l[BF_UT], l[BF_VT], l[BF_WT], l[BF_P], b.g[BG_DX],
                      b.g[BG_DY], b.g[BG_DZ], dt, lpc, bbar);
-  for (int64_t it = 0; it < n_iter; ++it) {
-    for (int colour : {0, 1}) {
-      hipLaunchKernelGGL(k_adj_sor_a, grd, blk, 0, s, d, cell, b.A[0], b.A[1], b.A[2], b.omega, colour, lpc);
-      hipLaunchKernelGGL(k_adj_sor_b, grd, blk, 0, s, d, cell, b.A[0], b.A[1], b.A[2], b.omega, dt, colour, lpc,
-                         bbar);
-    }
-  }
+  adj_sor(b, n_iter, dt, b.ad.scr[2], lpc, bbar);
   // each thread reads and writes only its own cell of the velocity adjoints
   hipLaunchKernelGGL(k_adj_div, grd, blk, 0, s, d, cell, bbar, l[BF_UT], l[BF_VT], l[BF_WT], b.g[BG_DX], b.g[BG_DY],
                      b.g[BG_DZ], l[BF_UT], l[BF_VT], l[BF_WT]);
@@ -820,7 +832,7 @@ int ad_ready(Block &b) {
     if (dev_alloc(&w.s[q], b.n2)) return -1;
     hipLaunchKernelGGL(k_ad_fill, dim3(div_up(b.n2, 256)), dim3(256), 0, s, w.s[q], b.n2, 0.0);
   }
-  for (int q = 0; q < 2; ++q)
+  for (int q = 0; q < 3; ++q)
     if (dev_alloc(&w.scr[q], b.n2)) return -1;
   if (dev_alloc(&w.kin, 32)) return -1;
   hipLaunchKernelGGL(k_ad_fill, dim3(1), dim3(64), 0, s, w.kin, (int64_t)32, 0.0);
