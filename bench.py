@@ -18,6 +18,9 @@ residual, not the budget, ends every solve).  `mixed` holds the mixed-precision
 run of the same steps on the same inputs to the same residual (FP32 SOR
 smoothing, 256 sweeps per FP64 defect correction, the residual tested after
 every correction), its speed-up and its per-field error against the FP64 run.
+`mixed_fp32_momentum` is the same mixed run with the dense momentum pass's
+tendency evaluated in FP32 (the precision recipe's FP32 momentum stencil),
+with its speed-up and per-field error against the FP64 run.
 `reference_rule` is the FP64 method under the reference's own stopping rule
 (update norm < 1e-8 tested every fifth iteration, pcItaMax = 500, which this
 workload reaches: 501 iterations per step).  `roofline` describes the FP64
@@ -98,10 +101,11 @@ def make_case(args, rule: str = "reference"):
                                 pc_ita_max=args.pc_ita_max, epsi=args.epsi)
 
 
-def run_path(args, precision: str, keep_fields: bool, rule: str = "residual"):
+def run_path(args, precision: str, keep_fields: bool, rule: str = "residual", momentum: str = "fp64"):
     """One run of the workload: `precision` fp64 or mixed, `rule` residual
     (FP64 residual < resid_tol) or reference (the reference's update-norm
-    rule; FP64 only)."""
+    rule; FP64 only), `momentum` fp64 or fp32 (the dense momentum pass's
+    tendency in FP32)."""
     from biocfd_amd import hip as H
     from biocfd_amd import tune
     from biocfd_amd.driver import Simulation
@@ -113,6 +117,7 @@ def run_path(args, precision: str, keep_fields: bool, rule: str = "residual"):
         H.call("biocfd_copy_probe", int(args.copy_probe_mb) << 20, 4)
     sim = Simulation(case, precision=precision)
     H.call("biocfd_momentum_config", args.momentum_variant, 0 if args.momentum_divisions else 1)
+    H.call("biocfd_momentum_precision", 1 if momentum == "fp32" else 0)
     if precision == "fp64":
         if rule == "residual":
             H.call("biocfd_sor_stop_rule", 1, args.check_every)
@@ -176,6 +181,7 @@ def run_path(args, precision: str, keep_fields: bool, rule: str = "residual"):
         H.call("biocfd_stress_calculation", 1, case.mu_f, case.rho_f, H.ptr(kv), H.ptr(res))
         out["forces"] = res.copy()
     H.call("biocfd_sor_stop_rule", 0, 5)
+    H.call("biocfd_momentum_precision", 0)
     sim.close()
     return out
 
@@ -540,6 +546,8 @@ def main():
                     help="dense momentum kernel (include/biocfd_hip.h, biocfd_momentum_config): 0 = all three "
                          "components per thread per plane, 1 = one launch per component, 2 = as 0 with four waves "
                          "per SIMD, 3..8 = plane march of 64 x TJ column tiles (4 = TJ 4, 32 planes, the default)")
+    ap.add_argument("--mixed-momentum", choices=["fp64", "both"], default="both",
+                    help="both: a second mixed run with the FP32 momentum tendency, reported beside the first")
     ap.add_argument("--momentum-divisions", action="store_true",
                     help="keep every grid division of the momentum stencil a division (no power-of-two products)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -575,13 +583,18 @@ def main():
 
     both = args.precision == "both"
     resid = args.rule in ("both", "residual")
-    g = m = None
+    g = m = m32 = None
     if resid:
         if args.precision in ("both", "fp64"):
             g = run_path(args, "fp64", keep_fields=both, rule="residual")
             g["rule"] = "residual"
         if args.precision in ("both", "mixed"):
             m = run_path(args, "mixed", keep_fields=both, rule="residual")
+            if args.mixed_momentum == "both":
+                # the same mixed solve (the variant the first run chose) with the FP32 momentum tendency
+                a32 = argparse.Namespace(**vars(args))
+                a32.mixed_variant, a32.mixed_planes = str(m["kernel"]["variant"]), m["kernel"]["planes"]
+                m32 = run_path(a32, "mixed", keep_fields=both, rule="residual", momentum="fp32")
     ref = None
     if args.rule in ("both", "reference") and args.precision in ("both", "fp64"):
         ref = run_path(args, "fp64", keep_fields=False, rule="reference")
@@ -642,6 +655,18 @@ def main():
             mixed["fp64_residual_linf"] = g["residual_linf"]
             mixed["rel_err_vs_fp64"] = field_errors(g, m)
         out["mixed"] = mixed
+    if m32 is not None:
+        f32 = {
+            "momentum": "FP32 tendency in the dense pass (biocfd_momentum_precision(1)); thin-surface cells FP64",
+            "value": round(m32["mcell_steps_per_s"], 4),
+            "ms_per_step": round(m32["ms_per_step"], 3),
+            "sweeps_per_step": m32["sor_iters_per_step"],
+            "residual_linf": m32["residual_linf"],
+        }
+        if g is not None:
+            f32["speedup_vs_fp64"] = round(m32["mcell_steps_per_s"] / g["mcell_steps_per_s"], 3)
+            f32["rel_err_vs_fp64"] = field_errors(g, m32)
+        out["mixed_fp32_momentum"] = f32
     if ref is not None and main_run is not ref:
         out["reference_rule"] = {
             "value": round(ref["mcell_steps_per_s"], 3),

@@ -197,6 +197,7 @@ struct Block {
   double *A[3] = {};                  // Acx, Acy, Acz (n,3)
   std::vector<double> hA[3];
   double *ca[BTB_N] = {}, *ck[BTB_N] = {};
+  float *ca32[BTB_N] = {}, *ck32[BTB_N] = {}, *h32[3] = {};   // FP32 copies (FP32 momentum tendency)
   std::vector<double> hca[BTB_N], hck[BTB_N];
   SorState *sor = nullptr;            // device
   int64_t counts[BC_N] = {0, 0, 0, 0, 0, 0};

@@ -25,6 +25,9 @@ struct MomArgs {
   const uint8_t *cell, *cell2;
   const double *dx, *dy, *dz;
   const double *ca[BTB_N], *ck[BTB_N];
+  // FP32 copies of the upwind tables and spacings (the FP32 tendency)
+  const float *ca32[BTB_N], *ck32[BTB_N];
+  const float *dx32, *dy32, *dz32;
   double dt, re;
 };
 
@@ -38,20 +41,55 @@ struct Fields {
 // rows, dy, dz): a constant-address-space pointer lets the compiler use
 // scalar loads.  Nothing writes these arrays while the kernels run.
 typedef const double __attribute__((address_space(4))) *cdp;
+template <class R>
+using cptr = const R __attribute__((address_space(4))) *;
+
+// the upwind tables and spacings in the value type R
+template <class R>
+__device__ inline const R *mom_ca(const MomArgs &a, int t) {
+  if constexpr (std::is_same_v<R, float>) return a.ca32[t];
+  else return a.ca[t];
+}
+template <class R>
+__device__ inline const R *mom_ck(const MomArgs &a, int t) {
+  if constexpr (std::is_same_v<R, float>) return a.ck32[t];
+  else return a.ck[t];
+}
+template <class R>
+__device__ inline const R *mom_h(const MomArgs &a, int axis) {
+  if constexpr (std::is_same_v<R, float>) return axis == 0 ? a.dx32 : axis == 1 ? a.dy32 : a.dz32;
+  else return axis == 0 ? a.dx : axis == 1 ? a.dy : a.dz;
+}
 
 // (a, c: the table rows of the stencil's column, ca/ck + 6 (col - 1))
-template <class T, class TA, class TC>
-__device__ inline T upwind3r(T vel, TA a, TC c, T f2, T f1, T f0, T fm1, T fm2, double hA, double hB) {
-  return vel * (a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2) / (a[5] * hA) +
-         fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2) / (2.0 * c[5] * hB);
+// x / y; for the FP32 tendency through the hardware reciprocal (about 1 ulp),
+// correctly rounded otherwise
+template <class T, class H>
+__device__ inline T qdiv(T x, H y) {
+  if constexpr (std::is_same_v<T, float>) return x * __builtin_amdgcn_rcpf(y);
+  else return x / y;
 }
+
+template <class T, class TA, class TC, class H>
+__device__ inline T upwind3r(T vel, TA a, TC c, T f2, T f1, T f0, T fm1, T fm2, H hA, H hB) {
+  return qdiv(vel * (a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2), a[5] * hA) +
+         qdiv(fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2), H(2) * c[5] * hB);
+}
+
+// the type of the grid quantities and constants the stencil combines with
+// operands of type T: float for the FP32 tendency, double otherwise (FP64
+// values and Dual values with FP64 parts)
+template <class T>
+using real_of = std::conditional_t<std::is_same_v<T, float>, float, double>;
 
 // Grid spacings around cell (i,j,k) and its linear offset, shared by the
 // three component evaluations.
-struct MomCell {
+template <class R>
+struct MomCellT {
   int64_t c, sy, sz;
-  double dxr, dx, dxl, dye, dy, dyw, dzt, dz, dzb;
+  R dxr, dx, dxl, dye, dy, dyw, dzt, dz, dzb;
 };
+using MomCell = MomCellT<double>;
 
 __device__ inline MomCell mom_cell(const MomArgs &a, int i, int j, int k) {
   const Dims &d = a.d;
@@ -101,6 +139,17 @@ __device__ inline double gdiv(double x, double y) {
   }
 }
 
+// FP32: y = 2^e, 1/y has biased exponent 254 - E
+template <bool P2>
+__device__ inline float gdiv(float x, float y) {
+  if constexpr (P2) {
+    const int ey = __float_as_int(y) & 0x7F800000;
+    return x * __int_as_float(0x7F000000 - ey);
+  } else {
+    return qdiv(x, y);
+  }
+}
+
 // Dual (dual.hpp): the tangent-linear and adjoint passes below evaluate the
 // component functions on values with a directional derivative.
 template <bool P2>
@@ -123,6 +172,8 @@ struct GAcc {
   template <int di, int dj, int dk> __device__ double v() const { return at<di, dj, dk>(fv); }
   template <int di, int dj, int dk> __device__ double w() const { return at<di, dj, dk>(fw); }
   template <int di, int dj, int dk> __device__ double p() const { return at<di, dj, dk>(fp); }
+  // the cell's own u, v or w (C = 0, 1, 2), to which the tendency is added
+  template <int C> __device__ double own() const { return at<0, 0, 0>(C == 0 ? fu : C == 1 ? fv : fw); }
   // grid spacings dx(i+o), dy(j+o), dz(k+o), o = 0, -1, -2
   template <int o> __device__ double hx() const { return a->dx[i + o]; }
   template <int o> __device__ double hy() const { return ((cdp)a->dy)[j + o]; }
@@ -141,12 +192,13 @@ struct GAcc {
 #define P(di, dj, dk) A.template p<di, dj, dk>()
 
 // ut (NavierStokes.f90:600-688)
-template <bool P2, class Acc>
-__device__ inline auto mom_u(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+template <bool P2, class Acc, class MC>
+__device__ inline auto mom_u(const MomArgs &a, const Acc &A, const MC &m, int i, int j, int k,
                                bool near_ok) {
   using T = std::decay_t<decltype(A.template u<0, 0, 0>())>;
-  const double alpha = 1.0;
-  const double dxr = m.dxr, dx = m.dx, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
+  using R = real_of<T>;
+  const R alpha = 1;
+  const R dxr = m.dxr, dx = m.dx, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
   const T u1a = U(-1, 0, 0) + U(0, 0, 0), u22 = U(-1, 0, 0) - U(0, 0, 0);
   const T u3 = U(0, 0, 0) + U(1, 0, 0), u4 = U(0, 0, 0) - U(1, 0, 0);
   const T u5 = U(0, -1, 0) + U(0, 0, 0), u6 = U(0, -1, 0) - U(0, 0, 0);
@@ -155,19 +207,19 @@ __device__ inline auto mom_u(const MomArgs &a, const Acc &A, const MomCell &m, i
   const T u11 = U(0, 0, 0) + U(0, 0, 1), u12 = U(0, 0, 0) - U(0, 0, 1);
   const T v1a = V(0, -1, 0) + V(1, -1, 0), v22 = V(0, 0, 0) + V(1, 0, 0);
   const T w1a = W(0, 0, -1) + W(1, 0, -1), w22 = W(0, 0, 0) + W(1, 0, 0);
-  const T dpdx = gdiv<P2>(P(0, 0, 0) - P(1, 0, 0), 0.5 * (dxr + dx));
-  const double dx2xr = dx + dxr, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt, dz2zb = dz + dzb;
+  const T dpdx = gdiv<P2>(P(0, 0, 0) - P(1, 0, 0), R(0.5) * (dxr + dx));
+  const R dx2xr = dx + dxr, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt, dz2zb = dz + dzb;
   const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
   T duudx, dvudy, dwudz;
   const T vu_e = V(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, 0, 0) - V(1, 0, 0));
   const T vu_w = V(1, -1, 0) + gdiv<P2>(dxr, dx2xr) * (V(0, -1, 0) - V(1, -1, 0));
-  const T v_in = 0.5 * (vu_e + vu_w);
+  const T v_in = R(0.5) * (vu_e + vu_w);
   const T wu_n = W(1, 0, 0) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, 0) - W(1, 0, 0));
   const T wu_s = W(1, 0, -1) + gdiv<P2>(dxr, dx2xr) * (W(0, 0, -1) - W(1, 0, -1));
-  const T w_in = 0.5 * (wu_n + wu_s);
+  const T w_in = R(0.5) * (wu_n + wu_s);
   if (i > 2 && i < nx && j > 2 && j < ny + 1 && k > 2 && k < nz + 1 && near_ok) {
-    const double ddy = 0.5 * (A.template hy<-1>() + A.template hy<-2>()), ddye = 0.5 * (A.template hy<-1>() + A.template hy<0>());
-    const double ddz = 0.5 * (A.template hz<-1>() + A.template hz<-2>()), ddzr = 0.5 * (A.template hz<-1>() + A.template hz<0>());
+    const R ddy = R(0.5) * (A.template hy<-1>() + A.template hy<-2>()), ddye = R(0.5) * (A.template hy<-1>() + A.template hy<0>());
+    const R ddz = R(0.5) * (A.template hz<-1>() + A.template hz<-2>()), ddzr = R(0.5) * (A.template hz<-1>() + A.template hz<0>());
     duudx = upwind3r(U(0, 0, 0), A.template xa<BTB_UU>(), A.template xk<BTB_UU>(), U(2, 0, 0), U(1, 0, 0), U(0, 0, 0),
                     U(-1, 0, 0), U(-2, 0, 0), A.template hx<0>(), A.template hx<-1>());
     dvudy = upwind3r(v_in, A.template ya<BTB_VU>(), A.template yk<BTB_VU>(), U(0, 2, 0), U(0, 1, 0), U(0, 0, 0),
@@ -175,25 +227,26 @@ __device__ inline auto mom_u(const MomArgs &a, const Acc &A, const MomCell &m, i
     dwudz = upwind3r(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), U(0, 0, 2), U(0, 0, 1), U(0, 0, 0),
                     U(0, 0, -1), U(0, 0, -2), ddzr, ddz);
   } else {
-    duudx = gdiv<P2>(-0.25 * (u1a * u1a + alpha * fabs(u1a) * u22 - u3 * u3 - alpha * fabs(u3) * u4), A.template hx<-1>());
-    dvudy = gdiv<P2>(-0.25 * (v1a * u5 + alpha * fabs(v1a) * u6 - v22 * u7 - alpha * fabs(v22) * u8), A.template hy<-1>());
-    dwudz = gdiv<P2>(-0.25 * (w1a * u9 + alpha * fabs(w1a) * u10 - w22 * u11 - alpha * fabs(w22) * u12), A.template hz<-1>());
+    duudx = gdiv<P2>(R(-0.25) * (u1a * u1a + alpha * fabs(u1a) * u22 - u3 * u3 - alpha * fabs(u3) * u4), A.template hx<-1>());
+    dvudy = gdiv<P2>(R(-0.25) * (v1a * u5 + alpha * fabs(v1a) * u6 - v22 * u7 - alpha * fabs(v22) * u8), A.template hy<-1>());
+    dwudz = gdiv<P2>(R(-0.25) * (w1a * u9 + alpha * fabs(w1a) * u10 - w22 * u11 - alpha * fabs(w22) * u12), A.template hz<-1>());
   }
-  const T d2x = gdiv<P2>(2.0, dx2xr) * (gdiv<P2>(-u4, dxr) + gdiv<P2>(u22, dx));
-  const T d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-u8, 0.5 * dy2ye) + gdiv<P2>(u6, 0.5 * dy2yw));
-  const T d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-u12, 0.5 * dz2zt) + gdiv<P2>(u10, 0.5 * dz2zb));
-  const T visc = (d2x + d2y + d2z) / a.re;
+  const T d2x = gdiv<P2>(R(2), dx2xr) * (gdiv<P2>(-u4, dxr) + gdiv<P2>(u22, dx));
+  const T d2y = gdiv<P2>(R(2), R(0.5) * (dy2ye + dy2yw)) * (gdiv<P2>(-u8, R(0.5) * dy2ye) + gdiv<P2>(u6, R(0.5) * dy2yw));
+  const T d2z = gdiv<P2>(R(2), R(0.5) * (dz2zt + dz2zb)) * (gdiv<P2>(-u12, R(0.5) * dz2zt) + gdiv<P2>(u10, R(0.5) * dz2zb));
+  const T visc = qdiv(d2x + d2y + d2z, R(a.re));
   const T res = (-duudx - dvudy - dwudz + visc);
-  return U(0, 0, 0) + a.dt * (res + dpdx);
+  return A.template own<0>() + a.dt * (res + dpdx);
 }
 
 // vt (NavierStokes.f90:690-780)
-template <bool P2, class Acc>
-__device__ inline auto mom_v(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+template <bool P2, class Acc, class MC>
+__device__ inline auto mom_v(const MomArgs &a, const Acc &A, const MC &m, int i, int j, int k,
                                bool near_ok) {
   using T = std::decay_t<decltype(A.template u<0, 0, 0>())>;
-  const double alpha = 1.0;
-  const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
+  using R = real_of<T>;
+  const R alpha = 1;
+  const R dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dzt = m.dzt, dz = m.dz, dzb = m.dzb;
   const T u13 = U(-1, 0, 0) + U(-1, 1, 0), u14 = U(0, 0, 0) + U(0, 1, 0);
   const T v3 = V(-1, 0, 0) + V(0, 0, 0), v4 = V(-1, 0, 0) - V(0, 0, 0);
   const T v5 = V(0, 0, 0) + V(1, 0, 0), v6 = V(0, 0, 0) - V(1, 0, 0);
@@ -202,19 +255,19 @@ __device__ inline auto mom_v(const MomArgs &a, const Acc &A, const MomCell &m, i
   const T v11 = V(0, 0, -1) + V(0, 0, 0), v12 = V(0, 0, -1) - V(0, 0, 0);
   const T v13 = V(0, 0, 0) + V(0, 0, 1), v14 = V(0, 0, 0) - V(0, 0, 1);
   const T w3 = W(0, 0, -1) + W(0, 1, -1), w4 = W(0, 0, 0) + W(0, 1, 0);
-  const T dpdy = gdiv<P2>(P(0, 0, 0) - P(0, 1, 0), 0.5 * (dye + dy));
-  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dz2zt = dz + dzt, dz2zb = dz + dzb;
+  const T dpdy = gdiv<P2>(P(0, 0, 0) - P(0, 1, 0), R(0.5) * (dye + dy));
+  const R dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dz2zt = dz + dzt, dz2zb = dz + dzb;
   const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
   const T uv_e = U(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(0, 0, 0) - U(0, 1, 0));
   const T uv_w = U(-1, 1, 0) + gdiv<P2>(dye, dy2ye) * (U(-1, 0, 0) - U(-1, 1, 0));
-  const T u_in = 0.5 * (uv_e + uv_w);
+  const T u_in = R(0.5) * (uv_e + uv_w);
   const T wv_n = W(0, 1, 0) + gdiv<P2>(dye, dy2ye) * (W(0, 0, 0) - W(0, 1, 0));
   const T wv_s = W(0, 1, -1) + gdiv<P2>(dye, dy2ye) * (W(0, 0, -1) - W(0, 1, -1));
-  const T w_in = 0.5 * (wv_n + wv_s);
+  const T w_in = R(0.5) * (wv_n + wv_s);
   T duvdx, dvvdy, dwvdz;
   if (i > 2 && i < nx + 1 && j > 2 && j < ny && k > 2 && k < nz + 1 && near_ok) {
-    const double ddx = 0.5 * (A.template hx<-1>() + A.template hx<-2>()), ddxr = 0.5 * (A.template hx<-1>() + A.template hx<0>());
-    const double ddz = 0.5 * (A.template hz<-1>() + A.template hz<-2>()), ddzr = 0.5 * (A.template hz<-1>() + A.template hz<0>());
+    const R ddx = R(0.5) * (A.template hx<-1>() + A.template hx<-2>()), ddxr = R(0.5) * (A.template hx<-1>() + A.template hx<0>());
+    const R ddz = R(0.5) * (A.template hz<-1>() + A.template hz<-2>()), ddzr = R(0.5) * (A.template hz<-1>() + A.template hz<0>());
     duvdx = upwind3r(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), V(2, 0, 0), V(1, 0, 0), V(0, 0, 0),
                     V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx);
     dvvdy = upwind3r(V(0, 0, 0), A.template ya<BTB_VV>(), A.template yk<BTB_VV>(), V(0, 2, 0), V(0, 1, 0), V(0, 0, 0),
@@ -222,25 +275,26 @@ __device__ inline auto mom_v(const MomArgs &a, const Acc &A, const MomCell &m, i
     dwvdz = upwind3r(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), V(0, 0, 2), V(0, 0, 1), V(0, 0, 0),
                     V(0, 0, -1), V(0, 0, -2), ddzr, ddz);
   } else {
-    duvdx = gdiv<P2>(-0.25 * (u13 * v3 + alpha * fabs(u13) * v4 - u14 * v5 - alpha * fabs(u14) * v6), A.template hx<-1>());
-    dvvdy = gdiv<P2>(-0.25 * (v7 * v7 + alpha * fabs(v7) * v8 - v9 * v9 - alpha * fabs(v9) * v10), A.template hy<-1>());
-    dwvdz = gdiv<P2>(-0.25 * (w3 * v11 + alpha * fabs(w3) * v12 - w4 * v13 - alpha * fabs(w4) * v14), A.template hz<-1>());
+    duvdx = gdiv<P2>(R(-0.25) * (u13 * v3 + alpha * fabs(u13) * v4 - u14 * v5 - alpha * fabs(u14) * v6), A.template hx<-1>());
+    dvvdy = gdiv<P2>(R(-0.25) * (v7 * v7 + alpha * fabs(v7) * v8 - v9 * v9 - alpha * fabs(v9) * v10), A.template hy<-1>());
+    dwvdz = gdiv<P2>(R(-0.25) * (w3 * v11 + alpha * fabs(w3) * v12 - w4 * v13 - alpha * fabs(w4) * v14), A.template hz<-1>());
   }
-  const T d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-v6, 0.5 * dx2xr) + gdiv<P2>(v4, 0.5 * dx2xl));
-  const T d2y = gdiv<P2>(2.0, dy2ye) * (gdiv<P2>(-v10, dye) + gdiv<P2>(v8, dy));
-  const T d2z = gdiv<P2>(2.0, 0.5 * (dz2zt + dz2zb)) * (gdiv<P2>(-v14, 0.5 * dz2zt) + gdiv<P2>(v12, 0.5 * dz2zb));
-  const T visc = (d2x + d2y + d2z) / a.re;
+  const T d2x = gdiv<P2>(R(2), R(0.5) * (dx2xr + dx2xl)) * (gdiv<P2>(-v6, R(0.5) * dx2xr) + gdiv<P2>(v4, R(0.5) * dx2xl));
+  const T d2y = gdiv<P2>(R(2), dy2ye) * (gdiv<P2>(-v10, dye) + gdiv<P2>(v8, dy));
+  const T d2z = gdiv<P2>(R(2), R(0.5) * (dz2zt + dz2zb)) * (gdiv<P2>(-v14, R(0.5) * dz2zt) + gdiv<P2>(v12, R(0.5) * dz2zb));
+  const T visc = qdiv(d2x + d2y + d2z, R(a.re));
   const T res = (-duvdx - dvvdy - dwvdz + visc);
-  return V(0, 0, 0) + a.dt * (res + dpdy);
+  return A.template own<1>() + a.dt * (res + dpdy);
 }
 
 // wt (NavierStokes.f90:782-858)
-template <bool P2, class Acc>
-__device__ inline auto mom_w(const MomArgs &a, const Acc &A, const MomCell &m, int i, int j, int k,
+template <bool P2, class Acc, class MC>
+__device__ inline auto mom_w(const MomArgs &a, const Acc &A, const MC &m, int i, int j, int k,
                                bool near_ok) {
   using T = std::decay_t<decltype(A.template u<0, 0, 0>())>;
-  const double alpha = 1.0;
-  const double dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz;
+  using R = real_of<T>;
+  const R alpha = 1;
+  const R dxr = m.dxr, dx = m.dx, dxl = m.dxl, dye = m.dye, dy = m.dy, dyw = m.dyw, dzt = m.dzt, dz = m.dz;
   const T u15 = U(-1, 0, 0) + U(-1, 0, 1), u16 = U(0, 0, 0) + U(0, 0, 1);
   const T v15 = V(0, -1, 0) + V(0, -1, 1), v16 = V(0, 0, 0) + V(0, 0, 1);
   const T w5 = W(-1, 0, 0) + W(0, 0, 0), w6 = W(-1, 0, 0) - W(0, 0, 0);
@@ -249,19 +303,19 @@ __device__ inline auto mom_w(const MomArgs &a, const Acc &A, const MomCell &m, i
   const T w11 = W(0, 0, 0) + W(0, 1, 0), w12 = W(0, 0, 0) - W(0, 1, 0);
   const T w13 = W(0, 0, -1) + W(0, 0, 0), w14 = W(0, 0, -1) - W(0, 0, 0);
   const T w15 = W(0, 0, 0) + W(0, 0, 1), w16 = W(0, 0, 0) - W(0, 0, 1);
-  const T dpdz = gdiv<P2>(P(0, 0, 0) - P(0, 0, 1), 0.5 * (dzt + dz));
-  const double dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt;
+  const T dpdz = gdiv<P2>(P(0, 0, 0) - P(0, 0, 1), R(0.5) * (dzt + dz));
+  const R dx2xr = dx + dxr, dx2xl = dx + dxl, dy2ye = dy + dye, dy2yw = dy + dyw, dz2zt = dz + dzt;
   const int nx = a.d.nx, ny = a.d.ny, nz = a.d.nz;
   const T uw_e = U(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(0, 0, 0) - U(0, 0, 1));
   const T uw_w = U(-1, 0, 1) + gdiv<P2>(dzt, dz2zt) * (U(-1, 0, 0) - U(-1, 0, 1));
-  const T u_in = 0.5 * (uw_e + uw_w);
+  const T u_in = R(0.5) * (uw_e + uw_w);
   const T vw_n = V(0, 0, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, 0, 0) - V(0, 0, 1));
   const T vw_s = V(0, -1, 1) + gdiv<P2>(dzt, dz2zt) * (V(0, -1, 0) - V(0, -1, 1));
-  const T v_in = 0.5 * (vw_n + vw_s);
+  const T v_in = R(0.5) * (vw_n + vw_s);
   T duwdx, dvwdy, dwwdz;
   if (i > 2 && i < nx + 1 && j > 2 && j < ny + 1 && k > 2 && k < nz && near_ok) {
-    const double ddx = 0.5 * (A.template hx<-1>() + A.template hx<-2>()), ddxr = 0.5 * (A.template hx<-1>() + A.template hx<0>());
-    const double ddy = 0.5 * (A.template hy<-1>() + A.template hy<-2>()), ddye = 0.5 * (A.template hy<-1>() + A.template hy<0>());
+    const R ddx = R(0.5) * (A.template hx<-1>() + A.template hx<-2>()), ddxr = R(0.5) * (A.template hx<-1>() + A.template hx<0>());
+    const R ddy = R(0.5) * (A.template hy<-1>() + A.template hy<-2>()), ddye = R(0.5) * (A.template hy<-1>() + A.template hy<0>());
     duwdx = upwind3r(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), W(2, 0, 0), W(1, 0, 0), W(0, 0, 0),
                     W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx);
     dvwdy = upwind3r(v_in, A.template ya<BTB_VW>(), A.template yk<BTB_VW>(), W(0, 2, 0), W(0, 1, 0), W(0, 0, 0),
@@ -269,16 +323,16 @@ __device__ inline auto mom_w(const MomArgs &a, const Acc &A, const MomCell &m, i
     dwwdz = upwind3r(W(0, 0, 0), A.template za<BTB_WW>(), A.template zk<BTB_WW>(), W(0, 0, 2), W(0, 0, 1), W(0, 0, 0),
                     W(0, 0, -1), W(0, 0, -2), A.template hz<0>(), A.template hz<-1>());
   } else {
-    duwdx = gdiv<P2>(-0.5 * (u15 * w5 + alpha * fabs(u15) * w6 - u16 * w7 - alpha * fabs(u16) * w8), A.template hx<-1>());
-    dvwdy = gdiv<P2>(-0.5 * (v15 * w9 + alpha * fabs(v15) * w10 - v16 * w11 - alpha * fabs(v16) * w12), A.template hy<-1>());
-    dwwdz = gdiv<P2>(-0.5 * (w13 * w13 + alpha * fabs(w13) * w14 - w15 * w15 - alpha * fabs(w15) * w16), A.template hz<-1>());
+    duwdx = gdiv<P2>(-R(0.5) * (u15 * w5 + alpha * fabs(u15) * w6 - u16 * w7 - alpha * fabs(u16) * w8), A.template hx<-1>());
+    dvwdy = gdiv<P2>(-R(0.5) * (v15 * w9 + alpha * fabs(v15) * w10 - v16 * w11 - alpha * fabs(v16) * w12), A.template hy<-1>());
+    dwwdz = gdiv<P2>(-R(0.5) * (w13 * w13 + alpha * fabs(w13) * w14 - w15 * w15 - alpha * fabs(w15) * w16), A.template hz<-1>());
   }
-  const T d2x = gdiv<P2>(2.0, 0.5 * (dx2xr + dx2xl)) * (gdiv<P2>(-w8, 0.5 * dx2xr) + gdiv<P2>(w6, 0.5 * dx2xl));
-  const T d2y = gdiv<P2>(2.0, 0.5 * (dy2ye + dy2yw)) * (gdiv<P2>(-w12, 0.5 * dy2ye) + gdiv<P2>(w10, 0.5 * dy2yw));
-  const T d2z = gdiv<P2>(2.0, dz2zt) * (gdiv<P2>(-w16, dzt) + gdiv<P2>(w14, dz));
-  const T visc = (d2x + d2y + d2z) / a.re;
+  const T d2x = gdiv<P2>(R(2), R(0.5) * (dx2xr + dx2xl)) * (gdiv<P2>(-w8, R(0.5) * dx2xr) + gdiv<P2>(w6, R(0.5) * dx2xl));
+  const T d2y = gdiv<P2>(R(2), R(0.5) * (dy2ye + dy2yw)) * (gdiv<P2>(-w12, R(0.5) * dy2ye) + gdiv<P2>(w10, R(0.5) * dy2yw));
+  const T d2z = gdiv<P2>(R(2), dz2zt) * (gdiv<P2>(-w16, dzt) + gdiv<P2>(w14, dz));
+  const T visc = qdiv(d2x + d2y + d2z, R(a.re));
   const T res = (-duwdx - dvwdy - dwwdz + visc);
-  return W(0, 0, 0) + a.dt * (res + dpdz);
+  return A.template own<2>() + a.dt * (res + dpdz);
 }
 #undef U
 #undef V
@@ -383,11 +437,11 @@ __global__ void __launch_bounds__(256) k_momentum_c(MomArgs a, Fields F, double 
 // are staged as 0: the stencil reads them only where its index windows have
 // already failed, as the per-plane kernels do.  Same arithmetic, same operand
 // order: same bits.
-template <int TJ>
+template <int TJ, class R = double>
 struct MarchPlane {
   static constexpr int kW = 68, kH = TJ + 4;   // two-cell halo box
-  double f[3][kH * kW];                        // u, v, w of plane k
-  double p[(TJ + 1) * 65];                     // p: rows j..j+TJ, columns i..i+64
+  R f[3][kH * kW];                             // u, v, w of plane k
+  R p[(TJ + 1) * 65];                          // p: rows j..j+TJ, columns i..i+64
   uint8_t c1[(TJ + 2) * 66];                   // cell: rows j-1..j+TJ, columns i-1..i+64
   uint8_t c2[kH * kW];                         // cell2
 };
@@ -412,77 +466,82 @@ __device__ inline void halo_pos(int q, int &r, int &c) {
   }
 }
 
-template <int TJ>
+// The march's operand accessor.  R = double: the FP64 stencil; R = float:
+// the FP32 tendency, whose LDS planes and tables hold FP32 values and whose
+// own-column values (kept in FP64 for the final update) are rounded to FP32
+// where the stencil reads them.
+template <int TJ, class R = double>
 struct MAcc {
-  const double *su, *sv, *sw;   // own position in the staged u, v, w planes
-  const double *sp;             // own position in the staged p plane
-  const double *xt;             // own row of the staged x tables
+  const R *su, *sv, *sw;        // own position in the staged u, v, w planes
+  const R *sp;                  // own position in the staged p plane
+  const R *xt;                  // own row of the staged x tables
   const MomArgs *a;
   int j, k;
-  double dxr, dxc, dxl;           // dx(i), dx(i-1), dx(i-2)
+  R dxr, dxc, dxl;                // dx(i), dx(i-1), dx(i-2)
   double um2, um1, u0, up1, up2, vm2, vm1, v0, vp1, vp2, wm2, wm1, w0, wp1, wp2, p0, pp1;
   double u_wt, v_st, w_eb, w_nb;   // u(i-1,j,k+1), v(i,j-1,k+1), w(i+1,j,k-1), w(i,j+1,k-1)
 
   template <int dk>
-  __device__ static double col(double m2, double m1, double p1, double p2) {
-    if constexpr (dk == -2) return m2;
-    else if constexpr (dk == -1) return m1;
-    else if constexpr (dk == 1) return p1;
+  __device__ static R col(double m2, double m1, double p1, double p2) {
+    if constexpr (dk == -2) return R(m2);
+    else if constexpr (dk == -1) return R(m1);
+    else if constexpr (dk == 1) return R(p1);
     else {
       static_assert(dk == 2, "own-column offset");
-      return p2;
+      return R(p2);
     }
   }
   template <int di, int dj, int dk>
-  __device__ double u() const {
+  __device__ R u() const {
     if constexpr (dk == 0) {
-      if constexpr (di == 0 && dj == 0) return u0;
-      else return su[dj * MarchPlane<TJ>::kW + di];
+      if constexpr (di == 0 && dj == 0) return R(u0);
+      else return su[dj * MarchPlane<TJ, R>::kW + di];
     } else if constexpr (di == 0 && dj == 0) {
       return col<dk>(um2, um1, up1, up2);
     } else {
       static_assert(di == -1 && dj == 0 && dk == 1, "u operand off the plane");
-      return u_wt;
+      return R(u_wt);
     }
   }
   template <int di, int dj, int dk>
-  __device__ double v() const {
+  __device__ R v() const {
     if constexpr (dk == 0) {
-      if constexpr (di == 0 && dj == 0) return v0;
-      else return sv[dj * MarchPlane<TJ>::kW + di];
+      if constexpr (di == 0 && dj == 0) return R(v0);
+      else return sv[dj * MarchPlane<TJ, R>::kW + di];
     } else if constexpr (di == 0 && dj == 0) {
       return col<dk>(vm2, vm1, vp1, vp2);
     } else {
       static_assert(di == 0 && dj == -1 && dk == 1, "v operand off the plane");
-      return v_st;
+      return R(v_st);
     }
   }
   template <int di, int dj, int dk>
-  __device__ double w() const {
+  __device__ R w() const {
     if constexpr (dk == 0) {
-      if constexpr (di == 0 && dj == 0) return w0;
-      else return sw[dj * MarchPlane<TJ>::kW + di];
+      if constexpr (di == 0 && dj == 0) return R(w0);
+      else return sw[dj * MarchPlane<TJ, R>::kW + di];
     } else if constexpr (di == 0 && dj == 0) {
       return col<dk>(wm2, wm1, wp1, wp2);
     } else if constexpr (di == 1 && dj == 0 && dk == -1) {
-      return w_eb;
+      return R(w_eb);
     } else {
       static_assert(di == 0 && dj == 1 && dk == -1, "w operand off the plane");
-      return w_nb;
+      return R(w_nb);
     }
   }
+  template <int C> __device__ double own() const { return C == 0 ? u0 : C == 1 ? v0 : w0; }
   template <int di, int dj, int dk>
-  __device__ double p() const {
-    if constexpr (di == 0 && dj == 0 && dk == 0) return p0;
+  __device__ R p() const {
+    if constexpr (di == 0 && dj == 0 && dk == 0) return R(p0);
     else if constexpr (di == 1 && dj == 0 && dk == 0) return sp[1];
     else if constexpr (di == 0 && dj == 1 && dk == 0) return sp[65];
     else {
       static_assert(di == 0 && dj == 0 && dk == 1, "p operand");
-      return pp1;
+      return R(pp1);
     }
   }
   template <int o>
-  __device__ double hx() const {
+  __device__ R hx() const {
     if constexpr (o == 0) return dxr;
     else if constexpr (o == -1) return dxc;
     else {
@@ -490,25 +549,25 @@ struct MAcc {
       return dxl;
     }
   }
-  template <int o> __device__ double hy() const { return ((cdp)a->dy)[j + o]; }
-  template <int o> __device__ double hz() const { return ((cdp)a->dz)[k + o]; }
+  template <int o> __device__ R hy() const { return ((cptr<R>)mom_h<R>(*a, 1))[j + o]; }
+  template <int o> __device__ R hz() const { return ((cptr<R>)mom_h<R>(*a, 2))[k + o]; }
   template <int T>
-  __device__ const double *xa() const {
+  __device__ const R *xa() const {
     static_assert(T == BTB_UU || T == BTB_UW, "staged x table");
     return xt + (T == BTB_UU ? 0 : 2) * 384;
   }
   template <int T>
-  __device__ const double *xk() const {
+  __device__ const R *xk() const {
     static_assert(T == BTB_UU || T == BTB_UW, "staged x table");
     return xt + (T == BTB_UU ? 1 : 3) * 384;
   }
-  template <int T> __device__ cdp ya() const { return (cdp)(a->ca[T] + 6 * (j - 1)); }
-  template <int T> __device__ cdp yk() const { return (cdp)(a->ck[T] + 6 * (j - 1)); }
-  template <int T> __device__ cdp za() const { return (cdp)(a->ca[T] + 6 * (k - 1)); }
-  template <int T> __device__ cdp zk() const { return (cdp)(a->ck[T] + 6 * (k - 1)); }
+  template <int T> __device__ cptr<R> ya() const { return (cptr<R>)(mom_ca<R>(*a, T) + 6 * (j - 1)); }
+  template <int T> __device__ cptr<R> yk() const { return (cptr<R>)(mom_ck<R>(*a, T) + 6 * (j - 1)); }
+  template <int T> __device__ cptr<R> za() const { return (cptr<R>)(mom_ca<R>(*a, T) + 6 * (k - 1)); }
+  template <int T> __device__ cptr<R> zk() const { return (cptr<R>)(mom_ck<R>(*a, T) + 6 * (k - 1)); }
 };
 
-template <int TJ, int KC, int WPE, bool P2>
+template <int TJ, int KC, int WPE, bool P2, class R = double>
 __global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt,
                  const uint8_t *__restrict__ deferred, int gx, int gy) {
@@ -517,8 +576,8 @@ k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restric
   static_assert(NH1 <= NT && NHP <= NT, "one-cell halos: one position per thread");
   // planes alternate between two buffers, so one barrier per plane separates
   // the staging of plane k from the reads of plane k - 2
-  __shared__ MarchPlane<TJ> S[2];
-  __shared__ double tab[4][64 * 6];   // ca UU, ck UU, ca UW, ck UW rows of the tile's columns
+  __shared__ MarchPlane<TJ, R> S[2];
+  __shared__ R tab[4][64 * 6];   // ca UU, ck UU, ca UW, ck UW rows of the tile's columns
   int ti, tj, tc;
   mom_tile(gx, gy, ti, tj, tc);
   const Dims &d = a.d;
@@ -592,7 +651,7 @@ k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restric
     h1 = hg1 >= 0 ? a.cell[hg1 + pk] : (uint8_t)0;
     hp = hgp >= 0 ? F.p[hgp + pk] : 0.0;
   };
-  auto store_halo = [&](MarchPlane<TJ> &P) {
+  auto store_halo = [&](MarchPlane<TJ, R> &P) {
 #pragma unroll
     for (int q2 = 0; q2 < S2; ++q2) {
       if (ho2[q2] >= 0) {
@@ -609,16 +668,17 @@ k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restric
   // x-direction table rows of the tile's columns (rows 1..nx+2 exist)
   for (int q = tid; q < 4 * 384; q += NT) {
     const int tq = q / 384, r = q % 384, colx = i0 + r / 6;
-    const double *src = tq == 0 ? a.ca[BTB_UU] : tq == 1 ? a.ck[BTB_UU] : tq == 2 ? a.ca[BTB_UW] : a.ck[BTB_UW];
-    double x = 0.0;
+    const R *src = tq == 0 ? mom_ca<R>(a, BTB_UU) : tq == 1 ? mom_ck<R>(a, BTB_UU) : tq == 2 ? mom_ca<R>(a, BTB_UW) : mom_ck<R>(a, BTB_UW);
+    R x = 0;
     if (colx <= d.nx + 2) x = src[6 * (colx - 1) + r % 6];
     tab[tq][r] = x;
   }
-  double dxr = 0.0, dxc = 0.0, dxl = 0.0;
+  R dxr = 0, dxc = 0, dxl = 0;
   if (mine) {
-    dxr = a.dx[i];
-    dxc = a.dx[i - 1];
-    dxl = a.dx[i - 2];
+    const R *hx = mom_h<R>(a, 0);
+    dxr = hx[i];
+    dxc = hx[i - 1];
+    dxl = hx[i - 2];
   }
   double um2 = ldf(F.u, kb - 2), um1 = ldf(F.u, kb - 1), u0 = ldf(F.u, kb), up1 = ldf(F.u, kb + 1),
          up2 = ldf(F.u, kb + 2);
@@ -649,7 +709,7 @@ k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restric
 
   const int o2 = (row + 2) * kW + lane + 2, o1 = (row + 1) * 66 + lane + 1, op = row * 65 + lane;
   for (int k = kb; k < ke; ++k) {
-    MarchPlane<TJ> &P = S[(k - kb) & 1];
+    MarchPlane<TJ, R> &P = S[(k - kb) & 1];
     const int64_t c = c1 + (int64_t)(k - 1) * sz;
     load_off(k, cc0, dfr, u_wt, v_st, w_eb, w_nb);
     P.f[0][o2] = u0;
@@ -662,19 +722,20 @@ k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restric
     __syncthreads();
     if (k + 1 < ke) load_halo(k + 1);
     if (mine && cc0 == 0 && dfr == 0) {
-      MomCell m;
+      MomCellT<R> m;
       m.c = c;
       m.sy = d.NX;
       m.sz = sz;
       m.dxr = dxr; m.dx = dxc; m.dxl = dxl;
-      m.dye = ((cdp)a.dy)[j]; m.dy = ((cdp)a.dy)[j - 1]; m.dyw = ((cdp)a.dy)[j - 2];
-      m.dzt = ((cdp)a.dz)[k]; m.dz = ((cdp)a.dz)[k - 1]; m.dzb = ((cdp)a.dz)[k - 2];
+      const cptr<R> hy = (cptr<R>)mom_h<R>(a, 1), hz = (cptr<R>)mom_h<R>(a, 2);
+      m.dye = hy[j]; m.dy = hy[j - 1]; m.dyw = hy[j - 2];
+      m.dzt = hz[k]; m.dz = hz[k - 1]; m.dzb = hz[k - 2];
       const uint8_t *s1 = &P.c1[o1], *s2 = &P.c2[o2];
       const unsigned hit1 = (s1[-1] == 2) | (s1[1] == 2) | (s1[-66] == 2) | (s1[66] == 2) | (cm1 == 2) | (cp1 == 2);
       const unsigned hit2 = (s2[-2] == 2) | (s2[2] == 2) | (s2[-2 * kW] == 2) | (s2[2 * kW] == 2) | (c2m2 == 2) |
                             (c2p2 == 2);
       const bool near_ok = (i > 2) & (j > 2) & (k > 2) & ((hit1 | hit2) == 0u);
-      MAcc<TJ> A;
+      MAcc<TJ, R> A;
       A.su = &P.f[0][o2];
       A.sv = &P.f[1][o2];
       A.sw = &P.f[2][o2];
@@ -1023,6 +1084,7 @@ struct TsAcc {
   template <int di, int dj, int dk> __device__ double v() const { return rd(1, di, dj, dk); }
   template <int di, int dj, int dk> __device__ double w() const { return rd(2, di, dj, dk); }
   template <int di, int dj, int dk> __device__ double p() const { return rd(3, di, dj, dk); }
+  template <int C> __device__ double own() const { return rd(C, 0, 0, 0); }
   template <int o> __device__ double hx() const { return a->dx[i + o]; }
   template <int o> __device__ double hy() const { return a->dy[j + o]; }
   template <int o> __device__ double hz() const { return a->dz[k + o]; }
@@ -1094,6 +1156,8 @@ int g_ts_parallel = 1;
 int g_mom_variant = 4;   // the fastest at 256^3 on MI355X (0.72 ms against 1.02 ms for variant 0)
 // grid divisions as products by exact reciprocals where the block's spacings allow
 int g_mom_pow2 = 1;
+// 1: the dense pass evaluates the tendency in FP32 (biocfd_momentum_precision)
+int g_mom_fp32 = 0;
 
 // every spacing of every axis is one and the same power of two per axis
 // (the grid divisors of the momentum stencil are then exact powers of two)
@@ -1110,15 +1174,20 @@ bool pow2_grid(const Block &b) {
   return true;
 }
 
-template <int TJ, int KC, int WPE, bool P2>
+template <int TJ, int KC, int WPE, bool P2, class R = double>
 void launch_march(const MomArgs &a, const Fields &F, Block *b, hipStream_t s, const uint8_t *deferred) {
   const int gx = (int)div_up(b->d.nx, 64), gy = (int)div_up(b->d.ny, TJ), gz = (int)div_up(b->d.nz, KC);
-  hipLaunchKernelGGL((k_momentum_march<TJ, KC, WPE, P2>), dim3((unsigned)((int64_t)gx * gy * gz)), dim3(64, TJ, 1), 0, s,
-                     a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], deferred, gx, gy);
+  hipLaunchKernelGGL((k_momentum_march<TJ, KC, WPE, P2, R>), dim3((unsigned)((int64_t)gx * gy * gz)), dim3(64, TJ, 1), 0,
+                     s, a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], deferred, gx, gy);
 }
 
 template <bool P2>
 void launch_dense_p2(const MomArgs &a, const Fields &F, Block *b, hipStream_t s, const uint8_t *deferred) {
+  if (g_mom_fp32) {
+    // FP32 tendency of the dense pass (the deferred TS-adjacent cells stay FP64)
+    launch_march<4, 32, 2, P2, float>(a, F, b, s, deferred);
+    return;
+  }
   switch (g_mom_variant) {
     case 3: launch_march<4, 16, 2, P2>(a, F, b, s, deferred); return;
     case 4: launch_march<4, 32, 2, P2>(a, F, b, s, deferred); return;
@@ -1149,6 +1218,16 @@ void launch_dense(const MomArgs &a, const Fields &F, Block *b, hipStream_t s, co
     launch_dense_p2<false>(a, F, b, s, deferred);
 }
 }  // namespace
+
+extern "C" int biocfd_momentum_precision(int fp32) {
+  BCFD_API_LOCK();
+  if (fp32 != 0 && fp32 != 1) {
+    set_error("momentum_precision: %d is neither 0 (FP64) nor 1 (FP32 tendency)", fp32);
+    return -1;
+  }
+  g_mom_fp32 = fp32;
+  return 0;
+}
 
 extern "C" int biocfd_momentum_config(int variant, int pow2_products) {
   BCFD_API_LOCK();
@@ -1184,7 +1263,11 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   a.cell = b->tag[BT_CELL];
   a.cell2 = b->tag[BT_CELL2];
   a.dx = b->g[BG_DX]; a.dy = b->g[BG_DY]; a.dz = b->g[BG_DZ];
-  for (int t = 0; t < BTB_N; ++t) { a.ca[t] = b->ca[t]; a.ck[t] = b->ck[t]; }
+  for (int t = 0; t < BTB_N; ++t) {
+    a.ca[t] = b->ca[t]; a.ck[t] = b->ck[t];
+    a.ca32[t] = b->ca32[t]; a.ck32[t] = b->ck32[t];
+  }
+  a.dx32 = b->h32[0]; a.dy32 = b->h32[1]; a.dz32 = b->h32[2];
   a.dt = dt;
   a.re = re;
   Fields F{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
@@ -1279,6 +1362,11 @@ struct DAcc : GAcc {
   __device__ Dual w() const { return {this->template at<di, dj, dk>(fw), this->template at<di, dj, dk>(tw)}; }
   template <int di, int dj, int dk>
   __device__ Dual p() const { return {this->template at<di, dj, dk>(fp), this->template at<di, dj, dk>(tp)}; }
+  template <int C> __device__ Dual own() const {
+    if constexpr (C == 0) return u<0, 0, 0>();
+    else if constexpr (C == 1) return v<0, 0, 0>();
+    else return w<0, 0, 0>();
+  }
 };
 
 // every operand position of the three component functions: 15 of u, v and
@@ -1657,6 +1745,7 @@ struct TsDAcc {
   template <int di, int dj, int dk> __device__ Dual v() const { return rd(1, di, dj, dk); }
   template <int di, int dj, int dk> __device__ Dual w() const { return rd(2, di, dj, dk); }
   template <int di, int dj, int dk> __device__ Dual p() const { return rd(3, di, dj, dk); }
+  template <int C> __device__ Dual own() const { return rd(C, 0, 0, 0); }
   template <int o> __device__ double hx() const { return base.template hx<o>(); }
   template <int o> __device__ double hy() const { return base.template hy<o>(); }
   template <int o> __device__ double hz() const { return base.template hz<o>(); }
@@ -1830,7 +1919,11 @@ static int momentum_args(Block &b, double dt, double re, MomArgs &a) {
   a.cell = b.tag[BT_CELL];
   a.cell2 = b.tag[BT_CELL2];
   a.dx = b.g[BG_DX]; a.dy = b.g[BG_DY]; a.dz = b.g[BG_DZ];
-  for (int t = 0; t < BTB_N; ++t) { a.ca[t] = b.ca[t]; a.ck[t] = b.ck[t]; }
+  for (int t = 0; t < BTB_N; ++t) {
+    a.ca[t] = b.ca[t]; a.ck[t] = b.ck[t];
+    a.ca32[t] = b.ca32[t]; a.ck32[t] = b.ck32[t];
+  }
+  a.dx32 = b.h32[0]; a.dy32 = b.h32[1]; a.dz32 = b.h32[2];
   a.dt = dt;
   a.re = re;
   return 0;

@@ -229,8 +229,12 @@ static void free_block(Block &b) {
   for (int t = 0; t < BTB_N; ++t) {
     if (b.ca[t]) (void)hipFree(b.ca[t]);
     if (b.ck[t]) (void)hipFree(b.ck[t]);
+    if (b.ca32[t]) (void)hipFree(b.ca32[t]);
+    if (b.ck32[t]) (void)hipFree(b.ck32[t]);
     b.ca[t] = b.ck[t] = nullptr;
+    b.ca32[t] = b.ck32[t] = nullptr;
   }
+  for (auto &p : b.h32) if (p) { (void)hipFree(p); p = nullptr; }
   if (b.sor) (void)hipFree(b.sor);
   if (b.scan_tmp) (void)hipFree(b.scan_tmp);
   if (b.list_tmp) (void)hipFree(b.list_tmp);
@@ -499,6 +503,18 @@ int biocfd_non_uni_coeff(int id) {
     BCFD_HIP(hipMemcpyAsync(b->ca[t], b->hca[t].data(), sizeof(double) * m, hipMemcpyHostToDevice, g_rt.stream));
     BCFD_HIP(hipMemcpyAsync(b->ck[t], b->hck[t].data(), sizeof(double) * m, hipMemcpyHostToDevice, g_rt.stream));
   }
+  // FP32 copies of the tables and spacings for the FP32 momentum tendency
+  std::vector<float> f32;
+  auto put32 = [&](float **dst, const std::vector<double> &src) -> int {
+    f32.assign(src.begin(), src.end());
+    if (dev_alloc(dst, (int64_t)f32.size())) return -1;
+    BCFD_HIP(hipMemcpy(*dst, f32.data(), sizeof(float) * f32.size(), hipMemcpyHostToDevice));
+    return 0;
+  };
+  for (int t = 0; t < BTB_N; ++t)
+    if (put32(&b->ca32[t], b->hca[t]) || put32(&b->ck32[t], b->hck[t])) return -1;
+  for (int a = 0; a < 3; ++a)
+    if (put32(&b->h32[a], *dd[a])) return -1;
   BCFD_HIP(hipStreamSynchronize(g_rt.stream));
   return 0;
 }

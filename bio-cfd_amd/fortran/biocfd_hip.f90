@@ -823,6 +823,10 @@ module biocfd_hip
       import :: c_int
       integer(c_int), value :: variant, pow2_products
     end function
+    integer(c_int) function biocfd_momentum_precision(fp32) bind(C, name="biocfd_momentum_precision")
+      import :: c_int
+      integer(c_int), value :: fp32
+    end function
 
     integer(c_int) function biocfd_ts_pass_config(parallel) bind(C, name="biocfd_ts_pass_config")
       import :: c_int

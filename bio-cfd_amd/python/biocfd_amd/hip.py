@@ -126,6 +126,7 @@ SIGNATURES = {
     "biocfd_output_fields": [_I, _P, _P, _P, _P, _P, _P, _P],
     "biocfd_sor_batching": [_I],
     "biocfd_momentum_config": [_I, _I],
+    "biocfd_momentum_precision": [_I],
     "biocfd_ts_pass_config": [_I],
     "biocfd_ad_reset": [_I],
     "biocfd_ad_upload": [_I, _I, _P],

@@ -137,6 +137,14 @@ int biocfd_ns_momentum(int blk, double dt, double re);
  * are products by their exact reciprocals (the same correctly rounded
  * values).  All settings give the same bits. */
 int biocfd_momentum_config(int variant, int pow2_products);
+/* Precision of the dense momentum pass (no reference counterpart; the
+ * mixed-precision step of the precision recipe): 0 (default) = FP64, the
+ * reference's arithmetic; 1 = the tendency dt*(-convection + diffusion/re -
+ * grad p) in FP32 (FP32 copies of the planes, upwind tables and spacings,
+ * the marching kernel's 64 x 4 x 32 shape) added to the FP64 velocity in
+ * FP64.  The deferred cells of the thin-surface pass stay FP64.  Mode 1
+ * changes the results at FP32 rounding level. */
+int biocfd_momentum_precision(int fp32);
 /* The deferred cells of nsMomentum2order's thin-surface pass
  * (NavierStokes.f90:310-490, 860-907) (no reference counterpart):
  * parallel = 1 (the default) evaluates them all at once, each reading the

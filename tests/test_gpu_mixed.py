@@ -162,3 +162,94 @@ def test_mixed_batched_fine_blocks_bit_identical(tol):
             if x.shape != y.shape or x.dtype != y.dtype or x.tobytes() != y.tobytes():
                 bad.append(f"block {g + 1} {k} differs")
     assert not bad, "\n".join(bad[:40])
+
+
+# FP32 momentum tendency (biocfd_momentum_precision(1)) on top of the mixed
+# solve: the dense pass's tendency is evaluated in FP32, whose third-order
+# upwind sums cancel terms of order |u|^2, so each step adds an absolute
+# velocity error of order 1e-8 (dt = 2e-3, unit inflow) to every component.
+# The velocity contract (1e-6) is relative to the velocity scale (the largest
+# |u|, |v|, |w| of the step): on the channel case v and w are pure 1e-3 noise,
+# and the same absolute error is 6e-6 of their own scale.  p, pc and b are
+# relative to their own scale.
+VEL = ("u", "v", "w", "ut", "vt", "wt")
+FP32_MOM_TOL = {"vel": 1e-6, "p": 1e-4, "pc": 1e-4, "b": 1e-4}
+
+
+@pytest.mark.parametrize("pow2", [1, 0])
+@pytest.mark.parametrize("name", ["channel", "sphere", "nested", "wing"])
+def test_fp32_momentum_step_against_fp64_oracle(name, pow2):
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = small_cases()[name]
+    ref = run_oracle(case, (STEPS,))[STEPS]
+    sim = Simulation(case, precision="mixed")
+    try:
+        H.call("biocfd_mixed_config", 26, 32, 32, 0.0)
+        H.call("biocfd_momentum_config", 4, pow2)
+        H.call("biocfd_momentum_precision", 1)
+        for _ in range(STEPS):
+            sim.step()
+        got = sim.dump()
+    finally:
+        H.call("biocfd_momentum_precision", 0)
+        H.call("biocfd_momentum_config", 4, 1)
+        sim.close()
+        H.call("biocfd_mixed_config", *DEFAULT)
+    bad, report = [], []
+    for g, (r, o) in enumerate(zip(ref, got)):
+        vmax = max(float(np.max(np.abs(np.asarray(r[f])))) for f in ("u", "v", "w"))
+        for k, x in r.items():
+            if k in SKIP or k not in o or k.startswith(("ca_", "ck_")):
+                continue
+            x = np.asarray(x)
+            y = np.asarray(o[k])
+            if x.dtype.kind in "iu":
+                if not np.array_equal(x, y):
+                    bad.append(f"block {g + 1} {k}: {np.count_nonzero(x != y)} integer mismatches")
+            elif k in VEL:
+                e = float(np.max(np.abs(y - x))) / vmax
+                tol = FP32_MOM_TOL["vel"]
+                report.append(f"{k}={e:.2e}")
+                if not e <= tol:
+                    bad.append(f"block {g + 1} {k}: error {e:.3e} of the velocity scale > {tol:.0e}")
+            else:
+                e = rel_err(x, y)
+                tol = FP32_MOM_TOL.get(k, 1e-4)
+                report.append(f"{k}={e:.2e}")
+                if not e <= tol:
+                    bad.append(f"block {g + 1} {k}: relative error {e:.3e} > {tol:.0e}")
+    print(f"{name} pow2={pow2}: " + " ".join(report))
+    assert not bad, "\n".join(bad[:40])
+
+
+def test_fp32_momentum_tendency_on_the_256_workload():
+    """One momentum call on the reduced 256^3 sphere workload (64x32x32 grid,
+    one step in): the FP32 tendency against the FP64 one, relative to the
+    tendency's own scale, per component."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = CS.config_sphere_256(n=(64, 32, 32), markers=512, nsteps=2, pc_ita_max=40)
+    sim = Simulation(case)
+    try:
+        sim.step()
+        u = {f: sim.field(1, f) for f in ("u", "v", "w")}
+        out = {}
+        for fp32 in (0, 1):
+            H.call("biocfd_momentum_precision", fp32)
+            H.call("biocfd_ns_momentum", 1, case.dt, case.re)
+            H.call("biocfd_synchronize")
+            out[fp32] = {f: sim.field(1, f + "t") for f in ("u", "v", "w")}
+    finally:
+        H.call("biocfd_momentum_precision", 0)
+        sim.close()
+    for f in ("u", "v", "w"):
+        tend = out[0][f] - u[f]
+        scale = float(np.max(np.abs(tend)))
+        err = float(np.max(np.abs(out[1][f] - out[0][f])))
+        print(f"{f}: tendency scale {scale:.3e}, FP32 - FP64 {err:.3e} ({err / scale:.2e})")
+        assert scale > 0
+        assert err <= 1e-5 * scale
