@@ -118,6 +118,7 @@ def run_path(args, precision: str, keep_fields: bool, rule: str = "residual", mo
     sim = Simulation(case, precision=precision)
     H.call("biocfd_momentum_config", args.momentum_variant, 0 if args.momentum_divisions else 1)
     H.call("biocfd_momentum_precision", 1 if momentum == "fp32" else 0)
+    H.call("biocfd_sor_graphs", args.sor_graphs)
     if precision == "fp64":
         if rule == "residual":
             H.call("biocfd_sor_stop_rule", 1, args.check_every)
@@ -343,6 +344,7 @@ def adjoint_summary(args, fd: bool = True):
 
     case = CS.config_adjoint(n=args.adjoint_n, markers=args.adjoint_markers, nsteps=args.adjoint_steps,
                              pc_ita_max=args.pc_ita_max, epsi=0.0)
+    H.call("biocfd_sor_graphs", args.sor_graphs)
     run = AdjointRun(case, every=8, avg_steps=max(1, args.adjoint_steps // 2), tune=True)
     try:
         run.forward()                       # warm-up (kernel loads, buffers)
@@ -546,6 +548,8 @@ def main():
                     help="dense momentum kernel (include/biocfd_hip.h, biocfd_momentum_config): 0 = all three "
                          "components per thread per plane, 1 = one launch per component, 2 = as 0 with four waves "
                          "per SIMD, 3..8 = plane march of 64 x TJ column tiles (4 = TJ 4, 32 planes, the default)")
+    ap.add_argument("--sor-graphs", type=int, choices=[0, 1], default=1,
+                    help="1: FP64 SOR pass chunks replayed from HIP graphs (biocfd_sor_graphs); 0: pass by pass")
     ap.add_argument("--mixed-momentum", choices=["fp64", "both"], default="both",
                     help="both: a second mixed run with the FP32 momentum tendency, reported beside the first")
     ap.add_argument("--momentum-divisions", action="store_true",

@@ -182,6 +182,19 @@ struct AdWork {
   // forcing-call records (slot * 6 + routine) and the slot being recorded
   std::vector<ForceRec> recs;
   int rec_slot = -1;
+  // the transposed SOR's launches as one graph (fixed count: no host polls)
+  hipGraphExec_t sor_exec = nullptr;
+  long long sor_key[3] = {0, 0, 0};   // iterations, dt bits, cell tags
+};
+
+// a captured chunk of SOR passes of one block (k_pcor.hip, graph_chunk)
+struct SorGraph {
+  static constexpr int kKey = 10, kPtrs = 9;
+  long long key[kKey];
+  const void *ptrs[kPtrs];       // the block's buffers the launches name
+  long long d_iters, d_q;        // iterations and passes the chunk runs
+  long long end_it, end_q;       // its last end kernel, relative to the chunk's start
+  hipGraphExec_t exec = nullptr;
 };
 
 struct Block {
@@ -216,6 +229,7 @@ struct Block {
   // residual halves (colour-split)
   float *e32[2] = {nullptr, nullptr}, *r32[2] = {nullptr, nullptr};
   int64_t mx_n = 0;
+  std::vector<SorGraph> sor_graphs;   // captured SOR pass chunks
   double *partials = nullptr;         // per-workgroup maxima of the reductions
   int64_t cap_partials = 0;
   MoveState mv;
@@ -331,6 +345,8 @@ int norm_copy(Block &b, double dt);
 int projection(Block &b, int64_t n_iter, double dt);
 int stress_sums(Block &b, double mu_f, double rho_f, const double kin[BK_N], double out[4]);
 void ad_release(Block &b);
+void sor_graphs_release(Block &b);
+int sor_graphs_on();
 int mixed_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt);
 bool mixed_batchable();
 

@@ -31,6 +31,8 @@
 // transpose adds w_s * l_target to each source (atomically: sources are
 // shared between targets) and zeroes l_target, the prolongation's axis passes
 // in reverse order (a cell two passes write keeps the last pass's value).
+#include <cstring>
+
 #include "biocfd_rt.hpp"
 
 namespace bcfd {
@@ -149,7 +151,7 @@ __global__ void k_adj_sor_colour(Dims d, const uint8_t *__restrict__ cell, const
 }
 
 // lpc <- (B_1^T R_1^T ... B_N^T R_N^T applied backwards), bbar accumulated
-void adj_sor(const Block &b, int64_t n_iter, double dt, double *wd, double *lpc, double *bbar) {
+void adj_sor_launches(const Block &b, int64_t n_iter, double dt, double *wd, double *lpc, double *bbar) {
   hipStream_t s = rt().stream;
   const dim3 blk(256), grd(div_up(b.n2, 256));
   const uint8_t *cell = b.tag[BT_CELL];
@@ -158,6 +160,43 @@ void adj_sor(const Block &b, int64_t n_iter, double dt, double *wd, double *lpc,
   for (int64_t q = 0; q <= 2 * n_iter; ++q)
     hipLaunchKernelGGL(k_adj_sor_colour, grc, blk, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], wd, dt, b.omega,
                        q % 2 == 0 ? 1 : 0, q == 0 ? 1 : 0, q == 2 * n_iter ? 1 : 0, lpc, bbar);
+}
+
+// the same launches; on the block's own adjoint buffers (w != nullptr) and
+// with SOR graphs on, as one graph captured on first use and replayed while
+// the iteration count, dt and the tag array stay the same
+int adj_sor(Block &b, int64_t n_iter, double dt, double *wd, double *lpc, double *bbar, AdWork *w) {
+  if (!w || !sor_graphs_on()) {
+    adj_sor_launches(b, n_iter, dt, wd, lpc, bbar);
+    BCFD_CHECK_LAUNCH();
+    return 0;
+  }
+  long long dt_bits;
+  memcpy(&dt_bits, &dt, sizeof dt_bits);
+  const long long key[3] = {(long long)n_iter, dt_bits, (long long)(intptr_t)b.tag[BT_CELL]};
+  if (w->sor_exec && memcmp(key, w->sor_key, sizeof key)) {
+    (void)hipGraphExecDestroy(w->sor_exec);
+    w->sor_exec = nullptr;
+  }
+  hipStream_t s = rt().stream;
+  if (!w->sor_exec) {
+    BCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    adj_sor_launches(b, n_iter, dt, wd, lpc, bbar);
+    const hipError_t el = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (el != hipSuccess || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      set_error("adjoint SOR graph capture: %s", hipGetErrorString(el != hipSuccess ? el : ec));
+      return -1;
+    }
+    const hipError_t ei = hipGraphInstantiate(&w->sor_exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BCFD_HIP(ei);
+    memcpy(w->sor_key, key, sizeof key);
+  }
+  BCFD_HIP(hipGraphLaunch(w->sor_exec, s));
+  return 0;
 }
 
 // utbar = ubar + D^T bbar; computeDiv: b(i) = (ut(i) - ut(i-1)) / dx(i) + ...
@@ -367,7 +406,7 @@ int biocfd_projection_adjoint(int id, int64_t n_iter, double dt, const double *u
   const uint8_t *cell = b->tag[BT_CELL];
   hipLaunchKernelGGL(k_adj_grad, grd, blk, 0, s, d, cell, ub, vb, wb, pb, b->g[BG_DX], b->g[BG_DY], b->g[BG_DZ], dt,
                      lpc, bbar);
-  adj_sor(*b, n_iter, dt, out, lpc, bbar);
+  if (adj_sor(*b, n_iter, dt, out, lpc, bbar, nullptr)) return -1;
   // utbar into out, vtbar into lpc, wtbar into pb (neither is read any more)
   hipLaunchKernelGGL(k_adj_div, grd, blk, 0, s, d, cell, bbar, ub, vb, wb, b->g[BG_DX], b->g[BG_DY], b->g[BG_DZ],
                      out, lpc, pb);
@@ -729,7 +768,7 @@ int poisson_adj(Block &b, int64_t n_iter, double dt, double uc) {
   double *lpc = b.ad.scr[0], *bbar = b.ad.scr[1];
   hipLaunchKernelGGL(k_adj_grad, grd, blk, 0, s, d, cell, l[BF_UT], l[BF_VT], l[BF_WT], l[BF_P], b.g[BG_DX],
                      b.g[BG_DY], b.g[BG_DZ], dt, lpc, bbar);
-  adj_sor(b, n_iter, dt, b.ad.scr[2], lpc, bbar);
+  if (adj_sor(b, n_iter, dt, b.ad.scr[2], lpc, bbar, &b.ad)) return -1;
   // each thread reads and writes only its own cell of the velocity adjoints
   hipLaunchKernelGGL(k_adj_div, grd, blk, 0, s, d, cell, bbar, l[BF_UT], l[BF_VT], l[BF_WT], b.g[BG_DX], b.g[BG_DY],
                      b.g[BG_DZ], l[BF_UT], l[BF_VT], l[BF_WT]);
@@ -842,6 +881,7 @@ int ad_ready(Block &b) {
 
 void ad_release(Block &b) {
   AdWork &w = b.ad;
+  if (w.sor_exec) (void)hipGraphExecDestroy(w.sor_exec);
   for (auto &p : w.s) if (p) { (void)hipFree(p); p = nullptr; }
   for (auto &p : w.scr) if (p) { (void)hipFree(p); p = nullptr; }
   for (auto &p : w.gh) if (p) { (void)hipFree(p); p = nullptr; }

@@ -14,6 +14,8 @@
 // neighbours and black cells read the new red values, so updating pc in place
 // colour by colour is the same computation; |pc - pco| of an iteration is
 // the per-cell change, reduced with one atomic per wavefront.
+#include <cstring>
+
 #include "biocfd_rt.hpp"
 #include "reduce.hpp"
 #include "sor_kernel.hpp"
@@ -183,6 +185,50 @@ __global__ void __launch_bounds__(1024) k_sor_fused_end(SorState *st, long long 
   st->n_iter = it;
   st->call_iters = it - start;
   st->passes = (int)passes;
+  st->derr2 = st->derr4;
+  if (st->derr4 < epsi || it > pc_ita_max) {
+    st->done = 1;
+    st->exhausted += !(st->derr4 < epsi);
+  }
+}
+
+// k_sor_fused_end with the iteration and pass counts relative to the
+// previous end kernel of the call (dit iterations and dpass passes later),
+// which set n_iter, call_iters and passes: the same state updates, with
+// arguments that do not depend on where the call started, so that a
+// captured chunk of passes can be replayed
+__global__ void __launch_bounds__(1024) k_sor_fused_end_rel(SorState *st, long long dit, long long dpass, int check,
+                                                            long long pc_ita_max, double epsi,
+                                                            const double *partials, int64_t n_partials,
+                                                            int first) {
+  if (st->done) return;
+  __shared__ double s_m[16];
+  if (check) {
+    double m = 0.0;
+    for (int64_t q = threadIdx.x; q < n_partials; q += blockDim.x) m = m > partials[q] ? m : partials[q];
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (check) {
+    double m = 0.0;
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
+    st->derr4 = m;
+  }
+  const long long it = st->n_iter + dit;
+  if (first && check && st->derr4 < epsi) {
+    st->n_iter = it - 1;
+    st->call_iters += dit - 1;
+    st->passes += (int)dpass - 1;
+    st->derr2 = st->derr4;
+    st->done = 1;
+    st->rollback = 1;
+    return;
+  }
+  st->n_iter = it;
+  st->call_iters += dit;
+  st->passes += (int)dpass;
   st->derr2 = st->derr4;
   if (st->derr4 < epsi || it > pc_ita_max) {
     st->done = 1;
@@ -665,9 +711,12 @@ static int rollback_pass(Block &b, SorState *st, int variant, int planes) {
   return 0;
 }
 
+// (end_it, end_q): the iteration and pass count of the call's last end
+// kernel; with `rel` the end kernels take their counts relative to it
+// (k_sor_fused_end_rel), as a captured chunk needs.
 static int enqueue_fused_passes(Block &b, SorState *st, int variant, int planes, long long start, long long total,
                                 long long chunk, long long &done_iters, long long &q, int64_t pc_ita_max,
-                                double epsi) {
+                                double epsi, bool rel, long long &end_it, long long &end_q) {
   hipStream_t s = rt().stream;
   const FusedVariant &v = kFused[variant];
   int gx, gy, gz;
@@ -715,11 +764,23 @@ static int enqueue_fused_passes(Block &b, SorState *st, int variant, int planes,
     ++q;
     if (g_stop.rule == 1 && (check || done_iters == total)) {
       const int64_t nres = residual_check_launch(b, a.dst, st);
-      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, 1,
-                         (long long)pc_ita_max, epsi, b.partials, nres, 0);
+      if (rel)
+        hipLaunchKernelGGL(k_sor_fused_end_rel, dim3(1), dim3(1024), 0, s, st, last_it - end_it, q - end_q, 1,
+                           (long long)pc_ita_max, epsi, b.partials, nres, 0);
+      else
+        hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, 1,
+                           (long long)pc_ita_max, epsi, b.partials, nres, 0);
+      end_it = last_it;
+      end_q = q;
     } else if (check || done_iters == total) {
-      hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, check != 0,
-                         (long long)pc_ita_max, epsi, b.partials, nwg, check == 2 ? 1 : 0);
+      if (rel)
+        hipLaunchKernelGGL(k_sor_fused_end_rel, dim3(1), dim3(1024), 0, s, st, last_it - end_it, q - end_q,
+                           check != 0, (long long)pc_ita_max, epsi, b.partials, nwg, check == 2 ? 1 : 0);
+      else
+        hipLaunchKernelGGL(k_sor_fused_end, dim3(1), dim3(1024), 0, s, st, last_it, start, q, check != 0,
+                           (long long)pc_ita_max, epsi, b.partials, nwg, check == 2 ? 1 : 0);
+      end_it = last_it;
+      end_q = q;
     }
   }
   BCFD_CHECK_LAUNCH();
@@ -777,9 +838,10 @@ static int sor_finish(Block &b, const SorState *st, int variant) {
 // queue about `chunk` more iterations of the call (see enqueue_fused_passes)
 static int enqueue_iterations(Block &b, SorState *st, int variant, int planes, long long start, long long total,
                               long long chunk, long long &done_iters, long long &q, int64_t pc_ita_max,
-                              double epsi) {
+                              double epsi, long long &end_it, long long &end_q) {
   if (variant > 0)
-    return enqueue_fused_passes(b, st, variant, planes, start, total, chunk, done_iters, q, pc_ita_max, epsi);
+    return enqueue_fused_passes(b, st, variant, planes, start, total, chunk, done_iters, q, pc_ita_max, epsi, false,
+                                end_it, end_q);
   const long long n = chunk < total - done_iters ? chunk : total - done_iters;
   done_iters += n;
   return enqueue_split_iterations(b, st, n, pc_ita_max, epsi);
@@ -914,6 +976,97 @@ int sor_poll(const SorState *st, long long &k, bool &stop) {
   return 0;
 }
 
+// HIP graphs of SOR passes.  Between two host polls a call's passes follow
+// a pattern fixed by the iteration number modulo the check period (where
+// the check iterations fall, which passes are cut short), the pass parity
+// and the iterations and passes since the last end kernel; with end
+// kernels that take their counts relative to the previous one, the
+// launches of such a chunk do not depend on where it sits in the call.
+// The first chunk of each pattern is captured (the same launches, in the
+// same order, as the eager path would issue) and instantiated, and later
+// chunks of that pattern replay it: one launch instead of some 20 to 260.
+// A chunk never contains the call's last iteration (whose end kernel the
+// eager path issues), and a graph is kept only while the block's buffers
+// it names stay where they were.
+static int g_sor_graphs = 1;
+static long long g_graph_captures = 0, g_graph_replays = 0;
+
+// iterations per graph chunk: whole check periods, at least 40 iterations
+static long long graph_span() {
+  const long long C = check_period();
+  const long long P = g_stop.rule == 0 ? 10 : C;   // two-iteration passes against every fifth iteration
+  return P * ((40 + P - 1) / P);
+}
+
+static bool same_ptrs(const SorGraph &g, const Block &b) {
+  const void *now[SorGraph::kPtrs] = {b.f[BF_PC], b.pc2, b.partials, b.bs[0], b.bs[1], b.A[0], b.A[1], b.A[2], b.sor};
+  for (int q = 0; q < SorGraph::kPtrs; ++q)
+    if (now[q] != g.ptrs[q]) return false;
+  return true;
+}
+
+int sor_graphs_on() { return g_sor_graphs; }
+
+void sor_graphs_release(Block &b) {
+  for (SorGraph &g : b.sor_graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  b.sor_graphs.clear();
+}
+
+// one chunk of a call through its graph (captured on first use): advances
+// done_iters, q and the end-kernel counts as the eager path would
+static int graph_chunk(Block &b, int variant, int planes, long long start, long long total, long long &done_iters,
+                       long long &q, int64_t pc_ita_max, double epsi, long long &end_it, long long &end_q) {
+  hipStream_t s = rt().stream;
+  const long long G = graph_span(), P = g_stop.rule == 0 ? 10 : check_period();
+  const long long it0 = start + done_iters;   // iterations done before the chunk
+  long long epsi_bits;
+  memcpy(&epsi_bits, &epsi, sizeof epsi_bits);
+  const long long key[SorGraph::kKey] = {variant, planes, g_stop.rule, check_period(), it0 % P, q & 1,
+                                         it0 - end_it, q - end_q, (long long)pc_ita_max, epsi_bits};
+  SorGraph *hit = nullptr;
+  for (SorGraph &g : b.sor_graphs)
+    if (!memcmp(g.key, key, sizeof key)) { hit = &g; break; }
+  if (hit && !same_ptrs(*hit, b)) {
+    sor_graphs_release(b);
+    hit = nullptr;
+  }
+  if (!hit) {
+    long long d2 = done_iters, q2 = q, e_it = end_it, e_q = end_q;
+    BCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_fused_passes(b, b.sor, variant, planes, start, total, G, d2, q2, pc_ita_max, epsi, true,
+                                        e_it, e_q);
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return -1;
+    }
+    BCFD_HIP(ec);
+    SorGraph g;
+    memcpy(g.key, key, sizeof key);
+    const void *now[SorGraph::kPtrs] = {b.f[BF_PC], b.pc2, b.partials, b.bs[0], b.bs[1], b.A[0], b.A[1], b.A[2], b.sor};
+    memcpy(g.ptrs, now, sizeof now);
+    g.d_iters = d2 - done_iters;
+    g.d_q = q2 - q;
+    g.end_it = e_it - it0;
+    g.end_q = e_q - q;
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    BCFD_HIP(ei);
+    b.sor_graphs.push_back(g);
+    hit = &b.sor_graphs.back();
+    ++g_graph_captures;
+  }
+  BCFD_HIP(hipGraphLaunch(hit->exec, s));
+  ++g_graph_replays;
+  end_it = it0 + hit->end_it;
+  end_q = q + hit->end_q;
+  done_iters += hit->d_iters;
+  q += hit->d_q;
+  return 0;
+}
+
 int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   if (precision_mode() == 1) return mixed_sor(b, pc_ita_max, dt);
   hipStream_t s = rt().stream;
@@ -935,10 +1088,18 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   if (remaining < 1) remaining = 1;
   const long long total = remaining;
   const long long chunk = 50;
-  long long done_iters = 0, q = 0, polls = 0;
+  long long done_iters = 0, q = 0, polls = 0, end_it = start, end_q = 0;
   while (done_iters < total) {
-    if (enqueue_iterations(b, b.sor, variant, planes, start, total, chunk, done_iters, q, pc_ita_max, epsi))
+    // graph chunks once the call has issued an end kernel (the chunk's first
+    // relative counts need one), ending before the call's last iteration
+    const bool graph = g_sor_graphs && variant > 0 && done_iters > 0 && end_q > 0 &&
+                       done_iters + graph_span() + 2 < total;
+    if (graph) {
+      if (graph_chunk(b, variant, planes, start, total, done_iters, q, pc_ita_max, epsi, end_it, end_q)) return -1;
+    } else if (enqueue_iterations(b, b.sor, variant, planes, start, total, chunk, done_iters, q, pc_ita_max, epsi,
+                                  end_it, end_q)) {
       return -1;
+    }
     if (done_iters >= total) break;
     bool stop = false;
     if (sor_poll(b.sor, polls, stop)) return -1;
@@ -1128,10 +1289,10 @@ int sor_time(Block &b, int variant, int planes, int iters, double dt, double *ms
          hipEventRecord(e0, s) != hipSuccess;
   }
   // pcItaMax out of reach and a negative tolerance: exactly `iters` iterations
-  long long done_iters = 0, q = 0;
+  long long done_iters = 0, q = 0, end_it = 0, end_q = 0;
   if (!rc)
     rc = enqueue_iterations(b, g_probe_state, variant, planes, 0, iters, iters, done_iters, q, (int64_t)1 << 62,
-                            -1.0);
+                            -1.0, end_it, end_q);
   float t = 0.f;
   if (!rc) rc = hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                 hipEventElapsedTime(&t, e0, e1) != hipSuccess;
@@ -1274,6 +1435,19 @@ int biocfd_timing_read(double *sor_ms, int64_t *sor_iters, int64_t *sor_cell_ite
 int biocfd_sor_batching(int on) {
   BCFD_API_LOCK();
   g_sor_batch = on ? 1 : 0;
+  return 0;
+}
+
+int biocfd_sor_graphs(int on) {
+  BCFD_API_LOCK();
+  g_sor_graphs = on ? 1 : 0;
+  return 0;
+}
+
+int biocfd_sor_graph_stats(int64_t *captures, int64_t *launches) {
+  BCFD_API_LOCK();
+  *captures = g_graph_captures;
+  *launches = g_graph_replays;
   return 0;
 }
 

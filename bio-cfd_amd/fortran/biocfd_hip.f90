@@ -818,6 +818,14 @@ module biocfd_hip
       import :: c_int
       integer(c_int), value :: on
     end function
+    integer(c_int) function biocfd_sor_graphs(on) bind(C, name="biocfd_sor_graphs")
+      import :: c_int
+      integer(c_int), value :: on
+    end function
+    integer(c_int) function biocfd_sor_graph_stats(captures, launches) bind(C, name="biocfd_sor_graph_stats")
+      import :: c_int, c_int64_t
+      integer(c_int64_t) :: captures, launches
+    end function
     integer(c_int) function biocfd_momentum_config(variant, pow2_products) &
         bind(C, name="biocfd_momentum_config")
       import :: c_int

@@ -194,6 +194,16 @@ int biocfd_sor_stop_rule(int rule, int cadence);
  * table of the blocks (pair-owned kernels, FP64); off, one block after the
  * other.  Both give the same bits. */
 int biocfd_sor_batching(int on);
+/* HIP graphs of the FP64 SOR passes of a single-block call (no reference
+ * counterpart): on (the default), each run of passes between two host
+ * polls whose pattern (check iterations, pass parity) was seen before in the
+ * call's block replays a graph captured the first time, one graph launch
+ * instead of the passes' and end kernels' launches; off, every pass is
+ * launched.  Both give the same bits. */
+int biocfd_sor_graphs(int on);
+/* Graphs captured and graph launches of biocfd_sor_graphs since the
+ * library was loaded (no reference counterpart; a test and bench probe). */
+int biocfd_sor_graph_stats(int64_t *captures, int64_t *launches);
 /* info = {tile_i, tile_j, threads per workgroup, z-plane prefetch depth,
  * workgroups per CU its register budget is compiled for, iterations per
  * pass, tile family (0: per-phase cell lists, 1: pair-owned cells)} of a

@@ -313,3 +313,42 @@ def test_residual_rule_stops_at_first_test_below_tolerance(cadence):
     before = CS.config_sphere_64(n=24, markers=256, nsteps=1, pc_ita_max=n - cadence - 1, epsi=0.0)
     r_before = residual_max(_oracle(before, (1,))[1][0], case.dt)
     assert r_before >= tol, r_before
+
+
+@pytest.mark.parametrize("rule", [(0, 5), (1, 10)])
+def test_sor_graphs_give_the_same_bits(rule):
+    """The SOR calls with their pass chunks replayed from HIP graphs
+    (biocfd_sor_graphs(1), the default) and launched pass by pass give the
+    same fields, iteration counts and norms over four steps of the 32^3
+    sphere, under the reference's rule (tolerance 1e-8, pcItaMax 500: calls
+    of a few hundred iterations, some ending inside a graph chunk) and under
+    the residual rule (tests every 10th iteration)."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+
+    case = CS.config_sphere_64(n=32, markers=512, nsteps=4, pc_ita_max=500, epsi=1e-8 if rule[0] == 0 else 1e-6)
+    runs = {}
+    stats = {}
+    try:
+        for graphs in (0, 1):
+            def setup(g=graphs):
+                H.call("biocfd_sor_graphs", g)
+                H.call("biocfd_sor_stop_rule", *rule)
+            c0, l0 = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
+            H.call("biocfd_sor_graph_stats", H.ptr(c0), H.ptr(l0))
+            runs[graphs] = _run_gpu(case, (1, 4), setup=setup)
+            c1, l1 = np.zeros(1, dtype=np.int64), np.zeros(1, dtype=np.int64)
+            H.call("biocfd_sor_graph_stats", H.ptr(c1), H.ptr(l1))
+            stats[graphs] = (int(c1[0] - c0[0]), int(l1[0] - l0[0]))
+    finally:
+        H.call("biocfd_sor_graphs", 1)
+        H.call("biocfd_sor_stop_rule", 0, 5)
+    print("graphs captured / launched:", stats)
+    assert stats[0] == (0, 0)
+    # a few patterns captured, many more chunks replayed
+    assert 0 < stats[1][0] < stats[1][1], stats
+    for s in (1, 4):
+        a, b = runs[0][s][0], runs[1][s][0]
+        assert int(a["nIterPcor"]) > 50, a["nIterPcor"]
+        for k, x in a.items():
+            assert np.array_equal(np.asarray(x), np.asarray(b[k])), (s, k)
