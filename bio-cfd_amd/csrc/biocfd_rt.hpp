@@ -346,6 +346,7 @@ int projection(Block &b, int64_t n_iter, double dt);
 int stress_sums(Block &b, double mu_f, double rho_f, const double kin[BK_N], double out[4]);
 void ad_release(Block &b);
 void sor_graphs_release(Block &b);
+void batch_graphs_release();
 int sor_graphs_on();
 int mixed_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double dt);
 bool mixed_batchable();

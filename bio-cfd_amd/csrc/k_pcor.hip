@@ -301,6 +301,52 @@ __global__ void __launch_bounds__(1024) k_sor_batch_end(const BatchBlockT<double
   }
 }
 
+// k_sor_batch_end with counts relative to the previous end kernel of the
+// set's calls (dit iterations and dpass passes later), for captured chunks
+__global__ void __launch_bounds__(1024) k_sor_batch_end_rel(const BatchBlockT<double> *__restrict__ tab,
+                                                            long long dit, long long dpass, int check,
+                                                            long long pc_ita_max, double epsi,
+                                                            const double *__restrict__ partials, int first) {
+  const BatchBlockT<double> &e = tab[blockIdx.x];
+  SorState *st = e.st;
+  if (st->done) return;
+  __shared__ double s_m[16];
+  if (check) {
+    double m = 0.0;
+    for (int64_t q = threadIdx.x; q < e.ntiles; q += blockDim.x) {
+      const double v = partials[e.first + q];
+      m = m > v ? m : v;
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = m;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (check) {
+    double m = 0.0;
+    for (int q = 0; q < (int)(blockDim.x + 63) / 64; ++q) m = m > s_m[q] ? m : s_m[q];
+    st->derr4 = m;
+  }
+  const long long it = st->n_iter + dit;
+  if (first && check && st->derr4 < epsi) {
+    st->n_iter = it - 1;
+    st->call_iters += dit - 1;
+    st->passes += (int)dpass - 1;
+    st->derr2 = st->derr4;
+    st->done = 1;
+    st->rollback = 1;
+    return;
+  }
+  st->n_iter = it;
+  st->call_iters += dit;
+  st->passes += (int)dpass;
+  st->derr2 = st->derr4;
+  if (st->derr4 < epsi || it > pc_ita_max) {
+    st->done = 1;
+    st->exhausted += !(st->derr4 < epsi);
+  }
+}
+
 // the passes of one SOR call alternate pc -> pc2 -> pc ...; after an odd
 // count the result is in pc2
 __global__ void k_sor_finish(const SorState *__restrict__ st, double *__restrict__ pc,
@@ -1116,6 +1162,21 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
 // pair-owned kernels; blocks whose calls start at the same iteration share a
 // pass schedule.  Off: one block after the other.
 static int g_sor_batch = 1;
+// captured chunks of batched passes (block set in `blocks`)
+struct BatchGraph {
+  static constexpr int kKey = 12;
+  long long key[kKey];
+  std::vector<Block *> blocks;
+  long long d_iters, d_q, end_done, end_q;
+  hipGraphExec_t exec = nullptr;
+};
+static std::vector<BatchGraph> g_batch_graphs;
+
+void batch_graphs_release() {
+  for (BatchGraph &g : g_batch_graphs)
+    if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  g_batch_graphs.clear();
+}
 static BatchBlock *g_batch_tab = nullptr;
 static int64_t g_batch_cap = 0;
 static double *g_batch_partials = nullptr;
@@ -1173,39 +1234,102 @@ static int batched_calls(const std::vector<Block *> &blocks, const std::vector<l
   hipEvent_t e0 = nullptr;
   if (sor_timing_begin(&e0)) return -1;
   const long long s0 = starts[0];
-  long long done_iters = 0, q = 0, since_poll = 0, polls = 0;
+  long long done_iters = 0, q = 0, polls = 0, end_done = 0, end_q = 0;
   const bool first_checks = two_iteration_schedule(v);
-  while (done_iters < total) {
-    const long long it = s0 + done_iters + 1;
-    long long L = v.nit;
-    if (L > total - done_iters) L = total - done_iters;
-    for (long long bt : budget)
-      if (bt > done_iters && L > bt - done_iters) L = bt - done_iters;
-    int check;
-    if (first_checks) {
-      check = (it + L - 1) % 5 == 0 ? 1 : (L == 2 && it % 5 == 0 ? 2 : 0);
-    } else {
-      const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
-      if (L > to_check) L = to_check;
-      check = (it + L - 1) % 5 == 0;
+  long long min_budget = total;
+  for (long long bt : budget) min_budget = bt < min_budget ? bt : min_budget;
+  // passes until `upto` iterations of the set are queued; `rel`: end kernels
+  // relative to the previous one (a captured chunk)
+  auto passes = [&](long long upto, bool rel) -> int {
+    while (done_iters < upto) {
+      const long long it = s0 + done_iters + 1;
+      long long L = v.nit;
+      if (L > total - done_iters) L = total - done_iters;
+      for (long long bt : budget)
+        if (bt > done_iters && L > bt - done_iters) L = bt - done_iters;
+      int check;
+      if (first_checks) {
+        check = (it + L - 1) % 5 == 0 ? 1 : (L == 2 && it % 5 == 0 ? 2 : 0);
+      } else {
+        const long long to_check = (5 - it % 5) % 5 + 1;  // up to and including the next check iteration
+        if (L > to_check) L = to_check;
+        check = (it + L - 1) % 5 == 0;
+      }
+      if (check == 2)
+        v.batch_first(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1), 1,
+                      g_batch_partials);
+      else
+        (L == v.nit ? v.batch : v.batch1)(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb,
+                                          (int)(q & 1), check, g_batch_partials);
+      done_iters += L;
+      ++q;
+      bool ends = done_iters == total;
+      for (long long bt : budget) ends = ends || bt == done_iters;
+      if (check || ends) {
+        if (rel)
+          hipLaunchKernelGGL(k_sor_batch_end_rel, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab,
+                             done_iters - end_done, q - end_q, check != 0, (long long)pc_ita_max, epsi,
+                             g_batch_partials, check == 2 ? 1 : 0);
+        else
+          hipLaunchKernelGGL(k_sor_batch_end, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab, done_iters, q,
+                             check != 0, (long long)pc_ita_max, epsi, g_batch_partials, check == 2 ? 1 : 0);
+        end_done = done_iters;
+        end_q = q;
+      }
+      BCFD_CHECK_LAUNCH();
     }
-    if (check == 2)
-      v.batch_first(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1), 1, g_batch_partials);
-    else
-      (L == v.nit ? v.batch : v.batch1)(dim3((unsigned)total_tiles), dim3(v.nt), s, g_batch_tab, nb, (int)(q & 1),
-                                        check, g_batch_partials);
-    done_iters += L;
-    since_poll += L;
-    ++q;
-    bool ends = done_iters == total;
-    for (long long bt : budget) ends = ends || bt == done_iters;
-    if (check || ends)
-      hipLaunchKernelGGL(k_sor_batch_end, dim3((unsigned)nb), dim3(1024), 0, s, g_batch_tab, done_iters, q,
-                         check != 0, (long long)pc_ita_max, epsi, g_batch_partials, check == 2 ? 1 : 0);
-    BCFD_CHECK_LAUNCH();
-    if (since_poll >= 50 && done_iters < total) {
+    return 0;
+  };
+  while (done_iters < total) {
+    // graph chunks (as graph_chunk) once an end kernel has run, ending before
+    // the earliest budget end of the set
+    const long long G = graph_span();
+    if (g_sor_graphs && end_q > 0 && done_iters + G + 2 < min_budget) {
+      long long epsi_bits;
+      memcpy(&epsi_bits, &epsi, sizeof epsi_bits);
+      const long long key[BatchGraph::kKey] = {variant, planes, nb, total_tiles, (s0 + done_iters) % 10, q & 1,
+                                               done_iters - end_done, q - end_q, (long long)pc_ita_max, epsi_bits,
+                                               (long long)(intptr_t)g_batch_tab,
+                                               (long long)(intptr_t)g_batch_partials};
+      BatchGraph *hit = nullptr;
+      for (BatchGraph &g : g_batch_graphs)
+        if (!memcmp(g.key, key, sizeof key) && g.blocks == blocks) { hit = &g; break; }
+      const long long d0 = done_iters, q0 = q;
+      if (!hit) {
+        BCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = passes(done_iters + G, true);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (rc) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return -1;
+        }
+        BCFD_HIP(ec);
+        BatchGraph g;
+        memcpy(g.key, key, sizeof key);
+        g.blocks = blocks;
+        g.d_iters = done_iters - d0;
+        g.d_q = q - q0;
+        g.end_done = end_done - d0;
+        g.end_q = end_q - q0;
+        const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        BCFD_HIP(ei);
+        g_batch_graphs.push_back(g);
+        hit = &g_batch_graphs.back();
+        ++g_graph_captures;
+      }
+      BCFD_HIP(hipGraphLaunch(hit->exec, s));
+      ++g_graph_replays;
+      done_iters = d0 + hit->d_iters;
+      q = q0 + hit->d_q;
+      end_done = d0 + hit->end_done;
+      end_q = q0 + hit->end_q;
+    } else if (passes(done_iters + 50 < total ? done_iters + 50 : total, false)) {
+      return -1;
+    }
+    if (done_iters < total) {
       // every block stopped: no more passes (PcorVcor.F90:365-375)
-      since_poll = 0;
       bool stop = false;
       if (set_poll(blocks, polls, stop)) return -1;
       if (stop) break;

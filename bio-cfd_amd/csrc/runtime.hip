@@ -223,6 +223,7 @@ int biocfd_synchronize(void) {
 static void free_block(Block &b) {
   ad_release(b);
   sor_graphs_release(b);
+  batch_graphs_release();
   for (auto &p : b.g) if (p) { (void)hipFree(p); p = nullptr; }
   for (auto &p : b.f) if (p) { (void)hipFree(p); p = nullptr; }
   for (auto &p : b.tag) if (p) { (void)hipFree(p); p = nullptr; }

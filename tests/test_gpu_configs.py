@@ -315,18 +315,25 @@ def test_residual_rule_stops_at_first_test_below_tolerance(cadence):
     assert r_before >= tol, r_before
 
 
-@pytest.mark.parametrize("rule", [(0, 5), (1, 10)])
+@pytest.mark.parametrize("rule", [(0, 5), (1, 10), "batched"])
 def test_sor_graphs_give_the_same_bits(rule):
     """The SOR calls with their pass chunks replayed from HIP graphs
     (biocfd_sor_graphs(1), the default) and launched pass by pass give the
     same fields, iteration counts and norms over four steps of the 32^3
     sphere, under the reference's rule (tolerance 1e-8, pcItaMax 500: calls
     of a few hundred iterations, some ending inside a graph chunk) and under
-    the residual rule (tests every 10th iteration)."""
+    the residual rule (tests every 10th iteration); and on the five uneven
+    blocks with the fine blocks' calls batched (tolerance 1e-6, pcItaMax
+    300, blocks stopping at different iterations)."""
     from biocfd_amd import cases as CS
     from biocfd_amd import hip as H
 
-    case = CS.config_sphere_64(n=32, markers=512, nsteps=4, pc_ita_max=500, epsi=1e-8 if rule[0] == 0 else 1e-6)
+    if rule == "batched":
+        case = CS.config_batched(nsteps=4, pc_ita_max=300, epsi=1e-6)
+        rule = (0, 5)
+    else:
+        case = CS.config_sphere_64(n=32, markers=512, nsteps=4, pc_ita_max=500,
+                                   epsi=1e-8 if rule[0] == 0 else 1e-6)
     runs = {}
     stats = {}
     try:
@@ -348,7 +355,7 @@ def test_sor_graphs_give_the_same_bits(rule):
     # a few patterns captured, many more chunks replayed
     assert 0 < stats[1][0] < stats[1][1], stats
     for s in (1, 4):
-        a, b = runs[0][s][0], runs[1][s][0]
-        assert int(a["nIterPcor"]) > 50, a["nIterPcor"]
-        for k, x in a.items():
-            assert np.array_equal(np.asarray(x), np.asarray(b[k])), (s, k)
+        assert int(runs[0][s][0]["nIterPcor"]) > 50, runs[0][s][0]["nIterPcor"]
+        for g, (a, b) in enumerate(zip(runs[0][s], runs[1][s])):
+            for k, x in a.items():
+                assert np.array_equal(np.asarray(x), np.asarray(b[k])), (s, g + 1, k)
