@@ -231,6 +231,7 @@ def run_uneven(args, batching: int, variant, precision: str = "fp64"):
     sim = Simulation(case, precision=precision)
     H.call("biocfd_sor_batching", batching)
     H.call("biocfd_momentum_config", args.momentum_variant, 0 if args.momentum_divisions else 1)
+    H.call("biocfd_sor_graphs", args.sor_graphs)
     if variant is not None:
         H.call("biocfd_sor_config", variant, args.sor_planes)
     if precision == "mixed":
