@@ -232,6 +232,7 @@ def run_uneven(args, batching: int, variant, precision: str = "fp64"):
     H.call("biocfd_sor_batching", batching)
     H.call("biocfd_momentum_config", args.momentum_variant, 0 if args.momentum_divisions else 1)
     H.call("biocfd_sor_graphs", args.sor_graphs)
+    H.call("biocfd_interp_batching", args.interp_batching)
     if variant is not None:
         H.call("biocfd_sor_config", variant, args.sor_planes)
     if precision == "mixed":
@@ -551,6 +552,8 @@ def main():
                          "per SIMD, 3..8 = plane march of 64 x TJ column tiles (4 = TJ 4, 32 planes, the default)")
     ap.add_argument("--sor-graphs", type=int, choices=[0, 1], default=1,
                     help="1: FP64 SOR pass chunks replayed from HIP graphs (biocfd_sor_graphs); 0: pass by pass")
+    ap.add_argument("--interp-batching", type=int, choices=[0, 1], default=1,
+                    help="1: every interface's update of a pass in one launch (biocfd_interp_batching); 0: one by one")
     ap.add_argument("--mixed-momentum", choices=["fp64", "both"], default="both",
                     help="both: a second mixed run with the FP32 momentum tendency, reported beside the first")
     ap.add_argument("--momentum-divisions", action="store_true",

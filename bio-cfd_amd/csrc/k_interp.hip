@@ -3,6 +3,8 @@
 // (src/FineInterp_bound.f90), driven by the per-axis interface tables built
 // on the host (src/InterfaceDetail.f90:168-241).  One thread per table entry
 // (restriction) or per fine target cell (prolongation).
+#include <cstring>
+
 #include "biocfd_rt.hpp"
 
 namespace bcfd {
@@ -50,11 +52,12 @@ struct GridSet {
 };
 
 // fine -> coarse (CoarseInterp.f90:447-502 trilinear)
-__global__ void k_restrict(Dims cd, Dims fd, ITab t, int comp, int cintp, GridSet cg, GridSet fg,
-                           const double *__restrict__ fvar, double *__restrict__ cvar, double *__restrict__ cvar2) {
+// entry q of the restriction of one (interface, component)
+__device__ inline void restrict_at(const Dims &cd, const Dims &fd, const ITab &t, int comp, int cintp,
+                                   const GridSet &cg, const GridSet &fg, const double *__restrict__ fvar,
+                                   double *__restrict__ cvar, double *__restrict__ cvar2, int64_t q) {
   const int ni = t.cnt[0] - 2 * cintp + 1, nj = t.cnt[1] - 2 * cintp + 1, nk = t.cnt[2] - 2 * cintp + 1;
   if (ni <= 0 || nj <= 0 || nk <= 0) return;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= (int64_t)ni * nj * nk) return;
   const int i = cintp + (int)(q % ni), j = cintp + (int)((q / ni) % nj), k = cintp + (int)(q / ((int64_t)ni * nj));
   const int tx = DET(t, 0, 1, i), ty = DET(t, 1, 1, j), tz = DET(t, 2, 1, k);
@@ -85,18 +88,23 @@ __global__ void k_restrict(Dims cd, Dims fd, ITab t, int comp, int cintp, GridSe
   if (cvar2) cvar2[dst] = out;
 }
 
+__global__ void k_restrict(Dims cd, Dims fd, ITab t, int comp, int cintp, GridSet cg, GridSet fg,
+                           const double *__restrict__ fvar, double *__restrict__ cvar, double *__restrict__ cvar2) {
+  restrict_at(cd, fd, t, comp, cintp, cg, fg, fvar, cvar, cvar2, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // coarse -> fine on the six faces (FineInterp_bound.f90:554-603 trilinear
 // over a two-coarse-cell span).  Entries: face-major enumeration of the
 // (i,j,k) table entries with one index pinned to the first/last entry.
-__global__ void k_prolong(Dims cd, Dims fd, ITab t, int comp, int axis, GridSet cg, GridSet fg,
-                          const double *__restrict__ cvar, double *__restrict__ fvar, double *__restrict__ fvar2,
-                          int factor) {
+// element q of one axis pass of the prolongation of one (interface, component)
+__device__ inline void prolong_at(const Dims &cd, const Dims &fd, const ITab &t, int comp, int axis,
+                                  const GridSet &cg, const GridSet &fg, const double *__restrict__ cvar,
+                                  double *__restrict__ fvar, double *__restrict__ fvar2, int factor, int64_t q) {
   // entries of this axis pass: the pinned axis takes {1, cnt}
   int n[3] = {t.cnt[0], t.cnt[1], t.cnt[2]};
   n[axis] = 2;
   const int64_t nent = (int64_t)n[0] * n[1] * n[2];
   const int per = factor * factor * factor;  // fine cells per entry (upper bound)
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nent * per) return;
   const int64_t e = q / per;
   const int sub = (int)(q % per);
@@ -132,6 +140,50 @@ __global__ void k_prolong(Dims cd, Dims fd, ITab t, int comp, int axis, GridSet 
   const int64_t dst = ix3(fd, tx - ox, ty - oy, tz - oz);
   fvar[dst] = out;
   if (fvar2) fvar2[dst] = out;
+}
+
+__global__ void k_prolong(Dims cd, Dims fd, ITab t, int comp, int axis, GridSet cg, GridSet fg,
+                          const double *__restrict__ cvar, double *__restrict__ fvar, double *__restrict__ fvar2,
+                          int factor) {
+  prolong_at(cd, fd, t, comp, axis, cg, fg, cvar, fvar, fvar2, factor,
+             (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// One launch for the same update of every live interface (and component):
+// task b covers threads first[b] .. first[b + 1] - 1 (first[ntask] = the
+// launch's threads).  For the prolongation, one launch per axis pass.
+struct InterpTask {
+  Dims cd, fd;
+  ITab t;
+  int comp, cintp, factor;
+  GridSet cg, fg;
+  const double *src;
+  double *dst;
+  int64_t first[3];   // per axis pass (the restriction uses first[0])
+};
+
+__device__ inline int task_of(const InterpTask *__restrict__ tasks, int ntask, int axis, int64_t q) {
+  int lo = 0, hi = ntask - 1;   // the last task whose first <= q
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (tasks[mid].first[axis] <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_prolong_set(const InterpTask *__restrict__ tasks, int ntask, int axis, int64_t total) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const InterpTask &T = tasks[task_of(tasks, ntask, axis, q)];
+  prolong_at(T.cd, T.fd, T.t, T.comp, axis, T.cg, T.fg, T.src, T.dst, nullptr, T.factor, q - T.first[axis]);
+}
+
+__global__ void k_restrict_set(const InterpTask *__restrict__ tasks, int ntask, int64_t total) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  const InterpTask &T = tasks[task_of(tasks, ntask, 0, q)];
+  restrict_at(T.cd, T.fd, T.t, T.comp, T.cintp, T.cg, T.fg, T.src, T.dst, nullptr, q - T.first[0]);
 }
 
 // coarse -> fine volume fill after a block move (FineInterp.f90:10-281):
@@ -188,8 +240,151 @@ static ITab tab_of(const Interface &it, int comp) {
   return t;
 }
 
-// which: 0 = p,u,v,w ; 1 = ut,vt,wt ; 2 = pc
+// The fields an update of kind `which` moves, per component: 0 = p,u,v,w ;
+// 1 = ut,vt,wt ; 2 = pc
+static int update_fields(int which, int comp) {
+  static const int f0[4] = {BF_P, BF_U, BF_V, BF_W}, f1[4] = {-1, BF_UT, BF_VT, BF_WT};
+  if (which == 0) return f0[comp];
+  if (which == 1) return f1[comp];
+  return comp == 0 ? BF_PC : -1;
+}
+
+// Whether every live interface's update can run in one launch: each fine
+// block is the fine side of one interface only and no block is both a fine
+// and a coarse side (so no update reads what another writes); for the
+// restriction also the coarse cells the interfaces write (per component)
+// must not overlap.
+static bool batchable(bool restrict_kind) {
+  int as_b[kMaxBlocks + 1] = {}, as_a[kMaxBlocks + 1] = {};
+  for (int q = 1; q <= kMaxInterfaces; ++q) {
+    const Interface &it = rt().itf[q];
+    if (!it.live) continue;
+    if (++as_b[it.b] > 1) return false;
+    ++as_a[it.a];
+  }
+  for (int g = 1; g <= kMaxBlocks; ++g)
+    if (as_a[g] && as_b[g]) return false;
+  if (!restrict_kind) return true;
+  // coarse index boxes [lo, hi] per axis of the cells each restriction writes
+  struct Box { int a, comp; int64_t lo[3], hi[3]; };
+  std::vector<Box> boxes;
+  for (int q = 1; q <= kMaxInterfaces; ++q) {
+    const Interface &it = rt().itf[q];
+    if (!it.live) continue;
+    const int64_t cintp = rt().blk[it.b].cintp;
+    for (int comp = 0; comp < 4; ++comp) {
+      Box bx{it.a, comp, {0, 0, 0}, {0, 0, 0}};
+      bool empty = false;
+      for (int ax = 0; ax < 3; ++ax) {
+        const int64_t cnt = it.counter[comp][ax];
+        const std::vector<int64_t> &h = it.hdet[comp][ax];
+        const int64_t n0 = cintp, n1 = cnt - cintp;
+        if (n1 < n0 || n0 < 1 || (size_t)(3 * n1) > h.size()) { empty = true; break; }
+        const int o = comp == ax + 1;
+        bx.lo[ax] = h[3 * (n0 - 1)] - o;
+        bx.hi[ax] = h[3 * (n1 - 1)] - o;
+      }
+      if (empty) continue;
+      for (const Box &o : boxes) {
+        if (o.a != bx.a || o.comp != bx.comp) continue;
+        bool apart = false;
+        for (int ax = 0; ax < 3; ++ax) apart = apart || bx.hi[ax] < o.lo[ax] || o.hi[ax] < bx.lo[ax];
+        if (!apart) return false;
+      }
+      boxes.push_back(bx);
+    }
+  }
+  return true;
+}
+
+// device task tables of the batched updates, per kind (0 restriction, 1
+// prolongation) and `which`, re-uploaded when their contents change
+struct TaskTable {
+  std::vector<InterpTask> h;
+  InterpTask *d = nullptr;
+  int64_t cap = 0;
+  int64_t total[3] = {0, 0, 0};
+};
+static TaskTable g_tasks[2][3];
+
+static int upload_tasks(TaskTable &tt, std::vector<InterpTask> &v) {
+  if (v.size() == tt.h.size() && (v.empty() || !memcmp(v.data(), tt.h.data(), sizeof(InterpTask) * v.size())))
+    return 0;
+  hipStream_t s = rt().stream;
+  // a launch queued earlier may still read the old table
+  BCFD_HIP(hipStreamSynchronize(s));
+  if (grow(&tt.d, &tt.cap, (int64_t)v.size())) return -1;
+  tt.h = v;
+  if (!v.empty())
+    BCFD_HIP(hipMemcpyAsync(tt.d, tt.h.data(), sizeof(InterpTask) * v.size(), hipMemcpyHostToDevice, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  return 0;
+}
+
+// the updates of kind `which` of every live interface in one launch per
+// pass (see batchable)
+static int batched_update(bool restrict_kind, int which) {
+  std::vector<InterpTask> v;
+  int64_t total[3] = {0, 0, 0};
+  for (int q = 1; q <= kMaxInterfaces; ++q) {
+    Interface &it = rt().itf[q];
+    if (!it.live) continue;
+    Block &a = rt().blk[it.a], &b = rt().blk[it.b];
+    for (int comp = 0; comp < 4; ++comp) {
+      const int f = update_fields(which, comp);
+      if (f < 0) continue;
+      InterpTask T;
+      memset(&T, 0, sizeof T);
+      T.cd = a.d;
+      T.fd = b.d;
+      T.t = tab_of(it, comp);
+      T.comp = comp;
+      T.cintp = (int)b.cintp;
+      T.factor = (int)(it.b_msh / it.a_msh);
+      T.cg = grids_of(a, comp);
+      T.fg = grids_of(b, comp);
+      T.src = restrict_kind ? b.f[f] : a.f[f];
+      T.dst = restrict_kind ? a.f[f] : b.f[f];
+      if (restrict_kind) {
+        const int64_t n = (int64_t)(T.t.cnt[0] - 2 * T.cintp + 1) * (T.t.cnt[1] - 2 * T.cintp + 1) *
+                          (T.t.cnt[2] - 2 * T.cintp + 1);
+        if (n <= 0) continue;
+        T.first[0] = total[0];
+        total[0] += n;
+      } else {
+        for (int axis = 0; axis < 3; ++axis) {
+          int n[3] = {T.t.cnt[0], T.t.cnt[1], T.t.cnt[2]};
+          n[axis] = 2;
+          T.first[axis] = total[axis];
+          total[axis] += (int64_t)n[0] * n[1] * n[2] * T.factor * T.factor * T.factor;
+        }
+      }
+      v.push_back(T);
+    }
+  }
+  if (v.empty()) return 0;
+  TaskTable &tt = g_tasks[restrict_kind ? 0 : 1][which];
+  if (upload_tasks(tt, v)) return -1;
+  hipStream_t s = rt().stream;
+  const int nt = (int)v.size();
+  if (restrict_kind) {
+    hipLaunchKernelGGL(k_restrict_set, dim3(div_up(total[0], 256)), dim3(256), 0, s, tt.d, nt, total[0]);
+  } else {
+    for (int axis = 0; axis < 3; ++axis)
+      if (total[axis] > 0)
+        hipLaunchKernelGGL(k_prolong_set, dim3(div_up(total[axis], 256)), dim3(256), 0, s, tt.d, nt, axis,
+                           total[axis]);
+  }
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+// one launch per pass for all interfaces where allowed (the default), else
+// interface by interface
+static int g_interp_batch = 1;
+
 int interface_coarse_update(int which) {
+  if (g_interp_batch && batchable(true)) return batched_update(true, which);
   hipStream_t s = rt().stream;
   for (int q = 1; q <= kMaxInterfaces; ++q) {
     Interface &it = rt().itf[q];
@@ -254,6 +449,7 @@ int interface_fine_update_one(int q, int which) {
 }
 
 int interface_fine_update(int which) {
+  if (g_interp_batch && batchable(false)) return batched_update(false, which);
   for (int q = 1; q <= kMaxInterfaces; ++q)
     if (rt().itf[q].live && interface_fine_update_one(q, which)) return -1;
   return 0;
@@ -300,6 +496,16 @@ int interface_fine_update_mv(int q) {
 }  // namespace bcfd
 
 using namespace bcfd;
+
+extern "C" {
+
+int biocfd_interp_batching(int on) {
+  BCFD_API_LOCK();
+  g_interp_batch = on ? 1 : 0;
+  return 0;
+}
+
+}  // extern "C"
 
 extern "C" {
 

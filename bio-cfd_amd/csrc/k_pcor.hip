@@ -945,17 +945,41 @@ int sor_timing_end_set(hipEvent_t e0, const std::vector<Block *> &blocks, const 
 // once chunk k+1 is queued; stop = every block had stopped at chunk k - 1.
 static SorState *g_set_poll_h = nullptr;   // two sets of kMaxBlocks + 1 states
 static hipEvent_t g_set_poll_ev[2] = {nullptr, nullptr};
+// the set's states are gathered on the device (one small kernel) and read
+// back with one copy, instead of one copy per block
+static SorState **g_poll_ptrs = nullptr;   // kMaxBlocks + 1 device pointers
+static std::vector<SorState *> g_poll_ptrs_h;
+static SorState *g_poll_stage = nullptr;   // two sets of kMaxBlocks + 1 states
+
+__global__ void k_gather_states(SorState *const *__restrict__ ptrs, int n, SorState *__restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = *ptrs[i];
+}
 
 int set_poll(const std::vector<Block *> &blocks, long long &k, bool &stop) {
   hipStream_t s = rt().stream;
   if (!g_set_poll_h)
     BCFD_HIP(hipHostMalloc((void **)&g_set_poll_h, sizeof(SorState) * 2 * (kMaxBlocks + 1), hipHostMallocDefault));
+  if (!g_poll_stage && (dev_alloc(&g_poll_stage, 2 * (kMaxBlocks + 1)) || dev_alloc(&g_poll_ptrs, kMaxBlocks + 1)))
+    return -1;
   for (auto &e : g_set_poll_ev)
     if (!e) BCFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  std::vector<SorState *> ptrs(blocks.size());
+  for (size_t i = 0; i < blocks.size(); ++i) ptrs[i] = blocks[i]->sor;
+  if (ptrs != g_poll_ptrs_h) {
+    // a gather queued earlier may still read the old pointers
+    BCFD_HIP(hipStreamSynchronize(s));
+    g_poll_ptrs_h = ptrs;
+    BCFD_HIP(hipMemcpyAsync(g_poll_ptrs, g_poll_ptrs_h.data(), sizeof(SorState *) * ptrs.size(),
+                            hipMemcpyHostToDevice, s));
+    BCFD_HIP(hipStreamSynchronize(s));
+  }
   const int cur = (int)(k & 1);
-  SorState *set = g_set_poll_h + cur * (kMaxBlocks + 1);
-  for (size_t i = 0; i < blocks.size(); ++i)
-    BCFD_HIP(hipMemcpyAsync(&set[i], blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
+  const int n = (int)blocks.size();
+  SorState *set = g_set_poll_h + cur * (kMaxBlocks + 1), *stage = g_poll_stage + cur * (kMaxBlocks + 1);
+  hipLaunchKernelGGL(k_gather_states, dim3(div_up(n, 64)), dim3(64), 0, s, g_poll_ptrs, n, stage);
+  BCFD_CHECK_LAUNCH();
+  BCFD_HIP(hipMemcpyAsync(set, stage, sizeof(SorState) * (size_t)n, hipMemcpyDeviceToHost, s));
   BCFD_HIP(hipEventRecord(g_set_poll_ev[cur], s));
   stop = false;
   if (k > 0) {

@@ -822,6 +822,10 @@ module biocfd_hip
       import :: c_int
       integer(c_int), value :: on
     end function
+    integer(c_int) function biocfd_interp_batching(on) bind(C, name="biocfd_interp_batching")
+      import :: c_int
+      integer(c_int), value :: on
+    end function
     integer(c_int) function biocfd_sor_graph_stats(captures, launches) bind(C, name="biocfd_sor_graph_stats")
       import :: c_int, c_int64_t
       integer(c_int64_t) :: captures, launches

@@ -126,6 +126,7 @@ SIGNATURES = {
     "biocfd_output_fields": [_I, _P, _P, _P, _P, _P, _P, _P],
     "biocfd_sor_batching": [_I],
     "biocfd_sor_graphs": [_I],
+    "biocfd_interp_batching": [_I],
     "biocfd_sor_graph_stats": [_P, _P],
     "biocfd_momentum_config": [_I, _I],
     "biocfd_momentum_precision": [_I],

@@ -201,6 +201,15 @@ int biocfd_sor_batching(int on);
  * instead of the passes' and end kernels' launches; off, every pass is
  * launched.  Both give the same bits. */
 int biocfd_sor_graphs(int on);
+/* The coarse/fine interface updates inside poisson_solver (no reference
+ * counterpart; the reference updates one interface after the other,
+ * CoarseInterp.f90:447-502, FineInterp_bound.f90:554-603): on (the
+ * default), the same update of every live interface and component runs in
+ * one launch (per axis pass of the prolongation) from a task table, where
+ * no interface reads what another writes (each fine block on one interface,
+ * no block on both sides, the restrictions' coarse cells disjoint); else,
+ * and off, interface by interface.  Both give the same bits. */
+int biocfd_interp_batching(int on);
 /* Graphs captured and graph launches of biocfd_sor_graphs since the
  * library was loaded (no reference counterpart; a test and bench probe). */
 int biocfd_sor_graph_stats(int64_t *captures, int64_t *launches);
