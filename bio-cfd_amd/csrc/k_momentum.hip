@@ -13,6 +13,7 @@
 // by one in list order, by a single workgroup; every other fluid cell reads
 // only untouched values and runs in the dense parallel kernel.
 #include <cmath>
+#include <cstring>
 #include <type_traits>
 
 #include "biocfd_rt.hpp"
@@ -356,8 +357,8 @@ __device__ void momentum_cell(const MomArgs &a, const Fields &F, int i, int j, i
 // XCD e takes the e-th eighth of every plane's tiles in row order (a band of
 // j rows), so most of the j halo rows and k neighbour planes a tile reads
 // were brought into that XCD's L2 by its neighbours.
-__device__ inline void mom_tile(int gx, int gy, int &ti, int &tj, int &tk) {
-  const int L = blockIdx.x, T = gx * gy;
+__device__ inline void mom_tile(int L, int gx, int gy, int &ti, int &tj, int &tk) {
+  const int T = gx * gy;
   int t;
   if ((T & 7) == 0) {
     const int per = T >> 3, l = L >> 3;
@@ -385,7 +386,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)
 k_momentum(MomArgs a, Fields F, double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt,
            const uint8_t *__restrict__ deferred, int gx, int gy) {
   int ti, tj, tk;
-  mom_tile(gx, gy, ti, tj, tk);
+  mom_tile(blockIdx.x, gx, gy, ti, tj, tk);
   const int i = 2 + ti * 64 + threadIdx.x;
   const int j = 2 + tj * 4 + __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int k = 2 + tk;
@@ -404,7 +405,7 @@ template <int C, bool P2>
 __global__ void __launch_bounds__(256) k_momentum_c(MomArgs a, Fields F, double *__restrict__ out,
                                                     const uint8_t *__restrict__ deferred, int gx, int gy) {
   int ti, tj, tk;
-  mom_tile(gx, gy, ti, tj, tk);
+  mom_tile(blockIdx.x, gx, gy, ti, tj, tk);
   const int i = 2 + ti * 64 + threadIdx.x;
   const int j = 2 + tj * 4 + __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int k = 2 + tk;
@@ -567,10 +568,11 @@ struct MAcc {
   template <int T> __device__ cptr<R> zk() const { return (cptr<R>)(mom_ck<R>(*a, T) + 6 * (k - 1)); }
 };
 
-template <int TJ, int KC, int WPE, bool P2, class R = double>
-__global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
-k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt,
-                 const uint8_t *__restrict__ deferred, int gx, int gy) {
+// the march of tile L (workgroup index within the block) of one block
+template <int TJ, int KC, bool P2, class R>
+__device__ __forceinline__ void march_run(const MomArgs &a, const Fields &F, double *__restrict__ ut,
+                                          double *__restrict__ vt, double *__restrict__ wt,
+                                          const uint8_t *__restrict__ deferred, int gx, int gy, int L) {
   constexpr int NT = 64 * TJ, kW = MarchPlane<TJ>::kW;
   constexpr int NH2 = 4 * kW + 4 * TJ, NH1 = 2 * 66 + 2 * TJ, NHP = 65 + TJ, S2 = (NH2 + NT - 1) / NT;
   static_assert(NH1 <= NT && NHP <= NT, "one-cell halos: one position per thread");
@@ -579,7 +581,7 @@ k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restric
   __shared__ MarchPlane<TJ, R> S[2];
   __shared__ R tab[4][64 * 6];   // ca UU, ck UU, ca UW, ck UW rows of the tile's columns
   int ti, tj, tc;
-  mom_tile(gx, gy, ti, tj, tc);
+  mom_tile(L, gx, gy, ti, tj, tc);
   const Dims &d = a.d;
   const int lane = threadIdx.x, row = __builtin_amdgcn_readfirstlane(threadIdx.y);
   const int tid = lane + 64 * row;
@@ -763,6 +765,36 @@ k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restric
     cm1 = cc0; cc0 = cp1; cp1 = ldc(k + 2);
     c2m2 = c2m1; c2m1 = c20; c20 = c2p1; c2p1 = c2p2; c2p2 = ldc2(k + 3);
   }
+}
+
+template <int TJ, int KC, int WPE, bool P2, class R = double>
+__global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt,
+                 const uint8_t *__restrict__ deferred, int gx, int gy) {
+  march_run<TJ, KC, P2, R>(a, F, ut, vt, wt, deferred, gx, gy, (int)blockIdx.x);
+}
+
+// The dense pass of several blocks in one launch: block b of the table owns
+// workgroups first .. first + ntiles - 1 (first a multiple of 8, so that a
+// workgroup's XCD is the same as in a launch of its own and the tile
+// mapping keeps its XCD bands).
+struct MomTask {
+  MomArgs a;
+  Fields F;
+  double *ut, *vt, *wt;
+  const uint8_t *deferred;
+  int gx, gy, first, ntiles;
+};
+
+template <int TJ, int KC, int WPE, bool P2, class R = double>
+__global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_momentum_march_set(const MomTask *__restrict__ tab, int ntask) {
+  int b = 0;
+  while (b + 1 < ntask && (int)blockIdx.x >= tab[b + 1].first) ++b;
+  const MomTask &T = tab[b];
+  const int L = (int)blockIdx.x - T.first;
+  if (L >= T.ntiles) return;
+  march_run<TJ, KC, P2, R>(T.a, T.F, T.ut, T.vt, T.wt, T.deferred, T.gx, T.gy, L);
 }
 
 // ---------------------------------------------------------------------------
@@ -1251,14 +1283,12 @@ extern "C" int biocfd_ts_pass_config(int parallel) {
   return 0;
 }
 
-extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
-  BCFD_API_LOCK();
-  Block *b = get_block(id);
-  if (!b) return -1;
+namespace {
+
+int block_mom_args(Block *b, int id, double dt, double re, MomArgs &a) {
   for (int t = 0; t < BTB_N; ++t)
     if (!b->ca[t]) { set_error("non_uni_coeff has not been called for block %d", id); return -1; }
   if (!b->g[BG_DX]) { set_error("grids not uploaded"); return -1; }
-  MomArgs a;
   a.d = b->d;
   a.cell = b->tag[BT_CELL];
   a.cell2 = b->tag[BT_CELL2];
@@ -1270,27 +1300,37 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   a.dx32 = b->h32[0]; a.dy32 = b->h32[1]; a.dz32 = b->h32[2];
   a.dt = dt;
   a.re = re;
-  Fields F{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
+  return 0;
+}
+
+// before the dense pass: with TS cells, the positions the TS-adjacent cells
+// may write (touched) and the cells deferred to the TS pass; *deferred =
+// nullptr without TS cells
+int mom_pre(Block *b, uint8_t **deferred) {
+  *deferred = nullptr;
+  if (b->counts[BC_TS] == 0) return 0;
   hipStream_t s = rt().stream;
-  dim3 blk(64, 4, 1);
-  dim3 grd(div_up(b->d.nx, 64), div_up(b->d.ny, 4), (unsigned)b->d.nz);
-  const int n_ts = (int)b->counts[BC_TS];
-  if (n_ts == 0) {
-    launch_dense(a, F, b, s, nullptr);
-    BCFD_CHECK_LAUNCH();
-    return 0;
-  }
-  // TS path: touched positions -> deferred mask -> dense pass -> ordered pass;
-  // the masks, the deferred list and the ghost choices are kept on the block
+  const dim3 blk(64, 4, 1), grd(div_up(b->d.nx, 64), div_up(b->d.ny, 4), (unsigned)b->d.nz);
   IbData &ib = b->ib;
   if (grow(&ib.ts_mask, &ib.cap_ts_mask, 2 * b->n2)) return -1;
-  uint8_t *touched = ib.ts_mask, *deferred = ib.ts_mask + b->n2;
+  uint8_t *touched = ib.ts_mask, *dfr = ib.ts_mask + b->n2;
   BCFD_HIP(hipMemsetAsync(touched, 0, b->n2, s));
-  BCFD_HIP(hipMemsetAsync(deferred, 0, b->n2, s));
+  BCFD_HIP(hipMemsetAsync(dfr, 0, b->n2, s));
   hipLaunchKernelGGL(k_ts_touch, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched);
-  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched, deferred);
-  launch_dense(a, F, b, s, deferred);
+  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched, dfr);
   BCFD_CHECK_LAUNCH();
+  *deferred = dfr;
+  return 0;
+}
+
+// after the dense pass: the deferred cells (TS path: deferred list, ghost
+// choices, the ordered or parallel TS pass)
+int mom_post(Block *b, const MomArgs &a) {
+  const int n_ts = (int)b->counts[BC_TS];
+  if (n_ts == 0) return 0;
+  hipStream_t s = rt().stream;
+  IbData &ib = b->ib;
+  uint8_t *touched = ib.ts_mask, *deferred = ib.ts_mask + b->n2;
   int64_t cnt = 0;
   if (compact_mask_lin(*b, deferred, &ib.ts_list, &ib.cap_ts_list, &cnt)) return -1;
   const int *list = ib.ts_list;
@@ -1334,6 +1374,106 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
     }
   }
   BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+
+// the dense pass of blocks through one launch per power-of-two class (the
+// default march shape, FP64 or the FP32 tendency); other variants block by
+// block
+struct MomSetTable {
+  std::vector<MomTask> h;
+  MomTask *d = nullptr;
+  int64_t cap = 0;
+};
+MomSetTable g_mom_set[2];
+
+template <bool P2>
+int launch_dense_set(std::vector<MomTask> &v, MomSetTable &tt) {
+  if (v.empty()) return 0;
+  hipStream_t s = rt().stream;
+  if (v.size() != tt.h.size() || memcmp(v.data(), tt.h.data(), sizeof(MomTask) * v.size())) {
+    // a launch queued earlier may still read the old table
+    BCFD_HIP(hipStreamSynchronize(s));
+    if (grow(&tt.d, &tt.cap, (int64_t)v.size())) return -1;
+    tt.h = v;
+    BCFD_HIP(hipMemcpyAsync(tt.d, tt.h.data(), sizeof(MomTask) * v.size(), hipMemcpyHostToDevice, s));
+    BCFD_HIP(hipStreamSynchronize(s));
+  }
+  const int64_t wgs = v.back().first + v.back().ntiles;
+  if (g_mom_fp32)
+    hipLaunchKernelGGL((k_momentum_march_set<4, 32, 2, P2, float>), dim3((unsigned)wgs), dim3(64, 4, 1), 0, s, tt.d,
+                       (int)v.size());
+  else
+    hipLaunchKernelGGL((k_momentum_march_set<4, 32, 2, P2, double>), dim3((unsigned)wgs), dim3(64, 4, 1), 0, s,
+                       tt.d, (int)v.size());
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
+}  // namespace
+
+extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
+  BCFD_API_LOCK();
+  Block *b = get_block(id);
+  if (!b) return -1;
+  MomArgs a;
+  if (block_mom_args(b, id, dt, re, a)) return -1;
+  uint8_t *deferred = nullptr;
+  if (mom_pre(b, &deferred)) return -1;
+  Fields F{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
+  launch_dense(a, F, b, rt().stream, deferred);
+  BCFD_CHECK_LAUNCH();
+  return mom_post(b, a);
+}
+
+extern "C" int biocfd_ns_momentum_set(int first, int last, double dt, double re) {
+  BCFD_API_LOCK();
+  if (first < 1 || last < first || last > kMaxBlocks) {
+    set_error("ns_momentum_set: bad block range %d..%d", first, last);
+    return -1;
+  }
+  const int n = last - first + 1;
+  std::vector<MomArgs> args((size_t)n);
+  std::vector<uint8_t *> dfr((size_t)n);
+  for (int g = first; g <= last; ++g) {
+    Block *b = get_block(g);
+    if (!b || block_mom_args(b, g, dt, re, args[(size_t)(g - first)])) return -1;
+  }
+  for (int g = first; g <= last; ++g)
+    if (mom_pre(&rt().blk[g], &dfr[(size_t)(g - first)])) return -1;
+  if (g_mom_variant != 4 && !g_mom_fp32) {
+    for (int g = first; g <= last; ++g) {
+      Block *b = &rt().blk[g];
+      Fields F{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
+      launch_dense(args[(size_t)(g - first)], F, b, rt().stream, dfr[(size_t)(g - first)]);
+    }
+    BCFD_CHECK_LAUNCH();
+  } else {
+    std::vector<MomTask> v[2];   // power-of-two class 0 / 1
+    int64_t wgs[2] = {0, 0};
+    for (int g = first; g <= last; ++g) {
+      Block *b = &rt().blk[g];
+      const int cls = (g_mom_pow2 && pow2_grid(*b)) ? 1 : 0;
+      MomTask t;
+      memset(&t, 0, sizeof t);
+      t.a = args[(size_t)(g - first)];
+      t.F = Fields{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
+      t.ut = b->f[BF_UT];
+      t.vt = b->f[BF_VT];
+      t.wt = b->f[BF_WT];
+      t.deferred = dfr[(size_t)(g - first)];
+      t.gx = (int)div_up(b->d.nx, 64);
+      t.gy = (int)div_up(b->d.ny, 4);
+      t.ntiles = t.gx * t.gy * (int)div_up(b->d.nz, 32);
+      t.first = (int)wgs[cls];
+      wgs[cls] += (t.ntiles + 7) / 8 * 8;
+      v[cls].push_back(t);
+    }
+    if (launch_dense_set<false>(v[0], g_mom_set[0]) || launch_dense_set<true>(v[1], g_mom_set[1])) return -1;
+  }
+  for (int g = first; g <= last; ++g)
+    if (mom_post(&rt().blk[g], args[(size_t)(g - first)])) return -1;
   return 0;
 }
 

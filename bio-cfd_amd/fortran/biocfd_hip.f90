@@ -279,6 +279,11 @@ module biocfd_hip
       real(c_double), value :: dt
       real(c_double), value :: re
     end function
+    integer(c_int) function biocfd_ns_momentum_set(first, last, dt, re) bind(C, name="biocfd_ns_momentum_set")
+      import :: c_int, c_double
+      integer(c_int), value :: first, last
+      real(c_double), value :: dt, re
+    end function
 
     integer(c_int) function biocfd_velocity_bc(blk, dt, uc) bind(C, name="biocfd_velocity_bc")
       import :: c_int, c_int64_t, c_double, c_ptr

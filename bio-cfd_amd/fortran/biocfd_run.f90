@@ -793,9 +793,12 @@ contains
     real(real64) :: kv(0:BK_N - 1)
     ita = ita + 1
     totime = totime + dt
+    ! nsMomentum2order of every block in one call (one dense launch), then
+    ! each block's BCs and forcing: a block's momentum step touches only its
+    ! own arrays
+    call biocfd_check(biocfd_ns_momentum_set(1_c_int, int(nblocks, c_int), dt, re), 'ns_momentum_set')
     do g = 1, int(nblocks)
       b => blk(g)
-      call biocfd_check(biocfd_ns_momentum(g, dt, re), 'ns_momentum')
       if (g == 1) call biocfd_check(biocfd_velocity_bc(g, dt, uc), 'velocity_bc')
       if (b%has_ib == 1) then
         call biocfd_check(biocfd_solid_cell_bc(g), 'solid_cell_bc')

@@ -199,7 +199,7 @@ class Simulation:
         self.totime = self.totime + c.dt
 
         def momentum(st):
-            H.call("biocfd_ns_momentum", st.g, c.dt, c.re)
+            # nsMomentum2order itself ran for every block above
             if st.g == 1:
                 H.call("biocfd_velocity_bc", st.g, c.dt, c.uc)
             if st.body is not None:
@@ -243,6 +243,11 @@ class Simulation:
             H.call("biocfd_velocity_forcing_ghost", g, H.ptr(kv))
             H.call("biocfd_pressure_forcing_ghost", g, H.ptr(kv))
 
+        # the momentum step of every block in one call (one dense launch for
+        # all blocks; each block's step reads and writes only its own arrays,
+        # so the reference's per-block order is kept by the blocks' BCs and
+        # forcing that follow)
+        H.call("biocfd_ns_momentum_set", 1, len(self.blocks), c.dt, c.re)
         self._each(momentum)
         H.call("biocfd_poisson_solver", 1, len(self.blocks), c.pc_ita_max, c.epsi, c.dt, c.uc)
         self._each(pressure_forcing, bodies_only=True)

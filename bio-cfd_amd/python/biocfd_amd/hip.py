@@ -70,6 +70,7 @@ SIGNATURES = {
     "biocfd_coefficient_matrix": [_I, _I],
     "biocfd_non_uni_coeff": [_I],
     "biocfd_ns_momentum": [_I, _D, _D],
+    "biocfd_ns_momentum_set": [_I, _I, _D, _D],
     "biocfd_velocity_bc": [_I, _D, _D],
     "biocfd_solid_cell_bc": [_I],
     "biocfd_solid_cell_bc_move": [_I, _L],

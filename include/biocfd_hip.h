@@ -126,6 +126,14 @@ int biocfd_non_uni_coeff(int blk);
 /* ---- momentum and boundary conditions ---- */
 /* nsMomentum2order, NavierStokes.f90:252-912 */
 int biocfd_ns_momentum(int blk, double dt, double re);
+/* nsMomentum2order on blocks first..last (the reference calls it per block in
+ * an OpenMP loop, main.F90:133-148; each block's step reads and writes only
+ * its own arrays, so the calls are independent): the thin-surface pre-pass
+ * per block, the dense pass of all blocks in one launch over a table of the
+ * blocks (the default march shape, FP64 or the FP32 tendency; another
+ * momentum variant runs block by block), then each block's thin-surface
+ * pass.  The same bits as biocfd_ns_momentum on each block. */
+int biocfd_ns_momentum_set(int first, int last, double dt, double re);
 /* Dense momentum kernel of biocfd_ns_momentum (no reference counterpart):
  * variant 0 = the three components per thread, one workgroup per 64 x 4
  * cells of a plane; 1 = one launch per velocity component; 2 = as 0 with its
