@@ -97,68 +97,75 @@ __global__ void k_adj_grad(Dims d, const uint8_t *__restrict__ cell, const doubl
   lpc[x.c] = l;
 }
 
-// wd = w / D at fluid cells, 0 elsewhere (the ghost ring included)
+// wd = w / D at fluid cells, 0 elsewhere (the ghost ring included), and the
+// iterate moved to m = wd * lambda (m = 0 off the fluid cells)
 __global__ void k_adj_wd(Dims d, const uint8_t *__restrict__ cell, const double *__restrict__ Ax,
                          const double *__restrict__ Ay, const double *__restrict__ Az, double omega,
-                         double *__restrict__ wd) {
+                         double *__restrict__ wd, double *__restrict__ lpc) {
   Cell x;
   if (!cell_of(d, x)) return;
-  wd[x.c] = fluid(d, cell, x.i, x.j, x.k) ? omega / diag(Ax, Ay, Az, d, x.i, x.j, x.k) : 0.0;
+  const double w = fluid(d, cell, x.i, x.j, x.k) ? omega / diag(Ax, Ay, Az, d, x.i, x.j, x.k) : 0.0;
+  wd[x.c] = w;
+  lpc[x.c] = w * lpc[x.c];
 }
 
 // The transpose of one iteration (black sweep B after red sweep R) is
 // B^T(a), B^T(b), R^T(a), R^T(b), and step (b) of one colour and step (a)
 // of the other touch the same cells: one launch per colour does, at its
-// fluid cells, the pending (b) (unless first) and then the gather (a) from
-// the other colour's cells, which this launch does not write (unless last).
-// n iterations are 2 n + 1 launches, red first and last.  One thread per
-// interior cell of the launch's colour: thread (ii, j, k) takes i = 2 + 2 ii
-// + p, p the parity that gives (i + j + k) odd for red, even for black.
-// Non-fluid neighbours have wd = 0; a fluid cell's neighbours lie inside the
-// ghost ring, and the coefficient tables are read only for interior rows.
-__global__ void k_adj_sor_colour(Dims d, const uint8_t *__restrict__ cell, const double *__restrict__ Ax,
-                                 const double *__restrict__ Ay, const double *__restrict__ Az,
-                                 const double *__restrict__ wd, double dt, double omega, int red_cells, int first,
-                                 int last, double *__restrict__ lpc, double *__restrict__ bbar) {
-  const int W = (d.nx + 1) / 2;
-  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (int64_t)W * d.ny * d.nz) return;
-  const int ii = (int)(q % W);
-  const int j = 2 + (int)((q / W) % d.ny);
-  const int k = 2 + (int)(q / ((int64_t)W * d.ny));
+// fluid cells, the pending (b) (unless first) and then the gather (a) from the
+// other colour's cells, which this launch does not write (unless last).
+// n iterations are 2 n + 1 launches, red first and last.  In m = wd * lambda
+// the steps read only m of the neighbours:
+//   (b)  bbar_y += m_y / dt,  m_y = (1 - w) m_y
+//   (a)  m_x -= wd_x * sum over neighbours y of a_{y->x} m_y
+// (m = 0 at non-fluid cells).  One thread per interior cell of the launch's
+// colour on a 3-D grid (64 x 4 threads: i of the colour, j; one plane per
+// grid layer): thread (ii, j, k) takes i = 2 + 2 ii + p, p the parity that
+// gives (i + j + k) odd for red, even for black.  A fluid cell's neighbours
+// lie inside the ghost ring; the coefficient tables are read only for
+// interior rows.
+__global__ void __launch_bounds__(256) k_adj_sor_colour(Dims d, const uint8_t *__restrict__ cell,
+                                                        const double *__restrict__ Ax, const double *__restrict__ Ay,
+                                                        const double *__restrict__ Az, const double *__restrict__ wd,
+                                                        double rdt, double omega, int red_cells, int first, int last,
+                                                        double *__restrict__ m, double *__restrict__ bbar) {
+  const int ii = blockIdx.x * 64 + threadIdx.x;
+  const int j = 2 + blockIdx.y * 4 + threadIdx.y;
+  const int k = 2 + blockIdx.z;
   const int i = 2 + 2 * ii + ((red_cells + j + k) & 1);
-  if (i > d.nx + 1) return;
-  const int64_t c = ix3(d, i, j, k);
+  if (i > d.nx + 1 || j > d.ny + 1) return;
+  const int sy = d.NX, sz = d.NX * d.NY;
+  const int c = (i - 1) + sy * (j - 1) + sz * (k - 1);
   if (cell[c] != 0) return;
-  double l = lpc[c];
+  double mc = m[c];
   if (!first) {
-    bbar[c] += wd[c] / dt * l;
-    l = (1.0 - omega) * l;
+    bbar[c] += mc * rdt;
+    mc = (1.0 - omega) * mc;
   }
   if (!last) {
-    const int64_t sy = d.NX, sz = (int64_t)d.NX * d.NY;
-    double acc = 0.0;
     // neighbour y west of x: x is y's east neighbour (column 2), and so on
-    if (i > 2) acc += coef(Ax, d.nx, i - 1, 2) * wd[c - 1] * lpc[c - 1];
-    if (i <= d.nx) acc += coef(Ax, d.nx, i + 1, 0) * wd[c + 1] * lpc[c + 1];
-    if (j > 2) acc += coef(Ay, d.ny, j - 1, 2) * wd[c - sy] * lpc[c - sy];
-    if (j <= d.ny) acc += coef(Ay, d.ny, j + 1, 0) * wd[c + sy] * lpc[c + sy];
-    if (k > 2) acc += coef(Az, d.nz, k - 1, 2) * wd[c - sz] * lpc[c - sz];
-    if (k <= d.nz) acc += coef(Az, d.nz, k + 1, 0) * wd[c + sz] * lpc[c + sz];
-    l -= acc;
+    double acc = 0.0;
+    if (i > 2) acc += coef(Ax, d.nx, i - 1, 2) * m[c - 1];
+    if (i <= d.nx) acc += coef(Ax, d.nx, i + 1, 0) * m[c + 1];
+    if (j > 2) acc += coef(Ay, d.ny, j - 1, 2) * m[c - sy];
+    if (j <= d.ny) acc += coef(Ay, d.ny, j + 1, 0) * m[c + sy];
+    if (k > 2) acc += coef(Az, d.nz, k - 1, 2) * m[c - sz];
+    if (k <= d.nz) acc += coef(Az, d.nz, k + 1, 0) * m[c + sz];
+    mc -= wd[c] * acc;
   }
-  lpc[c] = l;
+  m[c] = mc;
 }
 
 // lpc <- (B_1^T R_1^T ... B_N^T R_N^T applied backwards), bbar accumulated
+// (lpc ends as wd * lambda: only bbar is read afterwards)
 void adj_sor_launches(const Block &b, int64_t n_iter, double dt, double *wd, double *lpc, double *bbar) {
   hipStream_t s = rt().stream;
   const dim3 blk(256), grd(div_up(b.n2, 256));
   const uint8_t *cell = b.tag[BT_CELL];
-  hipLaunchKernelGGL(k_adj_wd, grd, blk, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], b.omega, wd);
-  const dim3 grc(div_up((int64_t)((b.d.nx + 1) / 2) * b.d.ny * b.d.nz, 256));
+  hipLaunchKernelGGL(k_adj_wd, grd, blk, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], b.omega, wd, lpc);
+  const dim3 bc(64, 4, 1), gc((unsigned)div_up((b.d.nx + 1) / 2, 64), (unsigned)div_up(b.d.ny, 4), (unsigned)b.d.nz);
   for (int64_t q = 0; q <= 2 * n_iter; ++q)
-    hipLaunchKernelGGL(k_adj_sor_colour, grc, blk, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], wd, dt, b.omega,
+    hipLaunchKernelGGL(k_adj_sor_colour, gc, bc, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], wd, 1.0 / dt, b.omega,
                        q % 2 == 0 ? 1 : 0, q == 0 ? 1 : 0, q == 2 * n_iter ? 1 : 0, lpc, bbar);
 }
 
@@ -166,6 +173,10 @@ void adj_sor_launches(const Block &b, int64_t n_iter, double dt, double *wd, dou
 // with SOR graphs on, as one graph captured on first use and replayed while
 // the iteration count, dt and the tag array stay the same
 int adj_sor(Block &b, int64_t n_iter, double dt, double *wd, double *lpc, double *bbar, AdWork *w) {
+  if (b.n2 >= ((int64_t)1 << 31)) {
+    set_error("adjoint SOR: block of %lld cells (32-bit cell offsets)", (long long)b.n2);
+    return -1;
+  }
   if (!w || !sor_graphs_on()) {
     adj_sor_launches(b, n_iter, dt, wd, lpc, bbar);
     BCFD_CHECK_LAUNCH();
