@@ -239,6 +239,12 @@ def run_uneven(args, batching: int, variant, precision: str = "fp64"):
         H.call("biocfd_mixed_config", 26, 32, args.uneven_sweeps, 0.0)
     for _ in range(args.warmup):
         sim.step()
+    coarse = None
+    if args.uneven_coarse_tune and precision == "fp64":
+        # the coarse block's own calls: its kernel timed on its live state
+        from biocfd_amd import tune
+        cv, cp, _ = tune.autotune_block(1, dt=case.dt)
+        coarse = {"variant": cv, "planes": cp}
     H.call("biocfd_synchronize")
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -272,6 +278,7 @@ def run_uneven(args, batching: int, variant, precision: str = "fp64"):
     out = {"ms_per_step": round(1e3 * secs / args.steps, 3),
            "mcell_steps_per_s": round(cells * args.steps / secs / 1e6, 3),
            "nIterPcor_last_step": n_iter,
+           "coarse_sor_kernel": coarse,
            "timed_sor_step": {"ms": round(1e3 * (t3 - t2), 3), "sor_gpu_ms": round(float(sor_ms[0]), 3),
                               "sor_cell_iterations": int(sor_ci[0])}}
     bpc = SOR_BYTES_PER_CELL_ITER if precision == "fp64" else mixed_bytes_per_cell(args.uneven_sweeps) / args.uneven_sweeps
@@ -552,6 +559,8 @@ def main():
                          "per SIMD, 3..8 = plane march of 64 x TJ column tiles (4 = TJ 4, 32 planes, the default)")
     ap.add_argument("--sor-graphs", type=int, choices=[0, 1], default=1,
                     help="1: FP64 SOR pass chunks replayed from HIP graphs (biocfd_sor_graphs); 0: pass by pass")
+    ap.add_argument("--uneven-coarse-tune", type=int, choices=[0, 1], default=1,
+                    help="1: the uneven workload's coarse block gets its own SOR kernel, timed on its live state")
     ap.add_argument("--interp-batching", type=int, choices=[0, 1], default=1,
                     help="1: every interface's update of a pass in one launch (biocfd_interp_batching); 0: one by one")
     ap.add_argument("--mixed-momentum", choices=["fp64", "both"], default="both",

@@ -225,6 +225,7 @@ struct Block {
   int64_t split_n = 0;
   double *pc2 = nullptr;              // second pc array of the fused SOR iterations
   int64_t pc2_n = 0;
+  int sor_variant = -1, sor_planes = 0;   // this block's own FP64 SOR kernel (-1: the library's)
   // mixed-precision solve: FP32 correction pair (full layout) and FP32
   // residual halves (colour-split)
   float *e32[2] = {nullptr, nullptr}, *r32[2] = {nullptr, nullptr};

@@ -1141,7 +1141,8 @@ int redblack_sor(Block &b, int64_t pc_ita_max, double dt, double epsi) {
   if (precision_mode() == 1) return mixed_sor(b, pc_ita_max, dt);
   hipStream_t s = rt().stream;
   hipEvent_t e0 = nullptr;
-  const int variant = g_sor_variant, planes = planes_for({&b}, variant, g_sor_planes);
+  const int variant = b.sor_variant >= 0 ? b.sor_variant : g_sor_variant;
+  const int planes = b.sor_variant >= 0 ? b.sor_planes : planes_for({&b}, variant, g_sor_planes);
   if (g_stop.rule == 1 && variant == 0) {
     set_error("redblack_sor: the residual stopping rule needs a fused SOR variant (>= 1), not variant 0");
     return -1;
@@ -1613,6 +1614,23 @@ int biocfd_sor_stop_rule(int rule, int cadence) {
   }
   g_stop.rule = 1;
   g_stop.cadence = cadence;
+  return 0;
+}
+
+int biocfd_sor_block_config(int blk, int variant, int planes_per_workgroup) {
+  BCFD_API_LOCK();
+  Block *b = get_block(blk);
+  if (!b) return -1;
+  if (variant < -1 || variant >= kNumSorVariants) {
+    set_error("sor_block_config: variant %d outside -1..%d", variant, kNumSorVariants - 1);
+    return -1;
+  }
+  if (variant >= 0 && (planes_per_workgroup < 1 || planes_per_workgroup > 4096)) {
+    set_error("sor_block_config: planes per workgroup %d outside 1..4096", planes_per_workgroup);
+    return -1;
+  }
+  b->sor_variant = variant;
+  b->sor_planes = planes_per_workgroup;
   return 0;
 }
 

@@ -346,6 +346,11 @@ module biocfd_hip
       integer(c_int), value :: variant
       integer(c_int), value :: planes_per_workgroup
     end function
+    integer(c_int) function biocfd_sor_block_config(blk, variant, planes_per_workgroup) &
+        bind(C, name="biocfd_sor_block_config")
+      import :: c_int
+      integer(c_int), value :: blk, variant, planes_per_workgroup
+    end function
 
     integer(c_int) function biocfd_sor_stop_rule(rule, cadence) bind(C, name="biocfd_sor_stop_rule")
       import :: c_int

@@ -80,6 +80,7 @@ SIGNATURES = {
     "biocfd_redblack_sor": [_I, _L, _D, _D],
     "biocfd_correct": [_I, _D],
     "biocfd_sor_config": [_I, _I],
+    "biocfd_sor_block_config": [_I, _I, _I],
     "biocfd_sor_stop_rule": [_I, _I],
     "biocfd_sor_variant_info": [_I, _P],
     "biocfd_sor_time": [_I, _I, _I, _I, _D, _P],

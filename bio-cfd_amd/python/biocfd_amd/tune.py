@@ -60,6 +60,24 @@ def autotune(block: int = 1, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PL
     return best["variant"], best["planes"], table
 
 
+def autotune_block(block: int, dt: float = 1.0, iters: int = 40, planes=DEFAULT_PLANES, install: bool = True):
+    """The FP64 SOR kernel of one block's own calls (biocfd_sor_block_config):
+    every two-iteration pair-owned variant and planes count timed on the
+    block's live pc and b; return (variant, planes, table)."""
+    table = []
+    for v, ti, tj, nt, depth, minb, nit, fam in variants():
+        if fam not in (1, 2) or nit != 2:
+            continue
+        for p in planes:
+            ms = time_variant(block, v, p, iters, dt)
+            table.append({"variant": v, "tile": [ti, tj], "threads": nt, "planes": p,
+                          "us_per_iteration": round(1e3 * ms, 2)})
+    best = min(table, key=lambda r: r["us_per_iteration"])
+    if install:
+        H.call("biocfd_sor_block_config", block, best["variant"], best["planes"])
+    return best["variant"], best["planes"], table
+
+
 def mixed_variants():
     """[(variant, tile_i, tile_j, threads, prefetch_depth, min_wg_per_cu, sweeps_per_pass, family)] of
     the FP32 smoothing kernels."""

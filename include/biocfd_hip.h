@@ -186,6 +186,13 @@ int biocfd_correct(int blk, double dt);
  * A fused variant may run two iterations per pass (temporal blocking); every
  * variant gives the same bits, the choice only changes speed. */
 int biocfd_sor_config(int variant, int planes_per_workgroup);
+/* The FP64 SOR kernel of one block's own SOR calls (no reference
+ * counterpart): variant and planes per workgroup as biocfd_sor_config, for
+ * the calls poisson_solver makes on that block alone (the coarse block; a
+ * batched set of fine blocks keeps the library's variant); variant -1
+ * returns the block to the library's choice.  Every variant gives the same
+ * bits. */
+int biocfd_sor_block_config(int blk, int variant, int planes_per_workgroup);
 /* Stopping test of the FP64 SOR calls (rule 1 has no reference counterpart).
  * rule 0 (default): the reference's, max |pc - pco| < epsi over the fluid
  * cells tested every fifth iteration (PcorVcor.F90:351-375).  rule 1: the
