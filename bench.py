@@ -188,32 +188,35 @@ def run_path(args, precision: str, keep_fields: bool, rule: str = "residual", mo
 
 
 def tune_uneven(args):
-    """The FP64 SOR variant for the uneven workload: one batched time step
-    with each two-iteration pair-owned variant (after one set-up step), the
-    fastest wins.  Every variant gives the same bits, so the steps only
-    differ in time."""
+    """The FP64 SOR variant and planes per workgroup of the uneven
+    workload's batched fine blocks: one batched time step per two-iteration
+    pair-owned variant and planes count (after one set-up step and the
+    coarse block's own tuning), the fastest wins.  Every variant gives the
+    same bits, so the steps only differ in time."""
     from biocfd_amd import cases as CS
     from biocfd_amd import hip as H
     from biocfd_amd import tune
     from biocfd_amd.driver import Simulation
 
-    cands = [t for t in tune.variants() if t[7] in (1, 2) and t[6] == 2]
+    cands = [(t, p) for t in tune.variants() if t[7] in (1, 2) and t[6] == 2 for p in (16, 32, 64)]
     case = CS.config_uneven(nsteps=1 + len(cands))
     sim = Simulation(case)
     H.call("biocfd_sor_batching", 1)
     sim.step()
+    if args.uneven_coarse_tune:
+        tune.autotune_block(1, dt=case.dt)   # the coarse block's own kernel, as in the timed runs
     table = []
-    for t in cands:
-        H.call("biocfd_sor_config", t[0], args.sor_planes)
+    for t, p in cands:
+        H.call("biocfd_sor_config", t[0], p)
         H.call("biocfd_synchronize")
         t0 = time.perf_counter()
         sim.step()
         H.call("biocfd_synchronize")
-        table.append({"variant": t[0], "tile": [t[1], t[2]], "threads": t[3],
+        table.append({"variant": t[0], "tile": [t[1], t[2]], "threads": t[3], "planes": p,
                       "ms_per_step": round(1e3 * (time.perf_counter() - t0), 2)})
     sim.close()
     best = min(table, key=lambda r: r["ms_per_step"])
-    return best["variant"], table
+    return best["variant"], best["planes"], table
 
 
 def run_uneven(args, batching: int, variant, precision: str = "fp64"):
@@ -299,15 +302,20 @@ UNEVEN_TXT = ("uneven17: coarse 192x128x128 on [0,12]x[0,8]x[0,8] + 16 fine bloc
 
 def uneven_summary(args):
     """The secondary line of the default run: the uneven workload, FP64 with
-    the fine blocks batched, the library's default kernels."""
-    b, _, case = run_uneven(args, 1, None)
+    the fine blocks batched, the SOR kernels autotuned for it."""
+    variant, planes, _ = tune_uneven(args)
+    a2 = argparse.Namespace(**vars(args))
+    a2.sor_planes = planes
+    b, _, case = run_uneven(a2, 1, variant)
+    b["sor_kernel"] = {"variant": variant, "planes": planes}
     return {"workload": UNEVEN_TXT.format(sides=",".join(str(int(blk.n[0])) for blk in case.blocks[1:])),
             "cells": case.cells(), "steps": args.steps, "warmup": args.warmup, "dtype": "f64", **b}
 
 
 def uneven_main(args):
     if args.sor_variant == "auto":
-        variant, table = tune_uneven(args)
+        variant, planes, table = tune_uneven(args)
+        args.sor_planes = planes
         print(json.dumps({"uneven_autotune": table}), file=sys.stderr)
     else:
         variant = int(args.sor_variant)
@@ -318,7 +326,7 @@ def uneven_main(args):
         for r in sorted(runs):
             prec, mode = r.split("-", 1)
             out[r] = run_uneven(args, 1 if mode == "batched" else 0, variant, prec)[0]
-        print(json.dumps({"workload": "uneven17", "sor_variant": variant, **out}))
+        print(json.dumps({"workload": "uneven17", "sor_variant": variant, "sor_planes": args.sor_planes, **out}))
         return
     b, fb, case = run_uneven(args, 1, variant)
     p, fp, _ = run_uneven(args, 0, variant)
@@ -330,7 +338,7 @@ def uneven_main(args):
         "value": b["mcell_steps_per_s"], "unit": "Mcell-steps/s", "n_gpus": 1, "higher_is_better": True,
         "workload": UNEVEN_TXT.format(sides=",".join(str(int(blk.n[0])) for blk in case.blocks[1:])),
         "cells": case.cells(), "fine_cells": sum(sizes[1:]), "steps": args.steps, "warmup": args.warmup,
-        "sor_variant": variant, "autotuned": args.sor_variant == "auto",
+        "sor_variant": variant, "sor_planes": args.sor_planes, "autotuned": args.sor_variant == "auto",
         "fp64": {"batched": b, "per_block": p, "speedup": round(b["mcell_steps_per_s"] / p["mcell_steps_per_s"], 3),
                  "bit_identical": bool(all(np.array_equal(x, y) for x, y in zip(fb, fp)))},
         "mixed_equal_sweep_budget": {
