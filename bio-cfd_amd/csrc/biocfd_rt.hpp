@@ -211,6 +211,12 @@ struct Block {
   std::vector<double> hA[3];
   double *ca[BTB_N] = {}, *ck[BTB_N] = {};
   float *ca32[BTB_N] = {}, *ck32[BTB_N] = {}, *h32[3] = {};   // FP32 copies (FP32 momentum tendency)
+  // per table and cell index: the upwind denominators b1 = ca6 hA, b2 = (2 ck6) hB
+  // and their reciprocals rounded once {b1, 1/b1, b2, 1/b2} (k_momentum.hip);
+  // null where a denominator is outside [2^-300, 2^300]
+  double *rr[BTB_N] = {};
+  int *mom_fix = nullptr;             // cells the momentum march evaluates again (count, then cells)
+  int64_t cap_mom_fix = 0;
   std::vector<double> hca[BTB_N], hck[BTB_N];
   SorState *sor = nullptr;            // device
   int64_t counts[BC_N] = {0, 0, 0, 0, 0, 0};

@@ -29,7 +29,9 @@ struct MomArgs {
   // FP32 copies of the upwind tables and spacings (the FP32 tendency)
   const float *ca32[BTB_N], *ck32[BTB_N];
   const float *dx32, *dy32, *dz32;
-  double dt, re;
+  // the upwind denominators and their reciprocals per table (see rdiv)
+  const double *rr[BTB_N];
+  double dt, re, rre;   // rre = 1 / re rounded once
 };
 
 struct Fields {
@@ -63,6 +65,12 @@ __device__ inline const R *mom_h(const MomArgs &a, int axis) {
 }
 
 // (a, c: the table rows of the stencil's column, ca/ck + 6 (col - 1))
+// the type of the grid quantities and constants the stencil combines with
+// operands of type T: float for the FP32 tendency, double otherwise (FP64
+// values and Dual values with FP64 parts)
+template <class T>
+using real_of = std::conditional_t<std::is_same_v<T, float>, float, double>;
+
 // x / y; for the FP32 tendency through the hardware reciprocal (about 1 ulp),
 // correctly rounded otherwise
 template <class T, class H>
@@ -71,17 +79,54 @@ __device__ inline T qdiv(T x, H y) {
   else return x / y;
 }
 
-template <class T, class TA, class TC, class H>
-__device__ inline T upwind3r(T vel, TA a, TC c, T f2, T f1, T f0, T fm1, T fm2, H hA, H hB) {
-  return qdiv(vel * (a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2), a[5] * hA) +
-         qdiv(fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2), H(2) * c[5] * hB);
+// x / b from b and y = 1/b rounded once: q0 = x y is within two ulps of
+// x / b, one residual correction q1 = q0 - (b q0 - x) y (the residual
+// exact in one FMA) makes it faithful, and a second correction from a
+// faithful quotient with the correctly rounded reciprocal gives x / b
+// rounded to nearest (Markstein's theorem; the correction of IEEE division
+// by Newton-Raphson iteration with an FMA), bit for bit the division's
+// result while x, x y and the residuals are normal numbers.  Zero keeps its
+// sign (the residual is formed as b q - x and subtracted).  g[0], g[1]
+// collect the smallest and largest binary exponent of the numerators (0 for
+// zero): a cell with a numerator outside 2^-957 <= |x| < 2^700, or with a
+// non-finite result, is evaluated again with divisions.  b, y lie in
+// [2^-300, 2^300] (checked when the tables are made).
+__device__ inline double rdiv(double x, double b, double y, int *g) {
+  const int e = __builtin_amdgcn_frexp_exp(x);
+  g[0] = e < g[0] ? e : g[0];
+  g[1] = e > g[1] ? e : g[1];
+  const double q0 = x * y;
+  const double q1 = fma(-fma(b, q0, -x), y, q0);
+  return fma(-fma(b, q1, -x), y, q1);
 }
 
-// the type of the grid quantities and constants the stencil combines with
-// operands of type T: float for the FP32 tendency, double otherwise (FP64
-// values and Dual values with FP64 parts)
+// whether a cell evaluated with rdiv needs its divisions
+__device__ inline bool rdiv_out_of_range(const int *g, double o1, double o2, double o3) {
+  return g[0] < -956 || g[1] > 700 || !isfinite(o1) || !isfinite(o2) || !isfinite(o3);
+}
+
+// rr: the table's row {b1, 1/b1, b2, 1/b2} (accessors without rows pass
+// nullptr and divide)
+template <class T, class TA, class TC, class RR, class H>
+__device__ inline T upwind3r(T vel, TA a, TC c, RR rr, T f2, T f1, T f0, T fm1, T fm2, H hA, H hB, int *bad) {
+  if constexpr (std::is_same_v<RR, std::nullptr_t>) {
+    return qdiv(vel * (a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2), a[5] * hA) +
+           qdiv(fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2), H(2) * c[5] * hB);
+  } else {
+    return rdiv(vel * (a[0] * f2 + a[1] * f1 + a[2] * f0 + a[3] * fm1 + a[4] * fm2), rr[0], rr[1], bad) +
+           rdiv(fabs(vel) * (c[0] * f2 + c[1] * f1 + c[2] * f0 + c[3] * fm1 + c[4] * fm2), rr[2], rr[3], bad);
+  }
+}
+
+// the viscous sum / re: with reciprocal rows (bad != nullptr) through rdiv
 template <class T>
-using real_of = std::conditional_t<std::is_same_v<T, float>, float, double>;
+__device__ inline T visc_div(T x, const MomArgs &a, int *bad) {
+  if constexpr (std::is_same_v<T, double>) {
+    if (bad) return rdiv(x, a.re, a.rre, bad);
+  }
+  return qdiv(x, real_of<T>(a.re));
+}
+
 
 // Grid spacings around cell (i,j,k) and its linear offset, shared by the
 // three component evaluations.
@@ -185,6 +230,11 @@ struct GAcc {
   template <int T> __device__ cdp yk() const { return (cdp)(a->ck[T] + 6 * (j - 1)); }
   template <int T> __device__ cdp za() const { return (cdp)(a->ca[T] + 6 * (k - 1)); }
   template <int T> __device__ cdp zk() const { return (cdp)(a->ck[T] + 6 * (k - 1)); }
+  // no reciprocal rows: the stencil divides
+  template <int T> __device__ std::nullptr_t xr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t yr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t zr() const { return nullptr; }
+  __device__ int *bad() const { return nullptr; }
 };
 
 #define U(di, dj, dk) A.template u<di, dj, dk>()
@@ -221,12 +271,12 @@ __device__ inline auto mom_u(const MomArgs &a, const Acc &A, const MC &m, int i,
   if (i > 2 && i < nx && j > 2 && j < ny + 1 && k > 2 && k < nz + 1 && near_ok) {
     const R ddy = R(0.5) * (A.template hy<-1>() + A.template hy<-2>()), ddye = R(0.5) * (A.template hy<-1>() + A.template hy<0>());
     const R ddz = R(0.5) * (A.template hz<-1>() + A.template hz<-2>()), ddzr = R(0.5) * (A.template hz<-1>() + A.template hz<0>());
-    duudx = upwind3r(U(0, 0, 0), A.template xa<BTB_UU>(), A.template xk<BTB_UU>(), U(2, 0, 0), U(1, 0, 0), U(0, 0, 0),
-                    U(-1, 0, 0), U(-2, 0, 0), A.template hx<0>(), A.template hx<-1>());
-    dvudy = upwind3r(v_in, A.template ya<BTB_VU>(), A.template yk<BTB_VU>(), U(0, 2, 0), U(0, 1, 0), U(0, 0, 0),
-                    U(0, -1, 0), U(0, -2, 0), ddye, ddy);
-    dwudz = upwind3r(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), U(0, 0, 2), U(0, 0, 1), U(0, 0, 0),
-                    U(0, 0, -1), U(0, 0, -2), ddzr, ddz);
+    duudx = upwind3r(U(0, 0, 0), A.template xa<BTB_UU>(), A.template xk<BTB_UU>(), A.template xr<BTB_UU>(), U(2, 0, 0), U(1, 0, 0), U(0, 0, 0),
+                    U(-1, 0, 0), U(-2, 0, 0), A.template hx<0>(), A.template hx<-1>(), A.bad());
+    dvudy = upwind3r(v_in, A.template ya<BTB_VU>(), A.template yk<BTB_VU>(), A.template yr<BTB_VU>(), U(0, 2, 0), U(0, 1, 0), U(0, 0, 0),
+                    U(0, -1, 0), U(0, -2, 0), ddye, ddy, A.bad());
+    dwudz = upwind3r(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), A.template zr<BTB_WU>(), U(0, 0, 2), U(0, 0, 1), U(0, 0, 0),
+                    U(0, 0, -1), U(0, 0, -2), ddzr, ddz, A.bad());
   } else {
     duudx = gdiv<P2>(R(-0.25) * (u1a * u1a + alpha * fabs(u1a) * u22 - u3 * u3 - alpha * fabs(u3) * u4), A.template hx<-1>());
     dvudy = gdiv<P2>(R(-0.25) * (v1a * u5 + alpha * fabs(v1a) * u6 - v22 * u7 - alpha * fabs(v22) * u8), A.template hy<-1>());
@@ -235,7 +285,7 @@ __device__ inline auto mom_u(const MomArgs &a, const Acc &A, const MC &m, int i,
   const T d2x = gdiv<P2>(R(2), dx2xr) * (gdiv<P2>(-u4, dxr) + gdiv<P2>(u22, dx));
   const T d2y = gdiv<P2>(R(2), R(0.5) * (dy2ye + dy2yw)) * (gdiv<P2>(-u8, R(0.5) * dy2ye) + gdiv<P2>(u6, R(0.5) * dy2yw));
   const T d2z = gdiv<P2>(R(2), R(0.5) * (dz2zt + dz2zb)) * (gdiv<P2>(-u12, R(0.5) * dz2zt) + gdiv<P2>(u10, R(0.5) * dz2zb));
-  const T visc = qdiv(d2x + d2y + d2z, R(a.re));
+  const T visc = visc_div(d2x + d2y + d2z, a, A.bad());
   const T res = (-duudx - dvudy - dwudz + visc);
   return A.template own<0>() + a.dt * (res + dpdx);
 }
@@ -269,12 +319,12 @@ __device__ inline auto mom_v(const MomArgs &a, const Acc &A, const MC &m, int i,
   if (i > 2 && i < nx + 1 && j > 2 && j < ny && k > 2 && k < nz + 1 && near_ok) {
     const R ddx = R(0.5) * (A.template hx<-1>() + A.template hx<-2>()), ddxr = R(0.5) * (A.template hx<-1>() + A.template hx<0>());
     const R ddz = R(0.5) * (A.template hz<-1>() + A.template hz<-2>()), ddzr = R(0.5) * (A.template hz<-1>() + A.template hz<0>());
-    duvdx = upwind3r(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), V(2, 0, 0), V(1, 0, 0), V(0, 0, 0),
-                    V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx);
-    dvvdy = upwind3r(V(0, 0, 0), A.template ya<BTB_VV>(), A.template yk<BTB_VV>(), V(0, 2, 0), V(0, 1, 0), V(0, 0, 0),
-                    V(0, -1, 0), V(0, -2, 0), A.template hy<0>(), A.template hy<-1>());
-    dwvdz = upwind3r(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), V(0, 0, 2), V(0, 0, 1), V(0, 0, 0),
-                    V(0, 0, -1), V(0, 0, -2), ddzr, ddz);
+    duvdx = upwind3r(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), A.template xr<BTB_UW>(), V(2, 0, 0), V(1, 0, 0), V(0, 0, 0),
+                    V(-1, 0, 0), V(-2, 0, 0), ddxr, ddx, A.bad());
+    dvvdy = upwind3r(V(0, 0, 0), A.template ya<BTB_VV>(), A.template yk<BTB_VV>(), A.template yr<BTB_VV>(), V(0, 2, 0), V(0, 1, 0), V(0, 0, 0),
+                    V(0, -1, 0), V(0, -2, 0), A.template hy<0>(), A.template hy<-1>(), A.bad());
+    dwvdz = upwind3r(w_in, A.template za<BTB_WU>(), A.template zk<BTB_WU>(), A.template zr<BTB_WU>(), V(0, 0, 2), V(0, 0, 1), V(0, 0, 0),
+                    V(0, 0, -1), V(0, 0, -2), ddzr, ddz, A.bad());
   } else {
     duvdx = gdiv<P2>(R(-0.25) * (u13 * v3 + alpha * fabs(u13) * v4 - u14 * v5 - alpha * fabs(u14) * v6), A.template hx<-1>());
     dvvdy = gdiv<P2>(R(-0.25) * (v7 * v7 + alpha * fabs(v7) * v8 - v9 * v9 - alpha * fabs(v9) * v10), A.template hy<-1>());
@@ -283,7 +333,7 @@ __device__ inline auto mom_v(const MomArgs &a, const Acc &A, const MC &m, int i,
   const T d2x = gdiv<P2>(R(2), R(0.5) * (dx2xr + dx2xl)) * (gdiv<P2>(-v6, R(0.5) * dx2xr) + gdiv<P2>(v4, R(0.5) * dx2xl));
   const T d2y = gdiv<P2>(R(2), dy2ye) * (gdiv<P2>(-v10, dye) + gdiv<P2>(v8, dy));
   const T d2z = gdiv<P2>(R(2), R(0.5) * (dz2zt + dz2zb)) * (gdiv<P2>(-v14, R(0.5) * dz2zt) + gdiv<P2>(v12, R(0.5) * dz2zb));
-  const T visc = qdiv(d2x + d2y + d2z, R(a.re));
+  const T visc = visc_div(d2x + d2y + d2z, a, A.bad());
   const T res = (-duvdx - dvvdy - dwvdz + visc);
   return A.template own<1>() + a.dt * (res + dpdy);
 }
@@ -317,12 +367,12 @@ __device__ inline auto mom_w(const MomArgs &a, const Acc &A, const MC &m, int i,
   if (i > 2 && i < nx + 1 && j > 2 && j < ny + 1 && k > 2 && k < nz && near_ok) {
     const R ddx = R(0.5) * (A.template hx<-1>() + A.template hx<-2>()), ddxr = R(0.5) * (A.template hx<-1>() + A.template hx<0>());
     const R ddy = R(0.5) * (A.template hy<-1>() + A.template hy<-2>()), ddye = R(0.5) * (A.template hy<-1>() + A.template hy<0>());
-    duwdx = upwind3r(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), W(2, 0, 0), W(1, 0, 0), W(0, 0, 0),
-                    W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx);
-    dvwdy = upwind3r(v_in, A.template ya<BTB_VW>(), A.template yk<BTB_VW>(), W(0, 2, 0), W(0, 1, 0), W(0, 0, 0),
-                    W(0, -1, 0), W(0, -2, 0), ddye, ddy);
-    dwwdz = upwind3r(W(0, 0, 0), A.template za<BTB_WW>(), A.template zk<BTB_WW>(), W(0, 0, 2), W(0, 0, 1), W(0, 0, 0),
-                    W(0, 0, -1), W(0, 0, -2), A.template hz<0>(), A.template hz<-1>());
+    duwdx = upwind3r(u_in, A.template xa<BTB_UW>(), A.template xk<BTB_UW>(), A.template xr<BTB_UW>(), W(2, 0, 0), W(1, 0, 0), W(0, 0, 0),
+                    W(-1, 0, 0), W(-2, 0, 0), ddxr, ddx, A.bad());
+    dvwdy = upwind3r(v_in, A.template ya<BTB_VW>(), A.template yk<BTB_VW>(), A.template yr<BTB_VW>(), W(0, 2, 0), W(0, 1, 0), W(0, 0, 0),
+                    W(0, -1, 0), W(0, -2, 0), ddye, ddy, A.bad());
+    dwwdz = upwind3r(W(0, 0, 0), A.template za<BTB_WW>(), A.template zk<BTB_WW>(), A.template zr<BTB_WW>(), W(0, 0, 2), W(0, 0, 1), W(0, 0, 0),
+                    W(0, 0, -1), W(0, 0, -2), A.template hz<0>(), A.template hz<-1>(), A.bad());
   } else {
     duwdx = gdiv<P2>(-R(0.5) * (u15 * w5 + alpha * fabs(u15) * w6 - u16 * w7 - alpha * fabs(u16) * w8), A.template hx<-1>());
     dvwdy = gdiv<P2>(-R(0.5) * (v15 * w9 + alpha * fabs(v15) * w10 - v16 * w11 - alpha * fabs(v16) * w12), A.template hy<-1>());
@@ -331,7 +381,7 @@ __device__ inline auto mom_w(const MomArgs &a, const Acc &A, const MC &m, int i,
   const T d2x = gdiv<P2>(R(2), R(0.5) * (dx2xr + dx2xl)) * (gdiv<P2>(-w8, R(0.5) * dx2xr) + gdiv<P2>(w6, R(0.5) * dx2xl));
   const T d2y = gdiv<P2>(R(2), R(0.5) * (dy2ye + dy2yw)) * (gdiv<P2>(-w12, R(0.5) * dy2ye) + gdiv<P2>(w10, R(0.5) * dy2yw));
   const T d2z = gdiv<P2>(R(2), dz2zt) * (gdiv<P2>(-w16, dzt) + gdiv<P2>(w14, dz));
-  const T visc = qdiv(d2x + d2y + d2z, R(a.re));
+  const T visc = visc_div(d2x + d2y + d2z, a, A.bad());
   const T res = (-duwdx - dvwdy - dwwdz + visc);
   return A.template own<2>() + a.dt * (res + dpdz);
 }
@@ -566,13 +616,49 @@ struct MAcc {
   template <int T> __device__ cptr<R> yk() const { return (cptr<R>)(mom_ck<R>(*a, T) + 6 * (j - 1)); }
   template <int T> __device__ cptr<R> za() const { return (cptr<R>)(mom_ca<R>(*a, T) + 6 * (k - 1)); }
   template <int T> __device__ cptr<R> zk() const { return (cptr<R>)(mom_ck<R>(*a, T) + 6 * (k - 1)); }
+  // no reciprocal rows: the stencil divides
+  template <int T> __device__ std::nullptr_t xr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t yr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t zr() const { return nullptr; }
+  __device__ int *bad() const { return nullptr; }
+};
+
+// The FP64 march accessor with the upwind denominators' reciprocal rows:
+// x rows staged in LDS with the x tables, y and z rows wave-uniform; a
+// numerator outside rdiv's range sets *badp and the cell is evaluated again
+// through the base accessor, with divisions.
+template <int TJ>
+struct MAccFast : MAcc<TJ, double> {
+  const double *xrt;   // own column's row of the staged x rows (UU, then UW)
+  const double *yrt;   // own j row of the staged y rows (VU, VV, VW; 4 TJ apart)
+  const double *zrt;   // the plane's row of the staged z rows (WU, WW; 4 KC apart)
+  int ystride, zstride;
+  int *badp;           // the numerators' exponent range (rdiv)
+  template <int T>
+  __device__ const double *xr() const {
+    static_assert(T == BTB_UU || T == BTB_UW, "staged x rows");
+    return xrt + (T == BTB_UU ? 0 : 256);
+  }
+  template <int T>
+  __device__ const double *yr() const {
+    static_assert(T == BTB_VU || T == BTB_VV || T == BTB_VW, "staged y rows");
+    return yrt + (T == BTB_VU ? 0 : T == BTB_VV ? 1 : 2) * ystride;
+  }
+  template <int T>
+  __device__ const double *zr() const {
+    static_assert(T == BTB_WU || T == BTB_WW, "staged z rows");
+    return zrt + (T == BTB_WU ? 0 : 1) * zstride;
+  }
+  __device__ int *bad() const { return badp; }
 };
 
 // the march of tile L (workgroup index within the block) of one block
-template <int TJ, int KC, bool P2, class R>
+// fix (FAST): a count, then the cells to evaluate again (n2 entries)
+template <int TJ, int KC, bool P2, class R, bool FAST = false>
 __device__ __forceinline__ void march_run(const MomArgs &a, const Fields &F, double *__restrict__ ut,
                                           double *__restrict__ vt, double *__restrict__ wt,
-                                          const uint8_t *__restrict__ deferred, int gx, int gy, int L) {
+                                          const uint8_t *__restrict__ deferred, int gx, int gy, int L,
+                                          int *__restrict__ fix = nullptr) {
   constexpr int NT = 64 * TJ, kW = MarchPlane<TJ>::kW;
   constexpr int NH2 = 4 * kW + 4 * TJ, NH1 = 2 * 66 + 2 * TJ, NHP = 65 + TJ, S2 = (NH2 + NT - 1) / NT;
   static_assert(NH1 <= NT && NHP <= NT, "one-cell halos: one position per thread");
@@ -668,12 +754,49 @@ __device__ __forceinline__ void march_run(const MomArgs &a, const Fields &F, dou
   };
 
   // x-direction table rows of the tile's columns (rows 1..nx+2 exist)
-  for (int q = tid; q < 4 * 384; q += NT) {
-    const int tq = q / 384, r = q % 384, colx = i0 + r / 6;
+  // (the table unrolled, so that each source pointer is a kernel argument,
+  // not an element of a pointer array indexed at run time)
+#pragma unroll
+  for (int tq = 0; tq < 4; ++tq) {
     const R *src = tq == 0 ? mom_ca<R>(a, BTB_UU) : tq == 1 ? mom_ck<R>(a, BTB_UU) : tq == 2 ? mom_ca<R>(a, BTB_UW) : mom_ck<R>(a, BTB_UW);
-    R x = 0;
-    if (colx <= d.nx + 2) x = src[6 * (colx - 1) + r % 6];
-    tab[tq][r] = x;
+    for (int r = tid; r < 384; r += NT) {
+      const int colx = i0 + r / 6;
+      R x = 0;
+      if (colx <= d.nx + 2) x = src[6 * (colx - 1) + r % 6];
+      tab[tq][r] = x;
+    }
+  }
+  // with FAST, the reciprocal rows {b1, 1/b1, b2, 1/b2} of the x tables UU
+  // and UW for the tile's columns, of the y tables VU, VV, VW for its rows
+  // and of the z tables WU, WW for its planes (rows 1..n+2 exist)
+  __shared__ double tabr[FAST ? 2 : 1][FAST ? 256 : 1];
+  __shared__ double tyr[FAST ? 3 : 1][FAST ? 4 * TJ : 1];
+  __shared__ double tzr[FAST ? 2 : 1][FAST ? 4 * KC : 1];
+  if constexpr (FAST) {
+#pragma unroll
+    for (int tq = 0; tq < 2; ++tq) {
+      const double *src = a.rr[tq == 0 ? BTB_UU : BTB_UW];
+      for (int r = tid; r < 256; r += NT) {
+        const int colx = i0 + r / 4;
+        tabr[tq][r] = colx <= d.nx + 2 ? src[4 * (colx - 1) + r % 4] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int tq = 0; tq < 3; ++tq) {
+      const double *src = a.rr[tq == 0 ? BTB_VU : tq == 1 ? BTB_VV : BTB_VW];
+      for (int r = tid; r < 4 * TJ; r += NT) {
+        const int rowy = j0 + r / 4;
+        tyr[tq][r] = rowy <= d.ny + 2 ? src[4 * (rowy - 1) + r % 4] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int tq = 0; tq < 2; ++tq) {
+      const double *src = a.rr[tq == 0 ? BTB_WU : BTB_WW];
+      for (int r = tid; r < 4 * KC; r += NT) {
+        const int plz = kb + r / 4;
+        tzr[tq][r] = plz <= d.nz + 2 ? src[4 * (plz - 1) + r % 4] : 0.0;
+      }
+    }
   }
   R dxr = 0, dxc = 0, dxl = 0;
   if (mine) {
@@ -737,7 +860,16 @@ __device__ __forceinline__ void march_run(const MomArgs &a, const Fields &F, dou
       const unsigned hit2 = (s2[-2] == 2) | (s2[2] == 2) | (s2[-2 * kW] == 2) | (s2[2 * kW] == 2) | (c2m2 == 2) |
                             (c2p2 == 2);
       const bool near_ok = (i > 2) & (j > 2) & (k > 2) & ((hit1 | hit2) == 0u);
-      MAcc<TJ, R> A;
+      std::conditional_t<FAST, MAccFast<TJ>, MAcc<TJ, R>> A;
+      int g[2] = {0, 0};
+      if constexpr (FAST) {
+        A.xrt = &tabr[0][4 * lane];
+        A.yrt = &tyr[0][4 * row];
+        A.zrt = &tzr[0][4 * (k - kb)];
+        A.ystride = 4 * TJ;
+        A.zstride = 4 * KC;
+        A.badp = g;
+      }
       A.su = &P.f[0][o2];
       A.sv = &P.f[1][o2];
       A.sw = &P.f[2][o2];
@@ -752,9 +884,21 @@ __device__ __forceinline__ void march_run(const MomArgs &a, const Fields &F, dou
       A.wm2 = wm2; A.wm1 = wm1; A.w0 = w0; A.wp1 = wp1; A.wp2 = wp2;
       A.p0 = p0; A.pp1 = pp1;
       A.u_wt = u_wt; A.v_st = v_st; A.w_eb = w_eb; A.w_nb = w_nb;
-      ut[c] = mom_u<P2>(a, A, m, i, j, k, near_ok);
-      vt[c] = mom_v<P2>(a, A, m, i, j, k, near_ok);
-      wt[c] = mom_w<P2>(a, A, m, i, j, k, near_ok);
+      if constexpr (FAST) {
+        const double ou = mom_u<P2>(a, A, m, i, j, k, near_ok);
+        const double ov = mom_v<P2>(a, A, m, i, j, k, near_ok);
+        const double ow = mom_w<P2>(a, A, m, i, j, k, near_ok);
+        ut[c] = ou;
+        vt[c] = ov;
+        wt[c] = ow;
+        // a numerator outside rdiv's range: the cell goes on the list that
+        // k_momentum_fix evaluates again with divisions (fix[0]: the count)
+        if (rdiv_out_of_range(g, ou, ov, ow)) fix[1 + atomicAdd(fix, 1)] = (int)c;
+      } else {
+        ut[c] = mom_u<P2>(a, A, m, i, j, k, near_ok);
+        vt[c] = mom_v<P2>(a, A, m, i, j, k, near_ok);
+        wt[c] = mom_w<P2>(a, A, m, i, j, k, near_ok);
+      }
     }
     // roll the own column up a plane (the new values are first read two
     // planes on, so their loads have a plane's work to land)
@@ -767,12 +911,35 @@ __device__ __forceinline__ void march_run(const MomArgs &a, const Fields &F, dou
   }
 }
 
-template <int TJ, int KC, int WPE, bool P2, class R = double>
+template <int TJ, int KC, int WPE, bool P2, class R = double, bool FAST = false>
 __global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_momentum_march(MomArgs a, Fields F, double *__restrict__ ut, double *__restrict__ vt, double *__restrict__ wt,
-                 const uint8_t *__restrict__ deferred, int gx, int gy) {
-  march_run<TJ, KC, P2, R>(a, F, ut, vt, wt, deferred, gx, gy, (int)blockIdx.x);
+                 const uint8_t *__restrict__ deferred, int gx, int gy, int *__restrict__ fix) {
+  march_run<TJ, KC, P2, R, FAST>(a, F, ut, vt, wt, deferred, gx, gy, (int)blockIdx.x, fix);
 }
+
+// the cells of a FAST march's list again, with divisions (GAcc), by one
+// workgroup (the list is normally empty), which then empties the list for
+// the next call: every thread has read the count before the barrier
+template <bool P2>
+__global__ void __launch_bounds__(256) k_momentum_fix(MomArgs a, Fields F, double *__restrict__ ut,
+                                                      double *__restrict__ vt, double *__restrict__ wt,
+                                                      int *__restrict__ fix) {
+  const int n = fix[0];
+  __syncthreads();
+  if (threadIdx.x == 0) fix[0] = 0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int c = fix[1 + q];
+    const int i = 1 + c % a.d.NX, j = 1 + (c / a.d.NX) % a.d.NY, k = 1 + c / (a.d.NX * a.d.NY);
+    double ou, ov, ow;
+    momentum_cell<P2>(a, F, i, j, k, ou, ov, ow);
+    ut[c] = ou;
+    vt[c] = ov;
+    wt[c] = ow;
+  }
+}
+
+
 
 // The dense pass of several blocks in one launch: block b of the table owns
 // workgroups first .. first + ntiles - 1 (first a multiple of 8, so that a
@@ -783,19 +950,41 @@ struct MomTask {
   Fields F;
   double *ut, *vt, *wt;
   const uint8_t *deferred;
+  int *fix;
   int gx, gy, first, ntiles;
 };
 
-template <int TJ, int KC, int WPE, bool P2, class R = double>
+template <int TJ, int KC, int WPE, bool P2, class R = double, bool FAST = false>
 __global__ void __launch_bounds__(64 * TJ) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_momentum_march_set(const MomTask *__restrict__ tab, int ntask) {
+  // (FAST: each block's own list in its task)
   int b = 0;
   while (b + 1 < ntask && (int)blockIdx.x >= tab[b + 1].first) ++b;
   const MomTask &T = tab[b];
   const int L = (int)blockIdx.x - T.first;
   if (L >= T.ntiles) return;
-  march_run<TJ, KC, P2, R>(T.a, T.F, T.ut, T.vt, T.wt, T.deferred, T.gx, T.gy, L);
+  march_run<TJ, KC, P2, R, FAST>(T.a, T.F, T.ut, T.vt, T.wt, T.deferred, T.gx, T.gy, L, T.fix);
 }
+
+// k_momentum_fix for the blocks of a table (one workgroup per task)
+template <bool P2>
+__global__ void __launch_bounds__(256) k_momentum_fix_set(const MomTask *__restrict__ tab) {
+  const MomTask &T = tab[blockIdx.x];
+  const int n = T.fix[0];
+  __syncthreads();
+  if (threadIdx.x == 0) T.fix[0] = 0;
+  for (int q = threadIdx.x; q < n; q += blockDim.x) {
+    const int c = T.fix[1 + q];
+    const int i = 1 + c % T.a.d.NX, j = 1 + (c / T.a.d.NX) % T.a.d.NY, k = 1 + c / (T.a.d.NX * T.a.d.NY);
+    double ou, ov, ow;
+    momentum_cell<P2>(T.a, T.F, i, j, k, ou, ov, ow);
+    T.ut[c] = ou;
+    T.vt[c] = ov;
+    T.wt[c] = ow;
+  }
+}
+
+
 
 // ---------------------------------------------------------------------------
 // TS handling
@@ -1126,6 +1315,11 @@ struct TsAcc {
   template <int T> __device__ const double *yk() const { return a->ck[T] + 6 * (j - 1); }
   template <int T> __device__ const double *za() const { return a->ca[T] + 6 * (k - 1); }
   template <int T> __device__ const double *zk() const { return a->ck[T] + 6 * (k - 1); }
+  // no reciprocal rows: the stencil divides
+  template <int T> __device__ std::nullptr_t xr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t yr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t zr() const { return nullptr; }
+  __device__ int *bad() const { return nullptr; }
 };
 
 // every deferred cell at once, reading the fields as the ordered pass would
@@ -1190,6 +1384,9 @@ int g_mom_variant = 4;   // the fastest at 256^3 on MI355X (0.72 ms against 1.02
 int g_mom_pow2 = 1;
 // 1: the dense pass evaluates the tendency in FP32 (biocfd_momentum_precision)
 int g_mom_fp32 = 0;
+// 1: the march divides by the upwind denominators and re through corrected
+// products with their reciprocals (biocfd_momentum_divisions)
+int g_mom_rdiv = 1;
 
 // every spacing of every axis is one and the same power of two per axis
 // (the grid divisors of the momentum stencil are then exact powers of two)
@@ -1206,11 +1403,28 @@ bool pow2_grid(const Block &b) {
   return true;
 }
 
+// every table the momentum stencil divides by has its reciprocal rows
+bool rows_ready(const MomArgs &a) {
+  for (int t : {BTB_UU, BTB_VU, BTB_WU, BTB_UW, BTB_VV, BTB_VW, BTB_WW})
+    if (!a.rr[t]) return false;
+  return true;
+}
+
 template <int TJ, int KC, int WPE, bool P2, class R = double>
 void launch_march(const MomArgs &a, const Fields &F, Block *b, hipStream_t s, const uint8_t *deferred) {
   const int gx = (int)div_up(b->d.nx, 64), gy = (int)div_up(b->d.ny, TJ), gz = (int)div_up(b->d.nz, KC);
-  hipLaunchKernelGGL((k_momentum_march<TJ, KC, WPE, P2, R>), dim3((unsigned)((int64_t)gx * gy * gz)), dim3(64, TJ, 1), 0,
-                     s, a, F, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], deferred, gx, gy);
+  const dim3 g((unsigned)((int64_t)gx * gy * gz)), blk(64, TJ, 1);
+  if constexpr (std::is_same_v<R, double>) {
+    if (g_mom_rdiv && rows_ready(a) && b->mom_fix && b->n2 < ((int64_t)1 << 31)) {
+      hipLaunchKernelGGL((k_momentum_march<TJ, KC, WPE, P2, R, true>), g, blk, 0, s, a, F, b->f[BF_UT], b->f[BF_VT],
+                         b->f[BF_WT], deferred, gx, gy, b->mom_fix);
+      hipLaunchKernelGGL(k_momentum_fix<P2>, dim3(1), dim3(256), 0, s, a, F, b->f[BF_UT], b->f[BF_VT],
+                         b->f[BF_WT], b->mom_fix);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_momentum_march<TJ, KC, WPE, P2, R>), g, blk, 0, s, a, F, b->f[BF_UT], b->f[BF_VT],
+                     b->f[BF_WT], deferred, gx, gy, (int *)nullptr);
 }
 
 template <bool P2>
@@ -1251,6 +1465,12 @@ void launch_dense(const MomArgs &a, const Fields &F, Block *b, hipStream_t s, co
 }
 }  // namespace
 
+extern "C" int biocfd_momentum_divisions(int reciprocal_rows) {
+  BCFD_API_LOCK();
+  g_mom_rdiv = reciprocal_rows != 0;
+  return 0;
+}
+
 extern "C" int biocfd_momentum_precision(int fp32) {
   BCFD_API_LOCK();
   if (fp32 != 0 && fp32 != 1) {
@@ -1285,6 +1505,16 @@ extern "C" int biocfd_ts_pass_config(int parallel) {
 
 namespace {
 
+// the block's list of cells for k_momentum_fix: a count and n2 entries,
+// zero when allocated (each fix pass empties it again)
+int mom_fix_ready(Block &b) {
+  if (b.mom_fix && b.cap_mom_fix >= b.n2 + 1) return 0;
+  if (grow(&b.mom_fix, &b.cap_mom_fix, b.n2 + 1)) return -1;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, rt().stream, b.mom_fix, (int64_t)1, 0);
+  BCFD_CHECK_LAUNCH();
+  return 0;
+}
+
 int block_mom_args(Block *b, int id, double dt, double re, MomArgs &a) {
   for (int t = 0; t < BTB_N; ++t)
     if (!b->ca[t]) { set_error("non_uni_coeff has not been called for block %d", id); return -1; }
@@ -1298,8 +1528,10 @@ int block_mom_args(Block *b, int id, double dt, double re, MomArgs &a) {
     a.ca32[t] = b->ca32[t]; a.ck32[t] = b->ck32[t];
   }
   a.dx32 = b->h32[0]; a.dy32 = b->h32[1]; a.dz32 = b->h32[2];
+  for (int t = 0; t < BTB_N; ++t) a.rr[t] = b->rr[t];
   a.dt = dt;
   a.re = re;
+  a.rre = 1.0 / re;
   return 0;
 }
 
@@ -1401,7 +1633,14 @@ int launch_dense_set(std::vector<MomTask> &v, MomSetTable &tt) {
     BCFD_HIP(hipStreamSynchronize(s));
   }
   const int64_t wgs = v.back().first + v.back().ntiles;
-  if (g_mom_fp32)
+  bool fast = !g_mom_fp32 && g_mom_rdiv;
+  for (const MomTask &t : v)
+    fast = fast && rows_ready(t.a) && t.fix && (int64_t)t.a.d.NX * t.a.d.NY * t.a.d.NZ < ((int64_t)1 << 31);
+  if (fast) {
+    hipLaunchKernelGGL((k_momentum_march_set<4, 32, 2, P2, double, true>), dim3((unsigned)wgs), dim3(64, 4, 1), 0, s,
+                       tt.d, (int)v.size());
+    hipLaunchKernelGGL(k_momentum_fix_set<P2>, dim3((unsigned)v.size()), dim3(256), 0, s, (const MomTask *)tt.d);
+  } else if (g_mom_fp32)
     hipLaunchKernelGGL((k_momentum_march_set<4, 32, 2, P2, float>), dim3((unsigned)wgs), dim3(64, 4, 1), 0, s, tt.d,
                        (int)v.size());
   else
@@ -1421,6 +1660,7 @@ extern "C" int biocfd_ns_momentum(int id, double dt, double re) {
   if (block_mom_args(b, id, dt, re, a)) return -1;
   uint8_t *deferred = nullptr;
   if (mom_pre(b, &deferred)) return -1;
+  if (mom_fix_ready(*b)) return -1;
   Fields F{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
   launch_dense(a, F, b, rt().stream, deferred);
   BCFD_CHECK_LAUNCH();
@@ -1448,6 +1688,7 @@ extern "C" int biocfd_ns_momentum_set(int first, int last, double dt, double re)
   if (n == 1 || (g_mom_variant != 4 && !g_mom_fp32)) {
     for (int g = first; g <= last; ++g) {
       Block *b = &rt().blk[g];
+      if (mom_fix_ready(*b)) return -1;
       Fields F{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]};
       launch_dense(args[(size_t)(g - first)], F, b, rt().stream, dfr[(size_t)(g - first)]);
     }
@@ -1466,6 +1707,8 @@ extern "C" int biocfd_ns_momentum_set(int first, int last, double dt, double re)
       t.vt = b->f[BF_VT];
       t.wt = b->f[BF_WT];
       t.deferred = dfr[(size_t)(g - first)];
+      if (mom_fix_ready(*b)) return -1;
+      t.fix = b->mom_fix;
       t.gx = (int)div_up(b->d.nx, 64);
       t.gy = (int)div_up(b->d.ny, 4);
       t.ntiles = t.gx * t.gy * (int)div_up(b->d.nz, 32);
@@ -1898,6 +2141,11 @@ struct TsDAcc {
   template <int T> __device__ const double *yk() const { return base.template yk<T>(); }
   template <int T> __device__ const double *za() const { return base.template za<T>(); }
   template <int T> __device__ const double *zk() const { return base.template zk<T>(); }
+  // no reciprocal rows: the stencil divides
+  template <int T> __device__ std::nullptr_t xr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t yr() const { return nullptr; }
+  template <int T> __device__ std::nullptr_t zr() const { return nullptr; }
+  __device__ int *bad() const { return nullptr; }
 };
 
 __device__ inline void ts_cell(const Dims &d, const int *list, int q, int &i, int &j, int &k) {
@@ -2067,8 +2315,10 @@ static int momentum_args(Block &b, double dt, double re, MomArgs &a) {
     a.ca32[t] = b.ca32[t]; a.ck32[t] = b.ck32[t];
   }
   a.dx32 = b.h32[0]; a.dy32 = b.h32[1]; a.dz32 = b.h32[2];
+  for (int t = 0; t < BTB_N; ++t) a.rr[t] = b.rr[t];
   a.dt = dt;
   a.re = re;
+  a.rre = 1.0 / re;
   return 0;
 }
 

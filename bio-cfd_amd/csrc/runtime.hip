@@ -235,8 +235,11 @@ static void free_block(Block &b) {
     if (b.ck32[t]) (void)hipFree(b.ck32[t]);
     b.ca[t] = b.ck[t] = nullptr;
     b.ca32[t] = b.ck32[t] = nullptr;
+    if (b.rr[t]) (void)hipFree(b.rr[t]);
+    b.rr[t] = nullptr;
   }
   for (auto &p : b.h32) if (p) { (void)hipFree(p); p = nullptr; }
+  if (b.mom_fix) { (void)hipFree(b.mom_fix); b.mom_fix = nullptr; b.cap_mom_fix = 0; }
   if (b.sor) (void)hipFree(b.sor);
   if (b.scan_tmp) (void)hipFree(b.scan_tmp);
   if (b.list_tmp) (void)hipFree(b.list_tmp);
@@ -504,6 +507,35 @@ int biocfd_non_uni_coeff(int id) {
     if (dev_alloc(&b->ca[t], m) || dev_alloc(&b->ck[t], m)) return -1;
     BCFD_HIP(hipMemcpyAsync(b->ca[t], b->hca[t].data(), sizeof(double) * m, hipMemcpyHostToDevice, g_rt.stream));
     BCFD_HIP(hipMemcpyAsync(b->ck[t], b->hck[t].data(), sizeof(double) * m, hipMemcpyHostToDevice, g_rt.stream));
+  }
+  // the upwind denominators of each table and their reciprocals (the
+  // momentum kernel divides by them through a correction of the product
+  // with the reciprocal; NavierStokes.f90:619-647's ca6 hA and 2 ck6 hB with
+  // the stencil's spacings: a same-direction table at cell index i takes
+  // hA = d(i), hB = d(i-1), a cross table the centre spacings
+  // 0.5 (d(i-1) + d(i)) and 0.5 (d(i-1) + d(i-2)))
+  for (int t = 0; t < BTB_N; ++t) {
+    const std::vector<double> &d = *dd[axis[t]];
+    const int64_t nn = n[axis[t]];
+    std::vector<double> rr((size_t)(4 * (nn + 2)), 0.0);
+    bool ok = true;
+    for (int64_t i = 2; i <= nn; ++i) {
+      const double hA = cross[t] ? 0.5 * (d[i - 1] + d[i]) : d[i];
+      const double hB = cross[t] ? 0.5 * (d[i - 1] + d[i - 2]) : d[i - 1];
+      const double b1 = b->hca[t][(size_t)(6 * (i - 1) + 5)] * hA;
+      const double b2 = 2.0 * b->hck[t][(size_t)(6 * (i - 1) + 5)] * hB;
+      for (double x : {b1, b2})
+        ok = ok && std::fabs(x) >= 0x1p-300 && std::fabs(x) <= 0x1p300;
+      double *row = &rr[(size_t)(4 * (i - 1))];
+      row[0] = b1;
+      row[1] = 1.0 / b1;
+      row[2] = b2;
+      row[3] = 1.0 / b2;
+    }
+    if (b->rr[t]) { (void)hipFree(b->rr[t]); b->rr[t] = nullptr; }
+    if (!ok) continue;
+    if (dev_alloc(&b->rr[t], (int64_t)rr.size())) return -1;
+    BCFD_HIP(hipMemcpy(b->rr[t], rr.data(), sizeof(double) * rr.size(), hipMemcpyHostToDevice));
   }
   // FP32 copies of the tables and spacings for the FP32 momentum tendency
   std::vector<float> f32;

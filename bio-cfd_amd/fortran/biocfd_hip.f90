@@ -849,6 +849,10 @@ module biocfd_hip
       import :: c_int
       integer(c_int), value :: fp32
     end function
+    integer(c_int) function biocfd_momentum_divisions(reciprocal_rows) bind(C, name="biocfd_momentum_divisions")
+      import :: c_int
+      integer(c_int), value :: reciprocal_rows
+    end function
 
     integer(c_int) function biocfd_ts_pass_config(parallel) bind(C, name="biocfd_ts_pass_config")
       import :: c_int

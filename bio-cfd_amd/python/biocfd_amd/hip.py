@@ -132,6 +132,7 @@ SIGNATURES = {
     "biocfd_sor_graph_stats": [_P, _P],
     "biocfd_momentum_config": [_I, _I],
     "biocfd_momentum_precision": [_I],
+    "biocfd_momentum_divisions": [_I],
     "biocfd_ts_pass_config": [_I],
     "biocfd_ad_reset": [_I],
     "biocfd_ad_upload": [_I, _I, _P],

@@ -153,6 +153,14 @@ int biocfd_momentum_config(int variant, int pow2_products);
  * FP64.  The deferred cells of the thin-surface pass stay FP64.  Mode 1
  * changes the results at FP32 rounding level. */
 int biocfd_momentum_precision(int fp32);
+/* Divisions of the dense momentum march (no reference counterpart):
+ * reciprocal_rows = 1 (default) divides by the upwind denominators ca6 hA,
+ * 2 ck6 hB and by re through two FMA corrections of the product with their
+ * reciprocals rounded once (tables made by biocfd_non_uni_coeff), which is
+ * the correctly rounded quotient while the numerator lies in [2^-957, 2^700)
+ * or is zero; a cell with another numerator is evaluated with divisions.
+ * 0 = divisions everywhere.  Both give the same bits. */
+int biocfd_momentum_divisions(int reciprocal_rows);
 /* The deferred cells of nsMomentum2order's thin-surface pass
  * (NavierStokes.f90:310-490, 860-907) (no reference counterpart):
  * parallel = 1 (the default) evaluates them all at once, each reading the

@@ -253,3 +253,40 @@ def test_fp32_momentum_tendency_on_the_256_workload():
         print(f"{f}: tendency scale {scale:.3e}, FP32 - FP64 {err:.3e} ({err / scale:.2e})")
         assert scale > 0
         assert err <= 1e-5 * scale
+
+
+def test_momentum_reciprocal_rows_fall_back_to_divisions():
+    """Velocities with subnormal, tiny (1e-300) and huge (1e200) entries: the
+    cells whose numerators leave the range where the corrected products
+    with the reciprocals are exact are evaluated again with divisions, so
+    the march gives the bits of the divisions (biocfd_momentum_divisions 0)
+    everywhere, these cells included."""
+    from biocfd_amd import cases as CS
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = CS.config_sphere_256(n=(64, 32, 32), markers=512, nsteps=2, pc_ita_max=20)
+    sim = Simulation(case)
+    try:
+        sim.step()
+        rng = np.random.default_rng(7)
+        for f in ("u", "v", "w"):
+            a = sim.field(1, f)
+            idx = rng.integers(0, a.size, 300)
+            vals = np.array([5e-324, 1e-310, 1e-300, 3e-290, 1e200, -1e-305])
+            a.reshape(-1, order="F")[idx] = vals[rng.integers(0, len(vals), idx.size)]
+            a = np.asfortranarray(a)
+            H.call("biocfd_field_upload", 1, H.BF[f], H.ptr(a))
+        out = {}
+        for mode in (0, 1):
+            H.call("biocfd_momentum_divisions", mode)
+            H.call("biocfd_ns_momentum", 1, case.dt, case.re)
+            H.call("biocfd_synchronize")
+            out[mode] = {f: sim.field(1, f + "t") for f in ("u", "v", "w")}
+    finally:
+        H.call("biocfd_momentum_divisions", 1)
+        sim.close()
+    for f in ("u", "v", "w"):
+        a, b = out[0][f], out[1][f]
+        same = (a.view(np.int64) == b.view(np.int64)) | (np.isnan(a) & np.isnan(b))
+        assert same.all(), (f, int((~same).sum()))

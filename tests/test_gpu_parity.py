@@ -55,6 +55,35 @@ def test_time_step_parity(name, batching, mom, pow2):
         sim.close()
 
 
+@pytest.mark.parametrize("name,pow2", [("sphere", 1), ("wing", 0), ("sphere32", 1), ("wing32", 1), ("nested", 1)])
+def test_momentum_with_divisions(name, pow2):
+    """The plane march dividing by the upwind denominators and re
+    (biocfd_momentum_divisions(0)) instead of correcting products with the
+    reciprocals (the default): the same bits as the oracle either way."""
+    from biocfd_amd import hip as H
+    from biocfd_amd.driver import Simulation
+
+    case = small_cases()[name]
+    ref = run_oracle(case, STEPS)
+    sim = Simulation(case)
+    H.call("biocfd_momentum_config", MOM_DEFAULT, pow2)
+    H.call("biocfd_momentum_divisions", 0)
+    try:
+        bad = []
+        for s in range(0, max(STEPS) + 1):
+            if s > 0:
+                sim.step()
+            if s in STEPS:
+                got = sim.dump()
+                for g, (r, o) in enumerate(zip(ref[s], got)):
+                    bad += compare(r, o, label=f"{name} step {s} block {g + 1}: ")
+        assert not bad, "\n".join(bad[:40])
+    finally:
+        H.call("biocfd_momentum_divisions", 1)
+        H.call("biocfd_momentum_config", MOM_DEFAULT, 1)
+        sim.close()
+
+
 @pytest.mark.parametrize("name", ["wing", "wing32"])
 def test_ts_pass_in_list_order(name):
     """The thin-surface pass's deferred cells one after the other in list
