@@ -18,6 +18,7 @@
 
 #include "biocfd_rt.hpp"
 #include "dual.hpp"
+#include "rdiv.hpp"
 
 namespace bcfd {
 
@@ -79,25 +80,18 @@ __device__ inline T qdiv(T x, H y) {
   else return x / y;
 }
 
-// x / b from b and y = 1/b rounded once: q0 = x y is within two ulps of
-// x / b, one residual correction q1 = q0 - (b q0 - x) y (the residual
-// exact in one FMA) makes it faithful, and a second correction from a
-// faithful quotient with the correctly rounded reciprocal gives x / b
-// rounded to nearest (Markstein's theorem; the correction of IEEE division
-// by Newton-Raphson iteration with an FMA), bit for bit the division's
-// result while x, x y and the residuals are normal numbers.  Zero keeps its
-// sign (the residual is formed as b q - x and subtracted).  g[0], g[1]
-// collect the smallest and largest binary exponent of the numerators (0 for
-// zero): a cell with a numerator outside 2^-957 <= |x| < 2^700, or with a
-// non-finite result, is evaluated again with divisions.  b, y lie in
-// [2^-300, 2^300] (checked when the tables are made).
+// x / b from b and y = 1/b rounded once (rdiv.hpp: two FMA corrections of
+// x y, the correctly rounded quotient while x, x y and the residuals are
+// normal numbers).  g[0], g[1] collect the smallest and largest binary
+// exponent of the numerators (0 for zero): a cell with a numerator outside
+// 2^-957 <= |x| < 2^700, or with a non-finite result, is evaluated again
+// with divisions.  b, y lie in [2^-300, 2^300] (checked when the tables are
+// made).
 __device__ inline double rdiv(double x, double b, double y, int *g) {
   const int e = __builtin_amdgcn_frexp_exp(x);
   g[0] = e < g[0] ? e : g[0];
   g[1] = e > g[1] ? e : g[1];
-  const double q0 = x * y;
-  const double q1 = fma(-fma(b, q0, -x), y, q0);
-  return fma(-fma(b, q1, -x), y, q1);
+  return rdiv_core(x, b, y);
 }
 
 // whether a cell evaluated with rdiv needs its divisions
