@@ -129,31 +129,39 @@ __global__ void __launch_bounds__(256) k_adj_sor_colour(Dims d, const uint8_t *_
                                                         const double *__restrict__ Az, const double *__restrict__ wd,
                                                         double rdt, double omega, int red_cells, int first, int last,
                                                         double *__restrict__ m, double *__restrict__ bbar) {
+  // two planes per thread (k and k + 1: independent cells, their loads in
+  // flight together); j is wave-uniform (one row of 64 threads per wave)
   const int ii = blockIdx.x * 64 + threadIdx.x;
-  const int j = 2 + blockIdx.y * 4 + threadIdx.y;
-  const int k = 2 + blockIdx.z;
-  const int i = 2 + 2 * ii + ((red_cells + j + k) & 1);
-  if (i > d.nx + 1 || j > d.ny + 1) return;
+  const int j = 2 + blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.y);
+  if (j > d.ny + 1) return;
   const int sy = d.NX, sz = d.NX * d.NY;
-  const int c = (i - 1) + sy * (j - 1) + sz * (k - 1);
-  if (cell[c] != 0) return;
-  double mc = m[c];
-  if (!first) {
-    bbar[c] += mc * rdt;
-    mc = (1.0 - omega) * mc;
+  const double ays = j > 2 ? coef(Ay, d.ny, j - 1, 2) : 0.0, ayn = j <= d.ny ? coef(Ay, d.ny, j + 1, 0) : 0.0;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int k = 2 + 2 * blockIdx.z + h;
+    if (k > d.nz + 1) break;
+    const int i = 2 + 2 * ii + ((red_cells + j + k) & 1);
+    if (i > d.nx + 1) continue;
+    const int c = (i - 1) + sy * (j - 1) + sz * (k - 1);
+    if (cell[c] != 0) continue;
+    double mc = m[c];
+    if (!first) {
+      bbar[c] += mc * rdt;
+      mc = (1.0 - omega) * mc;
+    }
+    if (!last) {
+      // neighbour y west of x: x is y's east neighbour (column 2), and so on
+      double acc = 0.0;
+      if (i > 2) acc += coef(Ax, d.nx, i - 1, 2) * m[c - 1];
+      if (i <= d.nx) acc += coef(Ax, d.nx, i + 1, 0) * m[c + 1];
+      acc += ays * m[c - sy];
+      acc += ayn * m[c + sy];
+      if (k > 2) acc += coef(Az, d.nz, k - 1, 2) * m[c - sz];
+      if (k <= d.nz) acc += coef(Az, d.nz, k + 1, 0) * m[c + sz];
+      mc -= wd[c] * acc;
+    }
+    m[c] = mc;
   }
-  if (!last) {
-    // neighbour y west of x: x is y's east neighbour (column 2), and so on
-    double acc = 0.0;
-    if (i > 2) acc += coef(Ax, d.nx, i - 1, 2) * m[c - 1];
-    if (i <= d.nx) acc += coef(Ax, d.nx, i + 1, 0) * m[c + 1];
-    if (j > 2) acc += coef(Ay, d.ny, j - 1, 2) * m[c - sy];
-    if (j <= d.ny) acc += coef(Ay, d.ny, j + 1, 0) * m[c + sy];
-    if (k > 2) acc += coef(Az, d.nz, k - 1, 2) * m[c - sz];
-    if (k <= d.nz) acc += coef(Az, d.nz, k + 1, 0) * m[c + sz];
-    mc -= wd[c] * acc;
-  }
-  m[c] = mc;
 }
 
 // lpc <- (B_1^T R_1^T ... B_N^T R_N^T applied backwards), bbar accumulated
@@ -163,7 +171,8 @@ void adj_sor_launches(const Block &b, int64_t n_iter, double dt, double *wd, dou
   const dim3 blk(256), grd(div_up(b.n2, 256));
   const uint8_t *cell = b.tag[BT_CELL];
   hipLaunchKernelGGL(k_adj_wd, grd, blk, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], b.omega, wd, lpc);
-  const dim3 bc(64, 4, 1), gc((unsigned)div_up((b.d.nx + 1) / 2, 64), (unsigned)div_up(b.d.ny, 4), (unsigned)b.d.nz);
+  const dim3 bc(64, 4, 1),
+      gc((unsigned)div_up((b.d.nx + 1) / 2, 64), (unsigned)div_up(b.d.ny, 4), (unsigned)div_up(b.d.nz, 2));
   for (int64_t q = 0; q <= 2 * n_iter; ++q)
     hipLaunchKernelGGL(k_adj_sor_colour, gc, bc, 0, s, b.d, cell, b.A[0], b.A[1], b.A[2], wd, 1.0 / dt, b.omega,
                        q % 2 == 0 ? 1 : 0, q == 0 ? 1 : 0, q == 2 * n_iter ? 1 : 0, lpc, bbar);
