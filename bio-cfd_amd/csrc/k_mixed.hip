@@ -383,6 +383,7 @@ int sor_timing_begin(hipEvent_t *e0);
 int sor_timing_end(hipEvent_t e0, const Block &b, long long start);
 int sor_timing_end_set(hipEvent_t e0, const std::vector<Block *> &blocks, const std::vector<long long> &start);
 int set_poll(const std::vector<Block *> &blocks, long long &k, bool &stop);
+int read_states(const std::vector<Block *> &blocks, std::vector<SorState> &out);
 
 int mixed_sor(Block &b, int64_t pc_ita_max, double dt) {
   hipStream_t s = rt().stream;
@@ -539,10 +540,8 @@ int mixed_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, double
     hipLaunchKernelGGL(k_mx_reset, dim3(1), dim3(1), 0, s, b->sor, 0);
   }
   BCFD_CHECK_LAUNCH();
-  std::vector<SorState> st0(blocks.size());
-  for (size_t i = 0; i < blocks.size(); ++i)
-    BCFD_HIP(hipMemcpyAsync(&st0[i], blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
-  BCFD_HIP(hipStreamSynchronize(s));
+  std::vector<SorState> st0;
+  if (read_states(blocks, st0)) return -1;
   std::vector<long long> start(blocks.size());
   for (size_t i = 0; i < blocks.size(); ++i) start[i] = st0[i].n_iter;
   std::vector<bool> taken(blocks.size(), false);

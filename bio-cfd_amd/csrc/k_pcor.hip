@@ -956,14 +956,13 @@ __global__ void k_gather_states(SorState *const *__restrict__ ptrs, int n, SorSt
   if (i < n) out[i] = *ptrs[i];
 }
 
-int set_poll(const std::vector<Block *> &blocks, long long &k, bool &stop) {
+// the gather's buffers and the device copy of the set's state pointers
+static int gather_ready(const std::vector<Block *> &blocks) {
   hipStream_t s = rt().stream;
   if (!g_set_poll_h)
     BCFD_HIP(hipHostMalloc((void **)&g_set_poll_h, sizeof(SorState) * 2 * (kMaxBlocks + 1), hipHostMallocDefault));
   if (!g_poll_stage && (dev_alloc(&g_poll_stage, 2 * (kMaxBlocks + 1)) || dev_alloc(&g_poll_ptrs, kMaxBlocks + 1)))
     return -1;
-  for (auto &e : g_set_poll_ev)
-    if (!e) BCFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   std::vector<SorState *> ptrs(blocks.size());
   for (size_t i = 0; i < blocks.size(); ++i) ptrs[i] = blocks[i]->sor;
   if (ptrs != g_poll_ptrs_h) {
@@ -974,6 +973,27 @@ int set_poll(const std::vector<Block *> &blocks, long long &k, bool &stop) {
                             hipMemcpyHostToDevice, s));
     BCFD_HIP(hipStreamSynchronize(s));
   }
+  return 0;
+}
+
+// the SOR states of a set of blocks now (one gather, one copy, one wait)
+int read_states(const std::vector<Block *> &blocks, std::vector<SorState> &out) {
+  hipStream_t s = rt().stream;
+  if (gather_ready(blocks)) return -1;
+  const int n = (int)blocks.size();
+  hipLaunchKernelGGL(k_gather_states, dim3(div_up(n, 64)), dim3(64), 0, s, g_poll_ptrs, n, g_poll_stage);
+  BCFD_CHECK_LAUNCH();
+  BCFD_HIP(hipMemcpyAsync(g_set_poll_h, g_poll_stage, sizeof(SorState) * (size_t)n, hipMemcpyDeviceToHost, s));
+  BCFD_HIP(hipStreamSynchronize(s));
+  out.assign(g_set_poll_h, g_set_poll_h + n);
+  return 0;
+}
+
+int set_poll(const std::vector<Block *> &blocks, long long &k, bool &stop) {
+  hipStream_t s = rt().stream;
+  if (gather_ready(blocks)) return -1;
+  for (auto &e : g_set_poll_ev)
+    if (!e) BCFD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   const int cur = (int)(k & 1);
   const int n = (int)blocks.size();
   SorState *set = g_set_poll_h + cur * (kMaxBlocks + 1), *stage = g_poll_stage + cur * (kMaxBlocks + 1);
@@ -1394,12 +1414,9 @@ int redblack_sor_set(const std::vector<Block *> &blocks, int64_t pc_ita_max, dou
     hipLaunchKernelGGL(k_sor_reset_call, dim3(1), dim3(1), 0, s, b.sor, 0);
     BCFD_CHECK_LAUNCH();
   }
-  SorState *h = (SorState *)rt().h_pinned;
-  for (size_t i = 0; i < blocks.size(); ++i) {
-    BCFD_HIP(hipMemcpyAsync(h, blocks[i]->sor, sizeof(SorState), hipMemcpyDeviceToHost, s));
-    BCFD_HIP(hipStreamSynchronize(s));
-    start[i] = h->n_iter;
-  }
+  std::vector<SorState> st0;
+  if (read_states(blocks, st0)) return -1;
+  for (size_t i = 0; i < blocks.size(); ++i) start[i] = st0[i].n_iter;
   // blocks whose calls start at iterations equal modulo 5 share a pass
   // schedule (their check iterations coincide)
   std::vector<bool> taken(blocks.size(), false);
