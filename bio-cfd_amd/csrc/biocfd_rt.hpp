@@ -127,6 +127,11 @@ struct IbData {
   int64_t cap_ts_qmap = 0;
   uint8_t *ts_adjm = nullptr;
   int64_t cap_ts_adjm = 0;
+  // the deferred cells' 49 operands resolved: source words (where each is
+  // read) and two tables of values (operands, then tangents or adjoints)
+  int64_t *ts_src = nullptr;
+  double *ts_val = nullptr;
+  int64_t cap_ts_src = 0, cap_ts_val = 0;
   // stress_calculation: per-element terms, the coefficients and the miss counter
   double *stress_work = nullptr;
   int64_t cap_stress = 0;

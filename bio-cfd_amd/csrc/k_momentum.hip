@@ -1068,10 +1068,14 @@ __global__ void k_ts_defer(Dims d, const uint8_t *__restrict__ cell, const uint8
 // last TS cell (in TS order) whose element test at the target's position is
 // negative, 0 for none (best[36 q + 6 dq + t] for list entry q, direction
 // dq, target t).  It depends on positions and the surface only, not on the
-// field values, so all cells are done at once, one workgroup per entry.
+// field values, so all cells are done at once, one workgroup per entry.  The
+// workgroup scans the TS cells from the last down, blockDim at a time, and
+// stops after the first chunk in which every target of the cell has a hit:
+// later chunks hold smaller ordinals only.
 __global__ void __launch_bounds__(256) k_ts_best(MomArgs a, const int *list, int n_list, TsArgs ts,
                                                  int *__restrict__ best) {
   __shared__ int s_best[36];
+  __shared__ int s_done;
   const Dims &d = a.d;
   const int q = blockIdx.x;
   if (q >= n_list) return;
@@ -1088,23 +1092,34 @@ __global__ void __launch_bounds__(256) k_ts_best(MomArgs a, const int *list, int
   const double zs[7] = {ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.zp[k - 1], ts.z1[k], ts.z1[k - 1], ts.zp[k - 1]};
   // the seven tests in the order u2,u1,v2,v1,w2,w1,p mapped to BN_* slots
   const int test_nel[7] = {BN_U2, BN_U1, BN_V2, BN_V1, BN_W2, BN_W1, BN_P};
-  for (int nn = 1 + threadIdx.x; nn <= ts.n_ts; nn += blockDim.x) {
-    const int bt = ts.index_ts[nn - 1];
-    bool neg[BN_NNEL];
-    for (int t = 0; t < 7; ++t) {
-      const int e = ts.nel[test_nel[t]][bt - 1];
-      const double dn = (xs[t] - ts.elem[BE_XCENT][e - 1]) * ts.elem[BE_COSA][e - 1] +
-                        (ys[t] - ts.elem[BE_YCENT][e - 1]) * ts.elem[BE_COSB][e - 1] +
-                        (zs[t] - ts.elem[BE_ZCENT][e - 1]) * ts.elem[BE_COSG][e - 1];
-      neg[test_nel[t]] = dn < 0;
+  for (int hi = ts.n_ts; hi >= 1; hi -= (int)blockDim.x) {
+    const int nn = hi - (int)threadIdx.x;
+    if (nn >= 1) {
+      const int bt = ts.index_ts[nn - 1];
+      bool neg[BN_NNEL];
+      for (int t = 0; t < 7; ++t) {
+        const int e = ts.nel[test_nel[t]][bt - 1];
+        const double dn = (xs[t] - ts.elem[BE_XCENT][e - 1]) * ts.elem[BE_COSA][e - 1] +
+                          (ys[t] - ts.elem[BE_YCENT][e - 1]) * ts.elem[BE_COSB][e - 1] +
+                          (zs[t] - ts.elem[BE_ZCENT][e - 1]) * ts.elem[BE_COSG][e - 1];
+        neg[test_nel[t]] = dn < 0;
+      }
+      for (int dq = 0; dq < 6; ++dq) {
+        if (!(m & (1u << dq))) continue;
+        for (int t = 0; t < 6; ++t)
+          if (neg[kTargets[dq][t].test]) atomicMax(&s_best[dq * 6 + t], nn);
+      }
     }
-    for (int dq = 0; dq < 6; ++dq) {
-      if (!(m & (1u << dq))) continue;
-      for (int t = 0; t < 6; ++t)
-        if (neg[kTargets[dq][t].test]) atomicMax(&s_best[dq * 6 + t], nn);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int done = 1;
+      for (int slot = 0; slot < 36; ++slot)
+        if (((m >> (slot / 6)) & 1u) && s_best[slot] == 0) done = 0;
+      s_done = done;
     }
+    __syncthreads();
+    if (s_done) break;
   }
-  __syncthreads();
   if (threadIdx.x < 36) best[36 * (int64_t)q + threadIdx.x] = s_best[threadIdx.x];
 }
 
@@ -1254,52 +1269,136 @@ __device__ inline void ts_writers(const Dims &d, const TsView &v, int f, int i, 
     }
 }
 
-// stencil operands of deferred cell q as the ordered pass would read them
-struct TsAcc {
-  const double *f[4];
-  const Dims *d;
-  const TsView *tv;
-  const MomArgs *a;
-  int q, i, j, k;
-  // where the read of field fi at offset (di, dj, dk) comes from: kind 0..3
-  // the field itself, 4 + g the ghost array g; idx the entry
-  __device__ void src(int fi, int di, int dj, int dk, int &kind, int64_t &idx) const {
-    const int pi = i + di, pj = j + dj, pk = k + dk;
-    const int64_t p = ix3(*d, pi, pj, pk);
-    kind = fi;
-    idx = p;
-    if (!tv->touched[p]) return;
-    int last_q = -1, last_slot = -1, own_slot = -1, own_nn = 0;
-    ts_writers(*d, *tv, fi, pi, pj, pk, [&](int qw, int slot) {
-      if (qw < q) {
-        if (qw > last_q || (qw == last_q && slot > last_slot)) { last_q = qw; last_slot = slot; }
-      } else if (qw == q) {
-        const int nn = tv->best[36 * (int64_t)q + slot];
-        if (nn > 0 && slot > own_slot) {
-          own_slot = slot;
-          own_nn = nn;
-        }
-      }
-    });
-    if (own_slot >= 0) {
-      kind = 4 + kTargets[own_slot / 6][own_slot % 6].g;
-      idx = own_nn - 1;
-    } else if (last_q >= 0) {
-      kind = 4 + kTargets[last_slot / 6][last_slot % 6].gt;
-      idx = tv->n_ts - 1;
+// the 49 operands of a cell's three components: field (0 u, 1 v, 2 w, 3 p)
+// and offset
+struct Seed {
+  signed char f, i, j, k;
+};
+__constant__ Seed kSeeds[49] = {
+    {0, 0, 0, 0}, {0, 1, 0, 0}, {0, 2, 0, 0}, {0, -1, 0, 0}, {0, -2, 0, 0}, {0, 0, 1, 0}, {0, 0, 2, 0},
+    {0, 0, -1, 0}, {0, 0, -2, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 0, -1}, {0, 0, 0, -2}, {0, -1, 1, 0},
+    {0, -1, 0, 1},
+    {1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 0, 0}, {1, -1, 0, 0}, {1, -2, 0, 0}, {1, 0, 1, 0}, {1, 0, 2, 0},
+    {1, 0, -1, 0}, {1, 0, -2, 0}, {1, 0, 0, 1}, {1, 0, 0, 2}, {1, 0, 0, -1}, {1, 0, 0, -2}, {1, 1, -1, 0},
+    {1, 0, -1, 1},
+    {2, 0, 0, 0}, {2, 1, 0, 0}, {2, 2, 0, 0}, {2, -1, 0, 0}, {2, -2, 0, 0}, {2, 0, 1, 0}, {2, 0, 2, 0},
+    {2, 0, -1, 0}, {2, 0, -2, 0}, {2, 0, 0, 1}, {2, 0, 0, 2}, {2, 0, 0, -1}, {2, 0, 0, -2}, {2, 1, 0, -1},
+    {2, 0, 1, -1},
+    {3, 0, 0, 0}, {3, 1, 0, 0}, {3, 0, 1, 0}, {3, 0, 0, 1}};
+
+// the seed index of the operand of field F at offset (di, dj, dk)
+template <int F, int di, int dj, int dk>
+__device__ constexpr int sidx() {
+  if constexpr (F == 3) {
+    static_assert((di == 0 && dj == 0 && dk == 0) || (di == 1 && dj == 0 && dk == 0) ||
+                      (di == 0 && dj == 1 && dk == 0) || (di == 0 && dj == 0 && dk == 1),
+                  "p operand");
+    return 45 + (di == 1 ? 1 : dj == 1 ? 2 : dk == 1 ? 3 : 0);
+  } else {
+    constexpr int base = 15 * F;
+    if constexpr (di == 0 && dj == 0 && dk == 0) return base;
+    else if constexpr (dj == 0 && dk == 0) return base + (di == 1 ? 1 : di == 2 ? 2 : di == -1 ? 3 : 4);
+    else if constexpr (di == 0 && dk == 0) return base + (dj == 1 ? 5 : dj == 2 ? 6 : dj == -1 ? 7 : 8);
+    else if constexpr (di == 0 && dj == 0) return base + (dk == 1 ? 9 : dk == 2 ? 10 : dk == -1 ? 11 : 12);
+    else if constexpr (F == 0 && di == -1 && dj == 1 && dk == 0) return 13;
+    else if constexpr (F == 0 && di == -1 && dj == 0 && dk == 1) return 14;
+    else if constexpr (F == 1 && di == 1 && dj == -1 && dk == 0) return 28;
+    else if constexpr (F == 1 && di == 0 && dj == -1 && dk == 1) return 29;
+    else if constexpr (F == 2 && di == 1 && dj == 0 && dk == -1) return 43;
+    else {
+      static_assert(F == 2 && di == 0 && dj == 1 && dk == -1, "operand position");
+      return 44;
     }
   }
-  __device__ double rd(int fi, int di, int dj, int dk) const {
-    int kind;
-    int64_t idx;
-    src(fi, di, dj, dk, kind, idx);
-    return kind < 4 ? f[kind][idx] : tv->ghost[kind - 4][idx];
+}
+
+// where deferred cell q at (i, j, k) reads field fi at offset (di, dj, dk)
+// in the ordered pass: kind 0..3 the field itself, 4 + g the ghost array g;
+// idx the entry
+__device__ inline void ts_source(const Dims &d, const TsView &tv, int q, int i, int j, int k, int fi, int di,
+                                 int dj, int dk, int &kind, int64_t &idx) {
+  const int pi = i + di, pj = j + dj, pk = k + dk;
+  const int64_t p = ix3(d, pi, pj, pk);
+  kind = fi;
+  idx = p;
+  if (!tv.touched[p]) return;
+  int last_q = -1, last_slot = -1, own_slot = -1, own_nn = 0;
+  ts_writers(d, tv, fi, pi, pj, pk, [&](int qw, int slot) {
+    if (qw < q) {
+      if (qw > last_q || (qw == last_q && slot > last_slot)) { last_q = qw; last_slot = slot; }
+    } else if (qw == q) {
+      const int nn = tv.best[36 * (int64_t)q + slot];
+      if (nn > 0 && slot > own_slot) {
+        own_slot = slot;
+        own_nn = nn;
+      }
+    }
+  });
+  if (own_slot >= 0) {
+    kind = 4 + kTargets[own_slot / 6][own_slot % 6].g;
+    idx = own_nn - 1;
+  } else if (last_q >= 0) {
+    kind = 4 + kTargets[last_slot / 6][last_slot % 6].gt;
+    idx = tv.n_ts - 1;
   }
-  template <int di, int dj, int dk> __device__ double u() const { return rd(0, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ double v() const { return rd(1, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ double w() const { return rd(2, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ double p() const { return rd(3, di, dj, dk); }
-  template <int C> __device__ double own() const { return rd(C, 0, 0, 0); }
+}
+
+// The deferred cells' operands are resolved once per pass: k_ts_src finds
+// where each (cell, operand) pair reads from, k_ts_gather reads the values
+// into a seed-major table (entry s * n_list + q, s in kSeeds order), and the
+// TS kernels take their operands from the table (one coalesced load each,
+// no writer search in the stencil).  Source word: 32 * idx + kind.
+static_assert(4 + BGH_N <= 32, "source word kind field");
+
+__global__ void k_ts_src(Dims d, const int *__restrict__ list, int n_list, TsView tv, int64_t *__restrict__ src) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= 49 * (int64_t)n_list) return;
+  const int s = (int)(gid / n_list), q = (int)(gid - (int64_t)s * n_list);
+  const int lin = list[q];
+  const int i = 2 + lin % d.nx, j = 2 + (lin / d.nx) % d.ny, k = 2 + lin / (d.nx * d.ny);
+  const Seed sd = kSeeds[s];
+  int kind;
+  int64_t idx;
+  ts_source(d, tv, q, i, j, k, sd.f, sd.i, sd.j, sd.k, kind, idx);
+  src[gid] = 32 * idx + kind;
+}
+
+// the arrays a source word's kind names: u, v, w, p, then the ghost arrays
+template <class T>
+struct TsArrays {
+  T *a[4 + BGH_N];
+  __device__ T *of(int kind) const {
+    T *r = a[0];
+#pragma unroll
+    for (int t = 1; t < 4 + BGH_N; ++t) r = kind == t ? a[t] : r;
+    return r;
+  }
+};
+
+__global__ void k_ts_gather(const int64_t *__restrict__ src, int64_t n, TsArrays<const double> A,
+                            double *__restrict__ val) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n) return;
+  const int64_t w = src[gid];
+  val[gid] = A.of((int)(w & 31))[w >> 5];
+}
+
+// stencil operands of deferred cell q as the ordered pass reads them (from
+// the gathered table); spacings and upwind tables as the global accessor
+struct TsAcc {
+  const double *val;   // seed-major operand table
+  int n;               // entries per seed
+  const MomArgs *a;
+  int q, i, j, k;
+  template <int F, int di, int dj, int dk>
+  __device__ double at() const {
+    return val[(int64_t)sidx<F, di, dj, dk>() * n + q];
+  }
+  template <int di, int dj, int dk> __device__ double u() const { return at<0, di, dj, dk>(); }
+  template <int di, int dj, int dk> __device__ double v() const { return at<1, di, dj, dk>(); }
+  template <int di, int dj, int dk> __device__ double w() const { return at<2, di, dj, dk>(); }
+  template <int di, int dj, int dk> __device__ double p() const { return at<3, di, dj, dk>(); }
+  template <int C> __device__ double own() const { return at<C, 0, 0, 0>(); }
   template <int o> __device__ double hx() const { return a->dx[i + o]; }
   template <int o> __device__ double hy() const { return a->dy[j + o]; }
   template <int o> __device__ double hz() const { return a->dz[k + o]; }
@@ -1316,11 +1415,11 @@ struct TsAcc {
   __device__ int *bad() const { return nullptr; }
 };
 
-// every deferred cell at once, reading the fields as the ordered pass would
-__global__ void __launch_bounds__(256) k_ts_parallel(MomArgs a, const double *u, const double *v, const double *w,
-                                                     const double *p, double *__restrict__ ut, double *__restrict__ vt,
-                                                     double *__restrict__ wt, const int *__restrict__ list, int n_list,
-                                                     TsView tv) {
+// every deferred cell at once, its operands from the gathered table
+__global__ void __launch_bounds__(256) k_ts_parallel(MomArgs a, const double *__restrict__ val,
+                                                     double *__restrict__ ut, double *__restrict__ vt,
+                                                     double *__restrict__ wt, const int *__restrict__ list,
+                                                     int n_list) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_list) return;
   const Dims &d = a.d;
@@ -1328,10 +1427,26 @@ __global__ void __launch_bounds__(256) k_ts_parallel(MomArgs a, const double *u,
   const int i = 2 + lin % d.nx, j = 2 + (lin / d.nx) % d.ny, k = 2 + lin / (d.nx * d.ny);
   const MomCell m = mom_cell(a, i, j, k);
   const bool near_ok = mom_near_ok(a, i, j, k);
-  const TsAcc A{{u, v, w, p}, &a.d, &tv, &a, q, i, j, k};
+  const TsAcc A{val, n_list, &a, q, i, j, k};
   ut[m.c] = mom_u<false>(a, A, m, i, j, k, near_ok);
   vt[m.c] = mom_v<false>(a, A, m, i, j, k, near_ok);
   wt[m.c] = mom_w<false>(a, A, m, i, j, k, near_ok);
+}
+
+// the source words of every deferred cell's operands and their values (val:
+// 49 n_list entries) for the fields u, v, w, p and the ghost arrays of the
+// block's TS cells
+int ts_operands(Block &b, const int *list, int64_t cnt, const TsView &tv, TsArrays<const double> fa,
+                double **val) {
+  IbData &ib = b.ib;
+  const int64_t n = 49 * cnt;
+  if (grow(&ib.ts_src, &ib.cap_ts_src, n) || grow(&ib.ts_val, &ib.cap_ts_val, 2 * n)) return -1;
+  hipStream_t s = rt().stream;
+  hipLaunchKernelGGL(k_ts_src, dim3(div_up(n, 256)), dim3(256), 0, s, b.d, list, (int)cnt, tv, ib.ts_src);
+  hipLaunchKernelGGL(k_ts_gather, dim3(div_up(n, 256)), dim3(256), 0, s, ib.ts_src, n, fa, ib.ts_val);
+  BCFD_CHECK_LAUNCH();
+  *val = ib.ts_val;
+  return 0;
 }
 
 // the state the ordered pass leaves: each target position holds the restore
@@ -1591,8 +1706,12 @@ int mom_post(Block *b, const MomArgs &a) {
       const dim3 g1(div_up(cnt, 256)), g36(div_up(36 * cnt, 256));
       hipLaunchKernelGGL(k_ts_prep, g1, dim3(256), 0, s, b->d, b->tag[BT_CELL2], list, (int)cnt, ib.ts_qmap,
                          ib.ts_adjm, 0);
-      hipLaunchKernelGGL(k_ts_parallel, g1, dim3(256), 0, s, a, b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P],
-                         b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list, (int)cnt, tv);
+      TsArrays<const double> fa{{b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P]}};
+      for (int g = 0; g < BGH_N; ++g) fa.a[4 + g] = ib.ghost[g];
+      double *val = nullptr;
+      if (ts_operands(*b, list, cnt, tv, fa, &val)) return -1;
+      hipLaunchKernelGGL(k_ts_parallel, g1, dim3(256), 0, s, a, val, b->f[BF_UT], b->f[BF_VT], b->f[BF_WT], list,
+                         (int)cnt);
       hipLaunchKernelGGL(k_ts_restore, g36, dim3(256), 0, s, b->d, b->f[BF_U], b->f[BF_V], b->f[BF_W], b->f[BF_P],
                          list, (int)cnt, tv);
       hipLaunchKernelGGL(k_ts_prep, g1, dim3(256), 0, s, b->d, b->tag[BT_CELL2], list, (int)cnt, ib.ts_qmap,
@@ -1752,51 +1871,11 @@ struct DAcc : GAcc {
 // every operand position of the three component functions: 15 of u, v and
 // w each (the own component's 13-point upwind cross and the two positions the
 // other components' interpolated velocities add), 4 of p
-struct Seed {
-  signed char f, i, j, k;
-};
-__constant__ Seed kSeeds[49] = {
-    {0, 0, 0, 0}, {0, 1, 0, 0}, {0, 2, 0, 0}, {0, -1, 0, 0}, {0, -2, 0, 0}, {0, 0, 1, 0}, {0, 0, 2, 0},
-    {0, 0, -1, 0}, {0, 0, -2, 0}, {0, 0, 0, 1}, {0, 0, 0, 2}, {0, 0, 0, -1}, {0, 0, 0, -2}, {0, -1, 1, 0},
-    {0, -1, 0, 1},
-    {1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 0, 0}, {1, -1, 0, 0}, {1, -2, 0, 0}, {1, 0, 1, 0}, {1, 0, 2, 0},
-    {1, 0, -1, 0}, {1, 0, -2, 0}, {1, 0, 0, 1}, {1, 0, 0, 2}, {1, 0, 0, -1}, {1, 0, 0, -2}, {1, 1, -1, 0},
-    {1, 0, -1, 1},
-    {2, 0, 0, 0}, {2, 1, 0, 0}, {2, 2, 0, 0}, {2, -1, 0, 0}, {2, -2, 0, 0}, {2, 0, 1, 0}, {2, 0, 2, 0},
-    {2, 0, -1, 0}, {2, 0, -2, 0}, {2, 0, 0, 1}, {2, 0, 0, 2}, {2, 0, 0, -1}, {2, 0, 0, -2}, {2, 1, 0, -1},
-    {2, 0, 1, -1},
-    {3, 0, 0, 0}, {3, 1, 0, 0}, {3, 0, 1, 0}, {3, 0, 0, 1}};
-
 // ---- hand-written reverse of the component functions ----
 // bar[s] accumulates d(lambda . (ut, vt, wt)) / d(operand at seed s)
 // (kSeeds order); the reverse follows mom_u / mom_v / mom_w statement by
 // statement.  |x| has the derivative sg(x), 0 at 0 (as Dual's fabs).
 __device__ inline double sg(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : 0.0); }
-
-template <int F, int di, int dj, int dk>
-__device__ constexpr int sidx() {
-  if constexpr (F == 3) {
-    static_assert((di == 0 && dj == 0 && dk == 0) || (di == 1 && dj == 0 && dk == 0) ||
-                      (di == 0 && dj == 1 && dk == 0) || (di == 0 && dj == 0 && dk == 1),
-                  "p operand");
-    return 45 + (di == 1 ? 1 : dj == 1 ? 2 : dk == 1 ? 3 : 0);
-  } else {
-    constexpr int base = 15 * F;
-    if constexpr (di == 0 && dj == 0 && dk == 0) return base;
-    else if constexpr (dj == 0 && dk == 0) return base + (di == 1 ? 1 : di == 2 ? 2 : di == -1 ? 3 : 4);
-    else if constexpr (di == 0 && dk == 0) return base + (dj == 1 ? 5 : dj == 2 ? 6 : dj == -1 ? 7 : 8);
-    else if constexpr (di == 0 && dj == 0) return base + (dk == 1 ? 9 : dk == 2 ? 10 : dk == -1 ? 11 : 12);
-    else if constexpr (F == 0 && di == -1 && dj == 1 && dk == 0) return 13;
-    else if constexpr (F == 0 && di == -1 && dj == 0 && dk == 1) return 14;
-    else if constexpr (F == 1 && di == 1 && dj == -1 && dk == 0) return 28;
-    else if constexpr (F == 1 && di == 0 && dj == -1 && dk == 1) return 29;
-    else if constexpr (F == 2 && di == 1 && dj == 0 && dk == -1) return 43;
-    else {
-      static_assert(F == 2 && di == 0 && dj == 1 && dk == -1, "operand position");
-      return 44;
-    }
-  }
-}
 
 #define BU(di, dj, dk) bar[sidx<0, di, dj, dk>()]
 #define BV(di, dj, dk) bar[sidx<1, di, dj, dk>()]
@@ -2111,21 +2190,20 @@ struct TsSens {
   double *gh[BGH_N];
 };
 
+// TsAcc with the tangents of the operands beside them (same table layout)
 struct TsDAcc {
   TsAcc base;
-  const TsSens *t;
-  __device__ Dual rd(int fi, int di, int dj, int dk) const {
-    int kind;
-    int64_t idx;
-    base.src(fi, di, dj, dk, kind, idx);
-    if (kind < 4) return {base.f[kind][idx], t->f[kind][idx]};
-    return {base.tv->ghost[kind - 4][idx], t->gh[kind - 4][idx]};
+  const double *tval;
+  template <int F, int di, int dj, int dk>
+  __device__ Dual at() const {
+    const int64_t e = (int64_t)sidx<F, di, dj, dk>() * base.n + base.q;
+    return {base.val[e], tval[e]};
   }
-  template <int di, int dj, int dk> __device__ Dual u() const { return rd(0, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ Dual v() const { return rd(1, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ Dual w() const { return rd(2, di, dj, dk); }
-  template <int di, int dj, int dk> __device__ Dual p() const { return rd(3, di, dj, dk); }
-  template <int C> __device__ Dual own() const { return rd(C, 0, 0, 0); }
+  template <int di, int dj, int dk> __device__ Dual u() const { return at<0, di, dj, dk>(); }
+  template <int di, int dj, int dk> __device__ Dual v() const { return at<1, di, dj, dk>(); }
+  template <int di, int dj, int dk> __device__ Dual w() const { return at<2, di, dj, dk>(); }
+  template <int di, int dj, int dk> __device__ Dual p() const { return at<3, di, dj, dk>(); }
+  template <int C> __device__ Dual own() const { return at<C, 0, 0, 0>(); }
   template <int o> __device__ double hx() const { return base.template hx<o>(); }
   template <int o> __device__ double hy() const { return base.template hy<o>(); }
   template <int o> __device__ double hz() const { return base.template hz<o>(); }
@@ -2149,23 +2227,26 @@ __device__ inline void ts_cell(const Dims &d, const int *list, int q, int &i, in
   k = 2 + lin / (d.nx * d.ny);
 }
 
-__global__ void __launch_bounds__(256) k_ts_tlm(MomArgs a, Fields F, TsSens T, const int *__restrict__ list,
-                                                int n_list, TsView tv, double *__restrict__ tut,
-                                                double *__restrict__ tvt, double *__restrict__ twt) {
+__global__ void __launch_bounds__(256) k_ts_tlm(MomArgs a, const double *__restrict__ val,
+                                                const double *__restrict__ tval, const int *__restrict__ list,
+                                                int n_list, double *__restrict__ tut, double *__restrict__ tvt,
+                                                double *__restrict__ twt) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_list) return;
   int i, j, k;
   ts_cell(a.d, list, q, i, j, k);
   const MomCell m = mom_cell(a, i, j, k);
   const bool near_ok = mom_near_ok(a, i, j, k);
-  const TsDAcc A{{{F.u, F.v, F.w, F.p}, &a.d, &tv, &a, q, i, j, k}, &T};
+  const TsDAcc A{{val, n_list, &a, q, i, j, k}, tval};
   tut[m.c] = mom_u<false>(a, A, m, i, j, k, near_ok).t;
   tvt[m.c] = mom_v<false>(a, A, m, i, j, k, near_ok).t;
   twt[m.c] = mom_w<false>(a, A, m, i, j, k, near_ok).t;
 }
 
-__global__ void __launch_bounds__(256) k_ts_adj(MomArgs a, Fields F, TsSens L, const int *__restrict__ list,
-                                                int n_list, TsView tv, double *__restrict__ lut,
+// the operand adjoints of each deferred cell into bars (the operand table's
+// layout); k_ts_scatter adds them to their sources
+__global__ void __launch_bounds__(256) k_ts_adj(MomArgs a, const double *__restrict__ val, double *__restrict__ bars,
+                                                const int *__restrict__ list, int n_list, double *__restrict__ lut,
                                                 double *__restrict__ lvt, double *__restrict__ lwt) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n_list) return;
@@ -2176,24 +2257,28 @@ __global__ void __launch_bounds__(256) k_ts_adj(MomArgs a, Fields F, TsSens L, c
   lut[m.c] = 0.0;
   lvt[m.c] = 0.0;
   lwt[m.c] = 0.0;
-  if (gu == 0.0 && gv == 0.0 && gw == 0.0) return;
-  const bool near_ok = mom_near_ok(a, i, j, k);
-  const TsAcc A{{F.u, F.v, F.w, F.p}, &a.d, &tv, &a, q, i, j, k};
   double bar[49];
 #pragma unroll
   for (int s = 0; s < 49; ++s) bar[s] = 0.0;
-  mom_u_adj<false>(a, A, m, i, j, k, near_ok, gu, bar);
-  mom_v_adj<false>(a, A, m, i, j, k, near_ok, gv, bar);
-  mom_w_adj<false>(a, A, m, i, j, k, near_ok, gw, bar);
-#pragma unroll
-  for (int s = 0; s < 49; ++s) {
-    if (bar[s] == 0.0) continue;
-    const Seed sd = kSeeds[s];
-    int kind;
-    int64_t idx;
-    A.src(sd.f, sd.i, sd.j, sd.k, kind, idx);
-    atomicAdd(kind < 4 ? &L.f[kind][idx] : &L.gh[kind - 4][idx], bar[s]);
+  if (gu != 0.0 || gv != 0.0 || gw != 0.0) {
+    const bool near_ok = mom_near_ok(a, i, j, k);
+    const TsAcc A{val, n_list, &a, q, i, j, k};
+    mom_u_adj<false>(a, A, m, i, j, k, near_ok, gu, bar);
+    mom_v_adj<false>(a, A, m, i, j, k, near_ok, gv, bar);
+    mom_w_adj<false>(a, A, m, i, j, k, near_ok, gw, bar);
   }
+#pragma unroll
+  for (int s = 0; s < 49; ++s) bars[(int64_t)s * n_list + q] = bar[s];
+}
+
+__global__ void k_ts_scatter(const int64_t *__restrict__ src, const double *__restrict__ bars, int64_t n,
+                             TsArrays<double> L) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n) return;
+  const double b = bars[gid];
+  if (b == 0.0) return;
+  const int64_t w = src[gid];
+  atomicAdd(L.of((int)(w & 31)) + (w >> 5), b);
 }
 
 // transpose of k_ts_restore: the adjoint at every restored position goes to
@@ -2335,9 +2420,17 @@ int momentum_tlm(Block &b, double dt, double re) {
     hipLaunchKernelGGL(k_momentum_tlm<false>, dim3(g), dim3(256), 0, s, a, F, T, t[BF_UT], t[BF_VT], t[BF_WT],
                        deferred);
   if (cnt > 0) {
-    TsSens ts{{t[BF_U], t[BF_V], t[BF_W], t[BF_P]}, {}};
-    for (int q = 0; q < BGH_N; ++q) ts.gh[q] = b.ad.gh[q];
-    hipLaunchKernelGGL(k_ts_tlm, dim3(div_up(cnt, 256)), dim3(256), 0, s, a, F, ts, list, (int)cnt, tv, t[BF_UT],
+    TsArrays<const double> fa{{b.f[BF_U], b.f[BF_V], b.f[BF_W], b.f[BF_P]}};
+    TsArrays<const double> ta{{t[BF_U], t[BF_V], t[BF_W], t[BF_P]}};
+    for (int g = 0; g < BGH_N; ++g) {
+      fa.a[4 + g] = b.ib.ghost[g];
+      ta.a[4 + g] = b.ad.gh[g];
+    }
+    double *val = nullptr;
+    if (ts_operands(b, list, cnt, tv, fa, &val)) return -1;
+    double *tval = val + 49 * cnt;
+    hipLaunchKernelGGL(k_ts_gather, dim3(div_up(49 * cnt, 256)), dim3(256), 0, s, b.ib.ts_src, 49 * cnt, ta, tval);
+    hipLaunchKernelGGL(k_ts_tlm, dim3(div_up(cnt, 256)), dim3(256), 0, s, a, val, tval, list, (int)cnt, t[BF_UT],
                        t[BF_VT], t[BF_WT]);
     // the pass leaves each restored position holding its last writer's
     // restore ghost: so do the tangents
@@ -2366,8 +2459,18 @@ int momentum_adj(Block &b, double dt, double re) {
   // reverse order of the forward: the restores, the deferred cells, the dense pass
   if (cnt > 0) {
     hipLaunchKernelGGL(k_ts_restore_adj, dim3(div_up(36 * cnt, 256)), dim3(256), 0, s, b.d, ls, list, (int)cnt, tv);
-    hipLaunchKernelGGL(k_ts_adj, dim3(div_up(cnt, 256)), dim3(256), 0, s, a, F, ls, list, (int)cnt, tv, l[BF_UT],
+    TsArrays<const double> fa{{b.f[BF_U], b.f[BF_V], b.f[BF_W], b.f[BF_P]}};
+    TsArrays<double> la{{l[BF_U], l[BF_V], l[BF_W], l[BF_P]}};
+    for (int g = 0; g < BGH_N; ++g) {
+      fa.a[4 + g] = b.ib.ghost[g];
+      la.a[4 + g] = b.ad.gh[g];
+    }
+    double *val = nullptr;
+    if (ts_operands(b, list, cnt, tv, fa, &val)) return -1;
+    double *bars = val + 49 * cnt;
+    hipLaunchKernelGGL(k_ts_adj, dim3(div_up(cnt, 256)), dim3(256), 0, s, a, val, bars, list, (int)cnt, l[BF_UT],
                        l[BF_VT], l[BF_WT]);
+    hipLaunchKernelGGL(k_ts_scatter, dim3(div_up(49 * cnt, 256)), dim3(256), 0, s, b.ib.ts_src, bars, 49 * cnt, la);
   }
   const unsigned g = div_up((int64_t)b.d.nx * b.d.ny * b.d.nz, 256);
   if (g_mom_pow2 && pow2_grid(b))

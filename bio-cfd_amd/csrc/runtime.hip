@@ -274,7 +274,8 @@ static void free_block(Block &b) {
   if (ib.task_out) (void)hipFree(ib.task_out);
   if (ib.task_done) (void)hipFree(ib.task_done);
   if (ib.stress_work) (void)hipFree(ib.stress_work);
-  for (void *p : {(void *)ib.ts_mask, (void *)ib.ts_list, (void *)ib.ts_best, (void *)ib.ts_qmap, (void *)ib.ts_adjm})
+  for (void *p : {(void *)ib.ts_mask, (void *)ib.ts_list, (void *)ib.ts_best, (void *)ib.ts_qmap, (void *)ib.ts_adjm,
+                  (void *)ib.ts_src, (void *)ib.ts_val})
     if (p) (void)hipFree(p);
   b = Block();
 }
