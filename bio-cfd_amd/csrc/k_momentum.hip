@@ -1040,10 +1040,28 @@ __global__ void k_ts_touch(Dims d, const uint8_t *__restrict__ cell, const uint8
   }
 }
 
+// near: every position of the arrays whose 5x5x5 neighbourhood (clipped to
+// the arrays) holds a touched position; each touched position marks the
+// positions around it (touched positions are few, every mark stores 1)
+__global__ void k_ts_near(Dims d, const uint8_t *__restrict__ touched, uint8_t *__restrict__ near) {
+  const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = 1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = 1 + blockIdx.z;
+  if (i > d.NX || j > d.NY) return;
+  if (!touched[ix3(d, i, j, k)]) return;
+  for (int dk = -2; dk <= 2; ++dk)
+    for (int dj = -2; dj <= 2; ++dj)
+      for (int di = -2; di <= 2; ++di) {
+        const int a = i + di, b = j + dj, e = k + dk;
+        if (a < 1 || b < 1 || e < 1 || a > d.NX || b > d.NY || e > d.NZ) continue;
+        near[ix3(d, a, b, e)] = 1;
+      }
+}
+
 // defer a fluid cell if it is TS-adjacent or its 5x5x5 neighbourhood holds a
-// touched position
+// touched position (near)
 __global__ void k_ts_defer(Dims d, const uint8_t *__restrict__ cell, const uint8_t *__restrict__ cell2,
-                           const uint8_t *__restrict__ touched, uint8_t *__restrict__ deferred) {
+                           const uint8_t *__restrict__ near, uint8_t *__restrict__ deferred) {
   const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
   const int j = 2 + blockIdx.y * blockDim.y + threadIdx.y;
   const int k = 2 + blockIdx.z;
@@ -1052,14 +1070,7 @@ __global__ void k_ts_defer(Dims d, const uint8_t *__restrict__ cell, const uint8
   uint8_t r = 0;
   if (cell[c] == 0) {
     unsigned m;
-    if (ts_adjacent(d, cell2, i, j, k, &m)) r = 1;
-    for (int dk = -2; dk <= 2 && !r; ++dk)
-      for (int dj = -2; dj <= 2 && !r; ++dj)
-        for (int di = -2; di <= 2 && !r; ++di) {
-          const int a = i + di, b = j + dj, e = k + dk;
-          if (a < 1 || b < 1 || e < 1 || a > d.NX || b > d.NY || e > d.NZ) continue;
-          if (touched[ix3(d, a, b, e)]) r = 1;
-        }
+    r = (ts_adjacent(d, cell2, i, j, k, &m) || near[c]) ? 1 : 0;
   }
   deferred[c] = r;
 }
@@ -1653,12 +1664,13 @@ int mom_pre(Block *b, uint8_t **deferred) {
   hipStream_t s = rt().stream;
   const dim3 blk(64, 4, 1), grd(div_up(b->d.nx, 64), div_up(b->d.ny, 4), (unsigned)b->d.nz);
   IbData &ib = b->ib;
-  if (grow(&ib.ts_mask, &ib.cap_ts_mask, 2 * b->n2)) return -1;
-  uint8_t *touched = ib.ts_mask, *dfr = ib.ts_mask + b->n2;
-  BCFD_HIP(hipMemsetAsync(touched, 0, b->n2, s));
-  BCFD_HIP(hipMemsetAsync(dfr, 0, b->n2, s));
+  if (grow(&ib.ts_mask, &ib.cap_ts_mask, 3 * b->n2)) return -1;
+  uint8_t *touched = ib.ts_mask, *dfr = ib.ts_mask + b->n2, *near = ib.ts_mask + 2 * b->n2;
+  BCFD_HIP(hipMemsetAsync(ib.ts_mask, 0, 3 * b->n2, s));
   hipLaunchKernelGGL(k_ts_touch, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched);
-  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], touched, dfr);
+  hipLaunchKernelGGL(k_ts_near, dim3(div_up(b->d.NX, 64), div_up(b->d.NY, 4), (unsigned)b->d.NZ), blk, 0, s, b->d,
+                     touched, near);
+  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b->d, b->tag[BT_CELL], b->tag[BT_CELL2], near, dfr);
   BCFD_CHECK_LAUNCH();
   *deferred = dfr;
   return 0;
@@ -2321,12 +2333,14 @@ static int ts_prepare(Block &b, const MomArgs &a, TsView &tv, const int *&list, 
   if (n_ts == 0) return 0;
   IbData &ib = b.ib;
   hipStream_t s = rt().stream;
-  if (grow(&ib.ts_mask, &ib.cap_ts_mask, 2 * b.n2)) return -1;
-  uint8_t *touched = ib.ts_mask, *dfr = ib.ts_mask + b.n2;
+  if (grow(&ib.ts_mask, &ib.cap_ts_mask, 3 * b.n2)) return -1;
+  uint8_t *touched = ib.ts_mask, *dfr = ib.ts_mask + b.n2, *near = ib.ts_mask + 2 * b.n2;
   const dim3 blk(64, 4, 1), grd(div_up(b.d.nx, 64), div_up(b.d.ny, 4), (unsigned)b.d.nz);
-  hipLaunchKernelGGL(k_fill_u8, dim3(div_up(2 * b.n2, 256)), dim3(256), 0, s, ib.ts_mask, 2 * b.n2);
+  hipLaunchKernelGGL(k_fill_u8, dim3(div_up(3 * b.n2, 256)), dim3(256), 0, s, ib.ts_mask, 3 * b.n2);
   hipLaunchKernelGGL(k_ts_touch, grd, blk, 0, s, b.d, b.tag[BT_CELL], b.tag[BT_CELL2], touched);
-  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b.d, b.tag[BT_CELL], b.tag[BT_CELL2], touched, dfr);
+  hipLaunchKernelGGL(k_ts_near, dim3(div_up(b.d.NX, 64), div_up(b.d.NY, 4), (unsigned)b.d.NZ), blk, 0, s, b.d,
+                     touched, near);
+  hipLaunchKernelGGL(k_ts_defer, grd, blk, 0, s, b.d, b.tag[BT_CELL], b.tag[BT_CELL2], near, dfr);
   BCFD_CHECK_LAUNCH();
   if (compact_mask_lin(b, dfr, &ib.ts_list, &ib.cap_ts_list, &cnt)) return -1;
   deferred = dfr;
