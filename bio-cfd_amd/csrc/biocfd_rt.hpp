@@ -218,7 +218,7 @@ struct Block {
   float *ca32[BTB_N] = {}, *ck32[BTB_N] = {}, *h32[3] = {};   // FP32 copies (FP32 momentum tendency)
   // per table and cell index: the upwind denominators b1 = ca6 hA, b2 = (2 ck6) hB
   // and their reciprocals rounded once {b1, 1/b1, b2, 1/b2} (k_momentum.hip);
-  // null where a denominator is outside [2^-300, 2^300]
+  // null where a denominator is outside [2^-40, 2^40]
   double *rr[BTB_N] = {};
   int *mom_fix = nullptr;             // cells the momentum march evaluates again (count, then cells)
   int64_t cap_mom_fix = 0;

@@ -85,8 +85,8 @@ __device__ inline T qdiv(T x, H y) {
 // normal numbers).  g[0], g[1] collect the smallest and largest binary
 // exponent of the numerators (0 for zero): a cell with a numerator outside
 // 2^-957 <= |x| < 2^700, or with a non-finite result, is evaluated again
-// with divisions.  b, y lie in [2^-300, 2^300] (checked when the tables are
-// made).
+// with divisions.  b, y lie in [2^-40, 2^40] (checked when the tables are
+// made), so x y and the residuals stay normal over that numerator range.
 __device__ inline double rdiv(double x, double b, double y, int *g) {
   const int e = __builtin_amdgcn_frexp_exp(x);
   g[0] = e < g[0] ? e : g[0];

@@ -526,7 +526,7 @@ int biocfd_non_uni_coeff(int id) {
       const double b1 = b->hca[t][(size_t)(6 * (i - 1) + 5)] * hA;
       const double b2 = 2.0 * b->hck[t][(size_t)(6 * (i - 1) + 5)] * hB;
       for (double x : {b1, b2})
-        ok = ok && std::fabs(x) >= 0x1p-300 && std::fabs(x) <= 0x1p300;
+        ok = ok && std::fabs(x) >= 0x1p-40 && std::fabs(x) <= 0x1p40;
       double *row = &rr[(size_t)(4 * (i - 1))];
       row[0] = b1;
       row[1] = 1.0 / b1;

@@ -77,6 +77,14 @@ int main(int argc, char **argv) {
     check(0.0, b);
     check(-0.0, b);
   }
+  // the corners of the range: numerators at the smallest and largest
+  // exponents the kernel admits over divisors at 2^-40 and 2^40
+  for (long t = 0; t < 400000; ++t) {
+    const double x = (t & 1) ? rnd_double(-956, -940) : rnd_double(684, 699);
+    const double b = std::ldexp(1.0 + (double)(rnd() % 4096) / 4096.0, (t & 2) ? 39 : -40);
+    check(x, b);
+    check(x, std::ldexp(1.0, (t & 2) ? 40 : -40));
+  }
   // the kernel's denominators: ca6 h with third-order upwind coefficients of
   // stretched grids (ratios 0.8 .. 1.25) and spacings 1/8 .. 1/128
   for (long t = 0; t < 200000; ++t) {
