@@ -24,13 +24,19 @@
 
 namespace bcfd {
 
-// tile of a residual workgroup: 256 x 8 columns (1024 threads, rows ty and
-// ty + 4), marching kResPlanes planes.  Wide in i: each side-halo cell of a
-// row costs a whole cache line of pc and of e
-constexpr int kResTI = 256, kResTJ = 8, kResPlanes = 32, kResNT = 4 * kResTI;
-constexpr int kResPW = kResTI + 2, kResSH = kResTJ + 2;       // plane footprint with a one-cell halo
-constexpr int kResHalo = 2 * kResPW + 2 * kResTJ;             // halo cells, one per thread < kResHalo
-static_assert(kResHalo <= kResNT, "one halo cell per thread");
+// tile of a residual workgroup: TI x TJ columns of 1024 threads (rows ty
+// and ty + 1024 / TI), marching up to kResPlanes planes: 256 x 8 (wide in
+// i: each side-halo cell of a row costs a whole cache line of pc and of e)
+// or, for blocks at most 128 cells wide, 64 x 32
+constexpr int kResNT = 1024, kResPlanes = 32;
+template <int TI>
+struct ResTile {
+  static constexpr int R = kResNT / TI;                 // rows per half
+  static constexpr int TJ = 2 * R;
+  static constexpr int PW = TI + 2, SH = TJ + 2;        // plane footprint with a one-cell halo
+  static constexpr int Halo = 2 * PW + 2 * TJ;          // halo cells, one per thread < Halo
+  static_assert(Halo <= kResNT, "one halo cell per thread");
+};
 
 // With kUpdate: pc' = pc + e (every interior cell; e is zero off the fluid)
 // is written to pc_out and the residual is taken of pc'; without, of pc.
@@ -45,8 +51,8 @@ static_assert(kResHalo <= kResNT, "one halo cell per thread");
 // and computes plane k; i and j neighbours come from LDS, k neighbours from
 // registers.  HBM traffic per cell: pc 8, e 4, b/dt 8 read, pc' 8, r 4
 // written; the halo is re-read from L2.
-template <bool kUpdate, bool kStore = true>
-__global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const double *__restrict__ pc,
+template <int TI, bool kUpdate, bool kStore = true>
+__global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, int planes, const double *__restrict__ pc,
                                                    const float *__restrict__ e, double *__restrict__ pc_out,
                                                    const double *__restrict__ bred, const double *__restrict__ bblack,
                                                    const double *__restrict__ Ax, const double *__restrict__ Ay,
@@ -54,11 +60,13 @@ __global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const dou
                                                    float *__restrict__ rblack, const SorState *__restrict__ st,
                                                    double *__restrict__ partials) {
   if (st->done) return;
+  using T = ResTile<TI>;
+  constexpr int kResPW = T::PW, kResSH = T::SH, kResHalo = T::Halo, R = T::R;
   __shared__ double pl[2][kResSH][kResPW];
-  const int tid = threadIdx.x, tx = tid % kResTI, ty = tid / kResTI;
-  const int iL = 2 + blockIdx.x * kResTI, jL = 2 + blockIdx.y * kResTJ;
-  const int k0 = 2 + blockIdx.z * kResPlanes;
-  const int k1 = k0 + kResPlanes - 1 < d.nz + 1 ? k0 + kResPlanes - 1 : d.nz + 1;
+  const int tid = threadIdx.x, tx = tid % TI, ty = tid / TI;
+  const int iL = 2 + blockIdx.x * TI, jL = 2 + blockIdx.y * T::TJ;
+  const int k0 = 2 + blockIdx.z * planes;
+  const int k1 = k0 + planes - 1 < d.nz + 1 ? k0 + planes - 1 : d.nz + 1;
   const int64_t sz = (int64_t)d.NX * d.NY;
 
   // own cells (i, j[r]) at slot (tx + 1, ty + 1 + 4r); loaded where inside
@@ -68,8 +76,8 @@ __global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const dou
   bool ld[2], cmp[2];
   Coef3 ay[2];
   for (int r = 0; r < 2; ++r) {
-    const int j = jL + ty + 4 * r;
-    y[r] = ty + 1 + 4 * r;
+    const int j = jL + ty + R * r;
+    y[r] = ty + 1 + R * r;
     ld[r] = i <= d.NX && j <= d.NY;
     cmp[r] = i <= d.nx + 1 && j <= d.ny + 1;
     off[r] = ld[r] ? (i - 1) + d.NX * (j - 1) : 0;
@@ -117,7 +125,7 @@ __global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const dou
   auto load_rhs = [&](int k, double *rh) {
     const int64_t base = (int64_t)NXh * d.NY * (k - 1);
     for (int r = 0; r < 2; ++r) {
-      const bool red = ((i + jL + ty + 4 * r + k) & 1) != 0;
+      const bool red = ((i + jL + ty + R * r + k) & 1) != 0;
       rh[r] = cmp[r] ? (red ? bred : bblack)[base + hoff[r]] : not_fluid_marker();
     }
   };
@@ -149,7 +157,7 @@ __global__ void __launch_bounds__(kResNT) k_mx_resupd(Dims d, int NXh, const dou
         m = m > ar ? m : ar;
       }
       if (kStore) {
-        const bool red = ((i + jL + ty + 4 * r + k) & 1) != 0;
+        const bool red = ((i + jL + ty + R * r + k) & 1) != 0;
         (red ? rred : rblack)[hbase + hoff[r]] = rv;
       }
     }
@@ -327,8 +335,44 @@ static int ensure_mixed(Block &b) {
   return 0;
 }
 
+// the residual launch of a block: 256 x 8 tiles of 32 planes, 64 x 32 tiles
+// where the block is at most 128 cells wide, and fewer planes per workgroup
+// (down to 2) while the launch would have fewer workgroups than CUs.  Every
+// shape gives the same values and the same maximum.
+struct ResShape {
+  int ti, planes;
+  dim3 grid;
+};
+static ResShape res_shape(const Dims &d) {
+  ResShape r;
+  r.ti = d.nx > 128 ? 256 : 64;
+  const int tj = 2 * (kResNT / r.ti);
+  r.planes = kResPlanes;
+  r.grid = dim3(div_up(d.nx, r.ti), div_up(d.ny, tj), div_up(d.nz, r.planes));
+  while (r.planes > 2 && (int64_t)r.grid.x * r.grid.y * r.grid.z < 256) {
+    r.planes /= 2;
+    r.grid.z = div_up(d.nz, r.planes);
+  }
+  return r;
+}
+
 int64_t residual_workgroups(const Block &b) {
-  return (int64_t)div_up(b.d.nx, kResTI) * div_up(b.d.ny, kResTJ) * div_up(b.d.nz, kResPlanes);
+  const ResShape r = res_shape(b.d);
+  return (int64_t)r.grid.x * r.grid.y * r.grid.z;
+}
+
+template <bool kUpdate, bool kStore>
+static int64_t launch_resupd(Block &b, const double *src, const float *e, double *dst, float *rred, float *rblack,
+                             const SorState *st) {
+  const int NXh = (b.d.NX + 1) / 2;
+  const ResShape r = res_shape(b.d);
+  if (r.ti == 256)
+    hipLaunchKernelGGL((k_mx_resupd<256, kUpdate, kStore>), r.grid, dim3(kResNT), 0, rt().stream, b.d, NXh, r.planes,
+                       src, e, dst, b.bs[1], b.bs[0], b.A[0], b.A[1], b.A[2], rred, rblack, st, b.partials);
+  else
+    hipLaunchKernelGGL((k_mx_resupd<64, kUpdate, kStore>), r.grid, dim3(kResNT), 0, rt().stream, b.d, NXh, r.planes,
+                       src, e, dst, b.bs[1], b.bs[0], b.A[0], b.A[1], b.A[2], rred, rblack, st, b.partials);
+  return (int64_t)r.grid.x * r.grid.y * r.grid.z;
 }
 
 // b/dt of the fluid cells into the colour-split FP64 arrays (marker elsewhere)
@@ -336,15 +380,8 @@ int split_bdt(Block &b, double dt);
 
 // residual of pc (src, with e added when e != nullptr, written to dst)
 static int64_t residual_launch(Block &b, const double *src, const float *e, double *dst, const SorState *st) {
-  const int NXh = (b.d.NX + 1) / 2;
-  dim3 blk(kResNT, 1, 1), grd(div_up(b.d.nx, kResTI), div_up(b.d.ny, kResTJ), div_up(b.d.nz, kResPlanes));
-  if (e)
-    hipLaunchKernelGGL(k_mx_resupd<true>, grd, blk, 0, rt().stream, b.d, NXh, src, e, dst, b.bs[1], b.bs[0], b.A[0],
-                       b.A[1], b.A[2], b.r32[1], b.r32[0], st, b.partials);
-  else
-    hipLaunchKernelGGL(k_mx_resupd<false>, grd, blk, 0, rt().stream, b.d, NXh, src, e, dst, b.bs[1], b.bs[0],
-                       b.A[0], b.A[1], b.A[2], b.r32[1], b.r32[0], st, b.partials);
-  return (int64_t)grd.x * grd.y * grd.z;
+  if (e) return launch_resupd<true, true>(b, src, e, dst, b.r32[1], b.r32[0], st);
+  return launch_resupd<false, true>(b, src, e, dst, b.r32[1], b.r32[0], st);
 }
 
 // k sweeps from a zero correction, in passes of up to `nit` sweeps; returns
@@ -588,11 +625,7 @@ int mixed_time(Block &b, int variant, int planes, int iters, double dt, double *
 // block's partials (the FP64 method's residual stopping test; the caller
 // reduces them); returns the number of workgroups
 int64_t residual_check_launch(Block &b, const double *pc, const SorState *st) {
-  const int NXh = (b.d.NX + 1) / 2;
-  dim3 blk(kResNT, 1, 1), grd(div_up(b.d.nx, kResTI), div_up(b.d.ny, kResTJ), div_up(b.d.nz, kResPlanes));
-  hipLaunchKernelGGL((k_mx_resupd<false, false>), grd, blk, 0, rt().stream, b.d, NXh, pc, nullptr, nullptr, b.bs[1],
-                     b.bs[0], b.A[0], b.A[1], b.A[2], nullptr, nullptr, st, b.partials);
-  return (int64_t)grd.x * grd.y * grd.z;
+  return launch_resupd<false, false>(b, pc, nullptr, nullptr, nullptr, nullptr, st);
 }
 
 // max-norm of b/dt - A pc over the fluid cells (diagnostic, FP64)
