@@ -239,7 +239,7 @@ def run_uneven(args, batching: int, variant, precision: str = "fp64"):
     if variant is not None:
         H.call("biocfd_sor_config", variant, args.sor_planes)
     if precision == "mixed":
-        H.call("biocfd_mixed_config", 26, 32, args.uneven_sweeps, 0.0)
+        H.call("biocfd_mixed_config", 26, args.uneven_mixed_planes, args.uneven_sweeps, 0.0)
     for _ in range(args.warmup):
         sim.step()
     coarse = None
@@ -580,6 +580,8 @@ def main():
                     help="first run a plain device copy of this many MiB (byte-counter calibration)")
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--cpu-timeout", type=float, default=600.0)
+    ap.add_argument("--uneven-mixed-planes", type=int, default=8,
+                    help="planes per workgroup of the uneven workload's FP32 sweeps (8 measured fastest of 4, 8, 16, 32)")
     ap.add_argument("--uneven-sweeps", type=int, default=16,
                     help="FP32 sweeps per correction of the uneven workload's mixed runs")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven workload's secondary line")
