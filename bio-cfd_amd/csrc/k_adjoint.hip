@@ -608,6 +608,51 @@ struct Snapshot {
 };
 std::vector<Snapshot> g_snaps[kMaxBlocks + 1];
 
+// The copies of one snapshot save or restore in one launch: grid row y
+// copies bytes[y] from src[y] to dst[y] (device allocations, so both ends
+// are 16-byte aligned), 16 bytes per thread and the tail byte by byte.
+constexpr int kCopyMax = 64;
+struct CopyList {
+  const void *src[kCopyMax];
+  void *dst[kCopyMax];
+  int64_t bytes[kCopyMax];
+};
+
+__global__ void k_copy_list(CopyList L) {
+  const int e = blockIdx.y;
+  const char *__restrict__ s = (const char *)L.src[e];
+  char *__restrict__ d = (char *)L.dst[e];
+  const int64_t n = L.bytes[e], n16 = n / 16;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n16; q += stride)
+    ((uint4 *)d)[q] = ((const uint4 *)s)[q];
+  for (int64_t q = 16 * n16 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) d[q] = s[q];
+}
+
+// queue the copies, kCopyMax per launch
+struct CopyQueue {
+  CopyList L{};
+  int n = 0;
+  int64_t most = 0;
+  void add(void *dst, const void *src, size_t bytes) {
+    if (n == kCopyMax) flush();
+    L.src[n] = src;
+    L.dst[n] = dst;
+    L.bytes[n] = (int64_t)bytes;
+    most = (int64_t)bytes > most ? (int64_t)bytes : most;
+    ++n;
+  }
+  int flush() {
+    if (n == 0) return 0;
+    const int64_t blocks = (most / 16 + 255) / 256;
+    const unsigned gx = (unsigned)(blocks < 1 ? 1 : blocks > 512 ? 512 : blocks);
+    hipLaunchKernelGGL(k_copy_list, dim3(gx, (unsigned)n), dim3(256), 0, rt().stream, L);
+    n = 0;
+    most = 0;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+  }
+};
+
 // every saved buffer of block b with its size at the current capacities
 template <class F>
 void snapshot_visit(Block &b, F f) {
@@ -641,6 +686,7 @@ int snapshot_save(Block &b, int id, int slot) {
   IbData &ib = b.ib;
   size_t k = 0;
   int rc = 0;
+  CopyQueue cq;
   snapshot_visit(b, [&](void **p, size_t n) {
     if (rc || !*p) return;
     if (k >= sn.bufs.size()) { sn.bufs.push_back(nullptr); sn.bytes.push_back(0); }
@@ -652,13 +698,14 @@ int snapshot_save(Block &b, int id, int slot) {
       sn.bufs[k] = q;
       sn.bytes[k] = n;
     }
-    if (hipMemcpyAsync(sn.bufs[k], *p, n, hipMemcpyDeviceToDevice, rt().stream) != hipSuccess) {
-      set_error("snapshot: copy failed");
-      rc = -1;
-    }
+    cq.add(sn.bufs[k], *p, n);
     ++k;
   });
   if (rc) return -1;
+  if (cq.flush()) {
+    set_error("snapshot: copy failed");
+    return -1;
+  }
   sn.cap_ib = ib.cap_ib; sn.cap_ts = ib.cap_ts; sn.cap_solid = ib.cap_solid; sn.n_mapped = ib.n_mapped;
   for (int q = 0; q < BC_N; ++q) sn.counts[q] = b.counts[q];
   sn.live = true;
@@ -691,6 +738,7 @@ int snapshot_restore(Block &b, int id, int slot) {
   }
   size_t k = 0;
   int rc = 0;
+  CopyQueue cq;
   snapshot_visit(b, [&](void **p, size_t n) {
     if (rc || !*p) return;
     if (k >= sn.bufs.size() || sn.bytes[k] != n) {
@@ -698,13 +746,14 @@ int snapshot_restore(Block &b, int id, int slot) {
       rc = -1;
       return;
     }
-    if (hipMemcpyAsync(*p, sn.bufs[k], n, hipMemcpyDeviceToDevice, rt().stream) != hipSuccess) {
-      set_error("snapshot: copy failed");
-      rc = -1;
-    }
+    cq.add(*p, sn.bufs[k], n);
     ++k;
   });
   if (rc) return -1;
+  if (cq.flush()) {
+    set_error("snapshot: copy failed");
+    return -1;
+  }
   ib.n_mapped = sn.n_mapped;
   for (int q = 0; q < BC_N; ++q) b.counts[q] = sn.counts[q];
   return 0;
